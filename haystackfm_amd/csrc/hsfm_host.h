@@ -1,0 +1,89 @@
+// hsfm_host.h -- host-side objects shared by the translation units of libhsfm.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hsfm_internal.h"
+
+struct hsfm_device {
+    int ordinal = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+};
+
+namespace hsfm {
+
+// Thread-local error message (hsfm_last_error).
+int set_error(int code, const char *fmt, ...);
+#define HSFM_HIP(call)                                                                          \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return ::hsfm::set_error(HSFM_EDEVICE, "%s failed: %s (%s:%d)", #call,              \
+                                     hipGetErrorString(e_), __FILE__, __LINE__);                \
+    } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes);
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct DirBuffers {
+    DevBuf occ, sb, samp, sa_vals, lookup;
+    DevDir view{};
+};
+
+// Scratch of the query pipelines, owned by one index.
+struct Workspace {
+    DevBuf qcopy, qoffcopy, refb;            // host-API staging of inputs
+    DevBuf cnt, niv, ivs, status;             // locate fast path
+    DevBuf ovf_list, counters;                // overflow queue + atomic counters
+    DevBuf fb_sets, fb_ivs, fb_meta;          // fallback sets and results
+    DevBuf hit_off, rows, hits;               // resolve
+    DevBuf scan_tmp;
+    DevBuf mem_recs, mem_cnt, mem_off, mem_sorted, mem_hits, pos_off, positions, mem_flags, read_list;
+    DevBuf wave_sets;
+    uint64_t *h_pinned = nullptr;             // small readbacks
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr}; // main kernel, resolve kernel
+};
+
+} // namespace hsfm
+
+struct hsfm_index {
+    hsfm_device *dev = nullptr;
+    hsfm::DirBuffers fwd, rev;
+    bool has_rev = false;
+    hsfm::DevConst h_const{};
+    hsfm::DevBuf d_const;
+    hsfm::DevBuf seq_bounds;
+    uint32_t nseq = 0;
+    std::mutex mu;
+    hsfm::Workspace ws;
+    hsfm_stats stats{};
+};
+
+namespace hsfm {
+// Exclusive scan of u32 counts into u64 offsets (n+1 entries, last = total).
+int exclusive_scan_u32_u64(hsfm_index *idx, const uint32_t *d_in, uint64_t *d_out, uint32_t n);
+// Read back `count` u64 values from device memory (synchronizes the stream).
+int readback_u64(hsfm_index *idx, const uint64_t *d_src, uint64_t *h_dst, int count);
+int fill_u32(hsfm_index *idx, uint32_t *d, uint64_t n, uint32_t v);
+// HIP events bracketing the main (slot 0) and resolve (slot 1) kernels.
+int event_record(hsfm_index *idx, int slot, bool end);
+double event_ms(hsfm_index *idx, int slot);
+
+// Row expansion + LF-walk resolve + boundary map, shared by locate and MEM.
+//   rows of segment s: [lo[s], lo[s] + len[s]) written from dst offsets off[s]
+int launch_resolve(hsfm_index *idx, const DevDir &d, const uint32_t *d_rows, uint64_t nrows,
+                   const uint32_t *d_bounds, uint32_t nbounds, uint32_t *d_out_pairs,
+                   unsigned long long *d_guard);
+} // namespace hsfm

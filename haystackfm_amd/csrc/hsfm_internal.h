@@ -1,0 +1,186 @@
+// hsfm_internal.h -- device layout and rank primitives of the gfx950 query path.
+//
+// Device Occ layout (one per index direction), built once at upload from the
+// reference's interleaved OccTable records (src/occ/mod.rs:114-183):
+//
+//   record b (64 bytes, 16-byte aligned) covers BWT positions [64b, 64b+64):
+//     bytes  0..31  four u64 bitplanes; bit j of plane p = bit p of the lane
+//                   index of position 64b+j (unused planes are zero)
+//     bytes 32..63  sixteen u16 "below" counts: slot k (1..15) = number of
+//                   positions in [superblock start, 64b) whose lane is < k;
+//                   slot 0 is always 0
+//   superblock s (65536 positions = 1024 records): sixteen u32, slot k = number
+//                   of positions in [0, 65536 s) whose lane is < k (slot 0 = 0)
+//
+//   There are (n >> 6) + 1 records, so every count point x in [0, n] reads the
+//   single record x >> 6:  #lane-k in [0, x) = sb[k+1]-sb[k] + cum[k+1]-cum[k]
+//   + popcount(eq_k & ((1 << (x & 63)) - 1)), with slot 16 implied
+//   (sb: 65536 s, cum: 64 (b mod 1024)).  One 64-byte HBM sector per rank probe;
+//   the superblock table (64 B per 65536 positions) stays cache resident.
+//
+// Sampled-SA markers: one uint4 per 64 BWT rows {bits lo, bits hi, rank base, 0}
+// (from src/suffix_array/mod.rs:94-110); sa_vals copied unchanged.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/hsfm.h"
+
+namespace hsfm {
+
+constexpr uint32_t kRecShift = 6;   // 64 positions per record
+constexpr uint32_t kSbRecShift = 10; // 1024 records per superblock
+constexpr uint32_t kSbShift = 16;
+constexpr uint32_t kLocateRegCap = 4; // intervals held in registers per read
+
+// Per-index constants, resident in device memory and staged into LDS.
+struct DevConst {
+    uint16_t compat_lanes[256]; // bit k: lane k's symbol is compatible with query byte
+    int8_t lookup_digit[256];   // digit of a byte among the lookup core symbols, -1 if none
+    uint32_t c_lane[16];        // C[symbol of lane]
+    uint32_t lane_sym[16];
+    uint32_t num_lanes;
+    uint32_t lookup_radix;
+    uint32_t lookup_depth;      // 0 = no lookup table
+    uint32_t pad;
+};
+
+// Device view of one index direction (passed by value to kernels).
+struct DevDir {
+    const uint4 *occ;     // 4 uint4 per record
+    const uint32_t *sb;   // 16 u32 per superblock
+    const uint4 *samp;    // per 64 rows
+    const uint32_t *sa_vals;
+    const uint2 *lookup;  // (lo, hi) per k-mer, or null
+    uint32_t n;           // text length (including sentinels)
+    uint32_t num_samples;
+    uint32_t sample_rate;
+    uint32_t num_lanes;
+    uint64_t lookup_len;
+};
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t popc64(uint64_t v) { return (uint32_t)__popcll(v); }
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ u32x4 ld4(const uint4 *p) {
+    const uint4 v = *p;
+    return u32x4{v.x, v.y, v.z, v.w};
+}
+
+// One loaded record plus its superblock row, kept as scalar/vector values (no
+// arrays) so that per-lane dynamic selection compiles to v_cndmask chains.
+struct RecView {
+    uint64_t p0, p1, p2, p3;  // bitplanes
+    uint64_t c0, c1, c2, c3;  // cum slots 0..15, four u16 per word
+    u32x4 s0, s1, s2, s3;     // superblock slots 0..15
+    uint32_t sb16, cum16;     // implied slot-16 values
+};
+
+__device__ __forceinline__ void load_rec(const DevDir &d, uint32_t x, RecView &r) {
+    x = x < d.n ? x : d.n;
+    const uint32_t blk = x >> kRecShift;
+    const uint4 *rp = d.occ + ((size_t)blk << 2);
+    const uint4 *sp = reinterpret_cast<const uint4 *>(d.sb + ((size_t)(blk >> kSbRecShift) << 4));
+    const uint4 a = rp[0], b = rp[1], c = rp[2], e = rp[3];
+    r.s0 = ld4(sp);
+    r.s1 = ld4(sp + 1);
+    r.s2 = ld4(sp + 2);
+    r.s3 = ld4(sp + 3);
+    r.p0 = (uint64_t)a.x | ((uint64_t)a.y << 32);
+    r.p1 = (uint64_t)a.z | ((uint64_t)a.w << 32);
+    r.p2 = (uint64_t)b.x | ((uint64_t)b.y << 32);
+    r.p3 = (uint64_t)b.z | ((uint64_t)b.w << 32);
+    r.c0 = (uint64_t)c.x | ((uint64_t)c.y << 32);
+    r.c1 = (uint64_t)c.z | ((uint64_t)c.w << 32);
+    r.c2 = (uint64_t)e.x | ((uint64_t)e.y << 32);
+    r.c3 = (uint64_t)e.z | ((uint64_t)e.w << 32);
+    r.sb16 = (blk >> kSbRecShift) << kSbShift;
+    r.cum16 = (blk & ((1u << kSbRecShift) - 1)) << kRecShift;
+}
+
+// slot k (0..16) of the cum / superblock counts, k a runtime value
+__device__ __forceinline__ uint32_t rec_cum(const RecView &r, uint32_t k) {
+    const uint64_t w = (k & 8) ? ((k & 4) ? r.c3 : r.c2) : ((k & 4) ? r.c1 : r.c0);
+    const uint32_t v = (uint32_t)(w >> ((k & 3) * 16)) & 0xffffu;
+    return k >= 16 ? r.cum16 : (k == 0 ? 0u : v);
+}
+__device__ __forceinline__ uint32_t rec_sb(const RecView &r, uint32_t k) {
+    const u32x4 q = (k & 8) ? ((k & 4) ? r.s3 : r.s2) : ((k & 4) ? r.s1 : r.s0);
+    const uint32_t v = (k & 2) ? ((k & 1) ? q.w : q.z) : ((k & 1) ? q.y : q.x);
+    return k >= 16 ? r.sb16 : (k == 0 ? 0u : v);
+}
+__device__ __forceinline__ uint64_t rec_plane(const RecView &r, int j) {
+    return j == 0 ? r.p0 : j == 1 ? r.p1 : j == 2 ? r.p2 : r.p3;
+}
+
+// Counts of every lane k < L in [0, x): out[k].  x must be in [0, n].
+__device__ __forceinline__ void lane_counts(const RecView &r, uint32_t x, uint32_t L, uint32_t (&out)[16]) {
+    const uint64_t low = (1ull << (x & 63)) - 1;
+    uint32_t below_prev = 0; // sb[k] + cum[k]
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const uint64_t eq = ((k & 1) ? r.p0 : ~r.p0) & ((k & 2) ? r.p1 : ~r.p1) &
+                            ((k & 4) ? r.p2 : ~r.p2) & ((k & 8) ? r.p3 : ~r.p3);
+        const uint32_t below_next = rec_sb(r, (uint32_t)k + 1) + rec_cum(r, (uint32_t)k + 1);
+        const uint32_t v = (below_next - below_prev) + popc64(eq & low);
+        below_prev = below_next;
+        out[k] = ((uint32_t)k < L) ? v : 0u;
+    }
+}
+
+// Lane of BWT row `row` and #same-lane rows in [0, row): the LF step of
+// src/occ/mod.rs:456-480 on the device layout.
+__device__ __forceinline__ void lf_lane_rank(const RecView &r, uint32_t row, uint32_t &lane, uint32_t &rank) {
+    const uint32_t off = row & 63;
+    const uint32_t v = (uint32_t)((r.p0 >> off) & 1ull) | ((uint32_t)((r.p1 >> off) & 1ull) << 1) |
+                       ((uint32_t)((r.p2 >> off) & 1ull) << 2) | ((uint32_t)((r.p3 >> off) & 1ull) << 3);
+    uint64_t eq = ~0ull;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint64_t m = 0ull - (uint64_t)((v >> j) & 1u);
+        eq &= ~(rec_plane(r, j) ^ m);
+    }
+    const uint64_t low = (1ull << off) - 1;
+    lane = v;
+    // below(v+1) - below(v) as a masked sum over the 16 static slots: keeps the
+    // selection in registers (a select chain over the row gets rewritten into
+    // an indexed scratch access by the optimizer).
+    uint32_t acc = 0, prev = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const uint32_t next = rec_sb(r, (uint32_t)k + 1) + rec_cum(r, (uint32_t)k + 1);
+        acc += (next - prev) & (0u - (uint32_t)(v == (uint32_t)k));
+        prev = next;
+    }
+    rank = acc + popc64(eq & low);
+}
+
+// Sampled-SA lookup of row (src/suffix_array/mod.rs:185-201).
+__device__ __forceinline__ bool sa_sampled(const DevDir &d, uint32_t row, uint32_t &val) {
+    const uint4 w = d.samp[row >> 6];
+    const uint64_t bits = (uint64_t)w.x | ((uint64_t)w.y << 32);
+    const uint32_t off = row & 63;
+    if (!((bits >> off) & 1ull)) return false;
+    uint32_t rk = w.z + popc64(bits & ((1ull << off) - 1));
+    rk = rk < d.num_samples ? rk : d.num_samples - 1;
+    val = d.sa_vals[rk];
+    return true;
+}
+
+// wave64 helpers
+__device__ __forceinline__ uint32_t wave_lane() { return __lane_id(); }
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= (uint32_t)d) v += t;
+    }
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) { return __shfl(wave_incl_scan(v), 63, 64); }
+
+} // namespace hsfm

@@ -1,0 +1,465 @@
+// hsfm_locate.hip -- batched count / locate on the gfx950 device layout.
+//
+// Pipeline (replaces locate_batch_gpu, src/gpu/locate.rs:52-277):
+//   1. k_locate_search: one thread per read runs FmIndex::backward_search
+//      (src/fm_index/query.rs:219-277) with lookup-table seeding, holding up to
+//      kLocateRegCap SA intervals in registers.  Reads whose interval set grows
+//      past that are queued for step 2 (the reference GPU instead drops
+//      intervals beyond 16, locate_search.wgsl:177).
+//   2. k_locate_wave: one wave per queued read keeps the whole interval set in
+//      global memory and extends it cooperatively; sets larger than the per-wave
+//      capacity are re-run with 4x the capacity until they fit, so no interval
+//      is ever dropped.
+//   3. per-read hit counts -> exclusive scan -> k_expand writes the SA rows of
+//      every read in ascending row order.
+//   4. k_resolve (hsfm_runtime.hip) LF-walks each row and maps it to
+//      (seq_idx, pos_in_seq).
+// Intervals are unions of rows; merging adjacent intervals (query.rs:390-407)
+// does not change the row set, so it is not performed.
+#include <string.h>
+
+#include "hsfm_host.h"
+
+namespace hsfm {
+
+constexpr uint32_t KR = kLocateRegCap;
+constexpr uint8_t kNivOverflow = 0xff;
+
+struct LocateCounters {            // device counters (u64 each)
+    unsigned long long n_ovf;      // reads queued for the wave path
+    unsigned long long work;       // wave path work index
+    unsigned long long fb_used;    // fallback interval slots reserved
+    unsigned long long n_retry;    // reads that need a larger wave capacity
+    unsigned long long guard;      // walk guard hits
+    unsigned long long pad[3];
+};
+
+__device__ __forceinline__ void load_const_lds(const DevConst *cst, uint16_t *s_compat, int8_t *s_digit,
+                                               uint32_t *s_clane) {
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) {
+        s_compat[i] = cst->compat_lanes[i];
+        s_digit[i] = cst->lookup_digit[i];
+    }
+    if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
+}
+
+// Lookup seeding (query.rs:225-242, lookup.rs:101-111): the leftmost of the
+// last `depth` codes is the least significant base-radix digit.
+// Returns 0 = not seeded, 1 = seeded (lo,hi), 2 = seeded empty.
+__device__ __forceinline__ int lookup_seed(const DevDir &d, const int8_t *s_digit, uint32_t depth, uint32_t radix,
+                                           const uint8_t *p, uint32_t m, uint32_t &lo, uint32_t &hi) {
+    if (depth == 0 || m < depth || !d.lookup) return 0;
+    uint64_t idx = 0;
+    for (int t = (int)depth - 1; t >= 0; t--) {
+        const int dg = s_digit[p[m - depth + t]];
+        if (dg < 0) return 0;
+        idx = idx * radix + (uint64_t)dg;
+    }
+    if (idx >= d.lookup_len) idx = d.lookup_len - 1;
+    const uint2 iv = d.lookup[idx];
+    if (iv.x >= iv.y) return 2;
+    lo = iv.x < d.n ? iv.x : d.n;
+    hi = iv.y < d.n ? iv.y : d.n;
+    return 1;
+}
+
+__global__ __launch_bounds__(256) void k_locate_search(DevDir d, const DevConst *__restrict__ cst,
+                                                       const uint8_t *__restrict__ q,
+                                                       const uint64_t *__restrict__ qoff, uint32_t nq,
+                                                       uint32_t *__restrict__ cnt, uint8_t *__restrict__ niv,
+                                                       uint2 *__restrict__ ivs, uint32_t *__restrict__ ovf_list,
+                                                       LocateCounters *__restrict__ ctr) {
+    __shared__ uint16_t s_compat[256];
+    __shared__ int8_t s_digit[256];
+    __shared__ uint32_t s_clane[16];
+    load_const_lds(cst, s_compat, s_digit, s_clane);
+    __syncthreads();
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nq) return;
+    const uint32_t L = cst->num_lanes;
+    const uint64_t b0 = qoff[r];
+    const uint32_t m = (uint32_t)(qoff[r + 1] - b0);
+    const uint8_t *p = q + b0;
+    uint32_t lo[KR], hi[KR], ns = 1;
+#pragma unroll
+    for (uint32_t j = 0; j < KR; j++) { lo[j] = 0xffffffffu; hi[j] = 0xffffffffu; }
+    lo[0] = 0;
+    hi[0] = d.n;
+    int start = (int)m;
+    const int sd = lookup_seed(d, s_digit, cst->lookup_depth, cst->lookup_radix, p, m, lo[0], hi[0]);
+    if (sd == 2) ns = 0;
+    else if (sd == 1) start = (int)(m - cst->lookup_depth);
+    for (int i = start - 1; i >= 0 && ns; i--) {
+        const uint32_t cm = s_compat[p[i]];
+        uint32_t nlo[KR], nhi[KR], nn = 0;
+        bool ovf = false;
+#pragma unroll
+        for (uint32_t j = 0; j < KR; j++) { nlo[j] = 0xffffffffu; nhi[j] = 0xffffffffu; }
+#pragma unroll
+        for (uint32_t j = 0; j < KR; j++) {
+            if (j < ns) {
+                RecView rl, rh;
+                load_rec(d, lo[j], rl);
+                load_rec(d, hi[j], rh);
+                uint32_t cl[16], ch[16];
+                lane_counts(rl, lo[j], L, cl);
+                lane_counts(rh, hi[j], L, ch);
+#pragma unroll
+                for (int k = 0; k < 16; k++) {
+                    if (((cm >> k) & 1u) && ch[k] > cl[k]) {
+                        const uint32_t a = s_clane[k] + cl[k], b = s_clane[k] + ch[k];
+                        if (nn >= KR) ovf = true;
+#pragma unroll
+                        for (uint32_t t = 0; t < KR; t++)
+                            if (t == nn) { nlo[t] = a; nhi[t] = b; }
+                        nn++;
+                    }
+                }
+            }
+        }
+        if (ovf) {
+            niv[r] = kNivOverflow;
+            cnt[r] = 0;
+            const unsigned long long at = atomicAdd(&ctr->n_ovf, 1ull);
+            ovf_list[at] = r;
+            return;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < KR; j++) { lo[j] = nlo[j]; hi[j] = nhi[j]; }
+        ns = nn;
+    }
+    // sort the (<= KR) disjoint intervals by start; empty slots hold 0xffffffff
+#pragma unroll
+    for (uint32_t a = 0; a < KR; a++)
+#pragma unroll
+        for (uint32_t b = 0; b + 1 < KR - a; b++) {
+            const bool sw = lo[b] > lo[b + 1];
+            const uint32_t tl = sw ? lo[b + 1] : lo[b], th = sw ? hi[b + 1] : hi[b];
+            const uint32_t ul = sw ? lo[b] : lo[b + 1], uh = sw ? hi[b] : hi[b + 1];
+            lo[b] = tl; hi[b] = th; lo[b + 1] = ul; hi[b + 1] = uh;
+        }
+    uint32_t total = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < KR; j++)
+        if (j < ns) {
+            total += hi[j] - lo[j];
+            ivs[(size_t)r * KR + j] = make_uint2(lo[j], hi[j]);
+        }
+    niv[r] = (uint8_t)ns;
+    cnt[r] = total;
+}
+
+// One wave per queued read.  Set buffers: per wave 2 * cap (lo,hi) pairs.
+__global__ __launch_bounds__(64) void k_locate_wave(DevDir d, const DevConst *__restrict__ cst,
+                                                    const uint8_t *__restrict__ q,
+                                                    const uint64_t *__restrict__ qoff,
+                                                    const uint32_t *__restrict__ list, uint32_t nlist,
+                                                    uint32_t cap, uint2 *__restrict__ sets,
+                                                    uint32_t *__restrict__ cnt, uint2 *__restrict__ fb_meta,
+                                                    uint2 *__restrict__ fb_ivs, uint64_t fb_cap,
+                                                    uint32_t *__restrict__ retry_list,
+                                                    LocateCounters *__restrict__ ctr) {
+    __shared__ uint16_t s_compat[256];
+    __shared__ int8_t s_digit[256];
+    __shared__ uint32_t s_clane[16];
+    load_const_lds(cst, s_compat, s_digit, s_clane);
+    __syncthreads();
+    const uint32_t lane = threadIdx.x;
+    const uint32_t L = cst->num_lanes;
+    uint2 *bufA = sets + (size_t)blockIdx.x * 2 * cap;
+    uint2 *bufB = bufA + cap;
+    for (;;) {
+        uint32_t wi = 0;
+        if (lane == 0) wi = (uint32_t)atomicAdd(&ctr->work, 1ull);
+        wi = __shfl(wi, 0, 64);
+        if (wi >= nlist) break;
+        const uint32_t r = list[wi];
+        const uint64_t b0 = qoff[r];
+        const uint32_t m = (uint32_t)(qoff[r + 1] - b0);
+        const uint8_t *p = q + b0;
+        uint2 *A = bufA, *B = bufB;
+        uint32_t ns = 1, lo0 = 0, hi0 = d.n;
+        int start = (int)m;
+        const int sd = lookup_seed(d, s_digit, cst->lookup_depth, cst->lookup_radix, p, m, lo0, hi0);
+        if (sd == 2) ns = 0;
+        else if (sd == 1) start = (int)(m - cst->lookup_depth);
+        if (lane == 0) A[0] = make_uint2(lo0, hi0);
+        __syncthreads();
+        bool too_big = false;
+        for (int i = start - 1; i >= 0 && ns && !too_big; i--) {
+            const uint32_t cm = s_compat[p[i]];
+            uint32_t total = 0;
+            for (uint32_t base = 0; base < ns; base += 64) {
+                const uint32_t j = base + lane;
+                uint32_t cl[16], ch[16], cc = 0;
+                if (j < ns) {
+                    const uint2 iv = A[j];
+                    RecView rl, rh;
+                    load_rec(d, iv.x, rl);
+                    load_rec(d, iv.y, rh);
+                    lane_counts(rl, iv.x, L, cl);
+                    lane_counts(rh, iv.y, L, ch);
+#pragma unroll
+                    for (int k = 0; k < 16; k++) cc += (((cm >> k) & 1u) && ch[k] > cl[k]) ? 1u : 0u;
+                }
+                const uint32_t incl = wave_incl_scan(cc);
+                const uint32_t tot = __shfl(incl, 63, 64);
+                if (total + tot > cap) { too_big = true; break; }
+                if (j < ns && cc) {
+                    uint32_t w = total + incl - cc;
+#pragma unroll
+                    for (int k = 0; k < 16; k++)
+                        if (((cm >> k) & 1u) && ch[k] > cl[k]) {
+                            B[w++] = make_uint2(s_clane[k] + cl[k], s_clane[k] + ch[k]);
+                        }
+                }
+                total += tot;
+            }
+            __syncthreads();
+            uint2 *t = A; A = B; B = t;
+            ns = too_big ? ns : total;
+        }
+        if (too_big) {
+            if (lane == 0) {
+                const unsigned long long at = atomicAdd(&ctr->n_retry, 1ull);
+                retry_list[at] = r;
+            }
+            continue;
+        }
+        // sort by start (rank placement) when small enough, then publish
+        uint64_t slot = 0;
+        if (lane == 0) slot = atomicAdd(&ctr->fb_used, (unsigned long long)ns);
+        slot = __shfl(slot, 0, 64);
+        if (slot + ns > fb_cap) {
+            if (lane == 0) {
+                const unsigned long long at = atomicAdd(&ctr->n_retry, 1ull);
+                retry_list[at] = r;
+            }
+            continue;
+        }
+        uint32_t part = 0;
+        for (uint32_t j = lane; j < ns; j += 64) {
+            const uint2 iv = A[j];
+            part += iv.y - iv.x;
+            uint32_t rank = j;
+            if (ns <= 4096) {
+                rank = 0;
+                for (uint32_t t = 0; t < ns; t++) rank += A[t].x < iv.x ? 1u : 0u;
+            }
+            fb_ivs[slot + rank] = iv;
+        }
+        const uint32_t count = wave_sum(part);
+        if (lane == 0) {
+            cnt[r] = count;
+            fb_meta[r] = make_uint2((uint32_t)slot, ns);
+            // slot may exceed 32 bits only beyond 4G intervals; fb_cap bounds it
+        }
+        __syncthreads();
+    }
+}
+
+// Write the SA rows of every read at its hit offset, in ascending row order.
+__global__ __launch_bounds__(256) void k_expand(uint32_t nq, const uint8_t *__restrict__ niv,
+                                                const uint2 *__restrict__ ivs, const uint2 *__restrict__ fb_meta,
+                                                const uint2 *__restrict__ fb_ivs,
+                                                const uint64_t *__restrict__ hit_off, uint32_t *__restrict__ rows) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nq) return;
+    uint64_t w = hit_off[r];
+    const uint64_t e = hit_off[r + 1];
+    if (w == e) return;
+    const uint8_t k = niv[r];
+    const uint2 *src;
+    uint32_t ns;
+    if (k == kNivOverflow) {
+        const uint2 mt = fb_meta[r];
+        src = fb_ivs + mt.x;
+        ns = mt.y;
+    } else {
+        src = ivs + (size_t)r * KR;
+        ns = k;
+    }
+    for (uint32_t j = 0; j < ns && w < e; j++) {
+        const uint2 iv = src[j];
+        for (uint32_t row = iv.x; row < iv.y && w < e; row++) rows[w++] = row;
+    }
+}
+
+// Runs steps 1-2 and leaves per-read counts in ws.cnt (plus intervals).
+static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64_t *d_qoff, uint32_t nq) {
+    Workspace &ws = idx->ws;
+    hipStream_t s = idx->dev->stream;
+    const DevDir &d = idx->fwd.view;
+    int rc;
+    if ((rc = ws.cnt.ensure((size_t)nq * 4 + 4))) return rc;
+    if ((rc = ws.niv.ensure((size_t)nq + 4))) return rc;
+    if ((rc = ws.ivs.ensure((size_t)nq * KR * 8 + 8))) return rc;
+    if ((rc = ws.ovf_list.ensure((size_t)nq * 4 + 4))) return rc;
+    if ((rc = ws.read_list.ensure((size_t)nq * 4 + 4))) return rc;
+    if ((rc = ws.counters.ensure(sizeof(LocateCounters)))) return rc;
+    if ((rc = ws.fb_meta.ensure((size_t)nq * 8 + 8))) return rc;
+    LocateCounters *ctr = ws.counters.as<LocateCounters>();
+    if ((rc = fill_u32(idx, ws.counters.as<uint32_t>(), sizeof(LocateCounters) / 4, 0))) return rc;
+    if (nq) {
+        if ((rc = event_record(idx, 0, false))) return rc;
+        hipLaunchKernelGGL(k_locate_search, dim3((nq + 255) / 256), dim3(256), 0, s, d,
+                           idx->d_const.as<DevConst>(), d_q, d_qoff, nq, ws.cnt.as<uint32_t>(),
+                           ws.niv.as<uint8_t>(), ws.ivs.as<uint2>(), ws.ovf_list.as<uint32_t>(), ctr);
+        HSFM_HIP(hipGetLastError());
+        if ((rc = event_record(idx, 0, true))) return rc;
+    }
+    uint64_t h[8];
+    if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
+    uint64_t nlist = h[0];
+    idx->stats.fallback_reads = nlist;
+    uint32_t cap = 4096;
+    uint64_t fb_cap = nlist * 256 + 4096;
+    const uint32_t *list = ws.ovf_list.as<uint32_t>();
+    uint32_t *retry = ws.read_list.as<uint32_t>();
+    while (nlist) {
+        // per-wave set space is 2 * cap intervals; keep the total under 4 GiB
+        uint64_t waves = nlist < (uint64_t)idx->dev->num_cus * 8 ? nlist : (uint64_t)idx->dev->num_cus * 8;
+        const uint64_t wave_limit = (4ull << 30) / (2ull * cap * 8);
+        if (waves > wave_limit) waves = wave_limit;
+        if (waves == 0) waves = 1;
+        if ((rc = ws.wave_sets.ensure((size_t)waves * 2 * cap * 8))) return rc;
+        if (ws.fb_ivs.cap < fb_cap * 8) {
+            // keep the intervals already published by earlier passes
+            DevBuf grown;
+            if ((rc = grown.ensure(fb_cap * 8))) return rc;
+            if (ws.fb_ivs.p) HSFM_HIP(hipMemcpyAsync(grown.p, ws.fb_ivs.p, ws.fb_ivs.cap, hipMemcpyDeviceToDevice, s));
+            HSFM_HIP(hipStreamSynchronize(s));
+            ws.fb_ivs.release();
+            ws.fb_ivs = grown;
+        }
+        fb_cap = ws.fb_ivs.cap / 8;
+        if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->work), 2, 0))) return rc;
+        if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_retry), 2, 0))) return rc;
+        hipLaunchKernelGGL(k_locate_wave, dim3((uint32_t)waves), dim3(64), 0, s, d, idx->d_const.as<DevConst>(), d_q,
+                           d_qoff, list, (uint32_t)nlist, cap, ws.wave_sets.as<uint2>(), ws.cnt.as<uint32_t>(),
+                           ws.fb_meta.as<uint2>(), ws.fb_ivs.as<uint2>(), fb_cap, retry, ctr);
+        HSFM_HIP(hipGetLastError());
+        if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
+        const uint64_t nretry = h[3], used = h[2];
+        if (nretry == 0) break;
+        idx->stats.capacity_retries++;
+        // A retried read either outgrew its wave's set space or found the
+        // output list full: grow both and re-run only those reads.
+        if (cap >= (1u << 28)) return set_error(HSFM_ENOMEM, "interval set of a single read exceeds 2^28 intervals");
+        cap *= 4;
+        const uint64_t want = used + nretry * 1024;
+        fb_cap = fb_cap * 2 > want ? fb_cap * 2 : want;
+        const uint32_t *next_list = retry;
+        retry = const_cast<uint32_t *>(list);
+        list = next_list;
+        nlist = nretry;
+    }
+    return HSFM_OK;
+}
+
+} // namespace hsfm
+
+using namespace hsfm;
+
+static int locate_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *d_qoff, uint32_t nq,
+                              hsfm_locate_device_result *out) {
+    Workspace &ws = idx->ws;
+    hipStream_t s = idx->dev->stream;
+    idx->stats = hsfm_stats{};
+    int rc = locate_search_phase(idx, d_q, d_qoff, nq);
+    if (rc) return rc;
+    if ((rc = ws.hit_off.ensure(((size_t)nq + 1) * 8))) return rc;
+    if ((rc = exclusive_scan_u32_u64(idx, ws.cnt.as<uint32_t>(), ws.hit_off.as<uint64_t>(), nq))) return rc;
+    uint64_t total = 0;
+    if ((rc = readback_u64(idx, ws.hit_off.as<uint64_t>() + nq, &total, 1))) return rc;
+    if ((rc = ws.rows.ensure(total * 4 + 4))) return rc;
+    if ((rc = ws.hits.ensure(total * 8 + 8))) return rc;
+    LocateCounters *ctr = ws.counters.as<LocateCounters>();
+    if (total) {
+        hipLaunchKernelGGL(k_expand, dim3((nq + 255) / 256), dim3(256), 0, s, nq, ws.niv.as<uint8_t>(),
+                           ws.ivs.as<uint2>(), ws.fb_meta.as<uint2>(), ws.fb_ivs.as<uint2>(),
+                           ws.hit_off.as<uint64_t>(), ws.rows.as<uint32_t>());
+        HSFM_HIP(hipGetLastError());
+        if ((rc = launch_resolve(idx, idx->fwd.view, ws.rows.as<uint32_t>(), total, idx->seq_bounds.as<uint32_t>(),
+                                 idx->nseq, ws.hits.as<uint32_t>(), &ctr->guard)))
+            return rc;
+    }
+    uint64_t h[8];
+    if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
+    idx->stats.guard_hits = h[4];
+    idx->stats.main_kernel_ms = event_ms(idx, 0);
+    idx->stats.resolve_kernel_ms = total ? event_ms(idx, 1) : 0.0;
+    if (h[4]) return set_error(HSFM_EINDEX, "%llu LF walks exceeded the sample rate: index is inconsistent", (unsigned long long)h[4]);
+    out->num_queries = nq;
+    out->total_hits = total;
+    out->d_offsets = ws.hit_off.as<uint64_t>();
+    out->d_hits = ws.hits.as<uint32_t>();
+    return HSFM_OK;
+}
+
+static int stage_queries(hsfm_index *idx, const uint8_t *qflat, const uint64_t *qoff, uint32_t nq) {
+    Workspace &ws = idx->ws;
+    hipStream_t s = idx->dev->stream;
+    const uint64_t tot = qoff[nq];
+    for (uint32_t i = 0; i < nq; i++)
+        if (qoff[i + 1] < qoff[i]) return set_error(HSFM_EINVAL, "query offsets not monotone at %u", i);
+    int rc;
+    if ((rc = ws.qcopy.ensure(tot + 16))) return rc;
+    if ((rc = ws.qoffcopy.ensure(((size_t)nq + 1) * 8))) return rc;
+    if (tot) HSFM_HIP(hipMemcpyAsync(ws.qcopy.p, qflat, tot, hipMemcpyHostToDevice, s));
+    HSFM_HIP(hipMemcpyAsync(ws.qoffcopy.p, qoff, ((size_t)nq + 1) * 8, hipMemcpyHostToDevice, s));
+    return HSFM_OK;
+}
+
+extern "C" int hsfm_locate_device(hsfm_index *idx, const uint8_t *d_qflat, const uint64_t *d_qoff, uint32_t nq,
+                                  hsfm_locate_device_result *out) {
+    if (!idx || !out || (nq && (!d_qflat || !d_qoff))) return set_error(HSFM_EINVAL, "hsfm_locate_device: null argument");
+    std::lock_guard<std::mutex> g(idx->mu);
+    HSFM_HIP(hipSetDevice(idx->dev->ordinal));
+    return locate_device_impl(idx, d_qflat, d_qoff, nq, out);
+}
+
+extern "C" int hsfm_locate_batch(hsfm_index *idx, const uint8_t *qflat, const uint64_t *qoff, uint32_t nq,
+                                 hsfm_locate_result **out) {
+    if (!idx || !out || !qoff || (nq && !qflat)) return set_error(HSFM_EINVAL, "hsfm_locate_batch: null argument");
+    std::lock_guard<std::mutex> g(idx->mu);
+    HSFM_HIP(hipSetDevice(idx->dev->ordinal));
+    int rc = stage_queries(idx, qflat, qoff, nq);
+    if (rc) return rc;
+    hsfm_locate_device_result dr;
+    if ((rc = locate_device_impl(idx, idx->ws.qcopy.as<uint8_t>(), idx->ws.qoffcopy.as<uint64_t>(), nq, &dr))) return rc;
+    const size_t hdr = (sizeof(hsfm_locate_result) + 15) & ~(size_t)15;
+    const size_t bytes = hdr + ((size_t)nq + 1) * 8 + dr.total_hits * 8;
+    char *blk = (char *)malloc(bytes);
+    if (!blk) return set_error(HSFM_ENOMEM, "host allocation of %zu bytes failed", bytes);
+    hsfm_locate_result *res = (hsfm_locate_result *)blk;
+    res->num_queries = nq;
+    res->offsets = (uint64_t *)(blk + hdr);
+    res->hits = (uint32_t *)(blk + hdr + ((size_t)nq + 1) * 8);
+    hipStream_t s = idx->dev->stream;
+    hipError_t e = hipMemcpyAsync(res->offsets, dr.d_offsets, ((size_t)nq + 1) * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && dr.total_hits)
+        e = hipMemcpyAsync(res->hits, dr.d_hits, dr.total_hits * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        free(blk);
+        return set_error(HSFM_EDEVICE, "copy of locate results failed: %s", hipGetErrorString(e));
+    }
+    *out = res;
+    return HSFM_OK;
+}
+
+extern "C" int hsfm_count_batch(hsfm_index *idx, const uint8_t *qflat, const uint64_t *qoff, uint32_t nq,
+                                uint32_t *counts) {
+    if (!idx || !counts || !qoff || (nq && !qflat)) return set_error(HSFM_EINVAL, "hsfm_count_batch: null argument");
+    std::lock_guard<std::mutex> g(idx->mu);
+    HSFM_HIP(hipSetDevice(idx->dev->ordinal));
+    idx->stats = hsfm_stats{};
+    int rc = stage_queries(idx, qflat, qoff, nq);
+    if (rc) return rc;
+    if ((rc = locate_search_phase(idx, idx->ws.qcopy.as<uint8_t>(), idx->ws.qoffcopy.as<uint64_t>(), nq))) return rc;
+    hipStream_t s = idx->dev->stream;
+    if (nq) HSFM_HIP(hipMemcpyAsync(counts, idx->ws.cnt.p, (size_t)nq * 4, hipMemcpyDeviceToHost, s));
+    HSFM_HIP(hipStreamSynchronize(s));
+    return HSFM_OK;
+}

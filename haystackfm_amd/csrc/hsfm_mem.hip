@@ -1,0 +1,410 @@
+// hsfm_mem.hip -- SMEM / MEM finding, resolve and reference mapping on gfx950.
+//
+// Output contract (BidirFmIndex::find_smems / find_mems, src/fm_index/smem.rs:71-124,
+// in the MemHit shape of resolve_mem_hits_gpu, src/fm_index/bidir_index.rs:391-514):
+// the MEMs of a query are the pairs (i, R(i)) where R(i) is the end of the longest
+// IUPAC-compatible match starting at i, R(i) - i >= min_len and i is left-maximal
+// (i == 0 or R(i-1) < R(i)).  Because R is non-decreasing, no such MEM contains
+// another, so the containment filter of find_smems (smem.rs:87-103) keeps them
+// all: find_smems and find_mems return the same spans, counts and positions
+// (tests pin this on the oracle, including the randomized smem.rs:805-870 case).
+//
+// Algorithm per read (one wave, interval sets cooperatively extended):
+//   s0 = 0, E = 0
+//   loop: j = max(s0 + min_len, E + 1); stop if j > |q|
+//     backward: extend the set of q[k..j) left from k = j while it stays non-empty
+//               -> k = L(j), the start of the longest match ending at j
+//     if k > j - min_len: no MEM of length >= min_len starts in [s0, k); s0 = k
+//     else: extend right from j -> e = R(k); emit (k, e) with its intervals;
+//           s0 = k + 1, E = e
+// Every interval is one concrete (IUPAC-compatible) string, so the bidirectional
+// pairing of bidir.rs:80-130 holds; count_smaller_than (bidir.rs:136-143) runs
+// over every lane below the extension symbol.
+#include <string.h>
+
+#include "hsfm_host.h"
+
+namespace hsfm {
+
+struct MemCounters {
+    unsigned long long work;
+    unsigned long long rec_used;
+    unsigned long long n_retry;
+    unsigned long long guard;
+    unsigned long long pad[4];
+};
+
+struct MemRec { // 32 bytes
+    uint32_t read, local, qs, qe, lo, hi, pass, pad;
+};
+
+// Extend every interval of A (ns entries) by query code c.  LEFT: P -> cP on
+// the forward index; else P -> Pc on the reverse index.  Writes the non-empty
+// children to B and returns their number; sets too_big when cap is exceeded.
+template <bool LEFT>
+__device__ uint32_t wave_extend(const DevDir &D, uint32_t L, const uint32_t *s_clane, uint32_t cm,
+                                const uint4 *A, uint32_t ns, uint4 *B, uint32_t cap, bool &too_big) {
+    const uint32_t lane = __lane_id();
+    uint32_t total = 0;
+    for (uint32_t base = 0; base < ns; base += 64) {
+        const uint32_t j = base + lane;
+        uint32_t cl[16], ch[16], cc = 0;
+        uint4 iv = make_uint4(0, 0, 0, 0);
+        if (j < ns) {
+            iv = A[j];
+            const uint32_t size = iv.y - iv.x;
+            const uint32_t xlo = LEFT ? iv.x : iv.z;
+            const uint32_t xhi = LEFT ? iv.y : iv.z + size;
+            RecView rl, rh;
+            load_rec(D, xlo, rl);
+            load_rec(D, xhi, rh);
+            lane_counts(rl, xlo, L, cl);
+            lane_counts(rh, xhi, L, ch);
+#pragma unroll
+            for (int k = 0; k < 16; k++) cc += (((cm >> k) & 1u) && ch[k] > cl[k]) ? 1u : 0u;
+        }
+        const uint32_t incl = wave_incl_scan(cc);
+        const uint32_t tot = __shfl(incl, 63, 64);
+        if (total + tot > cap) {
+            too_big = true;
+            return total;
+        }
+        if (j < ns && cc) {
+            uint32_t w = total + incl - cc;
+            uint32_t acc = 0;
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const uint32_t d = ch[k] - cl[k];
+                if (((cm >> k) & 1u) && ch[k] > cl[k]) {
+                    if (LEFT) B[w++] = make_uint4(s_clane[k] + cl[k], s_clane[k] + ch[k], iv.z + acc, 0u);
+                    else B[w++] = make_uint4(iv.x + acc, iv.x + acc + d, s_clane[k] + cl[k], 0u);
+                }
+                acc += d;
+            }
+        }
+        total += tot;
+    }
+    return total;
+}
+
+__global__ __launch_bounds__(64) void k_mem_wave(DevDir fwd, DevDir rev, const DevConst *__restrict__ cst,
+                                                 const uint8_t *__restrict__ q, const uint64_t *__restrict__ qoff,
+                                                 const uint32_t *__restrict__ list, uint32_t nlist, uint32_t min_len,
+                                                 uint32_t cap, uint4 *__restrict__ sets, MemRec *__restrict__ recs,
+                                                 uint64_t rec_cap, uint32_t *__restrict__ mem_cnt,
+                                                 uint32_t *__restrict__ pass_of, uint32_t pass,
+                                                 uint32_t *__restrict__ retry_list, MemCounters *__restrict__ ctr) {
+    __shared__ uint16_t s_compat[256];
+    __shared__ uint32_t s_clane[16];
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_compat[i] = cst->compat_lanes[i];
+    if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
+    __syncthreads();
+    const uint32_t lane = threadIdx.x;
+    const uint32_t L = cst->num_lanes;
+    const uint32_t n = fwd.n;
+    uint4 *bufA = sets + (size_t)blockIdx.x * 2 * cap;
+    uint4 *bufB = bufA + cap;
+    for (;;) {
+        uint32_t wi = 0;
+        if (lane == 0) wi = (uint32_t)atomicAdd(&ctr->work, 1ull);
+        wi = __shfl(wi, 0, 64);
+        if (wi >= nlist) break;
+        const uint32_t r = list ? list[wi] : wi;
+        const uint64_t b0 = qoff[r];
+        const uint32_t m = (uint32_t)(qoff[r + 1] - b0);
+        const uint8_t *p = q + b0;
+        uint32_t s0 = 0, E = 0, nrec = 0;
+        bool too_big = false;
+        while (min_len && !too_big) {
+            const uint32_t j = (s0 + min_len > E + 1) ? s0 + min_len : E + 1;
+            if (j > m) break;
+            uint4 *A = bufA, *B = bufB;
+            if (lane == 0) A[0] = make_uint4(0, n, 0, 0);
+            __syncthreads();
+            uint32_t ns = 1, k = j;
+            while (k > 0) {
+                const uint32_t c2 = wave_extend<true>(fwd, L, s_clane, s_compat[p[k - 1]], A, ns, B, cap, too_big);
+                __syncthreads();
+                if (too_big || c2 == 0) break;
+                uint4 *t = A; A = B; B = t;
+                ns = c2;
+                k--;
+            }
+            if (too_big) break;
+            if (k + min_len > j) { // k > j - min_len
+                s0 = k;
+                continue;
+            }
+            uint32_t e = j;
+            while (e < m) {
+                const uint32_t c2 = wave_extend<false>(rev, L, s_clane, s_compat[p[e]], A, ns, B, cap, too_big);
+                __syncthreads();
+                if (too_big || c2 == 0) break;
+                uint4 *t = A; A = B; B = t;
+                ns = c2;
+                e++;
+            }
+            if (too_big) break;
+            // emit (k, e): one record per interval, sorted by fwd_lo
+            unsigned long long slot = 0;
+            if (lane == 0) slot = atomicAdd(&ctr->rec_used, (unsigned long long)ns);
+            slot = __shfl(slot, 0, 64);
+            if (slot + ns > rec_cap) {
+                too_big = true;
+                break;
+            }
+            for (uint32_t t = lane; t < ns; t += 64) {
+                const uint4 iv = A[t];
+                uint32_t rank = 0;
+                for (uint32_t u = 0; u < ns; u++) rank += A[u].x < iv.x ? 1u : 0u;
+                MemRec rc;
+                rc.read = r;
+                rc.local = nrec + rank;
+                rc.qs = k;
+                rc.qe = e;
+                rc.lo = iv.x;
+                rc.hi = iv.y;
+                rc.pass = pass;
+                rc.pad = 0;
+                recs[slot + rank] = rc;
+            }
+            nrec += ns;
+            s0 = k + 1;
+            E = e;
+            __syncthreads();
+        }
+        if (lane == 0) {
+            if (too_big) {
+                const unsigned long long at = atomicAdd(&ctr->n_retry, 1ull);
+                retry_list[at] = r;
+            } else {
+                mem_cnt[r] = nrec;
+                pass_of[r] = pass;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Place the records of successful attempts at their final position.
+__global__ __launch_bounds__(256) void k_mem_scatter(const MemRec *__restrict__ recs, uint64_t nrec,
+                                                     const uint32_t *__restrict__ pass_of,
+                                                     const uint64_t *__restrict__ mem_off, uint4 *__restrict__ out) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrec) return;
+    const MemRec rc = recs[t];
+    if (pass_of[rc.read] != rc.pass) return;
+    out[mem_off[rc.read] + rc.local] = make_uint4(rc.qs, rc.qe, rc.lo, rc.hi);
+}
+
+// Per record: positions to resolve = min(size, max_hits) (bidir_index.rs:315-337).
+__global__ __launch_bounds__(256) void k_mem_npos(const uint4 *__restrict__ recs, uint64_t nrec, uint32_t max_hits,
+                                                  uint32_t resolve, uint32_t *__restrict__ npos) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrec) return;
+    const uint4 r = recs[t];
+    const uint32_t size = r.w - r.z;
+    npos[t] = resolve ? (size < max_hits ? size : max_hits) : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_mem_hits(const uint4 *__restrict__ recs, uint64_t nrec, uint32_t max_hits,
+                                                  const uint64_t *__restrict__ pos_off, hsfm_mem_hit *__restrict__ hits,
+                                                  uint32_t *__restrict__ rows) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrec) return;
+    const uint4 r = recs[t];
+    const uint32_t size = r.w - r.z;
+    const uint64_t po = pos_off[t], pl = pos_off[t + 1] - po;
+    hsfm_mem_hit h;
+    h.query_start = r.x;
+    h.query_end = r.y;
+    h.match_count = size;
+    h.truncated = size > max_hits ? 1u : 0u;
+    h.pos_offset = po;
+    h.pos_len = (uint32_t)pl;
+    h._reserved = 0;
+    hits[t] = h;
+    for (uint32_t i = 0; i < (uint32_t)pl; i++) rows[po + i] = r.z + i;
+}
+
+} // namespace hsfm
+
+using namespace hsfm;
+
+static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *d_qoff, uint32_t nq, uint32_t min_len,
+                           const uint32_t *d_refb, uint32_t n_refs, uint32_t max_hits, hsfm_mem_device_result *out) {
+    Workspace &ws = idx->ws;
+    hipStream_t s = idx->dev->stream;
+    idx->stats = hsfm_stats{};
+    int rc;
+    if ((rc = ws.mem_cnt.ensure((size_t)nq * 4 + 4))) return rc;
+    if ((rc = ws.mem_flags.ensure((size_t)nq * 4 + 4))) return rc;
+    if ((rc = ws.ovf_list.ensure((size_t)nq * 4 + 4))) return rc;
+    if ((rc = ws.read_list.ensure((size_t)nq * 4 + 4))) return rc;
+    if ((rc = ws.counters.ensure(sizeof(MemCounters)))) return rc;
+    if ((rc = ws.mem_off.ensure(((size_t)nq + 1) * 8))) return rc;
+    MemCounters *ctr = ws.counters.as<MemCounters>();
+    if ((rc = fill_u32(idx, ws.counters.as<uint32_t>(), sizeof(MemCounters) / 4, 0))) return rc;
+    if ((rc = fill_u32(idx, ws.mem_cnt.as<uint32_t>(), nq, 0))) return rc;
+    if ((rc = fill_u32(idx, ws.mem_flags.as<uint32_t>(), nq, 0xffffffffu))) return rc;
+
+    uint64_t rec_cap = (uint64_t)nq * 8 + 4096;
+    if (ws.mem_recs.cap / sizeof(MemRec) > rec_cap) rec_cap = ws.mem_recs.cap / sizeof(MemRec);
+    uint32_t cap = 1024;
+    const uint32_t *list = nullptr; // identity list on the first pass
+    uint32_t *retry = ws.ovf_list.as<uint32_t>();
+    uint64_t nlist = nq;
+    uint32_t pass = 0;
+    uint64_t h[8];
+    while (nlist) {
+        uint64_t waves = nlist < (uint64_t)idx->dev->num_cus * 8 ? nlist : (uint64_t)idx->dev->num_cus * 8;
+        const uint64_t wave_limit = (4ull << 30) / (2ull * cap * 16);
+        if (waves > wave_limit) waves = wave_limit;
+        if (waves == 0) waves = 1;
+        if ((rc = ws.wave_sets.ensure((size_t)waves * 2 * cap * 16))) return rc;
+        if (ws.mem_recs.cap < rec_cap * sizeof(MemRec)) {
+            DevBuf grown;
+            if ((rc = grown.ensure(rec_cap * sizeof(MemRec)))) return rc;
+            if (ws.mem_recs.p) HSFM_HIP(hipMemcpyAsync(grown.p, ws.mem_recs.p, ws.mem_recs.cap, hipMemcpyDeviceToDevice, s));
+            HSFM_HIP(hipStreamSynchronize(s));
+            ws.mem_recs.release();
+            ws.mem_recs = grown;
+        }
+        rec_cap = ws.mem_recs.cap / sizeof(MemRec);
+        if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->work), 2, 0))) return rc;
+        if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_retry), 2, 0))) return rc;
+        if (pass == 0 && (rc = event_record(idx, 0, false))) return rc;
+        hipLaunchKernelGGL(k_mem_wave, dim3((uint32_t)waves), dim3(64), 0, s, idx->fwd.view, idx->rev.view,
+                           idx->d_const.as<DevConst>(), d_q, d_qoff, list, (uint32_t)nlist, min_len, cap,
+                           ws.wave_sets.as<uint4>(), ws.mem_recs.as<MemRec>(), rec_cap, ws.mem_cnt.as<uint32_t>(),
+                           ws.mem_flags.as<uint32_t>(), pass, retry, ctr);
+        HSFM_HIP(hipGetLastError());
+        if (pass == 0 && (rc = event_record(idx, 0, true))) return rc;
+        if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
+        const uint64_t nretry = h[2], used = h[1];
+        if (nretry == 0) break;
+        idx->stats.capacity_retries++;
+        idx->stats.fallback_reads += nretry;
+        if (cap >= (1u << 26)) return set_error(HSFM_ENOMEM, "interval set of a single read exceeds 2^26 intervals");
+        cap *= 4;
+        const uint64_t want = used + nretry * 4096;
+        rec_cap = rec_cap * 2 > want ? rec_cap * 2 : want;
+        // next pass runs only the retried reads
+        uint32_t *next_list = retry;
+        retry = (next_list == ws.ovf_list.as<uint32_t>()) ? ws.read_list.as<uint32_t>() : ws.ovf_list.as<uint32_t>();
+        list = next_list;
+        nlist = nretry;
+        pass++;
+    }
+    const uint64_t rec_used = h[1];
+    if ((rc = exclusive_scan_u32_u64(idx, ws.mem_cnt.as<uint32_t>(), ws.mem_off.as<uint64_t>(), nq))) return rc;
+    uint64_t nhits = 0;
+    if ((rc = readback_u64(idx, ws.mem_off.as<uint64_t>() + nq, &nhits, 1))) return rc;
+    idx->stats.records = nhits;
+    if ((rc = ws.mem_sorted.ensure(nhits * 16 + 16))) return rc;
+    if (rec_used) {
+        hipLaunchKernelGGL(k_mem_scatter, dim3((uint32_t)((rec_used + 255) / 256)), dim3(256), 0, s,
+                           ws.mem_recs.as<MemRec>(), rec_used, ws.mem_flags.as<uint32_t>(), ws.mem_off.as<uint64_t>(),
+                           ws.mem_sorted.as<uint4>());
+        HSFM_HIP(hipGetLastError());
+    }
+    if ((rc = ws.cnt.ensure(nhits * 4 + 4))) return rc;
+    if ((rc = ws.pos_off.ensure((nhits + 1) * 8))) return rc;
+    if (nhits) {
+        hipLaunchKernelGGL(k_mem_npos, dim3((uint32_t)((nhits + 255) / 256)), dim3(256), 0, s,
+                           ws.mem_sorted.as<uint4>(), nhits, max_hits, n_refs ? 1u : 0u, ws.cnt.as<uint32_t>());
+        HSFM_HIP(hipGetLastError());
+    }
+    if ((rc = exclusive_scan_u32_u64(idx, ws.cnt.as<uint32_t>(), ws.pos_off.as<uint64_t>(), (uint32_t)nhits))) return rc;
+    uint64_t npos = 0;
+    if ((rc = readback_u64(idx, ws.pos_off.as<uint64_t>() + nhits, &npos, 1))) return rc;
+    if ((rc = ws.mem_hits.ensure(nhits * sizeof(hsfm_mem_hit) + 64))) return rc;
+    if ((rc = ws.rows.ensure(npos * 4 + 4))) return rc;
+    if ((rc = ws.positions.ensure(npos * 8 + 8))) return rc;
+    if (nhits) {
+        hipLaunchKernelGGL(k_mem_hits, dim3((uint32_t)((nhits + 255) / 256)), dim3(256), 0, s,
+                           ws.mem_sorted.as<uint4>(), nhits, max_hits, ws.pos_off.as<uint64_t>(),
+                           ws.mem_hits.as<hsfm_mem_hit>(), ws.rows.as<uint32_t>());
+        HSFM_HIP(hipGetLastError());
+    }
+    if (npos) {
+        if ((rc = launch_resolve(idx, idx->fwd.view, ws.rows.as<uint32_t>(), npos, d_refb, n_refs,
+                                 ws.positions.as<uint32_t>(), &ctr->guard)))
+            return rc;
+        if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
+        idx->stats.guard_hits = h[3];
+        if (h[3]) return set_error(HSFM_EINDEX, "%llu LF walks exceeded the sample rate: index is inconsistent", (unsigned long long)h[3]);
+    }
+    idx->stats.main_kernel_ms = event_ms(idx, 0);
+    idx->stats.resolve_kernel_ms = npos ? event_ms(idx, 1) : 0.0;
+    out->num_queries = nq;
+    out->num_hits = nhits;
+    out->num_positions = npos;
+    out->d_hit_offsets = ws.mem_off.as<uint64_t>();
+    out->d_hits = ws.mem_hits.as<hsfm_mem_hit>();
+    out->d_positions = ws.positions.as<uint32_t>();
+    return HSFM_OK;
+}
+
+extern "C" int hsfm_find_mems_device(hsfm_index *idx, const uint8_t *d_qflat, const uint64_t *d_qoff, uint32_t nq,
+                                     uint32_t min_len, int mode, const uint32_t *d_ref_boundaries, uint32_t n_refs,
+                                     uint32_t max_hits, hsfm_mem_device_result *out) {
+    if (!idx || !out || (nq && (!d_qflat || !d_qoff)) || (n_refs && !d_ref_boundaries))
+        return set_error(HSFM_EINVAL, "hsfm_find_mems_device: null argument");
+    if (!idx->has_rev) return set_error(HSFM_EINVAL, "MEM search needs an index uploaded with its reverse index");
+    if (mode != HSFM_MODE_SMEM && mode != HSFM_MODE_MEM) return set_error(HSFM_EINVAL, "unknown mode %d", mode);
+    std::lock_guard<std::mutex> g(idx->mu);
+    HSFM_HIP(hipSetDevice(idx->dev->ordinal));
+    return mem_device_impl(idx, d_qflat, d_qoff, nq, min_len, d_ref_boundaries, n_refs, max_hits, out);
+}
+
+extern "C" int hsfm_find_mems_batch(hsfm_index *idx, const uint8_t *qflat, const uint64_t *qoff, uint32_t nq,
+                                    uint32_t min_len, int mode, const uint32_t *ref_boundaries, uint32_t n_refs,
+                                    uint32_t max_hits, hsfm_mem_result **out) {
+    if (!idx || !out || !qoff || (nq && !qflat) || (n_refs && !ref_boundaries))
+        return set_error(HSFM_EINVAL, "hsfm_find_mems_batch: null argument");
+    if (!idx->has_rev) return set_error(HSFM_EINVAL, "MEM search needs an index uploaded with its reverse index");
+    if (mode != HSFM_MODE_SMEM && mode != HSFM_MODE_MEM) return set_error(HSFM_EINVAL, "unknown mode %d", mode);
+    std::lock_guard<std::mutex> g(idx->mu);
+    HSFM_HIP(hipSetDevice(idx->dev->ordinal));
+    Workspace &ws = idx->ws;
+    hipStream_t s = idx->dev->stream;
+    for (uint32_t i = 0; i < nq; i++)
+        if (qoff[i + 1] < qoff[i]) return set_error(HSFM_EINVAL, "query offsets not monotone at %u", i);
+    const uint64_t tot = qoff[nq];
+    int rc;
+    if ((rc = ws.qcopy.ensure(tot + 16))) return rc;
+    if ((rc = ws.qoffcopy.ensure(((size_t)nq + 1) * 8))) return rc;
+    if ((rc = ws.refb.ensure((size_t)n_refs * 4 + 4))) return rc;
+    if (tot) HSFM_HIP(hipMemcpyAsync(ws.qcopy.p, qflat, tot, hipMemcpyHostToDevice, s));
+    HSFM_HIP(hipMemcpyAsync(ws.qoffcopy.p, qoff, ((size_t)nq + 1) * 8, hipMemcpyHostToDevice, s));
+    if (n_refs) HSFM_HIP(hipMemcpyAsync(ws.refb.p, ref_boundaries, (size_t)n_refs * 4, hipMemcpyHostToDevice, s));
+    hsfm_mem_device_result dr;
+    if ((rc = mem_device_impl(idx, ws.qcopy.as<uint8_t>(), ws.qoffcopy.as<uint64_t>(), nq, min_len,
+                              ws.refb.as<uint32_t>(), n_refs, max_hits, &dr)))
+        return rc;
+    const size_t hdr = (sizeof(hsfm_mem_result) + 15) & ~(size_t)15;
+    const size_t b_off = ((size_t)nq + 1) * 8;
+    const size_t b_hits = dr.num_hits * sizeof(hsfm_mem_hit);
+    const size_t bytes = hdr + b_off + b_hits + dr.num_positions * 8;
+    char *blk = (char *)malloc(bytes);
+    if (!blk) return set_error(HSFM_ENOMEM, "host allocation of %zu bytes failed", bytes);
+    hsfm_mem_result *res = (hsfm_mem_result *)blk;
+    res->num_queries = nq;
+    res->hit_offsets = (uint64_t *)(blk + hdr);
+    res->hits = (hsfm_mem_hit *)(blk + hdr + b_off);
+    res->positions = (uint32_t *)(blk + hdr + b_off + b_hits);
+    res->num_hits = dr.num_hits;
+    res->num_positions = dr.num_positions;
+    hipError_t e = hipMemcpyAsync(res->hit_offsets, dr.d_hit_offsets, b_off, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && b_hits) e = hipMemcpyAsync(res->hits, dr.d_hits, b_hits, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && dr.num_positions)
+        e = hipMemcpyAsync(res->positions, dr.d_positions, dr.num_positions * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        free(blk);
+        return set_error(HSFM_EDEVICE, "copy of MEM results failed: %s", hipGetErrorString(e));
+    }
+    *out = res;
+    return HSFM_OK;
+}

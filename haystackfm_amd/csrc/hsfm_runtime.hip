@@ -1,0 +1,228 @@
+// hsfm_runtime.hip -- device context, errors, buffers, scans and the shared
+// LF-walk resolve kernel of libhsfm.
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include "hsfm_host.h"
+
+namespace hsfm {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int DevBuf::ensure(size_t bytes) {
+    if (bytes <= cap && p) return HSFM_OK;
+    if (p) {
+        (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    size_t want = bytes < 256 ? 256 : bytes;
+    want = (want + 255) & ~(size_t)255;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return set_error(HSFM_ENOMEM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    cap = want;
+    return HSFM_OK;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+__global__ void k_fill_u32(uint32_t *d, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        d[i] = v;
+}
+
+int fill_u32(hsfm_index *idx, uint32_t *d, uint64_t n, uint32_t v) {
+    if (n == 0) return HSFM_OK;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, idx->dev->stream, d, n, v);
+    HSFM_HIP(hipGetLastError());
+    return HSFM_OK;
+}
+
+int event_record(hsfm_index *idx, int slot, bool end) {
+    hipEvent_t &e = idx->ws.ev[2 * slot + (end ? 1 : 0)];
+    if (!e) HSFM_HIP(hipEventCreate(&e));
+    HSFM_HIP(hipEventRecord(e, idx->dev->stream));
+    return HSFM_OK;
+}
+
+double event_ms(hsfm_index *idx, int slot) {
+    hipEvent_t a = idx->ws.ev[2 * slot], b = idx->ws.ev[2 * slot + 1];
+    if (!a || !b) return 0.0;
+    float ms = 0.f;
+    if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return (double)ms;
+}
+
+struct CastU64 {
+    __host__ __device__ uint64_t operator()(uint32_t v) const { return (uint64_t)v; }
+};
+
+__global__ void k_zero_first(uint64_t *d) { d[0] = 0; }
+
+int exclusive_scan_u32_u64(hsfm_index *idx, const uint32_t *d_in, uint64_t *d_out, uint32_t n) {
+    hipStream_t s = idx->dev->stream;
+    hipLaunchKernelGGL(k_zero_first, dim3(1), dim3(1), 0, s, d_out);
+    HSFM_HIP(hipGetLastError());
+    if (n == 0) return HSFM_OK;
+    hipcub::TransformInputIterator<uint64_t, CastU64, const uint32_t *> it(d_in, CastU64());
+    size_t tmp = 0;
+    HSFM_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, it, d_out + 1, (int)n, s));
+    int rc = idx->ws.scan_tmp.ensure(tmp);
+    if (rc) return rc;
+    HSFM_HIP(hipcub::DeviceScan::InclusiveSum(idx->ws.scan_tmp.p, tmp, it, d_out + 1, (int)n, s));
+    return HSFM_OK;
+}
+
+int readback_u64(hsfm_index *idx, const uint64_t *d_src, uint64_t *h_dst, int count) {
+    hipStream_t s = idx->dev->stream;
+    if (!idx->ws.h_pinned) HSFM_HIP(hipHostMalloc((void **)&idx->ws.h_pinned, 4096, hipHostMallocDefault));
+    HSFM_HIP(hipMemcpyAsync(idx->ws.h_pinned, d_src, (size_t)count * 8, hipMemcpyDeviceToHost, s));
+    HSFM_HIP(hipStreamSynchronize(s));
+    memcpy(h_dst, idx->ws.h_pinned, (size_t)count * 8);
+    return HSFM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Resolve: one thread per SA row.  LF-walk to the nearest sampled row
+// (src/fm_index/query.rs:282-291, same walk as locate_resolve.wgsl:138-175 and
+// mem_resolve.wgsl:123-135), then the boundary partition point of
+// FmIndex::map_position (query.rs:342-357) / ref_map.wgsl:31-59.  The walk is
+// bounded by sample_rate steps (the reference GPU guard); a valid index never
+// reaches it, a counter records it if it does.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_resolve(DevDir d, const DevConst *__restrict__ cst,
+                                                 const uint32_t *__restrict__ rows, uint64_t nrows,
+                                                 const uint32_t *__restrict__ bounds, uint32_t nb,
+                                                 uint32_t *__restrict__ out,
+                                                 unsigned long long *__restrict__ guard) {
+    __shared__ uint32_t s_clane[16];
+    if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
+    __syncthreads();
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrows) return;
+    const uint32_t nm1 = d.n - 1;
+    uint32_t row = rows[t];
+    row = row < nm1 ? row : nm1;
+    uint32_t steps = 0, pos = 0xffffffffu;
+    for (;;) {
+        const uint4 w = d.samp[row >> 6];
+        RecView r;
+        load_rec(d, row, r);
+        const uint64_t bits = (uint64_t)w.x | ((uint64_t)w.y << 32);
+        const uint32_t off = row & 63;
+        if ((bits >> off) & 1ull) {
+            uint32_t rk = w.z + popc64(bits & ((1ull << off) - 1));
+            rk = rk < d.num_samples ? rk : d.num_samples - 1;
+            pos = d.sa_vals[rk] + steps;
+            break;
+        }
+        if (steps >= d.sample_rate) {
+            atomicAdd(guard, 1ull);
+            break;
+        }
+        uint32_t lane, rank;
+        lf_lane_rank(r, row, lane, rank);
+        const uint32_t nx = s_clane[lane] + rank;
+        row = nx < nm1 ? nx : nm1;
+        steps++;
+    }
+    uint32_t id = 0xffffffffu, o = 0xffffffffu;
+    if (pos != 0xffffffffu) {
+        uint32_t lo = 0, hi = nb;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (bounds[mid] <= pos) lo = mid + 1;
+            else hi = mid;
+        }
+        id = lo;
+        o = pos - (lo ? bounds[lo - 1] : 0u);
+    }
+    out[2 * t] = id;
+    out[2 * t + 1] = o;
+}
+
+int launch_resolve(hsfm_index *idx, const DevDir &d, const uint32_t *d_rows, uint64_t nrows,
+                   const uint32_t *d_bounds, uint32_t nbounds, uint32_t *d_out_pairs,
+                   unsigned long long *d_guard) {
+    if (nrows == 0) return HSFM_OK;
+    const uint64_t blocks = (nrows + 255) / 256;
+    int rc = event_record(idx, 1, false);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_resolve, dim3((uint32_t)blocks), dim3(256), 0, idx->dev->stream, d,
+                       idx->d_const.as<DevConst>(), d_rows, nrows, d_bounds, nbounds, d_out_pairs,
+                       d_guard);
+    HSFM_HIP(hipGetLastError());
+    return event_record(idx, 1, true);
+}
+
+} // namespace hsfm
+
+using namespace hsfm;
+
+extern "C" const char *hsfm_last_error(void) { return g_err.c_str(); }
+
+extern "C" int hsfm_device_open(int ordinal, hsfm_device **out) {
+    if (!out) return set_error(HSFM_EINVAL, "hsfm_device_open: null out");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+        return set_error(HSFM_EDEVICE, "no HIP device available (%s)", hipGetErrorString(e));
+    if (ordinal < 0 || ordinal >= n)
+        return set_error(HSFM_EINVAL, "device ordinal %d out of range (%d devices)", ordinal, n);
+    HSFM_HIP(hipSetDevice(ordinal));
+    hipDeviceProp_t prop;
+    HSFM_HIP(hipGetDeviceProperties(&prop, ordinal));
+    hsfm_device *d = new hsfm_device();
+    d->ordinal = ordinal;
+    d->num_cus = prop.multiProcessorCount;
+    e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete d;
+        return set_error(HSFM_EDEVICE, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
+    *out = d;
+    return HSFM_OK;
+}
+
+extern "C" int hsfm_device_close(hsfm_device *dev) {
+    if (!dev) return HSFM_OK;
+    (void)hipSetDevice(dev->ordinal);
+    if (dev->stream) (void)hipStreamDestroy(dev->stream);
+    delete dev;
+    return HSFM_OK;
+}
+
+extern "C" void hsfm_free_result(void *r) {
+    // Results are single malloc blocks (header + payload).
+    free(r);
+}
+
+extern "C" int hsfm_last_stats(hsfm_index *idx, hsfm_stats *out) {
+    if (!idx || !out) return set_error(HSFM_EINVAL, "hsfm_last_stats: null argument");
+    *out = idx->stats;
+    return HSFM_OK;
+}

@@ -1,0 +1,185 @@
+/*
+ * hsfm.h -- C ABI of the MI355X (gfx950) batched query path of haystackfm.
+ *
+ * This is the drop-in boundary.  The reference crate (sriram98v/haystackfm)
+ * exposes its GPU query path as Rust methods; each entry point below replaces
+ * one of them and is what that crate's FFI layer binds (see INTEGRATION.md for
+ * the extern "C" block and the Rust wrappers).  Plain pointers and sizes only.
+ *
+ * Conventions
+ *   - Every function returns 0 on success, a negative HSFM_E* code on error;
+ *     hsfm_last_error() then returns a thread-local message.  This maps to the
+ *     reference's FmIndexError::GpuError(String) (src/error.rs:26-29) and, for
+ *     malformed index bytes, FmIndexError::DeserializeError (src/error.rs:18-20).
+ *   - Query bytes are the reference's encoded IUPAC codes ($=0, A..T=1..4, N=5,
+ *     R..V=6..15; src/alphabet.rs:18-52); any other byte matches nothing.
+ *   - Queries are CSR: qflat (concatenated codes) + qoff[nq+1] (u64 offsets).
+ *   - Host-buffer entry points copy inputs to the device and results back.
+ *     *_device entry points take device pointers and leave results in
+ *     library-owned device memory that stays valid until the next call on the
+ *     same index.  Calls on one index are serialized internally.
+ */
+#ifndef HSFM_H
+#define HSFM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HSFM_OK 0
+#define HSFM_EINVAL (-1)    /* bad argument or inconsistent index view      */
+#define HSFM_EDESER (-2)    /* malformed FmIndex byte stream                */
+#define HSFM_EDEVICE (-3)   /* HIP runtime / device error                   */
+#define HSFM_ENOMEM (-4)    /* device or host allocation failed             */
+#define HSFM_EINDEX (-5)    /* index invariant violated at query time       */
+
+typedef struct hsfm_device hsfm_device; /* GpuContext (src/gpu/mod.rs:19-66)       */
+typedef struct hsfm_index hsfm_index;   /* device-resident FmIndex / BidirFmIndex */
+
+/*
+ * Borrowed view of one FmIndex, field for field as serialized by
+ * FmIndex::to_bytes (src/fm_index/serialize.rs:55-67; OccTable
+ * src/occ/mod.rs:44-78; SampledSuffixArray src/suffix_array/mod.rs:54-63;
+ * LookupTable src/fm_index/lookup.rs:19-27).  compat_mask[b] is the bitmask of
+ * reference symbols that alphabet_fns.compatible_fn(b) returns
+ * (src/alphabet.rs:203-212), so custom Alphabet impls keep their semantics.
+ */
+typedef struct {
+    uint32_t c_array[16];
+    uint8_t occ_num_lanes, occ_num_planes;
+    uint8_t occ_symbol_to_lane[16], occ_lane_to_symbol[16];
+    const uint32_t *occ_superblock_checkpoints; uint64_t occ_superblock_len;
+    const uint8_t *occ_block_data; uint64_t occ_block_data_len;
+    uint64_t occ_block_stride;
+    uint32_t occ_encoding; /* 0 = Bitplane, 1 = OneHot (occ/mod.rs:93-97) */
+    uint32_t occ_text_len;
+    const uint8_t *sa_word_data; uint64_t sa_word_data_len;
+    const uint32_t *sa_vals; uint64_t sa_vals_len;
+    uint32_t sa_sample_rate, sa_text_len;
+    uint32_t text_len, num_sequences;
+    const uint32_t *seq_boundaries; uint64_t seq_boundaries_len;
+    uint32_t has_lookup, lookup_depth;
+    const uint8_t *lookup_core_symbols; uint64_t lookup_core_len;
+    const uint32_t *lookup_intervals; uint64_t lookup_intervals_len; /* (lo,hi) pairs */
+    uint8_t alphabet_tag;
+    uint16_t compat_mask[256];
+} hsfm_index_view;
+
+/* Device context.  Replaces GpuContext::new + context_cache::get_or_init
+ * (src/gpu/mod.rs:26-66, src/gpu/context_cache.rs:11-19). */
+int hsfm_device_open(int ordinal, hsfm_device **out);
+int hsfm_device_close(hsfm_device *dev);
+
+/* Zero-copy parse of an FmIndex::to_bytes stream into a view whose pointers
+ * point into `bytes`.  Replaces the bincode half of FmIndex::from_bytes
+ * (src/fm_index/serialize.rs:32-52); unknown alphabet tags are rejected as
+ * there.  compat_mask is filled for the built-in tags 0 (IupacDna) and
+ * 1 (ExactDna) (src/alphabet.rs:240-254). */
+int hsfm_index_view_from_bytes(const uint8_t *bytes, uint64_t len, hsfm_index_view *out);
+
+/* Upload one FmIndex (rev == NULL) or a BidirFmIndex (fwd + rev) and convert
+ * it to the device layout.  Replaces the per-call uploads of
+ * locate_batch_gpu (src/gpu/locate.rs:69-111), FindIndexBuffers::new
+ * (src/gpu/mem_find.rs:56-102) and ResolveIndexBuffers::new
+ * (src/gpu/mem_resolve.rs:44-84): here the index is uploaded once and stays
+ * resident.  The views are only read during the call. */
+int hsfm_index_upload(hsfm_device *dev, const hsfm_index_view *fwd, const hsfm_index_view *rev,
+                      hsfm_index **out);
+int hsfm_index_free(hsfm_index *idx);
+
+/* ---- batched locate ----------------------------------------------------- */
+typedef struct {
+    uint32_t num_queries;
+    uint64_t *offsets;  /* num_queries + 1 */
+    uint32_t *hits;     /* 2 * offsets[num_queries]: (seq_idx, pos_in_seq) */
+} hsfm_locate_result;
+
+/* locate_batch_gpu (src/gpu/locate.rs:52-56) / FmIndex::locate_gpu
+ * (src/fm_index/query.rs:367-384): per query, the (seq_idx, pos) of every
+ * occurrence, with FmIndex::locate's semantics (query.rs:22-39, 219-277)
+ * including lookup-table seeding; hits of a query are in ascending SA-row
+ * order.  Free the result with hsfm_free_result. */
+int hsfm_locate_batch(hsfm_index *idx, const uint8_t *qflat, const uint64_t *qoff, uint32_t nq,
+                      hsfm_locate_result **out);
+/* FmIndex::count over a batch (src/fm_index/query.rs:9-14); counts[nq]. */
+int hsfm_count_batch(hsfm_index *idx, const uint8_t *qflat, const uint64_t *qoff, uint32_t nq,
+                     uint32_t *counts);
+
+typedef struct {
+    uint32_t num_queries;
+    uint64_t total_hits;
+    const uint64_t *d_offsets; /* device, num_queries + 1 */
+    const uint32_t *d_hits;    /* device, 2 * total_hits */
+} hsfm_locate_device_result;
+/* Device-resident variant of hsfm_locate_batch: d_qflat / d_qoff are device
+ * pointers; results stay in device memory owned by idx. */
+int hsfm_locate_device(hsfm_index *idx, const uint8_t *d_qflat, const uint64_t *d_qoff,
+                       uint32_t nq, hsfm_locate_device_result *out);
+
+/* ---- SMEM / MEM ---------------------------------------------------------- */
+#define HSFM_MODE_SMEM 0 /* find_smems_gpu (src/fm_index/bidir_index.rs:225-245) */
+#define HSFM_MODE_MEM 1  /* find_mems_gpu  (src/fm_index/bidir_index.rs:256-276) */
+
+/* One MemHit (src/gpu/mem_hit.rs:6-19): one record per (MEM, SA interval),
+ * match_count = interval size, positions = the first min(size, max_hits) rows
+ * of the interval in SA order, truncated = size > max_hits
+ * (bidir_index.rs:452-468, 307-359).  Records of a query are sorted by
+ * (query_start, query_end, interval start). */
+typedef struct {
+    uint32_t query_start, query_end, match_count, truncated;
+    uint64_t pos_offset; /* into positions (pairs) */
+    uint32_t pos_len, _reserved;
+} hsfm_mem_hit;
+
+typedef struct {
+    uint32_t num_queries;
+    uint64_t *hit_offsets;  /* num_queries + 1, into hits */
+    hsfm_mem_hit *hits;
+    uint32_t *positions;    /* 2 * total: (ref_id, offset_within_ref) */
+    uint64_t num_hits, num_positions;
+} hsfm_mem_result;
+
+/* BidirFmIndex::find_smems_gpu / find_mems_gpu (bidir_index.rs:225-276) with
+ * the CPU definitions find_smems / find_mems (src/fm_index/smem.rs:71-124).
+ * ref_boundaries[i] = exclusive end of reference i in the concatenated text;
+ * n_refs == 0 skips position resolution (positions empty), as the reference
+ * does.  idx must have been uploaded with a reverse index. */
+int hsfm_find_mems_batch(hsfm_index *idx, const uint8_t *qflat, const uint64_t *qoff, uint32_t nq,
+                         uint32_t min_len, int mode, const uint32_t *ref_boundaries,
+                         uint32_t n_refs, uint32_t max_hits, hsfm_mem_result **out);
+
+typedef struct {
+    uint32_t num_queries;
+    uint64_t num_hits, num_positions;
+    const uint64_t *d_hit_offsets; /* device, num_queries + 1 */
+    const hsfm_mem_hit *d_hits;    /* device */
+    const uint32_t *d_positions;   /* device, 2 * num_positions */
+} hsfm_mem_device_result;
+/* Device-resident variant: d_qflat, d_qoff and d_ref_boundaries are device
+ * pointers; results stay in device memory owned by idx. */
+int hsfm_find_mems_device(hsfm_index *idx, const uint8_t *d_qflat, const uint64_t *d_qoff,
+                          uint32_t nq, uint32_t min_len, int mode,
+                          const uint32_t *d_ref_boundaries, uint32_t n_refs, uint32_t max_hits,
+                          hsfm_mem_device_result *out);
+
+/* ---- misc ---------------------------------------------------------------- */
+void hsfm_free_result(void *result);
+const char *hsfm_last_error(void);
+
+/* Statistics of the last call on idx (fallback path usage, walk-guard hits). */
+typedef struct {
+    uint64_t fallback_reads;   /* reads that left the register fast path   */
+    uint64_t capacity_retries; /* relaunches with larger per-wave set space */
+    uint64_t guard_hits;       /* LF walks stopped by the sample-rate guard */
+    uint64_t records;          /* MEM interval records emitted               */
+    double main_kernel_ms;     /* search / MEM-find kernel, first pass (HIP events) */
+    double resolve_kernel_ms;  /* LF-walk resolve kernel (HIP events)             */
+} hsfm_stats;
+int hsfm_last_stats(hsfm_index *idx, hsfm_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,141 @@
+"""GPU parity: find_smems_gpu / find_mems_gpu through the C ABI vs the oracle."""
+import numpy as np
+import pytest
+
+import haystackfm_amd as hf
+from oracle import oracle as O
+from tests.helpers import random_text, smem_random_cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(seqs=None, **kw):
+    o = O.OracleBidir.build(seqs, **kw)
+    return o, hf.BidirFmIndex.from_bytes(o.to_bytes())
+
+
+def expected_hits(o, mems, ref_bounds, cap):
+    """MemHit list the device must return for the oracle's Mems: one record
+    per SA interval, intervals of a MEM sorted by start, positions = the first
+    min(size, cap) rows of each interval (bidir_index.rs:307-359, 452-468)."""
+    out = []
+    for m in mems:
+        for lo, hi, _, _ in sorted(m.ivs.tolist()):
+            size = hi - lo
+            k = min(size, cap) if len(ref_bounds) else 0
+            if k:
+                pos = o.fwd.resolve_rows(np.arange(lo, lo + k, dtype=np.uint32))
+                plist = [O.ref_map(int(p), ref_bounds) for p in pos]
+            else:
+                plist = []
+            out.append((m.query_start, m.query_end, size, size > cap, plist))
+    return out
+
+
+def _as_tuples(hits):
+    return [(h.query_start, h.query_end, h.match_count, h.truncated, h.positions) for h in hits]
+
+
+def check(o, g, queries, min_len, ref_bounds=None, cap=1024, modes=(0, 1)):
+    rb = o.fwd.seq_boundaries if ref_bounds is None else np.asarray(ref_bounds, np.uint32)
+    for mode in modes:
+        exp = o.find_mems_batch(queries, min_len=min_len, mode=mode, threads=4)
+        fn = g.find_smems_gpu if mode == 0 else g.find_mems_gpu
+        got = fn(queries, min_len, rb, cap)
+        assert len(got) == len(exp)
+        for qi, (gh, em) in enumerate(zip(got, exp)):
+            assert _as_tuples(gh) == expected_hits(o, em, rb, cap), (mode, qi)
+        st = g.last_stats()
+        assert st["guard_hits"] == 0
+
+
+def test_known_answers(golden):
+    for case in golden["smem"]["exact"]:
+        o, g = _pair(case["ref"], sample_rate=1)
+        check(o, g, [case["query"]], case["min_len"])
+
+
+def test_brute_force_cases(golden):
+    for ref, q in golden["smem"]["brute_force"]:
+        o, g = _pair([ref], sample_rate=1)
+        check(o, g, [q], 1)
+        text, _ = O.concatenate([ref])
+        got = g.find_smems_gpu([q], 1, o.fwd.seq_boundaries)[0]
+        spans = sorted({(h.query_start, h.query_end) for h in got})
+        assert spans == [(s, e) for s, e, _ in O.brute_mems(text, q, 1)]
+
+
+def test_regressions(golden):
+    gs = golden["smem"]
+    q = gs["regression_query"]
+    for reg in gs["regressions"]:
+        seqs = [q[a:e] for a, e, _ in reg["refs"]]
+        o, g = _pair(seqs, headers=[h for _, _, h in reg["refs"]], sample_rate=1)
+        check(o, g, [q], reg["min_len"])
+        got = g.find_smems_gpu([q], reg["min_len"], o.fwd.seq_boundaries)[0]
+        assert {r for h in got for r, _ in h.positions} == {0, 1}
+
+
+def test_randomized_lcg_cases():
+    cases = smem_random_cases()
+    for seqs, names, full in cases[:40]:
+        o, g = _pair(seqs, headers=names, sample_rate=1)
+        check(o, g, [full], 15)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_random_iupac_batches(seed):
+    rng = np.random.default_rng(300 + seed)
+    seqs = [O.decode(random_text(rng, int(rng.integers(500, 4000)), iupac_frac=0.02)) for _ in range(3)]
+    o, g = _pair(seqs, sample_rate=int(rng.choice([1, 8, 32])))
+    text, _ = O.concatenate(seqs)
+    qs = []
+    for _ in range(120):
+        m = int(rng.integers(0, 60))
+        if rng.random() < 0.7 and len(text) > m + 1:
+            st = int(rng.integers(0, len(text) - m))
+            q = text[st:st + m].copy()
+            q[q == 0] = 1
+            flip = rng.random(m) < 0.05
+            q[flip] = random_text(rng, int(flip.sum()), alphabet="ACGTN")
+        else:
+            q = random_text(rng, m, alphabet="ACGT")
+        qs.append(q)
+    check(o, g, qs, int(rng.integers(1, 12)))
+
+
+def test_truncation_and_no_boundaries():
+    o, g = _pair(["ACGT" * 300, "ACGTTACG" * 40], sample_rate=4)
+    qs = ["ACGTACGT", "ACG", "TTACGACGT"]
+    check(o, g, qs, 3, cap=7)
+    # empty ref_boundaries: counts and truncation flags only, no positions
+    got = g.find_mems_gpu(qs, 3, [], 7)
+    exp = o.find_mems_batch(qs, min_len=3, mode=1)
+    for gh, em in zip(got, exp):
+        assert _as_tuples(gh) == expected_hits(o, em, [], 7)
+        assert all(h.positions == [] for h in gh)
+
+
+def test_user_ref_boundaries_differ_from_index():
+    """ref_map uses the caller's boundaries, not the index's (ref_map.wgsl:31-59)."""
+    o, g = _pair(["ACGTTGCAACGGT", "TTGACCAGTAGG"], sample_rate=2)
+    rb = [5, 14, 30]
+    check(o, g, ["ACGTTG", "CAGTA"], 2, ref_bounds=rb)
+
+
+def test_empty_inputs():
+    o, g = _pair(["ACGTACGT"], sample_rate=1)
+    assert g.find_smems_gpu([], 3, o.fwd.seq_boundaries) == []
+    got = g.find_smems_gpu(["", "AC", "ACGT"], 0, o.fwd.seq_boundaries)
+    assert got == [[], [], []]
+    check(o, g, ["", np.array([0, 0], np.uint8), np.array([250, 1, 2], np.uint8)], 1)
+
+
+def test_iupac_heavy_sets_use_capacity_retries():
+    """An IUPAC-rich text and N-rich queries make interval sets large; every
+    read must still come back exactly."""
+    rng = np.random.default_rng(5)
+    seqs = [O.decode(random_text(rng, 30000, iupac_frac=0.25))]
+    o, g = _pair(seqs, sample_rate=16)
+    qs = [random_text(rng, int(rng.integers(5, 25)), alphabet="ACGTN", iupac_frac=0.3) for _ in range(60)]
+    check(o, g, qs, 2, modes=(0,))
