@@ -132,7 +132,15 @@ def cpu_baseline(idx_bytes, sq, so, lq, lo, bounds, args, gpu_smem_sample, gpu_l
         return time.perf_counter() - t0
 
     def smem_run(k):
-        ob.find_mems_batch(qflat=sq[:int(so[k])], qoff=so[:k + 1], min_len=args.min_len, mode=0, threads=cores)
+        # find_smems + locate of the first min(size, max_hits) rows of each interval,
+        # the same work the GPU pipeline does (find -> resolve -> ref map)
+        mems = ob.find_mems_batch(qflat=sq[:int(so[k])], qoff=so[:k + 1], min_len=args.min_len, mode=0,
+                                  threads=cores)
+        rows = [np.arange(lo_, lo_ + min(hi_ - lo_, args.max_hits), dtype=np.uint32)
+                for ms in mems for m in ms for lo_, hi_, _, _ in m.ivs.tolist()]
+        if rows:
+            pos = ob.fwd.resolve_rows(np.concatenate(rows), threads=cores)
+            np.searchsorted(bounds, pos, side="right")
 
     def loc_run(k):
         ob.fwd.locate_batch(qflat=lq[:int(lo[k])], qoff=lo[:k + 1], threads=cores)
@@ -143,7 +151,8 @@ def cpu_baseline(idx_bytes, sq, so, lq, lo, bounds, args, gpu_smem_sample, gpu_l
     k2 = int(min(len(so) - 1, max(k, k * budget / max(t, 1e-3))))
     t2 = timed(smem_run, k2)
     res["smem"] = {"value": k2 / t2, "unit": "reads/s", "cores": cores, "kind": "port",
-                   "sample": f"first {k2} reads of the SMEM batch, oracle find_smems + locate-free, {cores} threads"}
+                   "sample": f"first {k2} reads of the SMEM batch: oracle find_smems + resolve of the reported rows "
+                             f"+ ref map, {cores} threads"}
     k = min(len(lo) - 1, 1024 * cores)
     t = timed(loc_run, k)
     k2 = int(min(len(lo) - 1, max(k, k * budget / max(t, 1e-3))))
@@ -152,7 +161,7 @@ def cpu_baseline(idx_bytes, sq, so, lq, lo, bounds, args, gpu_smem_sample, gpu_l
                      "sample": f"first {k2} reads of the locate batch, oracle FmIndex::locate, {cores} threads"}
     # algorithmic probes of the reference algorithm on a small sample
     O.probes_enable(True)
-    ns = min(2000, len(so) - 1)
+    ns = min(300, len(so) - 1)
     O.probes_reset()
     ob.find_mems_batch(qflat=sq[:int(so[ns])], qoff=so[:ns + 1], min_len=args.min_len, mode=0, threads=1)
     p = O.probes_get()
@@ -224,7 +233,8 @@ def main():
         lk.append(bidir.fwd.last_stats()["main_kernel_ms"])
     torch.cuda.synchronize()
     loc_s = (time.perf_counter() - t0) / args.steps
-    log(f"locate step {loc_s * 1e3:.1f} ms, {lr[0]} hits, stats {bidir.fwd.last_stats()}")
+    loc_stats = bidir.fwd.last_stats()
+    log(f"locate step {loc_s * 1e3:.1f} ms, {lr[0]} hits, stats {loc_stats}")
 
     # sample outputs for the parity diff (host entry points, same kernels)
     n = min(args.check_reads, nq)
@@ -237,13 +247,23 @@ def main():
     value = nq / smem_s
     kms = float(np.mean(kernel_ms))
     roof = None
-    if "smem_probes_per_read" in cpu and kms > 0:
-        bytes_per_launch = cpu["smem_probes_per_read"] * 64.0 * nq
+    probes = smem_stats.get("occ_probes", 0)
+    if probes and kms > 0:
+        # algorithmic bytes: the 64 B Occ records (and seed-table sectors) the
+        # kernel's algorithm must fetch, counted on the device, per launch
+        bytes_per_launch = probes * 64.0
         achieved = bytes_per_launch / (kms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_mem_wave", "kernel_ms": round(kms, 3),
-                "bytes_per_read": round(cpu["smem_probes_per_read"] * 64.0, 1)}
+                "kernel": "k_mem_thread", "kernel_ms": round(kms, 3),
+                "bytes_per_read": round(bytes_per_launch / nq, 1)}
+    loc_roof = None
+    lkms = float(np.mean(lk))
+    if loc_stats.get("occ_probes") and lkms > 0:
+        a = loc_stats["occ_probes"] * 64.0 / (lkms * 1e-3) / 1e9
+        loc_roof = {"bound": "hbm", "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(a / HBM_PEAK_GBS, 4), "kernel": "k_locate_search", "kernel_ms": round(lkms, 3),
+                    "bytes_per_read": round(loc_stats["occ_probes"] * 64.0 / nl, 1)}
     cpu_line = None
     if "smem" in cpu:
         cpu_line = cpu["smem"]
@@ -271,9 +291,11 @@ def main():
         "parity_sample": cpu["parity_sample"],
         "locate": {"value": round(nl / loc_s, 1) if cpu["parity_ok"] else None, "unit": "reads/s",
                    "ms_per_step": round(loc_s * 1e3, 3), "reads": nl, "search_kernel_ms": round(float(np.mean(lk)), 3),
-                   "cpu_baseline": cpu.get("locate"),
-                   "search_probes_per_read": cpu.get("locate_search_probes_per_read")},
+                   "cpu_baseline": cpu.get("locate"), "roofline": loc_roof,
+                   "stats": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in loc_stats.items()},
+                   "reference_algorithm_probes_per_read": cpu.get("locate_search_probes_per_read")},
         "smem_stats": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in smem_stats.items()},
+        "reference_algorithm_probes_per_read": cpu.get("smem_probes_per_read"),
         "learned_seed_speedup": None,
     }
     print(json.dumps(out), flush=True)

@@ -217,7 +217,7 @@ class _MemDev(C.Structure):
 class _Stats(C.Structure):
     _fields_ = [("fallback_reads", C.c_uint64), ("capacity_retries", C.c_uint64),
                 ("guard_hits", C.c_uint64), ("records", C.c_uint64), ("main_kernel_ms", C.c_double),
-                ("resolve_kernel_ms", C.c_double)]
+                ("resolve_kernel_ms", C.c_double), ("occ_probes", C.c_uint64), ("diag", C.c_uint64 * 4)]
 
 
 _lib_obj = None
@@ -380,6 +380,7 @@ class FmIndex:
         self._seq_boundaries = np.frombuffer(C.string_at(view.seq_boundaries, 4 * view.seq_boundaries_len),
                                              np.uint32).copy()
         self._dev = None  # (ctx, handle)
+        self._pair_rev = None  # reverse index when this is the forward half of a BidirFmIndex
 
     @classmethod
     def from_bytes(cls, data: bytes) -> "FmIndex":
@@ -393,6 +394,7 @@ class FmIndex:
     def _handle(self, ctx: GpuContext | None = None, rev: "FmIndex | None" = None):
         if self._dev is not None:
             return self._dev[1]
+        rev = rev if rev is not None else self._pair_rev
         ctx = ctx or context_cache.get_or_init()
         h = C.c_void_p()
         rv = C.byref(rev._view) if rev is not None else None
@@ -409,7 +411,7 @@ class FmIndex:
     def last_stats(self) -> dict:
         st = _Stats()
         _check(_lib().hsfm_last_stats(self._handle(), C.byref(st)), "stats")
-        return {k: getattr(st, k) for k, _ in _Stats._fields_}
+        return {k: (list(getattr(st, k)) if k == "diag" else getattr(st, k)) for k, _ in _Stats._fields_}
 
     # queries ---------------------------------------------------------------
     def locate_batch(self, queries, ctx: GpuContext | None = None) -> tuple[np.ndarray, np.ndarray]:
@@ -474,6 +476,7 @@ class BidirFmIndex:
 
     def __init__(self, fwd: FmIndex, rev: FmIndex):
         self.fwd, self.rev = fwd, rev
+        fwd._pair_rev = rev  # one device index serves locate (forward) and MEM (both)
 
     @classmethod
     def from_bytes(cls, data: bytes) -> "BidirFmIndex":
@@ -498,7 +501,7 @@ class BidirFmIndex:
     def last_stats(self) -> dict:
         st = _Stats()
         _check(_lib().hsfm_last_stats(self._handle(), C.byref(st)), "stats")
-        return {k: getattr(st, k) for k, _ in _Stats._fields_}
+        return {k: (list(getattr(st, k)) if k == "diag" else getattr(st, k)) for k, _ in _Stats._fields_}
 
     def find_mems_batch(self, queries, min_len: int, mode: int, ref_boundaries=(), max_hits_per_mem: int = 1024):
         """Raw CSR result: (hit_offsets[nq+1], hits[nh, 8] u32 view, positions[np, 2])."""

@@ -51,7 +51,7 @@ struct Workspace {
     DevBuf hit_off, rows, hits;               // resolve
     DevBuf scan_tmp;
     DevBuf mem_recs, mem_cnt, mem_off, mem_sorted, mem_hits, pos_off, positions, mem_flags, read_list;
-    DevBuf wave_sets;
+    DevBuf wave_sets, mem_slots, mem_status;
     uint64_t *h_pinned = nullptr;             // small readbacks
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr}; // main kernel, resolve kernel
 };
@@ -69,6 +69,9 @@ struct hsfm_index {
     std::mutex mu;
     hsfm::Workspace ws;
     hsfm_stats stats{};
+    // bidirectional seed table of every ACGT F-mer (built on first MEM call)
+    hsfm::DevBuf seed_off, seed_iv, seed_lsl;
+    uint32_t seed_F = 0;
 };
 
 namespace hsfm {

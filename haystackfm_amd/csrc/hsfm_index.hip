@@ -246,7 +246,8 @@ void free_workspace(Workspace &w) {
     DevBuf *all[] = {&w.qcopy, &w.qoffcopy, &w.refb, &w.cnt, &w.niv, &w.ivs, &w.status, &w.ovf_list,
                      &w.counters, &w.fb_sets, &w.fb_ivs, &w.fb_meta, &w.hit_off, &w.rows, &w.hits,
                      &w.scan_tmp, &w.mem_recs, &w.mem_cnt, &w.mem_off, &w.mem_sorted, &w.mem_hits,
-                     &w.pos_off, &w.positions, &w.mem_flags, &w.read_list, &w.wave_sets};
+                     &w.pos_off, &w.positions, &w.mem_flags, &w.read_list, &w.wave_sets,
+                     &w.mem_slots, &w.mem_status};
     for (DevBuf *b : all) b->release();
     if (w.h_pinned) (void)hipHostFree(w.h_pinned);
     w.h_pinned = nullptr;
@@ -350,6 +351,28 @@ extern "C" int hsfm_index_upload(hsfm_device *dev, const hsfm_index_view *fwd, c
             c.compat_lanes[b] = m;
             c.lookup_digit[b] = -1;
         }
+        // seed-table expansion: ACGT map to digits; another byte expands to the
+        // bases whose compat sets are contained in its own, when their union is
+        // exactly its compat set (true for every IupacDna ambiguity code)
+        for (int b = 0; b < 256; b++) {
+            uint8_t e = 0;
+            if (b >= 1 && b <= 4) {
+                e = (uint8_t)(0x10 | (b - 1));
+            } else {
+                const uint16_t mb = fwd->compat_mask[b];
+                uint16_t uni = 0;
+                uint8_t bases = 0;
+                for (int x = 0; x < 4; x++) {
+                    const uint16_t mx = fwd->compat_mask[x + 1];
+                    if (mx && (mx & ~mb) == 0) {
+                        bases |= (uint8_t)(1u << x);
+                        uni |= mx;
+                    }
+                }
+                if (bases && uni == mb) e = bases;
+            }
+            c.seed_expand[b] = e;
+        }
         if (fwd->has_lookup) {
             c.lookup_depth = fwd->lookup_depth;
             c.lookup_radix = (uint32_t)fwd->lookup_core_len;
@@ -388,6 +411,9 @@ extern "C" int hsfm_index_free(hsfm_index *idx) {
     idx->d_const.release();
     idx->seq_bounds.release();
     free_workspace(idx->ws);
+    idx->seed_off.release();
+    idx->seed_iv.release();
+    idx->seed_lsl.release();
     delete idx;
     return HSFM_OK;
 }

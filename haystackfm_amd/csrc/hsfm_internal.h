@@ -44,6 +44,10 @@ struct DevConst {
     uint32_t lookup_radix;
     uint32_t lookup_depth;      // 0 = no lookup table
     uint32_t pad;
+    // Seed-table expansion of a query byte: bit 4 set = exact base with digit in
+    // bits 0-1; else bits 0-3 = set of bases whose compat sets union to this
+    // byte's compat set (IupacDna ambiguity codes); 0 = no expansion.
+    uint8_t seed_expand[256];
 };
 
 // Device view of one index direction (passed by value to kernels).
@@ -168,6 +172,30 @@ __device__ __forceinline__ bool sa_sampled(const DevDir &d, uint32_t row, uint32
     rk = rk < d.num_samples ? rk : d.num_samples - 1;
     val = d.sa_vals[rk];
     return true;
+}
+
+// Counts of every lane at the two probe points of a bidirectional interval
+// {fwd_lo, fwd_hi, rev_lo, -}: fwd points for a left extension (P -> cP on the
+// forward index), rev points for a right extension (P -> Pc on the reverse
+// index), as in bidir.rs:80-130.
+template <bool LEFT>
+__device__ __forceinline__ void interval_counts(const DevDir &D, uint32_t L, const uint4 &iv, uint32_t (&cl)[16],
+                                                uint32_t (&ch)[16]) {
+    const uint32_t size = iv.y - iv.x;
+    const uint32_t xlo = LEFT ? iv.x : iv.z;
+    const uint32_t xhi = LEFT ? iv.y : iv.z + size;
+    RecView rl, rh;
+    load_rec(D, xlo, rl);
+    load_rec(D, xhi, rh);
+    lane_counts(rl, xlo, L, cl);
+    lane_counts(rh, xhi, L, ch);
+}
+// Child of iv for lane k: clane = C of the lane, (a, b) = counts at the two
+// points, acc = sum over lanes below k of (b - a) (count_smaller_than).
+template <bool LEFT>
+__device__ __forceinline__ uint4 bidir_child(const uint4 &iv, uint32_t clane, uint32_t a, uint32_t b, uint32_t acc) {
+    return LEFT ? make_uint4(clane + a, clane + b, iv.z + acc, 0u)
+                : make_uint4(iv.x + acc, iv.x + acc + (b - a), clane + a, 0u);
 }
 
 // wave64 helpers

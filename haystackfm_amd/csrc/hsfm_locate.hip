@@ -18,6 +18,8 @@
 // does not change the row set, so it is not performed.
 #include <string.h>
 
+#include <hipcub/hipcub.hpp>
+
 #include "hsfm_host.h"
 
 namespace hsfm {
@@ -31,7 +33,8 @@ struct LocateCounters {            // device counters (u64 each)
     unsigned long long fb_used;    // fallback interval slots reserved
     unsigned long long n_retry;    // reads that need a larger wave capacity
     unsigned long long guard;      // walk guard hits
-    unsigned long long pad[3];
+    unsigned long long probes;     // records fetched by k_locate_search
+    unsigned long long pad[2];
 };
 
 __device__ __forceinline__ void load_const_lds(const DevConst *cst, uint16_t *s_compat, int8_t *s_digit,
@@ -63,6 +66,12 @@ __device__ __forceinline__ int lookup_seed(const DevDir &d, const int8_t *s_digi
     return 1;
 }
 
+__device__ __forceinline__ uint32_t locate_read(const DevDir &d, const DevConst *cst, const uint8_t *q,
+                                                const uint64_t *qoff, uint32_t r, uint32_t *cnt, uint8_t *niv,
+                                                uint2 *ivs, uint32_t *ovf_list, LocateCounters *ctr,
+                                                const uint16_t *s_compat, const int8_t *s_digit,
+                                                const uint32_t *s_clane);
+
 __global__ __launch_bounds__(256) void k_locate_search(DevDir d, const DevConst *__restrict__ cst,
                                                        const uint8_t *__restrict__ q,
                                                        const uint64_t *__restrict__ qoff, uint32_t nq,
@@ -75,7 +84,21 @@ __global__ __launch_bounds__(256) void k_locate_search(DevDir d, const DevConst 
     load_const_lds(cst, s_compat, s_digit, s_clane);
     __syncthreads();
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nq) return;
+    uint32_t probes = 0;
+    if (r < nq) probes = locate_read(d, cst, q, qoff, r, cnt, niv, ivs, ovf_list, ctr, s_compat, s_digit, s_clane);
+#pragma unroll
+    for (int dd = 32; dd >= 1; dd >>= 1) probes += __shfl_xor(probes, dd, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&ctr->probes, (unsigned long long)probes);
+}
+
+// Backward search of one read (the body of k_locate_search); returns the
+// number of 64 B records it fetched.
+__device__ __forceinline__ uint32_t locate_read(const DevDir &d, const DevConst *cst, const uint8_t *q,
+                                                const uint64_t *qoff, uint32_t r, uint32_t *cnt, uint8_t *niv,
+                                                uint2 *ivs, uint32_t *ovf_list, LocateCounters *ctr,
+                                                const uint16_t *s_compat, const int8_t *s_digit,
+                                                const uint32_t *s_clane) {
+    uint32_t probes = 0;
     const uint32_t L = cst->num_lanes;
     const uint64_t b0 = qoff[r];
     const uint32_t m = (uint32_t)(qoff[r + 1] - b0);
@@ -87,6 +110,7 @@ __global__ __launch_bounds__(256) void k_locate_search(DevDir d, const DevConst 
     hi[0] = d.n;
     int start = (int)m;
     const int sd = lookup_seed(d, s_digit, cst->lookup_depth, cst->lookup_radix, p, m, lo[0], hi[0]);
+    if (sd) probes++;
     if (sd == 2) ns = 0;
     else if (sd == 1) start = (int)(m - cst->lookup_depth);
     for (int i = start - 1; i >= 0 && ns; i--) {
@@ -101,6 +125,7 @@ __global__ __launch_bounds__(256) void k_locate_search(DevDir d, const DevConst 
                 RecView rl, rh;
                 load_rec(d, lo[j], rl);
                 load_rec(d, hi[j], rh);
+                probes += 1u + ((lo[j] >> kRecShift) != (hi[j] >> kRecShift) ? 1u : 0u);
                 uint32_t cl[16], ch[16];
                 lane_counts(rl, lo[j], L, cl);
                 lane_counts(rh, hi[j], L, ch);
@@ -122,7 +147,7 @@ __global__ __launch_bounds__(256) void k_locate_search(DevDir d, const DevConst 
             cnt[r] = 0;
             const unsigned long long at = atomicAdd(&ctr->n_ovf, 1ull);
             ovf_list[at] = r;
-            return;
+            return probes;
         }
 #pragma unroll
         for (uint32_t j = 0; j < KR; j++) { lo[j] = nlo[j]; hi[j] = nhi[j]; }
@@ -147,6 +172,7 @@ __global__ __launch_bounds__(256) void k_locate_search(DevDir d, const DevConst 
         }
     niv[r] = (uint8_t)ns;
     cnt[r] = total;
+    return probes;
 }
 
 // One wave per queued read.  Set buffers: per wave 2 * cap (lo,hi) pairs.
@@ -226,7 +252,7 @@ __global__ __launch_bounds__(64) void k_locate_wave(DevDir d, const DevConst *__
             }
             continue;
         }
-        // sort by start (rank placement) when small enough, then publish
+        // publish the set; it is put in row order by a segmented sort afterwards
         uint64_t slot = 0;
         if (lane == 0) slot = atomicAdd(&ctr->fb_used, (unsigned long long)ns);
         slot = __shfl(slot, 0, 64);
@@ -241,12 +267,7 @@ __global__ __launch_bounds__(64) void k_locate_wave(DevDir d, const DevConst *__
         for (uint32_t j = lane; j < ns; j += 64) {
             const uint2 iv = A[j];
             part += iv.y - iv.x;
-            uint32_t rank = j;
-            if (ns <= 4096) {
-                rank = 0;
-                for (uint32_t t = 0; t < ns; t++) rank += A[t].x < iv.x ? 1u : 0u;
-            }
-            fb_ivs[slot + rank] = iv;
+            fb_ivs[slot + j] = iv; // ordered afterwards by k_fb_sort_* + segmented sort
         }
         const uint32_t count = wave_sum(part);
         if (lane == 0) {
@@ -256,6 +277,25 @@ __global__ __launch_bounds__(64) void k_locate_wave(DevDir d, const DevConst *__
         }
         __syncthreads();
     }
+}
+
+// Segments of the wave-path reads and 64-bit sort keys (start << 32 | end) of
+// their intervals; intervals are disjoint, so key order is row order.
+__global__ void k_fb_segments(const uint32_t *__restrict__ reads, uint32_t nreads, const uint2 *__restrict__ fb_meta,
+                              uint32_t *__restrict__ seg_begin, uint32_t *__restrict__ seg_end) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nreads) return;
+    const uint2 mt = fb_meta[reads[i]];
+    seg_begin[i] = mt.x;
+    seg_end[i] = mt.x + mt.y;
+}
+__global__ void k_fb_keys(const uint2 *__restrict__ iv, uint64_t n, unsigned long long *__restrict__ keys) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        keys[i] = ((unsigned long long)iv[i].x << 32) | iv[i].y;
+}
+__global__ void k_fb_unkeys(const unsigned long long *__restrict__ keys, uint64_t n, uint2 *__restrict__ iv) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        iv[i] = make_uint2((uint32_t)(keys[i] >> 32), (uint32_t)keys[i]);
 }
 
 // Write the SA rows of every read at its hit offset, in ascending row order.
@@ -312,6 +352,10 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
     if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
     uint64_t nlist = h[0];
     idx->stats.fallback_reads = nlist;
+    if (nlist) {
+        if ((rc = ws.status.ensure(nlist * 4))) return rc;
+        HSFM_HIP(hipMemcpyAsync(ws.status.p, ws.ovf_list.p, nlist * 4, hipMemcpyDeviceToDevice, s));
+    }
     uint32_t cap = 4096;
     uint64_t fb_cap = nlist * 256 + 4096;
     const uint32_t *list = ws.ovf_list.as<uint32_t>();
@@ -354,6 +398,32 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
         list = next_list;
         nlist = nretry;
     }
+    if (idx->stats.fallback_reads) {
+        // put every wave-path read's intervals in row order (FmIndex::locate order)
+        const uint32_t nfb = (uint32_t)idx->stats.fallback_reads;
+        uint64_t hh[8];
+        if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), hh, 8))) return rc;
+        const uint64_t used = hh[2];
+        if ((rc = ws.fb_sets.ensure(used * 16 + (uint64_t)nfb * 8 + 64))) return rc;
+        unsigned long long *k_in = ws.fb_sets.as<unsigned long long>();
+        unsigned long long *k_out = k_in + used;
+        uint32_t *seg_b = reinterpret_cast<uint32_t *>(k_out + used);
+        uint32_t *seg_e = seg_b + nfb;
+        hipLaunchKernelGGL(k_fb_segments, dim3((nfb + 255) / 256), dim3(256), 0, s, ws.status.as<uint32_t>(), nfb,
+                           ws.fb_meta.as<uint2>(), seg_b, seg_e);
+        HSFM_HIP(hipGetLastError());
+        const uint32_t kb = (uint32_t)((used + 255) / 256 < 65536 ? (used + 255) / 256 : 65536);
+        hipLaunchKernelGGL(k_fb_keys, dim3(kb ? kb : 1), dim3(256), 0, s, ws.fb_ivs.as<uint2>(), used, k_in);
+        HSFM_HIP(hipGetLastError());
+        size_t tmp = 0;
+        HSFM_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tmp, k_in, k_out, (int)used, (int)nfb, seg_b,
+                                                             seg_e, 0, 64, s));
+        if ((rc = ws.scan_tmp.ensure(tmp))) return rc;
+        HSFM_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(ws.scan_tmp.p, tmp, k_in, k_out, (int)used, (int)nfb,
+                                                             seg_b, seg_e, 0, 64, s));
+        hipLaunchKernelGGL(k_fb_unkeys, dim3(kb ? kb : 1), dim3(256), 0, s, k_out, used, ws.fb_ivs.as<uint2>());
+        HSFM_HIP(hipGetLastError());
+    }
     return HSFM_OK;
 }
 
@@ -387,6 +457,7 @@ static int locate_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_
     uint64_t h[8];
     if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
     idx->stats.guard_hits = h[4];
+    idx->stats.occ_probes = h[5];
     idx->stats.main_kernel_ms = event_ms(idx, 0);
     idx->stats.resolve_kernel_ms = total ? event_ms(idx, 1) : 0.0;
     if (h[4]) return set_error(HSFM_EINDEX, "%llu LF walks exceeded the sample rate: index is inconsistent", (unsigned long long)h[4]);

@@ -20,6 +20,7 @@
 // Every interval is one concrete (IUPAC-compatible) string, so the bidirectional
 // pairing of bidir.rs:80-130 holds; count_smaller_than (bidir.rs:136-143) runs
 // over every lane below the extension symbol.
+#include <stdlib.h>
 #include <string.h>
 
 #include "hsfm_host.h"
@@ -31,7 +32,9 @@ struct MemCounters {
     unsigned long long rec_used;
     unsigned long long n_retry;
     unsigned long long guard;
-    unsigned long long pad[4];
+    unsigned long long probes;   // records fetched by k_mem_thread
+    unsigned long long diag[4];  // deferrals: seed union, extension, slots; plain restarts
+    unsigned long long pad[7];
 };
 
 struct MemRec { // 32 bytes
@@ -186,6 +189,323 @@ __global__ __launch_bounds__(64) void k_mem_wave(DevDir fwd, DevDir rev, const D
     }
 }
 
+// ---------------------------------------------------------------------------
+// Seed table: for every ACGT F-mer W (index = sum digit(W[t]) * 4^t, leftmost
+// code least significant as in lookup.rs:101-111) the bidirectional intervals
+// of all compatible concrete strings (empty when W does not occur) and lsl(W),
+// the length of W's longest occurring suffix.  Built level by level on the
+// device: the node for a string of length l is its first code prepended to the
+// node of its suffix of length l-1 (one left extension, bidir.rs:112-130).
+// ---------------------------------------------------------------------------
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_seed_level(DevDir fwd, const DevConst *__restrict__ cst,
+                                                    const uint64_t *__restrict__ poff, const uint4 *__restrict__ piv,
+                                                    uint32_t nchild, uint32_t *__restrict__ cnt,
+                                                    const uint64_t *__restrict__ coff, uint4 *__restrict__ civ) {
+    __shared__ uint32_t s_clane[16];
+    if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
+    __syncthreads();
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchild) return;
+    const uint32_t L = cst->num_lanes;
+    const uint32_t cm = cst->compat_lanes[1 + (c & 3)];
+    const uint32_t p = c >> 2;
+    uint32_t total = 0;
+    uint64_t w = WRITE ? coff[c] : 0;
+    for (uint64_t i = poff[p]; i < poff[p + 1]; i++) {
+        const uint4 iv = piv[i];
+        uint32_t cl[16], ch[16];
+        interval_counts<true>(fwd, L, iv, cl, ch);
+        uint32_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const uint32_t d = ch[k] - cl[k];
+            if (((cm >> k) & 1u) && ch[k] > cl[k]) {
+                if (WRITE) civ[w++] = bidir_child<true>(iv, s_clane[k], cl[k], ch[k], acc);
+                total++;
+            }
+            acc += d;
+        }
+    }
+    if (!WRITE) cnt[c] = total;
+}
+
+__global__ void k_seed_lsl(uint32_t nchild, uint32_t level, const uint32_t *__restrict__ cnt,
+                           const uint8_t *__restrict__ plsl, uint8_t *__restrict__ clsl) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchild) return;
+    clsl[c] = cnt[c] ? (uint8_t)level : plsl[c >> 2];
+}
+
+__global__ void k_seed_root(uint64_t *off, uint4 *iv, uint8_t *lsl, uint32_t n) {
+    off[0] = 0;
+    off[1] = 1;
+    iv[0] = make_uint4(0, n, 0, 0);
+    lsl[0] = 0;
+}
+
+struct SeedView {
+    const uint64_t *off;
+    const uint4 *iv;
+    const uint8_t *lsl;
+    uint32_t F;
+};
+
+// ---------------------------------------------------------------------------
+// Thread-per-read MEM finder: the min_len-skip algorithm of k_mem_wave with the
+// interval set of each read in per-thread LDS slots (KT intervals).  Each
+// restart seeds q[j-F..j) from the seed table (ambiguity codes expand to the
+// union of their bases' entries, at most two per window).  Reads whose set or
+// record list outgrows the slots are deferred to k_mem_wave.
+// ---------------------------------------------------------------------------
+constexpr int KT = 8;
+constexpr int TPB = 256;
+
+template <bool LEFT>
+__device__ __forceinline__ uint32_t thread_extend(const DevDir &D, uint32_t L, const uint32_t *s_clane, uint32_t cm,
+                                                  uint32_t (*lo)[TPB], uint32_t (*hi)[TPB], uint32_t (*rl)[TPB],
+                                                  uint32_t (*nlo)[TPB], uint32_t (*nhi)[TPB], uint32_t (*nrl)[TPB],
+                                                  uint32_t ns, uint32_t tid, bool &overflow, uint32_t &probes) {
+    uint32_t nn = 0;
+    for (uint32_t j = 0; j < ns; j++) {
+        const uint4 iv = make_uint4(lo[j][tid], hi[j][tid], rl[j][tid], 0u);
+        uint32_t cl[16], ch[16];
+        interval_counts<LEFT>(D, L, iv, cl, ch);
+        const uint32_t xlo = LEFT ? iv.x : iv.z, xhi = LEFT ? iv.y : iv.z + (iv.y - iv.x);
+        probes += 1u + ((xlo >> kRecShift) != (xhi >> kRecShift) ? 1u : 0u);
+        uint32_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const uint32_t d = ch[k] - cl[k];
+            if (((cm >> k) & 1u) && ch[k] > cl[k]) {
+                if (nn < (uint32_t)KT) {
+                    const uint4 c = bidir_child<LEFT>(iv, s_clane[k], cl[k], ch[k], acc);
+                    nlo[nn][tid] = c.x;
+                    nhi[nn][tid] = c.y;
+                    nrl[nn][tid] = c.z;
+                } else {
+                    overflow = true;
+                }
+                nn++;
+            }
+            acc += d;
+        }
+    }
+    return nn;
+}
+
+__device__ __forceinline__ uint32_t mem_thread_read(const DevDir &fwd, const DevDir &rev, const DevConst *cst,
+                                                 const SeedView &sv, const uint8_t *q, const uint64_t *qoff, uint32_t r,
+                                                 uint32_t min_len, uint4 *slots, uint32_t *mem_cnt, uint8_t *status,
+                                                 uint32_t *defer_list, MemCounters *ctr,
+                                                 uint32_t (*s_lo)[KT][TPB], uint32_t (*s_hi)[KT][TPB],
+                                                 uint32_t (*s_rl)[KT][TPB], const uint16_t *s_compat,
+                                                 const uint8_t *s_expand, const uint32_t *s_clane, uint32_t tid);
+
+__global__ __launch_bounds__(TPB) void k_mem_thread(DevDir fwd, DevDir rev, const DevConst *__restrict__ cst,
+                                                    SeedView sv, const uint8_t *__restrict__ q,
+                                                    const uint64_t *__restrict__ qoff, uint32_t nq, uint32_t min_len,
+                                                    uint4 *__restrict__ slots, uint32_t *__restrict__ mem_cnt,
+                                                    uint8_t *__restrict__ status, uint32_t *__restrict__ defer_list,
+                                                    MemCounters *__restrict__ ctr) {
+    __shared__ uint32_t s_lo[2][KT][TPB], s_hi[2][KT][TPB], s_rl[2][KT][TPB];
+    __shared__ uint16_t s_compat[256];
+    __shared__ uint8_t s_expand[256];
+    __shared__ uint32_t s_clane[16];
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) {
+        s_compat[i] = cst->compat_lanes[i];
+        s_expand[i] = cst->seed_expand[i];
+    }
+    if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
+    __syncthreads();
+    const uint32_t tid = threadIdx.x;
+    const uint32_t r = blockIdx.x * blockDim.x + tid;
+    uint32_t probes = 0;
+    if (r < nq) probes = mem_thread_read(fwd, rev, cst, sv, q, qoff, r, min_len, slots, mem_cnt, status, defer_list, ctr,
+                                         s_lo, s_hi, s_rl, s_compat, s_expand, s_clane, tid);
+    // one atomic per wave for the probe total
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) probes += __shfl_xor(probes, d, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&ctr->probes, (unsigned long long)probes);
+}
+
+__device__ __forceinline__ uint32_t mem_thread_read(const DevDir &fwd, const DevDir &rev, const DevConst *cst,
+                                                 const SeedView &sv, const uint8_t *q, const uint64_t *qoff, uint32_t r,
+                                                 uint32_t min_len, uint4 *slots, uint32_t *mem_cnt, uint8_t *status,
+                                                 uint32_t *defer_list, MemCounters *ctr,
+                                                 uint32_t (*s_lo)[KT][TPB], uint32_t (*s_hi)[KT][TPB],
+                                                 uint32_t (*s_rl)[KT][TPB], const uint16_t *s_compat,
+                                                 const uint8_t *s_expand, const uint32_t *s_clane, uint32_t tid) {
+    const uint32_t L = cst->num_lanes;
+    const uint32_t n = fwd.n;
+    const uint64_t b0 = qoff[r], b1 = qoff[r + 1];
+    const uint32_t m = (uint32_t)(b1 - b0);
+    const uint8_t *p = q + b0;
+    const uint64_t slot_base = (b0 >> 3) + 8ull * r;
+    const uint32_t slot_cap = (uint32_t)((b1 >> 3) - (b0 >> 3)) + 8u;
+    const uint32_t F = sv.F;
+    uint32_t s0 = 0, E = 0, nrec = 0, probes = 0, why = 0, plain = 0;
+    int cur = 0;
+    bool defer = false;
+    while (min_len && !defer) {
+        const uint32_t j = (s0 + min_len > E + 1) ? s0 + min_len : E + 1;
+        if (j > m) break;
+        uint32_t ns = 0, k = j;
+        bool seeded = false;
+        if (F && j >= F) {
+            // digits of q[j-F..j); up to two ambiguity codes expand
+            uint32_t base_idx = 0, apos[2] = {0, 0}, amask[2] = {1, 1}, namb = 0;
+            bool usable = true;
+            for (uint32_t t = 0; t < F; t++) {
+                const uint32_t e = s_expand[p[j - F + t]];
+                if (e & 0x10u) base_idx |= (e & 3u) << (2 * t);
+                else if (e && namb < 2) { apos[namb] = t; amask[namb] = e; namb++; }
+                else usable = false;
+            }
+            if (usable) {
+                // Stream the table entries of every expansion straight into the
+                // first left extension (by q[j-F-1]) when min_len > F, so only the
+                // (much smaller) set of q[j-F-1..j) is held; with min_len == F the
+                // F-mer set itself is materialized.
+                const bool stream = min_len > F;
+                const uint32_t cm1 = stream ? s_compat[p[j - F - 1]] : 0u;
+                uint32_t lslmax = 0;
+                for (uint32_t x0 = 0; x0 < 4 && !defer; x0++) {
+                    if (!((amask[0] >> x0) & 1u)) continue;
+                    for (uint32_t x1 = 0; x1 < 4 && !defer; x1++) {
+                        if (!((amask[1] >> x1) & 1u)) continue;
+                        uint32_t idx = base_idx;
+                        if (namb > 0) idx |= x0 << (2 * apos[0]);
+                        if (namb > 1) idx |= x1 << (2 * apos[1]);
+                        const uint32_t l = sv.lsl[idx];
+                        probes++;
+                        lslmax = l > lslmax ? l : lslmax;
+                        if (l != F) continue;
+                        probes += 2; // offsets + intervals
+                        for (uint64_t i = sv.off[idx]; i < sv.off[idx + 1] && !defer; i++) {
+                            const uint4 iv = sv.iv[i];
+                            if (!stream) {
+                                bool dup = false;
+                                for (uint32_t u = 0; u < ns; u++) dup |= s_lo[cur][u][tid] == iv.x;
+                                if (dup) continue;
+                                if (ns == (uint32_t)KT) { defer = true; why = 0; break; }
+                                s_lo[cur][ns][tid] = iv.x;
+                                s_hi[cur][ns][tid] = iv.y;
+                                s_rl[cur][ns][tid] = iv.z;
+                                ns++;
+                                continue;
+                            }
+                            uint32_t cl[16], ch[16];
+                            interval_counts<true>(fwd, L, iv, cl, ch);
+                            probes += 1u + ((iv.x >> kRecShift) != (iv.y >> kRecShift) ? 1u : 0u);
+                            uint32_t acc = 0;
+#pragma unroll
+                            for (int kk = 0; kk < 16; kk++) {
+                                const uint32_t d = ch[kk] - cl[kk];
+                                if (((cm1 >> kk) & 1u) && ch[kk] > cl[kk]) {
+                                    const uint4 c = bidir_child<true>(iv, s_clane[kk], cl[kk], ch[kk], acc);
+                                    bool dup = false;
+                                    for (uint32_t u = 0; u < ns; u++) dup |= s_lo[cur][u][tid] == c.x;
+                                    if (!dup) {
+                                        if (ns == (uint32_t)KT) {
+                                            defer = true;
+                                            why = 0;
+                                        } else {
+                                            s_lo[cur][ns][tid] = c.x;
+                                            s_hi[cur][ns][tid] = c.y;
+                                            s_rl[cur][ns][tid] = c.z;
+                                            ns++;
+                                        }
+                                    }
+                                }
+                                acc += d;
+                            }
+                        }
+                    }
+                }
+                if (defer) break;
+                if (lslmax < F) { // q[j-lslmax-1..j) does not occur: no long MEM starts before it
+                    s0 = j - lslmax;
+                    continue;
+                }
+                if (stream && ns == 0) { // q[j-F..j) occurs, q[j-F-1..j) does not: L(j) = j-F
+                    s0 = j - F;
+                    continue;
+                }
+                k = stream ? j - F - 1 : j - F;
+                seeded = true;
+            }
+        }
+        if (!seeded) {
+            plain++;
+            s_lo[cur][0][tid] = 0;
+            s_hi[cur][0][tid] = n;
+            s_rl[cur][0][tid] = 0;
+            ns = 1;
+            k = j;
+        }
+        bool ovf = false;
+        while (k > 0) {
+            const uint32_t nn = thread_extend<true>(fwd, L, s_clane, s_compat[p[k - 1]], s_lo[cur], s_hi[cur], s_rl[cur],
+                                                    s_lo[cur ^ 1], s_hi[cur ^ 1], s_rl[cur ^ 1], ns, tid, ovf, probes);
+            if (ovf) break;
+            if (nn == 0) break;
+            cur ^= 1;
+            ns = nn;
+            k--;
+        }
+        if (ovf) { defer = true; why = 1; break; }
+        if (k + min_len > j) {
+            s0 = k;
+            continue;
+        }
+        uint32_t e = j;
+        while (e < m) {
+            const uint32_t nn = thread_extend<false>(rev, L, s_clane, s_compat[p[e]], s_lo[cur], s_hi[cur], s_rl[cur],
+                                                     s_lo[cur ^ 1], s_hi[cur ^ 1], s_rl[cur ^ 1], ns, tid, ovf, probes);
+            if (ovf) break;
+            if (nn == 0) break;
+            cur ^= 1;
+            ns = nn;
+            e++;
+        }
+        if (ovf) { defer = true; why = 1; break; }
+        if (nrec + ns > slot_cap) { defer = true; why = 2; break; }
+        for (uint32_t t = 0; t < ns; t++) {
+            const uint32_t lo = s_lo[cur][t][tid];
+            uint32_t rank = 0;
+            for (uint32_t u = 0; u < ns; u++) rank += s_lo[cur][u][tid] < lo ? 1u : 0u;
+            slots[slot_base + nrec + rank] = make_uint4(k, e, lo, s_hi[cur][t][tid]);
+        }
+        nrec += ns;
+        s0 = k + 1;
+        E = e;
+    }
+    if (defer) {
+        status[r] = 1;
+        mem_cnt[r] = 0;
+        const unsigned long long at = atomicAdd(&ctr->n_retry, 1ull);
+        defer_list[at] = r;
+        atomicAdd(&ctr->diag[why], 1ull);
+    } else {
+        status[r] = 0;
+        mem_cnt[r] = nrec;
+    }
+    if (plain) atomicAdd(&ctr->diag[3], (unsigned long long)plain);
+    return probes;
+}
+
+// Copy the records of thread-path reads from their slots to the final order.
+__global__ __launch_bounds__(256) void k_mem_gather(uint32_t nq, const uint64_t *__restrict__ qoff,
+                                                    const uint8_t *__restrict__ status, const uint32_t *__restrict__ mem_cnt,
+                                                    const uint64_t *__restrict__ mem_off, const uint4 *__restrict__ slots,
+                                                    uint4 *__restrict__ out) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nq || status[r] != 0) return;
+    const uint64_t base = (qoff[r] >> 3) + 8ull * r, o = mem_off[r];
+    for (uint32_t i = 0; i < mem_cnt[r]; i++) out[o + i] = slots[base + i];
+}
+
 // Place the records of successful attempts at their final position.
 __global__ __launch_bounds__(256) void k_mem_scatter(const MemRec *__restrict__ recs, uint64_t nrec,
                                                      const uint32_t *__restrict__ pass_of,
@@ -231,6 +551,51 @@ __global__ __launch_bounds__(256) void k_mem_hits(const uint4 *__restrict__ recs
 
 using namespace hsfm;
 
+namespace hsfm {
+// Build the seed table of all ACGT F-mers (k_seed_level, level by level).
+static int build_seed_table(hsfm_index *idx, uint32_t F) {
+    if (idx->seed_F == F) return HSFM_OK;
+    hipStream_t s = idx->dev->stream;
+    DevBuf poff, piv, plsl;
+    int rc;
+    if ((rc = poff.ensure(16)) || (rc = piv.ensure(16)) || (rc = plsl.ensure(4))) return rc;
+    hipLaunchKernelGGL(k_seed_root, dim3(1), dim3(1), 0, s, poff.as<uint64_t>(), piv.as<uint4>(), plsl.as<uint8_t>(),
+                       idx->fwd.view.n);
+    HSFM_HIP(hipGetLastError());
+    DevBuf cnt;
+    for (uint32_t l = 1; l <= F; l++) {
+        const uint32_t nchild = 1u << (2 * l);
+        DevBuf coff, civ, clsl;
+        if ((rc = cnt.ensure((size_t)nchild * 4)) || (rc = coff.ensure(((size_t)nchild + 1) * 8)) ||
+            (rc = clsl.ensure(nchild)))
+            return rc;
+        const dim3 grid((nchild + 255) / 256);
+        hipLaunchKernelGGL(k_seed_level<false>, grid, dim3(256), 0, s, idx->fwd.view, idx->d_const.as<DevConst>(),
+                           poff.as<uint64_t>(), piv.as<uint4>(), nchild, cnt.as<uint32_t>(), nullptr, nullptr);
+        HSFM_HIP(hipGetLastError());
+        if ((rc = exclusive_scan_u32_u64(idx, cnt.as<uint32_t>(), coff.as<uint64_t>(), nchild))) return rc;
+        uint64_t total = 0;
+        if ((rc = readback_u64(idx, coff.as<uint64_t>() + nchild, &total, 1))) return rc;
+        if ((rc = civ.ensure(total * 16 + 16))) return rc;
+        hipLaunchKernelGGL(k_seed_level<true>, grid, dim3(256), 0, s, idx->fwd.view, idx->d_const.as<DevConst>(),
+                           poff.as<uint64_t>(), piv.as<uint4>(), nchild, nullptr, coff.as<uint64_t>(), civ.as<uint4>());
+        HSFM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_seed_lsl, grid, dim3(256), 0, s, nchild, l, cnt.as<uint32_t>(), plsl.as<uint8_t>(),
+                           clsl.as<uint8_t>());
+        HSFM_HIP(hipGetLastError());
+        HSFM_HIP(hipStreamSynchronize(s));
+        poff.release(); piv.release(); plsl.release();
+        poff = coff; piv = civ; plsl = clsl;
+        coff = DevBuf(); civ = DevBuf(); clsl = DevBuf();
+    }
+    cnt.release();
+    idx->seed_off.release(); idx->seed_iv.release(); idx->seed_lsl.release();
+    idx->seed_off = poff; idx->seed_iv = piv; idx->seed_lsl = plsl;
+    idx->seed_F = F;
+    return HSFM_OK;
+}
+} // namespace hsfm
+
 static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *d_qoff, uint32_t nq, uint32_t min_len,
                            const uint32_t *d_refb, uint32_t n_refs, uint32_t max_hits, hsfm_mem_device_result *out) {
     Workspace &ws = idx->ws;
@@ -247,15 +612,46 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     if ((rc = fill_u32(idx, ws.counters.as<uint32_t>(), sizeof(MemCounters) / 4, 0))) return rc;
     if ((rc = fill_u32(idx, ws.mem_cnt.as<uint32_t>(), nq, 0))) return rc;
     if ((rc = fill_u32(idx, ws.mem_flags.as<uint32_t>(), nq, 0xffffffffu))) return rc;
+    uint64_t h[16];
 
-    uint64_t rec_cap = (uint64_t)nq * 8 + 4096;
+    // path choice: the thread-per-read kernel seeds every restart from the
+    // 12-mer table, which needs min_len >= 12; shorter min_len (and
+    // HSFM_MEM_PATH=wave) run every read on the wave kernel.
+    const uint32_t F = 12;
+    const char *path = getenv("HSFM_MEM_PATH");
+    const bool use_thread = nq > 0 && min_len >= F && !(path && strcmp(path, "wave") == 0);
+    const uint32_t *list = nullptr; // identity list
+    uint64_t nlist = nq;
+    uint32_t *retry = ws.ovf_list.as<uint32_t>();
+    if (use_thread) {
+        if ((rc = build_seed_table(idx, F))) return rc;
+        uint64_t total_len = 0;
+        if ((rc = readback_u64(idx, d_qoff + nq, &total_len, 1))) return rc;
+        const uint64_t nslots = (total_len >> 3) + 8ull * nq;
+        if ((rc = ws.mem_slots.ensure(nslots * 16 + 16))) return rc;
+        if ((rc = ws.mem_status.ensure((size_t)nq + 16))) return rc;
+        SeedView sv{idx->seed_off.as<uint64_t>(), idx->seed_iv.as<uint4>(), idx->seed_lsl.as<uint8_t>(), F};
+        if ((rc = event_record(idx, 0, false))) return rc;
+        hipLaunchKernelGGL(k_mem_thread, dim3((nq + TPB - 1) / TPB), dim3(TPB), 0, s, idx->fwd.view, idx->rev.view,
+                           idx->d_const.as<DevConst>(), sv, d_q, d_qoff, nq, min_len, ws.mem_slots.as<uint4>(),
+                           ws.mem_cnt.as<uint32_t>(), ws.mem_status.as<uint8_t>(), ws.ovf_list.as<uint32_t>(), ctr);
+        HSFM_HIP(hipGetLastError());
+        if ((rc = event_record(idx, 0, true))) return rc;
+        uint64_t hd[16];
+        if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), hd, 16))) return rc;
+        nlist = hd[2];
+        idx->stats.occ_probes = hd[4];
+        for (int i = 0; i < 4; i++) idx->stats.diag[i] = hd[5 + i];
+        list = ws.ovf_list.as<uint32_t>();
+        retry = ws.read_list.as<uint32_t>();
+        idx->stats.fallback_reads = nlist;
+    }
+
+    uint64_t rec_cap = nlist * 8 + 4096;
     if (ws.mem_recs.cap / sizeof(MemRec) > rec_cap) rec_cap = ws.mem_recs.cap / sizeof(MemRec);
     uint32_t cap = 1024;
-    const uint32_t *list = nullptr; // identity list on the first pass
-    uint32_t *retry = ws.ovf_list.as<uint32_t>();
-    uint64_t nlist = nq;
     uint32_t pass = 0;
-    uint64_t h[8];
+    uint64_t rec_used = 0;
     while (nlist) {
         uint64_t waves = nlist < (uint64_t)idx->dev->num_cus * 8 ? nlist : (uint64_t)idx->dev->num_cus * 8;
         const uint64_t wave_limit = (4ull << 30) / (2ull * cap * 16);
@@ -273,21 +669,22 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         rec_cap = ws.mem_recs.cap / sizeof(MemRec);
         if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->work), 2, 0))) return rc;
         if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_retry), 2, 0))) return rc;
-        if (pass == 0 && (rc = event_record(idx, 0, false))) return rc;
+        const bool timed = !use_thread && pass == 0;
+        if (timed && (rc = event_record(idx, 0, false))) return rc;
         hipLaunchKernelGGL(k_mem_wave, dim3((uint32_t)waves), dim3(64), 0, s, idx->fwd.view, idx->rev.view,
                            idx->d_const.as<DevConst>(), d_q, d_qoff, list, (uint32_t)nlist, min_len, cap,
                            ws.wave_sets.as<uint4>(), ws.mem_recs.as<MemRec>(), rec_cap, ws.mem_cnt.as<uint32_t>(),
                            ws.mem_flags.as<uint32_t>(), pass, retry, ctr);
         HSFM_HIP(hipGetLastError());
-        if (pass == 0 && (rc = event_record(idx, 0, true))) return rc;
+        if (timed && (rc = event_record(idx, 0, true))) return rc;
         if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
-        const uint64_t nretry = h[2], used = h[1];
+        const uint64_t nretry = h[2];
+        rec_used = h[1];
         if (nretry == 0) break;
         idx->stats.capacity_retries++;
-        idx->stats.fallback_reads += nretry;
         if (cap >= (1u << 26)) return set_error(HSFM_ENOMEM, "interval set of a single read exceeds 2^26 intervals");
         cap *= 4;
-        const uint64_t want = used + nretry * 4096;
+        const uint64_t want = rec_used + nretry * 4096;
         rec_cap = rec_cap * 2 > want ? rec_cap * 2 : want;
         // next pass runs only the retried reads
         uint32_t *next_list = retry;
@@ -296,12 +693,18 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         nlist = nretry;
         pass++;
     }
-    const uint64_t rec_used = h[1];
+    if (!use_thread) idx->stats.fallback_reads = 0;
     if ((rc = exclusive_scan_u32_u64(idx, ws.mem_cnt.as<uint32_t>(), ws.mem_off.as<uint64_t>(), nq))) return rc;
     uint64_t nhits = 0;
     if ((rc = readback_u64(idx, ws.mem_off.as<uint64_t>() + nq, &nhits, 1))) return rc;
     idx->stats.records = nhits;
     if ((rc = ws.mem_sorted.ensure(nhits * 16 + 16))) return rc;
+    if (use_thread) {
+        hipLaunchKernelGGL(k_mem_gather, dim3((nq + 255) / 256), dim3(256), 0, s, nq, d_qoff, ws.mem_status.as<uint8_t>(),
+                           ws.mem_cnt.as<uint32_t>(), ws.mem_off.as<uint64_t>(), ws.mem_slots.as<uint4>(),
+                           ws.mem_sorted.as<uint4>());
+        HSFM_HIP(hipGetLastError());
+    }
     if (rec_used) {
         hipLaunchKernelGGL(k_mem_scatter, dim3((uint32_t)((rec_used + 255) / 256)), dim3(256), 0, s,
                            ws.mem_recs.as<MemRec>(), rec_used, ws.mem_flags.as<uint32_t>(), ws.mem_off.as<uint64_t>(),

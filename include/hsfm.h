@@ -176,6 +176,9 @@ typedef struct {
     uint64_t records;          /* MEM interval records emitted               */
     double main_kernel_ms;     /* search / MEM-find kernel, first pass (HIP events) */
     double resolve_kernel_ms;  /* LF-walk resolve kernel (HIP events)             */
+    uint64_t occ_probes;       /* 64 B records the main kernel's algorithm fetched */
+    uint64_t diag[4];          /* MEM thread path: deferrals by seed union, by
+                                  extension, by record slots; plain restarts      */
 } hsfm_stats;
 int hsfm_last_stats(hsfm_index *idx, hsfm_stats *out);
 

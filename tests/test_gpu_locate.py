@@ -16,7 +16,10 @@ def _pair(seqs=None, **kw):
 
 def _check_batch(o, g, queries):
     offs, hits = g.locate_batch(queries)
-    eo, eh = o.locate_batch(queries, threads=4)
+    if isinstance(queries, tuple):
+        eo, eh = o.locate_batch(qflat=queries[0], qoff=queries[1], threads=4)
+    else:
+        eo, eh = o.locate_batch(queries, threads=4)
     assert np.array_equal(offs, eo)
     # identical order too: ascending SA rows, as FmIndex::locate returns them
     assert np.array_equal(hits, eh)

@@ -139,3 +139,56 @@ def test_iupac_heavy_sets_use_capacity_retries():
     o, g = _pair(seqs, sample_rate=16)
     qs = [random_text(rng, int(rng.integers(5, 25)), alphabet="ACGTN", iupac_frac=0.3) for _ in range(60)]
     check(o, g, qs, 2, modes=(0,))
+
+
+def _mem_reads(rng, text, nreads, length, sub, amb):
+    qs = []
+    for _ in range(nreads):
+        st = int(rng.integers(0, len(text) - length))
+        q = text[st:st + length].copy()
+        q[q == 0] = 1
+        flip = rng.random(length) < sub
+        q[flip] = random_text(rng, int(flip.sum()), alphabet="ACGT")
+        ambm = rng.random(length) < amb
+        q[ambm] = random_text(rng, int(ambm.sum()), alphabet="NRYSWKMBDHV")
+        if rng.random() < 0.5:
+            q = q[::-1].copy()  # unrelated orientation: mostly short matches
+        qs.append(q)
+    return qs
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_thread_path_seed_table(seed):
+    """min_len >= 12 runs the thread-per-read kernel seeded from the 12-mer
+    table, including ambiguity codes in the seed windows."""
+    rng = np.random.default_rng(700 + seed)
+    text, bounds = O.gen_reference(40 + seed, 300_000, 4, gc=0.41, repeat_frac=0.2, n_units=30, unit_min=300,
+                                   unit_max=2000, iupac_frac=0.005)
+    o = O.OracleBidir.build(text=text, bounds=bounds, sample_rate=32, lookup_depth=10, threads=8)
+    g = hf.BidirFmIndex.from_bytes(o.to_bytes())
+    qs = _mem_reads(rng, text, 400, 150, 0.02, 0.01)
+    qs += [random_text(rng, 150, alphabet="ACGT") for _ in range(40)]
+    check(o, g, qs, 12 + 3 * seed)
+
+
+def test_thread_and_wave_paths_agree(monkeypatch):
+    rng = np.random.default_rng(11)
+    text, bounds = O.gen_reference(9, 200_000, 3, gc=0.41, repeat_frac=0.2, n_units=20, unit_min=300,
+                                   unit_max=1500, iupac_frac=0.01)
+    o = O.OracleBidir.build(text=text, bounds=bounds, sample_rate=16, threads=8)
+    g = hf.BidirFmIndex.from_bytes(o.to_bytes())
+    qs = _mem_reads(rng, text, 500, 150, 0.02, 0.02)
+    a = g.find_mems_batch(qs, 18, 0, bounds, 64)
+    monkeypatch.setenv("HSFM_MEM_PATH", "wave")
+    b = g.find_mems_batch(qs, 18, 0, bounds, 64)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_locate_then_mem_on_one_bidir_index():
+    """The forward half of a BidirFmIndex serves locate and MEM search in any order."""
+    o, g = _pair(["ACGTTGCAACGGTACGATTACA", "TTGACCAGTAGGAC"], sample_rate=4)
+    offs, hits = g.fwd.locate_batch(["ACG", "TTG"])
+    eo, eh = o.fwd.locate_batch(["ACG", "TTG"])
+    assert np.array_equal(offs, eo) and np.array_equal(hits, eh)
+    check(o, g, ["ACGTTGCAAC", "CCAGTAGGA"], 3)
