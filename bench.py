@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cache", default=os.environ.get("HSFM_BENCH_CACHE", "/tmp/hsfm_bench_cache"))
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 1 GiB device buffer 3 times (scales FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -182,6 +184,14 @@ def main():
                                    "replicating the index, see DESIGN.md)"}))
         sys.exit(2)
     import torch
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.int32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": src.numel() * 4, "copies": 3}), flush=True)
+        return
     import haystackfm_amd as hf
 
     wl = workload(args)

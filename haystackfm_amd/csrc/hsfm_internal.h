@@ -135,6 +135,55 @@ __device__ __forceinline__ void lane_counts(const RecView &r, uint32_t x, uint32
     }
 }
 
+// Counts of every lane in [0, xlo) and [0, xhi) (xlo <= xhi <= n) from the two
+// records and the superblock row of xlo only; the rare pair that spans two
+// superblocks reloads the row of xhi.  Lanes the index does not use count 0
+// because no position holds their code.
+__device__ __forceinline__ void pair_counts(const DevDir &d, uint32_t xlo, uint32_t xhi, uint32_t (&cl)[16],
+                                            uint32_t (&ch)[16]) {
+    const uint32_t bl = xlo >> kRecShift, bh = xhi >> kRecShift;
+    const uint4 *rpl = d.occ + ((size_t)bl << 2);
+    const uint4 *rph = d.occ + ((size_t)bh << 2);
+    const uint4 *spl = reinterpret_cast<const uint4 *>(d.sb + ((size_t)(bl >> kSbRecShift) << 4));
+    const uint4 la = rpl[0], lb = rpl[1], lc = rpl[2], ld = rpl[3];
+    const uint4 ha = rph[0], hb = rph[1], hc = rph[2], hd = rph[3];
+    const u32x4 s0 = ld4(spl), s1 = ld4(spl + 1), s2 = ld4(spl + 2), s3 = ld4(spl + 3);
+    const uint64_t lp0 = (uint64_t)la.x | ((uint64_t)la.y << 32), lp1 = (uint64_t)la.z | ((uint64_t)la.w << 32);
+    const uint64_t lp2 = (uint64_t)lb.x | ((uint64_t)lb.y << 32), lp3 = (uint64_t)lb.z | ((uint64_t)lb.w << 32);
+    const uint64_t hp0 = (uint64_t)ha.x | ((uint64_t)ha.y << 32), hp1 = (uint64_t)ha.z | ((uint64_t)ha.w << 32);
+    const uint64_t hp2 = (uint64_t)hb.x | ((uint64_t)hb.y << 32), hp3 = (uint64_t)hb.z | ((uint64_t)hb.w << 32);
+    const uint32_t lcw[8] = {lc.x, lc.y, lc.z, lc.w, ld.x, ld.y, ld.z, ld.w};
+    const uint32_t hcw[8] = {hc.x, hc.y, hc.z, hc.w, hd.x, hd.y, hd.z, hd.w};
+    const uint32_t sbw[16] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w,
+                              s2.x, s2.y, s2.z, s2.w, s3.x, s3.y, s3.z, s3.w};
+    const uint64_t lowl = (1ull << (xlo & 63)) - 1, lowh = (1ull << (xhi & 63)) - 1;
+    const uint32_t sb16 = (bl >> kSbRecShift) << kSbShift;
+    const uint32_t lcum16 = (bl & ((1u << kSbRecShift) - 1)) << kRecShift;
+    const uint32_t hcum16 = (bh & ((1u << kSbRecShift) - 1)) << kRecShift;
+    uint32_t pl = 0, ph = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const uint64_t el = ((k & 1) ? lp0 : ~lp0) & ((k & 2) ? lp1 : ~lp1) & ((k & 4) ? lp2 : ~lp2) &
+                            ((k & 8) ? lp3 : ~lp3);
+        const uint64_t eh = ((k & 1) ? hp0 : ~hp0) & ((k & 2) ? hp1 : ~hp1) & ((k & 4) ? hp2 : ~hp2) &
+                            ((k & 8) ? hp3 : ~hp3);
+        // slot k+1 of the cumulative counts (slot 16 = positions before the record)
+        const uint32_t sbn = k == 15 ? sb16 : sbw[k + 1];
+        const uint32_t cln = k == 15 ? lcum16 : (lcw[(k + 1) >> 1] >> (((k + 1) & 1) * 16)) & 0xffffu;
+        const uint32_t chn = k == 15 ? hcum16 : (hcw[(k + 1) >> 1] >> (((k + 1) & 1) * 16)) & 0xffffu;
+        const uint32_t nl = sbn + cln, nh = sbn + chn;
+        cl[k] = (nl - pl) + popc64(el & lowl);
+        ch[k] = (nh - ph) + popc64(eh & lowh);
+        pl = nl;
+        ph = nh;
+    }
+    if ((bl >> kSbRecShift) != (bh >> kSbRecShift)) {
+        RecView rh;
+        load_rec(d, xhi, rh);
+        lane_counts(rh, xhi, 16, ch);
+    }
+}
+
 // Lane of BWT row `row` and #same-lane rows in [0, row): the LF step of
 // src/occ/mod.rs:456-480 on the device layout.
 __device__ __forceinline__ void lf_lane_rank(const RecView &r, uint32_t row, uint32_t &lane, uint32_t &rank) {

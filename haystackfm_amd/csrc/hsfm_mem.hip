@@ -34,7 +34,8 @@ struct MemCounters {
     unsigned long long guard;
     unsigned long long probes;   // records fetched by k_mem_thread
     unsigned long long diag[4];  // deferrals: seed union, extension, slots; plain restarts
-    unsigned long long pad[7];
+    unsigned long long pwork;    // work index of k_mem_persist
+    unsigned long long pad[6];
 };
 
 struct MemRec { // 32 bytes
@@ -495,6 +496,259 @@ __device__ __forceinline__ uint32_t mem_thread_read(const DevDir &fwd, const Dev
     return probes;
 }
 
+// ---------------------------------------------------------------------------
+// Persistent MEM finder: the min_len-skip algorithm of mem_thread_read run as a
+// per-lane state machine.  Every pass of the main loop performs at most one
+// Occ probe per lane (one interval of the current set, or one seed-table entry
+// streamed into its first left extension), so lanes of a wave stay on the same
+// code path whatever phase their read is in; a lane that finishes a read takes
+// the next one from a device work queue.  Sets live in per-lane LDS slots of
+// PK intervals; reads that outgrow them are deferred.
+// ---------------------------------------------------------------------------
+enum : uint32_t {
+    SM_FETCH = 0, // take the next read from the queue
+    SM_CAND,      // next restart point j
+    SM_SEEDX,     // next seed-table expansion of q[j-F..j)
+    SM_ENDLEFT,   // left extension ended at k
+    SM_EMIT,      // right extension ended at e: write the records
+    SM_SEEDE,     // probe: one seed entry, extended left by q[j-F-1]
+    SM_LEFT,      // probe: one interval of the set of q[k..j), by q[k-1]
+    SM_RIGHT,     // probe: one interval of the set of q[k..e), by q[e]
+    SM_DONE
+};
+
+template <int PK, int PB>
+__global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, const DevConst *__restrict__ cst,
+                                                    SeedView sv, const uint8_t *__restrict__ q,
+                                                    const uint64_t *__restrict__ qoff,
+                                                    const uint32_t *__restrict__ list, uint32_t nlist,
+                                                    uint32_t min_len, uint4 *__restrict__ slots,
+                                                    uint32_t *__restrict__ mem_cnt, uint8_t *__restrict__ status,
+                                                    uint32_t *__restrict__ defer_list,
+                                                    MemCounters *__restrict__ ctr) {
+    __shared__ uint32_t s_lo[2][PK][PB], s_hi[2][PK][PB], s_rl[2][PK][PB];
+    __shared__ uint16_t s_compat[256];
+    __shared__ uint8_t s_expand[256];
+    __shared__ uint32_t s_clane[16];
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) {
+        s_compat[i] = cst->compat_lanes[i];
+        s_expand[i] = cst->seed_expand[i];
+    }
+    if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
+    __syncthreads();
+    const uint32_t tid = threadIdx.x;
+    const uint32_t L = cst->num_lanes;
+    const uint32_t n = fwd.n;
+    const uint32_t F = sv.F;
+    const bool seeding = F != 0 && min_len > F;
+
+    uint32_t st = SM_FETCH, r = 0, m = 0, s0 = 0, E = 0, j = 0, k = 0, e = 0;
+    uint32_t ns = 0, nn = 0, t = 0, cur = 0, nrec = 0, slot_cap = 0;
+    uint32_t xi = 0, base_idx = 0, apos = 0, amask = 0x11, lslmax = 0, probes = 0, restarts = 0;
+    uint64_t slot_base = 0, ient = 0, iend = 0;
+    const uint8_t *p = q;
+    uint4 piv = make_uint4(0, 0, 0, 0);
+
+    for (;;) {
+        // ---- control: move this lane to a probe state (or SM_DONE) ----
+        while (st < SM_SEEDE) {
+            if (st == SM_FETCH) {
+                const uint64_t act = __ballot(1);
+                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)act) - 1u;
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+                uint32_t base = 0;
+                if (__lane_id() == leader) base = (uint32_t)atomicAdd(&ctr->pwork, (unsigned long long)__popcll(act));
+                base = __shfl(base, (int)leader, 64);
+                const uint32_t wi = base + rank;
+                if (wi >= nlist) { st = SM_DONE; break; }
+                r = list ? list[wi] : wi;
+                const uint64_t b0 = qoff[r];
+                m = (uint32_t)(qoff[r + 1] - b0);
+                p = q + b0;
+                slot_base = (b0 >> 3) + 8ull * r;
+                slot_cap = (uint32_t)((qoff[r + 1] >> 3) - (b0 >> 3)) + 8u;
+                s0 = 0; E = 0; nrec = 0;
+                st = SM_CAND;
+            } else if (st == SM_CAND) {
+                j = (s0 + min_len > E + 1) ? s0 + min_len : E + 1;
+                if (min_len == 0 || j > m) {
+                    status[r] = 0;
+                    mem_cnt[r] = nrec;
+                    st = SM_FETCH;
+                    continue;
+                }
+                bool usable = false;
+                if (seeding && j > F) {
+                    // digits of q[j-F..j); up to two ambiguity codes expand
+                    uint32_t bi = 0, ap = 0, am = 0x11, namb = 0;
+                    usable = true;
+                    for (uint32_t u = 0; u < F; u++) {
+                        const uint32_t x = s_expand[p[j - F + u]];
+                        if (x & 0x10u) bi |= (x & 3u) << (2 * u);
+                        else if (x && namb < 2) {
+                            ap |= u << (8 * namb);
+                            am = (am & ~(0xfu << (4 * namb))) | (x << (4 * namb));
+                            namb++;
+                        } else usable = false;
+                    }
+                    base_idx = bi; apos = ap | (namb << 16); amask = am;
+                }
+                nn = 0;
+                if (usable) {
+                    xi = 0;
+                    lslmax = 0;
+                    st = SM_SEEDX;
+                } else {
+                    restarts++;
+                    s_lo[cur][0][tid] = 0;
+                    s_hi[cur][0][tid] = n;
+                    s_rl[cur][0][tid] = 0;
+                    ns = 1; t = 0; k = j;
+                    piv = make_uint4(0, n, 0, 0);
+                    st = SM_LEFT;
+                }
+            } else if (st == SM_SEEDX) {
+                // next (x0, x1) with x0 in amask[3:0], x1 in amask[7:4]
+                while (xi < 16 && !(((amask >> (xi & 3)) & 1u) && ((amask >> (4 + (xi >> 2))) & 1u))) xi++;
+                if (xi == 16) {
+                    if (lslmax < F) { // q[j-lslmax-1..j) does not occur: L(j) = j - lslmax
+                        s0 = j - lslmax;
+                        st = SM_CAND;
+                    } else if (nn == 0) { // q[j-F..j) occurs, q[j-F-1..j) does not: L(j) = j - F
+                        s0 = j - F;
+                        st = SM_CAND;
+                    } else {
+                        cur ^= 1; ns = nn; nn = 0; t = 0;
+                        k = j - F - 1;
+                        if (k == 0) st = SM_ENDLEFT;
+                        else {
+                            piv = make_uint4(s_lo[cur][0][tid], s_hi[cur][0][tid], s_rl[cur][0][tid], 0u);
+                            st = SM_LEFT;
+                        }
+                    }
+                    continue;
+                }
+                const uint32_t namb = apos >> 16;
+                uint32_t idx = base_idx;
+                if (namb > 0) idx |= (xi & 3u) << (2 * (apos & 0xffu));
+                if (namb > 1) idx |= (xi >> 2) << (2 * ((apos >> 8) & 0xffu));
+                xi++;
+                const uint32_t l = sv.lsl[idx];
+                probes++;
+                lslmax = l > lslmax ? l : lslmax;
+                if (l != F) continue;
+                ient = sv.off[idx];
+                iend = sv.off[idx + 1];
+                probes += 2;
+                if (ient == iend) continue;
+                piv = sv.iv[ient];
+                st = SM_SEEDE;
+            } else if (st == SM_ENDLEFT) {
+                if (k + min_len > j) {
+                    s0 = k;
+                    st = SM_CAND;
+                    continue;
+                }
+                e = j; t = 0; nn = 0;
+                if (e < m) {
+                    piv = make_uint4(s_lo[cur][0][tid], s_hi[cur][0][tid], s_rl[cur][0][tid], 0u);
+                    st = SM_RIGHT;
+                } else {
+                    st = SM_EMIT;
+                }
+            } else { // SM_EMIT
+                if (nrec + ns > slot_cap) {
+                    status[r] = 1;
+                    mem_cnt[r] = 0;
+                    defer_list[atomicAdd(&ctr->n_retry, 1ull)] = r;
+                    atomicAdd(&ctr->diag[2], 1ull);
+                    st = SM_FETCH;
+                    continue;
+                }
+                for (uint32_t u = 0; u < ns; u++) {
+                    const uint32_t lo = s_lo[cur][u][tid];
+                    uint32_t rank = 0;
+                    for (uint32_t w = 0; w < ns; w++) rank += s_lo[cur][w][tid] < lo ? 1u : 0u;
+                    slots[slot_base + nrec + rank] = make_uint4(k, e, lo, s_hi[cur][u][tid]);
+                }
+                nrec += ns;
+                s0 = k + 1;
+                E = e;
+                st = SM_CAND;
+            }
+        }
+        if (__ballot(st != SM_DONE) == 0) break;
+        if (st == SM_DONE) continue;
+
+        // ---- probe: one interval, extended by one query code ----
+        const bool left = st != SM_RIGHT;
+        const uint32_t size = piv.y - piv.x;
+        const uint32_t xlo = left ? piv.x : piv.z;
+        const uint32_t xhi = left ? piv.y : piv.z + size;
+        const uint32_t qpos = st == SM_SEEDE ? j - F - 1 : (left ? k - 1 : e);
+        const uint32_t cm = s_compat[p[qpos]];
+        const DevDir &D = left ? fwd : rev;
+        probes += 1u + ((xlo >> kRecShift) != (xhi >> kRecShift) ? 1u : 0u);
+        uint32_t cl[16], ch[16];
+        pair_counts(D, xlo, xhi, cl, ch);
+        const uint32_t nx = cur ^ 1u;
+        const bool dedup = st == SM_SEEDE;
+        bool ovf = false;
+        uint32_t acc = 0;
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) {
+            const uint32_t d = ch[kk] - cl[kk];
+            if (((cm >> kk) & 1u) && d != 0u) {
+                const uint32_t ca = s_clane[kk] + cl[kk];
+                const uint4 c = left ? make_uint4(ca, ca + d, piv.z + acc, 0u)
+                                     : make_uint4(piv.x + acc, piv.x + acc + d, ca, 0u);
+                bool dup = false;
+                if (dedup)
+                    for (uint32_t u = 0; u < nn; u++) dup |= s_lo[nx][u][tid] == c.x;
+                if (!dup) {
+                    if (nn == (uint32_t)PK) {
+                        ovf = true;
+                    } else {
+                        s_lo[nx][nn][tid] = c.x;
+                        s_hi[nx][nn][tid] = c.y;
+                        s_rl[nx][nn][tid] = c.z;
+                        nn++;
+                    }
+                }
+            }
+            acc += d;
+        }
+
+        // ---- advance ----
+        if (ovf) {
+            status[r] = 1;
+            mem_cnt[r] = 0;
+            defer_list[atomicAdd(&ctr->n_retry, 1ull)] = r;
+            atomicAdd(&ctr->diag[st == SM_SEEDE ? 0 : 1], 1ull);
+            st = SM_FETCH;
+        } else if (st == SM_SEEDE) {
+            ient++;
+            if (ient < iend) piv = sv.iv[ient];
+            else st = SM_SEEDX;
+        } else if (++t < ns) {
+            piv = make_uint4(s_lo[cur][t][tid], s_hi[cur][t][tid], s_rl[cur][t][tid], 0u);
+        } else if (nn == 0) {
+            st = st == SM_LEFT ? SM_ENDLEFT : SM_EMIT;
+        } else {
+            cur ^= 1u; ns = nn; nn = 0; t = 0;
+            bool more;
+            if (st == SM_LEFT) { k--; more = k > 0; if (!more) st = SM_ENDLEFT; }
+            else { e++; more = e < m; if (!more) st = SM_EMIT; }
+            if (more) piv = make_uint4(s_lo[cur][0][tid], s_hi[cur][0][tid], s_rl[cur][0][tid], 0u);
+        }
+    }
+    if (restarts) atomicAdd(&ctr->diag[3], (unsigned long long)restarts);
+#pragma unroll
+    for (int dd = 32; dd >= 1; dd >>= 1) probes += __shfl_xor(probes, dd, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&ctr->probes, (unsigned long long)probes);
+}
+
 // Copy the records of thread-path reads from their slots to the final order.
 __global__ __launch_bounds__(256) void k_mem_gather(uint32_t nq, const uint64_t *__restrict__ qoff,
                                                     const uint8_t *__restrict__ status, const uint32_t *__restrict__ mem_cnt,
@@ -596,6 +850,15 @@ static int build_seed_table(hsfm_index *idx, uint32_t F) {
 }
 } // namespace hsfm
 
+// Workgroups of a persistent kernel: as many as stay resident on every CU.
+template <typename K>
+static uint32_t persist_grid(hsfm_index *idx, K kernel, uint32_t block) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)block, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    return (uint32_t)per_cu * (uint32_t)idx->dev->num_cus;
+}
+
 static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *d_qoff, uint32_t nq, uint32_t min_len,
                            const uint32_t *d_refb, uint32_t n_refs, uint32_t max_hits, hsfm_mem_device_result *out) {
     Workspace &ws = idx->ws;
@@ -631,13 +894,23 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         if ((rc = ws.mem_slots.ensure(nslots * 16 + 16))) return rc;
         if ((rc = ws.mem_status.ensure((size_t)nq + 16))) return rc;
         SeedView sv{idx->seed_off.as<uint64_t>(), idx->seed_iv.as<uint4>(), idx->seed_lsl.as<uint8_t>(), F};
+        const bool old_thread = path && strcmp(path, "thread") == 0;
+        uint64_t hd[16];
         if ((rc = event_record(idx, 0, false))) return rc;
-        hipLaunchKernelGGL(k_mem_thread, dim3((nq + TPB - 1) / TPB), dim3(TPB), 0, s, idx->fwd.view, idx->rev.view,
-                           idx->d_const.as<DevConst>(), sv, d_q, d_qoff, nq, min_len, ws.mem_slots.as<uint4>(),
-                           ws.mem_cnt.as<uint32_t>(), ws.mem_status.as<uint8_t>(), ws.ovf_list.as<uint32_t>(), ctr);
+        if (old_thread) {
+            hipLaunchKernelGGL(k_mem_thread, dim3((nq + TPB - 1) / TPB), dim3(TPB), 0, s, idx->fwd.view, idx->rev.view,
+                               idx->d_const.as<DevConst>(), sv, d_q, d_qoff, nq, min_len, ws.mem_slots.as<uint4>(),
+                               ws.mem_cnt.as<uint32_t>(), ws.mem_status.as<uint8_t>(), ws.ovf_list.as<uint32_t>(),
+                               ctr);
+        } else {
+            const uint32_t g1 = persist_grid(idx, k_mem_persist<8, 256>, 256);
+            hipLaunchKernelGGL((k_mem_persist<8, 256>), dim3(g1), dim3(256), 0, s, idx->fwd.view, idx->rev.view,
+                               idx->d_const.as<DevConst>(), sv, d_q, d_qoff, (const uint32_t *)nullptr, nq, min_len,
+                               ws.mem_slots.as<uint4>(), ws.mem_cnt.as<uint32_t>(), ws.mem_status.as<uint8_t>(),
+                               ws.ovf_list.as<uint32_t>(), ctr);
+        }
         HSFM_HIP(hipGetLastError());
         if ((rc = event_record(idx, 0, true))) return rc;
-        uint64_t hd[16];
         if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), hd, 16))) return rc;
         nlist = hd[2];
         idx->stats.occ_probes = hd[4];
@@ -645,6 +918,24 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         list = ws.ovf_list.as<uint32_t>();
         retry = ws.read_list.as<uint32_t>();
         idx->stats.fallback_reads = nlist;
+        if (nlist && !old_thread) {
+            // second level: 32 intervals per lane for the deferred reads
+            if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_retry), 2, 0))) return rc;
+            if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->pwork), 2, 0))) return rc;
+            const uint32_t g2 = persist_grid(idx, k_mem_persist<32, 64>, 64);
+            const uint32_t need = (uint32_t)((nlist + 63) / 64);
+            hipLaunchKernelGGL((k_mem_persist<32, 64>), dim3(need < g2 ? need : g2), dim3(64), 0, s, idx->fwd.view,
+                               idx->rev.view, idx->d_const.as<DevConst>(), sv, d_q, d_qoff, list, (uint32_t)nlist,
+                               min_len, ws.mem_slots.as<uint4>(), ws.mem_cnt.as<uint32_t>(),
+                               ws.mem_status.as<uint8_t>(), retry, ctr);
+            HSFM_HIP(hipGetLastError());
+            if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), hd, 16))) return rc;
+            nlist = hd[2];
+            idx->stats.diag[3] = hd[8];
+            list = retry; // reads that outgrew 32 intervals go to the wave kernel
+            retry = ws.ovf_list.as<uint32_t>();
+            idx->stats.capacity_retries++;
+        }
     }
 
     uint64_t rec_cap = nlist * 8 + 4096;

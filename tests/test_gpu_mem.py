@@ -171,7 +171,8 @@ def test_thread_path_seed_table(seed):
     check(o, g, qs, 12 + 3 * seed)
 
 
-def test_thread_and_wave_paths_agree(monkeypatch):
+@pytest.mark.parametrize("other", ["wave", "thread"])
+def test_persistent_and_other_paths_agree(monkeypatch, other):
     rng = np.random.default_rng(11)
     text, bounds = O.gen_reference(9, 200_000, 3, gc=0.41, repeat_frac=0.2, n_units=20, unit_min=300,
                                    unit_max=1500, iupac_frac=0.01)
@@ -179,10 +180,32 @@ def test_thread_and_wave_paths_agree(monkeypatch):
     g = hf.BidirFmIndex.from_bytes(o.to_bytes())
     qs = _mem_reads(rng, text, 500, 150, 0.02, 0.02)
     a = g.find_mems_batch(qs, 18, 0, bounds, 64)
-    monkeypatch.setenv("HSFM_MEM_PATH", "wave")
+    monkeypatch.setenv("HSFM_MEM_PATH", other)
     b = g.find_mems_batch(qs, 18, 0, bounds, 64)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+def test_persistent_kernel_deferral_levels():
+    """Copies of one unit with scattered ambiguity codes: the sets of a 13-mer
+    window hold one interval per distinct copy, outgrowing the 8-interval and
+    32-interval levels, so reads end on the wave kernel; all stay exact."""
+    rng = np.random.default_rng(21)
+    unit = random_text(rng, 400)
+    copies = []
+    for _ in range(70):
+        u = unit.copy()
+        mask = rng.random(400) < 0.03
+        u[mask] = rng.choice(O.IUPAC_AMBIG, size=int(mask.sum())).astype(np.uint8)
+        copies.append(u)
+    seqs = [O.decode(np.concatenate(copies[:35])), O.decode(np.concatenate(copies[35:]))]
+    o, g = _pair(seqs, sample_rate=8)
+    text, _ = O.concatenate(seqs)
+    qs = _mem_reads(rng, text, 200, 120, 0.01, 0.0)
+    qs += [unit[i:i + 100].copy() for i in range(0, 300, 7)]
+    check(o, g, qs, 13)
+    st = g.last_stats()
+    assert st["fallback_reads"] > 0 and st["capacity_retries"] > 0
 
 
 def test_locate_then_mem_on_one_bidir_index():
