@@ -265,7 +265,7 @@ def main():
         achieved = bytes_per_launch / (kms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_mem_thread", "kernel_ms": round(kms, 3),
+                "kernel": "k_mem_persist", "kernel_ms": round(kms, 3),
                 "bytes_per_read": round(bytes_per_launch / nq, 1)}
     loc_roof = None
     lkms = float(np.mean(lk))

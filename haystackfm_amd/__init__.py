@@ -217,7 +217,8 @@ class _MemDev(C.Structure):
 class _Stats(C.Structure):
     _fields_ = [("fallback_reads", C.c_uint64), ("capacity_retries", C.c_uint64),
                 ("guard_hits", C.c_uint64), ("records", C.c_uint64), ("main_kernel_ms", C.c_double),
-                ("resolve_kernel_ms", C.c_double), ("occ_probes", C.c_uint64), ("diag", C.c_uint64 * 4)]
+                ("resolve_kernel_ms", C.c_double), ("occ_probes", C.c_uint64), ("diag", C.c_uint64 * 4),
+                ("fallback_kernel_ms", C.c_double)]
 
 
 _lib_obj = None

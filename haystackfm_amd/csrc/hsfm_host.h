@@ -53,7 +53,7 @@ struct Workspace {
     DevBuf mem_recs, mem_cnt, mem_off, mem_sorted, mem_hits, pos_off, positions, mem_flags, read_list;
     DevBuf wave_sets, mem_slots, mem_status;
     uint64_t *h_pinned = nullptr;             // small readbacks
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr}; // main kernel, resolve kernel
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}; // main, resolve, fallback kernels
 };
 
 } // namespace hsfm
@@ -83,6 +83,16 @@ int fill_u32(hsfm_index *idx, uint32_t *d, uint64_t n, uint32_t v);
 // HIP events bracketing the main (slot 0) and resolve (slot 1) kernels.
 int event_record(hsfm_index *idx, int slot, bool end);
 double event_ms(hsfm_index *idx, int slot);
+
+// Workgroups of a persistent kernel: as many as stay resident on every CU.
+template <typename K>
+inline uint32_t persist_grid(hsfm_index *idx, K kernel, uint32_t block) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)block, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    return (uint32_t)per_cu * (uint32_t)idx->dev->num_cus;
+}
+
 
 // Row expansion + LF-walk resolve + boundary map, shared by locate and MEM.
 //   rows of segment s: [lo[s], lo[s] + len[s]) written from dst offsets off[s]

@@ -16,6 +16,7 @@
 //      (seq_idx, pos_in_seq).
 // Intervals are unions of rows; merging adjacent intervals (query.rs:390-407)
 // does not change the row set, so it is not performed.
+#include <stdlib.h>
 #include <string.h>
 
 #include <hipcub/hipcub.hpp>
@@ -33,8 +34,9 @@ struct LocateCounters {            // device counters (u64 each)
     unsigned long long fb_used;    // fallback interval slots reserved
     unsigned long long n_retry;    // reads that need a larger wave capacity
     unsigned long long guard;      // walk guard hits
-    unsigned long long probes;     // records fetched by k_locate_search
-    unsigned long long pad[2];
+    unsigned long long probes;     // records fetched by the main search kernel
+    unsigned long long pwork;      // work index of k_locate_persist
+    unsigned long long pad[1];
 };
 
 __device__ __forceinline__ void load_const_lds(const DevConst *cst, uint16_t *s_compat, int8_t *s_digit,
@@ -173,6 +175,151 @@ __device__ __forceinline__ uint32_t locate_read(const DevDir &d, const DevConst 
     niv[r] = (uint8_t)ns;
     cnt[r] = total;
     return probes;
+}
+
+// ---------------------------------------------------------------------------
+// Persistent backward search: locate_read as a per-lane state machine.  Each
+// pass of the main loop extends one interval of one read by one code, so the
+// lanes of a wave stay on one code path; a lane whose read ends takes the next
+// read from a device work queue.  The interval set of a lane lives in a ring
+// of PR LDS slots (current set at [head, head+ns), children behind it).  A
+// final set of more than KR intervals is published, in row order, to fb_ivs;
+// reads whose set outgrows the ring go to k_locate_wave.
+// ---------------------------------------------------------------------------
+enum : uint32_t { LS_FETCH = 0, LS_NEXT, LS_STEP, LS_DONE };
+
+template <int PR, int PB>
+__global__ __launch_bounds__(PB) void k_locate_persist(DevDir d, const DevConst *__restrict__ cst,
+                                                       const uint8_t *__restrict__ q,
+                                                       const uint64_t *__restrict__ qoff, uint32_t nq,
+                                                       uint32_t *__restrict__ cnt, uint8_t *__restrict__ niv,
+                                                       uint2 *__restrict__ ivs, uint2 *__restrict__ fb_meta,
+                                                       uint2 *__restrict__ fb_ivs, uint64_t fb_cap,
+                                                       uint32_t *__restrict__ ovf_list,
+                                                       LocateCounters *__restrict__ ctr) {
+    static_assert((PR & (PR - 1)) == 0, "ring size must be a power of two");
+    constexpr uint32_t RM = PR - 1;
+    __shared__ uint32_t s_lo[PR][PB], s_hi[PR][PB];
+    __shared__ uint16_t s_compat[256];
+    __shared__ int8_t s_digit[256];
+    __shared__ uint32_t s_clane[16];
+    load_const_lds(cst, s_compat, s_digit, s_clane);
+    __syncthreads();
+    const uint32_t tid = threadIdx.x;
+    const uint32_t n = d.n;
+    const uint32_t depth = cst->lookup_depth, radix = cst->lookup_radix;
+    uint32_t st = LS_FETCH, r = 0, i = 0, head = 0, ns = 0, nn = 0, t = 0, probes = 0;
+    const uint8_t *p = q;
+    uint2 piv = make_uint2(0, 0);
+
+    for (;;) {
+        while (st < LS_STEP) {
+            if (st == LS_FETCH) {
+                const uint64_t act = __ballot(1);
+                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)act) - 1u;
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+                uint32_t base = 0;
+                if (__lane_id() == leader) base = (uint32_t)atomicAdd(&ctr->pwork, (unsigned long long)__popcll(act));
+                base = __shfl(base, (int)leader, 64);
+                r = base + rank;
+                if (r >= nq) { st = LS_DONE; break; }
+                const uint64_t b0 = qoff[r];
+                const uint32_t m = (uint32_t)(qoff[r + 1] - b0);
+                p = q + b0;
+                uint32_t lo = 0, hi = n;
+                i = m;
+                ns = 1;
+                const int sd = lookup_seed(d, s_digit, depth, radix, p, m, lo, hi);
+                if (sd) probes++;
+                if (sd == 2) ns = 0;
+                else if (sd == 1) i = m - depth;
+                s_lo[head][tid] = lo;
+                s_hi[head][tid] = hi;
+                st = LS_NEXT;
+            } else { // LS_NEXT: extend the set by p[i-1], or finish the read
+                if (ns != 0 && i != 0) {
+                    t = 0;
+                    nn = 0;
+                    piv = make_uint2(s_lo[head][tid], s_hi[head][tid]);
+                    st = LS_STEP;
+                    break;
+                }
+                // rows in ascending order: rank of each interval by its start
+                uint32_t total = 0;
+                for (uint32_t u = 0; u < ns; u++) total += s_hi[(head + u) & RM][tid] - s_lo[(head + u) & RM][tid];
+                if (ns <= KR) {
+                    for (uint32_t u = 0; u < ns; u++) {
+                        const uint32_t lo = s_lo[(head + u) & RM][tid];
+                        uint32_t rk = 0;
+                        for (uint32_t w = 0; w < ns; w++) rk += s_lo[(head + w) & RM][tid] < lo ? 1u : 0u;
+                        ivs[(size_t)r * KR + rk] = make_uint2(lo, s_hi[(head + u) & RM][tid]);
+                    }
+                    niv[r] = (uint8_t)ns;
+                    cnt[r] = total;
+                } else {
+                    const unsigned long long slot = atomicAdd(&ctr->fb_used, (unsigned long long)ns);
+                    if (slot + ns > fb_cap) {
+                        niv[r] = kNivOverflow;
+                        cnt[r] = 0;
+                        ovf_list[atomicAdd(&ctr->n_ovf, 1ull)] = r;
+                    } else {
+                        for (uint32_t u = 0; u < ns; u++) {
+                            const uint32_t lo = s_lo[(head + u) & RM][tid];
+                            uint32_t rk = 0;
+                            for (uint32_t w = 0; w < ns; w++) rk += s_lo[(head + w) & RM][tid] < lo ? 1u : 0u;
+                            fb_ivs[slot + rk] = make_uint2(lo, s_hi[(head + u) & RM][tid]);
+                        }
+                        fb_meta[r] = make_uint2((uint32_t)slot, ns);
+                        niv[r] = kNivOverflow;
+                        cnt[r] = total;
+                    }
+                }
+                st = LS_FETCH;
+            }
+        }
+        if (__ballot(st != LS_DONE) == 0) break;
+        if (st == LS_DONE) continue;
+
+        // ---- probe: one interval extended by p[i-1] ----
+        const uint32_t cm = s_compat[p[i - 1]];
+        probes += 1u + ((piv.x >> kRecShift) != (piv.y >> kRecShift) ? 1u : 0u);
+        uint32_t cl[16], ch[16];
+        pair_counts(d, piv.x, piv.y, cl, ch);
+        const uint32_t cbase = head + ns;
+        const uint32_t room = PR - (ns - t - 1);
+        bool ovf = false;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            if (((cm >> k) & 1u) && ch[k] > cl[k]) {
+                if (nn + 1 > room) {
+                    ovf = true;
+                } else {
+                    const uint32_t w = (cbase + nn) & RM;
+                    s_lo[w][tid] = s_clane[k] + cl[k];
+                    s_hi[w][tid] = s_clane[k] + ch[k];
+                    nn++;
+                }
+            }
+        }
+        if (ovf) {
+            niv[r] = kNivOverflow;
+            cnt[r] = 0;
+            ovf_list[atomicAdd(&ctr->n_ovf, 1ull)] = r;
+            st = LS_FETCH;
+        } else if (++t < ns) {
+            const uint32_t w = (head + t) & RM;
+            piv = make_uint2(s_lo[w][tid], s_hi[w][tid]);
+        } else {
+            head = (head + ns) & RM;
+            ns = nn;
+            i--;
+            st = LS_NEXT;
+        }
+    }
+#pragma unroll
+    for (int dd = 32; dd >= 1; dd >>= 1) probes += __shfl_xor(probes, dd, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&ctr->probes, (unsigned long long)probes);
 }
 
 // One wave per queued read.  Set buffers: per wave 2 * cap (lo,hi) pairs.
@@ -340,13 +487,26 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
     if ((rc = ws.fb_meta.ensure((size_t)nq * 8 + 8))) return rc;
     LocateCounters *ctr = ws.counters.as<LocateCounters>();
     if ((rc = fill_u32(idx, ws.counters.as<uint32_t>(), sizeof(LocateCounters) / 4, 0))) return rc;
+    // final sets of more than KR intervals are published to fb_ivs
+    if ((rc = ws.fb_ivs.ensure(((size_t)nq * 2 + 65536) * 8))) return rc;
     if (nq) {
+        const char *path = getenv("HSFM_LOCATE_PATH");
         if ((rc = event_record(idx, 0, false))) return rc;
-        hipLaunchKernelGGL(k_locate_search, dim3((nq + 255) / 256), dim3(256), 0, s, d,
-                           idx->d_const.as<DevConst>(), d_q, d_qoff, nq, ws.cnt.as<uint32_t>(),
-                           ws.niv.as<uint8_t>(), ws.ivs.as<uint2>(), ws.ovf_list.as<uint32_t>(), ctr);
+        if (path && strcmp(path, "thread") == 0) {
+            hipLaunchKernelGGL(k_locate_search, dim3((nq + 255) / 256), dim3(256), 0, s, d,
+                               idx->d_const.as<DevConst>(), d_q, d_qoff, nq, ws.cnt.as<uint32_t>(),
+                               ws.niv.as<uint8_t>(), ws.ivs.as<uint2>(), ws.ovf_list.as<uint32_t>(), ctr);
+        } else {
+            const uint32_t g = persist_grid(idx, k_locate_persist<16, 256>, 256);
+            const uint32_t need = (nq + 255) / 256;
+            hipLaunchKernelGGL((k_locate_persist<16, 256>), dim3(need < g ? need : g), dim3(256), 0, s, d,
+                               idx->d_const.as<DevConst>(), d_q, d_qoff, nq, ws.cnt.as<uint32_t>(),
+                               ws.niv.as<uint8_t>(), ws.ivs.as<uint2>(), ws.fb_meta.as<uint2>(),
+                               ws.fb_ivs.as<uint2>(), (uint64_t)(ws.fb_ivs.cap / 8), ws.ovf_list.as<uint32_t>(), ctr);
+        }
         HSFM_HIP(hipGetLastError());
         if ((rc = event_record(idx, 0, true))) return rc;
+        if ((rc = event_record(idx, 2, false))) return rc;
     }
     uint64_t h[8];
     if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
@@ -357,7 +517,7 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
         HSFM_HIP(hipMemcpyAsync(ws.status.p, ws.ovf_list.p, nlist * 4, hipMemcpyDeviceToDevice, s));
     }
     uint32_t cap = 4096;
-    uint64_t fb_cap = nlist * 256 + 4096;
+    uint64_t fb_cap = h[2] + nlist * 256 + 4096;
     const uint32_t *list = ws.ovf_list.as<uint32_t>();
     uint32_t *retry = ws.read_list.as<uint32_t>();
     while (nlist) {
@@ -419,10 +579,13 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
         HSFM_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tmp, k_in, k_out, (int)used, (int)nfb, seg_b,
                                                              seg_e, 0, 64, s));
         if ((rc = ws.scan_tmp.ensure(tmp))) return rc;
+        // intervals outside the sorted segments (already in row order) pass through
+        HSFM_HIP(hipMemcpyAsync(k_out, k_in, used * 8, hipMemcpyDeviceToDevice, s));
         HSFM_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(ws.scan_tmp.p, tmp, k_in, k_out, (int)used, (int)nfb,
                                                              seg_b, seg_e, 0, 64, s));
         hipLaunchKernelGGL(k_fb_unkeys, dim3(kb ? kb : 1), dim3(256), 0, s, k_out, used, ws.fb_ivs.as<uint2>());
         HSFM_HIP(hipGetLastError());
+        if ((rc = event_record(idx, 2, true))) return rc;
     }
     return HSFM_OK;
 }
@@ -460,6 +623,7 @@ static int locate_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_
     idx->stats.occ_probes = h[5];
     idx->stats.main_kernel_ms = event_ms(idx, 0);
     idx->stats.resolve_kernel_ms = total ? event_ms(idx, 1) : 0.0;
+    idx->stats.fallback_kernel_ms = idx->stats.fallback_reads ? event_ms(idx, 2) : 0.0;
     if (h[4]) return set_error(HSFM_EINDEX, "%llu LF walks exceeded the sample rate: index is inconsistent", (unsigned long long)h[4]);
     out->num_queries = nq;
     out->total_hits = total;

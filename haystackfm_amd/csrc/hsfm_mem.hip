@@ -42,6 +42,53 @@ struct MemRec { // 32 bytes
     uint32_t read, local, qs, qe, lo, hi, pass, pad;
 };
 
+// Seed table view (built by build_seed_table): per ACGT F-mer index, the
+// offsets of its interval list, the bidirectional intervals, and lsl.
+struct SeedView {
+    const uint64_t *off;
+    const uint4 *iv;
+    const uint8_t *lsl;
+    uint32_t F;
+};
+
+// A seed window q[j-F..j) as table indices: ACGT codes give fixed digits, up
+// to kSeedAmb ambiguity codes expand to every base they contain (the union of
+// the bases' compatible sets is the code's own).  pos holds 8 bits, mask 4
+// bits (the bases) per code; unused codes keep mask 1, digit 0.
+constexpr uint32_t kSeedAmb = 3;
+constexpr uint32_t kSeedCombos = 1u << (2 * kSeedAmb);
+struct SeedWin {
+    uint32_t base, pos, mask, namb;
+};
+__device__ __forceinline__ bool seed_window(const uint8_t *w, uint32_t F, const uint8_t *s_expand, SeedWin &sw,
+                                            uint32_t max_amb = kSeedAmb) {
+    sw.base = 0; sw.pos = 0; sw.mask = 0x111u; sw.namb = 0;
+    bool ok = true;
+    for (uint32_t u = 0; u < F; u++) {
+        const uint32_t x = s_expand[w[u]];
+        if (x & 0x10u) sw.base |= (x & 3u) << (2 * u);
+        else if (x && sw.namb < max_amb) {
+            sw.pos |= u << (8 * sw.namb);
+            sw.mask = (sw.mask & ~(0xfu << (4 * sw.namb))) | (x << (4 * sw.namb));
+            sw.namb++;
+        } else ok = false;
+    }
+    return ok;
+}
+// Table index of expansion xi (base digit i = bits 2i..2i+1 of xi); false if
+// a digit is not a base of its code.
+__device__ __forceinline__ bool seed_combo(const SeedWin &sw, uint32_t xi, uint32_t &idx) {
+    idx = sw.base;
+    bool ok = true;
+#pragma unroll
+    for (uint32_t i = 0; i < kSeedAmb; i++) {
+        const uint32_t d = (xi >> (2 * i)) & 3u;
+        ok &= ((sw.mask >> (4 * i + d)) & 1u) != 0;
+        if (i < sw.namb) idx |= d << (2 * ((sw.pos >> (8 * i)) & 0xffu));
+    }
+    return ok;
+}
+
 // Extend every interval of A (ns entries) by query code c.  LEFT: P -> cP on
 // the forward index; else P -> Pc on the reverse index.  Writes the non-empty
 // children to B and returns their number; sets too_big when cap is exceeded.
@@ -91,22 +138,36 @@ __device__ uint32_t wave_extend(const DevDir &D, uint32_t L, const uint32_t *s_c
     return total;
 }
 
+// One wave per read.  LDS: both sets in LDS (kWaveLdsCap intervals each);
+// otherwise in global memory, cap intervals each.  Restarts whose window
+// q[j-F..j) is all ACGT start from the seed table when min_len > F.
+constexpr uint32_t kWaveLdsCap = 1024;
+
+template <bool LDS>
 __global__ __launch_bounds__(64) void k_mem_wave(DevDir fwd, DevDir rev, const DevConst *__restrict__ cst,
-                                                 const uint8_t *__restrict__ q, const uint64_t *__restrict__ qoff,
+                                                 SeedView sv, const uint8_t *__restrict__ q,
+                                                 const uint64_t *__restrict__ qoff,
                                                  const uint32_t *__restrict__ list, uint32_t nlist, uint32_t min_len,
                                                  uint32_t cap, uint4 *__restrict__ sets, MemRec *__restrict__ recs,
                                                  uint64_t rec_cap, uint32_t *__restrict__ mem_cnt,
                                                  uint32_t *__restrict__ pass_of, uint32_t pass,
                                                  uint32_t *__restrict__ retry_list, MemCounters *__restrict__ ctr) {
     __shared__ uint16_t s_compat[256];
+    __shared__ uint8_t s_expand[256];
     __shared__ uint32_t s_clane[16];
-    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_compat[i] = cst->compat_lanes[i];
+    __shared__ uint4 s_sets[LDS ? 2 * kWaveLdsCap : 1];
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) {
+        s_compat[i] = cst->compat_lanes[i];
+        s_expand[i] = cst->seed_expand[i];
+    }
     if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
     __syncthreads();
     const uint32_t lane = threadIdx.x;
     const uint32_t L = cst->num_lanes;
     const uint32_t n = fwd.n;
-    uint4 *bufA = sets + (size_t)blockIdx.x * 2 * cap;
+    const uint32_t F = sv.F;
+    if (LDS) cap = kWaveLdsCap;
+    uint4 *bufA = LDS ? s_sets : sets + (size_t)blockIdx.x * 2 * cap;
     uint4 *bufB = bufA + cap;
     for (;;) {
         uint32_t wi = 0;
@@ -123,9 +184,59 @@ __global__ __launch_bounds__(64) void k_mem_wave(DevDir fwd, DevDir rev, const D
             const uint32_t j = (s0 + min_len > E + 1) ? s0 + min_len : E + 1;
             if (j > m) break;
             uint4 *A = bufA, *B = bufB;
-            if (lane == 0) A[0] = make_uint4(0, n, 0, 0);
-            __syncthreads();
             uint32_t ns = 1, k = j;
+            bool seeded = false;
+            if (F && min_len > F && j >= F) {
+                // q[j-F..j) from the table: the union of its expansions' entries
+                SeedWin sw;
+                if (seed_window(p + j - F, F, s_expand, sw)) {
+                    const uint32_t namb = sw.namb;
+                    uint32_t lslmax = 0, total = 0;
+                    for (uint32_t xi = 0; xi < kSeedCombos; xi++) {
+                        uint32_t idx;
+                        if (!seed_combo(sw, xi, idx)) continue;
+                        const uint32_t l = sv.lsl[idx];
+                        lslmax = l > lslmax ? l : lslmax;
+                        if (l != F) continue;
+                        const uint64_t o0 = sv.off[idx], o1 = sv.off[idx + 1];
+                        if (total + (o1 - o0) <= cap)
+                            for (uint32_t t = lane; t < (uint32_t)(o1 - o0); t += 64) A[total + t] = sv.iv[o0 + t];
+                        total += (uint32_t)(o1 - o0);
+                    }
+                    if (lslmax < F) { // q[j-lslmax-1..j) does not occur: L(j) = j - lslmax
+                        s0 = j - lslmax;
+                        continue;
+                    }
+                    if (total > cap) {
+                        too_big = true;
+                        break;
+                    }
+                    __syncthreads();
+                    ns = total;
+                    if (namb) { // drop entries repeated across expansions (same fwd_lo)
+                        uint32_t kept = 0;
+                        for (uint32_t base = 0; base < total; base += 64) {
+                            const uint32_t t = base + lane;
+                            bool keep = t < total;
+                            uint4 iv = make_uint4(0, 0, 0, 0);
+                            if (keep) {
+                                iv = A[t];
+                                for (uint32_t u = 0; u < t && keep; u++) keep = A[u].x != iv.x;
+                            }
+                            const uint32_t incl = wave_incl_scan(keep ? 1u : 0u);
+                            if (keep) B[kept + incl - 1] = iv;
+                            kept += __shfl(incl, 63, 64);
+                        }
+                        __syncthreads();
+                        uint4 *tmp = A; A = B; B = tmp;
+                        ns = kept;
+                    }
+                    k = j - F;
+                    seeded = true;
+                }
+            }
+            if (!seeded && lane == 0) A[0] = make_uint4(0, n, 0, 0);
+            __syncthreads();
             while (k > 0) {
                 const uint32_t c2 = wave_extend<true>(fwd, L, s_clane, s_compat[p[k - 1]], A, ns, B, cap, too_big);
                 __syncthreads();
@@ -245,12 +356,6 @@ __global__ void k_seed_root(uint64_t *off, uint4 *iv, uint8_t *lsl, uint32_t n) 
     lsl[0] = 0;
 }
 
-struct SeedView {
-    const uint64_t *off;
-    const uint4 *iv;
-    const uint8_t *lsl;
-    uint32_t F;
-};
 
 // ---------------------------------------------------------------------------
 // Thread-per-read MEM finder: the min_len-skip algorithm of k_mem_wave with the
@@ -502,8 +607,10 @@ __device__ __forceinline__ uint32_t mem_thread_read(const DevDir &fwd, const Dev
 // Occ probe per lane (one interval of the current set, or one seed-table entry
 // streamed into its first left extension), so lanes of a wave stay on the same
 // code path whatever phase their read is in; a lane that finishes a read takes
-// the next one from a device work queue.  Sets live in per-lane LDS slots of
-// PK intervals; reads that outgrow them are deferred.
+// the next one from a device work queue.  The interval sets of a lane live in
+// a ring of PR LDS slots: the current set at [head, head+ns), its children
+// appended behind it, so a set may hold up to PR intervals less the parents
+// still unprocessed; reads that outgrow the ring are deferred.
 // ---------------------------------------------------------------------------
 enum : uint32_t {
     SM_FETCH = 0, // take the next read from the queue
@@ -517,7 +624,7 @@ enum : uint32_t {
     SM_DONE
 };
 
-template <int PK, int PB>
+template <int PR, int PB>
 __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, const DevConst *__restrict__ cst,
                                                     SeedView sv, const uint8_t *__restrict__ q,
                                                     const uint64_t *__restrict__ qoff,
@@ -526,7 +633,9 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
                                                     uint32_t *__restrict__ mem_cnt, uint8_t *__restrict__ status,
                                                     uint32_t *__restrict__ defer_list,
                                                     MemCounters *__restrict__ ctr) {
-    __shared__ uint32_t s_lo[2][PK][PB], s_hi[2][PK][PB], s_rl[2][PK][PB];
+    static_assert((PR & (PR - 1)) == 0, "ring size must be a power of two");
+    __shared__ uint32_t s_lo[PR][PB], s_hi[PR][PB], s_rl[PR][PB];
+    constexpr uint32_t RM = PR - 1;
     __shared__ uint16_t s_compat[256];
     __shared__ uint8_t s_expand[256];
     __shared__ uint32_t s_clane[16];
@@ -543,8 +652,9 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
     const bool seeding = F != 0 && min_len > F;
 
     uint32_t st = SM_FETCH, r = 0, m = 0, s0 = 0, E = 0, j = 0, k = 0, e = 0;
-    uint32_t ns = 0, nn = 0, t = 0, cur = 0, nrec = 0, slot_cap = 0;
-    uint32_t xi = 0, base_idx = 0, apos = 0, amask = 0x11, lslmax = 0, probes = 0, restarts = 0;
+    uint32_t ns = 0, nn = 0, t = 0, head = 0, nrec = 0, slot_cap = 0;
+    uint32_t xi = 0, lslmax = 0, probes = 0, restarts = 0;
+    SeedWin sw{0, 0, 0x111u, 0};
     uint64_t slot_base = 0, ient = 0, iend = 0;
     const uint8_t *p = q;
     uint4 piv = make_uint4(0, 0, 0, 0);
@@ -578,40 +688,29 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
                     st = SM_FETCH;
                     continue;
                 }
-                bool usable = false;
-                if (seeding && j > F) {
-                    // digits of q[j-F..j); up to two ambiguity codes expand
-                    uint32_t bi = 0, ap = 0, am = 0x11, namb = 0;
-                    usable = true;
-                    for (uint32_t u = 0; u < F; u++) {
-                        const uint32_t x = s_expand[p[j - F + u]];
-                        if (x & 0x10u) bi |= (x & 3u) << (2 * u);
-                        else if (x && namb < 2) {
-                            ap |= u << (8 * namb);
-                            am = (am & ~(0xfu << (4 * namb))) | (x << (4 * namb));
-                            namb++;
-                        } else usable = false;
-                    }
-                    base_idx = bi; apos = ap | (namb << 16); amask = am;
-                }
+                // at most two ambiguity codes here: a third makes the expansion
+                // loop too long for one lane; such windows go to the wave kernel
+                const bool usable = seeding && j > F && seed_window(p + j - F, F, s_expand, sw, 2);
                 nn = 0;
+                t = 0;
                 if (usable) {
+                    ns = 0; // seed entries come from the table, not the ring
                     xi = 0;
                     lslmax = 0;
                     st = SM_SEEDX;
                 } else {
                     restarts++;
-                    s_lo[cur][0][tid] = 0;
-                    s_hi[cur][0][tid] = n;
-                    s_rl[cur][0][tid] = 0;
-                    ns = 1; t = 0; k = j;
+                    s_lo[head][tid] = 0;
+                    s_hi[head][tid] = n;
+                    s_rl[head][tid] = 0;
+                    ns = 1; k = j;
                     piv = make_uint4(0, n, 0, 0);
                     st = SM_LEFT;
                 }
             } else if (st == SM_SEEDX) {
-                // next (x0, x1) with x0 in amask[3:0], x1 in amask[7:4]
-                while (xi < 16 && !(((amask >> (xi & 3)) & 1u) && ((amask >> (4 + (xi >> 2))) & 1u))) xi++;
-                if (xi == 16) {
+                uint32_t idx = 0;
+                while (xi < kSeedCombos && !seed_combo(sw, xi, idx)) xi++;
+                if (xi == kSeedCombos) {
                     if (lslmax < F) { // q[j-lslmax-1..j) does not occur: L(j) = j - lslmax
                         s0 = j - lslmax;
                         st = SM_CAND;
@@ -619,20 +718,16 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
                         s0 = j - F;
                         st = SM_CAND;
                     } else {
-                        cur ^= 1; ns = nn; nn = 0; t = 0;
+                        ns = nn; nn = 0; t = 0;
                         k = j - F - 1;
                         if (k == 0) st = SM_ENDLEFT;
                         else {
-                            piv = make_uint4(s_lo[cur][0][tid], s_hi[cur][0][tid], s_rl[cur][0][tid], 0u);
+                            piv = make_uint4(s_lo[head][tid], s_hi[head][tid], s_rl[head][tid], 0u);
                             st = SM_LEFT;
                         }
                     }
                     continue;
                 }
-                const uint32_t namb = apos >> 16;
-                uint32_t idx = base_idx;
-                if (namb > 0) idx |= (xi & 3u) << (2 * (apos & 0xffu));
-                if (namb > 1) idx |= (xi >> 2) << (2 * ((apos >> 8) & 0xffu));
                 xi++;
                 const uint32_t l = sv.lsl[idx];
                 probes++;
@@ -652,7 +747,7 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
                 }
                 e = j; t = 0; nn = 0;
                 if (e < m) {
-                    piv = make_uint4(s_lo[cur][0][tid], s_hi[cur][0][tid], s_rl[cur][0][tid], 0u);
+                    piv = make_uint4(s_lo[head][tid], s_hi[head][tid], s_rl[head][tid], 0u);
                     st = SM_RIGHT;
                 } else {
                     st = SM_EMIT;
@@ -667,10 +762,10 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
                     continue;
                 }
                 for (uint32_t u = 0; u < ns; u++) {
-                    const uint32_t lo = s_lo[cur][u][tid];
+                    const uint32_t lo = s_lo[(head + u) & RM][tid];
                     uint32_t rank = 0;
-                    for (uint32_t w = 0; w < ns; w++) rank += s_lo[cur][w][tid] < lo ? 1u : 0u;
-                    slots[slot_base + nrec + rank] = make_uint4(k, e, lo, s_hi[cur][u][tid]);
+                    for (uint32_t w = 0; w < ns; w++) rank += s_lo[(head + w) & RM][tid] < lo ? 1u : 0u;
+                    slots[slot_base + nrec + rank] = make_uint4(k, e, lo, s_hi[(head + u) & RM][tid]);
                 }
                 nrec += ns;
                 s0 = k + 1;
@@ -692,7 +787,9 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
         probes += 1u + ((xlo >> kRecShift) != (xhi >> kRecShift) ? 1u : 0u);
         uint32_t cl[16], ch[16];
         pair_counts(D, xlo, xhi, cl, ch);
-        const uint32_t nx = cur ^ 1u;
+        const uint32_t cbase = head + ns; // children start behind the current set
+        // slots not held by parents still to be processed (this one is in piv)
+        const uint32_t room = st == SM_SEEDE ? PR : PR - (ns - t - 1);
         const bool dedup = st == SM_SEEDE;
         bool ovf = false;
         uint32_t acc = 0;
@@ -705,14 +802,15 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
                                      : make_uint4(piv.x + acc, piv.x + acc + d, ca, 0u);
                 bool dup = false;
                 if (dedup)
-                    for (uint32_t u = 0; u < nn; u++) dup |= s_lo[nx][u][tid] == c.x;
+                    for (uint32_t u = 0; u < nn; u++) dup |= s_lo[(cbase + u) & RM][tid] == c.x;
                 if (!dup) {
-                    if (nn == (uint32_t)PK) {
+                    if (nn + 1 > room) {
                         ovf = true;
                     } else {
-                        s_lo[nx][nn][tid] = c.x;
-                        s_hi[nx][nn][tid] = c.y;
-                        s_rl[nx][nn][tid] = c.z;
+                        const uint32_t w = (cbase + nn) & RM;
+                        s_lo[w][tid] = c.x;
+                        s_hi[w][tid] = c.y;
+                        s_rl[w][tid] = c.z;
                         nn++;
                     }
                 }
@@ -732,15 +830,16 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
             if (ient < iend) piv = sv.iv[ient];
             else st = SM_SEEDX;
         } else if (++t < ns) {
-            piv = make_uint4(s_lo[cur][t][tid], s_hi[cur][t][tid], s_rl[cur][t][tid], 0u);
+            const uint32_t w = (head + t) & RM;
+            piv = make_uint4(s_lo[w][tid], s_hi[w][tid], s_rl[w][tid], 0u);
         } else if (nn == 0) {
             st = st == SM_LEFT ? SM_ENDLEFT : SM_EMIT;
         } else {
-            cur ^= 1u; ns = nn; nn = 0; t = 0;
+            head = (head + ns) & RM; ns = nn; nn = 0; t = 0;
             bool more;
             if (st == SM_LEFT) { k--; more = k > 0; if (!more) st = SM_ENDLEFT; }
             else { e++; more = e < m; if (!more) st = SM_EMIT; }
-            if (more) piv = make_uint4(s_lo[cur][0][tid], s_hi[cur][0][tid], s_rl[cur][0][tid], 0u);
+            if (more) piv = make_uint4(s_lo[head][tid], s_hi[head][tid], s_rl[head][tid], 0u);
         }
     }
     if (restarts) atomicAdd(&ctr->diag[3], (unsigned long long)restarts);
@@ -850,15 +949,6 @@ static int build_seed_table(hsfm_index *idx, uint32_t F) {
 }
 } // namespace hsfm
 
-// Workgroups of a persistent kernel: as many as stay resident on every CU.
-template <typename K>
-static uint32_t persist_grid(hsfm_index *idx, K kernel, uint32_t block) {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)block, 0) != hipSuccess || per_cu < 1)
-        per_cu = 1;
-    return (uint32_t)per_cu * (uint32_t)idx->dev->num_cus;
-}
-
 static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *d_qoff, uint32_t nq, uint32_t min_len,
                            const uint32_t *d_refb, uint32_t n_refs, uint32_t max_hits, hsfm_mem_device_result *out) {
     Workspace &ws = idx->ws;
@@ -885,6 +975,7 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     const bool use_thread = nq > 0 && min_len >= F && !(path && strcmp(path, "wave") == 0);
     const uint32_t *list = nullptr; // identity list
     uint64_t nlist = nq;
+    SeedView sv{nullptr, nullptr, nullptr, 0};
     uint32_t *retry = ws.ovf_list.as<uint32_t>();
     if (use_thread) {
         if ((rc = build_seed_table(idx, F))) return rc;
@@ -893,7 +984,7 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         const uint64_t nslots = (total_len >> 3) + 8ull * nq;
         if ((rc = ws.mem_slots.ensure(nslots * 16 + 16))) return rc;
         if ((rc = ws.mem_status.ensure((size_t)nq + 16))) return rc;
-        SeedView sv{idx->seed_off.as<uint64_t>(), idx->seed_iv.as<uint4>(), idx->seed_lsl.as<uint8_t>(), F};
+        sv = SeedView{idx->seed_off.as<uint64_t>(), idx->seed_iv.as<uint4>(), idx->seed_lsl.as<uint8_t>(), F};
         const bool old_thread = path && strcmp(path, "thread") == 0;
         uint64_t hd[16];
         if ((rc = event_record(idx, 0, false))) return rc;
@@ -903,14 +994,15 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
                                ws.mem_cnt.as<uint32_t>(), ws.mem_status.as<uint8_t>(), ws.ovf_list.as<uint32_t>(),
                                ctr);
         } else {
-            const uint32_t g1 = persist_grid(idx, k_mem_persist<8, 256>, 256);
-            hipLaunchKernelGGL((k_mem_persist<8, 256>), dim3(g1), dim3(256), 0, s, idx->fwd.view, idx->rev.view,
+            const uint32_t g1 = persist_grid(idx, k_mem_persist<16, 256>, 256);
+            hipLaunchKernelGGL((k_mem_persist<16, 256>), dim3(g1), dim3(256), 0, s, idx->fwd.view, idx->rev.view,
                                idx->d_const.as<DevConst>(), sv, d_q, d_qoff, (const uint32_t *)nullptr, nq, min_len,
                                ws.mem_slots.as<uint4>(), ws.mem_cnt.as<uint32_t>(), ws.mem_status.as<uint8_t>(),
                                ws.ovf_list.as<uint32_t>(), ctr);
         }
         HSFM_HIP(hipGetLastError());
         if ((rc = event_record(idx, 0, true))) return rc;
+        if ((rc = event_record(idx, 2, false))) return rc;
         if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), hd, 16))) return rc;
         nlist = hd[2];
         idx->stats.occ_probes = hd[4];
@@ -918,37 +1010,24 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         list = ws.ovf_list.as<uint32_t>();
         retry = ws.read_list.as<uint32_t>();
         idx->stats.fallback_reads = nlist;
-        if (nlist && !old_thread) {
-            // second level: 32 intervals per lane for the deferred reads
-            if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_retry), 2, 0))) return rc;
-            if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->pwork), 2, 0))) return rc;
-            const uint32_t g2 = persist_grid(idx, k_mem_persist<32, 64>, 64);
-            const uint32_t need = (uint32_t)((nlist + 63) / 64);
-            hipLaunchKernelGGL((k_mem_persist<32, 64>), dim3(need < g2 ? need : g2), dim3(64), 0, s, idx->fwd.view,
-                               idx->rev.view, idx->d_const.as<DevConst>(), sv, d_q, d_qoff, list, (uint32_t)nlist,
-                               min_len, ws.mem_slots.as<uint4>(), ws.mem_cnt.as<uint32_t>(),
-                               ws.mem_status.as<uint8_t>(), retry, ctr);
-            HSFM_HIP(hipGetLastError());
-            if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), hd, 16))) return rc;
-            nlist = hd[2];
-            idx->stats.diag[3] = hd[8];
-            list = retry; // reads that outgrew 32 intervals go to the wave kernel
-            retry = ws.ovf_list.as<uint32_t>();
-            idx->stats.capacity_retries++;
-        }
     }
 
     uint64_t rec_cap = nlist * 8 + 4096;
     if (ws.mem_recs.cap / sizeof(MemRec) > rec_cap) rec_cap = ws.mem_recs.cap / sizeof(MemRec);
-    uint32_t cap = 1024;
+    // First pass: sets in LDS (kWaveLdsCap intervals); reads that outgrow them
+    // re-run with global-memory sets of 4x the capacity per pass.
+    uint32_t cap = kWaveLdsCap;
     uint32_t pass = 0;
     uint64_t rec_used = 0;
+    const uint32_t lds_waves = persist_grid(idx, k_mem_wave<true>, 64);
     while (nlist) {
-        uint64_t waves = nlist < (uint64_t)idx->dev->num_cus * 8 ? nlist : (uint64_t)idx->dev->num_cus * 8;
+        const bool lds = cap == kWaveLdsCap;
+        uint64_t waves = lds ? lds_waves : (uint64_t)idx->dev->num_cus * 8;
+        if (waves > nlist) waves = nlist;
         const uint64_t wave_limit = (4ull << 30) / (2ull * cap * 16);
         if (waves > wave_limit) waves = wave_limit;
         if (waves == 0) waves = 1;
-        if ((rc = ws.wave_sets.ensure((size_t)waves * 2 * cap * 16))) return rc;
+        if (!lds && (rc = ws.wave_sets.ensure((size_t)waves * 2 * cap * 16))) return rc;
         if (ws.mem_recs.cap < rec_cap * sizeof(MemRec)) {
             DevBuf grown;
             if ((rc = grown.ensure(rec_cap * sizeof(MemRec)))) return rc;
@@ -962,10 +1041,16 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_retry), 2, 0))) return rc;
         const bool timed = !use_thread && pass == 0;
         if (timed && (rc = event_record(idx, 0, false))) return rc;
-        hipLaunchKernelGGL(k_mem_wave, dim3((uint32_t)waves), dim3(64), 0, s, idx->fwd.view, idx->rev.view,
-                           idx->d_const.as<DevConst>(), d_q, d_qoff, list, (uint32_t)nlist, min_len, cap,
-                           ws.wave_sets.as<uint4>(), ws.mem_recs.as<MemRec>(), rec_cap, ws.mem_cnt.as<uint32_t>(),
-                           ws.mem_flags.as<uint32_t>(), pass, retry, ctr);
+        if (lds)
+            hipLaunchKernelGGL(k_mem_wave<true>, dim3((uint32_t)waves), dim3(64), 0, s, idx->fwd.view, idx->rev.view,
+                               idx->d_const.as<DevConst>(), sv, d_q, d_qoff, list, (uint32_t)nlist, min_len, cap,
+                               (uint4 *)nullptr, ws.mem_recs.as<MemRec>(), rec_cap, ws.mem_cnt.as<uint32_t>(),
+                               ws.mem_flags.as<uint32_t>(), pass, retry, ctr);
+        else
+            hipLaunchKernelGGL(k_mem_wave<false>, dim3((uint32_t)waves), dim3(64), 0, s, idx->fwd.view,
+                               idx->rev.view, idx->d_const.as<DevConst>(), sv, d_q, d_qoff, list, (uint32_t)nlist,
+                               min_len, cap, ws.wave_sets.as<uint4>(), ws.mem_recs.as<MemRec>(), rec_cap,
+                               ws.mem_cnt.as<uint32_t>(), ws.mem_flags.as<uint32_t>(), pass, retry, ctr);
         HSFM_HIP(hipGetLastError());
         if (timed && (rc = event_record(idx, 0, true))) return rc;
         if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
@@ -974,7 +1059,7 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         if (nretry == 0) break;
         idx->stats.capacity_retries++;
         if (cap >= (1u << 26)) return set_error(HSFM_ENOMEM, "interval set of a single read exceeds 2^26 intervals");
-        cap *= 4;
+        cap = lds ? 4096 : cap * 4;
         const uint64_t want = rec_used + nretry * 4096;
         rec_cap = rec_cap * 2 > want ? rec_cap * 2 : want;
         // next pass runs only the retried reads
@@ -985,6 +1070,7 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         pass++;
     }
     if (!use_thread) idx->stats.fallback_reads = 0;
+    if (idx->stats.fallback_reads && (rc = event_record(idx, 2, true))) return rc;
     if ((rc = exclusive_scan_u32_u64(idx, ws.mem_cnt.as<uint32_t>(), ws.mem_off.as<uint64_t>(), nq))) return rc;
     uint64_t nhits = 0;
     if ((rc = readback_u64(idx, ws.mem_off.as<uint64_t>() + nq, &nhits, 1))) return rc;
@@ -1031,6 +1117,7 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     }
     idx->stats.main_kernel_ms = event_ms(idx, 0);
     idx->stats.resolve_kernel_ms = npos ? event_ms(idx, 1) : 0.0;
+    idx->stats.fallback_kernel_ms = idx->stats.fallback_reads ? event_ms(idx, 2) : 0.0;
     out->num_queries = nq;
     out->num_hits = nhits;
     out->num_positions = npos;

@@ -179,6 +179,8 @@ typedef struct {
     uint64_t occ_probes;       /* 64 B records the main kernel's algorithm fetched */
     uint64_t diag[4];          /* MEM thread path: deferrals by seed union, by
                                   extension, by record slots; plain restarts      */
+    double fallback_kernel_ms; /* kernels that re-ran the reads the main kernel
+                                  deferred (larger sets)                          */
 } hsfm_stats;
 int hsfm_last_stats(hsfm_index *idx, hsfm_stats *out);
 

@@ -204,8 +204,17 @@ def test_persistent_kernel_deferral_levels():
     qs = _mem_reads(rng, text, 200, 120, 0.01, 0.0)
     qs += [unit[i:i + 100].copy() for i in range(0, 300, 7)]
     check(o, g, qs, 13)
-    st = g.last_stats()
-    assert st["fallback_reads"] > 0 and st["capacity_retries"] > 0
+    assert g.last_stats()["fallback_reads"] > 0
+
+
+def test_wave_sets_beyond_lds_capacity():
+    """All-N queries over an IUPAC-rich text: the interval sets exceed the
+    1024-interval LDS sets and re-run with global-memory sets."""
+    rng = np.random.default_rng(8)
+    o, g = _pair([O.decode(random_text(rng, 20000, iupac_frac=0.3))], sample_rate=16)
+    qs = [np.full(10, 5, np.uint8), np.concatenate([random_text(rng, 20), np.full(6, 5, np.uint8)])]
+    check(o, g, qs, 3, modes=(0,))
+    assert g.last_stats()["capacity_retries"] > 0
 
 
 def test_locate_then_mem_on_one_bidir_index():
