@@ -492,7 +492,7 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
     if (nq) {
         const char *path = getenv("HSFM_LOCATE_PATH");
         if ((rc = event_record(idx, 0, false))) return rc;
-        if (path && strcmp(path, "thread") == 0) {
+        if (!(path && strcmp(path, "persist") == 0)) {
             hipLaunchKernelGGL(k_locate_search, dim3((nq + 255) / 256), dim3(256), 0, s, d,
                                idx->d_const.as<DevConst>(), d_q, d_qoff, nq, ws.cnt.as<uint32_t>(),
                                ws.niv.as<uint8_t>(), ws.ivs.as<uint2>(), ws.ovf_list.as<uint32_t>(), ctr);
