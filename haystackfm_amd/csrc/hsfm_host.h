@@ -83,6 +83,9 @@ int fill_u32(hsfm_index *idx, uint32_t *d, uint64_t n, uint32_t v);
 // HIP events bracketing the main (slot 0) and resolve (slot 1) kernels.
 int event_record(hsfm_index *idx, int slot, bool end);
 double event_ms(hsfm_index *idx, int slot);
+// Table of the bidirectional intervals of every ACGT F-mer's compatible strings
+// (kept on the index until it is freed).
+int build_seed_table(hsfm_index *idx, uint32_t F);
 
 // Workgroups of a persistent kernel: as many as stay resident on every CU.
 template <typename K>

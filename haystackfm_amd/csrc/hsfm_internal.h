@@ -135,27 +135,37 @@ __device__ __forceinline__ void lane_counts(const RecView &r, uint32_t x, uint32
     }
 }
 
-// Counts of every lane in [0, xlo) and [0, xhi) (xlo <= xhi <= n) from the two
-// records and the superblock row of xlo only; the rare pair that spans two
-// superblocks reloads the row of xhi.  Lanes the index does not use count 0
-// because no position holds their code.
-__device__ __forceinline__ void pair_counts(const DevDir &d, uint32_t xlo, uint32_t xhi, uint32_t (&cl)[16],
-                                            uint32_t (&ch)[16]) {
+// The loads of one probe point pair: the records of xlo and xhi and the
+// superblock row of xlo (issued by pair_issue, consumed by pair_eval, so a
+// caller can overlap them with other work).  The fields double as generic
+// load slots for the small control-state loads of the pipelined kernels.
+struct PairRaw {
+    uint4 l0, l1, l2, l3, h0, h1, h2, h3, s0, s1, s2, s3;
+};
+__device__ __forceinline__ void pair_issue(const DevDir &d, uint32_t xlo, uint32_t xhi, PairRaw &w) {
     const uint32_t bl = xlo >> kRecShift, bh = xhi >> kRecShift;
     const uint4 *rpl = d.occ + ((size_t)bl << 2);
     const uint4 *rph = d.occ + ((size_t)bh << 2);
     const uint4 *spl = reinterpret_cast<const uint4 *>(d.sb + ((size_t)(bl >> kSbRecShift) << 4));
-    const uint4 la = rpl[0], lb = rpl[1], lc = rpl[2], ld = rpl[3];
-    const uint4 ha = rph[0], hb = rph[1], hc = rph[2], hd = rph[3];
-    const u32x4 s0 = ld4(spl), s1 = ld4(spl + 1), s2 = ld4(spl + 2), s3 = ld4(spl + 3);
-    const uint64_t lp0 = (uint64_t)la.x | ((uint64_t)la.y << 32), lp1 = (uint64_t)la.z | ((uint64_t)la.w << 32);
-    const uint64_t lp2 = (uint64_t)lb.x | ((uint64_t)lb.y << 32), lp3 = (uint64_t)lb.z | ((uint64_t)lb.w << 32);
-    const uint64_t hp0 = (uint64_t)ha.x | ((uint64_t)ha.y << 32), hp1 = (uint64_t)ha.z | ((uint64_t)ha.w << 32);
-    const uint64_t hp2 = (uint64_t)hb.x | ((uint64_t)hb.y << 32), hp3 = (uint64_t)hb.z | ((uint64_t)hb.w << 32);
-    const uint32_t lcw[8] = {lc.x, lc.y, lc.z, lc.w, ld.x, ld.y, ld.z, ld.w};
-    const uint32_t hcw[8] = {hc.x, hc.y, hc.z, hc.w, hd.x, hd.y, hd.z, hd.w};
-    const uint32_t sbw[16] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w,
-                              s2.x, s2.y, s2.z, s2.w, s3.x, s3.y, s3.z, s3.w};
+    w.l0 = rpl[0]; w.l1 = rpl[1]; w.l2 = rpl[2]; w.l3 = rpl[3];
+    w.h0 = rph[0]; w.h1 = rph[1]; w.h2 = rph[2]; w.h3 = rph[3];
+    w.s0 = spl[0]; w.s1 = spl[1]; w.s2 = spl[2]; w.s3 = spl[3];
+}
+// Counts of every lane in [0, xlo) and [0, xhi) (xlo <= xhi <= n) from the two
+// records and the superblock row of xlo only; the rare pair that spans two
+// superblocks reloads the row of xhi.  Lanes the index does not use count 0
+// because no position holds their code.
+__device__ __forceinline__ void pair_eval(const DevDir &d, const PairRaw &w, uint32_t xlo, uint32_t xhi,
+                                          uint32_t (&cl)[16], uint32_t (&ch)[16]) {
+    const uint32_t bl = xlo >> kRecShift, bh = xhi >> kRecShift;
+    const uint64_t lp0 = (uint64_t)w.l0.x | ((uint64_t)w.l0.y << 32), lp1 = (uint64_t)w.l0.z | ((uint64_t)w.l0.w << 32);
+    const uint64_t lp2 = (uint64_t)w.l1.x | ((uint64_t)w.l1.y << 32), lp3 = (uint64_t)w.l1.z | ((uint64_t)w.l1.w << 32);
+    const uint64_t hp0 = (uint64_t)w.h0.x | ((uint64_t)w.h0.y << 32), hp1 = (uint64_t)w.h0.z | ((uint64_t)w.h0.w << 32);
+    const uint64_t hp2 = (uint64_t)w.h1.x | ((uint64_t)w.h1.y << 32), hp3 = (uint64_t)w.h1.z | ((uint64_t)w.h1.w << 32);
+    const uint32_t lcw[8] = {w.l2.x, w.l2.y, w.l2.z, w.l2.w, w.l3.x, w.l3.y, w.l3.z, w.l3.w};
+    const uint32_t hcw[8] = {w.h2.x, w.h2.y, w.h2.z, w.h2.w, w.h3.x, w.h3.y, w.h3.z, w.h3.w};
+    const uint32_t sbw[16] = {w.s0.x, w.s0.y, w.s0.z, w.s0.w, w.s1.x, w.s1.y, w.s1.z, w.s1.w,
+                              w.s2.x, w.s2.y, w.s2.z, w.s2.w, w.s3.x, w.s3.y, w.s3.z, w.s3.w};
     const uint64_t lowl = (1ull << (xlo & 63)) - 1, lowh = (1ull << (xhi & 63)) - 1;
     const uint32_t sb16 = (bl >> kSbRecShift) << kSbShift;
     const uint32_t lcum16 = (bl & ((1u << kSbRecShift) - 1)) << kRecShift;
@@ -182,6 +192,62 @@ __device__ __forceinline__ void pair_counts(const DevDir &d, uint32_t xlo, uint3
         load_rec(d, xhi, rh);
         lane_counts(rh, xhi, 16, ch);
     }
+}
+// pair_eval for callers that only walk the lanes in order: the counts at xlo
+// first (16 values), then the counts at xhi one lane at a time, handed to
+// visit(k, count at xlo, count at xhi).  The superblock row of xhi replaces
+// that of xlo (reloaded, rare) between the two halves.
+template <typename V>
+__device__ __forceinline__ void pair_visit(const DevDir &d, PairRaw &w, uint32_t xlo, uint32_t xhi, V &&visit) {
+    const uint32_t bl = xlo >> kRecShift, bh = xhi >> kRecShift;
+    uint32_t cl[16];
+    {
+        const uint64_t p0 = (uint64_t)w.l0.x | ((uint64_t)w.l0.y << 32), p1 = (uint64_t)w.l0.z | ((uint64_t)w.l0.w << 32);
+        const uint64_t p2 = (uint64_t)w.l1.x | ((uint64_t)w.l1.y << 32), p3 = (uint64_t)w.l1.z | ((uint64_t)w.l1.w << 32);
+        const uint32_t cw[8] = {w.l2.x, w.l2.y, w.l2.z, w.l2.w, w.l3.x, w.l3.y, w.l3.z, w.l3.w};
+        const uint32_t sbw[16] = {w.s0.x, w.s0.y, w.s0.z, w.s0.w, w.s1.x, w.s1.y, w.s1.z, w.s1.w,
+                                  w.s2.x, w.s2.y, w.s2.z, w.s2.w, w.s3.x, w.s3.y, w.s3.z, w.s3.w};
+        const uint64_t low = (1ull << (xlo & 63)) - 1;
+        const uint32_t sb16 = (bl >> kSbRecShift) << kSbShift;
+        const uint32_t cum16 = (bl & ((1u << kSbRecShift) - 1)) << kRecShift;
+        uint32_t prev = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const uint64_t e = ((k & 1) ? p0 : ~p0) & ((k & 2) ? p1 : ~p1) & ((k & 4) ? p2 : ~p2) & ((k & 8) ? p3 : ~p3);
+            const uint32_t nx = (k == 15 ? sb16 : sbw[k + 1]) +
+                                (k == 15 ? cum16 : (cw[(k + 1) >> 1] >> (((k + 1) & 1) * 16)) & 0xffffu);
+            cl[k] = (nx - prev) + popc64(e & low);
+            prev = nx;
+        }
+    }
+    if ((bl >> kSbRecShift) != (bh >> kSbRecShift)) {
+        const uint4 *sph = reinterpret_cast<const uint4 *>(d.sb + ((size_t)(bh >> kSbRecShift) << 4));
+        w.s0 = sph[0]; w.s1 = sph[1]; w.s2 = sph[2]; w.s3 = sph[3];
+    }
+    const uint64_t p0 = (uint64_t)w.h0.x | ((uint64_t)w.h0.y << 32), p1 = (uint64_t)w.h0.z | ((uint64_t)w.h0.w << 32);
+    const uint64_t p2 = (uint64_t)w.h1.x | ((uint64_t)w.h1.y << 32), p3 = (uint64_t)w.h1.z | ((uint64_t)w.h1.w << 32);
+    const uint32_t cw[8] = {w.h2.x, w.h2.y, w.h2.z, w.h2.w, w.h3.x, w.h3.y, w.h3.z, w.h3.w};
+    const uint32_t sbw[16] = {w.s0.x, w.s0.y, w.s0.z, w.s0.w, w.s1.x, w.s1.y, w.s1.z, w.s1.w,
+                              w.s2.x, w.s2.y, w.s2.z, w.s2.w, w.s3.x, w.s3.y, w.s3.z, w.s3.w};
+    const uint64_t low = (1ull << (xhi & 63)) - 1;
+    const uint32_t sb16 = (bh >> kSbRecShift) << kSbShift;
+    const uint32_t cum16 = (bh & ((1u << kSbRecShift) - 1)) << kRecShift;
+    uint32_t prev = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const uint64_t e = ((k & 1) ? p0 : ~p0) & ((k & 2) ? p1 : ~p1) & ((k & 4) ? p2 : ~p2) & ((k & 8) ? p3 : ~p3);
+        const uint32_t nx = (k == 15 ? sb16 : sbw[k + 1]) +
+                            (k == 15 ? cum16 : (cw[(k + 1) >> 1] >> (((k + 1) & 1) * 16)) & 0xffffu);
+        visit(k, cl[k], (nx - prev) + popc64(e & low));
+        prev = nx;
+    }
+}
+
+__device__ __forceinline__ void pair_counts(const DevDir &d, uint32_t xlo, uint32_t xhi, uint32_t (&cl)[16],
+                                            uint32_t (&ch)[16]) {
+    PairRaw w;
+    pair_issue(d, xlo, xhi, w);
+    pair_eval(d, w, xlo, xhi, cl, ch);
 }
 
 // Lane of BWT row `row` and #same-lane rows in [0, row): the LF step of
@@ -245,6 +311,72 @@ template <bool LEFT>
 __device__ __forceinline__ uint4 bidir_child(const uint4 &iv, uint32_t clane, uint32_t a, uint32_t b, uint32_t acc) {
     return LEFT ? make_uint4(clane + a, clane + b, iv.z + acc, 0u)
                 : make_uint4(iv.x + acc, iv.x + acc + (b - a), clane + a, 0u);
+}
+
+// Seed table view (built by build_seed_table): per ACGT F-mer index, the
+// offsets of its interval list, the bidirectional intervals, and lsl.
+struct SeedView {
+    const uint64_t *off;
+    const uint4 *iv;
+    const uint8_t *lsl;
+    uint32_t F;
+};
+
+// A seed window q[j-F..j) as table indices: ACGT codes give fixed digits, up
+// to kSeedAmb ambiguity codes expand to every base they contain (the union of
+// the bases' compatible sets is the code's own).  pos holds 8 bits, mask 4
+// bits (the bases) per code; unused codes keep mask 1, digit 0.
+constexpr uint32_t kSeedAmb = 3;
+constexpr uint32_t kSeedCombos = 1u << (2 * kSeedAmb);
+struct SeedWin {
+    uint32_t base, pos, mask, namb;
+};
+__device__ __forceinline__ bool seed_window(const uint8_t *w, uint32_t F, const uint8_t *s_expand, SeedWin &sw,
+                                            uint32_t max_amb = kSeedAmb) {
+    sw.base = 0; sw.pos = 0; sw.mask = 0x111u; sw.namb = 0;
+    bool ok = true;
+    for (uint32_t u = 0; u < F; u++) {
+        const uint32_t x = s_expand[w[u]];
+        if (x & 0x10u) sw.base |= (x & 3u) << (2 * u);
+        else if (x && sw.namb < max_amb) {
+            sw.pos |= u << (8 * sw.namb);
+            sw.mask = (sw.mask & ~(0xfu << (4 * sw.namb))) | (x << (4 * sw.namb));
+            sw.namb++;
+        } else ok = false;
+    }
+    return ok;
+}
+// seed_window over the last F (<= 16) of 16 codes held one per register.
+__device__ __forceinline__ bool seed_window16(const uint32_t (&wb)[16], uint32_t F, const uint8_t *s_expand,
+                                              SeedWin &sw) {
+    sw.base = 0; sw.pos = 0; sw.mask = 0x111u; sw.namb = 0;
+    bool ok = true;
+#pragma unroll
+    for (uint32_t k = 0; k < 16; k++) {
+        if (k + F < 16) continue;
+        const uint32_t u = k + F - 16; // position in the window
+        const uint32_t x = s_expand[wb[k] & 0xffu];
+        if (x & 0x10u) sw.base |= (x & 3u) << (2 * u);
+        else if (x && sw.namb < kSeedAmb) {
+            sw.pos |= u << (8 * sw.namb);
+            sw.mask = (sw.mask & ~(0xfu << (4 * sw.namb))) | (x << (4 * sw.namb));
+            sw.namb++;
+        } else ok = false;
+    }
+    return ok;
+}
+// Table index of expansion xi (base digit i = bits 2i..2i+1 of xi); false if
+// a digit is not a base of its code.
+__device__ __forceinline__ bool seed_combo(const SeedWin &sw, uint32_t xi, uint32_t &idx) {
+    idx = sw.base;
+    bool ok = true;
+#pragma unroll
+    for (uint32_t i = 0; i < kSeedAmb; i++) {
+        const uint32_t d = (xi >> (2 * i)) & 3u;
+        ok &= ((sw.mask >> (4 * i + d)) & 1u) != 0;
+        if (i < sw.namb) idx |= d << (2 * ((sw.pos >> (8 * i)) & 0xffu));
+    }
+    return ok;
 }
 
 // wave64 helpers

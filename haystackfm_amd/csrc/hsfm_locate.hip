@@ -178,143 +178,332 @@ __device__ __forceinline__ uint32_t locate_read(const DevDir &d, const DevConst 
 }
 
 // ---------------------------------------------------------------------------
-// Persistent backward search: locate_read as a per-lane state machine.  Each
-// pass of the main loop extends one interval of one read by one code, so the
-// lanes of a wave stay on one code path; a lane whose read ends takes the next
-// read from a device work queue.  The interval set of a lane lives in a ring
-// of PR LDS slots (current set at [head, head+ns), children behind it).  A
-// final set of more than KR intervals is published, in row order, to fb_ivs;
-// reads whose set outgrows the ring go to k_locate_wave.
+// Pipelined persistent backward search.  Each lane runs locate_read as a state
+// machine.  Every pass of the loop issues, for each lane, one round of
+// independent loads -- one probe (two Occ records, a superblock row and the
+// next query code) or one control load (the read's offsets, its last 16
+// codes, a lookup entry, a seed-table entry) -- and then waits once, so a
+// wave spends one memory round trip per pass whatever mix of states its lanes
+// are in.  Control loads are parked in a per-lane LDS stage and consumed at the
+// start of the next pass.  Lanes take reads from a device work queue one read
+// ahead (the refill atomic is consumed a pass later); list, when given, names
+// the reads to run.  Interval sets live in a ring of PR LDS slots per lane;
+// final sets of more than KR intervals are published in row order to fb_ivs;
+// sets that outgrow the ring send the read to k_locate_wave.
+//
+// Reads that the lookup table cannot seed (an ambiguity code among the last
+// lookup_depth codes, or no lookup table) start from the seed table: the
+// union of its interval lists over the expansions of q[m-F..m) is the set of
+// every compatible string, which backward search from the full interval
+// reaches after F steps (query.rs:219-339 semantics; only the work differs).
 // ---------------------------------------------------------------------------
-enum : uint32_t { LS_FETCH = 0, LS_NEXT, LS_STEP, LS_DONE };
+enum : uint32_t {
+    LP_FETCH = 0, LP_HDR, LP_WIN, LP_LOOK, LP_SEEDX, LP_SEEDE, LP_STEP, LP_DONE,
+    LP_START, LP_FINISH, LP_SEEDNEXT // transitions without loads
+};
 
 template <int PR, int PB>
-__global__ __launch_bounds__(PB) void k_locate_persist(DevDir d, const DevConst *__restrict__ cst,
-                                                       const uint8_t *__restrict__ q,
-                                                       const uint64_t *__restrict__ qoff, uint32_t nq,
-                                                       uint32_t *__restrict__ cnt, uint8_t *__restrict__ niv,
-                                                       uint2 *__restrict__ ivs, uint2 *__restrict__ fb_meta,
-                                                       uint2 *__restrict__ fb_ivs, uint64_t fb_cap,
-                                                       uint32_t *__restrict__ ovf_list,
-                                                       LocateCounters *__restrict__ ctr) {
+__global__ __launch_bounds__(PB) void k_locate_pipe(DevDir d, const DevConst *__restrict__ cst, SeedView sv,
+                                                    const uint8_t *__restrict__ q, const uint64_t *__restrict__ qoff,
+                                                    const uint32_t *__restrict__ list, uint32_t nq,
+                                                    uint32_t *__restrict__ cnt, uint8_t *__restrict__ niv,
+                                                    uint2 *__restrict__ ivs, uint2 *__restrict__ fb_meta,
+                                                    uint2 *__restrict__ fb_ivs, uint64_t fb_cap,
+                                                    uint32_t *__restrict__ ovf_list, LocateCounters *__restrict__ ctr) {
     static_assert((PR & (PR - 1)) == 0, "ring size must be a power of two");
     constexpr uint32_t RM = PR - 1;
     __shared__ uint32_t s_lo[PR][PB], s_hi[PR][PB];
+    __shared__ uint32_t s_stage[5][PB]; // control loads of the last pass
     __shared__ uint16_t s_compat[256];
     __shared__ int8_t s_digit[256];
+    __shared__ uint8_t s_expand[256];
     __shared__ uint32_t s_clane[16];
     load_const_lds(cst, s_compat, s_digit, s_clane);
+    for (uint32_t x = threadIdx.x; x < 256; x += blockDim.x) s_expand[x] = cst->seed_expand[x];
     __syncthreads();
     const uint32_t tid = threadIdx.x;
+    const uint32_t lane = __lane_id();
     const uint32_t n = d.n;
     const uint32_t depth = cst->lookup_depth, radix = cst->lookup_radix;
-    uint32_t st = LS_FETCH, r = 0, i = 0, head = 0, ns = 0, nn = 0, t = 0, probes = 0;
+    const uint32_t F = sv.F;
+
+    uint32_t st = LP_FETCH, r = 0, nr = 0, m = 0, i = 0, head = 0, ns = 0, nn = 0, t = 0, probes = 0;
+    uint32_t aidx = 0, xi = 0;
+    uint64_t ient = 0, iend = 0;
     const uint8_t *p = q;
+    SeedWin sw{0, 0, 0x111u, 0};
     uint2 piv = make_uint2(0, 0);
+    {
+        uint32_t b = 0;
+        if (lane == 0) b = (uint32_t)atomicAdd(&ctr->pwork, 64ull);
+        nr = __shfl(b, 0, 64) + lane;
+    }
+    uint32_t abase = 0, arank = 0, aleader = 0;
+    bool pending = false, want = false;
 
     for (;;) {
-        while (st < LS_STEP) {
-            if (st == LS_FETCH) {
-                const uint64_t act = __ballot(1);
-                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)act) - 1u;
-                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
-                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
-                uint32_t base = 0;
-                if (__lane_id() == leader) base = (uint32_t)atomicAdd(&ctr->pwork, (unsigned long long)__popcll(act));
-                base = __shfl(base, (int)leader, 64);
-                r = base + rank;
-                if (r >= nq) { st = LS_DONE; break; }
-                const uint64_t b0 = qoff[r];
-                const uint32_t m = (uint32_t)(qoff[r + 1] - b0);
+        // (A) next-read refill issued last pass
+        if (pending) {
+            const uint32_t bb = __shfl(abase, (int)aleader, 64);
+            if (want) nr = bb + arank;
+            pending = false;
+        }
+        want = false;
+        // (B) consume the staged control loads; run load-free transitions
+        for (;;) {
+            if (st == LP_HDR) {
+                const uint64_t b0 = (uint64_t)s_stage[0][tid] | ((uint64_t)s_stage[1][tid] << 32);
+                const uint64_t b1 = (uint64_t)s_stage[2][tid] | ((uint64_t)s_stage[3][tid] << 32);
+                m = (uint32_t)(b1 - b0);
                 p = q + b0;
-                uint32_t lo = 0, hi = n;
-                i = m;
+                if (m == 0) { // the empty pattern matches every row
+                    s_lo[head][tid] = 0;
+                    s_hi[head][tid] = n;
+                    ns = 1;
+                    i = 0;
+                    st = LP_START;
+                    continue;
+                }
+                st = LP_WIN;
+                break;
+            } else if (st == LP_WIN) {
+                // last 16 codes, four per word: code k is p[m - 16 + k]
+                uint32_t wb[16];
+#pragma unroll
+                for (int k = 0; k < 16; k++) wb[k] = (s_stage[k >> 2][tid] >> (8 * (k & 3))) & 0xffu;
+                bool looked = false;
+                if (depth && m >= depth && d.lookup) {
+                    // lookup.rs:101-111: leftmost code is the least significant digit
+                    uint64_t idx = 0;
+                    bool ok = true;
+                    if (depth <= 16) {
+#pragma unroll
+                        for (int k = 15; k >= 0; k--)
+                            if ((uint32_t)k + depth >= 16) {
+                                const int dg = s_digit[wb[k]];
+                                ok &= dg >= 0;
+                                idx = idx * radix + (uint64_t)(dg < 0 ? 0 : dg);
+                            }
+                    } else {
+                        for (int k = (int)depth - 1; k >= 0; k--) {
+                            const int dg = s_digit[p[m - depth + k]];
+                            ok &= dg >= 0;
+                            idx = idx * radix + (uint64_t)(dg < 0 ? 0 : dg);
+                        }
+                    }
+                    if (ok) {
+                        aidx = (uint32_t)(idx < d.lookup_len ? idx : d.lookup_len - 1);
+                        st = LP_LOOK;
+                        looked = true;
+                    }
+                }
+                if (looked) break;
+                if (F && m >= F && seed_window16(wb, F, s_expand, sw)) {
+                    xi = 0;
+                    nn = 0;
+                    ns = 0;
+                    st = LP_SEEDNEXT;
+                    continue;
+                }
+                s_lo[head][tid] = 0;
+                s_hi[head][tid] = n;
                 ns = 1;
-                const int sd = lookup_seed(d, s_digit, depth, radix, p, m, lo, hi);
-                if (sd) probes++;
-                if (sd == 2) ns = 0;
-                else if (sd == 1) i = m - depth;
-                s_lo[head][tid] = lo;
-                s_hi[head][tid] = hi;
-                st = LS_NEXT;
-            } else { // LS_NEXT: extend the set by p[i-1], or finish the read
+                i = m;
+                st = LP_START;
+                continue;
+            } else if (st == LP_LOOK) {
+                probes++;
+                const uint32_t a0 = s_stage[0][tid], a1 = s_stage[1][tid];
+                if (a0 >= a1) {
+                    ns = 0;
+                } else {
+                    s_lo[head][tid] = a0 < n ? a0 : n;
+                    s_hi[head][tid] = a1 < n ? a1 : n;
+                    ns = 1;
+                }
+                i = m - depth;
+                st = LP_START;
+                continue;
+            } else if (st == LP_SEEDX) {
+                probes++;
+                const uint32_t l = s_stage[4][tid];
+                ient = (uint64_t)s_stage[0][tid] | ((uint64_t)s_stage[1][tid] << 32);
+                iend = (uint64_t)s_stage[2][tid] | ((uint64_t)s_stage[3][tid] << 32);
+                if (l == F && ient < iend) {
+                    st = LP_SEEDE;
+                    break;
+                }
+                st = LP_SEEDNEXT;
+                continue;
+            } else if (st == LP_SEEDE) {
+                probes++;
+                const uint32_t lo = s_stage[0][tid], hi = s_stage[1][tid];
+                bool dup = false;
+                for (uint32_t u = 0; u < nn; u++) dup |= s_lo[(head + u) & RM][tid] == lo;
+                if (!dup) {
+                    if (nn == PR) { // the seed union outgrows the ring
+                        niv[r] = kNivOverflow;
+                        cnt[r] = 0;
+                        ovf_list[atomicAdd(&ctr->n_ovf, 1ull)] = r;
+                        st = LP_FETCH;
+                        break;
+                    }
+                    s_lo[(head + nn) & RM][tid] = lo;
+                    s_hi[(head + nn) & RM][tid] = hi;
+                    nn++;
+                }
+                if (++ient < iend) break; // next entry (LP_SEEDE)
+                st = LP_SEEDNEXT;
+                continue;
+            } else if (st == LP_SEEDNEXT) {
+                uint32_t idx = 0;
+                while (xi < kSeedCombos && !seed_combo(sw, xi, idx)) xi++;
+                if (xi < kSeedCombos) {
+                    xi++;
+                    aidx = idx;
+                    st = LP_SEEDX;
+                    break;
+                }
+                ns = nn;
+                nn = 0;
+                i = m - F;
+                st = LP_START;
+                continue;
+            } else if (st == LP_START) {
                 if (ns != 0 && i != 0) {
                     t = 0;
                     nn = 0;
                     piv = make_uint2(s_lo[head][tid], s_hi[head][tid]);
-                    st = LS_STEP;
+                    st = LP_STEP;
                     break;
                 }
+                st = LP_FINISH;
+                continue;
+            } else if (st == LP_FINISH) {
                 // rows in ascending order: rank of each interval by its start
                 uint32_t total = 0;
                 for (uint32_t u = 0; u < ns; u++) total += s_hi[(head + u) & RM][tid] - s_lo[(head + u) & RM][tid];
-                if (ns <= KR) {
-                    for (uint32_t u = 0; u < ns; u++) {
-                        const uint32_t lo = s_lo[(head + u) & RM][tid];
-                        uint32_t rk = 0;
-                        for (uint32_t w = 0; w < ns; w++) rk += s_lo[(head + w) & RM][tid] < lo ? 1u : 0u;
-                        ivs[(size_t)r * KR + rk] = make_uint2(lo, s_hi[(head + u) & RM][tid]);
-                    }
-                    niv[r] = (uint8_t)ns;
-                    cnt[r] = total;
-                } else {
+                uint2 *dst = ivs + (size_t)r * KR;
+                bool keep = true;
+                if (ns > KR) {
                     const unsigned long long slot = atomicAdd(&ctr->fb_used, (unsigned long long)ns);
                     if (slot + ns > fb_cap) {
                         niv[r] = kNivOverflow;
                         cnt[r] = 0;
                         ovf_list[atomicAdd(&ctr->n_ovf, 1ull)] = r;
+                        keep = false;
                     } else {
-                        for (uint32_t u = 0; u < ns; u++) {
-                            const uint32_t lo = s_lo[(head + u) & RM][tid];
-                            uint32_t rk = 0;
-                            for (uint32_t w = 0; w < ns; w++) rk += s_lo[(head + w) & RM][tid] < lo ? 1u : 0u;
-                            fb_ivs[slot + rk] = make_uint2(lo, s_hi[(head + u) & RM][tid]);
-                        }
+                        dst = fb_ivs + slot;
                         fb_meta[r] = make_uint2((uint32_t)slot, ns);
-                        niv[r] = kNivOverflow;
-                        cnt[r] = total;
                     }
                 }
-                st = LS_FETCH;
-            }
-        }
-        if (__ballot(st != LS_DONE) == 0) break;
-        if (st == LS_DONE) continue;
-
-        // ---- probe: one interval extended by p[i-1] ----
-        const uint32_t cm = s_compat[p[i - 1]];
-        probes += 1u + ((piv.x >> kRecShift) != (piv.y >> kRecShift) ? 1u : 0u);
-        uint32_t cl[16], ch[16];
-        pair_counts(d, piv.x, piv.y, cl, ch);
-        const uint32_t cbase = head + ns;
-        const uint32_t room = PR - (ns - t - 1);
-        bool ovf = false;
-#pragma unroll
-        for (int k = 0; k < 16; k++) {
-            if (((cm >> k) & 1u) && ch[k] > cl[k]) {
-                if (nn + 1 > room) {
-                    ovf = true;
-                } else {
-                    const uint32_t w = (cbase + nn) & RM;
-                    s_lo[w][tid] = s_clane[k] + cl[k];
-                    s_hi[w][tid] = s_clane[k] + ch[k];
-                    nn++;
+                if (keep) {
+                    for (uint32_t u = 0; u < ns; u++) {
+                        const uint32_t lo = s_lo[(head + u) & RM][tid];
+                        uint32_t rk = 0;
+                        for (uint32_t v = 0; v < ns; v++) rk += s_lo[(head + v) & RM][tid] < lo ? 1u : 0u;
+                        dst[rk] = make_uint2(lo, s_hi[(head + u) & RM][tid]);
+                    }
+                    niv[r] = ns > KR ? kNivOverflow : (uint8_t)ns;
+                    cnt[r] = total;
                 }
+                st = LP_FETCH;
+                continue;
+            } else if (st == LP_FETCH) {
+                if (nr >= nq) {
+                    st = LP_DONE;
+                } else {
+                    r = list ? list[nr] : nr;
+                    want = true;
+                    st = LP_HDR;
+                }
+                break;
+            } else {
+                break; // LP_STEP, LP_DONE
             }
         }
-        if (ovf) {
-            niv[r] = kNivOverflow;
-            cnt[r] = 0;
-            ovf_list[atomicAdd(&ctr->n_ovf, 1ull)] = r;
-            st = LS_FETCH;
-        } else if (++t < ns) {
-            const uint32_t w = (head + t) & RM;
-            piv = make_uint2(s_lo[w][tid], s_hi[w][tid]);
-        } else {
-            head = (head + ns) & RM;
-            ns = nn;
-            i--;
-            st = LS_NEXT;
+        // (C) refill the prefetched read of lanes that just took theirs
+        const uint64_t act = __ballot(want);
+        if (act) {
+            aleader = (uint32_t)__ffsll((unsigned long long)act) - 1u;
+            arank = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+            if (lane == aleader) abase = (uint32_t)atomicAdd(&ctr->pwork, (unsigned long long)__popcll(act));
+            pending = true;
+        }
+        if (__ballot(st != LP_DONE) == 0) break;
+        // (D) issue this pass's loads: control loads or one probe, all into w
+        PairRaw w;
+        uint32_t qc = 0;
+        if (st == LP_HDR) {
+            const uint64_t b0 = qoff[r], b1 = qoff[r + 1];
+            w.l0 = make_uint4((uint32_t)b0, (uint32_t)(b0 >> 32), (uint32_t)b1, (uint32_t)(b1 >> 32));
+        } else if (st == LP_WIN) {
+            w.l0.x = m >= 16 ? p[m - 16] : 0u; w.l0.y = m >= 15 ? p[m - 15] : 0u;
+            w.l0.z = m >= 14 ? p[m - 14] : 0u; w.l0.w = m >= 13 ? p[m - 13] : 0u;
+            w.l1.x = m >= 12 ? p[m - 12] : 0u; w.l1.y = m >= 11 ? p[m - 11] : 0u;
+            w.l1.z = m >= 10 ? p[m - 10] : 0u; w.l1.w = m >= 9 ? p[m - 9] : 0u;
+            w.l2.x = m >= 8 ? p[m - 8] : 0u; w.l2.y = m >= 7 ? p[m - 7] : 0u;
+            w.l2.z = m >= 6 ? p[m - 6] : 0u; w.l2.w = m >= 5 ? p[m - 5] : 0u;
+            w.l3.x = m >= 4 ? p[m - 4] : 0u; w.l3.y = m >= 3 ? p[m - 3] : 0u;
+            w.l3.z = m >= 2 ? p[m - 2] : 0u; w.l3.w = p[m - 1];
+        } else if (st == LP_LOOK) {
+            const uint2 iv = d.lookup[aidx];
+            w.l0.x = iv.x;
+            w.l0.y = iv.y;
+        } else if (st == LP_SEEDX) {
+            const uint64_t o0 = sv.off[aidx], o1 = sv.off[aidx + 1];
+            w.l0 = make_uint4((uint32_t)o0, (uint32_t)(o0 >> 32), (uint32_t)o1, (uint32_t)(o1 >> 32));
+            w.l1.x = sv.lsl[aidx];
+        } else if (st == LP_SEEDE) {
+            w.l0 = sv.iv[ient];
+        } else if (st == LP_STEP) {
+            pair_issue(d, piv.x, piv.y, w);
+            qc = p[i - 1];
+            probes += 1u + ((piv.x >> kRecShift) != (piv.y >> kRecShift) ? 1u : 0u);
+        }
+        // (E) one wait: evaluate the probe, park the control loads
+        if (st == LP_STEP) {
+            const uint32_t cm = s_compat[qc];
+            const uint32_t cbase = head + ns;
+            const uint32_t room = PR - (ns - t - 1);
+            bool ovf = false;
+            pair_visit(d, w, piv.x, piv.y, [&](int k, uint32_t a, uint32_t b) {
+                if (((cm >> k) & 1u) && b > a) {
+                    if (nn + 1 > room) {
+                        ovf = true;
+                    } else {
+                        const uint32_t x = (cbase + nn) & RM;
+                        s_lo[x][tid] = s_clane[k] + a;
+                        s_hi[x][tid] = s_clane[k] + b;
+                        nn++;
+                    }
+                }
+            });
+            if (ovf) {
+                niv[r] = kNivOverflow;
+                cnt[r] = 0;
+                ovf_list[atomicAdd(&ctr->n_ovf, 1ull)] = r;
+                st = LP_FETCH;
+            } else if (++t < ns) {
+                const uint32_t x = (head + t) & RM;
+                piv = make_uint2(s_lo[x][tid], s_hi[x][tid]);
+            } else {
+                head = (head + ns) & RM;
+                ns = nn;
+                i--;
+                st = LP_START;
+            }
+        } else if (st == LP_WIN) {
+            s_stage[0][tid] = w.l0.x | (w.l0.y << 8) | (w.l0.z << 16) | (w.l0.w << 24);
+            s_stage[1][tid] = w.l1.x | (w.l1.y << 8) | (w.l1.z << 16) | (w.l1.w << 24);
+            s_stage[2][tid] = w.l2.x | (w.l2.y << 8) | (w.l2.z << 16) | (w.l2.w << 24);
+            s_stage[3][tid] = w.l3.x | (w.l3.y << 8) | (w.l3.z << 16) | (w.l3.w << 24);
+        } else if (st != LP_DONE) {
+            s_stage[0][tid] = w.l0.x;
+            s_stage[1][tid] = w.l0.y;
+            s_stage[2][tid] = w.l0.z;
+            s_stage[3][tid] = w.l0.w;
+            s_stage[4][tid] = w.l1.x;
         }
     }
 #pragma unroll
@@ -489,37 +678,61 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
     if ((rc = fill_u32(idx, ws.counters.as<uint32_t>(), sizeof(LocateCounters) / 4, 0))) return rc;
     // final sets of more than KR intervals are published to fb_ivs
     if ((rc = ws.fb_ivs.ensure(((size_t)nq * 2 + 65536) * 8))) return rc;
+    // 1) register kernel over every read (at most KR intervals per read)
+    // 2) the reads it gives up on: pipelined ring kernel, seed-table starts
+    // 3) what still outgrows the ring: one wave per read
+    const char *path = getenv("HSFM_LOCATE_PATH");
+    const bool pipe_all = path && strcmp(path, "pipe") == 0;
+    SeedView sv{nullptr, nullptr, nullptr, 0};
     if (nq) {
-        const char *path = getenv("HSFM_LOCATE_PATH");
+        const uint32_t F = 12;
+        if (idx->fwd.view.n > (1u << 20) && (rc = build_seed_table(idx, F))) return rc;
+        if (idx->seed_F)
+            sv = SeedView{idx->seed_off.as<uint64_t>(), idx->seed_iv.as<uint4>(), idx->seed_lsl.as<uint8_t>(),
+                          idx->seed_F};
+    }
+    const uint32_t pipe_grid = persist_grid(idx, k_locate_pipe<16, 256>, 256);
+    uint64_t h[8];
+    uint64_t nlist = nq;
+    const uint32_t *list = nullptr;
+    if (nq) {
         if ((rc = event_record(idx, 0, false))) return rc;
-        if (!(path && strcmp(path, "persist") == 0)) {
+        if (!pipe_all)
             hipLaunchKernelGGL(k_locate_search, dim3((nq + 255) / 256), dim3(256), 0, s, d,
                                idx->d_const.as<DevConst>(), d_q, d_qoff, nq, ws.cnt.as<uint32_t>(),
                                ws.niv.as<uint8_t>(), ws.ivs.as<uint2>(), ws.ovf_list.as<uint32_t>(), ctr);
-        } else {
-            const uint32_t g = persist_grid(idx, k_locate_persist<16, 256>, 256);
-            const uint32_t need = (nq + 255) / 256;
-            hipLaunchKernelGGL((k_locate_persist<16, 256>), dim3(need < g ? need : g), dim3(256), 0, s, d,
-                               idx->d_const.as<DevConst>(), d_q, d_qoff, nq, ws.cnt.as<uint32_t>(),
-                               ws.niv.as<uint8_t>(), ws.ivs.as<uint2>(), ws.fb_meta.as<uint2>(),
-                               ws.fb_ivs.as<uint2>(), (uint64_t)(ws.fb_ivs.cap / 8), ws.ovf_list.as<uint32_t>(), ctr);
-        }
         HSFM_HIP(hipGetLastError());
         if ((rc = event_record(idx, 0, true))) return rc;
         if ((rc = event_record(idx, 2, false))) return rc;
+        if (!pipe_all) {
+            if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
+            nlist = h[0];
+            list = ws.ovf_list.as<uint32_t>();
+        }
+        idx->stats.fallback_reads = pipe_all ? 0 : nlist;
+        if (nlist) {
+            if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_ovf), 2, 0))) return rc;
+            const uint32_t need = (uint32_t)((nlist + 255) / 256);
+            hipLaunchKernelGGL((k_locate_pipe<16, 256>), dim3(need < pipe_grid ? need : pipe_grid), dim3(256), 0, s,
+                               d, idx->d_const.as<DevConst>(), sv, d_q, d_qoff, list, (uint32_t)nlist,
+                               ws.cnt.as<uint32_t>(), ws.niv.as<uint8_t>(), ws.ivs.as<uint2>(),
+                               ws.fb_meta.as<uint2>(), ws.fb_ivs.as<uint2>(), (uint64_t)(ws.fb_ivs.cap / 8),
+                               ws.read_list.as<uint32_t>(), ctr);
+            HSFM_HIP(hipGetLastError());
+            if (pipe_all && (rc = event_record(idx, 0, true))) return rc;
+        }
     }
-    uint64_t h[8];
     if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
-    uint64_t nlist = h[0];
-    idx->stats.fallback_reads = nlist;
+    nlist = nq ? h[0] : 0; // reads for the wave kernel, listed in read_list
+    list = ws.read_list.as<uint32_t>();
     if (nlist) {
         if ((rc = ws.status.ensure(nlist * 4))) return rc;
-        HSFM_HIP(hipMemcpyAsync(ws.status.p, ws.ovf_list.p, nlist * 4, hipMemcpyDeviceToDevice, s));
+        HSFM_HIP(hipMemcpyAsync(ws.status.p, ws.read_list.p, nlist * 4, hipMemcpyDeviceToDevice, s));
     }
+    const uint64_t n_wave = nlist;
     uint32_t cap = 4096;
     uint64_t fb_cap = h[2] + nlist * 256 + 4096;
-    const uint32_t *list = ws.ovf_list.as<uint32_t>();
-    uint32_t *retry = ws.read_list.as<uint32_t>();
+    uint32_t *retry = ws.ovf_list.as<uint32_t>();
     while (nlist) {
         // per-wave set space is 2 * cap intervals; keep the total under 4 GiB
         uint64_t waves = nlist < (uint64_t)idx->dev->num_cus * 8 ? nlist : (uint64_t)idx->dev->num_cus * 8;
@@ -558,9 +771,9 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
         list = next_list;
         nlist = nretry;
     }
-    if (idx->stats.fallback_reads) {
+    if (n_wave) {
         // put every wave-path read's intervals in row order (FmIndex::locate order)
-        const uint32_t nfb = (uint32_t)idx->stats.fallback_reads;
+        const uint32_t nfb = (uint32_t)n_wave;
         uint64_t hh[8];
         if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), hh, 8))) return rc;
         const uint64_t used = hh[2];
@@ -585,8 +798,8 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
                                                              seg_b, seg_e, 0, 64, s));
         hipLaunchKernelGGL(k_fb_unkeys, dim3(kb ? kb : 1), dim3(256), 0, s, k_out, used, ws.fb_ivs.as<uint2>());
         HSFM_HIP(hipGetLastError());
-        if ((rc = event_record(idx, 2, true))) return rc;
     }
+    if (idx->stats.fallback_reads && (rc = event_record(idx, 2, true))) return rc;
     return HSFM_OK;
 }
 

@@ -42,63 +42,49 @@ struct MemRec { // 32 bytes
     uint32_t read, local, qs, qe, lo, hi, pass, pad;
 };
 
-// Seed table view (built by build_seed_table): per ACGT F-mer index, the
-// offsets of its interval list, the bidirectional intervals, and lsl.
-struct SeedView {
-    const uint64_t *off;
-    const uint4 *iv;
-    const uint8_t *lsl;
-    uint32_t F;
-};
-
-// A seed window q[j-F..j) as table indices: ACGT codes give fixed digits, up
-// to kSeedAmb ambiguity codes expand to every base they contain (the union of
-// the bases' compatible sets is the code's own).  pos holds 8 bits, mask 4
-// bits (the bases) per code; unused codes keep mask 1, digit 0.
-constexpr uint32_t kSeedAmb = 3;
-constexpr uint32_t kSeedCombos = 1u << (2 * kSeedAmb);
-struct SeedWin {
-    uint32_t base, pos, mask, namb;
-};
-__device__ __forceinline__ bool seed_window(const uint8_t *w, uint32_t F, const uint8_t *s_expand, SeedWin &sw,
-                                            uint32_t max_amb = kSeedAmb) {
-    sw.base = 0; sw.pos = 0; sw.mask = 0x111u; sw.namb = 0;
-    bool ok = true;
-    for (uint32_t u = 0; u < F; u++) {
-        const uint32_t x = s_expand[w[u]];
-        if (x & 0x10u) sw.base |= (x & 3u) << (2 * u);
-        else if (x && sw.namb < max_amb) {
-            sw.pos |= u << (8 * sw.namb);
-            sw.mask = (sw.mask & ~(0xfu << (4 * sw.namb))) | (x << (4 * sw.namb));
-            sw.namb++;
-        } else ok = false;
+// Inclusive prefix sum over the TB threads of a block (TB a multiple of 64);
+// *total receives the block sum.  s_part holds TB/64 + 1 words.
+template <int TB>
+__device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t *s_part, uint32_t &total) {
+    const uint32_t incl = wave_incl_scan(v);
+    if (TB == 64) {
+        total = __shfl(incl, 63, 64);
+        return incl;
     }
-    return ok;
+    const uint32_t wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) s_part[wv] = incl;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (uint32_t u = 0; u < (uint32_t)TB / 64; u++) {
+        const uint32_t x = s_part[u];
+        before += u < wv ? x : 0u;
+        all += x;
+    }
+    __syncthreads();
+    total = all;
+    return before + incl;
 }
-// Table index of expansion xi (base digit i = bits 2i..2i+1 of xi); false if
-// a digit is not a base of its code.
-__device__ __forceinline__ bool seed_combo(const SeedWin &sw, uint32_t xi, uint32_t &idx) {
-    idx = sw.base;
-    bool ok = true;
-#pragma unroll
-    for (uint32_t i = 0; i < kSeedAmb; i++) {
-        const uint32_t d = (xi >> (2 * i)) & 3u;
-        ok &= ((sw.mask >> (4 * i + d)) & 1u) != 0;
-        if (i < sw.namb) idx |= d << (2 * ((sw.pos >> (8 * i)) & 0xffu));
-    }
-    return ok;
+// Value of thread 0, for every thread of the block.
+template <int TB, typename T>
+__device__ __forceinline__ T block_bcast(T v, T *s_b) {
+    if (TB == 64) return __shfl(v, 0, 64);
+    if (threadIdx.x == 0) *s_b = v;
+    __syncthreads();
+    const T x = *s_b;
+    __syncthreads();
+    return x;
 }
 
-// Extend every interval of A (ns entries) by query code c.  LEFT: P -> cP on
-// the forward index; else P -> Pc on the reverse index.  Writes the non-empty
-// children to B and returns their number; sets too_big when cap is exceeded.
-template <bool LEFT>
-__device__ uint32_t wave_extend(const DevDir &D, uint32_t L, const uint32_t *s_clane, uint32_t cm,
-                                const uint4 *A, uint32_t ns, uint4 *B, uint32_t cap, bool &too_big) {
-    const uint32_t lane = __lane_id();
+// Extend every interval of A (ns entries) by query code c, TB intervals at a
+// time.  LEFT: P -> cP on the forward index; else P -> Pc on the reverse
+// index.  Writes the non-empty children to B and returns their number; sets
+// too_big when cap is exceeded.
+template <bool LEFT, int TB>
+__device__ uint32_t group_extend(const DevDir &D, const uint32_t *s_clane, uint32_t cm, const uint4 *A, uint32_t ns,
+                                 uint4 *B, uint32_t cap, bool &too_big, uint32_t *s_part) {
     uint32_t total = 0;
-    for (uint32_t base = 0; base < ns; base += 64) {
-        const uint32_t j = base + lane;
+    for (uint32_t base = 0; base < ns; base += TB) {
+        const uint32_t j = base + threadIdx.x;
         uint32_t cl[16], ch[16], cc = 0;
         uint4 iv = make_uint4(0, 0, 0, 0);
         if (j < ns) {
@@ -106,16 +92,12 @@ __device__ uint32_t wave_extend(const DevDir &D, uint32_t L, const uint32_t *s_c
             const uint32_t size = iv.y - iv.x;
             const uint32_t xlo = LEFT ? iv.x : iv.z;
             const uint32_t xhi = LEFT ? iv.y : iv.z + size;
-            RecView rl, rh;
-            load_rec(D, xlo, rl);
-            load_rec(D, xhi, rh);
-            lane_counts(rl, xlo, L, cl);
-            lane_counts(rh, xhi, L, ch);
+            pair_counts(D, xlo, xhi, cl, ch);
 #pragma unroll
             for (int k = 0; k < 16; k++) cc += (((cm >> k) & 1u) && ch[k] > cl[k]) ? 1u : 0u;
         }
-        const uint32_t incl = wave_incl_scan(cc);
-        const uint32_t tot = __shfl(incl, 63, 64);
+        uint32_t tot;
+        const uint32_t incl = block_incl_scan<TB>(cc, s_part, tot);
         if (total + tot > cap) {
             too_big = true;
             return total;
@@ -138,13 +120,14 @@ __device__ uint32_t wave_extend(const DevDir &D, uint32_t L, const uint32_t *s_c
     return total;
 }
 
-// One wave per read.  LDS: both sets in LDS (kWaveLdsCap intervals each);
-// otherwise in global memory, cap intervals each.  Restarts whose window
-// q[j-F..j) is all ACGT start from the seed table when min_len > F.
+// One workgroup of TB threads per read.  LDS: both sets in LDS (kWaveLdsCap
+// intervals each); otherwise in global memory, cap intervals each.  Restarts
+// take q[j-F..j) from the seed table when min_len > F (up to kSeedAmb
+// ambiguity codes expanded).
 constexpr uint32_t kWaveLdsCap = 1024;
 
-template <bool LDS>
-__global__ __launch_bounds__(64) void k_mem_wave(DevDir fwd, DevDir rev, const DevConst *__restrict__ cst,
+template <bool LDS, int TB>
+__global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const DevConst *__restrict__ cst,
                                                  SeedView sv, const uint8_t *__restrict__ q,
                                                  const uint64_t *__restrict__ qoff,
                                                  const uint32_t *__restrict__ list, uint32_t nlist, uint32_t min_len,
@@ -156,6 +139,9 @@ __global__ __launch_bounds__(64) void k_mem_wave(DevDir fwd, DevDir rev, const D
     __shared__ uint8_t s_expand[256];
     __shared__ uint32_t s_clane[16];
     __shared__ uint4 s_sets[LDS ? 2 * kWaveLdsCap : 1];
+    __shared__ uint32_t s_part[TB / 64 + 1];
+    __shared__ unsigned long long s_b64;
+    __shared__ uint32_t s_b32;
     for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) {
         s_compat[i] = cst->compat_lanes[i];
         s_expand[i] = cst->seed_expand[i];
@@ -163,7 +149,6 @@ __global__ __launch_bounds__(64) void k_mem_wave(DevDir fwd, DevDir rev, const D
     if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
     __syncthreads();
     const uint32_t lane = threadIdx.x;
-    const uint32_t L = cst->num_lanes;
     const uint32_t n = fwd.n;
     const uint32_t F = sv.F;
     if (LDS) cap = kWaveLdsCap;
@@ -172,7 +157,7 @@ __global__ __launch_bounds__(64) void k_mem_wave(DevDir fwd, DevDir rev, const D
     for (;;) {
         uint32_t wi = 0;
         if (lane == 0) wi = (uint32_t)atomicAdd(&ctr->work, 1ull);
-        wi = __shfl(wi, 0, 64);
+        wi = block_bcast<TB>(wi, &s_b32);
         if (wi >= nlist) break;
         const uint32_t r = list ? list[wi] : wi;
         const uint64_t b0 = qoff[r];
@@ -200,7 +185,7 @@ __global__ __launch_bounds__(64) void k_mem_wave(DevDir fwd, DevDir rev, const D
                         if (l != F) continue;
                         const uint64_t o0 = sv.off[idx], o1 = sv.off[idx + 1];
                         if (total + (o1 - o0) <= cap)
-                            for (uint32_t t = lane; t < (uint32_t)(o1 - o0); t += 64) A[total + t] = sv.iv[o0 + t];
+                            for (uint32_t t = lane; t < (uint32_t)(o1 - o0); t += TB) A[total + t] = sv.iv[o0 + t];
                         total += (uint32_t)(o1 - o0);
                     }
                     if (lslmax < F) { // q[j-lslmax-1..j) does not occur: L(j) = j - lslmax
@@ -215,7 +200,7 @@ __global__ __launch_bounds__(64) void k_mem_wave(DevDir fwd, DevDir rev, const D
                     ns = total;
                     if (namb) { // drop entries repeated across expansions (same fwd_lo)
                         uint32_t kept = 0;
-                        for (uint32_t base = 0; base < total; base += 64) {
+                        for (uint32_t base = 0; base < total; base += TB) {
                             const uint32_t t = base + lane;
                             bool keep = t < total;
                             uint4 iv = make_uint4(0, 0, 0, 0);
@@ -223,9 +208,10 @@ __global__ __launch_bounds__(64) void k_mem_wave(DevDir fwd, DevDir rev, const D
                                 iv = A[t];
                                 for (uint32_t u = 0; u < t && keep; u++) keep = A[u].x != iv.x;
                             }
-                            const uint32_t incl = wave_incl_scan(keep ? 1u : 0u);
+                            uint32_t tot;
+                            const uint32_t incl = block_incl_scan<TB>(keep ? 1u : 0u, s_part, tot);
                             if (keep) B[kept + incl - 1] = iv;
-                            kept += __shfl(incl, 63, 64);
+                            kept += tot;
                         }
                         __syncthreads();
                         uint4 *tmp = A; A = B; B = tmp;
@@ -238,7 +224,8 @@ __global__ __launch_bounds__(64) void k_mem_wave(DevDir fwd, DevDir rev, const D
             if (!seeded && lane == 0) A[0] = make_uint4(0, n, 0, 0);
             __syncthreads();
             while (k > 0) {
-                const uint32_t c2 = wave_extend<true>(fwd, L, s_clane, s_compat[p[k - 1]], A, ns, B, cap, too_big);
+                const uint32_t c2 = group_extend<true, TB>(fwd, s_clane, s_compat[p[k - 1]], A, ns, B, cap, too_big,
+                                                           s_part);
                 __syncthreads();
                 if (too_big || c2 == 0) break;
                 uint4 *t = A; A = B; B = t;
@@ -252,7 +239,8 @@ __global__ __launch_bounds__(64) void k_mem_wave(DevDir fwd, DevDir rev, const D
             }
             uint32_t e = j;
             while (e < m) {
-                const uint32_t c2 = wave_extend<false>(rev, L, s_clane, s_compat[p[e]], A, ns, B, cap, too_big);
+                const uint32_t c2 = group_extend<false, TB>(rev, s_clane, s_compat[p[e]], A, ns, B, cap, too_big,
+                                                            s_part);
                 __syncthreads();
                 if (too_big || c2 == 0) break;
                 uint4 *t = A; A = B; B = t;
@@ -263,12 +251,12 @@ __global__ __launch_bounds__(64) void k_mem_wave(DevDir fwd, DevDir rev, const D
             // emit (k, e): one record per interval, sorted by fwd_lo
             unsigned long long slot = 0;
             if (lane == 0) slot = atomicAdd(&ctr->rec_used, (unsigned long long)ns);
-            slot = __shfl(slot, 0, 64);
+            slot = block_bcast<TB>(slot, &s_b64);
             if (slot + ns > rec_cap) {
                 too_big = true;
                 break;
             }
-            for (uint32_t t = lane; t < ns; t += 64) {
+            for (uint32_t t = lane; t < ns; t += TB) {
                 const uint4 iv = A[t];
                 uint32_t rank = 0;
                 for (uint32_t u = 0; u < ns; u++) rank += A[u].x < iv.x ? 1u : 0u;
@@ -906,7 +894,7 @@ using namespace hsfm;
 
 namespace hsfm {
 // Build the seed table of all ACGT F-mers (k_seed_level, level by level).
-static int build_seed_table(hsfm_index *idx, uint32_t F) {
+int build_seed_table(hsfm_index *idx, uint32_t F) {
     if (idx->seed_F == F) return HSFM_OK;
     hipStream_t s = idx->dev->stream;
     DevBuf poff, piv, plsl;
@@ -1019,10 +1007,10 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     uint32_t cap = kWaveLdsCap;
     uint32_t pass = 0;
     uint64_t rec_used = 0;
-    const uint32_t lds_waves = persist_grid(idx, k_mem_wave<true>, 64);
+    const uint32_t lds_waves = persist_grid(idx, k_mem_wave<true, 64>, 64);
     while (nlist) {
         const bool lds = cap == kWaveLdsCap;
-        uint64_t waves = lds ? lds_waves : (uint64_t)idx->dev->num_cus * 8;
+        uint64_t waves = lds ? lds_waves : (uint64_t)idx->dev->num_cus * 2;
         if (waves > nlist) waves = nlist;
         const uint64_t wave_limit = (4ull << 30) / (2ull * cap * 16);
         if (waves > wave_limit) waves = wave_limit;
@@ -1042,12 +1030,13 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         const bool timed = !use_thread && pass == 0;
         if (timed && (rc = event_record(idx, 0, false))) return rc;
         if (lds)
-            hipLaunchKernelGGL(k_mem_wave<true>, dim3((uint32_t)waves), dim3(64), 0, s, idx->fwd.view, idx->rev.view,
+            hipLaunchKernelGGL((k_mem_wave<true, 64>), dim3((uint32_t)waves), dim3(64), 0, s, idx->fwd.view,
+                               idx->rev.view,
                                idx->d_const.as<DevConst>(), sv, d_q, d_qoff, list, (uint32_t)nlist, min_len, cap,
                                (uint4 *)nullptr, ws.mem_recs.as<MemRec>(), rec_cap, ws.mem_cnt.as<uint32_t>(),
                                ws.mem_flags.as<uint32_t>(), pass, retry, ctr);
         else
-            hipLaunchKernelGGL(k_mem_wave<false>, dim3((uint32_t)waves), dim3(64), 0, s, idx->fwd.view,
+            hipLaunchKernelGGL((k_mem_wave<false, 1024>), dim3((uint32_t)waves), dim3(1024), 0, s, idx->fwd.view,
                                idx->rev.view, idx->d_const.as<DevConst>(), sv, d_q, d_qoff, list, (uint32_t)nlist,
                                min_len, cap, ws.wave_sets.as<uint4>(), ws.mem_recs.as<MemRec>(), rec_cap,
                                ws.mem_cnt.as<uint32_t>(), ws.mem_flags.as<uint32_t>(), pass, retry, ctr);

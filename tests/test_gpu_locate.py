@@ -126,7 +126,7 @@ def test_lookup12_with_iupac_reference():
 
 def test_persistent_and_register_kernels_agree(monkeypatch):
     """The register kernel (default) and the persistent ring kernel
-    (HSFM_LOCATE_PATH=persist) return identical hits, including reads whose
+    (HSFM_LOCATE_PATH=pipe) return identical hits, including reads whose
     final set is published through the fallback interval list."""
     rng = np.random.default_rng(12)
     text, bounds = O.gen_reference(13, 300_000, 3, gc=0.41, repeat_frac=0.3, n_units=10, unit_min=300,
@@ -135,6 +135,6 @@ def test_persistent_and_register_kernels_agree(monkeypatch):
     g = hf.FmIndex.from_bytes(o.to_bytes())
     flat, off = O.gen_reads(14, text, bounds, 5000, 40, sub=0.01, both_strands=True)
     a = _check_batch(o, g, (flat, off))
-    monkeypatch.setenv("HSFM_LOCATE_PATH", "persist")
+    monkeypatch.setenv("HSFM_LOCATE_PATH", "pipe")
     b = g.locate_batch((flat, off))
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
