@@ -243,6 +243,44 @@ __device__ __forceinline__ void pair_visit(const DevDir &d, PairRaw &w, uint32_t
     }
 }
 
+// pair_visit with both points evaluated lane by lane in one loop (no per-lane
+// arrays); the pair that spans two superblocks takes the reload path.
+template <typename V>
+__device__ __forceinline__ void pair_visit_fused(const DevDir &d, PairRaw &w, uint32_t xlo, uint32_t xhi,
+                                                 V &&visit) {
+    const uint32_t bl = xlo >> kRecShift, bh = xhi >> kRecShift;
+    if ((bl >> kSbRecShift) != (bh >> kSbRecShift)) {
+        pair_visit(d, w, xlo, xhi, visit);
+        return;
+    }
+    const uint64_t lp0 = (uint64_t)w.l0.x | ((uint64_t)w.l0.y << 32), lp1 = (uint64_t)w.l0.z | ((uint64_t)w.l0.w << 32);
+    const uint64_t lp2 = (uint64_t)w.l1.x | ((uint64_t)w.l1.y << 32), lp3 = (uint64_t)w.l1.z | ((uint64_t)w.l1.w << 32);
+    const uint64_t hp0 = (uint64_t)w.h0.x | ((uint64_t)w.h0.y << 32), hp1 = (uint64_t)w.h0.z | ((uint64_t)w.h0.w << 32);
+    const uint64_t hp2 = (uint64_t)w.h1.x | ((uint64_t)w.h1.y << 32), hp3 = (uint64_t)w.h1.z | ((uint64_t)w.h1.w << 32);
+    const uint32_t lcw[8] = {w.l2.x, w.l2.y, w.l2.z, w.l2.w, w.l3.x, w.l3.y, w.l3.z, w.l3.w};
+    const uint32_t hcw[8] = {w.h2.x, w.h2.y, w.h2.z, w.h2.w, w.h3.x, w.h3.y, w.h3.z, w.h3.w};
+    const uint32_t sbw[16] = {w.s0.x, w.s0.y, w.s0.z, w.s0.w, w.s1.x, w.s1.y, w.s1.z, w.s1.w,
+                              w.s2.x, w.s2.y, w.s2.z, w.s2.w, w.s3.x, w.s3.y, w.s3.z, w.s3.w};
+    const uint64_t lowl = (1ull << (xlo & 63)) - 1, lowh = (1ull << (xhi & 63)) - 1;
+    const uint32_t sb16 = (bl >> kSbRecShift) << kSbShift;
+    const uint32_t lcum16 = (bl & ((1u << kSbRecShift) - 1)) << kRecShift;
+    const uint32_t hcum16 = (bh & ((1u << kSbRecShift) - 1)) << kRecShift;
+    uint32_t pl = 0, ph = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const uint64_t el = ((k & 1) ? lp0 : ~lp0) & ((k & 2) ? lp1 : ~lp1) & ((k & 4) ? lp2 : ~lp2) &
+                            ((k & 8) ? lp3 : ~lp3);
+        const uint64_t eh = ((k & 1) ? hp0 : ~hp0) & ((k & 2) ? hp1 : ~hp1) & ((k & 4) ? hp2 : ~hp2) &
+                            ((k & 8) ? hp3 : ~hp3);
+        const uint32_t sbn = k == 15 ? sb16 : sbw[k + 1];
+        const uint32_t nl = sbn + (k == 15 ? lcum16 : (lcw[(k + 1) >> 1] >> (((k + 1) & 1) * 16)) & 0xffffu);
+        const uint32_t nh = sbn + (k == 15 ? hcum16 : (hcw[(k + 1) >> 1] >> (((k + 1) & 1) * 16)) & 0xffffu);
+        visit(k, (nl - pl) + popc64(el & lowl), (nh - ph) + popc64(eh & lowh));
+        pl = nl;
+        ph = nh;
+    }
+}
+
 __device__ __forceinline__ void pair_counts(const DevDir &d, uint32_t xlo, uint32_t xhi, uint32_t (&cl)[16],
                                             uint32_t (&ch)[16]) {
     PairRaw w;

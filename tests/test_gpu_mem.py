@@ -171,7 +171,7 @@ def test_thread_path_seed_table(seed):
     check(o, g, qs, 12 + 3 * seed)
 
 
-@pytest.mark.parametrize("other", ["wave", "thread"])
+@pytest.mark.parametrize("other", ["wave", "thread", "pipe"])
 def test_persistent_and_other_paths_agree(monkeypatch, other):
     rng = np.random.default_rng(11)
     text, bounds = O.gen_reference(9, 200_000, 3, gc=0.41, repeat_frac=0.2, n_units=20, unit_min=300,
