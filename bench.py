@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cache", default=os.environ.get("HSFM_BENCH_CACHE", "/tmp/hsfm_bench_cache"))
+    ap.add_argument("--seed-keys", type=int, default=4_000_000,
+                    help="read 21-mers for the learned-seed vs plain-search leg (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 1 GiB device buffer 3 times (scales FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
@@ -177,6 +179,57 @@ def cpu_baseline(idx_bytes, sq, so, lq, lo, bounds, args, gpu_smem_sample, gpu_l
     return res
 
 
+def read_kmer_keys(flat, off, k, limit):
+    """Keys of the exact-ACGT k-mer windows inside reads (first `limit`)."""
+    import haystackfm_amd as hf
+    keys = hf.kmer_keys(flat, k)
+    start = np.arange(len(keys), dtype=np.int64)
+    rid = np.searchsorted(off, start, side="right") - 1
+    inside = start + k <= off[rid + 1]
+    keys = keys[inside & (keys != np.uint64(0xFFFFFFFFFFFFFFFF))]
+    if len(keys) > limit:
+        keys = keys[np.linspace(0, len(keys) - 1, limit).astype(np.int64)]
+    return keys
+
+
+def learned_seed_leg(bidir, flat, off, args):
+    """Learned seed index vs plain exact search on the same read k-mers: both
+    on the device, outputs compared bit for bit, kernel time by HIP events."""
+    import torch
+    import haystackfm_amd as hf
+    k = 21
+    keys = read_kmer_keys(flat, off, k, args.seed_keys)
+    t0 = time.time()
+    lsi = hf.LearnedSeedIndex(bidir, k=k, leaves=4096, hidden=16, train_steps=400, sample=1 << 16)
+    build_s = time.time() - t0
+    info = lsi.info()
+    dq = torch.from_numpy(keys.view(np.int64)).cuda()
+    d_plain = torch.empty((len(keys), 4), dtype=torch.int32, device="cuda")
+    d_lsi = torch.empty_like(d_plain)
+    hf.kmer_search_device(bidir, k, dq.data_ptr(), len(keys), d_plain.data_ptr())
+    lsi.seed_device(dq.data_ptr(), len(keys), d_lsi.data_ptr())
+    torch.cuda.synchronize()
+    plain_ms, lsi_ms, probes = [], [], 0
+    for _ in range(args.steps):
+        plain_ms.append(hf.kmer_search_device(bidir, k, dq.data_ptr(), len(keys), d_plain.data_ptr())["kernel_ms"])
+        st = lsi.seed_device(dq.data_ptr(), len(keys), d_lsi.data_ptr())
+        lsi_ms.append(st["kernel_ms"])
+        probes = st["probes"]
+    torch.cuda.synchronize()
+    exact = bool(torch.equal(d_plain, d_lsi))
+    pm, lm = float(np.mean(plain_ms)), float(np.mean(lsi_ms))
+    log(f"learned seed: {len(keys)} keys, plain {pm:.3f} ms, learned {lm:.3f} ms, exact {exact}, info {info}")
+    return {"k": k, "keys": int(len(keys)), "found": int(d_lsi[:, 3].sum().item()), "bit_exact": exact,
+            "plain_kernel_ms": round(pm, 4), "learned_kernel_ms": round(lm, 4),
+            "speedup": round(pm / lm, 3) if exact and lm > 0 else None,
+            "learned_probes_per_key": round(probes / max(1, len(keys)), 3),
+            "index_keys": int(info["num_keys"]), "leaves": 4096, "hidden": 16,
+            "max_err": int(info["max_err"]), "mean_window": round(info["mean_window"], 2),
+            "train_mse": [info["mse_initial"], info["mse_final"]],
+            "build_s": round(build_s, 2), "enum_ms": round(info["enum_ms"], 1),
+            "train_ms": round(info["train_ms"], 1), "fit_ms": round(info["fit_ms"], 1)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -252,6 +305,7 @@ def main():
                                    hf.HSFM_MODE_SMEM, bounds, args.max_hits)
     nls = min(args.check_reads * 5, nl)
     lsample = bidir.fwd.locate_batch((wl["lq"][:int(wl["lo"][nls])], wl["lo"][:nls + 1]))
+    lseed = learned_seed_leg(bidir, wl["sq"], wl["so"], args) if args.seed_keys else None
     cpu = cpu_baseline(idx_bytes, wl["sq"], wl["so"], wl["lq"], wl["lo"], bounds, args, sample, lsample)
 
     value = nq / smem_s
@@ -306,7 +360,8 @@ def main():
                    "reference_algorithm_probes_per_read": cpu.get("locate_search_probes_per_read")},
         "smem_stats": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in smem_stats.items()},
         "reference_algorithm_probes_per_read": cpu.get("smem_probes_per_read"),
-        "learned_seed_speedup": None,
+        "learned_seed_speedup": lseed["speedup"] if lseed else None,
+        "learned_seed": lseed,
     }
     print(json.dumps(out), flush=True)
 

@@ -31,6 +31,7 @@ __all__ = [
     "FmIndex", "BidirFmIndex", "MemHit", "GpuContext", "FmIndexError", "GpuError",
     "DeserializeError", "Alphabet", "AlphabetFns", "IupacDna", "ExactDna", "DnaSequence",
     "locate_batch_gpu", "context_cache", "encode", "alphabet_fns_from_tag", "lib_path",
+    "LearnedSeedIndex", "kmer_keys", "kmer_search", "kmer_search_device", "lsi_gradient",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -221,6 +222,21 @@ class _Stats(C.Structure):
                 ("fallback_kernel_ms", C.c_double)]
 
 
+class _LsiConfig(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("leaves", C.c_uint32), ("hidden", C.c_uint32), ("train_steps", C.c_uint32),
+                ("sample", C.c_uint32), ("lr", C.c_float)]
+
+
+class _LsiInfo(C.Structure):
+    _fields_ = [("num_keys", C.c_uint64), ("max_err", C.c_uint32), ("pad", C.c_uint32),
+                ("mean_window", C.c_double), ("mse_initial", C.c_double), ("mse_final", C.c_double),
+                ("enum_ms", C.c_double), ("train_ms", C.c_double), ("fit_ms", C.c_double)]
+
+
+class _SeedStats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("probes", C.c_uint64)]
+
+
 _lib_obj = None
 _lib_lock = threading.Lock()
 
@@ -260,6 +276,14 @@ def _lib():
                                             C.c_uint32, C.POINTER(_MemDev)]
         L.hsfm_free_result.argtypes = [vp]
         L.hsfm_last_stats.argtypes = [vp, C.POINTER(_Stats)]
+        L.hsfm_lsi_build.argtypes = [vp, C.POINTER(_LsiConfig), C.POINTER(vp)]
+        L.hsfm_lsi_get_info.argtypes = [vp, C.POINTER(_LsiInfo)]
+        L.hsfm_lsi_seed_device.argtypes = [vp, vp, C.c_uint64, vp, C.POINTER(_SeedStats)]
+        L.hsfm_kmer_search_device.argtypes = [vp, C.c_uint32, vp, C.c_uint64, vp, C.POINTER(_SeedStats)]
+        L.hsfm_lsi_seed.argtypes = [vp, vp, C.c_uint64, vp, C.POINTER(_SeedStats)]
+        L.hsfm_kmer_search.argtypes = [vp, C.c_uint32, vp, C.c_uint64, vp]
+        L.hsfm_lsi_grad.argtypes = [vp, vp, C.c_uint32, vp, vp, C.c_uint32, vp, vp]
+        L.hsfm_lsi_free.argtypes = [vp]
         _lib_obj = L
         return L
 
@@ -557,3 +581,93 @@ class BidirFmIndex:
                                             int(min_len), mode, C.c_void_p(d_ref_boundaries or None), n_refs,
                                             int(max_hits_per_mem), C.byref(r)), "find_mems_device")
         return int(r.num_hits), int(r.num_positions), int(r.d_hit_offsets or 0), int(r.d_hits or 0), int(r.d_positions or 0)
+
+
+# --------------------------------------------------------------------------
+# learned seed index
+# --------------------------------------------------------------------------
+def kmer_keys(codes, k: int) -> np.ndarray:
+    """Keys of every exact-ACGT k-mer window of codes (two bits per base, first
+    base most significant); windows with another code get key 2^64-1."""
+    c = np.asarray(codes, np.uint8)
+    m = len(c) - k + 1
+    if m <= 0:
+        return np.zeros(0, np.uint64)
+    ok = (c >= 1) & (c <= 4)
+    d = np.where(ok, c.astype(np.uint64) - 1, 0).astype(np.uint64)
+    keys = np.zeros(m, np.uint64)
+    for t in range(k):
+        keys = (keys << np.uint64(2)) | d[t:t + m]
+    bad = np.convolve(~ok, np.ones(k, int), "valid") > 0
+    keys[bad] = np.uint64(0xFFFFFFFFFFFFFFFF)
+    return keys
+
+
+class LearnedSeedIndex:
+    """Two-stage FP32 learned index over the exact-ACGT k-mers of an index.
+
+    seed(keys) returns, per key, (fwd_lo, fwd_hi, rev_lo, found): the interval
+    plain exact backward search gives for the k-mer (bidir.rs:112-130)."""
+
+    def __init__(self, index, k: int = 21, leaves: int = 4096, hidden: int = 16, train_steps: int = 400,
+                 sample: int = 1 << 16, lr: float = 0.01):
+        self.index = index
+        self.k = k
+        cfg = _LsiConfig(k, leaves, hidden, train_steps, sample, lr)
+        h = C.c_void_p()
+        _check(_lib().hsfm_lsi_build(index._handle(), C.byref(cfg), C.byref(h)), "lsi_build")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib_obj is not None:
+            _lib_obj.hsfm_lsi_free(h)
+            self._h = None
+
+    def info(self) -> dict:
+        st = _LsiInfo()
+        _check(_lib().hsfm_lsi_get_info(self._h, C.byref(st)), "lsi_info")
+        return {k: getattr(st, k) for k, _ in _LsiInfo._fields_ if k != "pad"}
+
+    def seed(self, keys, stats: dict | None = None) -> np.ndarray:
+        keys = np.ascontiguousarray(np.asarray(keys, np.uint64))
+        out = np.zeros((len(keys), 4), np.uint32)
+        st = _SeedStats()
+        _check(_lib().hsfm_lsi_seed(self._h, keys.ctypes.data, len(keys), out.ctypes.data, C.byref(st)), "lsi_seed")
+        if stats is not None:
+            stats.update(kernel_ms=st.kernel_ms, probes=st.probes)
+        return out
+
+    def seed_device(self, d_keys: int, n: int, d_out: int) -> dict:
+        st = _SeedStats()
+        _check(_lib().hsfm_lsi_seed_device(self._h, C.c_void_p(d_keys), n, C.c_void_p(d_out), C.byref(st)),
+               "lsi_seed_device")
+        return {"kernel_ms": st.kernel_ms, "probes": st.probes}
+
+
+def kmer_search(index, keys, k: int) -> np.ndarray:
+    """Plain exact search of k-mer keys on the device (the learned index's judge)."""
+    keys = np.ascontiguousarray(np.asarray(keys, np.uint64))
+    out = np.zeros((len(keys), 4), np.uint32)
+    _check(_lib().hsfm_kmer_search(index._handle(), k, keys.ctypes.data, len(keys), out.ctypes.data), "kmer_search")
+    return out
+
+
+def kmer_search_device(index, k: int, d_keys: int, n: int, d_out: int) -> dict:
+    st = _SeedStats()
+    _check(_lib().hsfm_kmer_search_device(index._handle(), k, C.c_void_p(d_keys), n, C.c_void_p(d_out),
+                                          C.byref(st)), "kmer_search_device")
+    return {"kernel_ms": st.kernel_ms}
+
+
+def lsi_gradient(params: np.ndarray, hidden: int, x: np.ndarray, y: np.ndarray, ctx: "GpuContext | None" = None):
+    """Stage-1 loss and gradient computed by the training kernel (for checks)."""
+    ctx = ctx or context_cache.get_or_init()
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    g = np.zeros(3 * hidden + 1, np.float32)
+    loss = np.zeros(1, np.float32)
+    _check(_lib().hsfm_lsi_grad(ctx._h, p.ctypes.data, hidden, x.ctypes.data, y.ctypes.data, len(x),
+                                g.ctypes.data, loss.ctypes.data), "lsi_grad")
+    return float(loss[0]), g

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhsfm.so")
 BUILD_LIB = os.path.join(HERE, "libhsfm_build.so")
-SOURCES = ["hsfm_runtime.hip", "hsfm_index.hip", "hsfm_locate.hip", "hsfm_mem.hip"]
+SOURCES = ["hsfm_runtime.hip", "hsfm_index.hip", "hsfm_locate.hip", "hsfm_mem.hip", "hsfm_lsi.hip"]
 ARCH = os.environ.get("HSFM_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -1,0 +1,734 @@
+// hsfm_lsi.hip -- learned seed index on gfx950.
+//
+// For an exact-ACGT k-mer W (k <= 31; 21 by default) the index answers the
+// bidirectional SA interval {fwd_lo, fwd_hi, rev_lo} that plain backward
+// search gives (extend_left over W from the full interval, bidir.rs:112-130,
+// the exact-code extension of LookupTable::build, lookup.rs:37-91), or "absent".
+//
+// Keys pack W two bits per base, first base most significant, so sorting the
+// keys sorts the k-mers in suffix-array order.  Build, all on the device:
+//   1. enumerate every present k-mer from the index itself, level by level:
+//      the node of cS is one exact left extension of the node of S; children
+//      are placed digit-major, which keeps each level sorted by key
+//   2. stage 1 (FP32 MLP, x -> f(x) ~ slot / N) trained by Adam on a sample
+//      of (key, slot) pairs with the forward/backward kernels below
+//   3. stage 2: every key routed to leaf floor(f(x) * L); per leaf a
+//      least-squares line slot ~ s0 + a * (key - kmin) + b (float64 sums,
+//      FP32 parameters) and the maximum error of its keys
+// Lookup: route, predict, then a binary search of the sorted keys restricted
+// to [pred - err, pred + err]; exact because err bounds every present key and
+// an absent key is absent from every window.
+#include <math.h>
+#include <string.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include "hsfm_host.h"
+
+namespace hsfm {
+
+constexpr uint32_t kLsiMaxHidden = 64;
+
+struct LsiLeaf {             // 32 bytes
+    unsigned long long kmin; // smallest key routed here
+    uint32_t s0;             // smallest slot routed here
+    uint32_t err;            // max |pred - slot| over the leaf's keys
+    float a, b;              // slot - s0 ~ a * (key - kmin) + b
+    uint32_t n, pad;
+};
+
+struct LsiModel {
+    const float *p; // stage 1: w1[H], b1[H], w2[H], b2
+    uint32_t hidden, leaves, k;
+    uint64_t nkeys;
+};
+
+// key / 4^k in [0, 1) in FP32 (the routing feature)
+__device__ __forceinline__ float lsi_feature(unsigned long long key, uint32_t k) {
+    const int sh = (int)(2 * k) - 24;
+    return sh > 0 ? (float)(uint32_t)(key >> sh) * 5.9604644775390625e-8f   // 2^-24
+                  : (float)(uint32_t)key * ldexpf(1.f, -(int)(2 * k));
+}
+__device__ __forceinline__ float lsi_stage1(const float *p, uint32_t H, float x) {
+    float f = p[3 * H];
+    for (uint32_t i = 0; i < H; i++) f = fmaf(p[2 * H + i], fmaxf(fmaf(p[i], x, p[H + i]), 0.f), f);
+    return f;
+}
+__device__ __forceinline__ uint32_t lsi_leaf(float f, uint32_t L) {
+    const float t = f * (float)L;
+    return !(t > 0.f) ? 0u : (t >= (float)(L - 1) ? L - 1 : (uint32_t)t); // NaN -> leaf 0
+}
+// predicted slot of key in its leaf (identical in the error pass and lookups)
+__device__ __forceinline__ long long lsi_pred(const LsiLeaf &lf, unsigned long long key) {
+    const float dx = (float)(key - lf.kmin);
+    return (long long)lf.s0 + (long long)floorf(fmaf(lf.a, dx, lf.b));
+}
+
+// ---- 1. enumeration ------------------------------------------------------
+// Exact lanes of A, C, G, T (lane_sym inverse); 0xff if the index lacks one.
+__device__ __forceinline__ void base_lanes(const DevConst *cst, uint32_t *s_bl) {
+    if (threadIdx.x < 4) {
+        uint32_t l = 0xffu;
+        for (uint32_t j = 0; j < cst->num_lanes; j++)
+            if (cst->lane_sym[j] == threadIdx.x + 1) l = j;
+        s_bl[threadIdx.x] = l;
+    }
+}
+
+// Children of every parent node (key of S, {fwd_lo, fwd_hi, rev_lo}): cS for
+// c in ACGT.  COUNT: flags[d * P + p] = 1 if child d exists.  Else: write the
+// child at off[d][p] + base[d] (digit-major: sorted by key).
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_lsi_expand(DevDir fwd, const DevConst *__restrict__ cst,
+                                                    const unsigned long long *__restrict__ pkey,
+                                                    const uint4 *__restrict__ piv, uint32_t P, uint32_t level,
+                                                    uint8_t *__restrict__ flags, const uint32_t *__restrict__ off,
+                                                    uint4 base, unsigned long long *__restrict__ ckey,
+                                                    uint4 *__restrict__ civ) {
+    __shared__ uint32_t s_bl[4], s_clane[16];
+    base_lanes(cst, s_bl);
+    if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
+    __syncthreads();
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const uint4 iv = piv[p];
+    uint32_t cl[16], ch[16];
+    pair_counts(fwd, iv.x, iv.y, cl, ch);
+    uint32_t below[17];
+    below[0] = 0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) below[j + 1] = below[j] + (ch[j] - cl[j]);
+    const unsigned long long pk = pkey[p];
+#pragma unroll
+    for (uint32_t d = 0; d < 4; d++) {
+        const uint32_t l = s_bl[d];
+        uint32_t a = 0, b = 0, acc = 0;
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            if ((uint32_t)j == l) { a = cl[j]; b = ch[j]; acc = below[j]; }
+        const bool ok = l < 16 && b > a;
+        if (!WRITE) {
+            flags[(size_t)d * P + p] = ok ? 1 : 0;
+        } else if (ok) {
+            uint32_t cs = 0;
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                if ((uint32_t)j == l) cs = s_clane[j];
+            const uint32_t bd = d == 0 ? base.x : d == 1 ? base.y : d == 2 ? base.z : base.w;
+            const size_t at = (size_t)bd + off[(size_t)d * P + p];
+            ckey[at] = ((unsigned long long)d << (2 * level)) | pk;
+            civ[at] = make_uint4(cs + a, cs + b, iv.z + acc, 0u);
+        }
+    }
+}
+
+struct U8ToU32 {
+    __device__ __forceinline__ uint32_t operator()(uint8_t v) const { return v; }
+};
+
+// ---- 2. stage-1 training -----------------------------------------------
+// Gradient of mean((f - y)^2) over n samples, accumulated into g[3H+1]
+// (zeroed by the caller); loss sum into *loss.
+__global__ __launch_bounds__(256) void k_lsi_grad(const float *__restrict__ p, uint32_t H,
+                                                  const float *__restrict__ x, const float *__restrict__ y,
+                                                  uint32_t n, float *__restrict__ g, float *__restrict__ loss) {
+    __shared__ float s_p[3 * kLsiMaxHidden + 1];
+    __shared__ float s_g[3 * kLsiMaxHidden + 2];
+    const uint32_t np = 3 * H + 1;
+    for (uint32_t i = threadIdx.x; i < np + 1; i += blockDim.x) {
+        if (i < np) s_p[i] = p[i];
+        s_g[i] = 0.f;
+    }
+    __syncthreads();
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = t < n;
+    const float xi = act ? x[t] : 0.f, yi = act ? y[t] : 0.f;
+    float f = s_p[3 * H];
+    for (uint32_t i = 0; i < H; i++) f = fmaf(s_p[2 * H + i], fmaxf(fmaf(s_p[i], xi, s_p[H + i]), 0.f), f);
+    const float e = act ? f - yi : 0.f;
+    const float df = 2.f * e / (float)n;
+    // per-parameter contributions, reduced over the wave then over the block
+    auto reduce_add = [&](float v, uint32_t slot) {
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&s_g[slot], v);
+    };
+    for (uint32_t i = 0; i < H; i++) {
+        const float pre = fmaf(s_p[i], xi, s_p[H + i]);
+        const float h = fmaxf(pre, 0.f);
+        const float dh = pre > 0.f ? df * s_p[2 * H + i] : 0.f;
+        reduce_add(dh * xi, i);          // w1
+        reduce_add(dh, H + i);           // b1
+        reduce_add(df * h, 2 * H + i);   // w2
+    }
+    reduce_add(df, 3 * H);               // b2
+    reduce_add(e * e, np);               // loss
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) atomicAdd(&g[i], s_g[i]);
+    if (threadIdx.x == 0) atomicAdd(loss, s_g[np]);
+}
+
+__global__ void k_lsi_adam(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
+                           const float *__restrict__ g, uint32_t np, float lr, float b1t, float b2t) {
+    const uint32_t i = threadIdx.x;
+    if (i >= np) return;
+    const float gi = g[i];
+    const float mi = 0.9f * m[i] + 0.1f * gi, vi = 0.999f * v[i] + 0.001f * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / (1.f - b1t)) / (sqrtf(vi / (1.f - b2t)) + 1e-8f);
+}
+
+// training sample: every stride-th slot
+__global__ void k_lsi_sample(const unsigned long long *__restrict__ keys, uint64_t nkeys, uint64_t stride,
+                             uint32_t ns, uint32_t k, float *__restrict__ x, float *__restrict__ y) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    const uint64_t s = (uint64_t)i * stride;
+    x[i] = lsi_feature(keys[s], k);
+    y[i] = (float)((double)s / (double)nkeys);
+}
+
+// ---- 3. stage 2 ----------------------------------------------------------
+constexpr uint32_t kLsiChunk = 64; // keys per thread in the leaf passes
+
+__global__ void k_lsi_leaf_init(LsiLeaf *lf, uint32_t L) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L) return;
+    LsiLeaf z;
+    z.kmin = ~0ull; z.s0 = 0xffffffffu; z.err = 0; z.a = 0.f; z.b = 0.f; z.n = 0; z.pad = 0;
+    lf[i] = z;
+}
+
+// route keys; per leaf min key, min slot, count (chunk-aggregated atomics)
+__global__ __launch_bounds__(256) void k_lsi_route(LsiModel md, const unsigned long long *__restrict__ keys,
+                                                   uint32_t *__restrict__ leaf_of, LsiLeaf *__restrict__ lf) {
+    __shared__ float s_p[3 * kLsiMaxHidden + 1];
+    for (uint32_t i = threadIdx.x; i < 3 * md.hidden + 1; i += blockDim.x) s_p[i] = md.p[i];
+    __syncthreads();
+    const uint64_t c0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kLsiChunk;
+    uint32_t cur = 0xffffffffu, cnt = 0, s0 = 0;
+    unsigned long long kmin = 0;
+    for (uint64_t s = c0; s < c0 + kLsiChunk && s < md.nkeys; s++) {
+        const unsigned long long key = keys[s];
+        const uint32_t l = lsi_leaf(lsi_stage1(s_p, md.hidden, lsi_feature(key, md.k)), md.leaves);
+        leaf_of[s] = l;
+        if (l != cur) {
+            if (cnt) {
+                atomicMin(&lf[cur].kmin, kmin);
+                atomicMin(&lf[cur].s0, s0);
+                atomicAdd(&lf[cur].n, cnt);
+            }
+            cur = l; cnt = 0; kmin = key; s0 = (uint32_t)s;
+        }
+        cnt++;
+        kmin = key < kmin ? key : kmin;
+    }
+    if (cnt) {
+        atomicMin(&lf[cur].kmin, kmin);
+        atomicMin(&lf[cur].s0, s0);
+        atomicAdd(&lf[cur].n, cnt);
+    }
+}
+
+// least-squares sums per leaf: n-weighted dx, y, dx^2, dx*y (float64)
+__global__ __launch_bounds__(256) void k_lsi_moments(uint64_t nkeys, const unsigned long long *__restrict__ keys,
+                                                     const uint32_t *__restrict__ leaf_of,
+                                                     const LsiLeaf *__restrict__ lf, double *__restrict__ mom) {
+    const uint64_t c0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kLsiChunk;
+    uint32_t cur = 0xffffffffu;
+    double sx = 0, sy = 0, sxx = 0, sxy = 0;
+    for (uint64_t s = c0; s < c0 + kLsiChunk && s < nkeys; s++) {
+        const uint32_t l = leaf_of[s];
+        if (l != cur) {
+            if (cur != 0xffffffffu) {
+                atomicAdd(&mom[4 * cur + 0], sx); atomicAdd(&mom[4 * cur + 1], sy);
+                atomicAdd(&mom[4 * cur + 2], sxx); atomicAdd(&mom[4 * cur + 3], sxy);
+            }
+            cur = l; sx = sy = sxx = sxy = 0;
+        }
+        const double dx = (double)(keys[s] - lf[l].kmin), dy = (double)(s - lf[l].s0);
+        sx += dx; sy += dy; sxx += dx * dx; sxy += dx * dy;
+    }
+    if (cur != 0xffffffffu) {
+        atomicAdd(&mom[4 * cur + 0], sx); atomicAdd(&mom[4 * cur + 1], sy);
+        atomicAdd(&mom[4 * cur + 2], sxx); atomicAdd(&mom[4 * cur + 3], sxy);
+    }
+}
+
+__global__ void k_lsi_solve(LsiLeaf *__restrict__ lf, const double *__restrict__ mom, uint32_t L) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L) return;
+    LsiLeaf f = lf[i];
+    if (f.n == 0) {
+        f.kmin = 0; f.s0 = 0; f.a = 0.f; f.b = 0.f;
+    } else {
+        const double n = f.n, sx = mom[4 * i], sy = mom[4 * i + 1], sxx = mom[4 * i + 2], sxy = mom[4 * i + 3];
+        const double den = n * sxx - sx * sx;
+        double a = 0, b = sy / n;
+        if (f.n >= 2 && den > 0) {
+            a = (n * sxy - sx * sy) / den;
+            b = (sy - a * sx) / n;
+        }
+        if (!isfinite(a) || !isfinite(b)) {
+            a = 0;
+            b = sy / n;
+        }
+        f.a = (float)a;
+        f.b = (float)b;
+    }
+    lf[i] = f;
+}
+
+__global__ __launch_bounds__(256) void k_lsi_err(uint64_t nkeys, const unsigned long long *__restrict__ keys,
+                                                 const uint32_t *__restrict__ leaf_of, LsiLeaf *__restrict__ lf) {
+    const uint64_t c0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kLsiChunk;
+    uint32_t cur = 0xffffffffu, e = 0;
+    for (uint64_t s = c0; s < c0 + kLsiChunk && s < nkeys; s++) {
+        const uint32_t l = leaf_of[s];
+        if (l != cur) {
+            if (cur != 0xffffffffu) atomicMax(&lf[cur].err, e);
+            cur = l; e = 0;
+        }
+        const long long d = lsi_pred(lf[l], keys[s]) - (long long)s;
+        const uint32_t ad = (uint32_t)(d < 0 ? -d : d);
+        e = ad > e ? ad : e;
+    }
+    if (cur != 0xffffffffu) atomicMax(&lf[cur].err, e);
+}
+
+// ---- lookup ----------------------------------------------------------------
+// out[4i..4i+4) = {fwd_lo, fwd_hi, rev_lo, found}; steps += key loads of the
+// bounded search (each depends on the previous one)
+__global__ __launch_bounds__(256) void k_lsi_lookup(LsiModel md, const unsigned long long *__restrict__ keys,
+                                                    const uint4 *__restrict__ pay, const LsiLeaf *__restrict__ lf,
+                                                    const unsigned long long *__restrict__ q, uint64_t nq,
+                                                    uint4 *__restrict__ out, unsigned long long *__restrict__ steps) {
+    __shared__ float s_p[3 * kLsiMaxHidden + 1];
+    for (uint32_t i = threadIdx.x; i < 3 * md.hidden + 1; i += blockDim.x) s_p[i] = md.p[i];
+    __syncthreads();
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t probes = 0;
+    if (t < nq && !md.nkeys) out[t] = make_uint4(0, 0, 0, 0);
+    if (t < nq && md.nkeys) {
+        const unsigned long long key = q[t];
+        const LsiLeaf f = lf[lsi_leaf(lsi_stage1(s_p, md.hidden, lsi_feature(key, md.k)), md.leaves)];
+        const long long pred = lsi_pred(f, key);
+        long long lo = pred - (long long)f.err, hi = pred + (long long)f.err; // inclusive window
+        lo = lo < 0 ? 0 : lo;
+        hi = hi >= (long long)md.nkeys ? (long long)md.nkeys - 1 : hi;
+        uint4 r = make_uint4(0, 0, 0, 0);
+        // lower bound of key in [lo, hi] (a binary search: its later probes
+        // stay in the lines of the earlier ones)
+        while (lo < hi) {
+            const long long mid = (lo + hi) >> 1;
+            probes++;
+            if (keys[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo <= hi) {
+            probes++;
+            if (keys[lo] == key) {
+                const uint4 v = pay[lo];
+                r = make_uint4(v.x, v.y, v.z, 1u);
+            }
+        }
+        out[t] = r;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) probes += __shfl_xor(probes, d, 64);
+    if ((threadIdx.x & 63) == 0 && steps) atomicAdd(steps, (unsigned long long)probes);
+}
+
+// Plain exact search of k-mer keys: k exact left extensions from the full
+// interval (the reference algorithm the learned index replaces).
+__global__ __launch_bounds__(256) void k_kmer_search(DevDir fwd, const DevConst *__restrict__ cst, uint32_t k,
+                                                     const unsigned long long *__restrict__ q, uint64_t nq,
+                                                     uint4 *__restrict__ out) {
+    __shared__ uint32_t s_bl[4], s_clane[16];
+    base_lanes(cst, s_bl);
+    if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
+    __syncthreads();
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq) return;
+    const unsigned long long key = q[t];
+    uint4 iv = make_uint4(0, fwd.n, 0, 0);
+    bool found = true;
+    for (uint32_t step = 0; step < k && found; step++) {
+        const uint32_t d = (uint32_t)(key >> (2 * step)) & 3u; // last base first
+        const uint32_t l = s_bl[d];
+        uint32_t cl[16], ch[16];
+        pair_counts(fwd, iv.x, iv.y, cl, ch);
+        uint32_t a = 0, b = 0, acc = 0, cs = 0, run = 0;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            if ((uint32_t)j == l) { a = cl[j]; b = ch[j]; acc = run; cs = s_clane[j]; }
+            run += ch[j] - cl[j];
+        }
+        found = l < 16 && b > a;
+        iv = make_uint4(cs + a, cs + b, iv.z + acc, 0u);
+    }
+    out[t] = found ? make_uint4(iv.x, iv.y, iv.z, 1u) : make_uint4(0, 0, 0, 0);
+}
+
+} // namespace hsfm
+
+using namespace hsfm;
+
+struct hsfm_lsi {
+    hsfm_index *idx = nullptr;
+    hsfm_lsi_config cfg{};
+    hsfm_lsi_info info{};
+    DevBuf keys, pay, leaves, params;
+};
+
+namespace {
+
+int lsi_scan_u8(hsfm_index *idx, const uint8_t *in, uint32_t *out, uint32_t n) {
+    hipStream_t s = idx->dev->stream;
+    if (n == 0) return HSFM_OK;
+    hipcub::TransformInputIterator<uint32_t, U8ToU32, const uint8_t *> it(in, U8ToU32());
+    size_t tmp = 0;
+    HSFM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, it, out, (int)n, s));
+    int rc = idx->ws.scan_tmp.ensure(tmp);
+    if (rc) return rc;
+    HSFM_HIP(hipcub::DeviceScan::ExclusiveSum(idx->ws.scan_tmp.p, tmp, it, out, (int)n, s));
+    return HSFM_OK;
+}
+
+double elapsed_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return ms;
+}
+
+// Enumerate every present exact-ACGT k-mer: sorted keys + {fwd_lo, fwd_hi, rev_lo}.
+int lsi_enumerate(hsfm_lsi *x) {
+    hsfm_index *idx = x->idx;
+    hipStream_t s = idx->dev->stream;
+    int rc;
+    DevBuf pk, pi;
+    if ((rc = pk.ensure(8)) || (rc = pi.ensure(16))) return rc;
+    const unsigned long long k0 = 0;
+    const uint4 root = make_uint4(0, idx->fwd.view.n, 0, 0);
+    HSFM_HIP(hipMemcpyAsync(pk.p, &k0, 8, hipMemcpyHostToDevice, s));
+    HSFM_HIP(hipMemcpyAsync(pi.p, &root, 16, hipMemcpyHostToDevice, s));
+    uint64_t P = 1;
+    for (uint32_t level = 0; level < x->cfg.k; level++) {
+        if (P > 0x7fffffffull) {
+            pk.release(); pi.release();
+            return set_error(HSFM_EINVAL, "learned seed index: %llu nodes at one level", (unsigned long long)P);
+        }
+        DevBuf flags, off, ck, ci;
+        if ((rc = flags.ensure(4 * P)) || (rc = off.ensure(4 * P * 4 + 16))) return rc;
+        const dim3 grid((uint32_t)((P + 255) / 256));
+        hipLaunchKernelGGL(k_lsi_expand<false>, grid, dim3(256), 0, s, idx->fwd.view, idx->d_const.as<DevConst>(),
+                           pk.as<unsigned long long>(), pi.as<uint4>(), (uint32_t)P, level, flags.as<uint8_t>(),
+                           nullptr, make_uint4(0, 0, 0, 0), nullptr, nullptr);
+        HSFM_HIP(hipGetLastError());
+        uint32_t tot[4];
+        uint4 base = make_uint4(0, 0, 0, 0);
+        for (int d = 0; d < 4; d++)
+            if ((rc = lsi_scan_u8(idx, flags.as<uint8_t>() + (size_t)d * P, off.as<uint32_t>() + (size_t)d * P,
+                                  (uint32_t)P)))
+                return rc;
+        // totals: last offset + last flag of each digit
+        for (int d = 0; d < 4; d++) {
+            uint32_t lo = 0;
+            uint8_t lf = 0;
+            HSFM_HIP(hipMemcpyAsync(&lo, off.as<uint32_t>() + (size_t)d * P + P - 1, 4, hipMemcpyDeviceToHost, s));
+            HSFM_HIP(hipMemcpyAsync(&lf, flags.as<uint8_t>() + (size_t)d * P + P - 1, 1, hipMemcpyDeviceToHost, s));
+            HSFM_HIP(hipStreamSynchronize(s));
+            tot[d] = lo + lf;
+        }
+        base.x = 0; base.y = tot[0]; base.z = tot[0] + tot[1]; base.w = tot[0] + tot[1] + tot[2];
+        const uint64_t C = (uint64_t)base.w + tot[3];
+        if ((rc = ck.ensure(C * 8 + 8)) || (rc = ci.ensure(C * 16 + 16))) return rc;
+        hipLaunchKernelGGL(k_lsi_expand<true>, grid, dim3(256), 0, s, idx->fwd.view, idx->d_const.as<DevConst>(),
+                           pk.as<unsigned long long>(), pi.as<uint4>(), (uint32_t)P, level, nullptr,
+                           off.as<uint32_t>(), base, ck.as<unsigned long long>(), ci.as<uint4>());
+        HSFM_HIP(hipGetLastError());
+        HSFM_HIP(hipStreamSynchronize(s));
+        flags.release(); off.release(); pk.release(); pi.release();
+        pk = ck; pi = ci;
+        P = C;
+        if (P == 0) break;
+    }
+    x->keys = pk;
+    x->pay = pi;
+    x->info.num_keys = P;
+    return HSFM_OK;
+}
+
+int lsi_train_and_fit(hsfm_lsi *x) {
+    hsfm_index *idx = x->idx;
+    hipStream_t s = idx->dev->stream;
+    const uint32_t H = x->cfg.hidden, L = x->cfg.leaves, np = 3 * H + 1;
+    const uint64_t N = x->info.num_keys;
+    int rc;
+    // stage-1 init: hidden unit i = relu(x - i/H); output = unit 0 (f(x) = x)
+    float hp[3 * kLsiMaxHidden + 1];
+    for (uint32_t i = 0; i < H; i++) {
+        hp[i] = 1.f;
+        hp[H + i] = -(float)i / (float)H;
+        hp[2 * H + i] = i == 0 ? 1.f : 0.f;
+    }
+    hp[3 * H] = 0.f;
+    if ((rc = x->params.ensure(np * 4))) return rc;
+    HSFM_HIP(hipMemcpyAsync(x->params.p, hp, np * 4, hipMemcpyHostToDevice, s));
+    DevBuf xs, ys, gm, lossb;
+    const uint32_t ns = (uint32_t)(N < x->cfg.sample ? N : x->cfg.sample);
+    if ((rc = xs.ensure((size_t)ns * 4 + 4)) || (rc = ys.ensure((size_t)ns * 4 + 4)) ||
+        (rc = gm.ensure(np * 4 * 3)) || (rc = lossb.ensure(8)))
+        return rc;
+    float *g = gm.as<float>(), *m = g + np, *v = m + np;
+    HSFM_HIP(hipMemsetAsync(gm.p, 0, np * 4 * 3, s));
+    hipEvent_t e0, e1;
+    HSFM_HIP(hipEventCreate(&e0));
+    HSFM_HIP(hipEventCreate(&e1));
+    HSFM_HIP(hipEventRecord(e0, s));
+    if (ns) {
+        const uint64_t stride = N / ns;
+        hipLaunchKernelGGL(k_lsi_sample, dim3((ns + 255) / 256), dim3(256), 0, s, x->keys.as<unsigned long long>(), N,
+                           stride ? stride : 1, ns, x->cfg.k, xs.as<float>(), ys.as<float>());
+        HSFM_HIP(hipGetLastError());
+        float b1t = 1.f, b2t = 1.f;
+        for (uint32_t step = 0; step <= x->cfg.train_steps; step++) {
+            HSFM_HIP(hipMemsetAsync(g, 0, np * 4, s));
+            HSFM_HIP(hipMemsetAsync(lossb.p, 0, 4, s));
+            hipLaunchKernelGGL(k_lsi_grad, dim3((ns + 255) / 256), dim3(256), 0, s, x->params.as<float>(), H,
+                               xs.as<float>(), ys.as<float>(), ns, g, lossb.as<float>());
+            HSFM_HIP(hipGetLastError());
+            if (step == 0 || step == x->cfg.train_steps) {
+                float l = 0.f;
+                HSFM_HIP(hipMemcpyAsync(&l, lossb.p, 4, hipMemcpyDeviceToHost, s));
+                HSFM_HIP(hipStreamSynchronize(s));
+                if (step == 0) x->info.mse_initial = l / ns;
+                else x->info.mse_final = l / ns;
+            }
+            if (step == x->cfg.train_steps) break;
+            b1t *= 0.9f;
+            b2t *= 0.999f;
+            hipLaunchKernelGGL(k_lsi_adam, dim3(1), dim3(256), 0, s, x->params.as<float>(), m, v, g, np, x->cfg.lr,
+                               b1t, b2t);
+            HSFM_HIP(hipGetLastError());
+        }
+        if (x->cfg.train_steps == 0) x->info.mse_final = x->info.mse_initial;
+    }
+    HSFM_HIP(hipEventRecord(e1, s));
+    x->info.train_ms = elapsed_ms(e0, e1);
+    xs.release(); ys.release(); gm.release(); lossb.release();
+
+    // stage 2
+    HSFM_HIP(hipEventRecord(e0, s));
+    DevBuf leaf_of, mom;
+    if ((rc = x->leaves.ensure((size_t)L * sizeof(LsiLeaf))) || (rc = leaf_of.ensure(N * 4 + 4)) ||
+        (rc = mom.ensure((size_t)L * 32)))
+        return rc;
+    HSFM_HIP(hipMemsetAsync(mom.p, 0, (size_t)L * 32, s));
+    hipLaunchKernelGGL(k_lsi_leaf_init, dim3((L + 255) / 256), dim3(256), 0, s, x->leaves.as<LsiLeaf>(), L);
+    LsiModel md{x->params.as<float>(), H, L, x->cfg.k, N};
+    const uint64_t chunks = (N + kLsiChunk - 1) / kLsiChunk;
+    const dim3 cg((uint32_t)((chunks + 255) / 256));
+    if (N) {
+        hipLaunchKernelGGL(k_lsi_route, cg, dim3(256), 0, s, md, x->keys.as<unsigned long long>(),
+                           leaf_of.as<uint32_t>(), x->leaves.as<LsiLeaf>());
+        hipLaunchKernelGGL(k_lsi_moments, cg, dim3(256), 0, s, N, x->keys.as<unsigned long long>(),
+                           leaf_of.as<uint32_t>(), x->leaves.as<LsiLeaf>(), mom.as<double>());
+    }
+    hipLaunchKernelGGL(k_lsi_solve, dim3((L + 255) / 256), dim3(256), 0, s, x->leaves.as<LsiLeaf>(),
+                       mom.as<double>(), L);
+    if (N)
+        hipLaunchKernelGGL(k_lsi_err, cg, dim3(256), 0, s, N, x->keys.as<unsigned long long>(), leaf_of.as<uint32_t>(),
+                           x->leaves.as<LsiLeaf>());
+    HSFM_HIP(hipGetLastError());
+    HSFM_HIP(hipEventRecord(e1, s));
+    x->info.fit_ms = elapsed_ms(e0, e1);
+    std::vector<LsiLeaf> h(L);
+    HSFM_HIP(hipMemcpyAsync(h.data(), x->leaves.p, (size_t)L * sizeof(LsiLeaf), hipMemcpyDeviceToHost, s));
+    HSFM_HIP(hipStreamSynchronize(s));
+    double wsum = 0;
+    uint32_t emax = 0;
+    for (uint32_t i = 0; i < L; i++) {
+        emax = h[i].err > emax ? h[i].err : emax;
+        wsum += (double)h[i].n * (2.0 * h[i].err + 1.0);
+    }
+    x->info.max_err = emax;
+    x->info.mean_window = N ? wsum / (double)N : 0.0;
+    leaf_of.release(); mom.release();
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return HSFM_OK;
+}
+
+} // namespace
+
+extern "C" int hsfm_lsi_build(hsfm_index *idx, const hsfm_lsi_config *cfg, hsfm_lsi **out) {
+    if (!idx || !cfg || !out) return set_error(HSFM_EINVAL, "hsfm_lsi_build: null argument");
+    if (cfg->k == 0 || cfg->k > 31) return set_error(HSFM_EINVAL, "hsfm_lsi_build: k must be in 1..31");
+    if (cfg->hidden == 0 || cfg->hidden > kLsiMaxHidden)
+        return set_error(HSFM_EINVAL, "hsfm_lsi_build: hidden must be in 1..%u", kLsiMaxHidden);
+    if (cfg->leaves == 0 || cfg->leaves > (1u << 24)) return set_error(HSFM_EINVAL, "hsfm_lsi_build: bad leaf count");
+    std::lock_guard<std::mutex> lk(idx->mu);
+    HSFM_HIP(hipSetDevice(idx->dev->ordinal));
+    hsfm_lsi *x = new hsfm_lsi();
+    x->idx = idx;
+    x->cfg = *cfg;
+    hipEvent_t e0, e1;
+    HSFM_HIP(hipEventCreate(&e0));
+    HSFM_HIP(hipEventCreate(&e1));
+    HSFM_HIP(hipEventRecord(e0, idx->dev->stream));
+    int rc = lsi_enumerate(x);
+    HSFM_HIP(hipEventRecord(e1, idx->dev->stream));
+    x->info.enum_ms = elapsed_ms(e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!rc) rc = lsi_train_and_fit(x);
+    if (rc) {
+        x->keys.release(); x->pay.release(); x->leaves.release(); x->params.release();
+        delete x;
+        return rc;
+    }
+    *out = x;
+    return HSFM_OK;
+}
+
+extern "C" int hsfm_lsi_get_info(const hsfm_lsi *x, hsfm_lsi_info *out) {
+    if (!x || !out) return set_error(HSFM_EINVAL, "hsfm_lsi_get_info: null argument");
+    *out = x->info;
+    return HSFM_OK;
+}
+
+extern "C" int hsfm_lsi_free(hsfm_lsi *x) {
+    if (!x) return HSFM_OK;
+    x->keys.release(); x->pay.release(); x->leaves.release(); x->params.release();
+    delete x;
+    return HSFM_OK;
+}
+
+extern "C" int hsfm_lsi_seed_device(hsfm_lsi *x, const uint64_t *d_keys, uint64_t n, uint32_t *d_out,
+                                    hsfm_seed_stats *st) {
+    if (!x || (n && (!d_keys || !d_out))) return set_error(HSFM_EINVAL, "hsfm_lsi_seed_device: null argument");
+    hsfm_index *idx = x->idx;
+    std::lock_guard<std::mutex> lk(idx->mu);
+    HSFM_HIP(hipSetDevice(idx->dev->ordinal));
+    hipStream_t s = idx->dev->stream;
+    int rc;
+    if ((rc = idx->ws.counters.ensure(64))) return rc;
+    HSFM_HIP(hipMemsetAsync(idx->ws.counters.p, 0, 8, s));
+    LsiModel md{x->params.as<float>(), x->cfg.hidden, x->cfg.leaves, x->cfg.k, x->info.num_keys};
+    if ((rc = event_record(idx, 0, false))) return rc;
+    if (n)
+        hipLaunchKernelGGL(k_lsi_lookup, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, md,
+                           x->keys.as<unsigned long long>(), x->pay.as<uint4>(), x->leaves.as<LsiLeaf>(),
+                           reinterpret_cast<const unsigned long long *>(d_keys), n, reinterpret_cast<uint4 *>(d_out),
+                           idx->ws.counters.as<unsigned long long>());
+    HSFM_HIP(hipGetLastError());
+    if ((rc = event_record(idx, 0, true))) return rc;
+    if (st) {
+        uint64_t probes = 0;
+        if ((rc = readback_u64(idx, idx->ws.counters.as<uint64_t>(), &probes, 1))) return rc;
+        st->kernel_ms = event_ms(idx, 0);
+        st->probes = probes;
+    }
+    return HSFM_OK;
+}
+
+extern "C" int hsfm_kmer_search_device(hsfm_index *idx, uint32_t k, const uint64_t *d_keys, uint64_t n,
+                                       uint32_t *d_out, hsfm_seed_stats *st) {
+    if (!idx || (n && (!d_keys || !d_out))) return set_error(HSFM_EINVAL, "hsfm_kmer_search_device: null argument");
+    if (k == 0 || k > 31) return set_error(HSFM_EINVAL, "hsfm_kmer_search_device: k must be in 1..31");
+    std::lock_guard<std::mutex> lk(idx->mu);
+    HSFM_HIP(hipSetDevice(idx->dev->ordinal));
+    hipStream_t s = idx->dev->stream;
+    int rc;
+    if ((rc = event_record(idx, 0, false))) return rc;
+    if (n)
+        hipLaunchKernelGGL(k_kmer_search, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, idx->fwd.view,
+                           idx->d_const.as<DevConst>(), k, reinterpret_cast<const unsigned long long *>(d_keys), n,
+                           reinterpret_cast<uint4 *>(d_out));
+    HSFM_HIP(hipGetLastError());
+    if ((rc = event_record(idx, 0, true))) return rc;
+    if (st) {
+        st->kernel_ms = event_ms(idx, 0);
+        st->probes = 0;
+    }
+    return HSFM_OK;
+}
+
+extern "C" int hsfm_lsi_grad(hsfm_device *dev, const float *params, uint32_t hidden, const float *x, const float *y,
+                             uint32_t n, float *grad_out, float *loss_out) {
+    if (!dev || !params || !grad_out || !loss_out || (n && (!x || !y)))
+        return set_error(HSFM_EINVAL, "hsfm_lsi_grad: null argument");
+    if (hidden == 0 || hidden > kLsiMaxHidden) return set_error(HSFM_EINVAL, "hsfm_lsi_grad: bad hidden size");
+    HSFM_HIP(hipSetDevice(dev->ordinal));
+    const uint32_t np = 3 * hidden + 1;
+    float *dp = nullptr, *dx = nullptr, *dy = nullptr, *dg = nullptr;
+    HSFM_HIP(hipMalloc((void **)&dp, np * 4));
+    HSFM_HIP(hipMalloc((void **)&dg, (np + 1) * 4));
+    HSFM_HIP(hipMalloc((void **)&dx, (size_t)n * 4 + 4));
+    HSFM_HIP(hipMalloc((void **)&dy, (size_t)n * 4 + 4));
+    hipStream_t s = dev->stream;
+    HSFM_HIP(hipMemcpyAsync(dp, params, np * 4, hipMemcpyHostToDevice, s));
+    if (n) {
+        HSFM_HIP(hipMemcpyAsync(dx, x, (size_t)n * 4, hipMemcpyHostToDevice, s));
+        HSFM_HIP(hipMemcpyAsync(dy, y, (size_t)n * 4, hipMemcpyHostToDevice, s));
+    }
+    HSFM_HIP(hipMemsetAsync(dg, 0, (np + 1) * 4, s));
+    if (n)
+        hipLaunchKernelGGL(k_lsi_grad, dim3((n + 255) / 256), dim3(256), 0, s, dp, hidden, dx, dy, n, dg, dg + np);
+    HSFM_HIP(hipGetLastError());
+    HSFM_HIP(hipMemcpyAsync(grad_out, dg, np * 4, hipMemcpyDeviceToHost, s));
+    HSFM_HIP(hipMemcpyAsync(loss_out, dg + np, 4, hipMemcpyDeviceToHost, s));
+    HSFM_HIP(hipStreamSynchronize(s));
+    if (n) *loss_out /= (float)n;
+    (void)hipFree(dp); (void)hipFree(dg); (void)hipFree(dx); (void)hipFree(dy);
+    return HSFM_OK;
+}
+
+// Host-array variants: stage keys and results through the index workspace.
+namespace {
+int stage_seed(hsfm_index *idx, const uint64_t *keys, uint64_t n, uint64_t **d_keys, uint32_t **d_out) {
+    int rc;
+    if ((rc = idx->ws.qcopy.ensure(n * 8 + 8)) || (rc = idx->ws.hits.ensure(n * 16 + 16))) return rc;
+    if (n) HSFM_HIP(hipMemcpyAsync(idx->ws.qcopy.p, keys, n * 8, hipMemcpyHostToDevice, idx->dev->stream));
+    *d_keys = idx->ws.qcopy.as<uint64_t>();
+    *d_out = idx->ws.hits.as<uint32_t>();
+    return HSFM_OK;
+}
+int unstage_seed(hsfm_index *idx, uint32_t *out, uint64_t n) {
+    if (n) HSFM_HIP(hipMemcpyAsync(out, idx->ws.hits.p, n * 16, hipMemcpyDeviceToHost, idx->dev->stream));
+    HSFM_HIP(hipStreamSynchronize(idx->dev->stream));
+    return HSFM_OK;
+}
+} // namespace
+
+extern "C" int hsfm_lsi_seed(hsfm_lsi *x, const uint64_t *keys, uint64_t n, uint32_t *out, hsfm_seed_stats *st) {
+    if (!x || (n && (!keys || !out))) return set_error(HSFM_EINVAL, "hsfm_lsi_seed: null argument");
+    uint64_t *dk;
+    uint32_t *dout;
+    int rc;
+    {
+        std::lock_guard<std::mutex> lk(x->idx->mu);
+        HSFM_HIP(hipSetDevice(x->idx->dev->ordinal));
+        if ((rc = stage_seed(x->idx, keys, n, &dk, &dout))) return rc;
+    }
+    if ((rc = hsfm_lsi_seed_device(x, dk, n, dout, st))) return rc;
+    std::lock_guard<std::mutex> lk(x->idx->mu);
+    return unstage_seed(x->idx, out, n);
+}
+
+extern "C" int hsfm_kmer_search(hsfm_index *idx, uint32_t k, const uint64_t *keys, uint64_t n, uint32_t *out) {
+    if (!idx || (n && (!keys || !out))) return set_error(HSFM_EINVAL, "hsfm_kmer_search: null argument");
+    uint64_t *dk;
+    uint32_t *dout;
+    int rc;
+    {
+        std::lock_guard<std::mutex> lk(idx->mu);
+        HSFM_HIP(hipSetDevice(idx->dev->ordinal));
+        if ((rc = stage_seed(idx, keys, n, &dk, &dout))) return rc;
+    }
+    if ((rc = hsfm_kmer_search_device(idx, k, dk, n, dout, nullptr))) return rc;
+    std::lock_guard<std::mutex> lk(idx->mu);
+    return unstage_seed(idx, out, n);
+}

@@ -184,6 +184,51 @@ typedef struct {
 } hsfm_stats;
 int hsfm_last_stats(hsfm_index *idx, hsfm_stats *out);
 
+/* ---- learned seed index ---------------------------------------------------
+ * New component (no reference entry point): replaces the first k exact
+ * backward-search steps of an exact-ACGT k-mer.  Its answer is that of plain
+ * search, extend_left over the k-mer (bidir.rs:112-130), as LookupTable::build
+ * computes it for depth-k prefixes (lookup.rs:37-91).  Keys pack the k-mer
+ * two bits per base, first base most significant (A=0, C=1, G=2, T=3). */
+typedef struct hsfm_lsi hsfm_lsi;
+typedef struct {
+    uint32_t k;           /* k-mer length, 1..31                            */
+    uint32_t leaves;      /* stage-2 leaves (4096)                          */
+    uint32_t hidden;      /* stage-1 MLP hidden units (<= 64)               */
+    uint32_t train_steps; /* Adam steps of stage 1; 0 keeps f(x) = x        */
+    uint32_t sample;      /* training sample size (keys, evenly strided)    */
+    float lr;             /* Adam learning rate                             */
+} hsfm_lsi_config;
+typedef struct {
+    uint64_t num_keys;    /* distinct present k-mers                        */
+    uint32_t max_err;     /* largest per-leaf error bound (slots)           */
+    uint32_t pad;
+    double mean_window;   /* mean search window 2*err+1 over the keys       */
+    double mse_initial, mse_final; /* stage-1 training loss before / after  */
+    double enum_ms, train_ms, fit_ms;
+} hsfm_lsi_info;
+typedef struct {
+    double kernel_ms;     /* HIP events around the kernel                  */
+    uint64_t probes;      /* key loads of the bounded search (dependent)    */
+} hsfm_seed_stats;
+/* Build from idx itself: every present k-mer, its interval, both stages. */
+int hsfm_lsi_build(hsfm_index *idx, const hsfm_lsi_config *cfg, hsfm_lsi **out);
+int hsfm_lsi_get_info(const hsfm_lsi *lsi, hsfm_lsi_info *out);
+/* Per key (device arrays): d_out[4i..4i+4) = {fwd_lo, fwd_hi, rev_lo, found}. */
+int hsfm_lsi_seed_device(hsfm_lsi *lsi, const uint64_t *d_keys, uint64_t n, uint32_t *d_out,
+                         hsfm_seed_stats *stats);
+/* The same answer by plain exact search (k left extensions per key). */
+int hsfm_kmer_search_device(hsfm_index *idx, uint32_t k, const uint64_t *d_keys, uint64_t n,
+                            uint32_t *d_out, hsfm_seed_stats *stats);
+/* Host-array variants of the two calls above. */
+int hsfm_lsi_seed(hsfm_lsi *lsi, const uint64_t *keys, uint64_t n, uint32_t *out, hsfm_seed_stats *stats);
+int hsfm_kmer_search(hsfm_index *idx, uint32_t k, const uint64_t *keys, uint64_t n, uint32_t *out);
+/* Stage-1 loss gradient of the training kernel on host data (for checks):
+ * params = w1[H], b1[H], w2[H], b2; grad_out has the same layout. */
+int hsfm_lsi_grad(hsfm_device *dev, const float *params, uint32_t hidden, const float *x,
+                  const float *y, uint32_t n, float *grad_out, float *loss_out);
+int hsfm_lsi_free(hsfm_lsi *lsi);
+
 #ifdef __cplusplus
 }
 #endif

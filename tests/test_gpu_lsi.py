@@ -1,0 +1,101 @@
+"""GPU checks of the learned seed index: exactness against plain exact search
+and the oracle, training-kernel gradients against the float64 oracle, and what
+the trained stage 1 gains over the untrained model."""
+import numpy as np
+import pytest
+
+import haystackfm_amd as hf
+from oracle import lsi as L
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _bidir(seed, total, **kw):
+    text, bounds = O.gen_reference(seed, total, 3, gc=0.41, **kw)
+    o = O.OracleBidir.build(text=text, bounds=bounds, sample_rate=16, threads=8)
+    return text, o, hf.BidirFmIndex.from_bytes(o.to_bytes())
+
+
+def _query_keys(rng, keys, k, n_absent=2000):
+    absent = rng.integers(0, 4 ** k, n_absent, dtype=np.uint64)
+    mutated = keys[rng.integers(0, len(keys), 2000)] ^ (np.uint64(1) << (2 * rng.integers(0, k, 2000)).astype(np.uint64))
+    return np.concatenate([keys, absent, mutated, keys[:1] + np.uint64(1)])
+
+
+@pytest.mark.parametrize("k", [9, 21])
+def test_lsi_matches_oracle_table(k):
+    """Every present k-mer of a small text and random absent ones: the learned
+    index, plain exact search on the device and the oracle agree."""
+    rng = np.random.default_rng(k)
+    text, o, g = _bidir(60 + k, 12_000, repeat_frac=0.3, n_units=5, unit_min=100, unit_max=600, iupac_frac=0.01)
+    keys, iv = L.kmer_table(text, o, k)
+    lsi = hf.LearnedSeedIndex(g, k=k, leaves=64, hidden=8, train_steps=100, sample=4096)
+    info = lsi.info()
+    assert info["num_keys"] == len(keys)
+    q = _query_keys(rng, keys, k)
+    got = lsi.seed(q)
+    plain = hf.kmer_search(g, q, k)
+    assert np.array_equal(got, plain)
+    for key, row in zip(q[::5], got[::5]):
+        exp = L.lookup(keys, iv, int(key))
+        assert (exp is None and row[3] == 0) or (exp is not None and tuple(row) == exp + (1,))
+
+
+def test_lsi_exact_on_larger_index():
+    """200 kbp with repeats and N: 21-mers of reads (1% substitutions) and of
+    random sequence, learned index == plain exact search."""
+    rng = np.random.default_rng(3)
+    text, o, g = _bidir(77, 200_000, repeat_frac=0.45, n_units=40, unit_min=300, unit_max=3000,
+                        iupac_frac=0.002, iupac_codes=[5])
+    lsi = hf.LearnedSeedIndex(g, k=21, leaves=4096, hidden=16, train_steps=300)
+    info = lsi.info()
+    assert 0 < info["num_keys"] <= len(text)
+    flat, off = O.gen_reads(5, text, o.fwd.seq_boundaries, 3000, 150, sub=0.01, both_strands=True)
+    q = hf.kmer_keys(flat, 21)
+    q = q[q != np.uint64(0xFFFFFFFFFFFFFFFF)]
+    q = np.concatenate([q, rng.integers(0, 4 ** 21, 5000, dtype=np.uint64)])
+    got = lsi.seed(q)
+    assert np.array_equal(got, hf.kmer_search(g, q, 21))
+    assert got[:, 3].sum() > len(q) // 3
+
+
+def test_lsi_gradient_matches_oracle():
+    rng = np.random.default_rng(11)
+    H = 16
+    params = {"w1": rng.normal(size=H), "b1": rng.normal(size=H) * 0.3, "w2": rng.normal(size=H) * 0.2,
+              "b2": np.array(0.1)}
+    x = rng.random(5000)
+    y = np.sqrt(x) + 0.01 * rng.normal(size=5000)
+    flat = np.concatenate([params["w1"], params["b1"], params["w2"], [params["b2"]]])
+    loss, g = hf.lsi_gradient(flat.astype(np.float32), H, x.astype(np.float32), y.astype(np.float32))
+    p32 = {kk: np.asarray(v, np.float32).astype(np.float64) for kk, v in params.items()}
+    x32, y32 = x.astype(np.float32).astype(np.float64), y.astype(np.float32).astype(np.float64)
+    ref = L.mlp_grad(p32, x32, y32)
+    ref_flat = np.concatenate([ref["w1"], ref["b1"], ref["w2"], [float(ref["b2"])]])
+    assert abs(loss - L.mlp_loss(p32, x32, y32)) <= 1e-4 * max(1.0, abs(loss))
+    scale = np.abs(ref_flat).max()
+    assert np.abs(g - ref_flat).max() <= 2e-4 * scale
+
+
+def test_learned_index_gains_over_the_methods_without_it():
+    """What the trained model gains: training lowers the stage-1 loss below
+    the untrained f(x) = x, and a lookup needs far fewer dependent key probes
+    (bounded search in the predicted window) than a binary search of the whole
+    key array, or than the k exact left extensions of plain search."""
+    rng = np.random.default_rng(2)
+    text, o, g = _bidir(91, 300_000, repeat_frac=0.45, n_units=40, unit_min=300, unit_max=3000)
+    base = hf.LearnedSeedIndex(g, k=21, leaves=4096, hidden=16, train_steps=0)
+    trained = hf.LearnedSeedIndex(g, k=21, leaves=4096, hidden=16, train_steps=400)
+    bi, ti = base.info(), trained.info()
+    assert ti["num_keys"] == bi["num_keys"]
+    assert ti["mse_final"] < 0.5 * bi["mse_final"]
+    flat, _ = O.gen_reads(8, text, o.fwd.seq_boundaries, 2000, 150, sub=0.01)
+    q = hf.kmer_keys(flat, 21)
+    q = q[q != np.uint64(0xFFFFFFFFFFFFFFFF)][:20000]
+    st = {}
+    got = trained.seed(q, st)
+    assert np.array_equal(got, hf.kmer_search(g, q, 21))
+    loads = st["probes"] / len(q)  # dependent key loads per lookup
+    assert loads < 0.5 * np.log2(ti["num_keys"])  # vs a binary search of all keys
+    assert loads < 21 / 2                          # vs 21 dependent exact extensions
