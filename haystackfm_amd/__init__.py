@@ -215,6 +215,9 @@ class _MemDev(C.Structure):
                 ("d_hit_offsets", C.c_void_p), ("d_hits", C.c_void_p), ("d_positions", C.c_void_p)]
 
 
+_BATCH_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(_MemDev))
+
+
 class _Stats(C.Structure):
     _fields_ = [("fallback_reads", C.c_uint64), ("capacity_retries", C.c_uint64),
                 ("guard_hits", C.c_uint64), ("records", C.c_uint64), ("main_kernel_ms", C.c_double),
@@ -235,6 +238,17 @@ class _LsiInfo(C.Structure):
 
 class _SeedStats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("probes", C.c_uint64)]
+
+
+class _BatchConfig(C.Structure):
+    _fields_ = [("batch_reads", C.c_uint32), ("autotune", C.c_int), ("main_kernel", C.c_int),
+                ("resolve_fanout", C.c_int), ("use_graph", C.c_int)]
+
+
+class _BatchStats(C.Structure):
+    _fields_ = [("batches", C.c_uint32), ("fallback_batches", C.c_uint32), ("main_kernel", C.c_int),
+                ("resolve_fanout", C.c_int), ("records", C.c_uint64), ("positions", C.c_uint64),
+                ("total_ms", C.c_double), ("tune_total_ms", C.c_double), ("tune_ms", C.c_double * 6)]
 
 
 _lib_obj = None
@@ -284,6 +298,13 @@ def _lib():
         L.hsfm_kmer_search.argtypes = [vp, C.c_uint32, vp, C.c_uint64, vp]
         L.hsfm_lsi_grad.argtypes = [vp, vp, C.c_uint32, vp, vp, C.c_uint32, vp, vp]
         L.hsfm_lsi_free.argtypes = [vp]
+        L.hsfm_find_mems_batched.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, C.c_int, vp, C.c_uint32,
+                                             C.c_uint32, C.POINTER(_BatchConfig), _BATCH_FN, vp,
+                                             C.POINTER(_BatchStats)]
+        L.hsfm_device_alloc.argtypes = [vp, C.c_uint64, C.POINTER(vp)]
+        L.hsfm_device_free.argtypes = [vp, vp]
+        L.hsfm_host_to_device.argtypes = [vp, vp, vp, C.c_uint64]
+        L.hsfm_device_to_host.argtypes = [vp, vp, vp, C.c_uint64]
         _lib_obj = L
         return L
 
@@ -551,6 +572,71 @@ class BidirFmIndex:
         finally:
             _lib().hsfm_free_result(res)
         return ho, hits, pos
+
+    def find_mems_batched(self, queries, min_len: int, mode: int, ref_boundaries=(), max_hits_per_mem: int = 1024,
+                          batch_reads: int = 65536, autotune: bool = False, main_kernel: int = -1,
+                          resolve_fanout: int = 0, use_graph: bool = True):
+        """The batched pipeline (one HIP graph per batch); same raw CSR as
+        find_mems_batch, plus the pipeline statistics."""
+        flat, off = _to_csr(queries)
+        nq = len(off) - 1
+        rb = np.ascontiguousarray(np.asarray(ref_boundaries, np.uint32))
+        h = self._handle()
+        dev = self.fwd._dev[0]._h
+        L = _lib()
+        bufs = []
+
+        def dalloc(arr):
+            p = C.c_void_p()
+            _check(L.hsfm_device_alloc(dev, max(arr.nbytes, 8), C.byref(p)), "device_alloc")
+            bufs.append(p)
+            if arr.nbytes:
+                _check(L.hsfm_host_to_device(dev, p, arr.ctypes.data, arr.nbytes), "host_to_device")
+            return p
+
+        parts = []
+
+        def on_batch(_user, first, nb, res_p):
+            r = res_p.contents
+            nh, npos = int(r.num_hits), int(r.num_positions)
+            ho = np.zeros(nb + 1, np.uint64)
+            _check(L.hsfm_device_to_host(dev, ho.ctypes.data, C.c_void_p(r.d_hit_offsets), ho.nbytes), "d2h")
+            hits = np.zeros((nh, 8), np.uint32)
+            pos = np.zeros((npos, 2), np.uint32)
+            if nh:
+                _check(L.hsfm_device_to_host(dev, hits.ctypes.data, C.c_void_p(r.d_hits), hits.nbytes), "d2h")
+            if npos:
+                _check(L.hsfm_device_to_host(dev, pos.ctypes.data, C.c_void_p(r.d_positions), pos.nbytes), "d2h")
+            parts.append((int(first), ho, hits, pos))
+
+        cb = _BATCH_FN(on_batch)
+        try:
+            dq, do, dr = dalloc(flat), dalloc(off), dalloc(rb)
+            cfg = _BatchConfig(batch_reads, int(autotune), main_kernel, resolve_fanout, int(use_graph))
+            st = _BatchStats()
+            _check(L.hsfm_find_mems_batched(h, dq, do, nq, int(min_len), mode, dr if len(rb) else None, len(rb),
+                                            int(max_hits_per_mem), C.byref(cfg), cb, None, C.byref(st)),
+                   "find_mems_batched")
+        finally:
+            for p in bufs:
+                L.hsfm_device_free(dev, p)
+        parts.sort(key=lambda t: t[0])
+        ho_all, hits_all, pos_all = [np.zeros(1, np.uint64)], [], []
+        nh_run = npos_run = 0
+        for _, ho, hits, pos in parts:
+            ho_all.append(ho[1:] + np.uint64(nh_run))
+            if len(hits):
+                po = (hits[:, 4].astype(np.uint64) | (hits[:, 5].astype(np.uint64) << np.uint64(32))) + np.uint64(npos_run)
+                hits = hits.copy()
+                hits[:, 4] = (po & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+                hits[:, 5] = (po >> np.uint64(32)).astype(np.uint32)
+            hits_all.append(hits)
+            pos_all.append(pos)
+            nh_run += len(hits)
+            npos_run += len(pos)
+        stats = {k: (list(getattr(st, k)) if k == "tune_ms" else getattr(st, k)) for k, _ in _BatchStats._fields_}
+        return (np.concatenate(ho_all), np.concatenate(hits_all) if hits_all else np.zeros((0, 8), np.uint32),
+                np.concatenate(pos_all) if pos_all else np.zeros((0, 2), np.uint32), stats)
 
     def _mem_hits(self, queries, min_len, mode, ref_boundaries, max_hits_per_mem) -> list[list[MemHit]]:
         ho, hits, pos = self.find_mems_batch(queries, min_len, mode, ref_boundaries, max_hits_per_mem)

@@ -72,6 +72,9 @@ struct hsfm_index {
     // bidirectional seed table of every ACGT F-mer (built on first MEM call)
     hsfm::DevBuf seed_off, seed_iv, seed_lsl;
     uint32_t seed_F = 0;
+    // batched MEM pipeline: configuration picked by the autotuner
+    int tuned_main = -1, tuned_rpt = -1;
+    uint32_t tuned_min_len = 0, tuned_batch = 0;
 };
 
 namespace hsfm {
@@ -86,6 +89,11 @@ double event_ms(hsfm_index *idx, int slot);
 // Table of the bidirectional intervals of every ACGT F-mer's compatible strings
 // (kept on the index until it is freed).
 int build_seed_table(hsfm_index *idx, uint32_t F);
+// Resolve rows[0, min(cap, *d_nrows)) with rpt (1, 2, 4) interleaved walks per
+// thread on a grid of `grid` blocks (no host-side row count needed).
+int launch_resolve_fan(hsfm_index *idx, const DevDir &d, const uint32_t *d_rows, uint64_t cap,
+                       const uint64_t *d_nrows, const uint32_t *d_bounds, uint32_t nbounds, uint32_t *d_out_pairs,
+                       unsigned long long *d_guard, int rpt, uint32_t grid);
 
 // Workgroups of a persistent kernel: as many as stay resident on every CU.
 template <typename K>

@@ -23,6 +23,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <hipcub/hipcub.hpp>
+
 #include "hsfm_host.h"
 
 namespace hsfm {
@@ -134,7 +136,8 @@ __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const D
                                                  uint32_t cap, uint4 *__restrict__ sets, MemRec *__restrict__ recs,
                                                  uint64_t rec_cap, uint32_t *__restrict__ mem_cnt,
                                                  uint32_t *__restrict__ pass_of, uint32_t pass,
-                                                 uint32_t *__restrict__ retry_list, MemCounters *__restrict__ ctr) {
+                                                 uint32_t *__restrict__ retry_list, MemCounters *__restrict__ ctr,
+                                                 const unsigned long long *__restrict__ d_nlist) {
     __shared__ uint16_t s_compat[256];
     __shared__ uint8_t s_expand[256];
     __shared__ uint32_t s_clane[16];
@@ -152,6 +155,7 @@ __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const D
     const uint32_t n = fwd.n;
     const uint32_t F = sv.F;
     if (LDS) cap = kWaveLdsCap;
+    if (d_nlist) nlist = *d_nlist < nlist ? (uint32_t)*d_nlist : nlist; // count left by the previous kernel
     uint4 *bufA = LDS ? s_sets : sets + (size_t)blockIdx.x * 2 * cap;
     uint4 *bufB = bufA + cap;
     for (;;) {
@@ -253,6 +257,11 @@ __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const D
             if (lane == 0) slot = atomicAdd(&ctr->rec_used, (unsigned long long)ns);
             slot = block_bcast<TB>(slot, &s_b64);
             if (slot + ns > rec_cap) {
+                // mark the part of the reservation that fits as dead records
+                for (uint64_t t = slot + lane; t < slot + ns && t < rec_cap; t += TB) {
+                    recs[t].read = 0xffffffffu;
+                    recs[t].pass = 0xffffffffu;
+                }
                 too_big = true;
                 break;
             }
@@ -634,7 +643,6 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
     if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
     __syncthreads();
     const uint32_t tid = threadIdx.x;
-    const uint32_t L = cst->num_lanes;
     const uint32_t n = fwd.n;
     const uint32_t F = sv.F;
     const bool seeding = F != 0 && min_len > F;
@@ -1178,21 +1186,30 @@ __global__ __launch_bounds__(256) void k_mem_gather(uint32_t nq, const uint64_t 
 }
 
 // Place the records of successful attempts at their final position.
-__global__ __launch_bounds__(256) void k_mem_scatter(const MemRec *__restrict__ recs, uint64_t nrec,
+// A reservation that did not fit the record buffer leaves dead slots (pass
+// ~0, or never written): skip them before using their read id.
+__global__ __launch_bounds__(256) void k_mem_scatter(const MemRec *__restrict__ recs, uint64_t nrec, uint32_t nq,
                                                      const uint32_t *__restrict__ pass_of,
-                                                     const uint64_t *__restrict__ mem_off, uint4 *__restrict__ out) {
+                                                     const uint64_t *__restrict__ mem_off, uint4 *__restrict__ out,
+                                                     const unsigned long long *__restrict__ d_nrec) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d_nrec) nrec = *d_nrec < nrec ? *d_nrec : nrec;
     if (t >= nrec) return;
     const MemRec rc = recs[t];
-    if (pass_of[rc.read] != rc.pass) return;
+    if (rc.pass == 0xffffffffu || rc.read >= nq || pass_of[rc.read] != rc.pass) return;
     out[mem_off[rc.read] + rc.local] = make_uint4(rc.qs, rc.qe, rc.lo, rc.hi);
 }
 
 // Per record: positions to resolve = min(size, max_hits) (bidir_index.rs:315-337).
 __global__ __launch_bounds__(256) void k_mem_npos(const uint4 *__restrict__ recs, uint64_t nrec, uint32_t max_hits,
-                                                  uint32_t resolve, uint32_t *__restrict__ npos) {
+                                                  uint32_t resolve, uint32_t *__restrict__ npos,
+                                                  const uint64_t *__restrict__ d_nrec) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nrec) return;
+    if (d_nrec && t >= *d_nrec) { // capacity-sized launch: no record here
+        npos[t] = 0;
+        return;
+    }
     const uint4 r = recs[t];
     const uint32_t size = r.w - r.z;
     npos[t] = resolve ? (size < max_hits ? size : max_hits) : 0u;
@@ -1200,9 +1217,15 @@ __global__ __launch_bounds__(256) void k_mem_npos(const uint4 *__restrict__ recs
 
 __global__ __launch_bounds__(256) void k_mem_hits(const uint4 *__restrict__ recs, uint64_t nrec, uint32_t max_hits,
                                                   const uint64_t *__restrict__ pos_off, hsfm_mem_hit *__restrict__ hits,
-                                                  uint32_t *__restrict__ rows) {
+                                                  uint32_t *__restrict__ rows, const uint64_t *__restrict__ d_nrec,
+                                                  uint64_t pos_cap, unsigned long long *__restrict__ ovf) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d_nrec) nrec = *d_nrec < nrec ? *d_nrec : nrec;
     if (t >= nrec) return;
+    if (pos_off[t + 1] > pos_cap) { // positions would overrun a capacity-sized buffer
+        atomicAdd(ovf, 1ull);
+        return;
+    }
     const uint4 r = recs[t];
     const uint32_t size = r.w - r.z;
     const uint64_t po = pos_off[t], pl = pos_off[t + 1] - po;
@@ -1371,12 +1394,12 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
                                idx->rev.view,
                                idx->d_const.as<DevConst>(), sv, d_q, d_qoff, list, (uint32_t)nlist, min_len, cap,
                                (uint4 *)nullptr, ws.mem_recs.as<MemRec>(), rec_cap, ws.mem_cnt.as<uint32_t>(),
-                               ws.mem_flags.as<uint32_t>(), pass, retry, ctr);
+                               ws.mem_flags.as<uint32_t>(), pass, retry, ctr, nullptr);
         else
             hipLaunchKernelGGL((k_mem_wave<false, 1024>), dim3((uint32_t)waves), dim3(1024), 0, s, idx->fwd.view,
                                idx->rev.view, idx->d_const.as<DevConst>(), sv, d_q, d_qoff, list, (uint32_t)nlist,
                                min_len, cap, ws.wave_sets.as<uint4>(), ws.mem_recs.as<MemRec>(), rec_cap,
-                               ws.mem_cnt.as<uint32_t>(), ws.mem_flags.as<uint32_t>(), pass, retry, ctr);
+                               ws.mem_cnt.as<uint32_t>(), ws.mem_flags.as<uint32_t>(), pass, retry, ctr, nullptr);
         HSFM_HIP(hipGetLastError());
         if (timed && (rc = event_record(idx, 0, true))) return rc;
         if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
@@ -1408,17 +1431,19 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
                            ws.mem_sorted.as<uint4>());
         HSFM_HIP(hipGetLastError());
     }
+    if (rec_used > rec_cap) rec_used = rec_cap; // reservations past the buffer were never written
     if (rec_used) {
         hipLaunchKernelGGL(k_mem_scatter, dim3((uint32_t)((rec_used + 255) / 256)), dim3(256), 0, s,
-                           ws.mem_recs.as<MemRec>(), rec_used, ws.mem_flags.as<uint32_t>(), ws.mem_off.as<uint64_t>(),
-                           ws.mem_sorted.as<uint4>());
+                           ws.mem_recs.as<MemRec>(), rec_used, nq, ws.mem_flags.as<uint32_t>(),
+                           ws.mem_off.as<uint64_t>(), ws.mem_sorted.as<uint4>(), nullptr);
         HSFM_HIP(hipGetLastError());
     }
     if ((rc = ws.cnt.ensure(nhits * 4 + 4))) return rc;
     if ((rc = ws.pos_off.ensure((nhits + 1) * 8))) return rc;
     if (nhits) {
         hipLaunchKernelGGL(k_mem_npos, dim3((uint32_t)((nhits + 255) / 256)), dim3(256), 0, s,
-                           ws.mem_sorted.as<uint4>(), nhits, max_hits, n_refs ? 1u : 0u, ws.cnt.as<uint32_t>());
+                           ws.mem_sorted.as<uint4>(), nhits, max_hits, n_refs ? 1u : 0u, ws.cnt.as<uint32_t>(),
+                           nullptr);
         HSFM_HIP(hipGetLastError());
     }
     if ((rc = exclusive_scan_u32_u64(idx, ws.cnt.as<uint32_t>(), ws.pos_off.as<uint64_t>(), (uint32_t)nhits))) return rc;
@@ -1430,7 +1455,7 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     if (nhits) {
         hipLaunchKernelGGL(k_mem_hits, dim3((uint32_t)((nhits + 255) / 256)), dim3(256), 0, s,
                            ws.mem_sorted.as<uint4>(), nhits, max_hits, ws.pos_off.as<uint64_t>(),
-                           ws.mem_hits.as<hsfm_mem_hit>(), ws.rows.as<uint32_t>());
+                           ws.mem_hits.as<hsfm_mem_hit>(), ws.rows.as<uint32_t>(), nullptr, ~0ull, nullptr);
         HSFM_HIP(hipGetLastError());
     }
     if (npos) {
@@ -1513,5 +1538,428 @@ extern "C" int hsfm_find_mems_batch(hsfm_index *idx, const uint8_t *qflat, const
         return set_error(HSFM_EDEVICE, "copy of MEM results failed: %s", hipGetErrorString(e));
     }
     *out = res;
+    return HSFM_OK;
+}
+
+// ===========================================================================
+// Batched MEM pipeline for uneven reads: fixed-size batches of B reads, each
+// run by one captured HIP graph (find -> second-level passes -> gather/scatter
+// -> positions -> resolve -> reference map) whose kernels take every size they
+// need from device counters, so a batch costs one graph launch and one host
+// wait (to hand its results to the caller).  The graph reads the batch's
+// offsets from a fixed staging buffer that a copy outside the graph refills.
+// Reads that outgrow the in-graph capacities send their batch through the
+// host-driven path instead.
+// ===========================================================================
+namespace hsfm {
+
+constexpr uint32_t kBatchGlobalCap = 65536; // intervals per set in the in-graph global pass
+constexpr uint32_t kBatchGlobalWaves = 64;  // workgroups of that pass
+enum : int { BM_PERSIST8 = 0, BM_PERSIST16, BM_PERSIST32, BM_WAVE, BM_COUNT };
+
+__global__ void k_pad_qoff(uint64_t *bq, uint32_t nb, uint32_t B) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x + nb + 1;
+    if (i <= B) bq[i] = bq[nb]; // padding reads have length 0
+}
+__global__ void k_npos_total(const uint64_t *mem_off, uint32_t B, const uint64_t *pos_off, uint64_t pos_cap,
+                             uint64_t *tot) {
+    const uint64_t t = pos_off[mem_off[B]];
+    *tot = t < pos_cap ? t : pos_cap;
+}
+// dead pass numbers in the whole record buffer (stale records never match)
+__global__ void k_fill_u64(uint64_t *d, uint64_t n, uint64_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        d[i] = v;
+}
+
+struct BatchBufs {
+    uint32_t B = 0;
+    uint64_t slot_cap = 0, rec_cap = 0, hit_cap = 0, pos_cap = 0;
+    DevBuf bq_off, slots, mem_cnt, status, pass_of, l1, l2, l3, ctr, recs, wsets;
+    DevBuf mem_off, sorted, npos, pos_off, hits, rows, positions, misc, tmp;
+    size_t tmp_bytes = 0;
+    void release() {
+        DevBuf *all[] = {&bq_off, &slots, &mem_cnt, &status, &pass_of, &l1, &l2, &l3, &ctr, &recs, &wsets,
+                         &mem_off, &sorted, &npos, &pos_off, &hits, &rows, &positions, &misc, &tmp};
+        for (DevBuf *b : all) b->release();
+    }
+};
+// misc: [0] positions-overflow count, [1] npos total, [2] walk guard
+struct BatchMisc {
+    unsigned long long pos_ovf, npos, guard, pad;
+};
+
+struct ToU64 {
+    __device__ __forceinline__ uint64_t operator()(uint32_t v) const { return v; }
+};
+__global__ void k_zero_u64(uint64_t *p) { *p = 0; }
+
+static int batch_scan(hsfm_index *idx, BatchBufs &bb, const uint32_t *in, uint64_t *out, uint64_t n) {
+    hipStream_t s = idx->dev->stream;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, out);
+    if (!n) return HSFM_OK;
+    hipcub::TransformInputIterator<uint64_t, ToU64, const uint32_t *> it(in, ToU64());
+    size_t tmp = bb.tmp_bytes;
+    HSFM_HIP(hipcub::DeviceScan::InclusiveSum(bb.tmp.p, tmp, it, out + 1, (int)n, s));
+    return HSFM_OK;
+}
+
+// The launches of one batch (captured into a graph, or run eagerly).
+static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, const uint8_t *d_q, uint32_t min_len,
+                        const uint32_t *d_refb, uint32_t n_refs, uint32_t max_hits, int main_kernel, int rpt) {
+    hipStream_t s = idx->dev->stream;
+    const uint32_t B = bb.B;
+    MemCounters *c = bb.ctr.as<MemCounters>();
+    const DevConst *cst = idx->d_const.as<DevConst>();
+    const uint64_t *bq = bb.bq_off.as<uint64_t>();
+    int rc;
+    if ((rc = fill_u32(idx, bb.ctr.as<uint32_t>(), 3 * sizeof(MemCounters) / 4, 0)) ||
+        (rc = fill_u32(idx, bb.misc.as<uint32_t>(), sizeof(BatchMisc) / 4, 0)) ||
+        (rc = fill_u32(idx, bb.mem_cnt.as<uint32_t>(), B, 0)) ||
+        (rc = fill_u32(idx, bb.pass_of.as<uint32_t>(), B, 0xffffffffu)) ||
+        (rc = fill_u32(idx, bb.status.as<uint32_t>(), (B + 3) / 4, main_kernel == BM_WAVE ? 0x01010101u : 0u)))
+        return rc;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, s, bb.recs.as<uint64_t>(), bb.rec_cap * 4, ~0ull);
+    const uint64_t half = bb.rec_cap / 2;
+    MemRec *r0 = bb.recs.as<MemRec>(), *r1 = r0 + half;
+    const uint32_t lds_waves = persist_grid(idx, k_mem_wave<true, 64>, 64);
+    const unsigned long long *second_count = &c[0].n_retry;
+    if (main_kernel == BM_WAVE) {
+        // one wave per read for every read
+        hipLaunchKernelGGL((k_mem_wave<true, 64>), dim3(lds_waves), dim3(64), 0, s, idx->fwd.view, idx->rev.view, cst,
+                           sv, d_q, bq, (const uint32_t *)nullptr, B, min_len, kWaveLdsCap, (uint4 *)nullptr, r0, half,
+                           bb.mem_cnt.as<uint32_t>(), bb.pass_of.as<uint32_t>(), 0u, bb.l2.as<uint32_t>(), &c[1],
+                           (const unsigned long long *)nullptr);
+    } else {
+        uint4 *slots = bb.slots.as<uint4>();
+        uint32_t *cnt = bb.mem_cnt.as<uint32_t>();
+        uint8_t *st = bb.status.as<uint8_t>();
+        if (main_kernel == BM_PERSIST8)
+            hipLaunchKernelGGL((k_mem_persist<8, 256>), dim3(persist_grid(idx, k_mem_persist<8, 256>, 256)), dim3(256),
+                               0, s, idx->fwd.view, idx->rev.view, cst, sv, d_q, bq, (const uint32_t *)nullptr, B,
+                               min_len, slots, cnt, st, bb.l1.as<uint32_t>(), &c[0]);
+        else if (main_kernel == BM_PERSIST32)
+            hipLaunchKernelGGL((k_mem_persist<32, 128>), dim3(persist_grid(idx, k_mem_persist<32, 128>, 128)),
+                               dim3(128), 0, s, idx->fwd.view, idx->rev.view, cst, sv, d_q, bq,
+                               (const uint32_t *)nullptr, B, min_len, slots, cnt, st, bb.l1.as<uint32_t>(), &c[0]);
+        else
+            hipLaunchKernelGGL((k_mem_persist<16, 256>), dim3(persist_grid(idx, k_mem_persist<16, 256>, 256)),
+                               dim3(256), 0, s, idx->fwd.view, idx->rev.view, cst, sv, d_q, bq,
+                               (const uint32_t *)nullptr, B, min_len, slots, cnt, st, bb.l1.as<uint32_t>(), &c[0]);
+        HSFM_HIP(hipGetLastError());
+        // deferred reads: one wave each, sets in LDS
+        hipLaunchKernelGGL((k_mem_wave<true, 64>), dim3(lds_waves), dim3(64), 0, s, idx->fwd.view, idx->rev.view, cst,
+                           sv, d_q, bq, bb.l1.as<uint32_t>(), B, min_len, kWaveLdsCap, (uint4 *)nullptr, r0, half, cnt,
+                           bb.pass_of.as<uint32_t>(), 0u, bb.l2.as<uint32_t>(), &c[1], second_count);
+    }
+    HSFM_HIP(hipGetLastError());
+    // sets beyond the LDS: 1024 threads per read, global sets
+    hipLaunchKernelGGL((k_mem_wave<false, 1024>), dim3(kBatchGlobalWaves), dim3(1024), 0, s, idx->fwd.view,
+                       idx->rev.view, cst, sv, d_q, bq, bb.l2.as<uint32_t>(), B, min_len, kBatchGlobalCap,
+                       bb.wsets.as<uint4>(), r1, half, bb.mem_cnt.as<uint32_t>(), bb.pass_of.as<uint32_t>(), 1u,
+                       bb.l3.as<uint32_t>(), &c[2], &c[1].n_retry);
+    HSFM_HIP(hipGetLastError());
+    uint64_t *mem_off = bb.mem_off.as<uint64_t>();
+    if ((rc = batch_scan(idx, bb, bb.mem_cnt.as<uint32_t>(), mem_off, B))) return rc;
+    uint4 *sorted = bb.sorted.as<uint4>();
+    if (main_kernel != BM_WAVE)
+        hipLaunchKernelGGL(k_mem_gather, dim3((B + 255) / 256), dim3(256), 0, s, B, bq, bb.status.as<uint8_t>(),
+                           bb.mem_cnt.as<uint32_t>(), mem_off, bb.slots.as<uint4>(), sorted);
+    const dim3 hg((uint32_t)((half + 255) / 256));
+    hipLaunchKernelGGL(k_mem_scatter, hg, dim3(256), 0, s, r0, half, B, bb.pass_of.as<uint32_t>(), mem_off, sorted,
+                       &c[1].rec_used);
+    hipLaunchKernelGGL(k_mem_scatter, hg, dim3(256), 0, s, r1, half, B, bb.pass_of.as<uint32_t>(), mem_off, sorted,
+                       &c[2].rec_used);
+    HSFM_HIP(hipGetLastError());
+    const dim3 cg((uint32_t)((bb.hit_cap + 255) / 256));
+    hipLaunchKernelGGL(k_mem_npos, cg, dim3(256), 0, s, sorted, bb.hit_cap, max_hits, n_refs ? 1u : 0u,
+                       bb.npos.as<uint32_t>(), mem_off + B);
+    if ((rc = batch_scan(idx, bb, bb.npos.as<uint32_t>(), bb.pos_off.as<uint64_t>(), bb.hit_cap))) return rc;
+    BatchMisc *misc = bb.misc.as<BatchMisc>();
+    hipLaunchKernelGGL(k_mem_hits, cg, dim3(256), 0, s, sorted, bb.hit_cap, max_hits, bb.pos_off.as<uint64_t>(),
+                       bb.hits.as<hsfm_mem_hit>(), bb.rows.as<uint32_t>(), mem_off + B, bb.pos_cap, &misc->pos_ovf);
+    hipLaunchKernelGGL(k_npos_total, dim3(1), dim3(1), 0, s, mem_off, B, bb.pos_off.as<uint64_t>(), bb.pos_cap,
+                       reinterpret_cast<uint64_t *>(&misc->npos));
+    HSFM_HIP(hipGetLastError());
+    if (n_refs)
+        if ((rc = launch_resolve_fan(idx, idx->fwd.view, bb.rows.as<uint32_t>(), bb.pos_cap,
+                                     reinterpret_cast<const uint64_t *>(&misc->npos), d_refb, n_refs,
+                                     bb.positions.as<uint32_t>(), &misc->guard, rpt, idx->dev->num_cus * 8)))
+            return rc;
+    return HSFM_OK;
+}
+
+struct BatchGraph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    void release() {
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+        x = nullptr;
+        g = nullptr;
+    }
+};
+
+static int batch_capture(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, const uint8_t *d_q, uint32_t min_len,
+                         const uint32_t *d_refb, uint32_t n_refs, uint32_t max_hits, int main_kernel, int rpt,
+                         BatchGraph &out) {
+    hipStream_t s = idx->dev->stream;
+    HSFM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = batch_record(idx, bb, sv, d_q, min_len, d_refb, n_refs, max_hits, main_kernel, rpt);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return set_error(HSFM_EDEVICE, "graph capture: %s", hipGetErrorString(e));
+    out.g = g;
+    HSFM_HIP(hipGraphInstantiate(&out.x, g, nullptr, nullptr, 0));
+    return HSFM_OK;
+}
+
+// Stage batch [r0, r0 + nb) into the graph's offset buffer.
+static int batch_stage(hsfm_index *idx, BatchBufs &bb, const uint64_t *d_qoff, uint32_t r0, uint32_t nb) {
+    hipStream_t s = idx->dev->stream;
+    HSFM_HIP(hipMemcpyAsync(bb.bq_off.p, d_qoff + r0, ((size_t)nb + 1) * 8, hipMemcpyDeviceToDevice, s));
+    if (nb < bb.B)
+        hipLaunchKernelGGL(k_pad_qoff, dim3((bb.B - nb + 255) / 256), dim3(256), 0, s, bb.bq_off.as<uint64_t>(), nb,
+                           bb.B);
+    HSFM_HIP(hipGetLastError());
+    return HSFM_OK;
+}
+
+} // namespace hsfm
+
+extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const uint64_t *d_qoff, uint32_t nq,
+                                      uint32_t min_len, int mode, const uint32_t *d_refb, uint32_t n_refs,
+                                      uint32_t max_hits, const hsfm_batch_config *cfg, hsfm_batch_fn fn, void *user,
+                                      hsfm_batch_stats *st) {
+    if (!idx || !cfg || (nq && (!d_q || !d_qoff)) || (n_refs && !d_refb))
+        return set_error(HSFM_EINVAL, "hsfm_find_mems_batched: null argument");
+    if (mode != HSFM_MODE_SMEM && mode != HSFM_MODE_MEM) return set_error(HSFM_EINVAL, "bad mode %d", mode);
+    if (!idx->has_rev) return set_error(HSFM_EINVAL, "MEM search needs an index uploaded with its reverse index");
+    if (cfg->batch_reads == 0) return set_error(HSFM_EINVAL, "batch_reads must be positive");
+    const uint32_t F = 12;
+    if (min_len <= F) return set_error(HSFM_EINVAL, "the batched pipeline needs min_len > %u", F);
+    std::unique_lock<std::mutex> lk(idx->mu);
+    HSFM_HIP(hipSetDevice(idx->dev->ordinal));
+    hipStream_t s = idx->dev->stream;
+    int rc;
+    hsfm_batch_stats stats{};
+    if ((rc = build_seed_table(idx, F))) return rc;
+    SeedView sv{idx->seed_off.as<uint64_t>(), idx->seed_iv.as<uint4>(), idx->seed_lsl.as<uint8_t>(), F};
+    // capacities from the host copy of the offsets
+    std::vector<uint64_t> hoff((size_t)nq + 1);
+    HSFM_HIP(hipMemcpyAsync(hoff.data(), d_qoff, ((size_t)nq + 1) * 8, hipMemcpyDeviceToHost, s));
+    HSFM_HIP(hipStreamSynchronize(s));
+    const uint32_t B = cfg->batch_reads < nq ? cfg->batch_reads : (nq ? nq : 1);
+    uint64_t max_slots = 0;
+    for (uint32_t r0 = 0; r0 < nq; r0 += B) {
+        const uint32_t r1 = r0 + B < nq ? r0 + B : nq;
+        const uint64_t sl = (hoff[r1] >> 3) + 8ull * B + 8; // slot indices reach (qoff >> 3) + 8 * local read
+        max_slots = sl > max_slots ? sl : max_slots;
+    }
+    BatchBufs bb;
+    bb.B = B;
+    bb.slot_cap = max_slots;
+    bb.rec_cap = 2ull * (16ull * B + 65536);
+    uint64_t max_batch_slots = 0;
+    for (uint32_t r0 = 0; r0 < nq; r0 += B) {
+        const uint32_t r1 = r0 + B < nq ? r0 + B : nq;
+        const uint64_t v = (hoff[r1] >> 3) - (hoff[r0] >> 3) + 8ull * B + 8;
+        max_batch_slots = v > max_batch_slots ? v : max_batch_slots;
+    }
+    bb.hit_cap = max_batch_slots + bb.rec_cap;
+    bb.pos_cap = 4 * bb.hit_cap + 65536;
+    {
+        size_t t1 = 0, t2 = 0;
+        hipcub::TransformInputIterator<uint64_t, ToU64, const uint32_t *> it(nullptr, ToU64());
+        HSFM_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, t1, it, (uint64_t *)nullptr, (int)B, s));
+        HSFM_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, t2, it, (uint64_t *)nullptr, (int)bb.hit_cap, s));
+        bb.tmp_bytes = t1 > t2 ? t1 : t2;
+    }
+    if ((rc = bb.bq_off.ensure(((size_t)B + 1) * 8)) || (rc = bb.slots.ensure(bb.slot_cap * 16)) ||
+        (rc = bb.mem_cnt.ensure((size_t)B * 4 + 4)) || (rc = bb.status.ensure((size_t)B + 8)) ||
+        (rc = bb.pass_of.ensure((size_t)B * 4 + 4)) || (rc = bb.l1.ensure((size_t)B * 4 + 4)) ||
+        (rc = bb.l2.ensure((size_t)B * 4 + 4)) || (rc = bb.l3.ensure((size_t)B * 4 + 4)) ||
+        (rc = bb.ctr.ensure(3 * sizeof(MemCounters))) || (rc = bb.recs.ensure(bb.rec_cap * sizeof(MemRec))) ||
+        (rc = bb.wsets.ensure((size_t)kBatchGlobalWaves * 2 * kBatchGlobalCap * 16)) ||
+        (rc = bb.mem_off.ensure(((size_t)B + 1) * 8)) || (rc = bb.sorted.ensure(bb.hit_cap * 16)) ||
+        (rc = bb.npos.ensure(bb.hit_cap * 4 + 4)) || (rc = bb.pos_off.ensure((bb.hit_cap + 1) * 8)) ||
+        (rc = bb.hits.ensure(bb.hit_cap * sizeof(hsfm_mem_hit))) || (rc = bb.rows.ensure(bb.pos_cap * 4)) ||
+        (rc = bb.positions.ensure(bb.pos_cap * 8)) || (rc = bb.misc.ensure(sizeof(BatchMisc))) ||
+        (rc = bb.tmp.ensure(bb.tmp_bytes + 256))) {
+        bb.release();
+        return rc;
+    }
+    hipEvent_t e0, e1;
+    HSFM_HIP(hipEventCreate(&e0));
+    HSFM_HIP(hipEventCreate(&e1));
+    auto cleanup = [&](int code) {
+        (void)hipStreamSynchronize(s);
+        bb.release();
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return code;
+    };
+    // configuration: explicit, cached, or tuned on the first batch
+    int main_kernel = cfg->main_kernel, rpt = cfg->resolve_fanout;
+    const bool cached = idx->tuned_main >= 0 && idx->tuned_min_len == min_len && idx->tuned_batch == B;
+    if (main_kernel < 0 || rpt <= 0) {
+        if (cached && !cfg->autotune) {
+            main_kernel = main_kernel < 0 ? idx->tuned_main : main_kernel;
+            rpt = rpt <= 0 ? idx->tuned_rpt : rpt;
+        } else if (cfg->autotune && nq) {
+            HSFM_HIP(hipEventRecord(e0, s));
+            const uint32_t nb0 = B < nq ? B : nq;
+            auto time_cfg = [&](int mk, int rp, double &ms) -> int {
+                BatchGraph gr;
+                int r2 = batch_capture(idx, bb, sv, d_q, min_len, d_refb, n_refs, max_hits, mk, rp, gr);
+                if (r2) return r2;
+                ms = 1e30;
+                for (int rep = 0; rep < 2; rep++) {
+                    if ((r2 = batch_stage(idx, bb, d_qoff, 0, nb0))) break;
+                    hipEvent_t a, b;
+                    HSFM_HIP(hipEventCreate(&a));
+                    HSFM_HIP(hipEventCreate(&b));
+                    HSFM_HIP(hipEventRecord(a, s));
+                    HSFM_HIP(hipGraphLaunch(gr.x, s));
+                    HSFM_HIP(hipEventRecord(b, s));
+                    float t = 0.f;
+                    HSFM_HIP(hipEventSynchronize(b));
+                    HSFM_HIP(hipEventElapsedTime(&t, a, b));
+                    (void)hipEventDestroy(a);
+                    (void)hipEventDestroy(b);
+                    if (rep) ms = t;
+                }
+                gr.release();
+                return r2;
+            };
+            double best = 1e30;
+            int bm = BM_PERSIST16;
+            for (int mk = 0; mk < BM_COUNT; mk++) {
+                double ms;
+                if ((rc = time_cfg(mk, 1, ms))) return cleanup(rc);
+                stats.tune_ms[mk] = ms;
+                if (ms < best) { best = ms; bm = mk; }
+            }
+            int br = 1;
+            const int rps[2] = {2, 4};
+            for (int i = 0; i < 2; i++) {
+                double ms;
+                if ((rc = time_cfg(bm, rps[i], ms))) return cleanup(rc);
+                stats.tune_ms[BM_COUNT + i] = ms;
+                if (ms < best) { best = ms; br = rps[i]; }
+            }
+            main_kernel = cfg->main_kernel >= 0 ? cfg->main_kernel : bm;
+            rpt = cfg->resolve_fanout > 0 ? cfg->resolve_fanout : br;
+            idx->tuned_main = bm;
+            idx->tuned_rpt = br;
+            idx->tuned_min_len = min_len;
+            idx->tuned_batch = B;
+            HSFM_HIP(hipEventRecord(e1, s));
+            HSFM_HIP(hipEventSynchronize(e1));
+            float t = 0.f;
+            HSFM_HIP(hipEventElapsedTime(&t, e0, e1));
+            stats.tune_total_ms = t;
+        } else {
+            main_kernel = main_kernel < 0 ? BM_PERSIST16 : main_kernel;
+            rpt = rpt <= 0 ? 1 : rpt;
+        }
+    }
+    if (main_kernel < 0 || main_kernel >= BM_COUNT || (rpt != 1 && rpt != 2 && rpt != 4))
+        return cleanup(set_error(HSFM_EINVAL, "bad batch configuration (main %d, fan-out %d)", main_kernel, rpt));
+    stats.main_kernel = main_kernel;
+    stats.resolve_fanout = rpt;
+    BatchGraph gr;
+    if (cfg->use_graph && (rc = batch_capture(idx, bb, sv, d_q, min_len, d_refb, n_refs, max_hits, main_kernel, rpt,
+                                              gr)))
+        return cleanup(rc);
+    HSFM_HIP(hipEventRecord(e0, s));
+    for (uint32_t r0 = 0; r0 < nq; r0 += B) {
+        const uint32_t nb = r0 + B < nq ? B : nq - r0;
+        if ((rc = batch_stage(idx, bb, d_qoff, r0, nb))) break;
+        if (cfg->use_graph) {
+            if (hipGraphLaunch(gr.x, s) != hipSuccess) {
+                rc = set_error(HSFM_EDEVICE, "hipGraphLaunch failed");
+                break;
+            }
+        } else if ((rc = batch_record(idx, bb, sv, d_q, min_len, d_refb, n_refs, max_hits, main_kernel, rpt))) {
+            break;
+        }
+        // one wait per batch: counts and overflow flags
+        uint64_t h[8];
+        MemCounters *c = bb.ctr.as<MemCounters>();
+        const uint64_t *m = bb.misc.as<uint64_t>();
+        if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(&c[2].n_retry), h, 1)) ||
+            (rc = readback_u64(idx, m, h + 1, 3)) ||
+            (rc = readback_u64(idx, bb.mem_off.as<uint64_t>() + B, h + 4, 1)))
+            break;
+        const uint64_t late = h[0], pos_ovf = h[1], npos = h[2], guard = h[3], nhits = h[4];
+        if (guard) {
+            rc = set_error(HSFM_EINDEX, "%llu LF walks exceeded the sample rate: index is inconsistent",
+                           (unsigned long long)guard);
+            break;
+        }
+        stats.batches++;
+        hsfm_mem_device_result res{};
+        if (late || pos_ovf || nhits > bb.hit_cap) {
+            // outgrew the in-graph capacities: this batch through the host-driven path
+            stats.fallback_batches++;
+            lk.unlock();
+            rc = hsfm_find_mems_device(idx, d_q, d_qoff + r0, nb, min_len, mode, d_refb, n_refs, max_hits, &res);
+            lk.lock();
+            if (rc) break;
+        } else {
+            res.num_queries = nb;
+            res.num_hits = nhits;
+            res.num_positions = n_refs ? npos : 0;
+            res.d_hit_offsets = bb.mem_off.as<uint64_t>();
+            res.d_hits = bb.hits.as<hsfm_mem_hit>();
+            res.d_positions = bb.positions.as<uint32_t>();
+        }
+        stats.records += res.num_hits;
+        stats.positions += res.num_positions;
+        if (fn) fn(user, r0, nb, &res);
+    }
+    HSFM_HIP(hipEventRecord(e1, s));
+    HSFM_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    HSFM_HIP(hipEventElapsedTime(&t, e0, e1));
+    stats.total_ms = t;
+    gr.release();
+    if (st) *st = stats;
+    return cleanup(rc);
+}
+
+extern "C" int hsfm_device_alloc(hsfm_device *dev, uint64_t bytes, void **out) {
+    if (!dev || !out) return set_error(HSFM_EINVAL, "hsfm_device_alloc: null argument");
+    HSFM_HIP(hipSetDevice(dev->ordinal));
+    *out = nullptr;
+    if (bytes && hipMalloc(out, bytes) != hipSuccess)
+        return set_error(HSFM_ENOMEM, "hipMalloc of %llu bytes failed", (unsigned long long)bytes);
+    return HSFM_OK;
+}
+extern "C" int hsfm_device_free(hsfm_device *dev, void *p) {
+    if (!dev) return set_error(HSFM_EINVAL, "hsfm_device_free: null device");
+    HSFM_HIP(hipSetDevice(dev->ordinal));
+    if (p) HSFM_HIP(hipFree(p));
+    return HSFM_OK;
+}
+extern "C" int hsfm_host_to_device(hsfm_device *dev, void *d_dst, const void *src, uint64_t bytes) {
+    if (!dev || (bytes && (!d_dst || !src))) return set_error(HSFM_EINVAL, "hsfm_host_to_device: null argument");
+    if (!bytes) return HSFM_OK;
+    HSFM_HIP(hipSetDevice(dev->ordinal));
+    HSFM_HIP(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, dev->stream));
+    HSFM_HIP(hipStreamSynchronize(dev->stream));
+    return HSFM_OK;
+}
+extern "C" int hsfm_device_to_host(hsfm_device *dev, void *dst, const void *d_src, uint64_t bytes) {
+    if (!dev || (bytes && (!dst || !d_src))) return set_error(HSFM_EINVAL, "hsfm_device_to_host: null argument");
+    if (!bytes) return HSFM_OK;
+    HSFM_HIP(hipSetDevice(dev->ordinal));
+    HSFM_HIP(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, dev->stream));
+    HSFM_HIP(hipStreamSynchronize(dev->stream));
     return HSFM_OK;
 }

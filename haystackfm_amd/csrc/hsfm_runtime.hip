@@ -165,6 +165,101 @@ __global__ __launch_bounds__(256) void k_resolve(DevDir d, const DevConst *__res
     out[2 * t + 1] = o;
 }
 
+// Resolve with a tunable fan-out: each thread walks RPT rows in lockstep
+// (independent chains, more loads in flight per thread), grid-stride over
+// rows[0, min(cap, *d_nrows)), so a fixed launch serves any row count.
+template <int RPT>
+__global__ __launch_bounds__(256) void k_resolve_fan(DevDir d, const DevConst *__restrict__ cst,
+                                                     const uint32_t *__restrict__ rows, uint64_t cap,
+                                                     const uint64_t *__restrict__ d_nrows,
+                                                     const uint32_t *__restrict__ bounds, uint32_t nb,
+                                                     uint32_t *__restrict__ out, unsigned long long *__restrict__ guard) {
+    __shared__ uint32_t s_clane[16];
+    if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
+    __syncthreads();
+    const uint64_t nrows = d_nrows ? (*d_nrows < cap ? *d_nrows : cap) : cap;
+    const uint32_t nm1 = d.n - 1;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * RPT;
+    for (uint64_t base = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * RPT; base < nrows; base += stride) {
+        uint32_t row[RPT], steps[RPT], pos[RPT];
+        bool live[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; j++) {
+            live[j] = base + j < nrows;
+            const uint32_t r0 = live[j] ? rows[base + j] : 0u;
+            row[j] = r0 < nm1 ? r0 : nm1;
+            steps[j] = 0;
+            pos[j] = 0xffffffffu;
+        }
+        bool any = true;
+        while (any) {
+            any = false;
+#pragma unroll
+            for (int j = 0; j < RPT; j++) {
+                if (!live[j]) continue;
+                const uint4 w = d.samp[row[j] >> 6];
+                RecView r;
+                load_rec(d, row[j], r);
+                const uint64_t bits = (uint64_t)w.x | ((uint64_t)w.y << 32);
+                const uint32_t off = row[j] & 63;
+                if ((bits >> off) & 1ull) {
+                    uint32_t rk = w.z + popc64(bits & ((1ull << off) - 1));
+                    rk = rk < d.num_samples ? rk : d.num_samples - 1;
+                    pos[j] = d.sa_vals[rk] + steps[j];
+                    live[j] = false;
+                    continue;
+                }
+                if (steps[j] >= d.sample_rate) {
+                    atomicAdd(guard, 1ull);
+                    live[j] = false;
+                    continue;
+                }
+                uint32_t lane, rank;
+                lf_lane_rank(r, row[j], lane, rank);
+                const uint32_t nx = s_clane[lane] + rank;
+                row[j] = nx < nm1 ? nx : nm1;
+                steps[j]++;
+                any = true;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < RPT; j++) {
+            if (base + j >= nrows) continue;
+            uint32_t id = 0xffffffffu, o = 0xffffffffu;
+            if (pos[j] != 0xffffffffu) {
+                uint32_t lo = 0, hi = nb;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (bounds[mid] <= pos[j]) lo = mid + 1;
+                    else hi = mid;
+                }
+                id = lo;
+                o = pos[j] - (lo ? bounds[lo - 1] : 0u);
+            }
+            out[2 * (base + j)] = id;
+            out[2 * (base + j) + 1] = o;
+        }
+    }
+}
+
+int launch_resolve_fan(hsfm_index *idx, const DevDir &d, const uint32_t *d_rows, uint64_t cap,
+                       const uint64_t *d_nrows, const uint32_t *d_bounds, uint32_t nbounds, uint32_t *d_out_pairs,
+                       unsigned long long *d_guard, int rpt, uint32_t grid) {
+    hipStream_t s = idx->dev->stream;
+    const DevConst *c = idx->d_const.as<DevConst>();
+    if (rpt == 4)
+        hipLaunchKernelGGL(k_resolve_fan<4>, dim3(grid), dim3(256), 0, s, d, c, d_rows, cap, d_nrows, d_bounds,
+                           nbounds, d_out_pairs, d_guard);
+    else if (rpt == 2)
+        hipLaunchKernelGGL(k_resolve_fan<2>, dim3(grid), dim3(256), 0, s, d, c, d_rows, cap, d_nrows, d_bounds,
+                           nbounds, d_out_pairs, d_guard);
+    else
+        hipLaunchKernelGGL(k_resolve_fan<1>, dim3(grid), dim3(256), 0, s, d, c, d_rows, cap, d_nrows, d_bounds,
+                           nbounds, d_out_pairs, d_guard);
+    HSFM_HIP(hipGetLastError());
+    return HSFM_OK;
+}
+
 int launch_resolve(hsfm_index *idx, const DevDir &d, const uint32_t *d_rows, uint64_t nrows,
                    const uint32_t *d_bounds, uint32_t nbounds, uint32_t *d_out_pairs,
                    unsigned long long *d_guard) {

@@ -164,6 +164,46 @@ int hsfm_find_mems_device(hsfm_index *idx, const uint8_t *d_qflat, const uint64_
                           const uint32_t *d_ref_boundaries, uint32_t n_refs, uint32_t max_hits,
                           hsfm_mem_device_result *out);
 
+/* ---- batched MEM pipeline (uneven reads) ----------------------------------
+ * The same results as hsfm_find_mems_device, produced batch by batch:
+ * reads [r0, r0 + batch_reads) run through one captured HIP graph (find,
+ * second-level passes, positions, resolve, reference map) and fn receives the
+ * batch's device-resident result (hit offsets relative to the batch, valid
+ * only during the call).  Reference analogue: the query chunking of
+ * resolve_mem_hits_gpu (bidir_index.rs:418-439) under output_budget_u32
+ * (gpu/mod.rs:68-82).  Needs min_len > 12 (seed-table restarts). */
+typedef struct {
+    uint32_t batch_reads;  /* reads per batch (65536)                          */
+    int autotune;          /* time the candidate configurations on batch 0     */
+    int main_kernel;       /* -1 = tuned/cached/default; 0..2 thread per read
+                              with 8/16/32-interval LDS sets, 3 wave per read  */
+    int resolve_fanout;    /* 0 = tuned/cached/default; else 1, 2 or 4 walks
+                              per thread                                       */
+    int use_graph;         /* capture each batch into one HIP graph            */
+} hsfm_batch_config;
+typedef struct {
+    uint32_t batches, fallback_batches; /* batches, those re-run host-driven   */
+    int main_kernel, resolve_fanout;    /* configuration used                  */
+    uint64_t records, positions;
+    double total_ms;                    /* HIP events around the batch loop    */
+    double tune_total_ms;
+    double tune_ms[6];  /* batch-0 time of main kernels 0..3 (fan-out 1), then
+                           the best main kernel with fan-out 2 and 4           */
+} hsfm_batch_stats;
+typedef void (*hsfm_batch_fn)(void *user, uint32_t first_read, uint32_t num_reads,
+                              const hsfm_mem_device_result *result);
+int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_qflat, const uint64_t *d_qoff,
+                           uint32_t nq, uint32_t min_len, int mode,
+                           const uint32_t *d_ref_boundaries, uint32_t n_refs, uint32_t max_hits,
+                           const hsfm_batch_config *cfg, hsfm_batch_fn fn, void *user,
+                           hsfm_batch_stats *stats);
+/* Device memory helpers for callers without a device allocator of their own
+ * (inputs of the *_device entry points, results inside a batch callback). */
+int hsfm_device_alloc(hsfm_device *dev, uint64_t bytes, void **out);
+int hsfm_device_free(hsfm_device *dev, void *d_ptr);
+int hsfm_host_to_device(hsfm_device *dev, void *d_dst, const void *src, uint64_t bytes);
+int hsfm_device_to_host(hsfm_device *dev, void *dst, const void *d_src, uint64_t bytes);
+
 /* ---- misc ---------------------------------------------------------------- */
 void hsfm_free_result(void *result);
 const char *hsfm_last_error(void);

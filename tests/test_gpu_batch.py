@@ -1,0 +1,78 @@
+"""GPU checks of the batched (graph-captured) MEM pipeline for uneven reads:
+every configuration returns exactly what the one-shot entry point returns."""
+import numpy as np
+import pytest
+
+import haystackfm_amd as hf
+from oracle import oracle as O
+from tests.helpers import random_text
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def uneven():
+    text, bounds = O.gen_reference(21, 400_000, 4, gc=0.41, repeat_frac=0.2, n_units=30, unit_min=300,
+                                   unit_max=3000, iupac_frac=0.005)
+    o = O.OracleBidir.build(text=text, bounds=bounds, sample_rate=32, threads=8)
+    g = hf.BidirFmIndex.from_bytes(o.to_bytes())
+    flat, off = O.gen_reads(22, text, bounds, 3000, 150, sigma=1.0, len_min=32, len_max=3000, sub=0.015,
+                            both_strands=True)
+    return o, g, (flat, off), bounds
+
+
+def _same(a, b):
+    assert np.array_equal(a[0], b[0])
+    assert np.array_equal(a[1], b[1])
+    assert np.array_equal(a[2], b[2])
+
+
+@pytest.mark.parametrize("main,fan,graph", [(1, 1, True), (0, 2, True), (2, 4, True), (3, 1, True), (1, 2, False)])
+def test_batched_matches_one_shot(uneven, main, fan, graph):
+    o, g, (flat, off), bounds = uneven
+    ref = g.find_mems_batch((flat, off), 19, hf.HSFM_MODE_SMEM, bounds, 1024)
+    got = g.find_mems_batched((flat, off), 19, hf.HSFM_MODE_SMEM, bounds, 1024, batch_reads=700,
+                              main_kernel=main, resolve_fanout=fan, use_graph=graph)
+    _same(got[:3], ref)
+    st = got[3]
+    assert st["batches"] == 5 and st["main_kernel"] == main and st["resolve_fanout"] == fan
+    assert st["records"] == len(ref[1]) and st["positions"] == len(ref[2])
+
+
+def test_batched_matches_oracle_sample(uneven):
+    o, g, (flat, off), bounds = uneven
+    n = 150
+    sub = (flat[:int(off[n])], off[:n + 1])
+    got = g.find_mems_batched(sub, 19, hf.HSFM_MODE_MEM, bounds, 64, batch_reads=64)
+    exp = o.find_mems_batch(qflat=sub[0], qoff=sub[1], min_len=19, mode=1, threads=4)
+    ho, hits = got[0], got[1]
+    for q, em in enumerate(exp):
+        spans = sorted({(int(h[0]), int(h[1])) for h in hits[ho[q]:ho[q + 1]]})
+        assert spans == sorted({(m.query_start, m.query_end) for m in em})
+
+
+def test_autotune_picks_and_caches(uneven):
+    _, g, (flat, off), bounds = uneven
+    ref = g.find_mems_batch((flat, off), 19, hf.HSFM_MODE_SMEM, bounds, 1024)
+    got = g.find_mems_batched((flat, off), 19, hf.HSFM_MODE_SMEM, bounds, 1024, batch_reads=1000, autotune=True)
+    _same(got[:3], ref)
+    st = got[3]
+    assert 0 <= st["main_kernel"] < 4 and st["resolve_fanout"] in (1, 2, 4)
+    assert all(t > 0 for t in st["tune_ms"])
+    again = g.find_mems_batched((flat, off), 19, hf.HSFM_MODE_SMEM, bounds, 1024, batch_reads=1000)
+    assert (again[3]["main_kernel"], again[3]["resolve_fanout"]) == (st["main_kernel"], st["resolve_fanout"])
+    _same(again[:3], ref)
+
+
+def test_oversized_batch_runs_host_driven():
+    """Reads of N over an IUPAC-rich text make sets beyond the in-graph global
+    capacity: their batch re-runs on the host-driven path, still exact."""
+    rng = np.random.default_rng(8)
+    seqs = [O.decode(random_text(rng, 120_000, iupac_frac=0.3))]
+    o = O.OracleBidir.build(seqs, sample_rate=16, threads=8)
+    g = hf.BidirFmIndex.from_bytes(o.to_bytes())
+    qs = [random_text(rng, 60, iupac_frac=0.01) for _ in range(20)] + [np.full(40, 5, np.uint8)]
+    ref = g.find_mems_batch(qs, 13, hf.HSFM_MODE_SMEM, [], 16)
+    got = g.find_mems_batched(qs, 13, hf.HSFM_MODE_SMEM, [], 16, batch_reads=8)
+    _same(got[:3], ref)
+    assert got[3]["fallback_batches"] >= 1
