@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cache", default=os.environ.get("HSFM_BENCH_CACHE", "/tmp/hsfm_bench_cache"))
+    ap.add_argument("--uneven-reads", type=int, default=2_000_000,
+                    help="reads of the uneven-length batched leg (0 skips it)")
     ap.add_argument("--seed-keys", type=int, default=4_000_000,
                     help="read 21-mers for the learned-seed vs plain-search leg (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -67,7 +69,8 @@ def parse():
 def workload(args):
     """Reference, bidirectional index bytes and read sets (cached on disk)."""
     from haystackfm_amd import index_build as IB
-    key = hashlib.sha1(f"v1-{args.ref_bp}-{args.reads}-{args.random_reads}-{args.locate_reads}".encode()).hexdigest()[:12]
+    key = hashlib.sha1(f"v2-{args.ref_bp}-{args.reads}-{args.random_reads}-{args.locate_reads}-{args.uneven_reads}"
+                       .encode()).hexdigest()[:12]
     os.makedirs(args.cache, exist_ok=True)
     path = os.path.join(args.cache, f"wl_{key}.npz")
     if os.path.exists(path):
@@ -85,7 +88,10 @@ def workload(args):
     sq, so = IB.synth_reads(0x5EED3, text, bounds, args.reads, 150, sub=0.02, indel=0.002, both_strands=True,
                             n_random=args.random_reads, gc=0.41, shuffle=True)
     lq, lo = IB.synth_reads(0x5EED2, text, bounds, args.locate_reads, 150, sub=0.01, both_strands=True)
-    d = dict(text=text, bounds=bounds, index=np.frombuffer(idx, np.uint8), sq=sq, so=so, lq=lq, lo=lo)
+    # uneven reads: lognormal(ln 150, 1.0) lengths clipped to [32, 10000], 1.5% substitutions, shuffled
+    uq, uo = IB.synth_reads(0x5EED4, text, bounds, args.uneven_reads, 150, sigma=1.0, len_min=32, len_max=10000,
+                            sub=0.015, both_strands=True, shuffle=True)
+    d = dict(text=text, bounds=bounds, index=np.frombuffer(idx, np.uint8), sq=sq, so=so, lq=lq, lo=lo, uq=uq, uo=uo)
     np.savez(path, **d)
     return d
 
@@ -190,6 +196,44 @@ def read_kmer_keys(flat, off, k, limit):
     if len(keys) > limit:
         keys = keys[np.linspace(0, len(keys) - 1, limit).astype(np.int64)]
     return keys
+
+
+def uneven_leg(bidir, wl, args):
+    """Uneven-length reads through the batched pipeline: batches of 64k reads,
+    one HIP graph per batch, configuration picked by the autotuner on the
+    first call; timed calls reuse it."""
+    import torch
+    import haystackfm_amd as hf
+    flat, off = wl["uq"], wl["uo"]
+    nq = len(off) - 1
+    bounds = wl["bounds"]
+    d_q = torch.from_numpy(flat).cuda()
+    d_o = torch.from_numpy(off.astype(np.int64)).cuda()
+    d_rb = torch.from_numpy(bounds.astype(np.int32)).cuda()
+    torch.cuda.synchronize()
+    run = lambda tune: bidir.find_mems_batched_device(d_q.data_ptr(), d_o.data_ptr(), nq, 19, hf.HSFM_MODE_SMEM,
+                                                      d_rb.data_ptr(), len(bounds), 1024, batch_reads=65536,
+                                                      autotune=tune, use_graph=True)
+    t0 = time.perf_counter()
+    tuned = run(True)
+    tune_wall = time.perf_counter() - t0
+    walls, stats = [], None
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        stats = run(False)
+        walls.append(time.perf_counter() - t0)
+    w = float(np.mean(walls))
+    log(f"uneven: {nq} reads, {w * 1e3:.1f} ms/pass, stats {stats}, tuning {tuned['tune_ms']}")
+    return {"reads": nq, "mean_len": round(float(off[-1]) / nq, 1), "min_len": 19, "batch_reads": 65536,
+            "value": round(nq / w, 1), "unit": "reads/s", "ms_per_pass": round(w * 1e3, 2),
+            "graph_ms": round(stats["total_ms"], 2), "batches": stats["batches"],
+            "fallback_batches": stats["fallback_batches"], "records": stats["records"],
+            "positions": stats["positions"],
+            "tuned": {"main_kernel": ["thread/read ring 8", "thread/read ring 16", "thread/read ring 32",
+                                      "wave/read"][stats["main_kernel"]],
+                      "resolve_fanout": stats["resolve_fanout"],
+                      "batch0_ms": [round(x, 3) for x in tuned["tune_ms"]],
+                      "tuning_wall_s": round(tune_wall, 2)}}
 
 
 def learned_seed_leg(bidir, flat, off, args):
@@ -306,6 +350,7 @@ def main():
     nls = min(args.check_reads * 5, nl)
     lsample = bidir.fwd.locate_batch((wl["lq"][:int(wl["lo"][nls])], wl["lo"][:nls + 1]))
     lseed = learned_seed_leg(bidir, wl["sq"], wl["so"], args) if args.seed_keys else None
+    uneven = uneven_leg(bidir, wl, args) if args.uneven_reads else None
     cpu = cpu_baseline(idx_bytes, wl["sq"], wl["so"], wl["lq"], wl["lo"], bounds, args, sample, lsample)
 
     value = nq / smem_s
@@ -362,6 +407,7 @@ def main():
         "reference_algorithm_probes_per_read": cpu.get("smem_probes_per_read"),
         "learned_seed_speedup": lseed["speedup"] if lseed else None,
         "learned_seed": lseed,
+        "uneven_batched": uneven,
     }
     print(json.dumps(out), flush=True)
 

@@ -638,6 +638,21 @@ class BidirFmIndex:
         return (np.concatenate(ho_all), np.concatenate(hits_all) if hits_all else np.zeros((0, 8), np.uint32),
                 np.concatenate(pos_all) if pos_all else np.zeros((0, 2), np.uint32), stats)
 
+    def find_mems_batched_device(self, d_qflat: int, d_qoff: int, nq: int, min_len: int, mode: int,
+                                 d_ref_boundaries: int, n_refs: int, max_hits_per_mem: int = 1024,
+                                 batch_reads: int = 65536, autotune: bool = False, main_kernel: int = -1,
+                                 resolve_fanout: int = 0, use_graph: bool = True, on_batch=None) -> dict:
+        """Batched pipeline on device-resident reads; on_batch(first, n, result)
+        receives each batch's device result (None: results are dropped)."""
+        cfg = _BatchConfig(batch_reads, int(autotune), main_kernel, resolve_fanout, int(use_graph))
+        st = _BatchStats()
+        cb = _BATCH_FN(lambda _u, first, n, r: on_batch(first, n, r.contents)) if on_batch else _BATCH_FN()
+        _check(_lib().hsfm_find_mems_batched(self._handle(), C.c_void_p(d_qflat), C.c_void_p(d_qoff), nq,
+                                             int(min_len), mode, C.c_void_p(d_ref_boundaries or None), n_refs,
+                                             int(max_hits_per_mem), C.byref(cfg), cb, None, C.byref(st)),
+               "find_mems_batched")
+        return {k: (list(getattr(st, k)) if k == "tune_ms" else getattr(st, k)) for k, _ in _BatchStats._fields_}
+
     def _mem_hits(self, queries, min_len, mode, ref_boundaries, max_hits_per_mem) -> list[list[MemHit]]:
         ho, hits, pos = self.find_mems_batch(queries, min_len, mode, ref_boundaries, max_hits_per_mem)
         out = []

@@ -52,6 +52,7 @@ struct Workspace {
     DevBuf scan_tmp;
     DevBuf mem_recs, mem_cnt, mem_off, mem_sorted, mem_hits, pos_off, positions, mem_flags, read_list;
     DevBuf wave_sets, mem_slots, mem_status;
+    DevBuf items, item_cnt, item_first, read_off; // MEM work items (read segments)
     uint64_t *h_pinned = nullptr;             // small readbacks
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}; // main, resolve, fallback kernels
 };

@@ -132,7 +132,8 @@ template <bool LDS, int TB>
 __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const DevConst *__restrict__ cst,
                                                  SeedView sv, const uint8_t *__restrict__ q,
                                                  const uint64_t *__restrict__ qoff,
-                                                 const uint32_t *__restrict__ list, uint32_t nlist, uint32_t min_len,
+                                                 const uint32_t *__restrict__ list, uint32_t nlist,
+                                                 const uint4 *__restrict__ items, uint32_t min_len,
                                                  uint32_t cap, uint4 *__restrict__ sets, MemRec *__restrict__ recs,
                                                  uint64_t rec_cap, uint32_t *__restrict__ mem_cnt,
                                                  uint32_t *__restrict__ pass_of, uint32_t pass,
@@ -163,15 +164,18 @@ __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const D
         if (lane == 0) wi = (uint32_t)atomicAdd(&ctr->work, 1ull);
         wi = block_bcast<TB>(wi, &s_b32);
         if (wi >= nlist) break;
-        const uint32_t r = list ? list[wi] : wi;
+        const uint32_t w = list ? list[wi] : wi;
+        const uint4 it = items ? items[w] : make_uint4(w, 0u, 0xffffffffu, 0u);
+        const uint32_t r = it.x;
         const uint64_t b0 = qoff[r];
         const uint32_t m = (uint32_t)(qoff[r + 1] - b0);
         const uint8_t *p = q + b0;
-        uint32_t s0 = 0, E = 0, nrec = 0;
+        const uint32_t ia = it.y, ib = it.z < m ? it.z : m; // starts owned by this item
+        uint32_t s0 = ia, E = 0, nrec = 0;
         bool too_big = false;
         while (min_len && !too_big) {
             const uint32_t j = (s0 + min_len > E + 1) ? s0 + min_len : E + 1;
-            if (j > m) break;
+            if (j > m || s0 >= ib) break;
             uint4 *A = bufA, *B = bufB;
             uint32_t ns = 1, k = j;
             bool seeded = false;
@@ -252,6 +256,12 @@ __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const D
                 e++;
             }
             if (too_big) break;
+            if (k >= ib) break; // starts in a later segment
+            if (k < ia) {       // starts in an earlier segment: not ours, scan on
+                s0 = ia;
+                E = e;
+                continue;
+            }
             // emit (k, e): one record per interval, sorted by fwd_lo
             unsigned long long slot = 0;
             if (lane == 0) slot = atomicAdd(&ctr->rec_used, (unsigned long long)ns);
@@ -270,7 +280,7 @@ __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const D
                 uint32_t rank = 0;
                 for (uint32_t u = 0; u < ns; u++) rank += A[u].x < iv.x ? 1u : 0u;
                 MemRec rc;
-                rc.read = r;
+                rc.read = w;
                 rc.local = nrec + rank;
                 rc.qs = k;
                 rc.qe = e;
@@ -288,10 +298,10 @@ __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const D
         if (lane == 0) {
             if (too_big) {
                 const unsigned long long at = atomicAdd(&ctr->n_retry, 1ull);
-                retry_list[at] = r;
+                retry_list[at] = w;
             } else {
-                mem_cnt[r] = nrec;
-                pass_of[r] = pass;
+                mem_cnt[w] = nrec;
+                pass_of[w] = pass;
             }
         }
         __syncthreads();
@@ -626,6 +636,7 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
                                                     SeedView sv, const uint8_t *__restrict__ q,
                                                     const uint64_t *__restrict__ qoff,
                                                     const uint32_t *__restrict__ list, uint32_t nlist,
+                                                    const uint4 *__restrict__ items,
                                                     uint32_t min_len, uint4 *__restrict__ slots,
                                                     uint32_t *__restrict__ mem_cnt, uint8_t *__restrict__ status,
                                                     uint32_t *__restrict__ defer_list,
@@ -647,7 +658,7 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
     const uint32_t F = sv.F;
     const bool seeding = F != 0 && min_len > F;
 
-    uint32_t st = SM_FETCH, r = 0, m = 0, s0 = 0, E = 0, j = 0, k = 0, e = 0;
+    uint32_t st = SM_FETCH, w = 0, r = 0, m = 0, ia = 0, ib = 0, s0 = 0, E = 0, j = 0, k = 0, e = 0;
     uint32_t ns = 0, nn = 0, t = 0, head = 0, nrec = 0, slot_cap = 0;
     uint32_t xi = 0, lslmax = 0, probes = 0, restarts = 0;
     SeedWin sw{0, 0, 0x111u, 0};
@@ -668,19 +679,24 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
                 base = __shfl(base, (int)leader, 64);
                 const uint32_t wi = base + rank;
                 if (wi >= nlist) { st = SM_DONE; break; }
-                r = list ? list[wi] : wi;
+                w = list ? list[wi] : wi;
+                uint4 it = make_uint4(w, 0u, 0xffffffffu, 0u);
+                if (items) it = items[w];
+                r = it.x;
                 const uint64_t b0 = qoff[r];
                 m = (uint32_t)(qoff[r + 1] - b0);
                 p = q + b0;
-                slot_base = (b0 >> 3) + 8ull * r;
-                slot_cap = (uint32_t)((qoff[r + 1] >> 3) - (b0 >> 3)) + 8u;
-                s0 = 0; E = 0; nrec = 0;
+                ia = it.y;
+                ib = it.z < m ? it.z : m;
+                slot_base = ((b0 + ia) >> 3) + 8ull * w;
+                slot_cap = (uint32_t)(((b0 + ib) >> 3) - ((b0 + ia) >> 3)) + 8u;
+                s0 = ia; E = 0; nrec = 0;
                 st = SM_CAND;
             } else if (st == SM_CAND) {
                 j = (s0 + min_len > E + 1) ? s0 + min_len : E + 1;
-                if (min_len == 0 || j > m) {
-                    status[r] = 0;
-                    mem_cnt[r] = nrec;
+                if (min_len == 0 || j > m || s0 >= ib) { // no MEM starts in [s0, ib)
+                    status[w] = 0;
+                    mem_cnt[w] = nrec;
                     st = SM_FETCH;
                     continue;
                 }
@@ -749,10 +765,22 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
                     st = SM_EMIT;
                 }
             } else { // SM_EMIT
+                if (k >= ib) { // starts in a later segment of the read
+                    status[w] = 0;
+                    mem_cnt[w] = nrec;
+                    st = SM_FETCH;
+                    continue;
+                }
+                if (k < ia) { // starts in an earlier segment: not ours, scan on
+                    s0 = ia;
+                    E = e;
+                    st = SM_CAND;
+                    continue;
+                }
                 if (nrec + ns > slot_cap) {
-                    status[r] = 1;
-                    mem_cnt[r] = 0;
-                    defer_list[atomicAdd(&ctr->n_retry, 1ull)] = r;
+                    status[w] = 1;
+                    mem_cnt[w] = 0;
+                    defer_list[atomicAdd(&ctr->n_retry, 1ull)] = w;
                     atomicAdd(&ctr->diag[2], 1ull);
                     st = SM_FETCH;
                     continue;
@@ -816,9 +844,9 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
 
         // ---- advance ----
         if (ovf) {
-            status[r] = 1;
-            mem_cnt[r] = 0;
-            defer_list[atomicAdd(&ctr->n_retry, 1ull)] = r;
+            status[w] = 1;
+            mem_cnt[w] = 0;
+            defer_list[atomicAdd(&ctr->n_retry, 1ull)] = w;
             atomicAdd(&ctr->diag[st == SM_SEEDE ? 0 : 1], 1ull);
             st = SM_FETCH;
         } else if (st == SM_SEEDE) {
@@ -1175,14 +1203,46 @@ __global__ __launch_bounds__(PB) void k_mem_pipe(DevDir fwd, DevDir rev, const D
 }
 
 // Copy the records of thread-path reads from their slots to the final order.
-__global__ __launch_bounds__(256) void k_mem_gather(uint32_t nq, const uint64_t *__restrict__ qoff,
+__global__ __launch_bounds__(256) void k_mem_gather(uint32_t nw, const uint64_t *__restrict__ qoff,
+                                                    const uint4 *__restrict__ items,
                                                     const uint8_t *__restrict__ status, const uint32_t *__restrict__ mem_cnt,
                                                     const uint64_t *__restrict__ mem_off, const uint4 *__restrict__ slots,
                                                     uint4 *__restrict__ out) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw || status[w] != 0) return;
+    const uint4 it = items ? items[w] : make_uint4(w, 0u, 0u, 0u);
+    const uint64_t base = ((qoff[it.x] + it.y) >> 3) + 8ull * w, o = mem_off[w];
+    for (uint32_t i = 0; i < mem_cnt[w]; i++) out[o + i] = slots[base + i];
+}
+
+// Work items: a read of m codes is split into ceil(m / seg) items, item s
+// owning the MEMs that start in [s * seg, (s + 1) * seg); every read has at
+// least one item.  Count pass, then fill at the scanned offsets.
+__global__ void k_items_count(uint32_t nq, const uint64_t *__restrict__ qoff, uint32_t seg,
+                              uint32_t *__restrict__ cnt) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nq || status[r] != 0) return;
-    const uint64_t base = (qoff[r] >> 3) + 8ull * r, o = mem_off[r];
-    for (uint32_t i = 0; i < mem_cnt[r]; i++) out[o + i] = slots[base + i];
+    if (r >= nq) return;
+    const uint64_t m = qoff[r + 1] - qoff[r];
+    cnt[r] = m <= seg ? 1u : (uint32_t)((m + seg - 1) / seg);
+}
+__global__ void k_items_fill(uint32_t nq, const uint64_t *__restrict__ qoff, uint32_t seg,
+                             const uint64_t *__restrict__ first, uint4 *__restrict__ items) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nq) return;
+    const uint32_t m = (uint32_t)(qoff[r + 1] - qoff[r]);
+    const uint64_t f = first[r];
+    const uint32_t ns = (uint32_t)(first[r + 1] - f);
+    for (uint32_t s = 0; s < ns; s++) {
+        const uint32_t a = s * seg, b = s + 1 == ns ? m : a + seg;
+        items[f + s] = make_uint4(r, a, b, 0u);
+    }
+}
+// hit offsets per read from the offsets per item (first[r] = first item of r)
+__global__ void k_read_off(uint32_t nq, const uint64_t *__restrict__ first, const uint64_t *__restrict__ item_off,
+                           uint64_t *__restrict__ read_off) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > nq) return;
+    read_off[r] = item_off[first[r]];
 }
 
 // Place the records of successful attempts at their final position.
@@ -1290,22 +1350,20 @@ int build_seed_table(hsfm_index *idx, uint32_t F) {
 }
 } // namespace hsfm
 
+// Codes per MEM work item (HSFM_SEGMENT overrides; 0 keeps whole reads).
+static uint32_t mem_segment_length() {
+    const char *e = getenv("HSFM_SEGMENT");
+    if (!e || !*e) return 256;
+    const long v = strtol(e, nullptr, 10);
+    return v <= 0 ? 0u : (v < 32 ? 32u : (uint32_t)v);
+}
+
 static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *d_qoff, uint32_t nq, uint32_t min_len,
                            const uint32_t *d_refb, uint32_t n_refs, uint32_t max_hits, hsfm_mem_device_result *out) {
     Workspace &ws = idx->ws;
     hipStream_t s = idx->dev->stream;
     idx->stats = hsfm_stats{};
     int rc;
-    if ((rc = ws.mem_cnt.ensure((size_t)nq * 4 + 4))) return rc;
-    if ((rc = ws.mem_flags.ensure((size_t)nq * 4 + 4))) return rc;
-    if ((rc = ws.ovf_list.ensure((size_t)nq * 4 + 4))) return rc;
-    if ((rc = ws.read_list.ensure((size_t)nq * 4 + 4))) return rc;
-    if ((rc = ws.counters.ensure(sizeof(MemCounters)))) return rc;
-    if ((rc = ws.mem_off.ensure(((size_t)nq + 1) * 8))) return rc;
-    MemCounters *ctr = ws.counters.as<MemCounters>();
-    if ((rc = fill_u32(idx, ws.counters.as<uint32_t>(), sizeof(MemCounters) / 4, 0))) return rc;
-    if ((rc = fill_u32(idx, ws.mem_cnt.as<uint32_t>(), nq, 0))) return rc;
-    if ((rc = fill_u32(idx, ws.mem_flags.as<uint32_t>(), nq, 0xffffffffu))) return rc;
     uint64_t h[16];
 
     // path choice: the thread-per-read kernel seeds every restart from the
@@ -1314,17 +1372,52 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     const uint32_t F = 12;
     const char *path = getenv("HSFM_MEM_PATH");
     const bool use_thread = nq > 0 && min_len >= F && !(path && strcmp(path, "wave") == 0);
+    const bool legacy = path && (strcmp(path, "thread") == 0 || strcmp(path, "pipe") == 0);
+
+    // work items: reads longer than seg codes are split so that one long read
+    // does not leave a single lane working after all others are done
+    const uint32_t seg = legacy ? 0u : mem_segment_length();
+    uint64_t W = nq;
+    const uint4 *items = nullptr;
+    if (seg && nq) {
+        if ((rc = ws.item_cnt.ensure((size_t)nq * 4 + 4)) || (rc = ws.item_first.ensure(((size_t)nq + 1) * 8)))
+            return rc;
+        hipLaunchKernelGGL(k_items_count, dim3((nq + 255) / 256), dim3(256), 0, s, nq, d_qoff, seg,
+                           ws.item_cnt.as<uint32_t>());
+        HSFM_HIP(hipGetLastError());
+        if ((rc = exclusive_scan_u32_u64(idx, ws.item_cnt.as<uint32_t>(), ws.item_first.as<uint64_t>(), nq)))
+            return rc;
+        if ((rc = readback_u64(idx, ws.item_first.as<uint64_t>() + nq, &W, 1))) return rc;
+        if (W > 0xffffffffull - 16) return set_error(HSFM_EINVAL, "too many read segments (%llu)", (unsigned long long)W);
+        if (W > nq) {
+            if ((rc = ws.items.ensure(W * 16))) return rc;
+            hipLaunchKernelGGL(k_items_fill, dim3((nq + 255) / 256), dim3(256), 0, s, nq, d_qoff, seg,
+                               ws.item_first.as<uint64_t>(), ws.items.as<uint4>());
+            HSFM_HIP(hipGetLastError());
+            items = ws.items.as<uint4>();
+        }
+    }
+    if ((rc = ws.mem_cnt.ensure(W * 4 + 4))) return rc;
+    if ((rc = ws.mem_flags.ensure(W * 4 + 4))) return rc;
+    if ((rc = ws.ovf_list.ensure(W * 4 + 4))) return rc;
+    if ((rc = ws.read_list.ensure(W * 4 + 4))) return rc;
+    if ((rc = ws.counters.ensure(sizeof(MemCounters)))) return rc;
+    if ((rc = ws.mem_off.ensure((W + 1) * 8))) return rc;
+    MemCounters *ctr = ws.counters.as<MemCounters>();
+    if ((rc = fill_u32(idx, ws.counters.as<uint32_t>(), sizeof(MemCounters) / 4, 0))) return rc;
+    if ((rc = fill_u32(idx, ws.mem_cnt.as<uint32_t>(), W, 0))) return rc;
+    if ((rc = fill_u32(idx, ws.mem_flags.as<uint32_t>(), W, 0xffffffffu))) return rc;
     const uint32_t *list = nullptr; // identity list
-    uint64_t nlist = nq;
+    uint64_t nlist = W;
     SeedView sv{nullptr, nullptr, nullptr, 0};
     uint32_t *retry = ws.ovf_list.as<uint32_t>();
     if (use_thread) {
         if ((rc = build_seed_table(idx, F))) return rc;
         uint64_t total_len = 0;
         if ((rc = readback_u64(idx, d_qoff + nq, &total_len, 1))) return rc;
-        const uint64_t nslots = (total_len >> 3) + 8ull * nq;
+        const uint64_t nslots = (total_len >> 3) + 8ull * W + 8;
         if ((rc = ws.mem_slots.ensure(nslots * 16 + 16))) return rc;
-        if ((rc = ws.mem_status.ensure((size_t)nq + 16))) return rc;
+        if ((rc = ws.mem_status.ensure(W + 16))) return rc;
         sv = SeedView{idx->seed_off.as<uint64_t>(), idx->seed_iv.as<uint4>(), idx->seed_lsl.as<uint8_t>(), F};
         const bool old_thread = path && strcmp(path, "thread") == 0;
         uint64_t hd[16];
@@ -1344,9 +1437,9 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         } else {
             const uint32_t g1 = persist_grid(idx, k_mem_persist<16, 256>, 256);
             hipLaunchKernelGGL((k_mem_persist<16, 256>), dim3(g1), dim3(256), 0, s, idx->fwd.view, idx->rev.view,
-                               idx->d_const.as<DevConst>(), sv, d_q, d_qoff, (const uint32_t *)nullptr, nq, min_len,
-                               ws.mem_slots.as<uint4>(), ws.mem_cnt.as<uint32_t>(), ws.mem_status.as<uint8_t>(),
-                               ws.ovf_list.as<uint32_t>(), ctr);
+                               idx->d_const.as<DevConst>(), sv, d_q, d_qoff, (const uint32_t *)nullptr, (uint32_t)W,
+                               items, min_len, ws.mem_slots.as<uint4>(), ws.mem_cnt.as<uint32_t>(),
+                               ws.mem_status.as<uint8_t>(), ws.ovf_list.as<uint32_t>(), ctr);
         }
         HSFM_HIP(hipGetLastError());
         if ((rc = event_record(idx, 0, true))) return rc;
@@ -1392,13 +1485,13 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         if (lds)
             hipLaunchKernelGGL((k_mem_wave<true, 64>), dim3((uint32_t)waves), dim3(64), 0, s, idx->fwd.view,
                                idx->rev.view,
-                               idx->d_const.as<DevConst>(), sv, d_q, d_qoff, list, (uint32_t)nlist, min_len, cap,
+                               idx->d_const.as<DevConst>(), sv, d_q, d_qoff, list, (uint32_t)nlist, items, min_len, cap,
                                (uint4 *)nullptr, ws.mem_recs.as<MemRec>(), rec_cap, ws.mem_cnt.as<uint32_t>(),
                                ws.mem_flags.as<uint32_t>(), pass, retry, ctr, nullptr);
         else
             hipLaunchKernelGGL((k_mem_wave<false, 1024>), dim3((uint32_t)waves), dim3(1024), 0, s, idx->fwd.view,
                                idx->rev.view, idx->d_const.as<DevConst>(), sv, d_q, d_qoff, list, (uint32_t)nlist,
-                               min_len, cap, ws.wave_sets.as<uint4>(), ws.mem_recs.as<MemRec>(), rec_cap,
+                               items, min_len, cap, ws.wave_sets.as<uint4>(), ws.mem_recs.as<MemRec>(), rec_cap,
                                ws.mem_cnt.as<uint32_t>(), ws.mem_flags.as<uint32_t>(), pass, retry, ctr, nullptr);
         HSFM_HIP(hipGetLastError());
         if (timed && (rc = event_record(idx, 0, true))) return rc;
@@ -1420,21 +1513,22 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     }
     if (!use_thread) idx->stats.fallback_reads = 0;
     if (idx->stats.fallback_reads && (rc = event_record(idx, 2, true))) return rc;
-    if ((rc = exclusive_scan_u32_u64(idx, ws.mem_cnt.as<uint32_t>(), ws.mem_off.as<uint64_t>(), nq))) return rc;
+    if ((rc = exclusive_scan_u32_u64(idx, ws.mem_cnt.as<uint32_t>(), ws.mem_off.as<uint64_t>(), (uint32_t)W)))
+        return rc;
     uint64_t nhits = 0;
-    if ((rc = readback_u64(idx, ws.mem_off.as<uint64_t>() + nq, &nhits, 1))) return rc;
+    if ((rc = readback_u64(idx, ws.mem_off.as<uint64_t>() + W, &nhits, 1))) return rc;
     idx->stats.records = nhits;
     if ((rc = ws.mem_sorted.ensure(nhits * 16 + 16))) return rc;
     if (use_thread) {
-        hipLaunchKernelGGL(k_mem_gather, dim3((nq + 255) / 256), dim3(256), 0, s, nq, d_qoff, ws.mem_status.as<uint8_t>(),
-                           ws.mem_cnt.as<uint32_t>(), ws.mem_off.as<uint64_t>(), ws.mem_slots.as<uint4>(),
-                           ws.mem_sorted.as<uint4>());
+        hipLaunchKernelGGL(k_mem_gather, dim3((uint32_t)((W + 255) / 256)), dim3(256), 0, s, (uint32_t)W, d_qoff, items,
+                           ws.mem_status.as<uint8_t>(), ws.mem_cnt.as<uint32_t>(), ws.mem_off.as<uint64_t>(),
+                           ws.mem_slots.as<uint4>(), ws.mem_sorted.as<uint4>());
         HSFM_HIP(hipGetLastError());
     }
     if (rec_used > rec_cap) rec_used = rec_cap; // reservations past the buffer were never written
     if (rec_used) {
         hipLaunchKernelGGL(k_mem_scatter, dim3((uint32_t)((rec_used + 255) / 256)), dim3(256), 0, s,
-                           ws.mem_recs.as<MemRec>(), rec_used, nq, ws.mem_flags.as<uint32_t>(),
+                           ws.mem_recs.as<MemRec>(), rec_used, (uint32_t)W, ws.mem_flags.as<uint32_t>(),
                            ws.mem_off.as<uint64_t>(), ws.mem_sorted.as<uint4>(), nullptr);
         HSFM_HIP(hipGetLastError());
     }
@@ -1473,6 +1567,13 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     out->num_hits = nhits;
     out->num_positions = npos;
     out->d_hit_offsets = ws.mem_off.as<uint64_t>();
+    if (items) { // offsets per read from the offsets per item
+        if ((rc = ws.read_off.ensure(((size_t)nq + 1) * 8))) return rc;
+        hipLaunchKernelGGL(k_read_off, dim3((nq + 256) / 256), dim3(256), 0, s, nq, ws.item_first.as<uint64_t>(),
+                           ws.mem_off.as<uint64_t>(), ws.read_off.as<uint64_t>());
+        HSFM_HIP(hipGetLastError());
+        out->d_hit_offsets = ws.read_off.as<uint64_t>();
+    }
     out->d_hits = ws.mem_hits.as<hsfm_mem_hit>();
     out->d_positions = ws.positions.as<uint32_t>();
     return HSFM_OK;
@@ -1573,17 +1674,24 @@ __global__ void k_fill_u64(uint64_t *d, uint64_t n, uint64_t v) {
 }
 
 struct BatchBufs {
-    uint32_t B = 0;
+    uint32_t B = 0, W = 0, seg = 0; // reads, work items (capacity), codes per item
     uint64_t slot_cap = 0, rec_cap = 0, hit_cap = 0, pos_cap = 0;
     DevBuf bq_off, slots, mem_cnt, status, pass_of, l1, l2, l3, ctr, recs, wsets;
     DevBuf mem_off, sorted, npos, pos_off, hits, rows, positions, misc, tmp;
+    DevBuf items, item_cnt, item_first, read_off;
     size_t tmp_bytes = 0;
     void release() {
         DevBuf *all[] = {&bq_off, &slots, &mem_cnt, &status, &pass_of, &l1, &l2, &l3, &ctr, &recs, &wsets,
-                         &mem_off, &sorted, &npos, &pos_off, &hits, &rows, &positions, &misc, &tmp};
+                         &mem_off, &sorted, &npos, &pos_off, &hits, &rows, &positions, &misc, &tmp,
+                         &items, &item_cnt, &item_first, &read_off};
         for (DevBuf *b : all) b->release();
     }
 };
+// items [item_first[B], W) of a batch: empty (read 0, no starts)
+__global__ void k_pad_items(uint4 *items, const uint64_t *first_end, uint32_t W) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < W && i >= *first_end) items[i] = make_uint4(0, 0, 0, 0);
+}
 // misc: [0] positions-overflow count, [1] npos total, [2] walk guard
 struct BatchMisc {
     unsigned long long pos_ovf, npos, guard, pad;
@@ -1608,16 +1716,17 @@ static int batch_scan(hsfm_index *idx, BatchBufs &bb, const uint32_t *in, uint64
 static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, const uint8_t *d_q, uint32_t min_len,
                         const uint32_t *d_refb, uint32_t n_refs, uint32_t max_hits, int main_kernel, int rpt) {
     hipStream_t s = idx->dev->stream;
-    const uint32_t B = bb.B;
+    const uint32_t B = bb.B, W = bb.W;
+    const uint4 *items = bb.items.as<uint4>();
     MemCounters *c = bb.ctr.as<MemCounters>();
     const DevConst *cst = idx->d_const.as<DevConst>();
     const uint64_t *bq = bb.bq_off.as<uint64_t>();
     int rc;
     if ((rc = fill_u32(idx, bb.ctr.as<uint32_t>(), 3 * sizeof(MemCounters) / 4, 0)) ||
         (rc = fill_u32(idx, bb.misc.as<uint32_t>(), sizeof(BatchMisc) / 4, 0)) ||
-        (rc = fill_u32(idx, bb.mem_cnt.as<uint32_t>(), B, 0)) ||
-        (rc = fill_u32(idx, bb.pass_of.as<uint32_t>(), B, 0xffffffffu)) ||
-        (rc = fill_u32(idx, bb.status.as<uint32_t>(), (B + 3) / 4, main_kernel == BM_WAVE ? 0x01010101u : 0u)))
+        (rc = fill_u32(idx, bb.mem_cnt.as<uint32_t>(), W, 0)) ||
+        (rc = fill_u32(idx, bb.pass_of.as<uint32_t>(), W, 0xffffffffu)) ||
+        (rc = fill_u32(idx, bb.status.as<uint32_t>(), (W + 3) / 4, main_kernel == BM_WAVE ? 0x01010101u : 0u)))
         return rc;
     hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, s, bb.recs.as<uint64_t>(), bb.rec_cap * 4, ~0ull);
     const uint64_t half = bb.rec_cap / 2;
@@ -1627,7 +1736,8 @@ static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, cons
     if (main_kernel == BM_WAVE) {
         // one wave per read for every read
         hipLaunchKernelGGL((k_mem_wave<true, 64>), dim3(lds_waves), dim3(64), 0, s, idx->fwd.view, idx->rev.view, cst,
-                           sv, d_q, bq, (const uint32_t *)nullptr, B, min_len, kWaveLdsCap, (uint4 *)nullptr, r0, half,
+                           sv, d_q, bq, (const uint32_t *)nullptr, W, items, min_len, kWaveLdsCap, (uint4 *)nullptr, r0,
+                           half,
                            bb.mem_cnt.as<uint32_t>(), bb.pass_of.as<uint32_t>(), 0u, bb.l2.as<uint32_t>(), &c[1],
                            (const unsigned long long *)nullptr);
     } else {
@@ -1636,49 +1746,54 @@ static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, cons
         uint8_t *st = bb.status.as<uint8_t>();
         if (main_kernel == BM_PERSIST8)
             hipLaunchKernelGGL((k_mem_persist<8, 256>), dim3(persist_grid(idx, k_mem_persist<8, 256>, 256)), dim3(256),
-                               0, s, idx->fwd.view, idx->rev.view, cst, sv, d_q, bq, (const uint32_t *)nullptr, B,
-                               min_len, slots, cnt, st, bb.l1.as<uint32_t>(), &c[0]);
+                               0, s, idx->fwd.view, idx->rev.view, cst, sv, d_q, bq, (const uint32_t *)nullptr, W,
+                               items, min_len, slots, cnt, st, bb.l1.as<uint32_t>(), &c[0]);
         else if (main_kernel == BM_PERSIST32)
             hipLaunchKernelGGL((k_mem_persist<32, 128>), dim3(persist_grid(idx, k_mem_persist<32, 128>, 128)),
                                dim3(128), 0, s, idx->fwd.view, idx->rev.view, cst, sv, d_q, bq,
-                               (const uint32_t *)nullptr, B, min_len, slots, cnt, st, bb.l1.as<uint32_t>(), &c[0]);
+                               (const uint32_t *)nullptr, W, items, min_len, slots, cnt, st, bb.l1.as<uint32_t>(),
+                               &c[0]);
         else
             hipLaunchKernelGGL((k_mem_persist<16, 256>), dim3(persist_grid(idx, k_mem_persist<16, 256>, 256)),
                                dim3(256), 0, s, idx->fwd.view, idx->rev.view, cst, sv, d_q, bq,
-                               (const uint32_t *)nullptr, B, min_len, slots, cnt, st, bb.l1.as<uint32_t>(), &c[0]);
+                               (const uint32_t *)nullptr, W, items, min_len, slots, cnt, st, bb.l1.as<uint32_t>(),
+                               &c[0]);
         HSFM_HIP(hipGetLastError());
         // deferred reads: one wave each, sets in LDS
         hipLaunchKernelGGL((k_mem_wave<true, 64>), dim3(lds_waves), dim3(64), 0, s, idx->fwd.view, idx->rev.view, cst,
-                           sv, d_q, bq, bb.l1.as<uint32_t>(), B, min_len, kWaveLdsCap, (uint4 *)nullptr, r0, half, cnt,
+                           sv, d_q, bq, bb.l1.as<uint32_t>(), W, items, min_len, kWaveLdsCap, (uint4 *)nullptr, r0,
+                           half, cnt,
                            bb.pass_of.as<uint32_t>(), 0u, bb.l2.as<uint32_t>(), &c[1], second_count);
     }
     HSFM_HIP(hipGetLastError());
     // sets beyond the LDS: 1024 threads per read, global sets
     hipLaunchKernelGGL((k_mem_wave<false, 1024>), dim3(kBatchGlobalWaves), dim3(1024), 0, s, idx->fwd.view,
-                       idx->rev.view, cst, sv, d_q, bq, bb.l2.as<uint32_t>(), B, min_len, kBatchGlobalCap,
+                       idx->rev.view, cst, sv, d_q, bq, bb.l2.as<uint32_t>(), W, items, min_len, kBatchGlobalCap,
                        bb.wsets.as<uint4>(), r1, half, bb.mem_cnt.as<uint32_t>(), bb.pass_of.as<uint32_t>(), 1u,
                        bb.l3.as<uint32_t>(), &c[2], &c[1].n_retry);
     HSFM_HIP(hipGetLastError());
     uint64_t *mem_off = bb.mem_off.as<uint64_t>();
-    if ((rc = batch_scan(idx, bb, bb.mem_cnt.as<uint32_t>(), mem_off, B))) return rc;
+    if ((rc = batch_scan(idx, bb, bb.mem_cnt.as<uint32_t>(), mem_off, W))) return rc;
     uint4 *sorted = bb.sorted.as<uint4>();
     if (main_kernel != BM_WAVE)
-        hipLaunchKernelGGL(k_mem_gather, dim3((B + 255) / 256), dim3(256), 0, s, B, bq, bb.status.as<uint8_t>(),
-                           bb.mem_cnt.as<uint32_t>(), mem_off, bb.slots.as<uint4>(), sorted);
+        hipLaunchKernelGGL(k_mem_gather, dim3((W + 255) / 256), dim3(256), 0, s, W, bq, items,
+                           bb.status.as<uint8_t>(), bb.mem_cnt.as<uint32_t>(), mem_off, bb.slots.as<uint4>(), sorted);
+    hipLaunchKernelGGL(k_read_off, dim3((B + 256) / 256), dim3(256), 0, s, B, bb.item_first.as<uint64_t>(), mem_off,
+                       bb.read_off.as<uint64_t>());
     const dim3 hg((uint32_t)((half + 255) / 256));
-    hipLaunchKernelGGL(k_mem_scatter, hg, dim3(256), 0, s, r0, half, B, bb.pass_of.as<uint32_t>(), mem_off, sorted,
+    hipLaunchKernelGGL(k_mem_scatter, hg, dim3(256), 0, s, r0, half, W, bb.pass_of.as<uint32_t>(), mem_off, sorted,
                        &c[1].rec_used);
-    hipLaunchKernelGGL(k_mem_scatter, hg, dim3(256), 0, s, r1, half, B, bb.pass_of.as<uint32_t>(), mem_off, sorted,
+    hipLaunchKernelGGL(k_mem_scatter, hg, dim3(256), 0, s, r1, half, W, bb.pass_of.as<uint32_t>(), mem_off, sorted,
                        &c[2].rec_used);
     HSFM_HIP(hipGetLastError());
     const dim3 cg((uint32_t)((bb.hit_cap + 255) / 256));
     hipLaunchKernelGGL(k_mem_npos, cg, dim3(256), 0, s, sorted, bb.hit_cap, max_hits, n_refs ? 1u : 0u,
-                       bb.npos.as<uint32_t>(), mem_off + B);
+                       bb.npos.as<uint32_t>(), mem_off + W);
     if ((rc = batch_scan(idx, bb, bb.npos.as<uint32_t>(), bb.pos_off.as<uint64_t>(), bb.hit_cap))) return rc;
     BatchMisc *misc = bb.misc.as<BatchMisc>();
     hipLaunchKernelGGL(k_mem_hits, cg, dim3(256), 0, s, sorted, bb.hit_cap, max_hits, bb.pos_off.as<uint64_t>(),
-                       bb.hits.as<hsfm_mem_hit>(), bb.rows.as<uint32_t>(), mem_off + B, bb.pos_cap, &misc->pos_ovf);
-    hipLaunchKernelGGL(k_npos_total, dim3(1), dim3(1), 0, s, mem_off, B, bb.pos_off.as<uint64_t>(), bb.pos_cap,
+                       bb.hits.as<hsfm_mem_hit>(), bb.rows.as<uint32_t>(), mem_off + W, bb.pos_cap, &misc->pos_ovf);
+    hipLaunchKernelGGL(k_npos_total, dim3(1), dim3(1), 0, s, mem_off, W, bb.pos_off.as<uint64_t>(), bb.pos_cap,
                        reinterpret_cast<uint64_t *>(&misc->npos));
     HSFM_HIP(hipGetLastError());
     if (n_refs)
@@ -1721,10 +1836,19 @@ static int batch_capture(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, con
 // Stage batch [r0, r0 + nb) into the graph's offset buffer.
 static int batch_stage(hsfm_index *idx, BatchBufs &bb, const uint64_t *d_qoff, uint32_t r0, uint32_t nb) {
     hipStream_t s = idx->dev->stream;
+    const uint32_t B = bb.B;
     HSFM_HIP(hipMemcpyAsync(bb.bq_off.p, d_qoff + r0, ((size_t)nb + 1) * 8, hipMemcpyDeviceToDevice, s));
-    if (nb < bb.B)
-        hipLaunchKernelGGL(k_pad_qoff, dim3((bb.B - nb + 255) / 256), dim3(256), 0, s, bb.bq_off.as<uint64_t>(), nb,
-                           bb.B);
+    if (nb < B)
+        hipLaunchKernelGGL(k_pad_qoff, dim3((B - nb + 255) / 256), dim3(256), 0, s, bb.bq_off.as<uint64_t>(), nb, B);
+    // work items of the batch, then empty items up to the capacity
+    hipLaunchKernelGGL(k_items_count, dim3((B + 255) / 256), dim3(256), 0, s, B, bb.bq_off.as<uint64_t>(), bb.seg,
+                       bb.item_cnt.as<uint32_t>());
+    int rc = batch_scan(idx, bb, bb.item_cnt.as<uint32_t>(), bb.item_first.as<uint64_t>(), B);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_items_fill, dim3((B + 255) / 256), dim3(256), 0, s, B, bb.bq_off.as<uint64_t>(), bb.seg,
+                       bb.item_first.as<uint64_t>(), bb.items.as<uint4>());
+    hipLaunchKernelGGL(k_pad_items, dim3((bb.W + 255) / 256), dim3(256), 0, s, bb.items.as<uint4>(),
+                       bb.item_first.as<uint64_t>() + B, bb.W);
     HSFM_HIP(hipGetLastError());
     return HSFM_OK;
 }
@@ -1754,38 +1878,52 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     HSFM_HIP(hipMemcpyAsync(hoff.data(), d_qoff, ((size_t)nq + 1) * 8, hipMemcpyDeviceToHost, s));
     HSFM_HIP(hipStreamSynchronize(s));
     const uint32_t B = cfg->batch_reads < nq ? cfg->batch_reads : (nq ? nq : 1);
-    uint64_t max_slots = 0;
-    for (uint32_t r0 = 0; r0 < nq; r0 += B) {
-        const uint32_t r1 = r0 + B < nq ? r0 + B : nq;
-        const uint64_t sl = (hoff[r1] >> 3) + 8ull * B + 8; // slot indices reach (qoff >> 3) + 8 * local read
-        max_slots = sl > max_slots ? sl : max_slots;
-    }
     BatchBufs bb;
     bb.B = B;
-    bb.slot_cap = max_slots;
-    bb.rec_cap = 2ull * (16ull * B + 65536);
-    uint64_t max_batch_slots = 0;
+    bb.seg = mem_segment_length();
+    if (!bb.seg) bb.seg = 0xffffffffu; // one item per read
+    // item capacity and slot ranges over all batches (slot index of item w:
+    // ((qoff + start) >> 3) + 8 w)
+    uint64_t max_w = 0, max_slots = 0, max_batch_slots = 0;
     for (uint32_t r0 = 0; r0 < nq; r0 += B) {
         const uint32_t r1 = r0 + B < nq ? r0 + B : nq;
-        const uint64_t v = (hoff[r1] >> 3) - (hoff[r0] >> 3) + 8ull * B + 8;
+        uint64_t w = B - (r1 - r0); // padding reads: one item each
+        for (uint32_t r = r0; r < r1; r++) {
+            const uint64_t m = hoff[r + 1] - hoff[r];
+            w += m <= bb.seg ? 1 : (m + bb.seg - 1) / bb.seg;
+        }
+        max_w = w > max_w ? w : max_w;
+    }
+    if (max_w > 0x7fffffffull) return set_error(HSFM_EINVAL, "too many read segments in a batch");
+    bb.W = (uint32_t)max_w;
+    for (uint32_t r0 = 0; r0 < nq; r0 += B) {
+        const uint32_t r1 = r0 + B < nq ? r0 + B : nq;
+        const uint64_t sl = (hoff[r1] >> 3) + 8ull * bb.W + 16;
+        max_slots = sl > max_slots ? sl : max_slots;
+        const uint64_t v = (hoff[r1] >> 3) - (hoff[r0] >> 3) + 9ull * bb.W + 8;
         max_batch_slots = v > max_batch_slots ? v : max_batch_slots;
     }
+    bb.slot_cap = max_slots;
+    bb.rec_cap = 2ull * (16ull * B + 65536);
     bb.hit_cap = max_batch_slots + bb.rec_cap;
     bb.pos_cap = 4 * bb.hit_cap + 65536;
     {
         size_t t1 = 0, t2 = 0;
         hipcub::TransformInputIterator<uint64_t, ToU64, const uint32_t *> it(nullptr, ToU64());
-        HSFM_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, t1, it, (uint64_t *)nullptr, (int)B, s));
+        HSFM_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, t1, it, (uint64_t *)nullptr, (int)bb.W, s));
         HSFM_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, t2, it, (uint64_t *)nullptr, (int)bb.hit_cap, s));
         bb.tmp_bytes = t1 > t2 ? t1 : t2;
     }
+    const size_t Wn = bb.W;
     if ((rc = bb.bq_off.ensure(((size_t)B + 1) * 8)) || (rc = bb.slots.ensure(bb.slot_cap * 16)) ||
-        (rc = bb.mem_cnt.ensure((size_t)B * 4 + 4)) || (rc = bb.status.ensure((size_t)B + 8)) ||
-        (rc = bb.pass_of.ensure((size_t)B * 4 + 4)) || (rc = bb.l1.ensure((size_t)B * 4 + 4)) ||
-        (rc = bb.l2.ensure((size_t)B * 4 + 4)) || (rc = bb.l3.ensure((size_t)B * 4 + 4)) ||
+        (rc = bb.mem_cnt.ensure(Wn * 4 + 4)) || (rc = bb.status.ensure(Wn + 8)) ||
+        (rc = bb.pass_of.ensure(Wn * 4 + 4)) || (rc = bb.l1.ensure(Wn * 4 + 4)) ||
+        (rc = bb.l2.ensure(Wn * 4 + 4)) || (rc = bb.l3.ensure(Wn * 4 + 4)) ||
+        (rc = bb.items.ensure(Wn * 16 + 16)) || (rc = bb.item_cnt.ensure((size_t)B * 4 + 4)) ||
+        (rc = bb.item_first.ensure(((size_t)B + 1) * 8)) || (rc = bb.read_off.ensure(((size_t)B + 1) * 8)) ||
         (rc = bb.ctr.ensure(3 * sizeof(MemCounters))) || (rc = bb.recs.ensure(bb.rec_cap * sizeof(MemRec))) ||
         (rc = bb.wsets.ensure((size_t)kBatchGlobalWaves * 2 * kBatchGlobalCap * 16)) ||
-        (rc = bb.mem_off.ensure(((size_t)B + 1) * 8)) || (rc = bb.sorted.ensure(bb.hit_cap * 16)) ||
+        (rc = bb.mem_off.ensure((Wn + 1) * 8)) || (rc = bb.sorted.ensure(bb.hit_cap * 16)) ||
         (rc = bb.npos.ensure(bb.hit_cap * 4 + 4)) || (rc = bb.pos_off.ensure((bb.hit_cap + 1) * 8)) ||
         (rc = bb.hits.ensure(bb.hit_cap * sizeof(hsfm_mem_hit))) || (rc = bb.rows.ensure(bb.pos_cap * 4)) ||
         (rc = bb.positions.ensure(bb.pos_cap * 8)) || (rc = bb.misc.ensure(sizeof(BatchMisc))) ||
@@ -1894,7 +2032,7 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
         const uint64_t *m = bb.misc.as<uint64_t>();
         if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(&c[2].n_retry), h, 1)) ||
             (rc = readback_u64(idx, m, h + 1, 3)) ||
-            (rc = readback_u64(idx, bb.mem_off.as<uint64_t>() + B, h + 4, 1)))
+            (rc = readback_u64(idx, bb.mem_off.as<uint64_t>() + bb.W, h + 4, 1)))
             break;
         const uint64_t late = h[0], pos_ovf = h[1], npos = h[2], guard = h[3], nhits = h[4];
         if (guard) {
@@ -1915,7 +2053,7 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
             res.num_queries = nb;
             res.num_hits = nhits;
             res.num_positions = n_refs ? npos : 0;
-            res.d_hit_offsets = bb.mem_off.as<uint64_t>();
+            res.d_hit_offsets = bb.read_off.as<uint64_t>();
             res.d_hits = bb.hits.as<hsfm_mem_hit>();
             res.d_positions = bb.positions.as<uint32_t>();
         }

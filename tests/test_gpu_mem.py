@@ -224,3 +224,20 @@ def test_locate_then_mem_on_one_bidir_index():
     eo, eh = o.fwd.locate_batch(["ACG", "TTG"])
     assert np.array_equal(offs, eo) and np.array_equal(hits, eh)
     check(o, g, ["ACGTTGCAAC", "CCAGTAGGA"], 3)
+
+
+@pytest.mark.parametrize("seg", ["0", "32", "100", "256"])
+def test_long_reads_split_into_segments(monkeypatch, seg):
+    """Reads of 300-4000 codes split into work items of `seg` codes (0: whole
+    reads) give the oracle's MEMs, including MEMs that cross item bounds."""
+    rng = np.random.default_rng(41)
+    text, bounds = O.gen_reference(42, 200_000, 3, gc=0.41, repeat_frac=0.3, n_units=20, unit_min=300,
+                                   unit_max=2000, iupac_frac=0.005)
+    o = O.OracleBidir.build(text=text, bounds=bounds, sample_rate=16, threads=8)
+    g = hf.BidirFmIndex.from_bytes(o.to_bytes())
+    flat, off = O.gen_reads(43, text, bounds, 60, 1500, sigma=0.6, len_min=300, len_max=4000, sub=0.015,
+                            both_strands=True)
+    qs = [flat[off[i]:off[i + 1]] for i in range(len(off) - 1)]
+    qs += [random_text(rng, 700) for _ in range(5)]
+    monkeypatch.setenv("HSFM_SEGMENT", seg)
+    check(o, g, qs, 19, cap=32)
