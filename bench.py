@@ -185,6 +185,22 @@ def cpu_baseline(idx_bytes, sq, so, lq, lo, bounds, args, gpu_smem_sample, gpu_l
     return res
 
 
+def measured_traffic(kernel, args, nq, nl):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (copy-calibrated), when they were taken on this workload."""
+    path = os.path.join(ROOT, "profiles", "traffic_smem_locate_100Mbp.json")
+    try:
+        with open(path) as f:
+            doc = json.load(f)
+    except (OSError, ValueError):
+        return None
+    w = doc.get("workload", {})
+    if w.get("reference_bp") != args.ref_bp or w.get("smem_reads") != nq or w.get("locate_reads") != nl:
+        return None
+    k = doc.get("kernels", {}).get(kernel)
+    return k["bytes_per_launch"] if k else None
+
+
 def read_kmer_keys(flat, off, k, limit):
     """Keys of the exact-ACGT k-mer windows inside reads (first `limit`)."""
     import haystackfm_amd as hf
@@ -363,7 +379,7 @@ def main():
         bytes_per_launch = probes * 64.0
         achieved = bytes_per_launch / (kms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_mem_persist", args, nq, nl),
                 "kernel": "k_mem_persist", "kernel_ms": round(kms, 3),
                 "bytes_per_read": round(bytes_per_launch / nq, 1)}
     loc_roof = None
@@ -371,7 +387,8 @@ def main():
     if loc_stats.get("occ_probes") and lkms > 0:
         a = loc_stats["occ_probes"] * 64.0 / (lkms * 1e-3) / 1e9
         loc_roof = {"bound": "hbm", "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(a / HBM_PEAK_GBS, 4), "kernel": "k_locate_search", "kernel_ms": round(lkms, 3),
+                    "frac": round(a / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_locate_search", args, nq, nl),
+                    "kernel": "k_locate_search", "kernel_ms": round(lkms, 3),
                     "bytes_per_read": round(loc_stats["occ_probes"] * 64.0 / nl, 1)}
     cpu_line = None
     if "smem" in cpu:
