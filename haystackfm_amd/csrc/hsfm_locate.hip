@@ -708,6 +708,7 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
             if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
             nlist = h[0];
             list = ws.ovf_list.as<uint32_t>();
+            idx->stats.occ_probes = h[5]; // the main kernel's own probes
         }
         idx->stats.fallback_reads = pipe_all ? 0 : nlist;
         if (nlist) {
@@ -833,7 +834,7 @@ static int locate_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_
     uint64_t h[8];
     if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
     idx->stats.guard_hits = h[4];
-    idx->stats.occ_probes = h[5];
+    if (!idx->stats.occ_probes) idx->stats.occ_probes = h[5];
     idx->stats.main_kernel_ms = event_ms(idx, 0);
     idx->stats.resolve_kernel_ms = total ? event_ms(idx, 1) : 0.0;
     idx->stats.fallback_kernel_ms = idx->stats.fallback_reads ? event_ms(idx, 2) : 0.0;
