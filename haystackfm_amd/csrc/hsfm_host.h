@@ -73,6 +73,9 @@ struct hsfm_index {
     // bidirectional seed table of every ACGT F-mer (built on first MEM call)
     hsfm::DevBuf seed_off, seed_iv, seed_lsl;
     uint32_t seed_F = 0;
+    // packed one-load records of the 12-mer table and of the 14-mer table
+    // (with the 14-mers' own interval list)
+    hsfm::DevBuf seed_rec12, seed_rec14, seed_iv14;
     // batched MEM pipeline: configuration picked by the autotuner
     int tuned_main = -1, tuned_rpt = -1;
     uint32_t tuned_min_len = 0, tuned_batch = 0;
@@ -90,6 +93,13 @@ double event_ms(hsfm_index *idx, int slot);
 // Table of the bidirectional intervals of every ACGT F-mer's compatible strings
 // (kept on the index until it is freed).
 int build_seed_table(hsfm_index *idx, uint32_t F);
+// Packed records of the F-mer table (F = 12 or 14), one 16-byte load per F-mer:
+// {fwd_lo, fwd_hi, rev_lo, meta} when the F-mer has one compatible interval,
+// else {offset lo, offset hi, 0, meta} into the interval list; meta = lsl |
+// count << 5.  Builds the 12-mer table first.
+int build_seed_packed(hsfm_index *idx, uint32_t F);
+// Seed length of the persistent MEM kernel for min_len (12 or 14; 0 = none).
+uint32_t mem_seed_length(hsfm_index *idx, uint32_t min_len);
 // Resolve rows[0, min(cap, *d_nrows)) with rpt (1, 2, 4) interleaved walks per
 // thread on a grid of `grid` blocks (no host-side row count needed).
 int launch_resolve_fan(hsfm_index *idx, const DevDir &d, const uint32_t *d_rows, uint64_t cap,

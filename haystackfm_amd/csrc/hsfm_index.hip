@@ -414,6 +414,9 @@ extern "C" int hsfm_index_free(hsfm_index *idx) {
     idx->seed_off.release();
     idx->seed_iv.release();
     idx->seed_lsl.release();
+    idx->seed_rec12.release();
+    idx->seed_rec14.release();
+    idx->seed_iv14.release();
     delete idx;
     return HSFM_OK;
 }

@@ -377,7 +377,12 @@ struct SeedView {
     const uint4 *iv;
     const uint8_t *lsl;
     uint32_t F;
+    // packed records (build_seed_packed): {fwd_lo, fwd_hi, rev_lo, meta} for an
+    // F-mer with one interval, {iv offset lo, hi, 0, meta} otherwise;
+    // meta = lsl | count << 5
+    const uint4 *rec;
 };
+constexpr uint32_t kSeedLslBits = 5;
 
 // A seed window q[j-F..j) as table indices: ACGT codes give fixed digits, up
 // to kSeedAmb ambiguity codes expand to every base they contain (the union of
@@ -394,6 +399,37 @@ __device__ __forceinline__ bool seed_window(const uint8_t *w, uint32_t F, const 
     bool ok = true;
     for (uint32_t u = 0; u < F; u++) {
         const uint32_t x = s_expand[w[u]];
+        if (x & 0x10u) sw.base |= (x & 3u) << (2 * u);
+        else if (x && sw.namb < max_amb) {
+            sw.pos |= u << (8 * sw.namb);
+            sw.mask = (sw.mask & ~(0xfu << (4 * sw.namb))) | (x << (4 * sw.namb));
+            sw.namb++;
+        } else ok = false;
+    }
+    return ok;
+}
+// seed_window of q[0..F) (12 <= F <= 16) read as aligned dwords issued
+// together: the dwords touched are those holding bytes of the window only, so
+// every load stays inside memory the window itself occupies.
+__device__ __forceinline__ bool seed_window_dw(const uint8_t *q, uint32_t F, const uint8_t *s_expand, SeedWin &sw,
+                                               uint32_t max_amb) {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(q);
+    const uint32_t *base = reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3);
+    const uint32_t sh = (uint32_t)(a & 3), last = (sh + F - 1) >> 2; // last dword index (3 or 4)
+    uint32_t d[5];
+#pragma unroll
+    for (uint32_t i = 0; i < 5; i++) d[i] = base[i < last ? i : last];
+    // the 16 bytes from the window start, four per word
+    uint32_t b[4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++)
+        b[i] = sh ? (d[i] >> (8 * sh)) | (d[i + 1] << (32 - 8 * sh)) : d[i];
+    sw.base = 0; sw.pos = 0; sw.mask = 0x111u; sw.namb = 0;
+    bool ok = true;
+#pragma unroll
+    for (uint32_t u = 0; u < 16; u++) {
+        if (u >= F) break;
+        const uint32_t x = s_expand[(b[u >> 2] >> (8 * (u & 3))) & 0xffu];
         if (x & 0x10u) sw.base |= (x & 3u) << (2 * u);
         else if (x && sw.namb < max_amb) {
             sw.pos |= u << (8 * sw.namb);
@@ -422,6 +458,9 @@ __device__ __forceinline__ bool seed_window16(const uint32_t (&wb)[16], uint32_t
     }
     return ok;
 }
+// Expansions of a window: xi in [0, 4^namb) (digit i of xi = the base taken
+// for ambiguity code i).
+__device__ __forceinline__ uint32_t seed_ncombo(const SeedWin &sw) { return 1u << (2 * sw.namb); }
 // Table index of expansion xi (base digit i = bits 2i..2i+1 of xi); false if
 // a digit is not a base of its code.
 __device__ __forceinline__ bool seed_combo(const SeedWin &sw, uint32_t xi, uint32_t &idx) {

@@ -357,8 +357,8 @@ __global__ __launch_bounds__(PB) void k_locate_pipe(DevDir d, const DevConst *__
                 continue;
             } else if (st == LP_SEEDNEXT) {
                 uint32_t idx = 0;
-                while (xi < kSeedCombos && !seed_combo(sw, xi, idx)) xi++;
-                if (xi < kSeedCombos) {
+                while (xi < seed_ncombo(sw) && !seed_combo(sw, xi, idx)) xi++;
+                if (xi < seed_ncombo(sw)) {
                     xi++;
                     aidx = idx;
                     st = LP_SEEDX;

@@ -185,7 +185,7 @@ __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const D
                 if (seed_window(p + j - F, F, s_expand, sw)) {
                     const uint32_t namb = sw.namb;
                     uint32_t lslmax = 0, total = 0;
-                    for (uint32_t xi = 0; xi < kSeedCombos; xi++) {
+                    for (uint32_t xi = 0; xi < seed_ncombo(sw); xi++) {
                         uint32_t idx;
                         if (!seed_combo(sw, xi, idx)) continue;
                         const uint32_t l = sv.lsl[idx];
@@ -610,7 +610,9 @@ __device__ __forceinline__ uint32_t mem_thread_read(const DevDir &fwd, const Dev
 
 // ---------------------------------------------------------------------------
 // Persistent MEM finder: the min_len-skip algorithm of mem_thread_read run as a
-// per-lane state machine.  Every pass of the main loop performs at most one
+// per-lane state machine.  Restarts seed q[j-F..j) from the packed F-mer table
+// (F = 12 or 14): one record load gives lsl, the interval count and, for most
+// F-mers, the single interval.  Every pass of the main loop performs at most one
 // Occ probe per lane (one interval of the current set, or one seed-table entry
 // streamed into its first left extension), so lanes of a wave stay on the same
 // code path whatever phase their read is in; a lane that finishes a read takes
@@ -626,13 +628,18 @@ enum : uint32_t {
     SM_ENDLEFT,   // left extension ended at k
     SM_EMIT,      // right extension ended at e: write the records
     SM_SEEDE,     // probe: one seed entry, extended left by q[j-F-1]
+    SM_SEEDR,     // load: the packed seed record of one expansion
     SM_LEFT,      // probe: one interval of the set of q[k..j), by q[k-1]
     SM_RIGHT,     // probe: one interval of the set of q[k..e), by q[e]
     SM_DONE
 };
 
+constexpr uint32_t kPersistChunk = 64; // work indices a wave claims per atomic
+// 256-thread blocks of the ring kernel fit three per CU by LDS; the second
+// launch bound keeps the registers within the same three waves per SIMD.
+
 template <int PR, int PB>
-__global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, const DevConst *__restrict__ cst,
+__global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fwd, DevDir rev, const DevConst *__restrict__ cst,
                                                     SeedView sv, const uint8_t *__restrict__ q,
                                                     const uint64_t *__restrict__ qoff,
                                                     const uint32_t *__restrict__ list, uint32_t nlist,
@@ -647,37 +654,61 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
     __shared__ uint16_t s_compat[256];
     __shared__ uint8_t s_expand[256];
     __shared__ uint32_t s_clane[16];
+    // per wave: the unclaimed rest [s_qb, s_qb + s_ql) of its chunk of work indices
+    __shared__ uint32_t s_qb[PB / 64], s_ql[PB / 64];
     for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) {
         s_compat[i] = cst->compat_lanes[i];
         s_expand[i] = cst->seed_expand[i];
     }
     if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
+    if (threadIdx.x < PB / 64) {
+        s_qb[threadIdx.x] = 0;
+        s_ql[threadIdx.x] = 0;
+    }
     __syncthreads();
     const uint32_t tid = threadIdx.x;
+    const uint32_t wv = tid >> 6;
     const uint32_t n = fwd.n;
     const uint32_t F = sv.F;
     const bool seeding = F != 0 && min_len > F;
 
     uint32_t st = SM_FETCH, w = 0, r = 0, m = 0, ia = 0, ib = 0, s0 = 0, E = 0, j = 0, k = 0, e = 0;
     uint32_t ns = 0, nn = 0, t = 0, head = 0, nrec = 0, slot_cap = 0;
-    uint32_t xi = 0, lslmax = 0, probes = 0, restarts = 0;
+    uint32_t xi = 0, lslmax = 0, probes = 0, restarts = 0, sidx = 0;
     SeedWin sw{0, 0, 0x111u, 0};
     uint64_t slot_base = 0, ient = 0, iend = 0;
     const uint8_t *p = q;
     uint4 piv = make_uint4(0, 0, 0, 0);
 
     for (;;) {
-        // ---- control: move this lane to a probe state (or SM_DONE) ----
+        // ---- control: move this lane to a memory state (or SM_DONE) ----
         while (st < SM_SEEDE) {
             if (st == SM_FETCH) {
+                // work indices come from the wave's chunk; one device atomic
+                // claims the next kPersistChunk when the chunk runs out, so the
+                // shared counter sees one atomic per chunk, not one per read
                 const uint64_t act = __ballot(1);
                 const uint32_t leader = (uint32_t)__ffsll((unsigned long long)act) - 1u;
                 const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
-                uint32_t base = 0;
-                if (__lane_id() == leader) base = (uint32_t)atomicAdd(&ctr->pwork, (unsigned long long)__popcll(act));
-                base = __shfl(base, (int)leader, 64);
-                const uint32_t wi = base + rank;
+                const uint32_t need = (uint32_t)__popcll(act);
+                uint32_t cb = s_qb[wv], cl = s_ql[wv], wi;
+                if (need <= cl) {
+                    wi = cb + rank;
+                    cb += need;
+                    cl -= need;
+                } else {
+                    uint32_t nb = 0;
+                    if (__lane_id() == leader) nb = (uint32_t)atomicAdd(&ctr->pwork, (unsigned long long)kPersistChunk);
+                    nb = __shfl(nb, (int)leader, 64);
+                    wi = rank < cl ? cb + rank : nb + (rank - cl);
+                    cb = nb + (need - cl);
+                    cl = kPersistChunk - (need - cl);
+                }
+                if (__lane_id() == leader) {
+                    s_qb[wv] = cb;
+                    s_ql[wv] = cl;
+                }
                 if (wi >= nlist) { st = SM_DONE; break; }
                 w = list ? list[wi] : wi;
                 uint4 it = make_uint4(w, 0u, 0xffffffffu, 0u);
@@ -702,7 +733,7 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
                 }
                 // at most two ambiguity codes here: a third makes the expansion
                 // loop too long for one lane; such windows go to the wave kernel
-                const bool usable = seeding && j > F && seed_window(p + j - F, F, s_expand, sw, 2);
+                const bool usable = seeding && j > F && seed_window_dw(p + j - F, F, s_expand, sw, 2);
                 nn = 0;
                 t = 0;
                 if (usable) {
@@ -721,8 +752,8 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
                 }
             } else if (st == SM_SEEDX) {
                 uint32_t idx = 0;
-                while (xi < kSeedCombos && !seed_combo(sw, xi, idx)) xi++;
-                if (xi == kSeedCombos) {
+                while (xi < seed_ncombo(sw) && !seed_combo(sw, xi, idx)) xi++;
+                if (xi >= seed_ncombo(sw)) {
                     if (lslmax < F) { // q[j-lslmax-1..j) does not occur: L(j) = j - lslmax
                         s0 = j - lslmax;
                         st = SM_CAND;
@@ -741,16 +772,8 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
                     continue;
                 }
                 xi++;
-                const uint32_t l = sv.lsl[idx];
-                probes++;
-                lslmax = l > lslmax ? l : lslmax;
-                if (l != F) continue;
-                ient = sv.off[idx];
-                iend = sv.off[idx + 1];
-                probes += 2;
-                if (ient == iend) continue;
-                piv = sv.iv[ient];
-                st = SM_SEEDE;
+                sidx = idx; // its record is loaded with the other lanes' probes
+                st = SM_SEEDR;
             } else if (st == SM_ENDLEFT) {
                 if (k + min_len > j) {
                     s0 = k;
@@ -800,50 +823,77 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
         if (__ballot(st != SM_DONE) == 0) break;
         if (st == SM_DONE) continue;
 
-        // ---- probe: one interval, extended by one query code ----
-        const bool left = st != SM_RIGHT;
-        const uint32_t size = piv.y - piv.x;
-        const uint32_t xlo = left ? piv.x : piv.z;
-        const uint32_t xhi = left ? piv.y : piv.z + size;
-        const uint32_t qpos = st == SM_SEEDE ? j - F - 1 : (left ? k - 1 : e);
-        const uint32_t cm = s_compat[p[qpos]];
-        const DevDir &D = left ? fwd : rev;
-        probes += 1u + ((xlo >> kRecShift) != (xhi >> kRecShift) ? 1u : 0u);
-        uint32_t cl[16], ch[16];
-        pair_counts(D, xlo, xhi, cl, ch);
-        const uint32_t cbase = head + ns; // children start behind the current set
-        // slots not held by parents still to be processed (this one is in piv)
-        const uint32_t room = st == SM_SEEDE ? PR : PR - (ns - t - 1);
-        const bool dedup = st == SM_SEEDE;
+        // ---- memory pass: a seed record, or one interval extended by one code.
+        // The record load is issued first and consumed after the probes, so
+        // its latency overlaps theirs instead of stalling the wave on its own.
+        const bool srec = st == SM_SEEDR;
+        uint4 R = make_uint4(0u, 0u, 0u, 0u);
+        if (srec) R = sv.rec[sidx];
         bool ovf = false;
-        uint32_t acc = 0;
+        if (!srec) {
+            const bool left = st != SM_RIGHT;
+            const uint32_t size = piv.y - piv.x;
+            const uint32_t xlo = left ? piv.x : piv.z;
+            const uint32_t xhi = left ? piv.y : piv.z + size;
+            const uint32_t qpos = st == SM_SEEDE ? j - F - 1 : (left ? k - 1 : e);
+            const uint32_t cm = s_compat[p[qpos]];
+            const DevDir &D = left ? fwd : rev;
+            probes += 1u + ((xlo >> kRecShift) != (xhi >> kRecShift) ? 1u : 0u);
+            uint32_t cl[16], ch[16];
+            pair_counts(D, xlo, xhi, cl, ch);
+            const uint32_t cbase = head + ns; // children start behind the current set
+            // slots not held by parents still to be processed (this one is in piv)
+            const uint32_t room = st == SM_SEEDE ? PR : PR - (ns - t - 1);
+            const bool dedup = st == SM_SEEDE;
+            uint32_t acc = 0;
 #pragma unroll
-        for (int kk = 0; kk < 16; kk++) {
-            const uint32_t d = ch[kk] - cl[kk];
-            if (((cm >> kk) & 1u) && d != 0u) {
-                const uint32_t ca = s_clane[kk] + cl[kk];
-                const uint4 c = left ? make_uint4(ca, ca + d, piv.z + acc, 0u)
-                                     : make_uint4(piv.x + acc, piv.x + acc + d, ca, 0u);
-                bool dup = false;
-                if (dedup)
-                    for (uint32_t u = 0; u < nn; u++) dup |= s_lo[(cbase + u) & RM][tid] == c.x;
-                if (!dup) {
-                    if (nn + 1 > room) {
-                        ovf = true;
-                    } else {
-                        const uint32_t w = (cbase + nn) & RM;
-                        s_lo[w][tid] = c.x;
-                        s_hi[w][tid] = c.y;
-                        s_rl[w][tid] = c.z;
-                        nn++;
+            for (int kk = 0; kk < 16; kk++) {
+                const uint32_t d = ch[kk] - cl[kk];
+                if (((cm >> kk) & 1u) && d != 0u) {
+                    const uint32_t ca = s_clane[kk] + cl[kk];
+                    const uint4 c = left ? make_uint4(ca, ca + d, piv.z + acc, 0u)
+                                         : make_uint4(piv.x + acc, piv.x + acc + d, ca, 0u);
+                    bool dup = false;
+                    if (dedup)
+                        for (uint32_t u = 0; u < nn; u++) dup |= s_lo[(cbase + u) & RM][tid] == c.x;
+                    if (!dup) {
+                        if (nn + 1 > room) {
+                            ovf = true;
+                        } else {
+                            const uint32_t w = (cbase + nn) & RM;
+                            s_lo[w][tid] = c.x;
+                            s_hi[w][tid] = c.y;
+                            s_rl[w][tid] = c.z;
+                            nn++;
+                        }
                     }
                 }
+                acc += d;
             }
-            acc += d;
         }
 
         // ---- advance ----
-        if (ovf) {
+        if (srec) {
+            // one packed record: lsl, interval count and the interval itself
+            // when it is the only one
+            probes++;
+            const uint32_t l = R.w & ((1u << kSeedLslBits) - 1u), cnt = R.w >> kSeedLslBits;
+            lslmax = l > lslmax ? l : lslmax;
+            if (l != F || cnt == 0) {
+                st = SM_SEEDX;
+            } else if (cnt == 1) {
+                ient = 0;
+                iend = 1;
+                piv = make_uint4(R.x, R.y, R.z, 0u);
+                st = SM_SEEDE;
+            } else {
+                ient = (uint64_t)R.x | ((uint64_t)R.y << 32);
+                iend = ient + cnt;
+                piv = sv.iv[ient];
+                probes++;
+                st = SM_SEEDE;
+            }
+        } else if (ovf) {
             status[w] = 1;
             mem_cnt[w] = 0;
             defer_list[atomicAdd(&ctr->n_retry, 1ull)] = w;
@@ -851,8 +901,12 @@ __global__ __launch_bounds__(PB) void k_mem_persist(DevDir fwd, DevDir rev, cons
             st = SM_FETCH;
         } else if (st == SM_SEEDE) {
             ient++;
-            if (ient < iend) piv = sv.iv[ient];
-            else st = SM_SEEDX;
+            if (ient < iend) {
+                piv = sv.iv[ient];
+                probes++;
+            } else {
+                st = SM_SEEDX;
+            }
         } else if (++t < ns) {
             const uint32_t w = (head + t) & RM;
             piv = make_uint4(s_lo[w][tid], s_hi[w][tid], s_rl[w][tid], 0u);
@@ -1000,8 +1054,8 @@ __global__ __launch_bounds__(PB) void k_mem_pipe(DevDir fwd, DevDir rev, const D
                 break;
             } else if (st == MP_SEEDNEXT) {
                 uint32_t idx = 0;
-                while (xi < kSeedCombos && !seed_combo(sw, xi, idx)) xi++;
-                if (xi < kSeedCombos) {
+                while (xi < seed_ncombo(sw) && !seed_combo(sw, xi, idx)) xi++;
+                if (xi < seed_ncombo(sw)) {
                     xi++;
                     aidx = idx;
                     st = MP_SEEDX;
@@ -1307,6 +1361,34 @@ using namespace hsfm;
 
 namespace hsfm {
 // Build the seed table of all ACGT F-mers (k_seed_level, level by level).
+// One level of the seed table: the children of level l-1 (poff, piv, plsl)
+// into (coff, civ, clsl); cnt receives the children's interval counts.
+static int seed_level(hsfm_index *idx, uint32_t l, DevBuf &poff, DevBuf &piv, DevBuf &plsl, DevBuf &cnt,
+                      DevBuf &coff, DevBuf &civ, DevBuf &clsl) {
+    hipStream_t s = idx->dev->stream;
+    const uint32_t nchild = 1u << (2 * l);
+    int rc;
+    if ((rc = cnt.ensure((size_t)nchild * 4)) || (rc = coff.ensure(((size_t)nchild + 1) * 8)) ||
+        (rc = clsl.ensure(nchild)))
+        return rc;
+    const dim3 grid((nchild + 255) / 256);
+    hipLaunchKernelGGL(k_seed_level<false>, grid, dim3(256), 0, s, idx->fwd.view, idx->d_const.as<DevConst>(),
+                       poff.as<uint64_t>(), piv.as<uint4>(), nchild, cnt.as<uint32_t>(), nullptr, nullptr);
+    HSFM_HIP(hipGetLastError());
+    if ((rc = exclusive_scan_u32_u64(idx, cnt.as<uint32_t>(), coff.as<uint64_t>(), nchild))) return rc;
+    uint64_t total = 0;
+    if ((rc = readback_u64(idx, coff.as<uint64_t>() + nchild, &total, 1))) return rc;
+    if ((rc = civ.ensure(total * 16 + 16))) return rc;
+    hipLaunchKernelGGL(k_seed_level<true>, grid, dim3(256), 0, s, idx->fwd.view, idx->d_const.as<DevConst>(),
+                       poff.as<uint64_t>(), piv.as<uint4>(), nchild, nullptr, coff.as<uint64_t>(), civ.as<uint4>());
+    HSFM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_seed_lsl, grid, dim3(256), 0, s, nchild, l, cnt.as<uint32_t>(), plsl.as<uint8_t>(),
+                       clsl.as<uint8_t>());
+    HSFM_HIP(hipGetLastError());
+    HSFM_HIP(hipStreamSynchronize(s));
+    return HSFM_OK;
+}
+
 int build_seed_table(hsfm_index *idx, uint32_t F) {
     if (idx->seed_F == F) return HSFM_OK;
     hipStream_t s = idx->dev->stream;
@@ -1318,35 +1400,99 @@ int build_seed_table(hsfm_index *idx, uint32_t F) {
     HSFM_HIP(hipGetLastError());
     DevBuf cnt;
     for (uint32_t l = 1; l <= F; l++) {
-        const uint32_t nchild = 1u << (2 * l);
         DevBuf coff, civ, clsl;
-        if ((rc = cnt.ensure((size_t)nchild * 4)) || (rc = coff.ensure(((size_t)nchild + 1) * 8)) ||
-            (rc = clsl.ensure(nchild)))
+        if ((rc = seed_level(idx, l, poff, piv, plsl, cnt, coff, civ, clsl))) {
+            coff.release(); civ.release(); clsl.release();
+            poff.release(); piv.release(); plsl.release(); cnt.release();
             return rc;
-        const dim3 grid((nchild + 255) / 256);
-        hipLaunchKernelGGL(k_seed_level<false>, grid, dim3(256), 0, s, idx->fwd.view, idx->d_const.as<DevConst>(),
-                           poff.as<uint64_t>(), piv.as<uint4>(), nchild, cnt.as<uint32_t>(), nullptr, nullptr);
-        HSFM_HIP(hipGetLastError());
-        if ((rc = exclusive_scan_u32_u64(idx, cnt.as<uint32_t>(), coff.as<uint64_t>(), nchild))) return rc;
-        uint64_t total = 0;
-        if ((rc = readback_u64(idx, coff.as<uint64_t>() + nchild, &total, 1))) return rc;
-        if ((rc = civ.ensure(total * 16 + 16))) return rc;
-        hipLaunchKernelGGL(k_seed_level<true>, grid, dim3(256), 0, s, idx->fwd.view, idx->d_const.as<DevConst>(),
-                           poff.as<uint64_t>(), piv.as<uint4>(), nchild, nullptr, coff.as<uint64_t>(), civ.as<uint4>());
-        HSFM_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_seed_lsl, grid, dim3(256), 0, s, nchild, l, cnt.as<uint32_t>(), plsl.as<uint8_t>(),
-                           clsl.as<uint8_t>());
-        HSFM_HIP(hipGetLastError());
-        HSFM_HIP(hipStreamSynchronize(s));
+        }
         poff.release(); piv.release(); plsl.release();
         poff = coff; piv = civ; plsl = clsl;
         coff = DevBuf(); civ = DevBuf(); clsl = DevBuf();
     }
     cnt.release();
-    idx->seed_off.release(); idx->seed_iv.release(); idx->seed_lsl.release();
+    idx->seed_off.release(); idx->seed_iv.release(); idx->seed_lsl.release(); idx->seed_rec12.release();
     idx->seed_off = poff; idx->seed_iv = piv; idx->seed_lsl = plsl;
     idx->seed_F = F;
     return HSFM_OK;
+}
+
+// Packed record of every F-mer c from its level arrays (count = off[c+1] -
+// off[c]); a count beyond the meta field sets *bad.
+__global__ void k_seed_pack(uint32_t nchild, const uint64_t *__restrict__ off, const uint4 *__restrict__ iv,
+                            const uint8_t *__restrict__ lsl, uint4 *__restrict__ rec, uint32_t *__restrict__ bad) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchild) return;
+    const uint64_t a = off[c], cnt = off[c + 1] - a;
+    if (cnt >= (1ull << (32 - kSeedLslBits))) {
+        *bad = 1u;
+        rec[c] = make_uint4(0, 0, 0, 0);
+        return;
+    }
+    const uint32_t meta = (uint32_t)lsl[c] | ((uint32_t)cnt << kSeedLslBits);
+    if (cnt == 1) {
+        const uint4 v = iv[a];
+        rec[c] = make_uint4(v.x, v.y, v.z, meta);
+    } else {
+        rec[c] = make_uint4((uint32_t)a, (uint32_t)(a >> 32), 0u, meta);
+    }
+}
+
+static int seed_pack(hsfm_index *idx, uint32_t F, const DevBuf &off, const DevBuf &iv, const DevBuf &lsl,
+                     DevBuf &rec) {
+    hipStream_t s = idx->dev->stream;
+    const uint32_t nchild = 1u << (2 * F);
+    DevBuf flag;
+    int rc;
+    if ((rc = rec.ensure((size_t)nchild * 16)) || (rc = flag.ensure(4))) return rc;
+    if ((rc = fill_u32(idx, flag.as<uint32_t>(), 1, 0))) return rc;
+    hipLaunchKernelGGL(k_seed_pack, dim3((nchild + 255) / 256), dim3(256), 0, s, nchild, off.as<uint64_t>(),
+                       iv.as<uint4>(), lsl.as<uint8_t>(), rec.as<uint4>(), flag.as<uint32_t>());
+    HSFM_HIP(hipGetLastError());
+    uint32_t bad = 0;
+    HSFM_HIP(hipMemcpyAsync(&bad, flag.p, 4, hipMemcpyDeviceToHost, s));
+    HSFM_HIP(hipStreamSynchronize(s));
+    flag.release();
+    if (bad) {
+        rec.release();
+        return set_error(HSFM_EINVAL, "a %u-mer has more compatible intervals than the seed record holds", F);
+    }
+    return HSFM_OK;
+}
+
+int build_seed_packed(hsfm_index *idx, uint32_t F) {
+    int rc;
+    if ((rc = build_seed_table(idx, 12))) return rc;
+    if (F == 12) {
+        if (idx->seed_rec12.p) return HSFM_OK;
+        return seed_pack(idx, 12, idx->seed_off, idx->seed_iv, idx->seed_lsl, idx->seed_rec12);
+    }
+    if (F != 14) return set_error(HSFM_EINVAL, "seed length %u not supported", F);
+    if (idx->seed_rec14.p) return HSFM_OK;
+    // levels 13 and 14 from the 12-mer table (which stays as it is)
+    DevBuf cnt, off13, iv13, lsl13, off14, iv14, lsl14;
+    rc = seed_level(idx, 13, idx->seed_off, idx->seed_iv, idx->seed_lsl, cnt, off13, iv13, lsl13);
+    if (!rc) rc = seed_level(idx, 14, off13, iv13, lsl13, cnt, off14, iv14, lsl14);
+    cnt.release(); off13.release(); iv13.release(); lsl13.release();
+    if (!rc) rc = seed_pack(idx, 14, off14, iv14, lsl14, idx->seed_rec14);
+    off14.release(); lsl14.release();
+    if (rc) {
+        iv14.release();
+        return rc;
+    }
+    idx->seed_iv14.release();
+    idx->seed_iv14 = iv14;
+    return HSFM_OK;
+}
+
+// 14-mer seeds pay off (fewer, smaller restart sets) once 12-mers repeat by
+// chance: texts of 2^22 positions and more.  HSFM_SEED_F=12 / 14 overrides.
+uint32_t mem_seed_length(hsfm_index *idx, uint32_t min_len) {
+    uint32_t F = idx->fwd.view.n >= (1u << 22) ? 14u : 12u;
+    const char *e = getenv("HSFM_SEED_F");
+    if (e && *e) F = strtol(e, nullptr, 10) == 14 ? 14u : 12u;
+    if (F == 14 && min_len <= 14) F = 12;
+    return min_len > 12 ? F : 0u;
 }
 } // namespace hsfm
 
@@ -1373,6 +1519,7 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     const char *path = getenv("HSFM_MEM_PATH");
     const bool use_thread = nq > 0 && min_len >= F && !(path && strcmp(path, "wave") == 0);
     const bool legacy = path && (strcmp(path, "thread") == 0 || strcmp(path, "pipe") == 0);
+    const uint32_t FP = mem_seed_length(idx, min_len); // seed length of the persistent kernel
 
     // work items: reads longer than seg codes are split so that one long read
     // does not leave a single lane working after all others are done
@@ -1419,6 +1566,14 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         if ((rc = ws.mem_slots.ensure(nslots * 16 + 16))) return rc;
         if ((rc = ws.mem_status.ensure(W + 16))) return rc;
         sv = SeedView{idx->seed_off.as<uint64_t>(), idx->seed_iv.as<uint4>(), idx->seed_lsl.as<uint8_t>(), F};
+        SeedView svp = sv;
+        if (FP && (rc = build_seed_packed(idx, FP))) return rc;
+        if (FP == 14)
+            svp = SeedView{nullptr, idx->seed_iv14.as<uint4>(), nullptr, 14u, idx->seed_rec14.as<uint4>()};
+        else if (FP == 12)
+            svp.rec = idx->seed_rec12.as<uint4>();
+        else
+            svp.F = 0; // min_len 12: no seeding in the persistent kernel
         const bool old_thread = path && strcmp(path, "thread") == 0;
         uint64_t hd[16];
         if ((rc = event_record(idx, 0, false))) return rc;
@@ -1437,7 +1592,7 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         } else {
             const uint32_t g1 = persist_grid(idx, k_mem_persist<16, 256>, 256);
             hipLaunchKernelGGL((k_mem_persist<16, 256>), dim3(g1), dim3(256), 0, s, idx->fwd.view, idx->rev.view,
-                               idx->d_const.as<DevConst>(), sv, d_q, d_qoff, (const uint32_t *)nullptr, (uint32_t)W,
+                               idx->d_const.as<DevConst>(), svp, d_q, d_qoff, (const uint32_t *)nullptr, (uint32_t)W,
                                items, min_len, ws.mem_slots.as<uint4>(), ws.mem_cnt.as<uint32_t>(),
                                ws.mem_status.as<uint8_t>(), ws.ovf_list.as<uint32_t>(), ctr);
         }
@@ -1713,7 +1868,8 @@ static int batch_scan(hsfm_index *idx, BatchBufs &bb, const uint32_t *in, uint64
 }
 
 // The launches of one batch (captured into a graph, or run eagerly).
-static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, const uint8_t *d_q, uint32_t min_len,
+static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, const SeedView &svp,
+                        const uint8_t *d_q, uint32_t min_len,
                         const uint32_t *d_refb, uint32_t n_refs, uint32_t max_hits, int main_kernel, int rpt) {
     hipStream_t s = idx->dev->stream;
     const uint32_t B = bb.B, W = bb.W;
@@ -1746,16 +1902,16 @@ static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, cons
         uint8_t *st = bb.status.as<uint8_t>();
         if (main_kernel == BM_PERSIST8)
             hipLaunchKernelGGL((k_mem_persist<8, 256>), dim3(persist_grid(idx, k_mem_persist<8, 256>, 256)), dim3(256),
-                               0, s, idx->fwd.view, idx->rev.view, cst, sv, d_q, bq, (const uint32_t *)nullptr, W,
+                               0, s, idx->fwd.view, idx->rev.view, cst, svp, d_q, bq, (const uint32_t *)nullptr, W,
                                items, min_len, slots, cnt, st, bb.l1.as<uint32_t>(), &c[0]);
         else if (main_kernel == BM_PERSIST32)
             hipLaunchKernelGGL((k_mem_persist<32, 128>), dim3(persist_grid(idx, k_mem_persist<32, 128>, 128)),
-                               dim3(128), 0, s, idx->fwd.view, idx->rev.view, cst, sv, d_q, bq,
+                               dim3(128), 0, s, idx->fwd.view, idx->rev.view, cst, svp, d_q, bq,
                                (const uint32_t *)nullptr, W, items, min_len, slots, cnt, st, bb.l1.as<uint32_t>(),
                                &c[0]);
         else
             hipLaunchKernelGGL((k_mem_persist<16, 256>), dim3(persist_grid(idx, k_mem_persist<16, 256>, 256)),
-                               dim3(256), 0, s, idx->fwd.view, idx->rev.view, cst, sv, d_q, bq,
+                               dim3(256), 0, s, idx->fwd.view, idx->rev.view, cst, svp, d_q, bq,
                                (const uint32_t *)nullptr, W, items, min_len, slots, cnt, st, bb.l1.as<uint32_t>(),
                                &c[0]);
         HSFM_HIP(hipGetLastError());
@@ -1815,12 +1971,13 @@ struct BatchGraph {
     }
 };
 
-static int batch_capture(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, const uint8_t *d_q, uint32_t min_len,
+static int batch_capture(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, const SeedView &svp,
+                         const uint8_t *d_q, uint32_t min_len,
                          const uint32_t *d_refb, uint32_t n_refs, uint32_t max_hits, int main_kernel, int rpt,
                          BatchGraph &out) {
     hipStream_t s = idx->dev->stream;
     HSFM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int rc = batch_record(idx, bb, sv, d_q, min_len, d_refb, n_refs, max_hits, main_kernel, rpt);
+    int rc = batch_record(idx, bb, sv, svp, d_q, min_len, d_refb, n_refs, max_hits, main_kernel, rpt);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(s, &g);
     if (rc) {
@@ -1873,6 +2030,10 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     hsfm_batch_stats stats{};
     if ((rc = build_seed_table(idx, F))) return rc;
     SeedView sv{idx->seed_off.as<uint64_t>(), idx->seed_iv.as<uint4>(), idx->seed_lsl.as<uint8_t>(), F};
+    const uint32_t FP = mem_seed_length(idx, min_len);
+    if ((rc = build_seed_packed(idx, FP))) return rc;
+    const SeedView svp = FP == 14 ? SeedView{nullptr, idx->seed_iv14.as<uint4>(), nullptr, 14u, idx->seed_rec14.as<uint4>()}
+                                  : SeedView{sv.off, sv.iv, sv.lsl, F, idx->seed_rec12.as<uint4>()};
     // capacities from the host copy of the offsets
     std::vector<uint64_t> hoff((size_t)nq + 1);
     HSFM_HIP(hipMemcpyAsync(hoff.data(), d_qoff, ((size_t)nq + 1) * 8, hipMemcpyDeviceToHost, s));
@@ -1953,7 +2114,7 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
             const uint32_t nb0 = B < nq ? B : nq;
             auto time_cfg = [&](int mk, int rp, double &ms) -> int {
                 BatchGraph gr;
-                int r2 = batch_capture(idx, bb, sv, d_q, min_len, d_refb, n_refs, max_hits, mk, rp, gr);
+                int r2 = batch_capture(idx, bb, sv, svp, d_q, min_len, d_refb, n_refs, max_hits, mk, rp, gr);
                 if (r2) return r2;
                 ms = 1e30;
                 for (int rep = 0; rep < 2; rep++) {
@@ -2011,7 +2172,7 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     stats.main_kernel = main_kernel;
     stats.resolve_fanout = rpt;
     BatchGraph gr;
-    if (cfg->use_graph && (rc = batch_capture(idx, bb, sv, d_q, min_len, d_refb, n_refs, max_hits, main_kernel, rpt,
+    if (cfg->use_graph && (rc = batch_capture(idx, bb, sv, svp, d_q, min_len, d_refb, n_refs, max_hits, main_kernel, rpt,
                                               gr)))
         return cleanup(rc);
     HSFM_HIP(hipEventRecord(e0, s));
@@ -2023,7 +2184,7 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
                 rc = set_error(HSFM_EDEVICE, "hipGraphLaunch failed");
                 break;
             }
-        } else if ((rc = batch_record(idx, bb, sv, d_q, min_len, d_refb, n_refs, max_hits, main_kernel, rpt))) {
+        } else if ((rc = batch_record(idx, bb, sv, svp, d_q, min_len, d_refb, n_refs, max_hits, main_kernel, rpt))) {
             break;
         }
         // one wait per batch: counts and overflow flags

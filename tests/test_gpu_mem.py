@@ -186,6 +186,24 @@ def test_persistent_and_other_paths_agree(monkeypatch, other):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("iupac", [0.005, 0.04])
+def test_packed_seed_tables_12_and_14(monkeypatch, iupac):
+    """Restarts seeded from the packed 12-mer and 14-mer records (one interval
+    inline, several through the interval list) give the oracle's MEMs; an
+    IUPAC-rich text makes many 14-mers own several compatible intervals."""
+    rng = np.random.default_rng(57)
+    text, bounds = O.gen_reference(58, 300_000, 4, gc=0.41, repeat_frac=0.2, n_units=30, unit_min=300,
+                                   unit_max=2000, iupac_frac=iupac)
+    o = O.OracleBidir.build(text=text, bounds=bounds, sample_rate=32, threads=8)
+    g = hf.BidirFmIndex.from_bytes(o.to_bytes())
+    qs = _mem_reads(rng, text, 300, 150, 0.02, 0.01)
+    qs += [random_text(rng, 150, alphabet="ACGT") for _ in range(20)]
+    for F in ("12", "14"):
+        monkeypatch.setenv("HSFM_SEED_F", F)
+        for min_len in (15, 19):
+            check(o, g, qs, min_len, modes=(0,))
+
+
 def test_persistent_kernel_deferral_levels():
     """Copies of one unit with scattered ambiguity codes: the sets of a 13-mer
     window hold one interval per distinct copy, outgrowing the 8-interval and
