@@ -408,36 +408,53 @@ __device__ __forceinline__ bool seed_window(const uint8_t *w, uint32_t F, const 
     }
     return ok;
 }
-// seed_window of q[0..F) (12 <= F <= 16) read as aligned dwords issued
-// together: the dwords touched are those holding bytes of the window only, so
-// every load stays inside memory the window itself occupies.
-__device__ __forceinline__ bool seed_window_dw(const uint8_t *q, uint32_t F, const uint8_t *s_expand, SeedWin &sw,
-                                               uint32_t max_amb) {
+// seed_window of q[0..F) (12 <= F <= 16) from aligned dwords: seed_window_load
+// issues the loads (only dwords holding bytes of the window, so every load
+// stays inside memory the window itself occupies), seed_window_decode reads
+// the codes out of them.
+__device__ __forceinline__ void seed_window_load(const uint8_t *q, uint32_t F, uint4 &d, uint32_t &d4) {
     const uintptr_t a = reinterpret_cast<uintptr_t>(q);
     const uint32_t *base = reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3);
-    const uint32_t sh = (uint32_t)(a & 3), last = (sh + F - 1) >> 2; // last dword index (3 or 4)
-    uint32_t d[5];
+    const uint32_t last = ((uint32_t)(a & 3) + F - 1) >> 2; // 2..4
+    d.x = base[0];
+    d.y = base[1];
+    d.z = base[2];
+    d.w = base[last < 3 ? last : 3];
+    d4 = base[last];
+}
+__device__ __forceinline__ bool seed_window_decode(const uint8_t *q, uint32_t F, const uint4 &d, uint32_t d4,
+                                                   const uint8_t *s_expand, SeedWin &sw, uint32_t max_amb) {
+    const uint32_t sh = (uint32_t)(reinterpret_cast<uintptr_t>(q) & 3);
+    const uint32_t dd[5] = {d.x, d.y, d.z, d.w, d4};
+    uint32_t b[4]; // the 16 bytes from the window start, four per word
 #pragma unroll
-    for (uint32_t i = 0; i < 5; i++) d[i] = base[i < last ? i : last];
-    // the 16 bytes from the window start, four per word
-    uint32_t b[4];
+    for (uint32_t i = 0; i < 4; i++) b[i] = sh ? (dd[i] >> (8 * sh)) | (dd[i + 1] << (32 - 8 * sh)) : dd[i];
+    // all sixteen table lookups first (independent LDS reads), then a
+    // branch-free pass: exact bases give digits, ambiguity codes are counted
+    // and the first max_amb (<= 3) of them recorded in order
+    uint32_t xs[16];
 #pragma unroll
-    for (uint32_t i = 0; i < 4; i++)
-        b[i] = sh ? (d[i] >> (8 * sh)) | (d[i + 1] << (32 - 8 * sh)) : d[i];
-    sw.base = 0; sw.pos = 0; sw.mask = 0x111u; sw.namb = 0;
-    bool ok = true;
+    for (uint32_t u = 0; u < 16; u++) xs[u] = s_expand[(b[u >> 2] >> (8 * (u & 3))) & 0xffu];
+    uint32_t base = 0, pos = 0, mask = 0x111u, na = 0;
+    bool bad = false;
 #pragma unroll
     for (uint32_t u = 0; u < 16; u++) {
-        if (u >= F) break;
-        const uint32_t x = s_expand[(b[u >> 2] >> (8 * (u & 3))) & 0xffu];
-        if (x & 0x10u) sw.base |= (x & 3u) << (2 * u);
-        else if (x && sw.namb < max_amb) {
-            sw.pos |= u << (8 * sw.namb);
-            sw.mask = (sw.mask & ~(0xfu << (4 * sw.namb))) | (x << (4 * sw.namb));
-            sw.namb++;
-        } else ok = false;
+        const uint32_t x = xs[u];
+        const bool in = u < F;
+        const bool ex = in && (x & 0x10u);
+        const bool am = in && !(x & 0x10u) && x != 0u;
+        bad |= in && x == 0u;
+        base |= ex ? (x & 3u) << (2 * u) : 0u;
+        const bool rec = am && na < max_amb;
+        pos |= rec ? u << (8 * na) : 0u;
+        mask = rec ? (mask & ~(0xfu << (4 * na))) | (x << (4 * na)) : mask;
+        na += am ? 1u : 0u;
     }
-    return ok;
+    sw.base = base;
+    sw.pos = pos;
+    sw.mask = mask;
+    sw.namb = na < max_amb ? na : max_amb;
+    return !bad && na <= max_amb;
 }
 // seed_window over the last F (<= 16) of 16 codes held one per register.
 __device__ __forceinline__ bool seed_window16(const uint32_t (&wb)[16], uint32_t F, const uint8_t *s_expand,
@@ -471,6 +488,39 @@ __device__ __forceinline__ bool seed_combo(const SeedWin &sw, uint32_t xi, uint3
         const uint32_t d = (xi >> (2 * i)) & 3u;
         ok &= ((sw.mask >> (4 * i + d)) & 1u) != 0;
         if (i < sw.namb) idx |= d << (2 * ((sw.pos >> (8 * i)) & 0xffu));
+    }
+    return ok;
+}
+
+// A seed window with up to kWideAmb ambiguity codes (expanded cooperatively
+// by a workgroup): 4-bit positions and base masks per code.
+constexpr uint32_t kWideAmb = 5;
+struct WideWin {
+    uint32_t base, pos, mask, namb;
+};
+__device__ __forceinline__ bool wide_window(const uint8_t *w, uint32_t F, const uint8_t *s_expand, WideWin &ww) {
+    ww.base = 0; ww.pos = 0; ww.mask = 0; ww.namb = 0;
+    bool ok = true;
+    for (uint32_t u = 0; u < F; u++) {
+        const uint32_t x = s_expand[w[u]];
+        if (x & 0x10u) ww.base |= (x & 3u) << (2 * u);
+        else if (x && ww.namb < kWideAmb) {
+            ww.pos |= u << (4 * ww.namb);
+            ww.mask |= x << (4 * ww.namb);
+            ww.namb++;
+        } else ok = false;
+    }
+    return ok;
+}
+// Table index of expansion xi in [0, 4^namb); false if a digit is not a base
+// of its code.
+__device__ __forceinline__ bool wide_combo(const WideWin &ww, uint32_t xi, uint32_t &idx) {
+    idx = ww.base;
+    bool ok = true;
+    for (uint32_t i = 0; i < ww.namb; i++) {
+        const uint32_t d = (xi >> (2 * i)) & 3u;
+        ok &= ((ww.mask >> (4 * i + d)) & 1u) != 0;
+        idx |= d << (2 * ((ww.pos >> (4 * i)) & 0xfu));
     }
     return ok;
 }

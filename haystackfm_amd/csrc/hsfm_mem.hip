@@ -145,7 +145,7 @@ __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const D
     __shared__ uint4 s_sets[LDS ? 2 * kWaveLdsCap : 1];
     __shared__ uint32_t s_part[TB / 64 + 1];
     __shared__ unsigned long long s_b64;
-    __shared__ uint32_t s_b32;
+    __shared__ uint32_t s_b32, s_lsl;
     for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) {
         s_compat[i] = cst->compat_lanes[i];
         s_expand[i] = cst->seed_expand[i];
@@ -180,22 +180,34 @@ __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const D
             uint32_t ns = 1, k = j;
             bool seeded = false;
             if (F && min_len > F && j >= F) {
-                // q[j-F..j) from the table: the union of its expansions' entries
-                SeedWin sw;
-                if (seed_window(p + j - F, F, s_expand, sw)) {
-                    const uint32_t namb = sw.namb;
-                    uint32_t lslmax = 0, total = 0;
-                    for (uint32_t xi = 0; xi < seed_ncombo(sw); xi++) {
-                        uint32_t idx;
-                        if (!seed_combo(sw, xi, idx)) continue;
-                        const uint32_t l = sv.lsl[idx];
-                        lslmax = l > lslmax ? l : lslmax;
-                        if (l != F) continue;
-                        const uint64_t o0 = sv.off[idx], o1 = sv.off[idx + 1];
-                        if (total + (o1 - o0) <= cap)
-                            for (uint32_t t = lane; t < (uint32_t)(o1 - o0); t += TB) A[total + t] = sv.iv[o0 + t];
-                        total += (uint32_t)(o1 - o0);
+                // q[j-F..j) from the table: the union of its expansions' entries,
+                // the expansions spread over the workgroup's threads
+                WideWin sw;
+                if (wide_window(p + j - F, F, s_expand, sw)) {
+                    const uint32_t namb = sw.namb, nc = 1u << (2 * namb);
+                    if (lane == 0) s_lsl = 0;
+                    __syncthreads();
+                    uint32_t total = 0;
+                    for (uint32_t x0 = 0; x0 < nc; x0 += TB) {
+                        const uint32_t xi = x0 + lane;
+                        uint32_t idx = 0, cnt = 0;
+                        uint64_t o0 = 0;
+                        if (xi < nc && wide_combo(sw, xi, idx)) {
+                            const uint32_t l = sv.lsl[idx];
+                            atomicMax(&s_lsl, l);
+                            if (l == F) {
+                                o0 = sv.off[idx];
+                                cnt = (uint32_t)(sv.off[idx + 1] - o0);
+                            }
+                        }
+                        uint32_t tot;
+                        const uint32_t at = total + block_incl_scan<TB>(cnt, s_part, tot) - cnt;
+                        for (uint32_t t = 0; t < cnt && at + t < cap; t++) A[at + t] = sv.iv[o0 + t];
+                        total += tot;
                     }
+                    __syncthreads();
+                    const uint32_t lslmax = s_lsl;
+                    __syncthreads();
                     if (lslmax < F) { // q[j-lslmax-1..j) does not occur: L(j) = j - lslmax
                         s0 = j - lslmax;
                         continue;
@@ -731,9 +743,15 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
                     st = SM_FETCH;
                     continue;
                 }
-                // at most two ambiguity codes here: a third makes the expansion
-                // loop too long for one lane; such windows go to the wave kernel
-                const bool usable = seeding && j > F && seed_window_dw(p + j - F, F, s_expand, sw, 2);
+                // up to three ambiguity codes (64 expansions, one record load
+                // each); windows with more restart unseeded
+                bool usable = false;
+                if (seeding && j > F) {
+                    uint4 wd;
+                    uint32_t wd4;
+                    seed_window_load(p + j - F, F, wd, wd4);
+                    usable = seed_window_decode(p + j - F, F, wd, wd4, s_expand, sw, kSeedAmb);
+                }
                 nn = 0;
                 t = 0;
                 if (usable) {
@@ -1627,7 +1645,13 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         if (ws.mem_recs.cap < rec_cap * sizeof(MemRec)) {
             DevBuf grown;
             if ((rc = grown.ensure(rec_cap * sizeof(MemRec)))) return rc;
-            if (ws.mem_recs.p) HSFM_HIP(hipMemcpyAsync(grown.p, ws.mem_recs.p, ws.mem_recs.cap, hipMemcpyDeviceToDevice, s));
+            const size_t kept = ws.mem_recs.p ? ws.mem_recs.cap : 0;
+            if (kept) HSFM_HIP(hipMemcpyAsync(grown.p, ws.mem_recs.p, kept, hipMemcpyDeviceToDevice, s));
+            // slots reserved past the old end were never written: the new part
+            // starts as dead records (read = pass = ~0), which the scatter skips
+            if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(static_cast<char *>(grown.p) + kept),
+                               (grown.cap - kept) / 4, 0xffffffffu)))
+                return rc;
             HSFM_HIP(hipStreamSynchronize(s));
             ws.mem_recs.release();
             ws.mem_recs = grown;
@@ -1656,7 +1680,9 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         if (nretry == 0) break;
         idx->stats.capacity_retries++;
         if (cap >= (1u << 26)) return set_error(HSFM_ENOMEM, "interval set of a single read exceeds 2^26 intervals");
-        cap = lds ? 4096 : cap * 4;
+        // sets that outgrow the LDS are usually far larger: go straight to
+        // 65536 intervals, then 4x per further pass
+        cap = lds ? 65536 : cap * 4;
         const uint64_t want = rec_used + nretry * 4096;
         rec_cap = rec_cap * 2 > want ? rec_cap * 2 : want;
         // next pass runs only the retried reads

@@ -204,6 +204,29 @@ def test_packed_seed_tables_12_and_14(monkeypatch, iupac):
             check(o, g, qs, min_len, modes=(0,))
 
 
+def test_ambiguity_clusters_in_seed_windows():
+    """Reads with 4-7 ambiguity codes inside one seed window: the ring kernel
+    hands them to the workgroup kernel, which expands up to five codes over its
+    threads and restarts unseeded beyond that; all stay exact."""
+    rng = np.random.default_rng(77)
+    text, bounds = O.gen_reference(78, 200_000, 3, gc=0.41, repeat_frac=0.2, n_units=20, unit_min=300,
+                                   unit_max=1500, iupac_frac=0.005)
+    o = O.OracleBidir.build(text=text, bounds=bounds, sample_rate=32, threads=8)
+    g = hf.BidirFmIndex.from_bytes(o.to_bytes())
+    qs = []
+    for i in range(120):
+        st = int(rng.integers(0, len(text) - 150))
+        q = text[st:st + 150].copy()
+        q[q == 0] = 1
+        c = int(rng.integers(10, 130))
+        namb = 4 + i % 4
+        at = c + rng.choice(12, size=namb, replace=False)
+        q[at] = random_text(rng, namb, alphabet="NRYSWKMBDHV")
+        qs.append(q)
+    check(o, g, qs, 15, modes=(0,))
+    check(o, g, qs[:40], 19, modes=(1,))
+
+
 def test_persistent_kernel_deferral_levels():
     """Copies of one unit with scattered ambiguity codes: the sets of a 13-mer
     window hold one interval per distinct copy, outgrowing the 8-interval and
