@@ -222,7 +222,7 @@ class _Stats(C.Structure):
     _fields_ = [("fallback_reads", C.c_uint64), ("capacity_retries", C.c_uint64),
                 ("guard_hits", C.c_uint64), ("records", C.c_uint64), ("main_kernel_ms", C.c_double),
                 ("resolve_kernel_ms", C.c_double), ("occ_probes", C.c_uint64), ("diag", C.c_uint64 * 4),
-                ("fallback_kernel_ms", C.c_double)]
+                ("fallback_kernel_ms", C.c_double), ("sa_full_rows", C.c_uint64)]
 
 
 class _LsiConfig(C.Structure):

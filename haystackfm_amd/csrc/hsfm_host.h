@@ -39,6 +39,8 @@ struct DevBuf {
 
 struct DirBuffers {
     DevBuf occ, sb, samp, sa_vals, lookup;
+    DevBuf sa_full;           // every row's SA value (see ensure_sa_full)
+    bool sa_full_tried = false;
     DevDir view{};
 };
 
@@ -100,6 +102,11 @@ int build_seed_table(hsfm_index *idx, uint32_t F);
 int build_seed_packed(hsfm_index *idx, uint32_t F);
 // Seed length of the persistent MEM kernel for min_len (12 or 14; 0 = none).
 uint32_t mem_seed_length(hsfm_index *idx, uint32_t min_len);
+// The forward direction's full suffix array (4 bytes per row), derived once on
+// the device by LF walks between samples; resolve then reads it instead of
+// walking.  Left unset (walks stay in use) if the walks do not cover every row
+// within the sample rate or the memory is not available.
+int ensure_sa_full(hsfm_index *idx);
 // Resolve rows[0, min(cap, *d_nrows)) with rpt (1, 2, 4) interleaved walks per
 // thread on a grid of `grid` blocks (no host-side row count needed).
 int launch_resolve_fan(hsfm_index *idx, const DevDir &d, const uint32_t *d_rows, uint64_t cap,

@@ -240,6 +240,9 @@ void free_dir(DirBuffers &d) {
     d.samp.release();
     d.sa_vals.release();
     d.lookup.release();
+    d.sa_full.release();
+    d.sa_full_tried = false;
+    d.view.sa_full = nullptr;
 }
 
 void free_workspace(Workspace &w) {

@@ -58,6 +58,9 @@ struct DevDir {
     const uint32_t *sa_vals;
     const uint2 *lookup;  // (lo, hi) per k-mer, or null
     uint32_t n;           // text length (including sentinels)
+    // SA value of every row, built on the device from the samples (resolve
+    // becomes one load), or null
+    const uint32_t *sa_full;
     uint32_t num_samples;
     uint32_t sample_rate;
     uint32_t num_lanes;

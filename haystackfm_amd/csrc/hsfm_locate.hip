@@ -813,8 +813,9 @@ static int locate_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_
     Workspace &ws = idx->ws;
     hipStream_t s = idx->dev->stream;
     idx->stats = hsfm_stats{};
-    int rc = locate_search_phase(idx, d_q, d_qoff, nq);
+    int rc = ensure_sa_full(idx);
     if (rc) return rc;
+    if ((rc = locate_search_phase(idx, d_q, d_qoff, nq))) return rc;
     if ((rc = ws.hit_off.ensure(((size_t)nq + 1) * 8))) return rc;
     if ((rc = exclusive_scan_u32_u64(idx, ws.cnt.as<uint32_t>(), ws.hit_off.as<uint64_t>(), nq))) return rc;
     uint64_t total = 0;

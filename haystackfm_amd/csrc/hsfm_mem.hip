@@ -1529,6 +1529,7 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     idx->stats = hsfm_stats{};
     int rc;
     uint64_t h[16];
+    if ((rc = ensure_sa_full(idx))) return rc;
 
     // path choice: the thread-per-read kernel seeds every restart from the
     // 12-mer table, which needs min_len >= 12; shorter min_len (and
@@ -2054,6 +2055,7 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     hipStream_t s = idx->dev->stream;
     int rc;
     hsfm_batch_stats stats{};
+    if ((rc = ensure_sa_full(idx))) return rc;
     if ((rc = build_seed_table(idx, F))) return rc;
     SeedView sv{idx->seed_off.as<uint64_t>(), idx->seed_iv.as<uint4>(), idx->seed_lsl.as<uint8_t>(), F};
     const uint32_t FP = mem_seed_length(idx, min_len);

@@ -128,7 +128,8 @@ __global__ __launch_bounds__(256) void k_resolve(DevDir d, const DevConst *__res
     uint32_t row = rows[t];
     row = row < nm1 ? row : nm1;
     uint32_t steps = 0, pos = 0xffffffffu;
-    for (;;) {
+    if (d.sa_full) pos = d.sa_full[row];
+    else for (;;) {
         const uint4 w = d.samp[row >> 6];
         RecView r;
         load_rec(d, row, r);
@@ -192,6 +193,11 @@ __global__ __launch_bounds__(256) void k_resolve_fan(DevDir d, const DevConst *_
             pos[j] = 0xffffffffu;
         }
         bool any = true;
+        if (d.sa_full) {
+#pragma unroll
+            for (int j = 0; j < RPT; j++) pos[j] = live[j] ? d.sa_full[row[j]] : 0xffffffffu;
+            any = false;
+        }
         while (any) {
             any = false;
 #pragma unroll
@@ -242,6 +248,166 @@ __global__ __launch_bounds__(256) void k_resolve_fan(DevDir d, const DevConst *_
     }
 }
 
+// ---------------------------------------------------------------------------
+// Full suffix array of the forward direction.  Every row's resolve value
+// (src/fm_index/query.rs:282-291) is the sample reached by its LF walk plus the
+// steps taken, so walking LF backward from each sampled row writes the values
+// of all rows between it and the previous sample of the same sequence (the
+// walk stops before stepping over a sentinel: sequence starts are sampled).
+// The rows after the last sample of a sequence are those on the walk from
+// the sequence's sentinel row, which k_sa_full_sent resolves the same way and
+// then fills.  A walk longer than the sample rate marks the table unusable.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sa_full(DevDir d, const DevConst *__restrict__ cst, uint32_t sent_lane,
+                                                 uint32_t nblk, uint32_t *__restrict__ sa,
+                                                 unsigned long long *__restrict__ bad) {
+    __shared__ uint32_t s_clane[16];
+    if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
+    __syncthreads();
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblk) return;
+    const uint4 w = d.samp[b];
+    uint64_t bits = (uint64_t)w.x | ((uint64_t)w.y << 32);
+    uint32_t rk = w.z;
+    const uint32_t n = d.n;
+    while (bits) {
+        const uint32_t bit = (uint32_t)__ffsll((unsigned long long)bits) - 1u;
+        bits &= bits - 1;
+        const uint32_t rs = b * 64 + bit;
+        const uint32_t rkc = rk < d.num_samples ? rk : d.num_samples - 1;
+        rk++;
+        if (rs >= n) break;
+        uint32_t pos = d.sa_vals[rkc], cur = rs;
+        sa[rs] = pos;
+        for (uint32_t steps = 0;; steps++) {
+            RecView r;
+            load_rec(d, cur, r);
+            uint32_t lane, rank;
+            lf_lane_rank(r, cur, lane, rank);
+            if (lane == sent_lane) break; // the previous position is a sentinel
+            if (steps >= d.sample_rate || pos == 0) {
+                atomicAdd(bad, 1ull);
+                break;
+            }
+            cur = s_clane[lane] + rank;
+            pos--;
+            uint32_t v;
+            if (cur >= n || sa_sampled(d, cur, v)) break;
+            sa[cur] = pos;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sa_full_sent(DevDir d, const DevConst *__restrict__ cst, uint32_t r0,
+                                                      uint32_t nrows, uint32_t *__restrict__ sa,
+                                                      unsigned long long *__restrict__ bad) {
+    __shared__ uint32_t s_clane[16];
+    if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
+    __syncthreads();
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrows) return;
+    const uint32_t x = r0 + t, n = d.n;
+    uint32_t v;
+    if (x >= n || sa_sampled(d, x, v)) return;
+    uint32_t cur = x, steps = 0;
+    for (;;) { // resolve x
+        if (sa_sampled(d, cur, v)) break;
+        if (steps >= d.sample_rate) {
+            atomicAdd(bad, 1ull);
+            return;
+        }
+        RecView r;
+        load_rec(d, cur, r);
+        uint32_t lane, rank;
+        lf_lane_rank(r, cur, lane, rank);
+        cur = s_clane[lane] + rank;
+        if (cur >= n) {
+            atomicAdd(bad, 1ull);
+            return;
+        }
+        steps++;
+    }
+    const uint32_t px = v + steps;
+    cur = x;
+    for (uint32_t i = 0; i < steps; i++) { // the rows on the way
+        sa[cur] = px - i;
+        RecView r;
+        load_rec(d, cur, r);
+        uint32_t lane, rank;
+        lf_lane_rank(r, cur, lane, rank);
+        cur = s_clane[lane] + rank;
+    }
+}
+
+__global__ void k_count_value(const uint32_t *__restrict__ a, uint64_t n, uint32_t v,
+                              unsigned long long *__restrict__ out) {
+    uint32_t c = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        c += a[i] == v ? 1u : 0u;
+    if (c) atomicAdd(out, (unsigned long long)c);
+}
+
+int ensure_sa_full(hsfm_index *idx) {
+    DirBuffers &db = idx->fwd;
+    const char *env = getenv("HSFM_SA_FULL"); // "0": keep the LF walks (for comparisons)
+    if (env && env[0] == '0') {
+        db.view.sa_full = nullptr;
+        return HSFM_OK;
+    }
+    if (db.sa_full_tried) {
+        db.view.sa_full = db.sa_full.as<uint32_t>();
+        return HSFM_OK;
+    }
+    db.sa_full_tried = true;
+    const uint32_t n = db.view.n;
+    if (n == 0 || db.view.sample_rate == 0 || db.view.num_samples == 0) return HSFM_OK;
+    // the sentinel's lane and its rows [C[$], C[$] + count)
+    const DevConst &hc = idx->h_const;
+    uint32_t sl = 16;
+    for (uint32_t l = 0; l < hc.num_lanes && l < 16; l++)
+        if (hc.lane_sym[l] == 0) sl = l;
+    uint32_t s0 = 0, s1 = 0;
+    if (sl < 16) {
+        s0 = hc.c_lane[sl];
+        s1 = n;
+        for (uint32_t l = 0; l < hc.num_lanes && l < 16; l++)
+            if (hc.c_lane[l] > s0 && hc.c_lane[l] < s1) s1 = hc.c_lane[l];
+    }
+    hipStream_t s = idx->dev->stream;
+    DevBuf flag;
+    if (db.sa_full.ensure((size_t)n * 4 + 16) || flag.ensure(16)) { // no memory: keep the walks
+        db.sa_full.release();
+        flag.release();
+        return HSFM_OK;
+    }
+    int rc;
+    if ((rc = fill_u32(idx, db.sa_full.as<uint32_t>(), n, 0xffffffffu)) ||
+        (rc = fill_u32(idx, flag.as<uint32_t>(), 4, 0)))
+        return rc;
+    const uint32_t nblk = (n + 63) / 64;
+    unsigned long long *d_bad = flag.as<unsigned long long>();
+    hipLaunchKernelGGL(k_sa_full, dim3((nblk + 255) / 256), dim3(256), 0, s, db.view, idx->d_const.as<DevConst>(), sl,
+                       nblk, db.sa_full.as<uint32_t>(), d_bad);
+    HSFM_HIP(hipGetLastError());
+    if (s1 > s0) {
+        hipLaunchKernelGGL(k_sa_full_sent, dim3((s1 - s0 + 255) / 256), dim3(256), 0, s, db.view,
+                           idx->d_const.as<DevConst>(), s0, s1 - s0, db.sa_full.as<uint32_t>(), d_bad);
+        HSFM_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_count_value, dim3(1024), dim3(256), 0, s, db.sa_full.as<uint32_t>(), (uint64_t)n,
+                       0xffffffffu, d_bad + 1);
+    HSFM_HIP(hipGetLastError());
+    uint64_t h[2] = {0, 0};
+    if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(d_bad), h, 2))) return rc;
+    flag.release();
+    if (h[0] || h[1]) { // a walk ran past the sample rate or a row was not reached
+        db.sa_full.release();
+        return HSFM_OK;
+    }
+    db.view.sa_full = db.sa_full.as<uint32_t>();
+    return HSFM_OK;
+}
+
 int launch_resolve_fan(hsfm_index *idx, const DevDir &d, const uint32_t *d_rows, uint64_t cap,
                        const uint64_t *d_nrows, const uint32_t *d_bounds, uint32_t nbounds, uint32_t *d_out_pairs,
                        unsigned long long *d_guard, int rpt, uint32_t grid) {
@@ -263,6 +429,7 @@ int launch_resolve_fan(hsfm_index *idx, const DevDir &d, const uint32_t *d_rows,
 int launch_resolve(hsfm_index *idx, const DevDir &d, const uint32_t *d_rows, uint64_t nrows,
                    const uint32_t *d_bounds, uint32_t nbounds, uint32_t *d_out_pairs,
                    unsigned long long *d_guard) {
+    idx->stats.sa_full_rows = d.sa_full ? d.n : 0;
     if (nrows == 0) return HSFM_OK;
     const uint64_t blocks = (nrows + 255) / 256;
     int rc = event_record(idx, 1, false);

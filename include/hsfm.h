@@ -221,6 +221,8 @@ typedef struct {
                                   extension, by record slots; plain restarts      */
     double fallback_kernel_ms; /* kernels that re-ran the reads the main kernel
                                   deferred (larger sets)                          */
+    uint64_t sa_full_rows;     /* rows of the device-built full suffix array that
+                                  resolve reads (0: LF walks to the samples)      */
 } hsfm_stats;
 int hsfm_last_stats(hsfm_index *idx, hsfm_stats *out);
 
