@@ -55,6 +55,7 @@ struct Workspace {
     DevBuf mem_recs, mem_cnt, mem_off, mem_sorted, mem_hits, pos_off, positions, mem_flags, read_list;
     DevBuf wave_sets, mem_slots, mem_status;
     DevBuf items, item_cnt, item_first, read_off; // MEM work items (read segments)
+    DevBuf qnib;                              // queries as base-set nibbles
     uint64_t *h_pinned = nullptr;             // small readbacks
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}; // main, resolve, fallback kernels
 };

@@ -459,6 +459,60 @@ __device__ __forceinline__ bool seed_window_decode(const uint8_t *q, uint32_t F,
     sw.namb = na < max_amb ? na : max_amb;
     return !bad && na <= max_amb;
 }
+// Queries as base-set nibbles (k_pack_bases): 4 bits per code, eight codes per
+// word, code i of the flat query array in bits 4(i & 7) of word i >> 3.  The
+// nibble of an exact base (codes 1..4) is that base alone; of another code,
+// the bases of its seed expansion (0 when it has none).  seed_window_nib
+// decodes the window of F (<= 14) codes starting at flat position g: digits
+// of the one-base nibbles, ambiguity codes (several bases) in order.  It
+// gives the same table indices as seed_window (a code whose expansion is a
+// single base reads as that base, the one index its expansion would give).
+__device__ __forceinline__ uint32_t base_nibble(uint32_t c, const uint8_t *s_expand) {
+    return (c >= 1 && c <= 4) ? 1u << (c - 1) : (uint32_t)(s_expand[c] & 15u);
+}
+__device__ __forceinline__ void seed_window_nib_load(const uint32_t *nib, uint64_t g, uint32_t &w0, uint32_t &w1,
+                                                     uint32_t &w2) {
+    const uint32_t *b = nib + (g >> 3);
+    w0 = b[0];
+    w1 = b[1];
+    w2 = b[2];
+}
+__device__ __forceinline__ bool seed_window_nib(uint64_t g, uint32_t F, uint32_t w0, uint32_t w1, uint32_t w2,
+                                                SeedWin &sw, uint32_t max_amb) {
+    const uint32_t sh = 4 * (uint32_t)(g & 7);
+    uint64_t x = ((uint64_t)w0 | ((uint64_t)w1 << 32)) >> sh;
+    if (sh) x |= (uint64_t)w2 << (64 - sh);
+    const uint64_t ones = 0x1111111111111111ull >> (64 - 4 * F); // bit 0 of each window nibble
+    x &= ones * 15u;
+    // a nibble with no base: no expansion; with two or more: ambiguity code
+    const uint64_t any = (x | (x >> 1) | (x >> 2) | (x >> 3)) & ones;
+    const uint64_t multi = ((x & (x >> 1)) | (x & (x >> 2)) | (x & (x >> 3)) | ((x >> 1) & (x >> 2)) |
+                            ((x >> 1) & (x >> 3)) | ((x >> 2) & (x >> 3))) & ones;
+    // digit of a one-base nibble n (1, 2, 4, 8 -> 0..3): (n >> 1) - (n >> 3)
+    uint64_t d = ((x >> 1) & (ones * 7u)) - ((x >> 3) & ones);
+    d &= ~(multi * 15u);
+    // four bits per digit -> two bits per digit, in window order
+    d = (d | (d >> 2)) & 0x0f0f0f0f0f0f0f0full;
+    d = (d | (d >> 4)) & 0x00ff00ff00ff00ffull;
+    d = (d | (d >> 8)) & 0x0000ffff0000ffffull;
+    d = (d | (d >> 16)) & 0xffffffffull;
+    sw.base = (uint32_t)d;
+    sw.pos = 0;
+    sw.mask = 0x111u;
+    const uint32_t na = (uint32_t)__popcll((unsigned long long)multi);
+    sw.namb = na < max_amb ? na : max_amb;
+    uint64_t mm = multi;
+#pragma unroll
+    for (uint32_t i = 0; i < kSeedAmb; i++) {
+        if (i < sw.namb) {
+            const uint32_t bit = (uint32_t)__ffsll((unsigned long long)mm) - 1u; // 4 * position
+            mm &= mm - 1;
+            sw.pos |= (bit >> 2) << (8 * i);
+            sw.mask = (sw.mask & ~(0xfu << (4 * i))) | ((uint32_t)((x >> bit) & 15u) << (4 * i));
+        }
+    }
+    return any == ones && na <= max_amb;
+}
 // seed_window over the last F (<= 16) of 16 codes held one per register.
 __device__ __forceinline__ bool seed_window16(const uint32_t (&wb)[16], uint32_t F, const uint8_t *s_expand,
                                               SeedWin &sw) {

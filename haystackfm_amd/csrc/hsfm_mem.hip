@@ -646,6 +646,37 @@ enum : uint32_t {
     SM_DONE
 };
 
+// Base-set nibbles of the flat query array (eight codes per word; words past
+// the end hold zero nibbles).
+__global__ __launch_bounds__(256) void k_pack_bases(const uint8_t *__restrict__ q, uint64_t len,
+                                                    const DevConst *__restrict__ cst, uint32_t *__restrict__ nib,
+                                                    uint64_t nwords) {
+    __shared__ uint8_t s_expand[256];
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_expand[i] = cst->seed_expand[i];
+    __syncthreads();
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nwords;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t v = 0;
+        const uint64_t g0 = t * 8;
+#pragma unroll
+        for (uint32_t u = 0; u < 8; u++)
+            if (g0 + u < len) v |= base_nibble(q[g0 + u], s_expand) << (4 * u);
+        nib[t] = v;
+    }
+}
+
+int pack_bases(hsfm_index *idx, const uint8_t *d_q, uint64_t len, DevBuf &out) {
+    const uint64_t nwords = (len + 7) / 8 + 4; // +4: the window reads three words from any start
+    int rc = out.ensure(nwords * 4);
+    if (rc) return rc;
+    uint64_t blocks = (nwords + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_pack_bases, dim3((uint32_t)blocks), dim3(256), 0, idx->dev->stream, d_q, len,
+                       idx->d_const.as<DevConst>(), out.as<uint32_t>(), nwords);
+    HSFM_HIP(hipGetLastError());
+    return HSFM_OK;
+}
+
 constexpr uint32_t kPersistChunk = 64; // work indices a wave claims per atomic
 // 256-thread blocks of the ring kernel fit three per CU by LDS; the second
 // launch bound keeps the registers within the same three waves per SIMD.
@@ -653,6 +684,7 @@ constexpr uint32_t kPersistChunk = 64; // work indices a wave claims per atomic
 template <int PR, int PB>
 __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fwd, DevDir rev, const DevConst *__restrict__ cst,
                                                     SeedView sv, const uint8_t *__restrict__ q,
+                                                    const uint32_t *__restrict__ qnib,
                                                     const uint64_t *__restrict__ qoff,
                                                     const uint32_t *__restrict__ list, uint32_t nlist,
                                                     const uint4 *__restrict__ items,
@@ -747,10 +779,10 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
                 // each); windows with more restart unseeded
                 bool usable = false;
                 if (seeding && j > F) {
-                    uint4 wd;
-                    uint32_t wd4;
-                    seed_window_load(p + j - F, F, wd, wd4);
-                    usable = seed_window_decode(p + j - F, F, wd, wd4, s_expand, sw, kSeedAmb);
+                    const uint64_t g = (uint64_t)(p - q) + j - F; // flat position of the window
+                    uint32_t w0, w1, w2;
+                    seed_window_nib_load(qnib, g, w0, w1, w2);
+                    usable = seed_window_nib(g, F, w0, w1, w2, sw, kSeedAmb);
                 }
                 nn = 0;
                 t = 0;
@@ -1583,6 +1615,7 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         if ((rc = readback_u64(idx, d_qoff + nq, &total_len, 1))) return rc;
         const uint64_t nslots = (total_len >> 3) + 8ull * W + 8;
         if ((rc = ws.mem_slots.ensure(nslots * 16 + 16))) return rc;
+        if ((rc = pack_bases(idx, d_q, total_len, ws.qnib))) return rc;
         if ((rc = ws.mem_status.ensure(W + 16))) return rc;
         sv = SeedView{idx->seed_off.as<uint64_t>(), idx->seed_iv.as<uint4>(), idx->seed_lsl.as<uint8_t>(), F};
         SeedView svp = sv;
@@ -1611,7 +1644,8 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         } else {
             const uint32_t g1 = persist_grid(idx, k_mem_persist<16, 256>, 256);
             hipLaunchKernelGGL((k_mem_persist<16, 256>), dim3(g1), dim3(256), 0, s, idx->fwd.view, idx->rev.view,
-                               idx->d_const.as<DevConst>(), svp, d_q, d_qoff, (const uint32_t *)nullptr, (uint32_t)W,
+                               idx->d_const.as<DevConst>(), svp, d_q, ws.qnib.as<uint32_t>(), d_qoff,
+                               (const uint32_t *)nullptr, (uint32_t)W,
                                items, min_len, ws.mem_slots.as<uint4>(), ws.mem_cnt.as<uint32_t>(),
                                ws.mem_status.as<uint8_t>(), ws.ovf_list.as<uint32_t>(), ctr);
         }
@@ -1861,11 +1895,12 @@ struct BatchBufs {
     DevBuf bq_off, slots, mem_cnt, status, pass_of, l1, l2, l3, ctr, recs, wsets;
     DevBuf mem_off, sorted, npos, pos_off, hits, rows, positions, misc, tmp;
     DevBuf items, item_cnt, item_first, read_off;
+    DevBuf qnib; // all queries as base-set nibbles (read by every batch's graph)
     size_t tmp_bytes = 0;
     void release() {
         DevBuf *all[] = {&bq_off, &slots, &mem_cnt, &status, &pass_of, &l1, &l2, &l3, &ctr, &recs, &wsets,
                          &mem_off, &sorted, &npos, &pos_off, &hits, &rows, &positions, &misc, &tmp,
-                         &items, &item_cnt, &item_first, &read_off};
+                         &items, &item_cnt, &item_first, &read_off, &qnib};
         for (DevBuf *b : all) b->release();
     }
 };
@@ -1896,7 +1931,7 @@ static int batch_scan(hsfm_index *idx, BatchBufs &bb, const uint32_t *in, uint64
 
 // The launches of one batch (captured into a graph, or run eagerly).
 static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, const SeedView &svp,
-                        const uint8_t *d_q, uint32_t min_len,
+                        const uint8_t *d_q, const uint32_t *qnib, uint32_t min_len,
                         const uint32_t *d_refb, uint32_t n_refs, uint32_t max_hits, int main_kernel, int rpt) {
     hipStream_t s = idx->dev->stream;
     const uint32_t B = bb.B, W = bb.W;
@@ -1929,16 +1964,16 @@ static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, cons
         uint8_t *st = bb.status.as<uint8_t>();
         if (main_kernel == BM_PERSIST8)
             hipLaunchKernelGGL((k_mem_persist<8, 256>), dim3(persist_grid(idx, k_mem_persist<8, 256>, 256)), dim3(256),
-                               0, s, idx->fwd.view, idx->rev.view, cst, svp, d_q, bq, (const uint32_t *)nullptr, W,
+                               0, s, idx->fwd.view, idx->rev.view, cst, svp, d_q, qnib, bq, (const uint32_t *)nullptr, W,
                                items, min_len, slots, cnt, st, bb.l1.as<uint32_t>(), &c[0]);
         else if (main_kernel == BM_PERSIST32)
             hipLaunchKernelGGL((k_mem_persist<32, 128>), dim3(persist_grid(idx, k_mem_persist<32, 128>, 128)),
-                               dim3(128), 0, s, idx->fwd.view, idx->rev.view, cst, svp, d_q, bq,
+                               dim3(128), 0, s, idx->fwd.view, idx->rev.view, cst, svp, d_q, qnib, bq,
                                (const uint32_t *)nullptr, W, items, min_len, slots, cnt, st, bb.l1.as<uint32_t>(),
                                &c[0]);
         else
             hipLaunchKernelGGL((k_mem_persist<16, 256>), dim3(persist_grid(idx, k_mem_persist<16, 256>, 256)),
-                               dim3(256), 0, s, idx->fwd.view, idx->rev.view, cst, svp, d_q, bq,
+                               dim3(256), 0, s, idx->fwd.view, idx->rev.view, cst, svp, d_q, qnib, bq,
                                (const uint32_t *)nullptr, W, items, min_len, slots, cnt, st, bb.l1.as<uint32_t>(),
                                &c[0]);
         HSFM_HIP(hipGetLastError());
@@ -1999,12 +2034,12 @@ struct BatchGraph {
 };
 
 static int batch_capture(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, const SeedView &svp,
-                         const uint8_t *d_q, uint32_t min_len,
+                         const uint8_t *d_q, const uint32_t *qnib, uint32_t min_len,
                          const uint32_t *d_refb, uint32_t n_refs, uint32_t max_hits, int main_kernel, int rpt,
                          BatchGraph &out) {
     hipStream_t s = idx->dev->stream;
     HSFM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int rc = batch_record(idx, bb, sv, svp, d_q, min_len, d_refb, n_refs, max_hits, main_kernel, rpt);
+    int rc = batch_record(idx, bb, sv, svp, d_q, qnib, min_len, d_refb, n_refs, max_hits, main_kernel, rpt);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(s, &g);
     if (rc) {
@@ -2069,6 +2104,10 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     const uint32_t B = cfg->batch_reads < nq ? cfg->batch_reads : (nq ? nq : 1);
     BatchBufs bb;
     bb.B = B;
+    if ((rc = pack_bases(idx, d_q, hoff[nq], bb.qnib))) {
+        bb.release();
+        return rc;
+    }
     bb.seg = mem_segment_length();
     if (!bb.seg) bb.seg = 0xffffffffu; // one item per read
     // item capacity and slot ranges over all batches (slot index of item w:
@@ -2142,7 +2181,7 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
             const uint32_t nb0 = B < nq ? B : nq;
             auto time_cfg = [&](int mk, int rp, double &ms) -> int {
                 BatchGraph gr;
-                int r2 = batch_capture(idx, bb, sv, svp, d_q, min_len, d_refb, n_refs, max_hits, mk, rp, gr);
+                int r2 = batch_capture(idx, bb, sv, svp, d_q, bb.qnib.as<uint32_t>(), min_len, d_refb, n_refs, max_hits, mk, rp, gr);
                 if (r2) return r2;
                 ms = 1e30;
                 for (int rep = 0; rep < 2; rep++) {
@@ -2200,7 +2239,7 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     stats.main_kernel = main_kernel;
     stats.resolve_fanout = rpt;
     BatchGraph gr;
-    if (cfg->use_graph && (rc = batch_capture(idx, bb, sv, svp, d_q, min_len, d_refb, n_refs, max_hits, main_kernel, rpt,
+    if (cfg->use_graph && (rc = batch_capture(idx, bb, sv, svp, d_q, bb.qnib.as<uint32_t>(), min_len, d_refb, n_refs, max_hits, main_kernel, rpt,
                                               gr)))
         return cleanup(rc);
     HSFM_HIP(hipEventRecord(e0, s));
@@ -2212,7 +2251,7 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
                 rc = set_error(HSFM_EDEVICE, "hipGraphLaunch failed");
                 break;
             }
-        } else if ((rc = batch_record(idx, bb, sv, svp, d_q, min_len, d_refb, n_refs, max_hits, main_kernel, rpt))) {
+        } else if ((rc = batch_record(idx, bb, sv, svp, d_q, bb.qnib.as<uint32_t>(), min_len, d_refb, n_refs, max_hits, main_kernel, rpt))) {
             break;
         }
         // one wait per batch: counts and overflow flags
