@@ -72,22 +72,28 @@ __device__ __forceinline__ uint32_t locate_read(const DevDir &d, const DevConst 
                                                 const uint64_t *qoff, uint32_t r, uint32_t *cnt, uint8_t *niv,
                                                 uint2 *ivs, uint32_t *ovf_list, LocateCounters *ctr,
                                                 const uint16_t *s_compat, const int8_t *s_digit,
-                                                const uint32_t *s_clane);
+                                                const uint32_t *s_clane, const uint4 *seed_rec, uint32_t F,
+                                                const uint8_t *s_expand);
 
 __global__ __launch_bounds__(256) void k_locate_search(DevDir d, const DevConst *__restrict__ cst,
                                                        const uint8_t *__restrict__ q,
                                                        const uint64_t *__restrict__ qoff, uint32_t nq,
                                                        uint32_t *__restrict__ cnt, uint8_t *__restrict__ niv,
                                                        uint2 *__restrict__ ivs, uint32_t *__restrict__ ovf_list,
-                                                       LocateCounters *__restrict__ ctr) {
+                                                       LocateCounters *__restrict__ ctr,
+                                                       const uint4 *__restrict__ seed_rec, uint32_t F) {
     __shared__ uint16_t s_compat[256];
     __shared__ int8_t s_digit[256];
+    __shared__ uint8_t s_expand[256];
     __shared__ uint32_t s_clane[16];
     load_const_lds(cst, s_compat, s_digit, s_clane);
+    for (uint32_t x = threadIdx.x; x < 256; x += blockDim.x) s_expand[x] = cst->seed_expand[x];
     __syncthreads();
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t probes = 0;
-    if (r < nq) probes = locate_read(d, cst, q, qoff, r, cnt, niv, ivs, ovf_list, ctr, s_compat, s_digit, s_clane);
+    if (r < nq)
+        probes = locate_read(d, cst, q, qoff, r, cnt, niv, ivs, ovf_list, ctr, s_compat, s_digit, s_clane, seed_rec,
+                             F, s_expand);
 #pragma unroll
     for (int dd = 32; dd >= 1; dd >>= 1) probes += __shfl_xor(probes, dd, 64);
     if ((threadIdx.x & 63) == 0) atomicAdd(&ctr->probes, (unsigned long long)probes);
@@ -99,7 +105,8 @@ __device__ __forceinline__ uint32_t locate_read(const DevDir &d, const DevConst 
                                                 const uint64_t *qoff, uint32_t r, uint32_t *cnt, uint8_t *niv,
                                                 uint2 *ivs, uint32_t *ovf_list, LocateCounters *ctr,
                                                 const uint16_t *s_compat, const int8_t *s_digit,
-                                                const uint32_t *s_clane) {
+                                                const uint32_t *s_clane, const uint4 *seed_rec, uint32_t F,
+                                                const uint8_t *s_expand) {
     uint32_t probes = 0;
     const uint32_t L = cst->num_lanes;
     const uint64_t b0 = qoff[r];
@@ -115,6 +122,51 @@ __device__ __forceinline__ uint32_t locate_read(const DevDir &d, const DevConst 
     if (sd) probes++;
     if (sd == 2) ns = 0;
     else if (sd == 1) start = (int)(m - cst->lookup_depth);
+    else if (seed_rec && m >= F) {
+        // the lookup cannot seed (an ambiguity code among the last codes):
+        // the packed F-mer records of the expansions of q[m-F..m) (one
+        // ambiguity code: at most four, loaded together) give the set of
+        // every compatible string when each holds a single interval
+        SeedWin sw;
+        if (seed_window(p + m - F, F, s_expand, sw, 1)) {
+            uint4 R[4];
+            bool val[4];
+#pragma unroll
+            for (uint32_t c = 0; c < 4; c++) {
+                uint32_t ix = 0;
+                val[c] = c < seed_ncombo(sw) && seed_combo(sw, c, ix);
+                R[c] = val[c] ? seed_rec[ix] : make_uint4(0, 0, 0, 0);
+            }
+            bool ok = true;
+            uint32_t k2 = 0;
+#pragma unroll
+            for (uint32_t c = 0; c < 4; c++) {
+                if (!val[c]) continue;
+                probes++;
+                const uint32_t l = R[c].w & ((1u << kSeedLslBits) - 1u), n2 = R[c].w >> kSeedLslBits;
+                if (l != F || n2 == 0) continue;
+                if (n2 > 1) ok = false;
+                bool dup = false;
+#pragma unroll
+                for (uint32_t t = 0; t < KR; t++) dup |= t < k2 && lo[t] == R[c].x;
+                if (!dup && k2 < KR) {
+#pragma unroll
+                    for (uint32_t t = 0; t < KR; t++)
+                        if (t == k2) { lo[t] = R[c].x; hi[t] = R[c].y; }
+                    k2++;
+                }
+            }
+            if (ok) {
+                ns = k2;
+                start = (int)(m - F);
+            } else {
+                lo[0] = 0;
+                hi[0] = d.n;
+#pragma unroll
+                for (uint32_t t = 1; t < KR; t++) { lo[t] = 0xffffffffu; hi[t] = 0xffffffffu; }
+            }
+        }
+    }
     for (int i = start - 1; i >= 0 && ns; i--) {
         const uint32_t cm = s_compat[p[i]];
         uint32_t nlo[KR], nhi[KR], nn = 0;
@@ -686,7 +738,7 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
     SeedView sv{nullptr, nullptr, nullptr, 0};
     if (nq) {
         const uint32_t F = 12;
-        if (idx->fwd.view.n > (1u << 20) && (rc = build_seed_table(idx, F))) return rc;
+        if (idx->fwd.view.n > (1u << 20) && (rc = build_seed_packed(idx, F))) return rc;
         if (idx->seed_F)
             sv = SeedView{idx->seed_off.as<uint64_t>(), idx->seed_iv.as<uint4>(), idx->seed_lsl.as<uint8_t>(),
                           idx->seed_F};
@@ -700,7 +752,8 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
         if (!pipe_all)
             hipLaunchKernelGGL(k_locate_search, dim3((nq + 255) / 256), dim3(256), 0, s, d,
                                idx->d_const.as<DevConst>(), d_q, d_qoff, nq, ws.cnt.as<uint32_t>(),
-                               ws.niv.as<uint8_t>(), ws.ivs.as<uint2>(), ws.ovf_list.as<uint32_t>(), ctr);
+                               ws.niv.as<uint8_t>(), ws.ivs.as<uint2>(), ws.ovf_list.as<uint32_t>(), ctr,
+                               idx->seed_rec12.as<uint4>(), idx->seed_rec12.p ? 12u : 0u);
         HSFM_HIP(hipGetLastError());
         if ((rc = event_record(idx, 0, true))) return rc;
         if ((rc = event_record(idx, 2, false))) return rc;
