@@ -241,8 +241,10 @@ void free_dir(DirBuffers &d) {
     d.sa_vals.release();
     d.lookup.release();
     d.sa_full.release();
+    d.text.release();
     d.sa_full_tried = false;
     d.view.sa_full = nullptr;
+    d.view.text = nullptr;
 }
 
 void free_workspace(Workspace &w) {

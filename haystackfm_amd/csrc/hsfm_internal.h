@@ -61,6 +61,8 @@ struct DevDir {
     // SA value of every row, built on the device from the samples (resolve
     // becomes one load), or null
     const uint32_t *sa_full;
+    // lane of every text position (built with sa_full; padded by 16 bytes), or null
+    const uint8_t *text;
     uint32_t num_samples;
     uint32_t sample_rate;
     uint32_t num_lanes;

@@ -641,6 +641,8 @@ enum : uint32_t {
     SM_EMIT,      // right extension ended at e: write the records
     SM_SEEDE,     // probe: one seed entry, extended left by q[j-F-1]
     SM_SEEDR,     // load: the packed seed record of one expansion
+    SM_TSA,       // load: text position of a single-row set (right extension)
+    SM_TEXT,      // load: 8 text and 8 query codes, compared directly
     SM_LEFT,      // probe: one interval of the set of q[k..j), by q[k-1]
     SM_RIGHT,     // probe: one interval of the set of q[k..e), by q[e]
     SM_DONE
@@ -715,6 +717,11 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
     const uint32_t n = fwd.n;
     const uint32_t F = sv.F;
     const bool seeding = F != 0 && min_len > F;
+    // right extensions of a single row compare the text directly: the row's
+    // only occurrence continues while the text lane is compatible with the
+    // query code (no code is compatible with the sentinel, so it stops there
+    // as the index extension does), and the row stays the same
+    const bool textmode = fwd.text != nullptr && fwd.sa_full != nullptr;
 
     uint32_t st = SM_FETCH, w = 0, r = 0, m = 0, ia = 0, ib = 0, s0 = 0, E = 0, j = 0, k = 0, e = 0;
     uint32_t ns = 0, nn = 0, t = 0, head = 0, nrec = 0, slot_cap = 0;
@@ -833,7 +840,7 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
                 e = j; t = 0; nn = 0;
                 if (e < m) {
                     piv = make_uint4(s_lo[head][tid], s_hi[head][tid], s_rl[head][tid], 0u);
-                    st = SM_RIGHT;
+                    st = (textmode && ns == 1 && piv.y - piv.x == 1) ? SM_TSA : SM_RIGHT;
                 } else {
                     st = SM_EMIT;
                 }
@@ -876,11 +883,19 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
         // ---- memory pass: a seed record, or one interval extended by one code.
         // The record load is issued first and consumed after the probes, so
         // its latency overlaps theirs instead of stalling the wave on its own.
-        const bool srec = st == SM_SEEDR;
+        const bool srec = st == SM_SEEDR, stext = st == SM_TSA || st == SM_TEXT;
         uint4 R = make_uint4(0u, 0u, 0u, 0u);
-        if (srec) R = sv.rec[sidx];
+        if (srec) {
+            R = sv.rec[sidx];
+        } else if (st == SM_TSA) {
+            R.x = fwd.sa_full[piv.x];
+        } else if (st == SM_TEXT) {
+            const uint2 tw = *reinterpret_cast<const uint2 *>(fwd.text + ((sidx + e) & ~7u));
+            const uint2 qw = *reinterpret_cast<const uint2 *>(reinterpret_cast<uintptr_t>(p + e) & ~(uintptr_t)7);
+            R = make_uint4(tw.x, tw.y, qw.x, qw.y);
+        }
         bool ovf = false;
-        if (!srec) {
+        if (!srec && !stext) {
             const bool left = st != SM_RIGHT;
             const uint32_t size = piv.y - piv.x;
             const uint32_t xlo = left ? piv.x : piv.z;
@@ -943,6 +958,29 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
                 probes++;
                 st = SM_SEEDE;
             }
+        } else if (st == SM_TSA) {
+            probes++;
+            sidx = R.x - k; // text position of query code e is sidx + e
+            st = SM_TEXT;
+        } else if (st == SM_TEXT) {
+            probes++;
+            const uint32_t to = (sidx + e) & 7u;
+            const uint32_t qo = (uint32_t)(reinterpret_cast<uintptr_t>(p + e) & 7);
+            const uint64_t tw = (uint64_t)R.x | ((uint64_t)R.y << 32), qw = (uint64_t)R.z | ((uint64_t)R.w << 32);
+            const uint32_t lim = min(min(8u - to, 8u - qo), m - e);
+            uint32_t adv = 0;
+            bool go = true;
+#pragma unroll
+            for (uint32_t i = 0; i < 8; i++) {
+                if (i < lim && go) {
+                    const uint32_t tl = (uint32_t)(tw >> (8 * (to + i))) & 0xffu;
+                    const uint32_t qc = (uint32_t)(qw >> (8 * (qo + i))) & 0xffu;
+                    if ((s_compat[qc] >> tl) & 1u) adv++;
+                    else go = false;
+                }
+            }
+            e += adv;
+            if (!go || e >= m) st = SM_EMIT;
         } else if (ovf) {
             status[w] = 1;
             mem_cnt[w] = 0;
@@ -967,7 +1005,10 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
             bool more;
             if (st == SM_LEFT) { k--; more = k > 0; if (!more) st = SM_ENDLEFT; }
             else { e++; more = e < m; if (!more) st = SM_EMIT; }
-            if (more) piv = make_uint4(s_lo[head][tid], s_hi[head][tid], s_rl[head][tid], 0u);
+            if (more) {
+                piv = make_uint4(s_lo[head][tid], s_hi[head][tid], s_rl[head][tid], 0u);
+                if (st == SM_RIGHT && textmode && ns == 1 && piv.y - piv.x == 1) st = SM_TSA;
+            }
         }
     }
     if (restarts) atomicAdd(&ctr->diag[3], (unsigned long long)restarts);

@@ -339,6 +339,33 @@ __global__ __launch_bounds__(256) void k_sa_full_sent(DevDir d, const DevConst *
     }
 }
 
+// Text lanes from the full suffix array: the first symbol of row r is the lane
+// whose C range holds r (lane counts from the record at n), and row r starts
+// at text position sa[r].
+__global__ __launch_bounds__(256) void k_text_from_sa(DevDir d, const DevConst *__restrict__ cst,
+                                                      const uint32_t *__restrict__ sa, uint8_t *__restrict__ text) {
+    __shared__ uint32_t s_lo[16], s_hi[16];
+    if (threadIdx.x == 0) {
+        RecView rv;
+        load_rec(d, d.n, rv);
+        uint32_t cnt[16];
+        lane_counts(rv, d.n, 16, cnt);
+        for (int l = 0; l < 16; l++) {
+            s_lo[l] = cst->c_lane[l];
+            s_hi[l] = (uint32_t)l < cst->num_lanes ? cst->c_lane[l] + cnt[l] : 0u;
+        }
+    }
+    __syncthreads();
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= d.n) return;
+    uint32_t lane = 0;
+#pragma unroll
+    for (int l = 0; l < 16; l++)
+        if (r >= s_lo[l] && r < s_hi[l]) lane = (uint32_t)l;
+    const uint32_t p = sa[r];
+    if (p < d.n) text[p] = (uint8_t)lane;
+}
+
 __global__ void k_count_value(const uint32_t *__restrict__ a, uint64_t n, uint32_t v,
                               unsigned long long *__restrict__ out) {
     uint32_t c = 0;
@@ -352,10 +379,12 @@ int ensure_sa_full(hsfm_index *idx) {
     const char *env = getenv("HSFM_SA_FULL"); // "0": keep the LF walks (for comparisons)
     if (env && env[0] == '0') {
         db.view.sa_full = nullptr;
+        db.view.text = nullptr;
         return HSFM_OK;
     }
     if (db.sa_full_tried) {
         db.view.sa_full = db.sa_full.as<uint32_t>();
+        db.view.text = db.text.as<uint8_t>();
         return HSFM_OK;
     }
     db.sa_full_tried = true;
@@ -405,6 +434,17 @@ int ensure_sa_full(hsfm_index *idx) {
         return HSFM_OK;
     }
     db.view.sa_full = db.sa_full.as<uint32_t>();
+    // the text as lanes (right extensions of single rows compare it directly)
+    if (!db.text.ensure((size_t)n + 16)) {
+        if ((rc = fill_u32(idx, db.text.as<uint32_t>(), ((size_t)n + 16) / 4, 0))) return rc;
+        hipLaunchKernelGGL(k_text_from_sa, dim3((n + 255) / 256), dim3(256), 0, s, db.view,
+                           idx->d_const.as<DevConst>(), db.sa_full.as<uint32_t>(), db.text.as<uint8_t>());
+        HSFM_HIP(hipGetLastError());
+        HSFM_HIP(hipStreamSynchronize(s));
+        db.view.text = db.text.as<uint8_t>();
+    } else {
+        db.text.release();
+    }
     return HSFM_OK;
 }
 
