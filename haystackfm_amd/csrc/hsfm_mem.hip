@@ -259,6 +259,28 @@ __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const D
             }
             uint32_t e = j;
             while (e < m) {
+                if (fwd.text && fwd.sa_full && ns == 1 && A[0].y - A[0].x == 1) {
+                    // a single row: its only occurrence continues while the text
+                    // is compatible (64 codes per step, first wave); the row and
+                    // so the record stay the same
+                    if (threadIdx.x < 64) {
+                        const uint32_t toff = fwd.sa_full[A[0].x] - k; // text position of code e: toff + e
+                        for (;;) {
+                            const uint32_t x = e + lane;
+                            bool ok = x < m && toff + x < fwd.n; // the sentinel at n - 1 stops it first
+                            if (ok) ok = (s_compat[p[x]] >> fwd.text[toff + x]) & 1u;
+                            const uint64_t bad = __ballot(!ok);
+                            const uint32_t adv = bad ? (uint32_t)__ffsll((unsigned long long)bad) - 1u : 64u;
+                            e += adv;
+                            if (adv < 64) break;
+                        }
+                        if (threadIdx.x == 0) s_b32 = e;
+                    }
+                    __syncthreads();
+                    e = s_b32;
+                    __syncthreads();
+                    break;
+                }
                 const uint32_t c2 = group_extend<false, TB>(rev, s_clane, s_compat[p[e]], A, ns, B, cap, too_big,
                                                             s_part);
                 __syncthreads();
