@@ -192,17 +192,33 @@ __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const D
                         const uint32_t xi = x0 + lane;
                         uint32_t idx = 0, cnt = 0;
                         uint64_t o0 = 0;
+                        uint4 one = make_uint4(0, 0, 0, 0);
                         if (xi < nc && wide_combo(sw, xi, idx)) {
-                            const uint32_t l = sv.lsl[idx];
-                            atomicMax(&s_lsl, l);
-                            if (l == F) {
-                                o0 = sv.off[idx];
-                                cnt = (uint32_t)(sv.off[idx + 1] - o0);
+                            if (sv.rec) { // packed record: one load per expansion
+                                const uint4 R = sv.rec[idx];
+                                const uint32_t l = R.w & ((1u << kSeedLslBits) - 1u);
+                                atomicMax(&s_lsl, l);
+                                if (l == F) {
+                                    cnt = R.w >> kSeedLslBits;
+                                    o0 = (uint64_t)R.x | ((uint64_t)R.y << 32);
+                                    one = make_uint4(R.x, R.y, R.z, 0u);
+                                }
+                            } else {
+                                const uint32_t l = sv.lsl[idx];
+                                atomicMax(&s_lsl, l);
+                                if (l == F) {
+                                    o0 = sv.off[idx];
+                                    cnt = (uint32_t)(sv.off[idx + 1] - o0);
+                                }
                             }
                         }
                         uint32_t tot;
                         const uint32_t at = total + block_incl_scan<TB>(cnt, s_part, tot) - cnt;
-                        for (uint32_t t = 0; t < cnt && at + t < cap; t++) A[at + t] = sv.iv[o0 + t];
+                        if (sv.rec && cnt == 1) {
+                            if (at < cap) A[at] = one;
+                        } else {
+                            for (uint32_t t = 0; t < cnt && at + t < cap; t++) A[at + t] = sv.iv[o0 + t];
+                        }
                         total += tot;
                     }
                     __syncthreads();
@@ -1671,6 +1687,7 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     const uint32_t *list = nullptr; // identity list
     uint64_t nlist = W;
     SeedView sv{nullptr, nullptr, nullptr, 0};
+    SeedView svw = sv; // seeds of the workgroup passes
     uint32_t *retry = ws.ovf_list.as<uint32_t>();
     if (use_thread) {
         if ((rc = build_seed_table(idx, F))) return rc;
@@ -1689,6 +1706,7 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
             svp.rec = idx->seed_rec12.as<uint4>();
         else
             svp.F = 0; // min_len 12: no seeding in the persistent kernel
+        svw = svp;
         const bool old_thread = path && strcmp(path, "thread") == 0;
         uint64_t hd[16];
         if ((rc = event_record(idx, 0, false))) return rc;
@@ -1762,12 +1780,12 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         if (lds)
             hipLaunchKernelGGL((k_mem_wave<true, 64>), dim3((uint32_t)waves), dim3(64), 0, s, idx->fwd.view,
                                idx->rev.view,
-                               idx->d_const.as<DevConst>(), sv, d_q, d_qoff, list, (uint32_t)nlist, items, min_len, cap,
+                               idx->d_const.as<DevConst>(), svw, d_q, d_qoff, list, (uint32_t)nlist, items, min_len, cap,
                                (uint4 *)nullptr, ws.mem_recs.as<MemRec>(), rec_cap, ws.mem_cnt.as<uint32_t>(),
                                ws.mem_flags.as<uint32_t>(), pass, retry, ctr, nullptr);
         else
             hipLaunchKernelGGL((k_mem_wave<false, 1024>), dim3((uint32_t)waves), dim3(1024), 0, s, idx->fwd.view,
-                               idx->rev.view, idx->d_const.as<DevConst>(), sv, d_q, d_qoff, list, (uint32_t)nlist,
+                               idx->rev.view, idx->d_const.as<DevConst>(), svw, d_q, d_qoff, list, (uint32_t)nlist,
                                items, min_len, cap, ws.wave_sets.as<uint4>(), ws.mem_recs.as<MemRec>(), rec_cap,
                                ws.mem_cnt.as<uint32_t>(), ws.mem_flags.as<uint32_t>(), pass, retry, ctr, nullptr);
         HSFM_HIP(hipGetLastError());
@@ -2017,7 +2035,7 @@ static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, cons
     if (main_kernel == BM_WAVE) {
         // one wave per read for every read
         hipLaunchKernelGGL((k_mem_wave<true, 64>), dim3(lds_waves), dim3(64), 0, s, idx->fwd.view, idx->rev.view, cst,
-                           sv, d_q, bq, (const uint32_t *)nullptr, W, items, min_len, kWaveLdsCap, (uint4 *)nullptr, r0,
+                           svp, d_q, bq, (const uint32_t *)nullptr, W, items, min_len, kWaveLdsCap, (uint4 *)nullptr, r0,
                            half,
                            bb.mem_cnt.as<uint32_t>(), bb.pass_of.as<uint32_t>(), 0u, bb.l2.as<uint32_t>(), &c[1],
                            (const unsigned long long *)nullptr);
@@ -2042,14 +2060,14 @@ static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, cons
         HSFM_HIP(hipGetLastError());
         // deferred reads: one wave each, sets in LDS
         hipLaunchKernelGGL((k_mem_wave<true, 64>), dim3(lds_waves), dim3(64), 0, s, idx->fwd.view, idx->rev.view, cst,
-                           sv, d_q, bq, bb.l1.as<uint32_t>(), W, items, min_len, kWaveLdsCap, (uint4 *)nullptr, r0,
+                           svp, d_q, bq, bb.l1.as<uint32_t>(), W, items, min_len, kWaveLdsCap, (uint4 *)nullptr, r0,
                            half, cnt,
                            bb.pass_of.as<uint32_t>(), 0u, bb.l2.as<uint32_t>(), &c[1], second_count);
     }
     HSFM_HIP(hipGetLastError());
     // sets beyond the LDS: 1024 threads per read, global sets
     hipLaunchKernelGGL((k_mem_wave<false, 1024>), dim3(kBatchGlobalWaves), dim3(1024), 0, s, idx->fwd.view,
-                       idx->rev.view, cst, sv, d_q, bq, bb.l2.as<uint32_t>(), W, items, min_len, kBatchGlobalCap,
+                       idx->rev.view, cst, svp, d_q, bq, bb.l2.as<uint32_t>(), W, items, min_len, kBatchGlobalCap,
                        bb.wsets.as<uint4>(), r1, half, bb.mem_cnt.as<uint32_t>(), bb.pass_of.as<uint32_t>(), 1u,
                        bb.l3.as<uint32_t>(), &c[2], &c[1].n_retry);
     HSFM_HIP(hipGetLastError());

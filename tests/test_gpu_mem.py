@@ -204,10 +204,13 @@ def test_packed_seed_tables_12_and_14(monkeypatch, iupac):
             check(o, g, qs, min_len, modes=(0,))
 
 
-def test_ambiguity_clusters_in_seed_windows():
+@pytest.mark.parametrize("seed_f", ["12", "14"])
+def test_ambiguity_clusters_in_seed_windows(monkeypatch, seed_f):
     """Reads with 4-7 ambiguity codes inside one seed window: the ring kernel
     hands them to the workgroup kernel, which expands up to five codes over its
-    threads and restarts unseeded beyond that; all stay exact."""
+    threads (one packed 12- or 14-mer record each) and restarts unseeded beyond
+    that; all stay exact."""
+    monkeypatch.setenv("HSFM_SEED_F", seed_f)
     rng = np.random.default_rng(77)
     text, bounds = O.gen_reference(78, 200_000, 3, gc=0.41, repeat_frac=0.2, n_units=20, unit_min=300,
                                    unit_max=1500, iupac_frac=0.005)
