@@ -694,14 +694,29 @@ __global__ __launch_bounds__(256) void k_pack_bases(const uint8_t *__restrict__ 
     __shared__ uint8_t s_expand[256];
     for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_expand[i] = cst->seed_expand[i];
     __syncthreads();
-    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nwords;
+    // two words (16 codes) per thread; whole 16-byte groups with one load when
+    // the array is 16-byte aligned
+    const bool vec = (reinterpret_cast<uintptr_t>(q) & 15) == 0;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; 2 * t < nwords;
          t += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t v = 0;
-        const uint64_t g0 = t * 8;
+        const uint64_t g0 = t * 16;
+        uint32_t b[4] = {0, 0, 0, 0};
+        if (vec && g0 + 16 <= len) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(q + g0);
+            b[0] = v.x; b[1] = v.y; b[2] = v.z; b[3] = v.w;
+        } else {
+            for (uint32_t u = 0; u < 16; u++)
+                if (g0 + u < len) b[u >> 2] |= (uint32_t)q[g0 + u] << (8 * (u & 3));
+        }
+        uint32_t w0 = 0, w1 = 0;
 #pragma unroll
-        for (uint32_t u = 0; u < 8; u++)
-            if (g0 + u < len) v |= base_nibble(q[g0 + u], s_expand) << (4 * u);
-        nib[t] = v;
+        for (uint32_t u = 0; u < 16; u++) {
+            const uint32_t nb = g0 + u < len ? base_nibble((b[u >> 2] >> (8 * (u & 3))) & 0xffu, s_expand) : 0u;
+            if (u < 8) w0 |= nb << (4 * u);
+            else w1 |= nb << (4 * (u - 8));
+        }
+        nib[2 * t] = w0;
+        if (2 * t + 1 < nwords) nib[2 * t + 1] = w1;
     }
 }
 
@@ -709,7 +724,7 @@ int pack_bases(hsfm_index *idx, const uint8_t *d_q, uint64_t len, DevBuf &out) {
     const uint64_t nwords = (len + 7) / 8 + 4; // +4: the window reads three words from any start
     int rc = out.ensure(nwords * 4);
     if (rc) return rc;
-    uint64_t blocks = (nwords + 255) / 256;
+    uint64_t blocks = (nwords / 2 + 256) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_pack_bases, dim3((uint32_t)blocks), dim3(256), 0, idx->dev->stream, d_q, len,
                        idx->d_const.as<DevConst>(), out.as<uint32_t>(), nwords);
@@ -1656,6 +1671,8 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     const uint32_t seg = legacy ? 0u : mem_segment_length();
     uint64_t W = nq;
     const uint4 *items = nullptr;
+    uint64_t total_len = 0; // flat length of the queries (read back with the item count)
+    if (nq && !(seg && nq) && (rc = readback_u64(idx, d_qoff + nq, &total_len, 1))) return rc;
     if (seg && nq) {
         if ((rc = ws.item_cnt.ensure((size_t)nq * 4 + 4)) || (rc = ws.item_first.ensure(((size_t)nq + 1) * 8)))
             return rc;
@@ -1664,7 +1681,14 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         HSFM_HIP(hipGetLastError());
         if ((rc = exclusive_scan_u32_u64(idx, ws.item_cnt.as<uint32_t>(), ws.item_first.as<uint64_t>(), nq)))
             return rc;
-        if ((rc = readback_u64(idx, ws.item_first.as<uint64_t>() + nq, &W, 1))) return rc;
+        {   // item count and flat length in one round trip
+            if (!ws.h_pinned) HSFM_HIP(hipHostMalloc((void **)&ws.h_pinned, 4096, hipHostMallocDefault));
+            HSFM_HIP(hipMemcpyAsync(ws.h_pinned, ws.item_first.as<uint64_t>() + nq, 8, hipMemcpyDeviceToHost, s));
+            HSFM_HIP(hipMemcpyAsync(ws.h_pinned + 1, d_qoff + nq, 8, hipMemcpyDeviceToHost, s));
+            HSFM_HIP(hipStreamSynchronize(s));
+            W = ws.h_pinned[0];
+            total_len = ws.h_pinned[1];
+        }
         if (W > 0xffffffffull - 16) return set_error(HSFM_EINVAL, "too many read segments (%llu)", (unsigned long long)W);
         if (W > nq) {
             if ((rc = ws.items.ensure(W * 16))) return rc;
@@ -1691,8 +1715,6 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     uint32_t *retry = ws.ovf_list.as<uint32_t>();
     if (use_thread) {
         if ((rc = build_seed_table(idx, F))) return rc;
-        uint64_t total_len = 0;
-        if ((rc = readback_u64(idx, d_qoff + nq, &total_len, 1))) return rc;
         const uint64_t nslots = (total_len >> 3) + 8ull * W + 8;
         if ((rc = ws.mem_slots.ensure(nslots * 16 + 16))) return rc;
         if ((rc = pack_bases(idx, d_q, total_len, ws.qnib))) return rc;
