@@ -230,6 +230,152 @@ __device__ __forceinline__ uint32_t locate_read(const DevDir &d, const DevConst 
 }
 
 // ---------------------------------------------------------------------------
+// Second register pass over the reads the first one gave up on: the same
+// backward search with room for C intervals per read.  Final sets of up to KR
+// intervals go to ivs as before; larger ones are published in row order to
+// fb_ivs (fb_meta = offset, count; niv = kNivOverflow).  Reads whose sets
+// outgrow C (or the published list) are queued for the workgroup kernel.
+// Reads the lookup table cannot seed start from the packed F-mer records of
+// the expansions of their last F codes (one ambiguity code) when those hold at
+// most C intervals together.
+// ---------------------------------------------------------------------------
+template <uint32_t C>
+__global__ __launch_bounds__(256) void k_locate_search_wide(DevDir d, const DevConst *__restrict__ cst,
+                                                            const uint8_t *__restrict__ q,
+                                                            const uint64_t *__restrict__ qoff,
+                                                            const uint32_t *__restrict__ list, uint32_t nlist,
+                                                            uint32_t *__restrict__ cnt, uint8_t *__restrict__ niv,
+                                                            uint2 *__restrict__ ivs, uint2 *__restrict__ fb_meta,
+                                                            uint2 *__restrict__ fb_ivs, uint64_t fb_cap,
+                                                            uint32_t *__restrict__ ovf_list,
+                                                            LocateCounters *__restrict__ ctr,
+                                                            const uint4 *__restrict__ seed_rec,
+                                                            const uint4 *__restrict__ seed_iv, uint32_t F) {
+    __shared__ uint16_t s_compat[256];
+    __shared__ int8_t s_digit[256];
+    __shared__ uint8_t s_expand[256];
+    __shared__ uint32_t s_clane[16];
+    load_const_lds(cst, s_compat, s_digit, s_clane);
+    for (uint32_t x = threadIdx.x; x < 256; x += blockDim.x) s_expand[x] = cst->seed_expand[x];
+    __syncthreads();
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nlist) return;
+    const uint32_t r = list[t];
+    const uint32_t L = cst->num_lanes;
+    const uint64_t b0 = qoff[r];
+    const uint32_t m = (uint32_t)(qoff[r + 1] - b0);
+    const uint8_t *p = q + b0;
+    uint32_t lo[C], hi[C], ns = 1;
+#pragma unroll
+    for (uint32_t j = 0; j < C; j++) { lo[j] = 0xffffffffu; hi[j] = 0xffffffffu; }
+    lo[0] = 0;
+    hi[0] = d.n;
+    int start = (int)m;
+    bool give_up = false;
+    const int sd = lookup_seed(d, s_digit, cst->lookup_depth, cst->lookup_radix, p, m, lo[0], hi[0]);
+    if (sd == 2) ns = 0;
+    else if (sd == 1) start = (int)(m - cst->lookup_depth);
+    else if (seed_rec && m >= F) {
+        SeedWin sw;
+        if (seed_window(p + m - F, F, s_expand, sw, 1)) {
+            uint32_t k2 = 0;
+            bool ok = true;
+            for (uint32_t c = 0; c < seed_ncombo(sw) && ok; c++) {
+                uint32_t ix = 0;
+                if (!seed_combo(sw, c, ix)) continue;
+                const uint4 R = seed_rec[ix];
+                const uint32_t l = R.w & ((1u << kSeedLslBits) - 1u), n2 = R.w >> kSeedLslBits;
+                if (l != F || n2 == 0) continue;
+                const uint64_t o = (uint64_t)R.x | ((uint64_t)R.y << 32);
+                for (uint32_t e = 0; e < n2 && ok; e++) {
+                    const uint2 iv = n2 == 1 ? make_uint2(R.x, R.y) : make_uint2(seed_iv[o + e].x, seed_iv[o + e].y);
+                    bool dup = false;
+#pragma unroll
+                    for (uint32_t u = 0; u < C; u++) dup |= u < k2 && lo[u] == iv.x;
+                    if (dup) continue;
+                    if (k2 >= C) { ok = false; break; }
+#pragma unroll
+                    for (uint32_t u = 0; u < C; u++)
+                        if (u == k2) { lo[u] = iv.x; hi[u] = iv.y; }
+                    k2++;
+                }
+            }
+            if (ok) {
+                ns = k2;
+                start = (int)(m - F);
+            } else {
+                give_up = true;
+            }
+        }
+    }
+    for (int i = start - 1; i >= 0 && ns && !give_up; i--) {
+        const uint32_t cm = s_compat[p[i]];
+        uint32_t nlo[C], nhi[C], nn = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < C; j++) { nlo[j] = 0xffffffffu; nhi[j] = 0xffffffffu; }
+#pragma unroll
+        for (uint32_t j = 0; j < C; j++) {
+            if (j < ns) {
+                uint32_t cl[16], ch[16];
+                pair_counts(d, lo[j], hi[j], cl, ch);
+#pragma unroll
+                for (int k = 0; k < 16; k++) {
+                    if (((cm >> k) & 1u) && ch[k] > cl[k]) {
+                        const uint32_t a = s_clane[k] + cl[k], b = s_clane[k] + ch[k];
+                        if (nn >= C) give_up = true;
+#pragma unroll
+                        for (uint32_t u = 0; u < C; u++)
+                            if (u == nn) { nlo[u] = a; nhi[u] = b; }
+                        nn++;
+                    }
+                }
+            }
+        }
+        if (give_up) break;
+#pragma unroll
+        for (uint32_t j = 0; j < C; j++) { lo[j] = nlo[j]; hi[j] = nhi[j]; }
+        ns = nn;
+    }
+    if (!give_up) {
+        // row order (disjoint intervals; empty slots hold 0xffffffff)
+#pragma unroll
+        for (uint32_t a = 0; a < C; a++)
+#pragma unroll
+            for (uint32_t b = 0; b + 1 < C - a; b++) {
+                const bool sw = lo[b] > lo[b + 1];
+                const uint32_t tl = sw ? lo[b + 1] : lo[b], th = sw ? hi[b + 1] : hi[b];
+                const uint32_t ul = sw ? lo[b] : lo[b + 1], uh = sw ? hi[b] : hi[b + 1];
+                lo[b] = tl; hi[b] = th; lo[b + 1] = ul; hi[b + 1] = uh;
+            }
+        uint32_t total = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < C; j++)
+            if (j < ns) total += hi[j] - lo[j];
+        if (ns <= KR) {
+#pragma unroll
+            for (uint32_t j = 0; j < KR; j++)
+                if (j < ns) ivs[(size_t)r * KR + j] = make_uint2(lo[j], hi[j]);
+            niv[r] = (uint8_t)ns;
+            cnt[r] = total;
+            return;
+        }
+        const unsigned long long slot = atomicAdd(&ctr->fb_used, (unsigned long long)ns);
+        if (slot + ns <= fb_cap) {
+#pragma unroll
+            for (uint32_t j = 0; j < C; j++)
+                if (j < ns) fb_ivs[slot + j] = make_uint2(lo[j], hi[j]);
+            fb_meta[r] = make_uint2((uint32_t)slot, ns);
+            niv[r] = kNivOverflow;
+            cnt[r] = total;
+            return;
+        }
+    }
+    niv[r] = kNivOverflow;
+    cnt[r] = 0;
+    ovf_list[atomicAdd(&ctr->n_ovf, 1ull)] = r;
+}
+
+// ---------------------------------------------------------------------------
 // Pipelined persistent backward search.  Each lane runs locate_read as a state
 // machine.  Every pass of the loop issues, for each lane, one round of
 // independent loads -- one probe (two Occ records, a superblock row and the
@@ -735,6 +881,7 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
     // 3) what still outgrows the ring: one wave per read
     const char *path = getenv("HSFM_LOCATE_PATH");
     const bool pipe_all = path && strcmp(path, "pipe") == 0;
+    const bool pipe_second = path && strcmp(path, "pipe2") == 0; // pipelined kernel as the second level
     SeedView sv{nullptr, nullptr, nullptr, 0};
     if (nq) {
         const uint32_t F = 12;
@@ -764,7 +911,17 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
             idx->stats.occ_probes = h[5]; // the main kernel's own probes
         }
         idx->stats.fallback_reads = pipe_all ? 0 : nlist;
-        if (nlist) {
+        if (nlist && !pipe_all && !pipe_second) {
+            // second register pass with room for 16 intervals per read
+            if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_ovf), 2, 0))) return rc;
+            hipLaunchKernelGGL(k_locate_search_wide<16>, dim3((uint32_t)((nlist + 255) / 256)), dim3(256), 0, s, d,
+                               idx->d_const.as<DevConst>(), d_q, d_qoff, list, (uint32_t)nlist,
+                               ws.cnt.as<uint32_t>(), ws.niv.as<uint8_t>(), ws.ivs.as<uint2>(),
+                               ws.fb_meta.as<uint2>(), ws.fb_ivs.as<uint2>(), (uint64_t)(ws.fb_ivs.cap / 8),
+                               ws.read_list.as<uint32_t>(), ctr, idx->seed_rec12.as<uint4>(),
+                               idx->seed_iv.as<uint4>(), idx->seed_rec12.p ? 12u : 0u);
+            HSFM_HIP(hipGetLastError());
+        } else if (nlist) {
             if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_ovf), 2, 0))) return rc;
             const uint32_t need = (uint32_t)((nlist + 255) / 256);
             hipLaunchKernelGGL((k_locate_pipe<16, 256>), dim3(need < pipe_grid ? need : pipe_grid), dim3(256), 0, s,

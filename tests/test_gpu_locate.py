@@ -124,10 +124,12 @@ def test_lookup12_with_iupac_reference():
     assert g.last_stats()["guard_hits"] == 0
 
 
-def test_persistent_and_register_kernels_agree(monkeypatch):
-    """The register kernel (default) and the persistent ring kernel
-    (HSFM_LOCATE_PATH=pipe) return identical hits, including reads whose
-    final set is published through the fallback interval list."""
+@pytest.mark.parametrize("path", ["pipe", "pipe2"])
+def test_persistent_and_register_kernels_agree(monkeypatch, path):
+    """The default levels (register kernel, then the 16-interval register pass)
+    and the pipelined ring kernel (HSFM_LOCATE_PATH=pipe for every read, pipe2
+    as the second level) return identical hits, including reads whose final
+    set is published through the fallback interval list."""
     rng = np.random.default_rng(12)
     text, bounds = O.gen_reference(13, 300_000, 3, gc=0.41, repeat_frac=0.3, n_units=10, unit_min=300,
                                    unit_max=1000, iupac_frac=0.02)
@@ -135,6 +137,22 @@ def test_persistent_and_register_kernels_agree(monkeypatch):
     g = hf.FmIndex.from_bytes(o.to_bytes())
     flat, off = O.gen_reads(14, text, bounds, 5000, 40, sub=0.01, both_strands=True)
     a = _check_batch(o, g, (flat, off))
-    monkeypatch.setenv("HSFM_LOCATE_PATH", "pipe")
+    assert g.last_stats()["fallback_reads"] > 0
+    monkeypatch.setenv("HSFM_LOCATE_PATH", path)
     b = g.locate_batch((flat, off))
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_second_register_pass_with_seed_records():
+    """A text above 2^20 positions with 2% ambiguity codes and lookup depth 12:
+    reads leave the first register pass at ambiguity codes near their end and
+    the second pass seeds the unseedable ones from the packed 12-mer records
+    (several intervals per record included); every hit equals the oracle's."""
+    rng = np.random.default_rng(15)
+    text, bounds = O.gen_reference(16, 1_300_000, 4, gc=0.41, repeat_frac=0.3, n_units=20, unit_min=300,
+                                   unit_max=1500, iupac_frac=0.02)
+    o = O.OracleIndex.build(text=text, bounds=bounds, sample_rate=32, lookup_depth=12, threads=8)
+    g = hf.FmIndex.from_bytes(o.to_bytes())
+    flat, off = O.gen_reads(17, text, bounds, 20000, 60, sub=0.005, both_strands=True)
+    _check_batch(o, g, (flat, off))
+    assert g.last_stats()["fallback_reads"] > 0
