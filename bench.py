@@ -59,6 +59,7 @@ def parse():
     ap.add_argument("--cache", default=os.environ.get("HSFM_BENCH_CACHE", "/tmp/hsfm_bench_cache"))
     ap.add_argument("--uneven-reads", type=int, default=2_000_000,
                     help="reads of the uneven-length batched leg (0 skips it)")
+    ap.add_argument("--lsi-leaves", type=int, default=1 << 20, help="stage-2 leaves of the learned seed index")
     ap.add_argument("--seed-keys", type=int, default=4_000_000,
                     help="read 21-mers for the learned-seed vs plain-search leg (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -260,7 +261,7 @@ def learned_seed_leg(bidir, flat, off, args):
     k = 21
     keys = read_kmer_keys(flat, off, k, args.seed_keys)
     t0 = time.time()
-    lsi = hf.LearnedSeedIndex(bidir, k=k, leaves=4096, hidden=16, train_steps=400, sample=1 << 16)
+    lsi = hf.LearnedSeedIndex(bidir, k=k, leaves=args.lsi_leaves, hidden=16, train_steps=400, sample=1 << 16)
     build_s = time.time() - t0
     info = lsi.info()
     dq = torch.from_numpy(keys.view(np.int64)).cuda()
@@ -283,7 +284,7 @@ def learned_seed_leg(bidir, flat, off, args):
             "plain_kernel_ms": round(pm, 4), "learned_kernel_ms": round(lm, 4),
             "speedup": round(pm / lm, 3) if exact and lm > 0 else None,
             "learned_probes_per_key": round(probes / max(1, len(keys)), 3),
-            "index_keys": int(info["num_keys"]), "leaves": 4096, "hidden": 16,
+            "index_keys": int(info["num_keys"]), "leaves": args.lsi_leaves, "hidden": 16,
             "max_err": int(info["max_err"]), "mean_window": round(info["mean_window"], 2),
             "train_mse": [info["mse_initial"], info["mse_final"]],
             "build_s": round(build_s, 2), "enum_ms": round(info["enum_ms"], 1),
