@@ -41,6 +41,7 @@ struct DirBuffers {
     DevBuf occ, sb, samp, sa_vals, lookup;
     DevBuf sa_full;           // every row's SA value (see ensure_sa_full)
     DevBuf text;              // the text as lanes, from sa_full
+    DevBuf isa;               // row of every text position, from sa_full
     bool sa_full_tried = false;
     DevDir view{};
 };

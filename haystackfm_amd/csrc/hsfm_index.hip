@@ -242,9 +242,11 @@ void free_dir(DirBuffers &d) {
     d.lookup.release();
     d.sa_full.release();
     d.text.release();
+    d.isa.release();
     d.sa_full_tried = false;
     d.view.sa_full = nullptr;
     d.view.text = nullptr;
+    d.view.isa = nullptr;
 }
 
 void free_workspace(Workspace &w) {

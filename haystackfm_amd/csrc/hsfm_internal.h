@@ -63,6 +63,8 @@ struct DevDir {
     const uint32_t *sa_full;
     // lane of every text position (built with sa_full; padded by 16 bytes), or null
     const uint8_t *text;
+    // row of every text position (the inverse of sa_full), or null
+    const uint32_t *isa;
     uint32_t num_samples;
     uint32_t sample_rate;
     uint32_t num_lanes;

@@ -99,6 +99,55 @@ __global__ __launch_bounds__(256) void k_locate_search(DevDir d, const DevConst 
     if ((threadIdx.x & 63) == 0) atomicAdd(&ctr->probes, (unsigned long long)probes);
 }
 
+
+// A single row lo with q[i..m) matched at text position sa_full[lo]: the
+// backward search from here follows that one occurrence (its only child is
+// the row of the previous text position, when the query code there is
+// compatible with the text), so the read matches iff the text left of the
+// occurrence is compatible with q[0..i), and then its one row is the row of
+// text position sa_full[lo] - i.  Compares 8 codes per load pair.
+__device__ __forceinline__ void locate_finish_text(const DevDir &d, const uint16_t *s_compat, const uint8_t *p,
+                                                   uint32_t i, uint32_t &lo, uint32_t &hi, uint32_t &ns,
+                                                   uint32_t &probes) {
+    uint32_t tp = d.sa_full[lo]; // text position of q[i]
+    probes++;
+    uint32_t tq = i;
+    bool ok = true;
+    while (tq > 0 && ok) {
+        if (tp == 0) { // nothing left of the text start
+            ok = false;
+            break;
+        }
+        const uint32_t tl = tp - 1;
+        const uintptr_t qa = reinterpret_cast<uintptr_t>(p + tq - 1);
+        const uint2 tw = *reinterpret_cast<const uint2 *>(d.text + (tl & ~7u));
+        const uint2 qw = *reinterpret_cast<const uint2 *>(qa & ~(uintptr_t)7);
+        probes++;
+        const uint32_t to = tl & 7u, qo = (uint32_t)(qa & 7);
+        const uint32_t lim = min(min(to + 1, qo + 1), tq);
+        const uint64_t t64 = (uint64_t)tw.x | ((uint64_t)tw.y << 32), q64 = (uint64_t)qw.x | ((uint64_t)qw.y << 32);
+        uint32_t adv = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; k++) {
+            if (k < lim && ok) {
+                const uint32_t lane = (uint32_t)(t64 >> (8 * (to - k))) & 0xffu;
+                const uint32_t code = (uint32_t)(q64 >> (8 * (qo - k))) & 0xffu;
+                if ((s_compat[code] >> lane) & 1u) adv++;
+                else ok = false;
+            }
+        }
+        tq -= adv;
+        tp -= adv;
+    }
+    if (ok) {
+        lo = d.isa[tp];
+        hi = lo + 1;
+        ns = 1;
+    } else {
+        ns = 0;
+    }
+}
+
 // Backward search of one read (the body of k_locate_search); returns the
 // number of 64 B records it fetched.
 __device__ __forceinline__ uint32_t locate_read(const DevDir &d, const DevConst *cst, const uint8_t *q,
@@ -167,6 +216,11 @@ __device__ __forceinline__ uint32_t locate_read(const DevDir &d, const DevConst 
             }
         }
     }
+    const bool textmode = d.isa != nullptr && d.text != nullptr && d.sa_full != nullptr;
+    if (textmode && ns == 1 && hi[0] - lo[0] == 1 && start > 0) {
+        locate_finish_text(d, s_compat, p, (uint32_t)start, lo[0], hi[0], ns, probes);
+        start = 0;
+    }
     for (int i = start - 1; i >= 0 && ns; i--) {
         const uint32_t cm = s_compat[p[i]];
         uint32_t nlo[KR], nhi[KR], nn = 0;
@@ -206,6 +260,10 @@ __device__ __forceinline__ uint32_t locate_read(const DevDir &d, const DevConst 
 #pragma unroll
         for (uint32_t j = 0; j < KR; j++) { lo[j] = nlo[j]; hi[j] = nhi[j]; }
         ns = nn;
+        if (textmode && ns == 1 && hi[0] - lo[0] == 1 && i > 0) {
+            locate_finish_text(d, s_compat, p, (uint32_t)i, lo[0], hi[0], ns, probes);
+            break;
+        }
     }
     // sort the (<= KR) disjoint intervals by start; empty slots hold 0xffffffff
 #pragma unroll
@@ -308,6 +366,12 @@ __global__ __launch_bounds__(256) void k_locate_search_wide(DevDir d, const DevC
             }
         }
     }
+    const bool textmode = d.isa != nullptr && d.text != nullptr && d.sa_full != nullptr;
+    uint32_t tprobes = 0;
+    if (!give_up && textmode && ns == 1 && hi[0] - lo[0] == 1 && start > 0) {
+        locate_finish_text(d, s_compat, p, (uint32_t)start, lo[0], hi[0], ns, tprobes);
+        start = 0;
+    }
     for (int i = start - 1; i >= 0 && ns && !give_up; i--) {
         const uint32_t cm = s_compat[p[i]];
         uint32_t nlo[C], nhi[C], nn = 0;
@@ -335,6 +399,10 @@ __global__ __launch_bounds__(256) void k_locate_search_wide(DevDir d, const DevC
 #pragma unroll
         for (uint32_t j = 0; j < C; j++) { lo[j] = nlo[j]; hi[j] = nhi[j]; }
         ns = nn;
+        if (textmode && ns == 1 && hi[0] - lo[0] == 1 && i > 0) {
+            locate_finish_text(d, s_compat, p, (uint32_t)i, lo[0], hi[0], ns, tprobes);
+            break;
+        }
     }
     if (!give_up) {
         // row order (disjoint intervals; empty slots hold 0xffffffff)

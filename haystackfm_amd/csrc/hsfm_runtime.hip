@@ -343,7 +343,8 @@ __global__ __launch_bounds__(256) void k_sa_full_sent(DevDir d, const DevConst *
 // whose C range holds r (lane counts from the record at n), and row r starts
 // at text position sa[r].
 __global__ __launch_bounds__(256) void k_text_from_sa(DevDir d, const DevConst *__restrict__ cst,
-                                                      const uint32_t *__restrict__ sa, uint8_t *__restrict__ text) {
+                                                      const uint32_t *__restrict__ sa, uint8_t *__restrict__ text,
+                                                      uint32_t *__restrict__ isa) {
     __shared__ uint32_t s_lo[16], s_hi[16];
     if (threadIdx.x == 0) {
         RecView rv;
@@ -363,7 +364,10 @@ __global__ __launch_bounds__(256) void k_text_from_sa(DevDir d, const DevConst *
     for (int l = 0; l < 16; l++)
         if (r >= s_lo[l] && r < s_hi[l]) lane = (uint32_t)l;
     const uint32_t p = sa[r];
-    if (p < d.n) text[p] = (uint8_t)lane;
+    if (p < d.n) {
+        text[p] = (uint8_t)lane;
+        if (isa) isa[p] = r;
+    }
 }
 
 __global__ void k_count_value(const uint32_t *__restrict__ a, uint64_t n, uint32_t v,
@@ -380,11 +384,13 @@ int ensure_sa_full(hsfm_index *idx) {
     if (env && env[0] == '0') {
         db.view.sa_full = nullptr;
         db.view.text = nullptr;
+        db.view.isa = nullptr;
         return HSFM_OK;
     }
     if (db.sa_full_tried) {
         db.view.sa_full = db.sa_full.as<uint32_t>();
         db.view.text = db.text.as<uint8_t>();
+        db.view.isa = db.isa.as<uint32_t>();
         return HSFM_OK;
     }
     db.sa_full_tried = true;
@@ -435,13 +441,18 @@ int ensure_sa_full(hsfm_index *idx) {
     }
     db.view.sa_full = db.sa_full.as<uint32_t>();
     // the text as lanes (right extensions of single rows compare it directly)
+    // and the inverse (the row of a text position: locate finishes single
+    // rows on the text and needs the row of the occurrence it ends on)
     if (!db.text.ensure((size_t)n + 16)) {
+        if (db.isa.ensure((size_t)n * 4 + 16)) db.isa.release(); // without memory for it: text only
         if ((rc = fill_u32(idx, db.text.as<uint32_t>(), ((size_t)n + 16) / 4, 0))) return rc;
         hipLaunchKernelGGL(k_text_from_sa, dim3((n + 255) / 256), dim3(256), 0, s, db.view,
-                           idx->d_const.as<DevConst>(), db.sa_full.as<uint32_t>(), db.text.as<uint8_t>());
+                           idx->d_const.as<DevConst>(), db.sa_full.as<uint32_t>(), db.text.as<uint8_t>(),
+                           db.isa.as<uint32_t>());
         HSFM_HIP(hipGetLastError());
         HSFM_HIP(hipStreamSynchronize(s));
         db.view.text = db.text.as<uint8_t>();
+        db.view.isa = db.isa.as<uint32_t>();
     } else {
         db.text.release();
     }
