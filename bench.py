@@ -59,7 +59,10 @@ def parse():
     ap.add_argument("--cache", default=os.environ.get("HSFM_BENCH_CACHE", "/tmp/hsfm_bench_cache"))
     ap.add_argument("--uneven-reads", type=int, default=2_000_000,
                     help="reads of the uneven-length batched leg (0 skips it)")
-    ap.add_argument("--lsi-leaves", type=int, default=1 << 20, help="stage-2 leaves of the learned seed index")
+    ap.add_argument("--lsi-leaves", type=int, default=4096,
+                    help="stage-2 leaves of the learned seed index (4096 as the learned-seed workload states)")
+    ap.add_argument("--lsi-leaves-alt", type=int, default=1 << 20,
+                    help="a second leaf count measured beside it (0: skip)")
     ap.add_argument("--seed-keys", type=int, default=4_000_000,
                     help="read 21-mers for the learned-seed vs plain-search leg (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -253,7 +256,7 @@ def uneven_leg(bidir, wl, args):
                       "tuning_wall_s": round(tune_wall, 2)}}
 
 
-def learned_seed_leg(bidir, flat, off, args):
+def learned_seed_leg(bidir, flat, off, args, leaves=None):
     """Learned seed index vs plain exact search on the same read k-mers: both
     on the device, outputs compared bit for bit, kernel time by HIP events."""
     import torch
@@ -261,7 +264,8 @@ def learned_seed_leg(bidir, flat, off, args):
     k = 21
     keys = read_kmer_keys(flat, off, k, args.seed_keys)
     t0 = time.time()
-    lsi = hf.LearnedSeedIndex(bidir, k=k, leaves=args.lsi_leaves, hidden=16, train_steps=400, sample=1 << 16)
+    leaves = leaves or args.lsi_leaves
+    lsi = hf.LearnedSeedIndex(bidir, k=k, leaves=leaves, hidden=16, train_steps=400, sample=1 << 16)
     build_s = time.time() - t0
     info = lsi.info()
     dq = torch.from_numpy(keys.view(np.int64)).cuda()
@@ -284,7 +288,7 @@ def learned_seed_leg(bidir, flat, off, args):
             "plain_kernel_ms": round(pm, 4), "learned_kernel_ms": round(lm, 4),
             "speedup": round(pm / lm, 3) if exact and lm > 0 else None,
             "learned_probes_per_key": round(probes / max(1, len(keys)), 3),
-            "index_keys": int(info["num_keys"]), "leaves": args.lsi_leaves, "hidden": 16,
+            "index_keys": int(info["num_keys"]), "leaves": leaves, "hidden": 16,
             "max_err": int(info["max_err"]), "mean_window": round(info["mean_window"], 2),
             "train_mse": [info["mse_initial"], info["mse_final"]],
             "build_s": round(build_s, 2), "enum_ms": round(info["enum_ms"], 1),
@@ -367,6 +371,8 @@ def main():
     nls = min(args.check_reads * 5, nl)
     lsample = bidir.fwd.locate_batch((wl["lq"][:int(wl["lo"][nls])], wl["lo"][:nls + 1]))
     lseed = learned_seed_leg(bidir, wl["sq"], wl["so"], args) if args.seed_keys else None
+    lseed_alt = (learned_seed_leg(bidir, wl["sq"], wl["so"], args, leaves=args.lsi_leaves_alt)
+                 if args.seed_keys and args.lsi_leaves_alt else None)
     uneven = uneven_leg(bidir, wl, args) if args.uneven_reads else None
     cpu = cpu_baseline(idx_bytes, wl["sq"], wl["so"], wl["lq"], wl["lo"], bounds, args, sample, lsample)
 
@@ -425,6 +431,7 @@ def main():
         "reference_algorithm_probes_per_read": cpu.get("smem_probes_per_read"),
         "learned_seed_speedup": lseed["speedup"] if lseed else None,
         "learned_seed": lseed,
+        "learned_seed_more_leaves": lseed_alt,
         "uneven_batched": uneven,
     }
     print(json.dumps(out), flush=True)

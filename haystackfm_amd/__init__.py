@@ -710,7 +710,7 @@ class LearnedSeedIndex:
     seed(keys) returns, per key, (fwd_lo, fwd_hi, rev_lo, found): the interval
     plain exact backward search gives for the k-mer (bidir.rs:112-130)."""
 
-    def __init__(self, index, k: int = 21, leaves: int = 1 << 20, hidden: int = 16, train_steps: int = 400,
+    def __init__(self, index, k: int = 21, leaves: int = 4096, hidden: int = 16, train_steps: int = 400,
                  sample: int = 1 << 16, lr: float = 0.01):
         self.index = index
         self.k = k
