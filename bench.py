@@ -14,6 +14,13 @@ inputs already resident in HBM):
 Also measured in the same run (reported beside the headline): batched locate of
 1,000,000 x 150 bp reads (both strands, 1% substitutions) on the same index.
 
+`--learned-1g` runs the learned-seed workload instead: a 1 Gbp reference in 24
+contigs (45% copies of 2,000 repeat families at 5-25% divergence, GC 0.41, 0.2%
+N; SA sample rate 64), 4,000,000 x 150 bp reads (both strands, 1% substitutions);
+the learned seed index (4,096 leaves) against plain exact search on one 21-mer per
+read, and the SMEM pipeline (min_len 21) over the reads.  Its JSON line reports
+the speedup as `value`.
+
 The GPU leg uses only haystackfm_amd (libhsfm.so behind the C ABI, plus the host
 index builder).  The oracle is used only in the cpu_baseline leg: to time the
 CPU restatement of the reference path on a bounded sample and to diff the GPU
@@ -65,6 +72,12 @@ def parse():
                     help="a second leaf count measured beside it (0: skip)")
     ap.add_argument("--seed-keys", type=int, default=4_000_000,
                     help="read 21-mers for the learned-seed vs plain-search leg (0 skips it)")
+    ap.add_argument("--learned-1g", action="store_true",
+                    help="run the 1 Gbp learned-seed workload instead (host index build takes minutes)")
+    ap.add_argument("--big-ref-bp", type=int, default=1_000_000_000,
+                    help="reference length of the --learned-1g workload")
+    ap.add_argument("--big-reads", type=int, default=4_000_000,
+                    help="150 bp reads of the --learned-1g workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 1 GiB device buffer 3 times (scales FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
@@ -205,6 +218,23 @@ def measured_traffic(kernel, args, nq, nl):
     return k["bytes_per_launch"] if k else None
 
 
+def measured_traffic_1g(kernel, args, keys):
+    """HBM bytes per launch of `kernel` on the 1 Gbp learned-seed workload from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes (copy-calibrated)."""
+    path = os.path.join(ROOT, "profiles", "traffic_learned_seed_1Gbp.json")
+    try:
+        with open(path) as f:
+            doc = json.load(f)
+    except (OSError, ValueError):
+        return None
+    w = doc.get("workload", {})
+    if (w.get("reference_bp") != args.big_ref_bp or w.get("reads") != args.big_reads
+            or w.get("seed_keys") != args.seed_keys or w.get("leaves") != args.lsi_leaves):
+        return None
+    k = doc.get("kernels", {}).get(kernel)
+    return k["bytes_per_launch"] if k else None
+
+
 def read_kmer_keys(flat, off, k, limit):
     """Keys of the exact-ACGT k-mer windows inside reads (first `limit`)."""
     import haystackfm_amd as hf
@@ -215,6 +245,24 @@ def read_kmer_keys(flat, off, k, limit):
     keys = keys[inside & (keys != np.uint64(0xFFFFFFFFFFFFFFFF))]
     if len(keys) > limit:
         keys = keys[np.linspace(0, len(keys) - 1, limit).astype(np.int64)]
+    return keys
+
+
+def read_window_keys(flat, off, k, limit):
+    """One k-mer window per read (at a read-dependent offset) for the first
+    `limit` reads whose window is exact ACGT: a cheap key set for large read
+    batches, where enumerating every window costs tens of seconds."""
+    nr = min(len(off) - 1, limit)
+    ln = (off[1:nr + 1] - off[:nr]).astype(np.int64)
+    r = np.arange(nr, dtype=np.int64)
+    ok = ln >= k
+    start = off[:nr].astype(np.int64) + (r * 7919) % np.maximum(ln - k + 1, 1)
+    w = flat[start[ok][:, None] + np.arange(k)[None, :]]
+    good = ((w >= 1) & (w <= 4)).all(axis=1)
+    d = (w[good].astype(np.uint64) - np.uint64(1))
+    keys = np.zeros(len(d), np.uint64)
+    for t in range(k):
+        keys = (keys << np.uint64(2)) | d[:, t]
     return keys
 
 
@@ -256,13 +304,14 @@ def uneven_leg(bidir, wl, args):
                       "tuning_wall_s": round(tune_wall, 2)}}
 
 
-def learned_seed_leg(bidir, flat, off, args, leaves=None):
+def learned_seed_leg(bidir, flat, off, args, leaves=None, keys=None):
     """Learned seed index vs plain exact search on the same read k-mers: both
     on the device, outputs compared bit for bit, kernel time by HIP events."""
     import torch
     import haystackfm_amd as hf
     k = 21
-    keys = read_kmer_keys(flat, off, k, args.seed_keys)
+    if keys is None:
+        keys = read_kmer_keys(flat, off, k, args.seed_keys)
     t0 = time.time()
     leaves = leaves or args.lsi_leaves
     lsi = hf.LearnedSeedIndex(bidir, k=k, leaves=leaves, hidden=16, train_steps=400, sample=1 << 16)
@@ -295,6 +344,194 @@ def learned_seed_leg(bidir, flat, off, args, leaves=None):
             "train_ms": round(info["train_ms"], 1), "fit_ms": round(info["fit_ms"], 1)}
 
 
+def workload_1g(args):
+    """The learned-seed workload: 1 Gbp in 24 contigs of uniform random
+    lengths, GC 0.41, 45% copies of 2,000 repeat families (300-6,000 bp) at
+    5-25% divergence, 0.2% N; bidirectional index with SA sample rate 64 and no
+    lookup table; reads of 150 bp from both strands with 1% substitutions."""
+    from haystackfm_amd import index_build as IB
+    key = hashlib.sha1(f"1g-v1-{args.big_ref_bp}-{args.big_reads}".encode()).hexdigest()[:12]
+    os.makedirs(args.cache, exist_ok=True)
+    path = os.path.join(args.cache, f"wl_{key}.npz")
+    if os.path.exists(path):
+        z = np.load(path, allow_pickle=False)
+        log(f"workload cache hit {path}")
+        return z["bounds"], z["index"].tobytes(), z["sq"], z["so"]
+    os.environ.setdefault("HSFM_BUILD_PROGRESS", "1")
+    t0 = time.time()
+    text, bounds = IB.synth_reference(0xC5EED, args.big_ref_bp, 24, gc=0.41, repeat_frac=0.45, n_units=2000,
+                                      unit_min=300, unit_max=6000, div_min=0.05, div_max=0.25,
+                                      iupac_frac=0.002, iupac_codes=[5])
+    log(f"reference {len(text)} bp in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    idx = IB.build_bidir_bytes(text, bounds, sample_rate=64, lookup_depth=0, threads=args.threads)
+    log(f"bidirectional index {len(idx) / 1e6:.0f} MB in {time.time() - t0:.1f}s")
+    sq, so = IB.synth_reads(0x5EED5, text, bounds, args.big_reads, 150, sub=0.01, both_strands=True, shuffle=True)
+    log(f"{args.big_reads} reads synthesised")
+    del text
+    np.savez(path, bounds=bounds, index=np.frombuffer(idx, np.uint8), sq=sq, so=so)
+    return bounds, idx, sq, so
+
+
+def keys_to_codes(keys, k):
+    """Base codes (A=1 .. T=4) of 2-bit keys, first base most significant."""
+    sh = np.arange(k - 1, -1, -1, dtype=np.uint64) * np.uint64(2)
+    return ((keys[:, None] >> sh[None, :]) & np.uint64(3)).astype(np.uint8) + 1
+
+
+def exact_kmer_interval(O, ob, codes):
+    """(1, fwd_lo, fwd_hi, rev_lo) of exact left extensions from the full
+    interval, last code first (the oracle's extend_left), or (0,) when absent."""
+    n = ob.fwd.text_len
+    iv = (0, n, 0, n)
+    for c in codes[::-1]:
+        iv = O.extend_left(ob.fwd, iv, int(c))
+        if iv is None or iv[1] <= iv[0]:
+            return (0,)
+    return (1, iv[0], iv[1], iv[2])
+
+
+def main_1g(args):
+    """Learned seed index vs plain exact search on the 1 Gbp workload, plus the
+    SMEM pipeline (min_len 21) over its reads; one JSON line."""
+    import torch
+    import haystackfm_amd as hf
+    bounds, idx, sq, so = workload_1g(args)
+    t0 = time.time()
+    bidir = hf.BidirFmIndex.from_bytes(idx)
+    bidir._handle()
+    log(f"index upload {time.time() - t0:.1f}s")
+    k = 21
+    keys = read_window_keys(sq, so, k, args.seed_keys)
+    log(f"{len(keys)} read 21-mers")
+    lseed = learned_seed_leg(bidir, sq, so, args, keys=keys)
+    lseed_alt = (learned_seed_leg(bidir, sq, so, args, leaves=args.lsi_leaves_alt, keys=keys)
+                 if args.lsi_leaves_alt else None)
+
+    # SMEM pipeline over the reads (min_len 21)
+    min_len = 21
+    dev = torch.device("cuda:0")
+    d_sq = torch.from_numpy(sq).to(dev)
+    d_so = torch.from_numpy(so.astype(np.int64)).to(dev)
+    d_rb = torch.from_numpy(bounds.astype(np.int32)).to(dev)
+    nq = len(so) - 1
+    torch.cuda.synchronize()
+
+    def smem_step():
+        return bidir.find_mems_device(d_sq.data_ptr(), d_so.data_ptr(), nq, min_len, hf.HSFM_MODE_SMEM,
+                                      d_rb.data_ptr(), len(bounds), args.max_hits)
+
+    for i in range(args.warmup):
+        smem_step()
+        log(f"SMEM warmup {i} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kms = []
+    for _ in range(args.steps):
+        r = smem_step()
+        kms.append(bidir.last_stats()["main_kernel_ms"])
+    torch.cuda.synchronize()
+    smem_s = (time.perf_counter() - t0) / args.steps
+    smem_stats = bidir.last_stats()
+    log(f"SMEM step {smem_s * 1e3:.1f} ms, {r[0]} MEM records, {r[1]} positions")
+
+    # parity samples: SMEM lists and 21-mer intervals against the oracle
+    from oracle import oracle as O
+    ob = O.OracleBidir.from_bytes(idx)
+    del idx
+    n = min(args.check_reads, nq)
+    got = bidir.find_mems_batch((sq[:int(so[n])], so[:n + 1]), min_len, hf.HSFM_MODE_SMEM, bounds, args.max_hits)
+    exp = ob.find_mems_batch(qflat=sq[:int(so[n])], qoff=so[:n + 1], min_len=min_len, mode=0, threads=args.threads)
+    ho, hits, pos = got
+    ok = True
+    for q in range(n):
+        want = []
+        for m in exp[q]:
+            for lo_, hi_, _, _ in sorted(m.ivs.tolist()):
+                size = hi_ - lo_
+                nrows = min(size, args.max_hits)
+                p = ob.fwd.resolve_rows(np.arange(lo_, lo_ + nrows, dtype=np.uint32))
+                want.append((m.query_start, m.query_end, size, int(size > args.max_hits),
+                             [O.ref_map(int(x), bounds) for x in p]))
+        have = []
+        for h in hits[ho[q]:ho[q + 1]]:
+            po = int(h[4]) | (int(h[5]) << 32)
+            have.append((int(h[0]), int(h[1]), int(h[2]), int(h[3]),
+                         [tuple(map(int, t)) for t in pos[po:po + int(h[6])]]))
+        if have != want:
+            ok = False
+            log(f"SMEM parity mismatch at read {q}")
+            break
+    ks = keys[np.linspace(0, len(keys) - 1, min(len(keys), 2000)).astype(np.int64)]
+    d_keys = torch.from_numpy(ks.view(np.int64)).cuda()
+    d_out = torch.empty((len(ks), 4), dtype=torch.int32, device="cuda")
+    hf.kmer_search_device(bidir, k, d_keys.data_ptr(), len(ks), d_out.data_ptr())
+    gout = d_out.cpu().numpy().view(np.uint32)
+    codes = keys_to_codes(ks, k)
+    for i in range(len(ks)):
+        want = exact_kmer_interval(O, ob, codes[i])
+        have = (int(gout[i, 3]), int(gout[i, 0]), int(gout[i, 1]), int(gout[i, 2]))
+        if have[:len(want)] != want:
+            ok = False
+            log(f"21-mer interval mismatch at key {i}")
+            break
+    ok = ok and bool(lseed["bit_exact"]) and (lseed_alt is None or bool(lseed_alt["bit_exact"]))
+
+    # CPU baseline: the oracle's plain backward search of the same 21-mers
+    cpu = None
+    if not args.no_cpu:
+        cores = args.threads
+        allc = keys_to_codes(keys, k)
+
+        def run(m):
+            flat = np.ascontiguousarray(allc[:m]).reshape(-1)
+            off = np.arange(m + 1, dtype=np.uint64) * np.uint64(k)
+            t = time.perf_counter()
+            ob.fwd.locate_batch(qflat=flat, qoff=off, threads=cores, counts_only=True)
+            return time.perf_counter() - t
+
+        m = min(len(keys), 20000 * cores)
+        t = run(m)
+        m2 = int(min(len(keys), max(m, m * args.cpu_seconds / max(t, 1e-3))))
+        t2 = run(m2)
+        cpu = {"value": m2 / t2, "unit": "keys/s", "cores": cores, "kind": "port",
+               "sample": f"first {m2} of the read 21-mers: oracle backward_search + count, {cores} threads"}
+    sp = lseed["speedup"]
+    probes = lseed["learned_probes_per_key"] * lseed["keys"]
+    lms = lseed["learned_kernel_ms"]
+    roof = None
+    if lms > 0:
+        a = probes * 64.0 / (lms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(a / HBM_PEAK_GBS, 4), "traffic": measured_traffic_1g("k_lsi_lookup", args, keys),
+                "kernel": "k_lsi_lookup", "kernel_ms": lms,
+                "bytes_per_key": round(lseed["learned_probes_per_key"] * 64.0, 1)}
+    out = {
+        "metric": "learned-seed vs plain exact search speedup (read 21-mers, 1 Gbp ref) bit-exact",
+        "value": sp if ok else None,
+        "unit": "x",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": lms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+u64", "data": "synthetic",
+        "config": {"workload": f"learned_seed: {args.big_ref_bp / 1e9:g} Gbp 24-contig reference (45% repeats from "
+                               "2000 families at 5-25% divergence, GC 0.41, 0.2% N), bidirectional index rate 64; "
+                               f"{args.big_reads} x 150 bp reads (1% subs, both strands); {lseed['keys']} read "
+                               f"21-mers, {args.lsi_leaves} leaves; SMEM min_len 21, max_hits {args.max_hits}",
+                   "reference_bp": int(args.big_ref_bp), "reads": nq, "read_len": 150, "min_len": min_len},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "parity": ok,
+        "parity_sample": {"smem_reads": n, "kmers": int(len(ks))},
+        "learned_seed": lseed,
+        "learned_seed_more_leaves": lseed_alt,
+        "smem": {"value": round(nq / smem_s, 1) if ok else None, "unit": "reads/s", "ms_per_step": round(smem_s * 1e3, 3),
+                 "main_kernel_ms": round(float(np.mean(kms)), 3),
+                 "stats": {k_: (round(v, 3) if isinstance(v, float) else v) for k_, v in smem_stats.items()}},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -309,6 +546,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": src.numel() * 4, "copies": 3}), flush=True)
+        return
+    if args.learned_1g:
+        main_1g(args)
         return
     import haystackfm_amd as hf
 

@@ -11,6 +11,7 @@
 // Serialization is bincode 1 in SerializableFmIndex field order
 // (serialize.rs:55-67).
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -24,6 +25,16 @@
 #include "../../include/hsfm_build.h"
 
 namespace {
+
+// Stage lines on stderr while a large text is sorted (HSFM_BUILD_PROGRESS=1):
+// a 1 Gbp suffix sort runs for minutes, and a caller watching its output can
+// tell a slow build from a stalled one.
+bool g_progress = false;
+void progress(const char *what, int64_t n, int level) {
+    if (!g_progress || n < (1 << 24)) return;
+    fprintf(stderr, "[build] %lld symbols, recursion level %d: %s\n", (long long)n, level, what);
+    fflush(stderr);
+}
 
 // ---------------------------------------------------------------------------
 // suffix sorting: induced sorting (SA-IS) with an implicit end marker that is
@@ -49,7 +60,7 @@ void induce(const CharT *T, int32_t *SA, int32_t n, const std::vector<uint8_t> &
 }
 
 template <class CharT>
-void sais(const CharT *T, int32_t *SA, int32_t n, int32_t K) {
+void sais(const CharT *T, int32_t *SA, int32_t n, int32_t K, int level = 0) {
     if (n <= 0) return;
     if (n == 1) { SA[0] = 0; return; }
     std::vector<uint8_t> st(n);
@@ -64,6 +75,7 @@ void sais(const CharT *T, int32_t *SA, int32_t n, int32_t K) {
     for (int32_t i = 1; i < n; i++)
         if (lms(i)) SA[--bkt[T[i]]] = i;
     induce(T, SA, n, st, cnt, bkt);
+    progress("LMS substrings sorted", n, level);
     int32_t n1 = 0;
     for (int32_t i = 0; i < n; i++)
         if (lms(SA[i])) SA[n1++] = SA[i];
@@ -82,7 +94,8 @@ void sais(const CharT *T, int32_t *SA, int32_t n, int32_t K) {
     for (int32_t i = n - 1, j = n - 1; i >= n1; i--)
         if (SA[i] >= 0) SA[j--] = SA[i];
     int32_t *s1 = SA + n - n1;
-    if (names < n1) sais<int32_t>(s1, SA, n1, names);
+    progress("LMS substrings named", n, level);
+    if (names < n1) sais<int32_t>(s1, SA, n1, names, level + 1);
     else for (int32_t i = 0; i < n1; i++) SA[s1[i]] = i;
     for (int32_t i = 1, j = 0; i < n; i++)
         if (lms(i)) s1[j++] = i;
@@ -96,6 +109,7 @@ void sais(const CharT *T, int32_t *SA, int32_t n, int32_t K) {
         SA[--bkt[T[j]]] = j;
     }
     induce(T, SA, n, st, cnt, bkt);
+    progress("suffixes induced", n, level);
 }
 
 // ---------------------------------------------------------------------------
@@ -292,6 +306,7 @@ bool build(const uint8_t *text, uint64_t n, const uint32_t *bounds, uint32_t nse
     ix.sa.n = (uint32_t)n;
     sa.clear();
     sa.shrink_to_fit();
+    progress("BWT and SA samples written", (int64_t)n, 0);
     build_occ(bwt.data(), (uint32_t)n, enc, ix.occ);
     if (depth) build_lookup(ix, depth, nthreads);
     return true;
@@ -383,6 +398,8 @@ const uint8_t kComp[16] = {0, 4, 3, 2, 1, 5, 7, 6, 8, 9, 11, 10, 15, 14, 13, 12}
 extern "C" int hsfm_build_index(const uint8_t *text, uint64_t n, const uint32_t *bounds, uint32_t nseq,
                                 const char *headers, uint32_t sample_rate, uint32_t lookup_depth, uint32_t encoding,
                                 uint8_t tag, int nthreads, uint8_t **out, uint64_t *out_len) {
+    const char *pv = getenv("HSFM_BUILD_PROGRESS");
+    g_progress = pv && *pv && *pv != '0';
     if (!text || !bounds || !out || !out_len) return -1;
     Index ix;
     if (!build(text, n, bounds, nseq, headers, sample_rate, lookup_depth, encoding, tag, nthreads, ix)) return -1;
@@ -396,6 +413,8 @@ extern "C" int hsfm_build_index(const uint8_t *text, uint64_t n, const uint32_t 
 extern "C" int hsfm_build_bidir(const uint8_t *text, uint64_t n, const uint32_t *bounds, uint32_t nseq,
                                 const char *headers, uint32_t sample_rate, uint32_t lookup_depth, uint32_t encoding,
                                 uint8_t tag, int nthreads, uint8_t **out, uint64_t *out_len) {
+    const char *pv = getenv("HSFM_BUILD_PROGRESS");
+    g_progress = pv && *pv && *pv != '0';
     if (!text || !bounds || !out || !out_len || n == 0) return -1;
     // reverse text: strip the trailing sentinel, reverse, one sentinel at the end
     const uint64_t m = text[n - 1] == 0 ? n - 1 : n;

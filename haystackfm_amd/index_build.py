@@ -57,7 +57,8 @@ def _build(fn, text, bounds, headers, sample_rate, lookup_depth, encoding, tag, 
             encoding, tag, threads, C.byref(out), C.byref(n))
     if rc != 0:
         raise ValueError(f"index construction failed ({rc})")
-    data = C.string_at(out, n.value)
+    # a numpy view copies lengths beyond 2 GiB (ctypes.string_at takes an int size)
+    data = np.ctypeslib.as_array(out, shape=(n.value,)).tobytes() if n.value else b""
     lib().hsfm_build_free(C.cast(out, C.c_void_p))
     return data
 
