@@ -127,6 +127,10 @@ __device__ uint32_t group_extend(const DevDir &D, const uint32_t *s_clane, uint3
 // take q[j-F..j) from the seed table when min_len > F (up to kSeedAmb
 // ambiguity codes expanded).
 constexpr uint32_t kWaveLdsCap = 1024;
+// Threads per read for reads the persistent kernel defers: their sets are
+// wider than its ring, so a step extends up to 256 intervals at once instead
+// of 64 per dependent round.
+constexpr int kDeferTB = 256;
 
 template <bool LDS, int TB>
 __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const DevConst *__restrict__ cst,
@@ -260,6 +264,34 @@ __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const D
             if (!seeded && lane == 0) A[0] = make_uint4(0, n, 0, 0);
             __syncthreads();
             while (k > 0) {
+                if (fwd.text && fwd.sa_full && fwd.isa && ns == 1 && A[0].y - A[0].x == 1) {
+                    // a single row: its only occurrence extends left while the
+                    // text is compatible (64 codes per step, first wave; the
+                    // sentinels stop it); the row of the longer string is the
+                    // inverse suffix array at its text start
+                    if (threadIdx.x < 64) {
+                        const uint32_t tk = fwd.sa_full[A[0].x]; // text position of query code k
+                        uint32_t kk = k;
+                        for (;;) {
+                            const uint32_t back = (k - kk) + 1u + lane; // code kk-1-lane sits at tk - back
+                            bool ok = lane < kk && back <= tk;
+                            if (ok) ok = (s_compat[p[kk - 1u - lane]] >> fwd.text[tk - back]) & 1u;
+                            const uint64_t bad = __ballot(!ok);
+                            const uint32_t adv = bad ? (uint32_t)__ffsll((unsigned long long)bad) - 1u : 64u;
+                            kk -= adv;
+                            if (adv < 64) break;
+                        }
+                        if (threadIdx.x == 0) {
+                            const uint32_t row = fwd.isa[tk - (k - kk)];
+                            A[0] = make_uint4(row, row + 1u, 0u, 0u);
+                            s_b32 = kk;
+                        }
+                    }
+                    __syncthreads();
+                    k = s_b32;
+                    __syncthreads();
+                    break;
+                }
                 const uint32_t c2 = group_extend<true, TB>(fwd, s_clane, s_compat[p[k - 1]], A, ns, B, cap, too_big,
                                                            s_part);
                 __syncthreads();
@@ -1771,7 +1803,10 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     uint32_t cap = kWaveLdsCap;
     uint32_t pass = 0;
     uint64_t rec_used = 0;
-    const uint32_t lds_waves = persist_grid(idx, k_mem_wave<true, 64>, 64);
+    // reads deferred by the persistent kernel: kDeferTB threads each; every
+    // read on the workgroup path (no persistent kernel): one wave each
+    const uint32_t lds_waves = use_thread ? persist_grid(idx, k_mem_wave<true, kDeferTB>, kDeferTB)
+                                          : persist_grid(idx, k_mem_wave<true, 64>, 64);
     while (nlist) {
         const bool lds = cap == kWaveLdsCap;
         uint64_t waves = lds ? lds_waves : (uint64_t)idx->dev->num_cus * 2;
@@ -1799,7 +1834,13 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_retry), 2, 0))) return rc;
         const bool timed = !use_thread && pass == 0;
         if (timed && (rc = event_record(idx, 0, false))) return rc;
-        if (lds)
+        if (lds && use_thread)
+            hipLaunchKernelGGL((k_mem_wave<true, kDeferTB>), dim3((uint32_t)waves), dim3(kDeferTB), 0, s,
+                               idx->fwd.view, idx->rev.view, idx->d_const.as<DevConst>(), svw, d_q, d_qoff, list,
+                               (uint32_t)nlist, items, min_len, cap, (uint4 *)nullptr, ws.mem_recs.as<MemRec>(),
+                               rec_cap, ws.mem_cnt.as<uint32_t>(), ws.mem_flags.as<uint32_t>(), pass, retry, ctr,
+                               nullptr);
+        else if (lds)
             hipLaunchKernelGGL((k_mem_wave<true, 64>), dim3((uint32_t)waves), dim3(64), 0, s, idx->fwd.view,
                                idx->rev.view,
                                idx->d_const.as<DevConst>(), svw, d_q, d_qoff, list, (uint32_t)nlist, items, min_len, cap,
@@ -2080,11 +2121,12 @@ static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, cons
                                (const uint32_t *)nullptr, W, items, min_len, slots, cnt, st, bb.l1.as<uint32_t>(),
                                &c[0]);
         HSFM_HIP(hipGetLastError());
-        // deferred reads: one wave each, sets in LDS
-        hipLaunchKernelGGL((k_mem_wave<true, 64>), dim3(lds_waves), dim3(64), 0, s, idx->fwd.view, idx->rev.view, cst,
-                           svp, d_q, bq, bb.l1.as<uint32_t>(), W, items, min_len, kWaveLdsCap, (uint4 *)nullptr, r0,
-                           half, cnt,
-                           bb.pass_of.as<uint32_t>(), 0u, bb.l2.as<uint32_t>(), &c[1], second_count);
+        // deferred reads: kDeferTB threads each, sets in LDS
+        hipLaunchKernelGGL((k_mem_wave<true, kDeferTB>),
+                           dim3(persist_grid(idx, k_mem_wave<true, kDeferTB>, kDeferTB)), dim3(kDeferTB), 0, s,
+                           idx->fwd.view, idx->rev.view, cst, svp, d_q, bq, bb.l1.as<uint32_t>(), W, items, min_len,
+                           kWaveLdsCap, (uint4 *)nullptr, r0, half, cnt, bb.pass_of.as<uint32_t>(), 0u,
+                           bb.l2.as<uint32_t>(), &c[1], second_count);
     }
     HSFM_HIP(hipGetLastError());
     // sets beyond the LDS: 1024 threads per read, global sets

@@ -285,3 +285,17 @@ def test_long_reads_split_into_segments(monkeypatch, seg):
     qs += [random_text(rng, 700) for _ in range(5)]
     monkeypatch.setenv("HSFM_SEGMENT", seg)
     check(o, g, qs, 19, cap=32)
+
+
+def test_rate64_multi_contig_repeats_with_n():
+    """The learned-seed workload's reference shape at 1.5 Mbp: 24 contigs, 45%
+    repeats from 200 families at 5-25% divergence, GC 0.41, 0.2% N, SA sample
+    rate 64; 150 bp reads from both strands with 1% substitutions, min_len 21,
+    SMEM and MEM lists against the oracle."""
+    text, bounds = O.gen_reference(505, 1_500_000, 24, gc=0.41, repeat_frac=0.45, n_units=200,
+                                   div_min=0.05, div_max=0.25, iupac_frac=0.002, iupac_codes=[5])
+    o = O.OracleBidir.build(text=text, bounds=bounds, sample_rate=64, threads=8)
+    g = hf.BidirFmIndex.from_bytes(o.to_bytes())
+    flat, off = O.gen_reads(9, text, bounds, 1200, 150, sub=0.01, both_strands=True)
+    qs = [flat[off[i]:off[i + 1]] for i in range(len(off) - 1)]
+    check(o, g, qs, 21)
