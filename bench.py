@@ -78,6 +78,8 @@ def parse():
                     help="reference length of the --learned-1g workload")
     ap.add_argument("--big-reads", type=int, default=4_000_000,
                     help="150 bp reads of the --learned-1g workload")
+    ap.add_argument("--random-access-calibration", action="store_true",
+                    help="only time chains of random dependent 64-byte record loads (libhsfm_calib.so)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 1 GiB device buffer 3 times (scales FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
@@ -200,6 +202,23 @@ def cpu_baseline(idx_bytes, sq, so, lq, lo, bounds, args, gpu_smem_sample, gpu_l
     res["locate_search_probes_per_read"] = p["occ_probes"] / nls
     O.probes_enable(False)
     return res
+
+
+def random_record_ceiling(achieved_gbps):
+    """The measured rate of random dependent 64-byte record loads on this GPU
+    (committed calibration run) and the fraction of it `achieved_gbps` reaches."""
+    path = os.path.join(ROOT, "profiles", "random_access_ceiling.json")
+    try:
+        with open(path) as f:
+            rows = json.load(f)["rows"]
+    except (OSError, ValueError, KeyError):
+        return None
+    for r in rows:
+        if r.get("mode") == "64B record" and r.get("window_bytes") == 1 << 32 and r.get("lanes") == 196608:
+            g = float(r["GBps_at_64B"])
+            return {"GBps": g, "frac": round(achieved_gbps / g, 4), "what": "random dependent 64-byte record loads, "
+                    "4 GiB table (profiles/random_access_ceiling.md)"}
+    return None
 
 
 def measured_traffic(kernel, args, nq, nl):
@@ -532,12 +551,41 @@ def main_1g(args):
     print(json.dumps(out), flush=True)
 
 
+def random_access_calibration():
+    """Records per second served to chains of random dependent loads (the
+    access pattern of a backward-search step) by libhsfm_calib.so: whole 64-byte
+    records, 16 bytes of a record, and lo/hi record pairs, over a 4 GiB table
+    (and a 256 MiB window of it), at the persistent kernels' lane count and 4x it."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(ROOT, "haystackfm_amd", "libhsfm_calib.so"))
+    fn = lib.hsfm_calib_random_records
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_int, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_int,
+                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    res = []
+    for window, lanes, mode in [(1 << 32, 196608, 0), (1 << 32, 196608, 1), (1 << 32, 196608, 2),
+                                (1 << 32, 786432, 0), (1 << 32, 786432, 2), (1 << 28, 196608, 0)]:
+        ms, recs = C.c_double(), C.c_double()
+        rc = fn(0, 1 << 32, window, lanes, 200, mode, 3, C.byref(ms), C.byref(recs))
+        if rc != 0:
+            raise RuntimeError(f"hsfm_calib_random_records failed ({rc})")
+        row = {"mode": ["64B record", "16B of a record", "lo/hi record pair"][mode], "window_bytes": window,
+               "lanes": lanes, "ms": round(ms.value, 3), "G_records_per_s": round(recs.value / ms.value / 1e6, 2),
+               "GBps_at_64B": round(recs.value * 64 / ms.value / 1e6, 1)}
+        log(row)
+        res.append(row)
+    print(json.dumps({"random_access_calibration": res}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (reads shard across GPUs by "
                                    "replicating the index, see DESIGN.md)"}))
         sys.exit(2)
+    if args.random_access_calibration:
+        random_access_calibration()
+        return
     import torch
     if args.copy_calibration:
         src = torch.ones(1 << 28, dtype=torch.int32, device="cuda:0")
@@ -628,7 +676,8 @@ def main():
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_mem_persist", args, nq, nl),
                 "kernel": "k_mem_persist", "kernel_ms": round(kms, 3),
-                "bytes_per_read": round(bytes_per_launch / nq, 1)}
+                "bytes_per_read": round(bytes_per_launch / nq, 1),
+                "measured_random_record_ceiling": random_record_ceiling(achieved)}
     loc_roof = None
     lkms = float(np.mean(lk))
     if loc_stats.get("occ_probes") and lkms > 0:
@@ -636,7 +685,8 @@ def main():
         loc_roof = {"bound": "hbm", "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(a / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_locate_search", args, nq, nl),
                     "kernel": "k_locate_search", "kernel_ms": round(lkms, 3),
-                    "bytes_per_read": round(loc_stats["occ_probes"] * 64.0 / nl, 1)}
+                    "bytes_per_read": round(loc_stats["occ_probes"] * 64.0 / nl, 1),
+                    "measured_random_record_ceiling": random_record_ceiling(a)}
     cpu_line = None
     if "smem" in cpu:
         cpu_line = cpu["smem"]

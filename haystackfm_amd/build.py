@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhsfm.so")
 BUILD_LIB = os.path.join(HERE, "libhsfm_build.so")
+CALIB_LIB = os.path.join(HERE, "libhsfm_calib.so")  # measurement aid (random-load ceiling), not the query path
 SOURCES = ["hsfm_runtime.hip", "hsfm_index.hip", "hsfm_locate.hip", "hsfm_mem.hip", "hsfm_lsi.hip"]
 ARCH = os.environ.get("HSFM_OFFLOAD_ARCH", "gfx950")
 
@@ -77,6 +78,14 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout.decode(errors="replace"))
     os.replace(tmp, LIB)
+    tmp = CALIB_LIB + ".tmp"
+    cmd = [cc, *HIPCC_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "hsfm_calib.hip")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("calibration library build failed:\n" + r.stdout.decode(errors="replace"))
+    os.replace(tmp, CALIB_LIB)
     return LIB
 
 
