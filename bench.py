@@ -564,12 +564,14 @@ def random_access_calibration():
                    C.POINTER(C.c_double), C.POINTER(C.c_double)]
     res = []
     for window, lanes, mode in [(1 << 32, 196608, 0), (1 << 32, 196608, 1), (1 << 32, 196608, 2),
-                                (1 << 32, 786432, 0), (1 << 32, 786432, 2), (1 << 28, 196608, 0)]:
+                                (1 << 32, 196608, 3), (1 << 32, 786432, 0), (1 << 32, 786432, 2),
+                                (1 << 28, 196608, 0)]:
         ms, recs = C.c_double(), C.c_double()
         rc = fn(0, 1 << 32, window, lanes, 200, mode, 3, C.byref(ms), C.byref(recs))
         if rc != 0:
             raise RuntimeError(f"hsfm_calib_random_records failed ({rc})")
-        row = {"mode": ["64B record", "16B of a record", "lo/hi record pair"][mode], "window_bytes": window,
+        row = {"mode": ["64B record", "16B of a record", "lo/hi record pair", "64B record, quad-cooperative"][mode],
+               "window_bytes": window,
                "lanes": lanes, "ms": round(ms.value, 3), "G_records_per_s": round(recs.value / ms.value / 1e6, 2),
                "GBps_at_64B": round(recs.value * 64 / ms.value / 1e6, 1)}
         log(row)

@@ -15,7 +15,8 @@ namespace {
 
 // mode 0: a whole 64-byte record (four 16-byte loads); mode 1: 16 bytes of a
 // record; mode 2: two records a few positions apart (the lo/hi ends of an
-// interval), both whole.
+// interval), both whole; mode 3: a whole record, loaded by the four lanes of a
+// quad together and exchanged by shuffles.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_calib_chase(const uint4 *__restrict__ tab, uint64_t mask, uint32_t iters,
                                                      uint32_t *__restrict__ sink) {
@@ -26,7 +27,35 @@ __global__ __launch_bounds__(256) void k_calib_chase(const uint4 *__restrict__ t
         const uint64_t r = ((x >> 20) ^ acc) & mask;
         const uint4 *p = tab + r * 4;
         uint32_t a = 0;
-        if (MODE == 1) {
+        if (MODE == 3) {
+            // the four lanes of a quad fetch each other's records together: in
+            // round i, lane l loads part (l & 3) of the record of quad lane i,
+            // so one instruction touches 16 records, each line once
+            const uint32_t ql = threadIdx.x & 3u, qb = threadIdx.x & ~3u;
+            const uint64_t rr = r;
+            uint4 mine[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint64_t ri = (uint64_t)__shfl((long long)rr, (int)(qb + i), 64) & mask;
+                const uint4 v = tab[ri * 4 + ql];
+                mine[i] = v; // part ql of quad lane i's record
+            }
+            // lane l gathers the four parts of its own record from its quad: in
+            // round k it reads lane s = (l + k) & 3, which offers the part it
+            // holds of record (s - k) & 3, i.e. part s of lane l's record
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t o = (ql - (uint32_t)k) & 3u;
+                const uint4 off = o == 0 ? mine[0] : o == 1 ? mine[1] : o == 2 ? mine[2] : mine[3];
+                const int src = (int)(qb + ((ql + (uint32_t)k) & 3u));
+                uint4 v;
+                v.x = __shfl(off.x, src, 64);
+                v.y = __shfl(off.y, src, 64);
+                v.z = __shfl(off.z, src, 64);
+                v.w = __shfl(off.w, src, 64);
+                a += v.x ^ v.w;
+            }
+        } else if (MODE == 1) {
             const uint4 v = p[0];
             a = v.x ^ v.w;
         } else {
@@ -64,7 +93,7 @@ extern "C" int hsfm_calib_random_records(int ordinal, uint64_t table_bytes, uint
                                          uint32_t iters, int mode, int reps, double *ms_out, double *records_out) {
     if (table_bytes < 65536 || (table_bytes & (table_bytes - 1)) || window_bytes < 64 ||
         (window_bytes & (window_bytes - 1)) || window_bytes > table_bytes || lanes == 0 || lanes % 256 ||
-        mode < 0 || mode > 2 || reps < 1 || !ms_out || !records_out)
+        mode < 0 || mode > 3 || reps < 1 || !ms_out || !records_out)
         return (int)hipErrorInvalidValue;
     hipError_t e = hipSetDevice(ordinal);
     if (e != hipSuccess) return (int)e;
@@ -82,7 +111,8 @@ extern "C" int hsfm_calib_random_records(int ordinal, uint64_t table_bytes, uint
         e = hipEventRecord(a, 0);
         if (mode == 0) hipLaunchKernelGGL(k_calib_chase<0>, grid, block, 0, 0, tab, mask, iters, sink);
         else if (mode == 1) hipLaunchKernelGGL(k_calib_chase<1>, grid, block, 0, 0, tab, mask, iters, sink);
-        else hipLaunchKernelGGL(k_calib_chase<2>, grid, block, 0, 0, tab, mask, iters, sink);
+        else if (mode == 2) hipLaunchKernelGGL(k_calib_chase<2>, grid, block, 0, 0, tab, mask, iters, sink);
+        else hipLaunchKernelGGL(k_calib_chase<3>, grid, block, 0, 0, tab, mask, iters, sink);
         if (e == hipSuccess) e = hipGetLastError();
         if (e == hipSuccess) e = hipEventRecord(b, 0);
         if (e == hipSuccess) e = hipEventSynchronize(b);

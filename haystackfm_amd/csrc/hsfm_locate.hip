@@ -444,6 +444,237 @@ __global__ __launch_bounds__(256) void k_locate_search_wide(DevDir d, const DevC
 }
 
 // ---------------------------------------------------------------------------
+// Second pass, one group of G lanes per read: the same search, seeding and
+// publication as k_locate_search_wide<G>, but each lane of the group owns one
+// interval of the set, so a step costs one dependent round trip instead of
+// one per interval.  A step's children are counted per lane, placed by a
+// prefix sum over the group (sub-wave shuffles) and gathered through the
+// group's LDS list; seeding and the text finish run on lane 0.  Every
+// decision (set size, give-up, loop index) is uniform over the group.
+// ---------------------------------------------------------------------------
+template <uint32_t G>
+__global__ __launch_bounds__(256) void k_locate_search_group(DevDir d, const DevConst *__restrict__ cst,
+                                                             const uint8_t *__restrict__ q,
+                                                             const uint64_t *__restrict__ qoff,
+                                                             const uint32_t *__restrict__ list, uint32_t nlist,
+                                                             uint32_t *__restrict__ cnt, uint8_t *__restrict__ niv,
+                                                             uint2 *__restrict__ ivs, uint2 *__restrict__ fb_meta,
+                                                             uint2 *__restrict__ fb_ivs, uint64_t fb_cap,
+                                                             uint32_t *__restrict__ ovf_list,
+                                                             LocateCounters *__restrict__ ctr,
+                                                             const uint4 *__restrict__ seed_rec,
+                                                             const uint4 *__restrict__ seed_iv, uint32_t F) {
+    static_assert(G >= 16 && G <= 64 && (G & (G - 1)) == 0, "group size: a power of two in 16..64");
+    constexpr uint32_t NG = 256 / G;
+    __shared__ uint16_t s_compat[256];
+    __shared__ int8_t s_digit[256];
+    __shared__ uint8_t s_expand[256];
+    __shared__ uint32_t s_clane[16];
+    __shared__ uint32_t s_lo[NG][G], s_hi[NG][G];
+    load_const_lds(cst, s_compat, s_digit, s_clane);
+    for (uint32_t x = threadIdx.x; x < 256; x += blockDim.x) s_expand[x] = cst->seed_expand[x];
+    __syncthreads();
+    const uint32_t gl = threadIdx.x % G, gi = threadIdx.x / G;
+    const uint32_t t = blockIdx.x * NG + gi;
+    if (t >= nlist) return; // the whole group
+    const uint32_t r = list[t];
+    const uint64_t b0 = qoff[r];
+    const uint32_t m = (uint32_t)(qoff[r + 1] - b0);
+    const uint8_t *p = q + b0;
+    uint32_t *glo = s_lo[gi], *ghi = s_hi[gi];
+
+    // seeding on lane 0, into the group's list (k_locate_search_wide's rules)
+    uint32_t ns = 0, give = 0;
+    int start = (int)m;
+    if (gl == 0) {
+        uint32_t lo0 = 0, hi0 = d.n;
+        ns = 1;
+        const int sd = lookup_seed(d, s_digit, cst->lookup_depth, cst->lookup_radix, p, m, lo0, hi0);
+        bool listed = false;
+        if (sd == 2) ns = 0;
+        else if (sd == 1) start = (int)(m - cst->lookup_depth);
+        else if (seed_rec && m >= F) {
+            SeedWin sw;
+            if (seed_window(p + m - F, F, s_expand, sw, 1)) {
+                uint32_t k2 = 0;
+                bool ok = true;
+                for (uint32_t c = 0; c < seed_ncombo(sw) && ok; c++) {
+                    uint32_t ix = 0;
+                    if (!seed_combo(sw, c, ix)) continue;
+                    const uint4 R = seed_rec[ix];
+                    const uint32_t l = R.w & ((1u << kSeedLslBits) - 1u), n2 = R.w >> kSeedLslBits;
+                    if (l != F || n2 == 0) continue;
+                    const uint64_t o = (uint64_t)R.x | ((uint64_t)R.y << 32);
+                    for (uint32_t e = 0; e < n2 && ok; e++) {
+                        const uint2 iv =
+                            n2 == 1 ? make_uint2(R.x, R.y) : make_uint2(seed_iv[o + e].x, seed_iv[o + e].y);
+                        bool dup = false;
+                        for (uint32_t u = 0; u < k2 && !dup; u++) dup = glo[u] == iv.x;
+                        if (dup) continue;
+                        if (k2 >= G) { ok = false; break; }
+                        glo[k2] = iv.x;
+                        ghi[k2] = iv.y;
+                        k2++;
+                    }
+                }
+                if (ok) {
+                    ns = k2;
+                    start = (int)(m - F);
+                    listed = true;
+                } else {
+                    give = 1;
+                }
+            }
+        }
+        if (!listed && ns == 1) {
+            glo[0] = lo0;
+            ghi[0] = hi0;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    ns = __shfl(ns, 0, G);
+    give = __shfl(give, 0, G);
+    start = __shfl(start, 0, G);
+    uint32_t mlo = 0xffffffffu, mhi = 0xffffffffu;
+    if (gl < ns) {
+        mlo = glo[gl];
+        mhi = ghi[gl];
+    }
+    const bool textmode = d.isa != nullptr && d.text != nullptr && d.sa_full != nullptr;
+    uint32_t tprobes = 0;
+    // Rows held by the group's set, saturated at G + 1 (every lane gets the sum).
+    auto group_rows = [&]() {
+        uint32_t v = gl < ns ? min(mhi - mlo, G + 1u) : 0u;
+#pragma unroll
+        for (uint32_t dd = G / 2; dd >= 1; dd >>= 1) v += __shfl_xor(v, dd, G);
+        return v;
+    };
+    // At most G rows with q[at..m) matched: every row is one text occurrence,
+    // and backward search from a row follows that occurrence alone (its only
+    // child is the row of the previous text position, when the query code is
+    // compatible with the text there).  So each lane takes one row and
+    // compares q[0..at) with the text left of it; the survivors' rows (the
+    // inverse suffix array at their text starts) are the final set, one row
+    // per interval, in the group's list.
+    auto finish_rows = [&](uint32_t at, uint32_t nrows) {
+        uint32_t sz = gl < ns ? mhi - mlo : 0u, off = sz;
+#pragma unroll
+        for (uint32_t dd = 1; dd < G; dd <<= 1) {
+            const uint32_t y = __shfl_up(off, dd, G);
+            if (gl >= dd) off += y;
+        }
+        off -= sz;
+        for (uint32_t u = 0; u < sz; u++) glo[off + u] = mlo + u;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        uint32_t rlo = gl < nrows ? glo[gl] : 0u, rhi = rlo + 1u, rns = gl < nrows ? 1u : 0u;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (rns) locate_finish_text(d, s_compat, p, at, rlo, rhi, rns, tprobes);
+        uint32_t pos = rns;
+#pragma unroll
+        for (uint32_t dd = 1; dd < G; dd <<= 1) {
+            const uint32_t y = __shfl_up(pos, dd, G);
+            if (gl >= dd) pos += y;
+        }
+        ns = __shfl(pos, G - 1, G);
+        if (rns) glo[pos - 1] = rlo;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        mlo = gl < ns ? glo[gl] : 0xffffffffu;
+        mhi = gl < ns ? mlo + 1u : 0xffffffffu;
+    };
+    if (!give && textmode && ns && start > 0) {
+        const uint32_t nrows = group_rows();
+        if (nrows <= G) {
+            finish_rows((uint32_t)start, nrows);
+            start = 0;
+        }
+    }
+    for (int i = start - 1; i >= 0 && ns && !give; i--) {
+        const uint32_t cm = s_compat[p[i]];
+        uint32_t cl[16], ch[16], c = 0;
+        if (gl < ns) {
+            pair_counts(d, mlo, mhi, cl, ch);
+#pragma unroll
+            for (int k = 0; k < 16; k++) c += (((cm >> k) & 1u) && ch[k] > cl[k]) ? 1u : 0u;
+        }
+        uint32_t inc = c;
+#pragma unroll
+        for (uint32_t dd = 1; dd < G; dd <<= 1) {
+            const uint32_t y = __shfl_up(inc, dd, G);
+            if (gl >= dd) inc += y;
+        }
+        const uint32_t tot = __shfl(inc, G - 1, G);
+        if (tot > G) {
+            give = 1;
+            break;
+        }
+        if (gl < ns) {
+            uint32_t w = inc - c;
+#pragma unroll
+            for (int k = 0; k < 16; k++)
+                if (((cm >> k) & 1u) && ch[k] > cl[k]) {
+                    glo[w] = s_clane[k] + cl[k];
+                    ghi[w] = s_clane[k] + ch[k];
+                    w++;
+                }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        ns = tot;
+        mlo = gl < ns ? glo[gl] : 0xffffffffu;
+        mhi = gl < ns ? ghi[gl] : 0xffffffffu;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (textmode && ns && i > 0) {
+            const uint32_t nrows = group_rows();
+            if (nrows <= G) {
+                finish_rows((uint32_t)i, nrows);
+                break;
+            }
+        }
+    }
+    if (!give) {
+        // row order: rank among the group's intervals (disjoint, distinct lo)
+        uint32_t rank = 0, size = gl < ns ? mhi - mlo : 0u;
+#pragma unroll
+        for (uint32_t u = 0; u < G; u++) {
+            const uint32_t lu = __shfl(mlo, u, G);
+            rank += (u < ns && lu < mlo) ? 1u : 0u;
+        }
+#pragma unroll
+        for (uint32_t dd = G / 2; dd >= 1; dd >>= 1) size += __shfl_xor(size, dd, G);
+        if (ns <= KR) {
+            if (gl < ns) ivs[(size_t)r * KR + rank] = make_uint2(mlo, mhi);
+            if (gl == 0) {
+                niv[r] = (uint8_t)ns;
+                cnt[r] = size;
+            }
+            return;
+        }
+        unsigned long long slot = 0;
+        if (gl == 0) slot = atomicAdd(&ctr->fb_used, (unsigned long long)ns);
+        slot = __shfl(slot, 0, G);
+        if (slot + ns <= fb_cap) {
+            if (gl < ns) fb_ivs[slot + rank] = make_uint2(mlo, mhi);
+            if (gl == 0) {
+                fb_meta[r] = make_uint2((uint32_t)slot, ns);
+                niv[r] = kNivOverflow;
+                cnt[r] = size;
+            }
+            return;
+        }
+    }
+    if (gl == 0) {
+        niv[r] = kNivOverflow;
+        cnt[r] = 0;
+        ovf_list[atomicAdd(&ctr->n_ovf, 1ull)] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Pipelined persistent backward search.  Each lane runs locate_read as a state
 // machine.  Every pass of the loop issues, for each lane, one round of
 // independent loads -- one probe (two Occ records, a superblock row and the
@@ -794,6 +1025,7 @@ __global__ __launch_bounds__(64) void k_locate_wave(DevDir d, const DevConst *__
     __syncthreads();
     const uint32_t lane = threadIdx.x;
     const uint32_t L = cst->num_lanes;
+    const bool textmode = d.isa != nullptr && d.text != nullptr && d.sa_full != nullptr && cap >= 64;
     uint2 *bufA = sets + (size_t)blockIdx.x * 2 * cap;
     uint2 *bufB = bufA + cap;
     for (;;) {
@@ -814,6 +1046,28 @@ __global__ __launch_bounds__(64) void k_locate_wave(DevDir d, const DevConst *__
         if (lane == 0) A[0] = make_uint2(lo0, hi0);
         __syncthreads();
         bool too_big = false;
+        // At most 64 rows with q[at..m) matched: every row is one text
+        // occurrence that backward search follows alone, so each lane takes a
+        // row and compares q[0..at) with the text left of it; the survivors
+        // (one row each) become the final set in A.  Returns true when done.
+        auto finish_rows = [&](uint32_t at) -> bool {
+            uint32_t part = 0;
+            for (uint32_t j = lane; j < ns; j += 64) part += min(A[j].y - A[j].x, 65u);
+            const uint32_t rows = wave_sum(part);
+            if (rows > 64) return false;
+            const uint2 iv = lane < ns ? A[lane] : make_uint2(0u, 0u); // rows <= 64 implies ns <= 64
+            const uint32_t sz = iv.y - iv.x, off = wave_incl_scan(sz) - sz;
+            for (uint32_t u = 0; u < sz; u++) B[off + u] = make_uint2(iv.x + u, 0u);
+            __syncthreads();
+            uint32_t rlo = lane < rows ? B[lane].x : 0u, rhi = rlo + 1u, rns = lane < rows ? 1u : 0u, pr = 0;
+            if (rns) locate_finish_text(d, s_compat, p, at, rlo, rhi, rns, pr);
+            const uint32_t pos = wave_incl_scan(rns);
+            ns = __shfl(pos, 63, 64);
+            if (rns) A[pos - 1] = make_uint2(rlo, rlo + 1u);
+            __syncthreads();
+            return true;
+        };
+        if (textmode && ns && start > 0 && finish_rows((uint32_t)start)) start = 0;
         for (int i = start - 1; i >= 0 && ns && !too_big; i--) {
             const uint32_t cm = s_compat[p[i]];
             uint32_t total = 0;
@@ -846,6 +1100,7 @@ __global__ __launch_bounds__(64) void k_locate_wave(DevDir d, const DevConst *__
             __syncthreads();
             uint2 *t = A; A = B; B = t;
             ns = too_big ? ns : total;
+            if (!too_big && textmode && ns && i > 0 && finish_rows((uint32_t)i)) break;
         }
         if (too_big) {
             if (lane == 0) {
@@ -950,6 +1205,7 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
     const char *path = getenv("HSFM_LOCATE_PATH");
     const bool pipe_all = path && strcmp(path, "pipe") == 0;
     const bool pipe_second = path && strcmp(path, "pipe2") == 0; // pipelined kernel as the second level
+    const bool wide_second = path && strcmp(path, "wide") == 0;  // thread-per-read second level
     SeedView sv{nullptr, nullptr, nullptr, 0};
     if (nq) {
         const uint32_t F = 12;
@@ -979,7 +1235,17 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
             idx->stats.occ_probes = h[5]; // the main kernel's own probes
         }
         idx->stats.fallback_reads = pipe_all ? 0 : nlist;
-        if (nlist && !pipe_all && !pipe_second) {
+        if (nlist && !pipe_all && !pipe_second && !wide_second) {
+            // second pass: 16 lanes per read, room for 16 intervals
+            if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_ovf), 2, 0))) return rc;
+            hipLaunchKernelGGL(k_locate_search_group<16>, dim3((uint32_t)((nlist + 15) / 16)), dim3(256), 0, s, d,
+                               idx->d_const.as<DevConst>(), d_q, d_qoff, list, (uint32_t)nlist,
+                               ws.cnt.as<uint32_t>(), ws.niv.as<uint8_t>(), ws.ivs.as<uint2>(),
+                               ws.fb_meta.as<uint2>(), ws.fb_ivs.as<uint2>(), (uint64_t)(ws.fb_ivs.cap / 8),
+                               ws.read_list.as<uint32_t>(), ctr, idx->seed_rec12.as<uint4>(),
+                               idx->seed_iv.as<uint4>(), idx->seed_rec12.p ? 12u : 0u);
+            HSFM_HIP(hipGetLastError());
+        } else if (nlist && wide_second) {
             // second register pass with room for 16 intervals per read
             if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_ovf), 2, 0))) return rc;
             hipLaunchKernelGGL(k_locate_search_wide<16>, dim3((uint32_t)((nlist + 255) / 256)), dim3(256), 0, s, d,

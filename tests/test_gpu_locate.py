@@ -124,12 +124,13 @@ def test_lookup12_with_iupac_reference():
     assert g.last_stats()["guard_hits"] == 0
 
 
-@pytest.mark.parametrize("path", ["pipe", "pipe2"])
+@pytest.mark.parametrize("path", ["pipe", "pipe2", "wide"])
 def test_persistent_and_register_kernels_agree(monkeypatch, path):
-    """The default levels (register kernel, then the 16-interval register pass)
-    and the pipelined ring kernel (HSFM_LOCATE_PATH=pipe for every read, pipe2
-    as the second level) return identical hits, including reads whose final
-    set is published through the fallback interval list."""
+    """The default levels (register kernel, then the 16-lanes-per-read pass)
+    and the alternatives (HSFM_LOCATE_PATH=pipe: the pipelined ring kernel for
+    every read; pipe2: it as the second level; wide: the thread-per-read
+    16-interval pass as the second level) return identical hits, including
+    reads whose final set is published through the fallback interval list."""
     rng = np.random.default_rng(12)
     text, bounds = O.gen_reference(13, 300_000, 3, gc=0.41, repeat_frac=0.3, n_units=10, unit_min=300,
                                    unit_max=1000, iupac_frac=0.02)
