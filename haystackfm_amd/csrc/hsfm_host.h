@@ -62,6 +62,9 @@ struct Workspace {
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}; // main, resolve, fallback kernels
 };
 
+struct BatchBufs; // buffers of the batched MEM pipeline (hsfm_mem.hip)
+void free_batch_bufs(BatchBufs *b);
+
 } // namespace hsfm
 
 struct hsfm_index {
@@ -84,6 +87,8 @@ struct hsfm_index {
     // batched MEM pipeline: configuration picked by the autotuner
     int tuned_main = -1, tuned_rpt = -1;
     uint32_t tuned_min_len = 0, tuned_batch = 0;
+    // its buffers, kept across calls (grown when a call needs more)
+    hsfm::BatchBufs *batch = nullptr;
 };
 
 namespace hsfm {

@@ -425,6 +425,7 @@ extern "C" int hsfm_index_free(hsfm_index *idx) {
     idx->seed_rec12.release();
     idx->seed_rec14.release();
     idx->seed_iv14.release();
+    free_batch_bufs(idx->batch);
     delete idx;
     return HSFM_OK;
 }

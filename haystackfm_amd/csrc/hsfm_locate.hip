@@ -319,7 +319,6 @@ __global__ __launch_bounds__(256) void k_locate_search_wide(DevDir d, const DevC
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nlist) return;
     const uint32_t r = list[t];
-    const uint32_t L = cst->num_lanes;
     const uint64_t b0 = qoff[r];
     const uint32_t m = (uint32_t)(qoff[r + 1] - b0);
     const uint8_t *p = q + b0;

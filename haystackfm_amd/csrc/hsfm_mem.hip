@@ -2018,9 +2018,15 @@ constexpr uint32_t kBatchGlobalCap = 65536; // intervals per set in the in-graph
 constexpr uint32_t kBatchGlobalWaves = 64;  // workgroups of that pass
 enum : int { BM_PERSIST8 = 0, BM_PERSIST16, BM_PERSIST32, BM_WAVE, BM_COUNT };
 
-__global__ void k_pad_qoff(uint64_t *bq, uint32_t nb, uint32_t B) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x + nb + 1;
-    if (i <= B) bq[i] = bq[nb]; // padding reads have length 0
+// The batch's read offsets from the call's: reads [r0, r0 + nb) of
+// args = {r0, nb}, then zero-length padding reads up to B (nb <= B and
+// r0 + nb <= nq, checked by the host).
+__global__ void k_stage_qoff(const uint64_t *__restrict__ qoff, const uint64_t *__restrict__ args,
+                             uint64_t *__restrict__ bq, uint32_t B) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > B) return;
+    const uint64_t r0 = args[0], nb = args[1];
+    bq[i] = qoff[r0 + (i <= nb ? i : nb)];
 }
 __global__ void k_npos_total(const uint64_t *mem_off, uint32_t B, const uint64_t *pos_off, uint64_t pos_cap,
                              uint64_t *tot) {
@@ -2040,14 +2046,21 @@ struct BatchBufs {
     DevBuf mem_off, sorted, npos, pos_off, hits, rows, positions, misc, tmp;
     DevBuf items, item_cnt, item_first, read_off;
     DevBuf qnib; // all queries as base-set nibbles (read by every batch's graph)
+    DevBuf bargs; // {first read, reads} of the batch, copied in before each launch
+    const uint64_t *qoff_all = nullptr; // the call's read offsets (nq + 1)
     size_t tmp_bytes = 0;
     void release() {
         DevBuf *all[] = {&bq_off, &slots, &mem_cnt, &status, &pass_of, &l1, &l2, &l3, &ctr, &recs, &wsets,
                          &mem_off, &sorted, &npos, &pos_off, &hits, &rows, &positions, &misc, &tmp,
-                         &items, &item_cnt, &item_first, &read_off, &qnib};
+                         &items, &item_cnt, &item_first, &read_off, &qnib, &bargs};
         for (DevBuf *b : all) b->release();
     }
 };
+void free_batch_bufs(BatchBufs *b) {
+    if (!b) return;
+    b->release();
+    delete b;
+}
 // items [item_first[B], W) of a batch: empty (read 0, no starts)
 __global__ void k_pad_items(uint4 *items, const uint64_t *first_end, uint32_t W) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2084,6 +2097,18 @@ static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, cons
     const DevConst *cst = idx->d_const.as<DevConst>();
     const uint64_t *bq = bb.bq_off.as<uint64_t>();
     int rc;
+    // stage the batch: its read offsets (from bargs), then its work items and
+    // empty items up to the capacity
+    hipLaunchKernelGGL(k_stage_qoff, dim3((B + 256) / 256), dim3(256), 0, s, bb.qoff_all, bb.bargs.as<uint64_t>(),
+                       bb.bq_off.as<uint64_t>(), B);
+    hipLaunchKernelGGL(k_items_count, dim3((B + 255) / 256), dim3(256), 0, s, B, bb.bq_off.as<uint64_t>(), bb.seg,
+                       bb.item_cnt.as<uint32_t>());
+    if ((rc = batch_scan(idx, bb, bb.item_cnt.as<uint32_t>(), bb.item_first.as<uint64_t>(), B))) return rc;
+    hipLaunchKernelGGL(k_items_fill, dim3((B + 255) / 256), dim3(256), 0, s, B, bb.bq_off.as<uint64_t>(), bb.seg,
+                       bb.item_first.as<uint64_t>(), bb.items.as<uint4>());
+    hipLaunchKernelGGL(k_pad_items, dim3((bb.W + 255) / 256), dim3(256), 0, s, bb.items.as<uint4>(),
+                       bb.item_first.as<uint64_t>() + B, bb.W);
+    HSFM_HIP(hipGetLastError());
     if ((rc = fill_u32(idx, bb.ctr.as<uint32_t>(), 3 * sizeof(MemCounters) / 4, 0)) ||
         (rc = fill_u32(idx, bb.misc.as<uint32_t>(), sizeof(BatchMisc) / 4, 0)) ||
         (rc = fill_u32(idx, bb.mem_cnt.as<uint32_t>(), W, 0)) ||
@@ -2197,23 +2222,16 @@ static int batch_capture(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, con
     return HSFM_OK;
 }
 
-// Stage batch [r0, r0 + nb) into the graph's offset buffer.
-static int batch_stage(hsfm_index *idx, BatchBufs &bb, const uint64_t *d_qoff, uint32_t r0, uint32_t nb) {
+// The batch's first read and size for the next launch (the graph stages the
+// offsets and work items itself).  The pinned slot is rewritten only after
+// the previous batch's wait, so the copy has run by then.
+static int batch_set_args(hsfm_index *idx, BatchBufs &bb, uint32_t r0, uint32_t nb) {
     hipStream_t s = idx->dev->stream;
-    const uint32_t B = bb.B;
-    HSFM_HIP(hipMemcpyAsync(bb.bq_off.p, d_qoff + r0, ((size_t)nb + 1) * 8, hipMemcpyDeviceToDevice, s));
-    if (nb < B)
-        hipLaunchKernelGGL(k_pad_qoff, dim3((B - nb + 255) / 256), dim3(256), 0, s, bb.bq_off.as<uint64_t>(), nb, B);
-    // work items of the batch, then empty items up to the capacity
-    hipLaunchKernelGGL(k_items_count, dim3((B + 255) / 256), dim3(256), 0, s, B, bb.bq_off.as<uint64_t>(), bb.seg,
-                       bb.item_cnt.as<uint32_t>());
-    int rc = batch_scan(idx, bb, bb.item_cnt.as<uint32_t>(), bb.item_first.as<uint64_t>(), B);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_items_fill, dim3((B + 255) / 256), dim3(256), 0, s, B, bb.bq_off.as<uint64_t>(), bb.seg,
-                       bb.item_first.as<uint64_t>(), bb.items.as<uint4>());
-    hipLaunchKernelGGL(k_pad_items, dim3((bb.W + 255) / 256), dim3(256), 0, s, bb.items.as<uint4>(),
-                       bb.item_first.as<uint64_t>() + B, bb.W);
-    HSFM_HIP(hipGetLastError());
+    if (!idx->ws.h_pinned) HSFM_HIP(hipHostMalloc((void **)&idx->ws.h_pinned, 4096, hipHostMallocDefault));
+    uint64_t *a = idx->ws.h_pinned + 256;
+    a[0] = r0;
+    a[1] = nb;
+    HSFM_HIP(hipMemcpyAsync(bb.bargs.p, a, 16, hipMemcpyHostToDevice, s));
     return HSFM_OK;
 }
 
@@ -2247,8 +2265,10 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     HSFM_HIP(hipMemcpyAsync(hoff.data(), d_qoff, ((size_t)nq + 1) * 8, hipMemcpyDeviceToHost, s));
     HSFM_HIP(hipStreamSynchronize(s));
     const uint32_t B = cfg->batch_reads < nq ? cfg->batch_reads : (nq ? nq : 1);
-    BatchBufs bb;
+    if (!idx->batch) idx->batch = new BatchBufs();
+    BatchBufs &bb = *idx->batch; // kept across calls: ensure() below only grows it
     bb.B = B;
+    bb.qoff_all = d_qoff;
     if ((rc = pack_bases(idx, d_q, hoff[nq], bb.qnib))) {
         bb.release();
         return rc;
@@ -2288,7 +2308,7 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
         bb.tmp_bytes = t1 > t2 ? t1 : t2;
     }
     const size_t Wn = bb.W;
-    if ((rc = bb.bq_off.ensure(((size_t)B + 1) * 8)) || (rc = bb.slots.ensure(bb.slot_cap * 16)) ||
+    if ((rc = bb.bq_off.ensure(((size_t)B + 1) * 8)) || (rc = bb.bargs.ensure(64)) || (rc = bb.slots.ensure(bb.slot_cap * 16)) ||
         (rc = bb.mem_cnt.ensure(Wn * 4 + 4)) || (rc = bb.status.ensure(Wn + 8)) ||
         (rc = bb.pass_of.ensure(Wn * 4 + 4)) || (rc = bb.l1.ensure(Wn * 4 + 4)) ||
         (rc = bb.l2.ensure(Wn * 4 + 4)) || (rc = bb.l3.ensure(Wn * 4 + 4)) ||
@@ -2309,7 +2329,6 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     HSFM_HIP(hipEventCreate(&e1));
     auto cleanup = [&](int code) {
         (void)hipStreamSynchronize(s);
-        bb.release();
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         return code;
@@ -2330,7 +2349,7 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
                 if (r2) return r2;
                 ms = 1e30;
                 for (int rep = 0; rep < 2; rep++) {
-                    if ((r2 = batch_stage(idx, bb, d_qoff, 0, nb0))) break;
+                    if ((r2 = batch_set_args(idx, bb, 0, nb0))) break;
                     hipEvent_t a, b;
                     HSFM_HIP(hipEventCreate(&a));
                     HSFM_HIP(hipEventCreate(&b));
@@ -2390,7 +2409,7 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     HSFM_HIP(hipEventRecord(e0, s));
     for (uint32_t r0 = 0; r0 < nq; r0 += B) {
         const uint32_t nb = r0 + B < nq ? B : nq - r0;
-        if ((rc = batch_stage(idx, bb, d_qoff, r0, nb))) break;
+        if ((rc = batch_set_args(idx, bb, r0, nb))) break;
         if (cfg->use_graph) {
             if (hipGraphLaunch(gr.x, s) != hipSuccess) {
                 rc = set_error(HSFM_EDEVICE, "hipGraphLaunch failed");
@@ -2403,10 +2422,17 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
         uint64_t h[8];
         MemCounters *c = bb.ctr.as<MemCounters>();
         const uint64_t *m = bb.misc.as<uint64_t>();
-        if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(&c[2].n_retry), h, 1)) ||
-            (rc = readback_u64(idx, m, h + 1, 3)) ||
-            (rc = readback_u64(idx, bb.mem_off.as<uint64_t>() + bb.W, h + 4, 1)))
-            break;
+        {   // the three reads in one round trip
+            uint64_t *hp = idx->ws.h_pinned;
+            if (hipMemcpyAsync(hp, &c[2].n_retry, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipMemcpyAsync(hp + 1, m, 24, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipMemcpyAsync(hp + 4, bb.mem_off.as<uint64_t>() + bb.W, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess) {
+                rc = set_error(HSFM_EDEVICE, "batch result read-back failed");
+                break;
+            }
+            memcpy(h, hp, 5 * 8);
+        }
         const uint64_t late = h[0], pos_ovf = h[1], npos = h[2], guard = h[3], nhits = h[4];
         if (guard) {
             rc = set_error(HSFM_EINDEX, "%llu LF walks exceeded the sample rate: index is inconsistent",
