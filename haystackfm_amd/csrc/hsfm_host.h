@@ -87,8 +87,12 @@ struct hsfm_index {
     // batched MEM pipeline: configuration picked by the autotuner
     int tuned_main = -1, tuned_rpt = -1;
     uint32_t tuned_min_len = 0, tuned_batch = 0;
-    // its buffers, kept across calls (grown when a call needs more)
-    hsfm::BatchBufs *batch = nullptr;
+    // its buffers, kept across calls (grown when a call needs more): set 0 on
+    // the device stream, further sets with streams of their own, so that
+    // consecutive batches overlap
+    static constexpr int kMaxBatchSets = 4;
+    hsfm::BatchBufs *batch[kMaxBatchSets] = {nullptr, nullptr, nullptr, nullptr};
+    hipStream_t bstream[kMaxBatchSets] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace hsfm {

@@ -414,6 +414,8 @@ extern "C" int hsfm_index_free(hsfm_index *idx) {
     if (!idx) return HSFM_OK;
     (void)hipSetDevice(idx->dev->ordinal);
     (void)hipStreamSynchronize(idx->dev->stream);
+    for (hipStream_t bs : idx->bstream)
+        if (bs) (void)hipStreamSynchronize(bs);
     free_dir(idx->fwd);
     free_dir(idx->rev);
     idx->d_const.release();
@@ -425,7 +427,10 @@ extern "C" int hsfm_index_free(hsfm_index *idx) {
     idx->seed_rec12.release();
     idx->seed_rec14.release();
     idx->seed_iv14.release();
-    free_batch_bufs(idx->batch);
+    for (int k = 0; k < hsfm_index::kMaxBatchSets; k++) {
+        free_batch_bufs(idx->batch[k]);
+        if (idx->bstream[k]) (void)hipStreamDestroy(idx->bstream[k]);
+    }
     delete idx;
     return HSFM_OK;
 }

@@ -2222,13 +2222,34 @@ static int batch_capture(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, con
     return HSFM_OK;
 }
 
+// Device buffers of one batch set for the capacities in bb (kept and grown).
+static int batch_alloc(BatchBufs &bb) {
+    const size_t Wn = bb.W;
+    const uint32_t B = bb.B;
+    int rc;
+    if ((rc = bb.bq_off.ensure(((size_t)B + 1) * 8)) || (rc = bb.bargs.ensure(64)) || (rc = bb.slots.ensure(bb.slot_cap * 16)) ||
+        (rc = bb.mem_cnt.ensure(Wn * 4 + 4)) || (rc = bb.status.ensure(Wn + 8)) ||
+        (rc = bb.pass_of.ensure(Wn * 4 + 4)) || (rc = bb.l1.ensure(Wn * 4 + 4)) ||
+        (rc = bb.l2.ensure(Wn * 4 + 4)) || (rc = bb.l3.ensure(Wn * 4 + 4)) ||
+        (rc = bb.items.ensure(Wn * 16 + 16)) || (rc = bb.item_cnt.ensure((size_t)B * 4 + 4)) ||
+        (rc = bb.item_first.ensure(((size_t)B + 1) * 8)) || (rc = bb.read_off.ensure(((size_t)B + 1) * 8)) ||
+        (rc = bb.ctr.ensure(3 * sizeof(MemCounters))) || (rc = bb.recs.ensure(bb.rec_cap * sizeof(MemRec))) ||
+        (rc = bb.wsets.ensure((size_t)kBatchGlobalWaves * 2 * kBatchGlobalCap * 16)) ||
+        (rc = bb.mem_off.ensure((Wn + 1) * 8)) || (rc = bb.sorted.ensure(bb.hit_cap * 16)) ||
+        (rc = bb.npos.ensure(bb.hit_cap * 4 + 4)) || (rc = bb.pos_off.ensure((bb.hit_cap + 1) * 8)) ||
+        (rc = bb.hits.ensure(bb.hit_cap * sizeof(hsfm_mem_hit))) || (rc = bb.rows.ensure(bb.pos_cap * 4)) ||
+        (rc = bb.positions.ensure(bb.pos_cap * 8)) || (rc = bb.misc.ensure(sizeof(BatchMisc))) ||
+        (rc = bb.tmp.ensure(bb.tmp_bytes + 256)))
+        return rc;
+    return HSFM_OK;
+}
+
 // The batch's first read and size for the next launch (the graph stages the
 // offsets and work items itself).  The pinned slot is rewritten only after
 // the previous batch's wait, so the copy has run by then.
-static int batch_set_args(hsfm_index *idx, BatchBufs &bb, uint32_t r0, uint32_t nb) {
-    hipStream_t s = idx->dev->stream;
+static int batch_set_args(hsfm_index *idx, BatchBufs &bb, uint32_t r0, uint32_t nb, hipStream_t s, int k) {
     if (!idx->ws.h_pinned) HSFM_HIP(hipHostMalloc((void **)&idx->ws.h_pinned, 4096, hipHostMallocDefault));
-    uint64_t *a = idx->ws.h_pinned + 256;
+    uint64_t *a = idx->ws.h_pinned + 384 + 8 * k; // this set's slot (read-back slots: 64 per set below 320)
     a[0] = r0;
     a[1] = nb;
     HSFM_HIP(hipMemcpyAsync(bb.bargs.p, a, 16, hipMemcpyHostToDevice, s));
@@ -2265,8 +2286,8 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     HSFM_HIP(hipMemcpyAsync(hoff.data(), d_qoff, ((size_t)nq + 1) * 8, hipMemcpyDeviceToHost, s));
     HSFM_HIP(hipStreamSynchronize(s));
     const uint32_t B = cfg->batch_reads < nq ? cfg->batch_reads : (nq ? nq : 1);
-    if (!idx->batch) idx->batch = new BatchBufs();
-    BatchBufs &bb = *idx->batch; // kept across calls: ensure() below only grows it
+    if (!idx->batch[0]) idx->batch[0] = new BatchBufs();
+    BatchBufs &bb = *idx->batch[0]; // kept across calls: ensure() below only grows it
     bb.B = B;
     bb.qoff_all = d_qoff;
     if ((rc = pack_bases(idx, d_q, hoff[nq], bb.qnib))) {
@@ -2307,20 +2328,7 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
         HSFM_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, t2, it, (uint64_t *)nullptr, (int)bb.hit_cap, s));
         bb.tmp_bytes = t1 > t2 ? t1 : t2;
     }
-    const size_t Wn = bb.W;
-    if ((rc = bb.bq_off.ensure(((size_t)B + 1) * 8)) || (rc = bb.bargs.ensure(64)) || (rc = bb.slots.ensure(bb.slot_cap * 16)) ||
-        (rc = bb.mem_cnt.ensure(Wn * 4 + 4)) || (rc = bb.status.ensure(Wn + 8)) ||
-        (rc = bb.pass_of.ensure(Wn * 4 + 4)) || (rc = bb.l1.ensure(Wn * 4 + 4)) ||
-        (rc = bb.l2.ensure(Wn * 4 + 4)) || (rc = bb.l3.ensure(Wn * 4 + 4)) ||
-        (rc = bb.items.ensure(Wn * 16 + 16)) || (rc = bb.item_cnt.ensure((size_t)B * 4 + 4)) ||
-        (rc = bb.item_first.ensure(((size_t)B + 1) * 8)) || (rc = bb.read_off.ensure(((size_t)B + 1) * 8)) ||
-        (rc = bb.ctr.ensure(3 * sizeof(MemCounters))) || (rc = bb.recs.ensure(bb.rec_cap * sizeof(MemRec))) ||
-        (rc = bb.wsets.ensure((size_t)kBatchGlobalWaves * 2 * kBatchGlobalCap * 16)) ||
-        (rc = bb.mem_off.ensure((Wn + 1) * 8)) || (rc = bb.sorted.ensure(bb.hit_cap * 16)) ||
-        (rc = bb.npos.ensure(bb.hit_cap * 4 + 4)) || (rc = bb.pos_off.ensure((bb.hit_cap + 1) * 8)) ||
-        (rc = bb.hits.ensure(bb.hit_cap * sizeof(hsfm_mem_hit))) || (rc = bb.rows.ensure(bb.pos_cap * 4)) ||
-        (rc = bb.positions.ensure(bb.pos_cap * 8)) || (rc = bb.misc.ensure(sizeof(BatchMisc))) ||
-        (rc = bb.tmp.ensure(bb.tmp_bytes + 256))) {
+    if ((rc = batch_alloc(bb))) {
         bb.release();
         return rc;
     }
@@ -2349,7 +2357,7 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
                 if (r2) return r2;
                 ms = 1e30;
                 for (int rep = 0; rep < 2; rep++) {
-                    if ((r2 = batch_set_args(idx, bb, 0, nb0))) break;
+                    if ((r2 = batch_set_args(idx, bb, 0, nb0, s, 0))) break;
                     hipEvent_t a, b;
                     HSFM_HIP(hipEventCreate(&a));
                     HSFM_HIP(hipEventCreate(&b));
@@ -2402,70 +2410,132 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
         return cleanup(set_error(HSFM_EINVAL, "bad batch configuration (main %d, fan-out %d)", main_kernel, rpt));
     stats.main_kernel = main_kernel;
     stats.resolve_fanout = rpt;
-    BatchGraph gr;
-    if (cfg->use_graph && (rc = batch_capture(idx, bb, sv, svp, d_q, bb.qnib.as<uint32_t>(), min_len, d_refb, n_refs, max_hits, main_kernel, rpt,
-                                              gr)))
+    // Several buffer sets, each with its own graph and stream, when there is
+    // more than one batch: batch b + 1 runs while batch b's tail drains (a
+    // 64k-read batch has fewer work items than the persistent kernel has
+    // lanes, so its last items leave most of the GPU idle) and while the host
+    // hands batch b to the callback.  Batches complete and reach the callback
+    // in order.
+    constexpr int KS = hsfm_index::kMaxBatchSets;
+    BatchGraph gr[KS];
+    BatchBufs *set[KS] = {&bb, nullptr, nullptr, nullptr};
+    hipStream_t sst[KS] = {s, nullptr, nullptr, nullptr};
+    const uint32_t nbatches = nq ? (nq + B - 1) / B : 0;
+    int nsets = 1;
+    if (cfg->use_graph && nbatches > 1) {
+        const char *e = getenv("HSFM_BATCH_SETS");
+        const long v = e && *e ? strtol(e, nullptr, 10) : 3; // 3 measured best of 2-4
+        nsets = v < 1 ? 1 : v > KS ? KS : (int)v;
+        if ((uint32_t)nsets > nbatches) nsets = (int)nbatches;
+    }
+    auto release_graphs = [&]() {
+        for (BatchGraph &g : gr) g.release();
+    };
+    if (cfg->use_graph && (rc = batch_capture(idx, bb, sv, svp, d_q, bb.qnib.as<uint32_t>(), min_len, d_refb, n_refs,
+                                              max_hits, main_kernel, rpt, gr[0])))
         return cleanup(rc);
+    for (int k = 1; k < nsets; k++) {
+        if (!idx->batch[k]) idx->batch[k] = new BatchBufs();
+        BatchBufs &bk = *idx->batch[k];
+        bk.B = bb.B; bk.W = bb.W; bk.seg = bb.seg;
+        bk.slot_cap = bb.slot_cap; bk.rec_cap = bb.rec_cap; bk.hit_cap = bb.hit_cap; bk.pos_cap = bb.pos_cap;
+        bk.tmp_bytes = bb.tmp_bytes;
+        bk.qoff_all = d_qoff;
+        if ((rc = batch_alloc(bk))) {
+            release_graphs();
+            return cleanup(rc);
+        }
+        if (!idx->bstream[k] && hipStreamCreateWithFlags(&idx->bstream[k], hipStreamNonBlocking) != hipSuccess) {
+            release_graphs();
+            return cleanup(set_error(HSFM_EDEVICE, "hipStreamCreate failed"));
+        }
+        set[k] = &bk;
+        sst[k] = idx->bstream[k];
+        if ((rc = batch_capture(idx, bk, sv, svp, d_q, bb.qnib.as<uint32_t>(), min_len, d_refb, n_refs, max_hits,
+                                main_kernel, rpt, gr[k]))) {
+            release_graphs();
+            return cleanup(rc);
+        }
+    }
     HSFM_HIP(hipEventRecord(e0, s));
-    for (uint32_t r0 = 0; r0 < nq; r0 += B) {
-        const uint32_t nb = r0 + B < nq ? B : nq - r0;
-        if ((rc = batch_set_args(idx, bb, r0, nb))) break;
-        if (cfg->use_graph) {
-            if (hipGraphLaunch(gr.x, s) != hipSuccess) {
-                rc = set_error(HSFM_EDEVICE, "hipGraphLaunch failed");
-                break;
-            }
-        } else if ((rc = batch_record(idx, bb, sv, svp, d_q, bb.qnib.as<uint32_t>(), min_len, d_refb, n_refs, max_hits, main_kernel, rpt))) {
-            break;
-        }
-        // one wait per batch: counts and overflow flags
+    for (int k = 1; k < nsets; k++) HSFM_HIP(hipStreamWaitEvent(sst[k], e0, 0)); // after the set-up on s
+    struct Pending {
+        uint32_t r0, nb;
+        int k;
+    };
+    Pending pend[KS];
+    int npend = 0;
+    // wait for batch pd, read its counters back (one round trip) and hand it on
+    auto finish = [&](const Pending &pd) -> int {
+        BatchBufs &b = *set[pd.k];
+        const hipStream_t st = sst[pd.k];
+        MemCounters *c = b.ctr.as<MemCounters>();
+        const uint64_t *m = b.misc.as<uint64_t>();
         uint64_t h[8];
-        MemCounters *c = bb.ctr.as<MemCounters>();
-        const uint64_t *m = bb.misc.as<uint64_t>();
-        {   // the three reads in one round trip
-            uint64_t *hp = idx->ws.h_pinned;
-            if (hipMemcpyAsync(hp, &c[2].n_retry, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
-                hipMemcpyAsync(hp + 1, m, 24, hipMemcpyDeviceToHost, s) != hipSuccess ||
-                hipMemcpyAsync(hp + 4, bb.mem_off.as<uint64_t>() + bb.W, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
-                hipStreamSynchronize(s) != hipSuccess) {
-                rc = set_error(HSFM_EDEVICE, "batch result read-back failed");
-                break;
-            }
-            memcpy(h, hp, 5 * 8);
-        }
+        uint64_t *hp = idx->ws.h_pinned + 64 * (pd.k + 1); // this set's read-back slots
+        if (hipMemcpyAsync(hp, &c[2].n_retry, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(hp + 1, m, 24, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(hp + 4, b.mem_off.as<uint64_t>() + b.W, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return set_error(HSFM_EDEVICE, "batch result read-back failed");
+        memcpy(h, hp, 5 * 8);
         const uint64_t late = h[0], pos_ovf = h[1], npos = h[2], guard = h[3], nhits = h[4];
-        if (guard) {
-            rc = set_error(HSFM_EINDEX, "%llu LF walks exceeded the sample rate: index is inconsistent",
-                           (unsigned long long)guard);
-            break;
-        }
+        if (guard)
+            return set_error(HSFM_EINDEX, "%llu LF walks exceeded the sample rate: index is inconsistent",
+                             (unsigned long long)guard);
         stats.batches++;
         hsfm_mem_device_result res{};
-        if (late || pos_ovf || nhits > bb.hit_cap) {
+        if (late || pos_ovf || nhits > b.hit_cap) {
             // outgrew the in-graph capacities: this batch through the host-driven path
             stats.fallback_batches++;
             lk.unlock();
-            rc = hsfm_find_mems_device(idx, d_q, d_qoff + r0, nb, min_len, mode, d_refb, n_refs, max_hits, &res);
+            const int r2 = hsfm_find_mems_device(idx, d_q, d_qoff + pd.r0, pd.nb, min_len, mode, d_refb, n_refs,
+                                                 max_hits, &res);
             lk.lock();
-            if (rc) break;
+            if (r2) return r2;
         } else {
-            res.num_queries = nb;
+            res.num_queries = pd.nb;
             res.num_hits = nhits;
             res.num_positions = n_refs ? npos : 0;
-            res.d_hit_offsets = bb.read_off.as<uint64_t>();
-            res.d_hits = bb.hits.as<hsfm_mem_hit>();
-            res.d_positions = bb.positions.as<uint32_t>();
+            res.d_hit_offsets = b.read_off.as<uint64_t>();
+            res.d_hits = b.hits.as<hsfm_mem_hit>();
+            res.d_positions = b.positions.as<uint32_t>();
         }
         stats.records += res.num_hits;
         stats.positions += res.num_positions;
-        if (fn) fn(user, r0, nb, &res);
+        if (fn) fn(user, pd.r0, pd.nb, &res);
+        return HSFM_OK;
+    };
+    uint32_t kb = 0;
+    for (uint32_t r0 = 0; r0 < nq && !rc; r0 += B, kb++) {
+        const uint32_t nb = r0 + B < nq ? B : nq - r0;
+        const int k = (int)(kb % (uint32_t)nsets);
+        if (npend == nsets) { // the oldest pending batch holds set k: finish it first
+            rc = finish(pend[0]);
+            for (int x = 1; x < npend; x++) pend[x - 1] = pend[x];
+            npend--;
+            if (rc) break;
+        }
+        if ((rc = batch_set_args(idx, *set[k], r0, nb, sst[k], k))) break;
+        if (cfg->use_graph) {
+            if (hipGraphLaunch(gr[k].x, sst[k]) != hipSuccess) {
+                rc = set_error(HSFM_EDEVICE, "hipGraphLaunch failed");
+                break;
+            }
+        } else if ((rc = batch_record(idx, bb, sv, svp, d_q, bb.qnib.as<uint32_t>(), min_len, d_refb, n_refs,
+                                      max_hits, main_kernel, rpt))) {
+            break;
+        }
+        pend[npend++] = Pending{r0, nb, k};
     }
+    for (int x = 0; x < npend && !rc; x++) rc = finish(pend[x]);
+    for (int k = 1; k < nsets; k++) (void)hipStreamSynchronize(sst[k]);
     HSFM_HIP(hipEventRecord(e1, s));
     HSFM_HIP(hipEventSynchronize(e1));
     float t = 0.f;
     HSFM_HIP(hipEventElapsedTime(&t, e0, e1));
     stats.total_ms = t;
-    gr.release();
+    release_graphs();
     if (st) *st = stats;
     return cleanup(rc);
 }
