@@ -32,6 +32,7 @@ __all__ = [
     "DeserializeError", "Alphabet", "AlphabetFns", "IupacDna", "ExactDna", "DnaSequence",
     "locate_batch_gpu", "context_cache", "encode", "alphabet_fns_from_tag", "lib_path",
     "LearnedSeedIndex", "kmer_keys", "kmer_search", "kmer_search_device", "lsi_gradient",
+    "reverse_complement", "reverse_complement_device",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -297,6 +298,8 @@ def _lib():
         L.hsfm_lsi_seed.argtypes = [vp, vp, C.c_uint64, vp, C.POINTER(_SeedStats)]
         L.hsfm_kmer_search.argtypes = [vp, C.c_uint32, vp, C.c_uint64, vp]
         L.hsfm_lsi_grad.argtypes = [vp, vp, C.c_uint32, vp, vp, C.c_uint32, vp, vp]
+        L.hsfm_revcomp_device.argtypes = [vp, vp, vp, C.c_uint32, vp]
+        L.hsfm_revcomp_batch.argtypes = [vp, vp, vp, C.c_uint32, vp]
         L.hsfm_lsi_free.argtypes = [vp]
         L.hsfm_find_mems_batched.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, C.c_int, vp, C.c_uint32,
                                              C.c_uint32, C.POINTER(_BatchConfig), _BATCH_FN, vp,
@@ -759,6 +762,27 @@ def kmer_search_device(index, k: int, d_keys: int, n: int, d_out: int) -> dict:
     _check(_lib().hsfm_kmer_search_device(index._handle(), k, C.c_void_p(d_keys), n, C.c_void_p(d_out),
                                           C.byref(st)), "kmer_search_device")
     return {"kernel_ms": st.kernel_ms}
+
+
+def reverse_complement(queries, ctx: "GpuContext | None" = None):
+    """Reverse complements of a query batch on the device, for searching the
+    other strand (no counterpart in the reference, which searches the given
+    strand only): returns (flat, offsets) in the layout of the input."""
+    ctx = ctx or context_cache.get_or_init()
+    flat, off = _to_csr(queries)
+    nq = len(off) - 1
+    out = np.zeros(max(int(off[-1]), 1), np.uint8)
+    if nq:
+        _check(_lib().hsfm_revcomp_batch(ctx._h, flat.ctypes.data, off.ctypes.data, nq, out.ctypes.data),
+               "reverse_complement")
+    return out[:int(off[-1])], off.copy()
+
+
+def reverse_complement_device(d_qflat: int, d_qoff: int, nq: int, d_out: int, ctx: "GpuContext | None" = None):
+    """reverse_complement on device-resident queries (d_out holds qoff[nq] bytes)."""
+    ctx = ctx or context_cache.get_or_init()
+    _check(_lib().hsfm_revcomp_device(ctx._h, C.c_void_p(d_qflat), C.c_void_p(d_qoff), nq, C.c_void_p(d_out)),
+           "reverse_complement_device")
 
 
 def lsi_gradient(params: np.ndarray, hidden: int, x: np.ndarray, y: np.ndarray, ctx: "GpuContext | None" = None):

@@ -539,3 +539,83 @@ extern "C" int hsfm_last_stats(hsfm_index *idx, hsfm_stats *out) {
     *out = idx->stats;
     return HSFM_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Reverse complement of a query batch (both-strand searching).  One thread per
+// output code: its query by binary search over the offsets, then the mirrored
+// input code through the IUPAC complement (the code whose base set is the
+// complemented base set: A<->T, C<->G, R<->Y, K<->M, B<->V, D<->H; N, S, W and
+// $ map to themselves; bytes >= 16 pass unchanged).
+// ---------------------------------------------------------------------------
+namespace {
+__constant__ uint8_t c_comp[16] = {0, 4, 3, 2, 1, 5, 7, 6, 8, 9, 11, 10, 15, 14, 13, 12};
+
+__global__ __launch_bounds__(256) void k_revcomp(const uint8_t *__restrict__ q, const uint64_t *__restrict__ qoff,
+                                                 uint32_t nq, uint64_t total, uint8_t *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    uint32_t lo = 0, hi = nq; // the query r with qoff[r] <= i < qoff[r + 1]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (qoff[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    const uint64_t b = qoff[lo], e = qoff[lo + 1];
+    const uint8_t c = q[b + (e - 1 - i)];
+    out[i] = c < 16 ? c_comp[c] : c;
+}
+} // namespace
+
+extern "C" int hsfm_revcomp_device(hsfm_device *dev, const uint8_t *d_qflat, const uint64_t *d_qoff, uint32_t nq,
+                                   uint8_t *d_out) {
+    if (!dev || (nq && (!d_qflat || !d_qoff || !d_out)))
+        return set_error(HSFM_EINVAL, "hsfm_revcomp_device: null argument");
+    if (nq == 0) return HSFM_OK;
+    HSFM_HIP(hipSetDevice(dev->ordinal));
+    hipStream_t s = dev->stream;
+    uint64_t ends[2] = {0, 0};
+    HSFM_HIP(hipMemcpyAsync(ends, d_qoff, 8, hipMemcpyDeviceToHost, s));
+    HSFM_HIP(hipMemcpyAsync(ends + 1, d_qoff + nq, 8, hipMemcpyDeviceToHost, s));
+    HSFM_HIP(hipStreamSynchronize(s));
+    if (ends[0] != 0 || ends[1] < ends[0]) return set_error(HSFM_EINVAL, "hsfm_revcomp_device: bad offsets");
+    const uint64_t total = ends[1];
+    if (total) {
+        hipLaunchKernelGGL(k_revcomp, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, s, d_qflat, d_qoff, nq,
+                           total, d_out);
+        HSFM_HIP(hipGetLastError());
+    }
+    HSFM_HIP(hipStreamSynchronize(s));
+    return HSFM_OK;
+}
+
+extern "C" int hsfm_revcomp_batch(hsfm_device *dev, const uint8_t *qflat, const uint64_t *qoff, uint32_t nq,
+                                  uint8_t *out) {
+    if (!dev || !qoff || (nq && (!qflat || !out))) return set_error(HSFM_EINVAL, "hsfm_revcomp_batch: null argument");
+    if (nq == 0) return HSFM_OK;
+    if (qoff[0] != 0) return set_error(HSFM_EINVAL, "hsfm_revcomp_batch: offsets must start at 0");
+    for (uint32_t r = 0; r < nq; r++)
+        if (qoff[r + 1] < qoff[r]) return set_error(HSFM_EINVAL, "hsfm_revcomp_batch: offsets must not decrease");
+    const uint64_t total = qoff[nq];
+    HSFM_HIP(hipSetDevice(dev->ordinal));
+    hipStream_t s = dev->stream;
+    uint8_t *dq = nullptr, *dout = nullptr;
+    uint64_t *doff = nullptr;
+    int rc = HSFM_OK;
+    if (hipMalloc((void **)&dq, total + 1) != hipSuccess || hipMalloc((void **)&dout, total + 1) != hipSuccess ||
+        hipMalloc((void **)&doff, ((size_t)nq + 1) * 8) != hipSuccess) {
+        rc = set_error(HSFM_ENOMEM, "hsfm_revcomp_batch: device allocation failed");
+    } else if (hipMemcpyAsync(dq, qflat, total, hipMemcpyHostToDevice, s) != hipSuccess ||
+               hipMemcpyAsync(doff, qoff, ((size_t)nq + 1) * 8, hipMemcpyHostToDevice, s) != hipSuccess) {
+        rc = set_error(HSFM_EDEVICE, "hsfm_revcomp_batch: copy failed");
+    } else if ((rc = hsfm_revcomp_device(dev, dq, doff, nq, dout)) == HSFM_OK) {
+        if (hipMemcpyAsync(out, dout, total, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = set_error(HSFM_EDEVICE, "hsfm_revcomp_batch: copy back failed");
+    }
+    (void)hipStreamSynchronize(s);
+    if (dq) (void)hipFree(dq);
+    if (dout) (void)hipFree(dout);
+    if (doff) (void)hipFree(doff);
+    return rc;
+}
+

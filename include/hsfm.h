@@ -204,6 +204,20 @@ int hsfm_device_free(hsfm_device *dev, void *d_ptr);
 int hsfm_host_to_device(hsfm_device *dev, void *d_dst, const void *src, uint64_t bytes);
 int hsfm_device_to_host(hsfm_device *dev, void *dst, const void *d_src, uint64_t bytes);
 
+/* ---- both strands ------------------------------------------------------- */
+/* Reverse complement of every query, for searching the other strand: query i
+ * of the output is query i read backwards with each code complemented (A<->T,
+ * C<->G, R<->Y, K<->M, B<->V, D<->H; N, S, W, $ and bytes >= 16 unchanged); the
+ * offsets are the input's.  No counterpart in the reference, which searches
+ * the given strand only (src/alphabet.rs:149-168 fixes the base sets the
+ * complement maps); both-strand callers run the forward and the output batch
+ * through the same entry points.  d_* are device pointers, out_flat holds
+ * qoff[nq] bytes. */
+int hsfm_revcomp_device(hsfm_device *dev, const uint8_t *d_qflat, const uint64_t *d_qoff, uint32_t nq,
+                        uint8_t *d_out);
+/* Host-array variant of hsfm_revcomp_device. */
+int hsfm_revcomp_batch(hsfm_device *dev, const uint8_t *qflat, const uint64_t *qoff, uint32_t nq, uint8_t *out);
+
 /* ---- misc ---------------------------------------------------------------- */
 void hsfm_free_result(void *result);
 const char *hsfm_last_error(void);
