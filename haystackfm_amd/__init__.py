@@ -194,6 +194,11 @@ class _LocRes(C.Structure):
                 ("hits", C.POINTER(C.c_uint32))]
 
 
+class _PosRes(C.Structure):
+    _fields_ = [("num_queries", C.c_uint32), ("offsets", C.POINTER(C.c_uint64)),
+                ("positions", C.POINTER(C.c_uint32))]
+
+
 class _LocDev(C.Structure):
     _fields_ = [("num_queries", C.c_uint32), ("total_hits", C.c_uint64),
                 ("d_offsets", C.c_void_p), ("d_hits", C.c_void_p)]
@@ -284,6 +289,7 @@ def _lib():
         L.hsfm_index_free.argtypes = [vp]
         L.hsfm_locate_batch.argtypes = [vp, vp, vp, C.c_uint32, C.POINTER(C.POINTER(_LocRes))]
         L.hsfm_count_batch.argtypes = [vp, vp, vp, C.c_uint32, vp]
+        L.hsfm_locate_positions_batch.argtypes = [vp, vp, vp, C.c_uint32, C.POINTER(C.POINTER(_PosRes))]
         L.hsfm_locate_device.argtypes = [vp, vp, vp, C.c_uint32, C.POINTER(_LocDev)]
         L.hsfm_find_mems_batch.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, C.c_int, vp, C.c_uint32,
                                            C.c_uint32, C.POINTER(C.POINTER(_MemRes))]
@@ -479,6 +485,34 @@ class FmIndex:
         finally:
             _lib().hsfm_free_result(res)
         return offs, hits
+
+    def locate_positions_batch(self, queries, ctx: GpuContext | None = None) -> tuple[np.ndarray, np.ndarray]:
+        """CSR result: offsets[nq+1], positions[total] = absolute text offsets
+        (FmIndex::locate_positions, query.rs:46-55), ascending SA-row order."""
+        flat, off = _to_csr(queries)
+        nq = len(off) - 1
+        h = self._handle(ctx)
+        res = C.POINTER(_PosRes)()
+        _check(_lib().hsfm_locate_positions_batch(h, flat.ctypes.data, off.ctypes.data, nq, C.byref(res)),
+               "locate_positions_batch")
+        try:
+            r = res.contents
+            offs = np.ctypeslib.as_array(r.offsets, shape=(nq + 1,)).copy()
+            tot = int(offs[-1])
+            pos = np.ctypeslib.as_array(r.positions, shape=(tot,)).copy() if tot else np.zeros(0, np.uint32)
+        finally:
+            _lib().hsfm_free_result(res)
+        return offs, pos
+
+    def locate_positions_many(self, patterns) -> list[np.ndarray]:
+        """FmIndex::locate_positions_many (query.rs:71-208) on the GPU: absolute
+        text offsets per pattern."""
+        offs, pos = self.locate_positions_batch(patterns)
+        return [pos[int(offs[i]):int(offs[i + 1])] for i in range(len(offs) - 1)]
+
+    def locate_positions(self, pattern) -> np.ndarray:
+        """FmIndex::locate_positions (query.rs:46-55) of one pattern."""
+        return self.locate_positions_many([pattern])[0]
 
     def locate_gpu(self, queries) -> list[list[tuple[str, int]]]:
         """FmIndex::locate_gpu (query.rs:367-384): (header, pos) per hit."""

@@ -21,6 +21,8 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <vector>
+
 #include "hsfm_host.h"
 
 namespace hsfm {
@@ -1437,6 +1439,49 @@ extern "C" int hsfm_locate_batch(hsfm_index *idx, const uint8_t *qflat, const ui
     if (e != hipSuccess) {
         free(blk);
         return set_error(HSFM_EDEVICE, "copy of locate results failed: %s", hipGetErrorString(e));
+    }
+    *out = res;
+    return HSFM_OK;
+}
+
+extern "C" int hsfm_locate_positions_batch(hsfm_index *idx, const uint8_t *qflat, const uint64_t *qoff, uint32_t nq,
+                                           hsfm_positions_result **out) {
+    if (!idx || !out || !qoff || (nq && !qflat))
+        return set_error(HSFM_EINVAL, "hsfm_locate_positions_batch: null argument");
+    std::lock_guard<std::mutex> g(idx->mu);
+    HSFM_HIP(hipSetDevice(idx->dev->ordinal));
+    int rc = stage_queries(idx, qflat, qoff, nq);
+    if (rc) return rc;
+    hsfm_locate_device_result dr;
+    if ((rc = locate_device_impl(idx, idx->ws.qcopy.as<uint8_t>(), idx->ws.qoffcopy.as<uint64_t>(), nq, &dr))) return rc;
+    const size_t hdr = (sizeof(hsfm_positions_result) + 15) & ~(size_t)15;
+    const size_t bytes = hdr + ((size_t)nq + 1) * 8 + dr.total_hits * 8;
+    char *blk = (char *)malloc(bytes);
+    if (!blk) return set_error(HSFM_ENOMEM, "host allocation of %zu bytes failed", bytes);
+    hsfm_positions_result *res = (hsfm_positions_result *)blk;
+    res->num_queries = nq;
+    res->offsets = (uint64_t *)(blk + hdr);
+    // (seq_idx, offset) pairs land behind the offsets, then fold in place into
+    // absolute positions: the start of sequence s is the end of sequence s - 1
+    // (seq_boundaries are cumulative exclusive ends, sentinels included)
+    uint32_t *pairs = (uint32_t *)(blk + hdr + ((size_t)nq + 1) * 8);
+    res->positions = pairs;
+    std::vector<uint32_t> ends(idx->nseq);
+    hipStream_t s = idx->dev->stream;
+    hipError_t e = hipMemcpyAsync(res->offsets, dr.d_offsets, ((size_t)nq + 1) * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && dr.total_hits)
+        e = hipMemcpyAsync(pairs, dr.d_hits, dr.total_hits * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && idx->nseq)
+        e = hipMemcpyAsync(ends.data(), idx->seq_bounds.p, (size_t)idx->nseq * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        free(blk);
+        return set_error(HSFM_EDEVICE, "copy of locate results failed: %s", hipGetErrorString(e));
+    }
+    for (uint64_t t = 0; t < dr.total_hits; t++) {
+        const uint32_t sq = pairs[2 * t], off = pairs[2 * t + 1];
+        const uint32_t start = sq == 0 || sq > idx->nseq ? 0u : ends[sq - 1];
+        pairs[t] = start + off; // t <= 2t: each write lands on a pair already read
     }
     *out = res;
     return HSFM_OK;

@@ -103,6 +103,18 @@ typedef struct {
  * order.  Free the result with hsfm_free_result. */
 int hsfm_locate_batch(hsfm_index *idx, const uint8_t *qflat, const uint64_t *qoff, uint32_t nq,
                       hsfm_locate_result **out);
+/* FmIndex::locate_positions / locate_positions_many (src/fm_index/query.rs:46-208)
+ * over a batch: per query, the absolute text offsets (into the concatenated
+ * text, sentinels included) of every occurrence, in ascending SA-row order
+ * (locate_positions' order; locate_positions_many leaves it open).  Free the
+ * result with hsfm_free_result. */
+typedef struct {
+    uint32_t num_queries;
+    uint64_t *offsets;   /* num_queries + 1 */
+    uint32_t *positions; /* offsets[num_queries] absolute text positions */
+} hsfm_positions_result;
+int hsfm_locate_positions_batch(hsfm_index *idx, const uint8_t *qflat, const uint64_t *qoff, uint32_t nq,
+                                hsfm_positions_result **out);
 /* FmIndex::count over a batch (src/fm_index/query.rs:9-14); counts[nq]. */
 int hsfm_count_batch(hsfm_index *idx, const uint8_t *qflat, const uint64_t *qoff, uint32_t nq,
                      uint32_t *counts);

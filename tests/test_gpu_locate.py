@@ -157,3 +157,26 @@ def test_second_register_pass_with_seed_records():
     flat, off = O.gen_reads(17, text, bounds, 20000, 60, sub=0.005, both_strands=True)
     _check_batch(o, g, (flat, off))
     assert g.last_stats()["fallback_reads"] > 0
+
+
+def test_locate_positions_match_the_text():
+    """locate_positions(_many) (query.rs:46-208): absolute offsets into the
+    concatenated text, the rows of locate_batch in the same order, each a
+    compatible occurrence (brute force over the text)."""
+    rng = np.random.default_rng(41)
+    seqs = [O.decode(random_text(rng, int(n), iupac_frac=0.02)) for n in (3000, 5000, 800)]
+    o = O.OracleIndex.build(seqs, sample_rate=4)
+    g = hf.FmIndex.from_bytes(o.to_bytes())
+    text, bounds = O.concatenate(seqs)
+    qs = [text[int(s):int(s) + int(m)].copy() for s, m in zip(rng.integers(0, len(text) - 30, 300),
+                                                             rng.integers(1, 25, 300))]
+    qs = [np.where(q == 0, 1, q).astype(np.uint8) for q in qs] + [np.zeros(0, np.uint8)]
+    offs, pos = g.locate_positions_batch(qs)
+    lo, hits = g.locate_batch(qs)
+    assert np.array_equal(offs, lo)
+    starts = np.concatenate([[0], np.asarray(bounds, np.int64)[:-1]])
+    assert np.array_equal(pos.astype(np.int64), starts[hits[:, 0].astype(np.int64)] + hits[:, 1])
+    for i, q in enumerate(qs[:120]):
+        assert sorted(pos[int(offs[i]):int(offs[i + 1])].tolist()) == sorted(O.naive_locate(text, q)), i
+    assert [p.tolist() for p in g.locate_positions_many(qs[:5])] == [
+        pos[int(offs[i]):int(offs[i + 1])].tolist() for i in range(5)]
