@@ -159,7 +159,6 @@ __device__ __forceinline__ uint32_t locate_read(const DevDir &d, const DevConst 
                                                 const uint32_t *s_clane, const uint4 *seed_rec, uint32_t F,
                                                 const uint8_t *s_expand) {
     uint32_t probes = 0;
-    const uint32_t L = cst->num_lanes;
     const uint64_t b0 = qoff[r];
     const uint32_t m = (uint32_t)(qoff[r + 1] - b0);
     const uint8_t *p = q + b0;
@@ -232,13 +231,9 @@ __device__ __forceinline__ uint32_t locate_read(const DevDir &d, const DevConst 
 #pragma unroll
         for (uint32_t j = 0; j < KR; j++) {
             if (j < ns) {
-                RecView rl, rh;
-                load_rec(d, lo[j], rl);
-                load_rec(d, hi[j], rh);
                 probes += 1u + ((lo[j] >> kRecShift) != (hi[j] >> kRecShift) ? 1u : 0u);
                 uint32_t cl[16], ch[16];
-                lane_counts(rl, lo[j], L, cl);
-                lane_counts(rh, hi[j], L, ch);
+                pair_counts(d, lo[j], hi[j], cl, ch); // one superblock row for both ends
 #pragma unroll
                 for (int k = 0; k < 16; k++) {
                     if (((cm >> k) & 1u) && ch[k] > cl[k]) {
@@ -1025,7 +1020,6 @@ __global__ __launch_bounds__(64) void k_locate_wave(DevDir d, const DevConst *__
     load_const_lds(cst, s_compat, s_digit, s_clane);
     __syncthreads();
     const uint32_t lane = threadIdx.x;
-    const uint32_t L = cst->num_lanes;
     const bool textmode = d.isa != nullptr && d.text != nullptr && d.sa_full != nullptr && cap >= 64;
     uint2 *bufA = sets + (size_t)blockIdx.x * 2 * cap;
     uint2 *bufB = bufA + cap;
@@ -1077,11 +1071,7 @@ __global__ __launch_bounds__(64) void k_locate_wave(DevDir d, const DevConst *__
                 uint32_t cl[16], ch[16], cc = 0;
                 if (j < ns) {
                     const uint2 iv = A[j];
-                    RecView rl, rh;
-                    load_rec(d, iv.x, rl);
-                    load_rec(d, iv.y, rh);
-                    lane_counts(rl, iv.x, L, cl);
-                    lane_counts(rh, iv.y, L, ch);
+                    pair_counts(d, iv.x, iv.y, cl, ch);
 #pragma unroll
                     for (int k = 0; k < 16; k++) cc += (((cm >> k) & 1u) && ch[k] > cl[k]) ? 1u : 0u;
                 }
