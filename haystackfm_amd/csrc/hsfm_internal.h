@@ -176,8 +176,24 @@ __device__ __forceinline__ void pair_issue(const DevDir &d, uint32_t xlo, uint32
     const uint4 *rph = d.occ + ((size_t)bh << 2);
     const uint4 *spl = reinterpret_cast<const uint4 *>(d.sb + ((size_t)(bl >> kSbRecShift) << 4));
     w.l0 = rpl[0]; w.l1 = rpl[1]; w.l2 = rpl[2]; w.l3 = rpl[3];
-    w.h0 = rph[0]; w.h1 = rph[1]; w.h2 = rph[2]; w.h3 = rph[3];
     w.s0 = spl[0]; w.s1 = spl[1]; w.s2 = spl[2]; w.s3 = spl[3];
+    // the record of xhi only when it is another one (nearly every narrow
+    // interval has both ends in one record): lanes with one record make no
+    // request, and the consumers take the xlo record's words (pair_hi)
+    w.h0 = w.h1 = w.h2 = w.h3 = make_uint4(0u, 0u, 0u, 0u);
+    if (bh != bl) {
+        w.h0 = rph[0]; w.h1 = rph[1]; w.h2 = rph[2]; w.h3 = rph[3];
+    }
+}
+// The words of the record of xhi issued by pair_issue: its own, or those of
+// the record of xlo when both ends lie in one record.
+__device__ __forceinline__ void pair_hi(const PairRaw &w, uint32_t xlo, uint32_t xhi, uint4 &h0, uint4 &h1,
+                                        uint4 &h2, uint4 &h3) {
+    const bool same = (xlo >> kRecShift) == (xhi >> kRecShift);
+    h0 = same ? w.l0 : w.h0;
+    h1 = same ? w.l1 : w.h1;
+    h2 = same ? w.l2 : w.h2;
+    h3 = same ? w.l3 : w.h3;
 }
 // Counts of every lane in [0, xlo) and [0, xhi) (xlo <= xhi <= n) from the two
 // records and the superblock row of xlo only; the rare pair that spans two
@@ -185,13 +201,15 @@ __device__ __forceinline__ void pair_issue(const DevDir &d, uint32_t xlo, uint32
 // because no position holds their code.
 __device__ __forceinline__ void pair_eval(const DevDir &d, const PairRaw &w, uint32_t xlo, uint32_t xhi,
                                           uint32_t (&cl)[16], uint32_t (&ch)[16]) {
+    uint4 h0, h1, h2, h3;
+    pair_hi(w, xlo, xhi, h0, h1, h2, h3);
     const uint32_t bl = xlo >> kRecShift, bh = xhi >> kRecShift;
     const uint64_t lp0 = (uint64_t)w.l0.x | ((uint64_t)w.l0.y << 32), lp1 = (uint64_t)w.l0.z | ((uint64_t)w.l0.w << 32);
     const uint64_t lp2 = (uint64_t)w.l1.x | ((uint64_t)w.l1.y << 32), lp3 = (uint64_t)w.l1.z | ((uint64_t)w.l1.w << 32);
-    const uint64_t hp0 = (uint64_t)w.h0.x | ((uint64_t)w.h0.y << 32), hp1 = (uint64_t)w.h0.z | ((uint64_t)w.h0.w << 32);
-    const uint64_t hp2 = (uint64_t)w.h1.x | ((uint64_t)w.h1.y << 32), hp3 = (uint64_t)w.h1.z | ((uint64_t)w.h1.w << 32);
+    const uint64_t hp0 = (uint64_t)h0.x | ((uint64_t)h0.y << 32), hp1 = (uint64_t)h0.z | ((uint64_t)h0.w << 32);
+    const uint64_t hp2 = (uint64_t)h1.x | ((uint64_t)h1.y << 32), hp3 = (uint64_t)h1.z | ((uint64_t)h1.w << 32);
     const uint32_t lcw[8] = {w.l2.x, w.l2.y, w.l2.z, w.l2.w, w.l3.x, w.l3.y, w.l3.z, w.l3.w};
-    const uint32_t hcw[8] = {w.h2.x, w.h2.y, w.h2.z, w.h2.w, w.h3.x, w.h3.y, w.h3.z, w.h3.w};
+    const uint32_t hcw[8] = {h2.x, h2.y, h2.z, h2.w, h3.x, h3.y, h3.z, h3.w};
     const uint32_t sbw[16] = {w.s0.x, w.s0.y, w.s0.z, w.s0.w, w.s1.x, w.s1.y, w.s1.z, w.s1.w,
                               w.s2.x, w.s2.y, w.s2.z, w.s2.w, w.s3.x, w.s3.y, w.s3.z, w.s3.w};
     const uint64_t lowl = (1ull << (xlo & 63)) - 1, lowh = (1ull << (xhi & 63)) - 1;
@@ -225,6 +243,8 @@ __device__ __forceinline__ void pair_eval(const DevDir &d, const PairRaw &w, uin
 // that of xlo (reloaded, rare) between the two halves.
 template <typename V>
 __device__ __forceinline__ void pair_visit(const DevDir &d, PairRaw &w, uint32_t xlo, uint32_t xhi, V &&visit) {
+    uint4 h0, h1, h2, h3;
+    pair_hi(w, xlo, xhi, h0, h1, h2, h3);
     const uint32_t bl = xlo >> kRecShift, bh = xhi >> kRecShift;
     uint32_t cl[16];
     {
@@ -250,9 +270,9 @@ __device__ __forceinline__ void pair_visit(const DevDir &d, PairRaw &w, uint32_t
         const uint4 *sph = reinterpret_cast<const uint4 *>(d.sb + ((size_t)(bh >> kSbRecShift) << 4));
         w.s0 = sph[0]; w.s1 = sph[1]; w.s2 = sph[2]; w.s3 = sph[3];
     }
-    const uint64_t p0 = (uint64_t)w.h0.x | ((uint64_t)w.h0.y << 32), p1 = (uint64_t)w.h0.z | ((uint64_t)w.h0.w << 32);
-    const uint64_t p2 = (uint64_t)w.h1.x | ((uint64_t)w.h1.y << 32), p3 = (uint64_t)w.h1.z | ((uint64_t)w.h1.w << 32);
-    const uint32_t cw[8] = {w.h2.x, w.h2.y, w.h2.z, w.h2.w, w.h3.x, w.h3.y, w.h3.z, w.h3.w};
+    const uint64_t p0 = (uint64_t)h0.x | ((uint64_t)h0.y << 32), p1 = (uint64_t)h0.z | ((uint64_t)h0.w << 32);
+    const uint64_t p2 = (uint64_t)h1.x | ((uint64_t)h1.y << 32), p3 = (uint64_t)h1.z | ((uint64_t)h1.w << 32);
+    const uint32_t cw[8] = {h2.x, h2.y, h2.z, h2.w, h3.x, h3.y, h3.z, h3.w};
     const uint32_t sbw[16] = {w.s0.x, w.s0.y, w.s0.z, w.s0.w, w.s1.x, w.s1.y, w.s1.z, w.s1.w,
                               w.s2.x, w.s2.y, w.s2.z, w.s2.w, w.s3.x, w.s3.y, w.s3.z, w.s3.w};
     const uint64_t low = (1ull << (xhi & 63)) - 1;
@@ -274,6 +294,8 @@ __device__ __forceinline__ void pair_visit(const DevDir &d, PairRaw &w, uint32_t
 template <typename V>
 __device__ __forceinline__ void pair_visit_fused(const DevDir &d, PairRaw &w, uint32_t xlo, uint32_t xhi,
                                                  V &&visit) {
+    uint4 h0, h1, h2, h3;
+    pair_hi(w, xlo, xhi, h0, h1, h2, h3);
     const uint32_t bl = xlo >> kRecShift, bh = xhi >> kRecShift;
     if ((bl >> kSbRecShift) != (bh >> kSbRecShift)) {
         pair_visit(d, w, xlo, xhi, visit);
@@ -281,10 +303,10 @@ __device__ __forceinline__ void pair_visit_fused(const DevDir &d, PairRaw &w, ui
     }
     const uint64_t lp0 = (uint64_t)w.l0.x | ((uint64_t)w.l0.y << 32), lp1 = (uint64_t)w.l0.z | ((uint64_t)w.l0.w << 32);
     const uint64_t lp2 = (uint64_t)w.l1.x | ((uint64_t)w.l1.y << 32), lp3 = (uint64_t)w.l1.z | ((uint64_t)w.l1.w << 32);
-    const uint64_t hp0 = (uint64_t)w.h0.x | ((uint64_t)w.h0.y << 32), hp1 = (uint64_t)w.h0.z | ((uint64_t)w.h0.w << 32);
-    const uint64_t hp2 = (uint64_t)w.h1.x | ((uint64_t)w.h1.y << 32), hp3 = (uint64_t)w.h1.z | ((uint64_t)w.h1.w << 32);
+    const uint64_t hp0 = (uint64_t)h0.x | ((uint64_t)h0.y << 32), hp1 = (uint64_t)h0.z | ((uint64_t)h0.w << 32);
+    const uint64_t hp2 = (uint64_t)h1.x | ((uint64_t)h1.y << 32), hp3 = (uint64_t)h1.z | ((uint64_t)h1.w << 32);
     const uint32_t lcw[8] = {w.l2.x, w.l2.y, w.l2.z, w.l2.w, w.l3.x, w.l3.y, w.l3.z, w.l3.w};
-    const uint32_t hcw[8] = {w.h2.x, w.h2.y, w.h2.z, w.h2.w, w.h3.x, w.h3.y, w.h3.z, w.h3.w};
+    const uint32_t hcw[8] = {h2.x, h2.y, h2.z, h2.w, h3.x, h3.y, h3.z, h3.w};
     const uint32_t sbw[16] = {w.s0.x, w.s0.y, w.s0.z, w.s0.w, w.s1.x, w.s1.y, w.s1.z, w.s1.w,
                               w.s2.x, w.s2.y, w.s2.z, w.s2.w, w.s3.x, w.s3.y, w.s3.z, w.s3.w};
     const uint64_t lowl = (1ull << (xlo & 63)) - 1, lowh = (1ull << (xhi & 63)) - 1;

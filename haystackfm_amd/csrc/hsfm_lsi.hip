@@ -318,9 +318,11 @@ __global__ __launch_bounds__(256) void k_lsi_lookup(LsiModel md, const unsigned 
         lo = lo < 0 ? 0 : lo;
         hi = hi >= (long long)md.nkeys ? (long long)md.nkeys - 1 : hi;
         uint4 r = make_uint4(0, 0, 0, 0);
-        // lower bound of key in [lo, hi] (a binary search: its later probes
-        // stay in the lines of the earlier ones)
-        while (lo < hi) {
+        // lower bound of key in [lo, hi]: binary steps down to at most kFinal
+        // keys, then those keys as 16-byte pairs in one round of independent
+        // loads (fewer dependent rounds; pairs halve the requests of the round)
+        constexpr int kFinal = 32, kPairs = kFinal / 2 + 1; // keys left for the last round, their pair loads
+        while (hi - lo >= kFinal) {
             const long long mid = (lo + hi) >> 1;
             probes++;
             if (keys[mid] < key) lo = mid + 1;
@@ -328,8 +330,21 @@ __global__ __launch_bounds__(256) void k_lsi_lookup(LsiModel md, const unsigned 
         }
         if (lo <= hi) {
             probes++;
-            if (keys[lo] == key) {
-                const uint4 v = pay[lo];
+            const long long b = lo & ~1ll; // pairs from an even index (the key buffer is padded by one key)
+            const ulonglong2 *kp = reinterpret_cast<const ulonglong2 *>(keys + b);
+            uint32_t below = 0;
+            bool hit = false;
+#pragma unroll
+            for (int u = 0; u < kPairs; u++) {
+                const long long a0 = b + 2 * u;
+                ulonglong2 v = make_ulonglong2(~0ull, ~0ull);
+                if (a0 <= hi) v = kp[u];
+                const bool in0 = a0 >= lo && a0 <= hi, in1 = a0 + 1 >= lo && a0 + 1 <= hi;
+                below += (in0 && v.x < key ? 1u : 0u) + (in1 && v.y < key ? 1u : 0u);
+                hit |= (in0 && v.x == key) || (in1 && v.y == key);
+            }
+            if (hit) {
+                const uint4 v = pay[lo + below];
                 r = make_uint4(v.x, v.y, v.z, 1u);
             }
         }
