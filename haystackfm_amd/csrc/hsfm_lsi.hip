@@ -298,8 +298,9 @@ __global__ __launch_bounds__(256) void k_lsi_err(uint64_t nkeys, const unsigned 
 }
 
 // ---- lookup ----------------------------------------------------------------
-// out[4i..4i+4) = {fwd_lo, fwd_hi, rev_lo, found}; steps += key loads of the
-// bounded search (each depends on the previous one)
+// out[4i..4i+4) = {fwd_lo, fwd_hi, rev_lo, found}; steps += the bounded
+// search's loads in 64-byte units (one per binary step, the bytes of the
+// final round of paired loads)
 __global__ __launch_bounds__(256) void k_lsi_lookup(LsiModel md, const unsigned long long *__restrict__ keys,
                                                     const uint4 *__restrict__ pay, const LsiLeaf *__restrict__ lf,
                                                     const unsigned long long *__restrict__ q, uint64_t nq,
@@ -329,8 +330,8 @@ __global__ __launch_bounds__(256) void k_lsi_lookup(LsiModel md, const unsigned 
             else hi = mid;
         }
         if (lo <= hi) {
-            probes++;
             const long long b = lo & ~1ll; // pairs from an even index (the key buffer is padded by one key)
+            probes += (uint32_t)((((hi - b) / 2 + 1) * 16 + 63) / 64); // the round's bytes in 64-byte units
             const ulonglong2 *kp = reinterpret_cast<const ulonglong2 *>(keys + b);
             uint32_t below = 0;
             bool hit = false;
