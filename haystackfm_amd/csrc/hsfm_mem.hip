@@ -2488,10 +2488,9 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
         if (late || pos_ovf || nhits > b.hit_cap) {
             // outgrew the in-graph capacities: this batch through the host-driven path
             stats.fallback_batches++;
-            lk.unlock();
-            const int r2 = hsfm_find_mems_device(idx, d_q, d_qoff + pd.r0, pd.nb, min_len, mode, d_refb, n_refs,
-                                                 max_hits, &res);
-            lk.lock();
+            // (with the index lock still held: its result lives in the index's
+            // workspace until the callback below has read it)
+            const int r2 = mem_device_impl(idx, d_q, d_qoff + pd.r0, pd.nb, min_len, d_refb, n_refs, max_hits, &res);
             if (r2) return r2;
         } else {
             res.num_queries = pd.nb;
