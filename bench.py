@@ -343,7 +343,9 @@ def learned_seed_leg(bidir, flat, off, args, leaves=None, keys=None):
     lsi.seed_device(dq.data_ptr(), len(keys), d_lsi.data_ptr())
     torch.cuda.synchronize()
     plain_ms, lsi_ms, probes = [], [], 0
-    for _ in range(args.steps):
+    # each call is about a millisecond: alternate the two kernels over 20 calls
+    # each (or the step count, if larger) so that the ratio is not one sample's
+    for _ in range(max(args.steps, 20)):
         plain_ms.append(hf.kmer_search_device(bidir, k, dq.data_ptr(), len(keys), d_plain.data_ptr())["kernel_ms"])
         st = lsi.seed_device(dq.data_ptr(), len(keys), d_lsi.data_ptr())
         lsi_ms.append(st["kernel_ms"])
