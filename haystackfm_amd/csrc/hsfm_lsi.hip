@@ -16,8 +16,10 @@
 //      least-squares line slot ~ s0 + a * (key - kmin) + b (float64 sums,
 //      FP32 parameters) and the maximum error of its keys
 // Lookup: route, predict, then a binary search of the sorted keys restricted
-// to [pred - err, pred + err]; exact because err bounds every present key and
-// an absent key is absent from every window.
+// to [pred - err, pred + err] (through fences of every 512th and every 32nd
+// key, then the last 33 keys, each level one round of independent loads);
+// exact because err bounds every present key and an absent key is absent from
+// every window.
 #include <math.h>
 #include <string.h>
 
@@ -28,6 +30,8 @@
 namespace hsfm {
 
 constexpr uint32_t kLsiMaxHidden = 64;
+constexpr int kLsiFences = 2;
+constexpr int kLsiFenceShift[kLsiFences] = {9, 5}; // fences: every 512th and every 32nd key
 
 struct LsiLeaf {             // 32 bytes
     unsigned long long kmin; // smallest key routed here
@@ -298,10 +302,57 @@ __global__ __launch_bounds__(256) void k_lsi_err(uint64_t nkeys, const unsigned 
 }
 
 // ---- lookup ----------------------------------------------------------------
+// fence[j] = keys[j << sh]; entries past the last key hold the largest key value
+__global__ void k_lsi_fence(const unsigned long long *__restrict__ keys, uint64_t nkeys, int sh,
+                            unsigned long long *__restrict__ fence, uint64_t nf) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < nf) fence[j] = (j << sh) < nkeys ? keys[j << sh] : ~0ull;
+}
+
+// Last e in [a, z] with f[e] < key, or j if there is none (f ascending).  A
+// range of more than 17 entries is cut by rounds of 16 independent loads spread
+// evenly over it; the last 17 or fewer are read in one round of 16-byte pairs
+// from an even index (f holds one entry past z).  loads += 64-byte units.
+__device__ __forceinline__ long long lsi_last_below(const unsigned long long *__restrict__ f, long long a,
+                                                    long long z, long long j, unsigned long long key,
+                                                    uint32_t &loads) {
+    constexpr int kSpread = 16, kPairs = 9;
+    while (z - a + 1 > 2 * kPairs - 1) {
+        const long long n = z - a + 1;
+        uint32_t c = 0;
+#pragma unroll
+        for (int i = 0; i < kSpread; i++) c += f[a + ((i + 1) * n) / (kSpread + 1)] < key ? 1u : 0u;
+        loads += kSpread;
+        const long long pz = c < kSpread ? a + ((c + 1) * n) / (kSpread + 1) - 1 : z; // probes are ascending
+        if (c) j = a + ((long long)c * n) / (kSpread + 1), a = j + 1;
+        z = pz;
+    }
+    if (a <= z) {
+        const long long b = a & ~1ll;
+        const ulonglong2 *fp = reinterpret_cast<const ulonglong2 *>(f + b);
+        uint32_t c = 0;
+#pragma unroll
+        for (int u = 0; u < kPairs; u++) {
+            const long long e = b + 2 * u;
+            ulonglong2 v = make_ulonglong2(~0ull, ~0ull);
+            if (e <= z) v = fp[u];
+            c += (e >= a && e <= z && v.x < key ? 1u : 0u) + (e + 1 >= a && e + 1 <= z && v.y < key ? 1u : 0u);
+        }
+        loads += (uint32_t)((((z - b) / 2 + 1) * 16 + 63) / 64);
+        if (c) j = a + c - 1;
+    }
+    return j;
+}
+
 // out[4i..4i+4) = {fwd_lo, fwd_hi, rev_lo, found}; steps += the bounded
-// search's loads in 64-byte units (one per binary step, the bytes of the
-// final round of paired loads)
+// search's loads in 64-byte units (one per spread probe, the bytes of each
+// round of paired loads).  One key per lane (a quad of lanes per key, loading
+// whole lines together, ran 3.5x slower: fewer keys in flight), two fence
+// levels (three, with rounds of at most nine loads for eight waves per SIMD,
+// ran 8% slower).
 __global__ __launch_bounds__(256) void k_lsi_lookup(LsiModel md, const unsigned long long *__restrict__ keys,
+                                                    const unsigned long long *__restrict__ fence0,
+                                                    const unsigned long long *__restrict__ fence1,
                                                     const uint4 *__restrict__ pay, const LsiLeaf *__restrict__ lf,
                                                     const unsigned long long *__restrict__ q, uint64_t nq,
                                                     uint4 *__restrict__ out, unsigned long long *__restrict__ steps) {
@@ -319,15 +370,22 @@ __global__ __launch_bounds__(256) void k_lsi_lookup(LsiModel md, const unsigned 
         lo = lo < 0 ? 0 : lo;
         hi = hi >= (long long)md.nkeys ? (long long)md.nkeys - 1 : hi;
         uint4 r = make_uint4(0, 0, 0, 0);
-        // lower bound of key in [lo, hi]: binary steps down to at most kFinal
-        // keys, then those keys as 16-byte pairs in one round of independent
-        // loads (fewer dependent rounds; pairs halve the requests of the round)
-        constexpr int kFinal = 32, kPairs = kFinal / 2 + 1; // keys left for the last round, their pair loads
-        while (hi - lo >= kFinal) {
-            const long long mid = (lo + hi) >> 1;
-            probes++;
-            if (keys[mid] < key) lo = mid + 1;
-            else hi = mid;
+        // lower bound of key in [lo, hi].  At each fence level (stride 2^sh
+        // keys) the last entry j in (lo >> sh, hi >> sh] below key, else
+        // lo >> sh, leaves the lower bound in [max(lo, j << sh), min(hi, (j + 1)
+        // << sh)] or just past hi: the 512-key level leaves at most 16 entries
+        // of the 32-key level, which leaves at most 33 keys, which are then
+        // read as 16-byte pairs in one round.
+        constexpr int kPairs = 17; // 33 keys from an even index: 17 pairs
+        if (lo <= hi) {
+#pragma unroll
+            for (int lv = 0; lv < kLsiFences; lv++) {
+                const int sh = kLsiFenceShift[lv];
+                const unsigned long long *fl = lv == 0 ? fence0 : fence1;
+                const long long j = lsi_last_below(fl, (lo >> sh) + 1, hi >> sh, lo >> sh, key, probes);
+                lo = lo > (j << sh) ? lo : (j << sh);
+                hi = hi < ((j + 1) << sh) ? hi : ((j + 1) << sh);
+            }
         }
         if (lo <= hi) {
             const long long b = lo & ~1ll; // pairs from an even index (the key buffer is padded by one key)
@@ -396,6 +454,7 @@ struct hsfm_lsi {
     hsfm_lsi_config cfg{};
     hsfm_lsi_info info{};
     DevBuf keys, pay, leaves, params;
+    DevBuf fence[kLsiFences]; // keys[j << kLsiFenceShift[lv]] for every j (lookup levels)
 };
 
 namespace {
@@ -473,6 +532,15 @@ int lsi_enumerate(hsfm_lsi *x) {
     x->keys = pk;
     x->pay = pi;
     x->info.num_keys = P;
+    for (int lv = 0; lv < kLsiFences; lv++) { // padded past the last key (pair loads read one entry ahead)
+        const int sh = kLsiFenceShift[lv];
+        DevBuf &f = x->fence[lv];
+        const uint64_t nf = (P >> sh) + 2;
+        if ((rc = f.ensure(nf * 8))) return rc;
+        hipLaunchKernelGGL(k_lsi_fence, dim3((uint32_t)((nf + 255) / 256)), dim3(256), 0, s,
+                           x->keys.as<unsigned long long>(), P, sh, f.as<unsigned long long>(), nf);
+        HSFM_HIP(hipGetLastError());
+    }
     return HSFM_OK;
 }
 
@@ -601,7 +669,7 @@ extern "C" int hsfm_lsi_build(hsfm_index *idx, const hsfm_lsi_config *cfg, hsfm_
     (void)hipEventDestroy(e1);
     if (!rc) rc = lsi_train_and_fit(x);
     if (rc) {
-        x->keys.release(); x->pay.release(); x->leaves.release(); x->params.release();
+        x->keys.release(); x->pay.release(); x->leaves.release(); x->params.release(); for (DevBuf &f : x->fence) f.release();
         delete x;
         return rc;
     }
@@ -617,7 +685,7 @@ extern "C" int hsfm_lsi_get_info(const hsfm_lsi *x, hsfm_lsi_info *out) {
 
 extern "C" int hsfm_lsi_free(hsfm_lsi *x) {
     if (!x) return HSFM_OK;
-    x->keys.release(); x->pay.release(); x->leaves.release(); x->params.release();
+    x->keys.release(); x->pay.release(); x->leaves.release(); x->params.release(); for (DevBuf &f : x->fence) f.release();
     delete x;
     return HSFM_OK;
 }
@@ -636,8 +704,8 @@ extern "C" int hsfm_lsi_seed_device(hsfm_lsi *x, const uint64_t *d_keys, uint64_
     if ((rc = event_record(idx, 0, false))) return rc;
     if (n)
         hipLaunchKernelGGL(k_lsi_lookup, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, md,
-                           x->keys.as<unsigned long long>(), x->pay.as<uint4>(), x->leaves.as<LsiLeaf>(),
-                           reinterpret_cast<const unsigned long long *>(d_keys), n, reinterpret_cast<uint4 *>(d_out),
+                           x->keys.as<unsigned long long>(), x->fence[0].as<unsigned long long>(),
+                           x->fence[1].as<unsigned long long>(), x->pay.as<uint4>(), x->leaves.as<LsiLeaf>(), reinterpret_cast<const unsigned long long *>(d_keys), n, reinterpret_cast<uint4 *>(d_out),
                            idx->ws.counters.as<unsigned long long>());
     HSFM_HIP(hipGetLastError());
     if ((rc = event_record(idx, 0, true))) return rc;
