@@ -17,7 +17,8 @@
 //      FP32 parameters) and the maximum error of its keys
 // Lookup: route, predict, then a binary search of the sorted keys restricted
 // to [pred - err, pred + err] (through fences of every 512th and every 32nd
-// key, then the last 33 keys, each level one round of independent loads);
+// key, above them one of every 2^sh-th key when the widest window needs it,
+// then the last 33 keys, each level one round of independent loads);
 // exact because err bounds every present key and an absent key is absent from
 // every window.
 #include <math.h>
@@ -30,8 +31,8 @@
 namespace hsfm {
 
 constexpr uint32_t kLsiMaxHidden = 64;
-constexpr int kLsiFences = 2;
-constexpr int kLsiFenceShift[kLsiFences] = {9, 5}; // fences: every 512th and every 32nd key
+constexpr int kLsiFences = 3; // fence 0: every 2^top-th key (top = 0: none); 1, 2: every 512th, 32nd key
+constexpr int kLsiFenceShift[kLsiFences] = {0, 9, 5};
 
 struct LsiLeaf {             // 32 bytes
     unsigned long long kmin; // smallest key routed here
@@ -45,6 +46,7 @@ struct LsiModel {
     const float *p; // stage 1: w1[H], b1[H], w2[H], b2
     uint32_t hidden, leaves, k;
     uint64_t nkeys;
+    uint32_t top; // shift of the top fence level, 0 if there is none
 };
 
 // key / 4^k in [0, 1) in FP32 (the routing feature)
@@ -309,24 +311,13 @@ __global__ void k_lsi_fence(const unsigned long long *__restrict__ keys, uint64_
     if (j < nf) fence[j] = (j << sh) < nkeys ? keys[j << sh] : ~0ull;
 }
 
-// Last e in [a, z] with f[e] < key, or j if there is none (f ascending).  A
-// range of more than 17 entries is cut by rounds of 16 independent loads spread
-// evenly over it; the last 17 or fewer are read in one round of 16-byte pairs
-// from an even index (f holds one entry past z).  loads += 64-byte units.
+// Last e in [a, z] with f[e] < key, or j if there is none (f ascending), for
+// a range of at most 17 entries: one round of 16-byte pairs from an even index
+// (f holds one entry past z).  loads += 64-byte units.
 __device__ __forceinline__ long long lsi_last_below(const unsigned long long *__restrict__ f, long long a,
                                                     long long z, long long j, unsigned long long key,
                                                     uint32_t &loads) {
-    constexpr int kSpread = 16, kPairs = 9;
-    while (z - a + 1 > 2 * kPairs - 1) {
-        const long long n = z - a + 1;
-        uint32_t c = 0;
-#pragma unroll
-        for (int i = 0; i < kSpread; i++) c += f[a + ((i + 1) * n) / (kSpread + 1)] < key ? 1u : 0u;
-        loads += kSpread;
-        const long long pz = c < kSpread ? a + ((c + 1) * n) / (kSpread + 1) - 1 : z; // probes are ascending
-        if (c) j = a + ((long long)c * n) / (kSpread + 1), a = j + 1;
-        z = pz;
-    }
+    constexpr int kPairs = 9;
     if (a <= z) {
         const long long b = a & ~1ll;
         const ulonglong2 *fp = reinterpret_cast<const ulonglong2 *>(f + b);
@@ -345,14 +336,14 @@ __device__ __forceinline__ long long lsi_last_below(const unsigned long long *__
 }
 
 // out[4i..4i+4) = {fwd_lo, fwd_hi, rev_lo, found}; steps += the bounded
-// search's loads in 64-byte units (one per spread probe, the bytes of each
-// round of paired loads).  One key per lane (a quad of lanes per key, loading
-// whole lines together, ran 3.5x slower: fewer keys in flight), two fence
-// levels (three, with rounds of at most nine loads for eight waves per SIMD,
-// ran 8% slower).
+// search's loads in 64-byte units (the bytes of each round of paired loads).
+// One key per lane (a quad of lanes per key, loading whole lines together, ran
+// 3.5x slower: fewer keys in flight); a final round of 33 keys (16-key fences
+// with rounds of at most nine loads, for eight waves per SIMD, ran 8% slower).
 __global__ __launch_bounds__(256) void k_lsi_lookup(LsiModel md, const unsigned long long *__restrict__ keys,
                                                     const unsigned long long *__restrict__ fence0,
                                                     const unsigned long long *__restrict__ fence1,
+                                                    const unsigned long long *__restrict__ fence2,
                                                     const uint4 *__restrict__ pay, const LsiLeaf *__restrict__ lf,
                                                     const unsigned long long *__restrict__ q, uint64_t nq,
                                                     uint4 *__restrict__ out, unsigned long long *__restrict__ steps) {
@@ -373,15 +364,18 @@ __global__ __launch_bounds__(256) void k_lsi_lookup(LsiModel md, const unsigned 
         // lower bound of key in [lo, hi].  At each fence level (stride 2^sh
         // keys) the last entry j in (lo >> sh, hi >> sh] below key, else
         // lo >> sh, leaves the lower bound in [max(lo, j << sh), min(hi, (j + 1)
-        // << sh)] or just past hi: the 512-key level leaves at most 16 entries
-        // of the 32-key level, which leaves at most 33 keys, which are then
+        // << sh)] or just past hi.  A window of at most 2 max_err + 1 keys
+        // spans at most 17 entries of the top level (its stride is chosen so)
+        // or, without one, of the 512-key level; each level then leaves at
+        // most 17 entries of the next and the 32-key level at most 33 keys,
         // read as 16-byte pairs in one round.
         constexpr int kPairs = 17; // 33 keys from an even index: 17 pairs
         if (lo <= hi) {
 #pragma unroll
             for (int lv = 0; lv < kLsiFences; lv++) {
-                const int sh = kLsiFenceShift[lv];
-                const unsigned long long *fl = lv == 0 ? fence0 : fence1;
+                if (lv == 0 && !md.top) continue;
+                const int sh = lv == 0 ? (int)md.top : kLsiFenceShift[lv];
+                const unsigned long long *fl = lv == 0 ? fence0 : lv == 1 ? fence1 : fence2;
                 const long long j = lsi_last_below(fl, (lo >> sh) + 1, hi >> sh, lo >> sh, key, probes);
                 lo = lo > (j << sh) ? lo : (j << sh);
                 hi = hi < ((j + 1) << sh) ? hi : ((j + 1) << sh);
@@ -454,7 +448,8 @@ struct hsfm_lsi {
     hsfm_lsi_config cfg{};
     hsfm_lsi_info info{};
     DevBuf keys, pay, leaves, params;
-    DevBuf fence[kLsiFences]; // keys[j << kLsiFenceShift[lv]] for every j (lookup levels)
+    DevBuf fence[kLsiFences]; // keys[j << shift] for every j (lookup levels; see kLsiFenceShift)
+    uint32_t top = 0;         // shift of fence 0, 0 if it is not built
 };
 
 namespace {
@@ -532,15 +527,6 @@ int lsi_enumerate(hsfm_lsi *x) {
     x->keys = pk;
     x->pay = pi;
     x->info.num_keys = P;
-    for (int lv = 0; lv < kLsiFences; lv++) { // padded past the last key (pair loads read one entry ahead)
-        const int sh = kLsiFenceShift[lv];
-        DevBuf &f = x->fence[lv];
-        const uint64_t nf = (P >> sh) + 2;
-        if ((rc = f.ensure(nf * 8))) return rc;
-        hipLaunchKernelGGL(k_lsi_fence, dim3((uint32_t)((nf + 255) / 256)), dim3(256), 0, s,
-                           x->keys.as<unsigned long long>(), P, sh, f.as<unsigned long long>(), nf);
-        HSFM_HIP(hipGetLastError());
-    }
     return HSFM_OK;
 }
 
@@ -611,7 +597,7 @@ int lsi_train_and_fit(hsfm_lsi *x) {
         return rc;
     HSFM_HIP(hipMemsetAsync(mom.p, 0, (size_t)L * 32, s));
     hipLaunchKernelGGL(k_lsi_leaf_init, dim3((L + 255) / 256), dim3(256), 0, s, x->leaves.as<LsiLeaf>(), L);
-    LsiModel md{x->params.as<float>(), H, L, x->cfg.k, N};
+    LsiModel md{x->params.as<float>(), H, L, x->cfg.k, N, 0};
     const uint64_t chunks = (N + kLsiChunk - 1) / kLsiChunk;
     const dim3 cg((uint32_t)((chunks + 255) / 256));
     if (N) {
@@ -640,6 +626,25 @@ int lsi_train_and_fit(hsfm_lsi *x) {
     x->info.max_err = emax;
     x->info.mean_window = N ? wsum / (double)N : 0.0;
     leaf_of.release(); mom.release();
+    // fences: the top level only when a window of 2 max_err + 1 keys can span
+    // more than 17 entries of the 512-key level, with the smallest stride at
+    // which it spans at most 17 (the lookup reads each level in one round of
+    // at most 17 entries; slots are u32, so a stride of 2^33 always suffices)
+    x->top = 0;
+    if ((2ull * emax) >> kLsiFenceShift[1] > 16) {
+        x->top = kLsiFenceShift[1] + 1;
+        while (x->top < 40 && (2ull * emax) >> x->top > 16) x->top++;
+    }
+    for (int lv = 0; lv < kLsiFences; lv++) { // padded past the last key (pair loads read one entry ahead)
+        const int sh = lv == 0 ? (int)x->top : kLsiFenceShift[lv];
+        if (lv == 0 && !x->top) continue;
+        DevBuf &f = x->fence[lv];
+        const uint64_t nf = (N >> sh) + 2;
+        if ((rc = f.ensure(nf * 8))) return rc;
+        hipLaunchKernelGGL(k_lsi_fence, dim3((uint32_t)((nf + 255) / 256)), dim3(256), 0, s,
+                           x->keys.as<unsigned long long>(), N, sh, f.as<unsigned long long>(), nf);
+        HSFM_HIP(hipGetLastError());
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return HSFM_OK;
@@ -700,12 +705,13 @@ extern "C" int hsfm_lsi_seed_device(hsfm_lsi *x, const uint64_t *d_keys, uint64_
     int rc;
     if ((rc = idx->ws.counters.ensure(64))) return rc;
     HSFM_HIP(hipMemsetAsync(idx->ws.counters.p, 0, 8, s));
-    LsiModel md{x->params.as<float>(), x->cfg.hidden, x->cfg.leaves, x->cfg.k, x->info.num_keys};
+    LsiModel md{x->params.as<float>(), x->cfg.hidden, x->cfg.leaves, x->cfg.k, x->info.num_keys, x->top};
     if ((rc = event_record(idx, 0, false))) return rc;
     if (n)
         hipLaunchKernelGGL(k_lsi_lookup, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, md,
                            x->keys.as<unsigned long long>(), x->fence[0].as<unsigned long long>(),
-                           x->fence[1].as<unsigned long long>(), x->pay.as<uint4>(), x->leaves.as<LsiLeaf>(), reinterpret_cast<const unsigned long long *>(d_keys), n, reinterpret_cast<uint4 *>(d_out),
+                           x->fence[1].as<unsigned long long>(), x->fence[2].as<unsigned long long>(),
+                           x->pay.as<uint4>(), x->leaves.as<LsiLeaf>(), reinterpret_cast<const unsigned long long *>(d_keys), n, reinterpret_cast<uint4 *>(d_out),
                            idx->ws.counters.as<unsigned long long>());
     HSFM_HIP(hipGetLastError());
     if ((rc = event_record(idx, 0, true))) return rc;

@@ -60,6 +60,24 @@ def test_lsi_exact_on_larger_index():
     assert got[:, 3].sum() > len(q) // 3
 
 
+def test_lsi_exact_with_wide_windows():
+    """Untrained stage 1 and one leaf: error bounds of thousands of keys, so
+    the lookup adds its top fence level; still equal to plain exact search."""
+    leaves = 1
+    rng = np.random.default_rng(leaves)
+    text, o, g = _bidir(78, 200_000, repeat_frac=0.3, n_units=20, unit_min=300, unit_max=3000)
+    lsi = hf.LearnedSeedIndex(g, k=21, leaves=leaves, hidden=8, train_steps=0)
+    info = lsi.info()
+    assert (2 * info["max_err"]) >> 9 > 16  # wider than 17 entries of the 512-key fence
+    flat, _ = O.gen_reads(6, text, o.fwd.seq_boundaries, 2000, 150, sub=0.01, both_strands=True)
+    q = hf.kmer_keys(flat, 21)
+    q = q[q != np.uint64(0xFFFFFFFFFFFFFFFF)]
+    q = np.concatenate([q, _query_keys(rng, q, 21), np.array([0, 4 ** 21 - 1, q.min(), q.max()], np.uint64)])
+    got = lsi.seed(q)
+    assert np.array_equal(got, hf.kmer_search(g, q, 21))
+    assert got[:, 3].sum() > len(q) // 3
+
+
 def test_lsi_gradient_matches_oracle():
     rng = np.random.default_rng(11)
     H = 16
