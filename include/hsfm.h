@@ -277,7 +277,8 @@ typedef struct {
 } hsfm_lsi_info;
 typedef struct {
     double kernel_ms;     /* HIP events around the kernel                  */
-    uint64_t probes;      /* key loads of the bounded search (dependent)    */
+    uint64_t probes;      /* learned: 64 B units the bounded search loaded  *
+                           * (fence and key rounds); plain search: 0        */
 } hsfm_seed_stats;
 /* Build from idx itself: every present k-mer, its interval, both stages. */
 int hsfm_lsi_build(hsfm_index *idx, const hsfm_lsi_config *cfg, hsfm_lsi **out);

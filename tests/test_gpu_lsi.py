@@ -100,7 +100,7 @@ def test_learned_index_gains_over_the_methods_without_it():
     """What the trained model gains: training lowers the stage-1 loss below
     the untrained f(x) = x, and a lookup needs far fewer dependent key probes
     (bounded search in the predicted window) than a binary search of the whole
-    key array, or than the k exact left extensions of plain search."""
+    key array loads, or than the k exact left extensions of plain search."""
     rng = np.random.default_rng(2)
     text, o, g = _bidir(91, 300_000, repeat_frac=0.45, n_units=40, unit_min=300, unit_max=3000)
     base = hf.LearnedSeedIndex(g, k=21, leaves=4096, hidden=16, train_steps=0)
@@ -114,6 +114,6 @@ def test_learned_index_gains_over_the_methods_without_it():
     st = {}
     got = trained.seed(q, st)
     assert np.array_equal(got, hf.kmer_search(g, q, 21))
-    loads = st["probes"] / len(q)  # dependent key loads per lookup
+    loads = st["probes"] / len(q)  # 64-byte units loaded per lookup (fence and key rounds)
     assert loads < 0.5 * np.log2(ti["num_keys"])  # vs a binary search of all keys
     assert loads < 21 / 2                          # vs 21 dependent exact extensions
