@@ -1803,6 +1803,8 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     uint32_t cap = kWaveLdsCap;
     uint32_t pass = 0;
     uint64_t rec_used = 0;
+    uint64_t nhits = 0;
+    bool scanned = false; // mem_off holds the scan of the final counts
     // reads deferred by the persistent kernel: kDeferTB threads each; every
     // read on the workgroup path (no persistent kernel): one wave each
     const uint32_t lds_waves = use_thread ? persist_grid(idx, k_mem_wave<true, kDeferTB>, kDeferTB)
@@ -1853,10 +1855,23 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
                                ws.mem_cnt.as<uint32_t>(), ws.mem_flags.as<uint32_t>(), pass, retry, ctr, nullptr);
         HSFM_HIP(hipGetLastError());
         if (timed && (rc = event_record(idx, 0, true))) return rc;
-        if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
+        if (use_thread && (rc = event_record(idx, 2, true))) return rc;
+        // the scan of the hit counts runs behind the pass, so its total comes
+        // back with the pass counters in one round trip (redone after retries)
+        if ((rc = exclusive_scan_u32_u64(idx, ws.mem_cnt.as<uint32_t>(), ws.mem_off.as<uint64_t>(), (uint32_t)W)))
+            return rc;
+        if (!ws.h_pinned) HSFM_HIP(hipHostMalloc((void **)&ws.h_pinned, 4096, hipHostMallocDefault));
+        HSFM_HIP(hipMemcpyAsync(ws.h_pinned, ctr, 64, hipMemcpyDeviceToHost, s));
+        HSFM_HIP(hipMemcpyAsync(ws.h_pinned + 8, ws.mem_off.as<uint64_t>() + W, 8, hipMemcpyDeviceToHost, s));
+        HSFM_HIP(hipStreamSynchronize(s));
+        memcpy(h, ws.h_pinned, 64);
         const uint64_t nretry = h[2];
         rec_used = h[1];
-        if (nretry == 0) break;
+        if (nretry == 0) {
+            nhits = ws.h_pinned[8];
+            scanned = true;
+            break;
+        }
         idx->stats.capacity_retries++;
         if (cap >= (1u << 26)) return set_error(HSFM_ENOMEM, "interval set of a single read exceeds 2^26 intervals");
         // sets that outgrow the LDS are usually far larger: go straight to
@@ -1872,11 +1887,11 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         pass++;
     }
     if (!use_thread) idx->stats.fallback_reads = 0;
-    if (idx->stats.fallback_reads && (rc = event_record(idx, 2, true))) return rc;
-    if ((rc = exclusive_scan_u32_u64(idx, ws.mem_cnt.as<uint32_t>(), ws.mem_off.as<uint64_t>(), (uint32_t)W)))
-        return rc;
-    uint64_t nhits = 0;
-    if ((rc = readback_u64(idx, ws.mem_off.as<uint64_t>() + W, &nhits, 1))) return rc;
+    if (!scanned) {
+        if ((rc = exclusive_scan_u32_u64(idx, ws.mem_cnt.as<uint32_t>(), ws.mem_off.as<uint64_t>(), (uint32_t)W)))
+            return rc;
+        if ((rc = readback_u64(idx, ws.mem_off.as<uint64_t>() + W, &nhits, 1))) return rc;
+    }
     idx->stats.records = nhits;
     if ((rc = ws.mem_sorted.ensure(nhits * 16 + 16))) return rc;
     if (use_thread) {
