@@ -237,6 +237,24 @@ def measured_traffic(kernel, args, nq, nl):
     return k["bytes_per_launch"] if k else None
 
 
+def measured_cache(kernel, args, nq, nl):
+    """L2 hit rate and L1-to-L2 read requests per probe of `kernel` from the
+    committed rocprofv3 TCC_HIT / TCC_MISS / TCP_TCC_READ_REQ pass, when it was
+    taken on this workload (no counter for the Infinity Cache exists on gfx950)."""
+    path = os.path.join(ROOT, "profiles", "l2_smem_locate_100Mbp.json")
+    try:
+        with open(path) as f:
+            doc = json.load(f)
+    except (OSError, ValueError):
+        return None
+    w = doc.get("workload", {})
+    if w.get("reference_bp") != args.ref_bp or w.get("smem_reads") != nq or w.get("locate_reads") != nl:
+        return None
+    k = doc.get("kernels", {}).get(kernel)
+    return {"l2_hit_rate": k["l2_hit_rate"], "l2_read_requests_per_probe": k["read_req_per_probe"],
+            "source": "profiles/l2_hit_rate_100Mbp.md"} if k else None
+
+
 def measured_traffic_1g(kernel, args, keys):
     """HBM bytes per launch of `kernel` on the 1 Gbp learned-seed workload from
     the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes (copy-calibrated)."""
@@ -681,7 +699,8 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_mem_persist", args, nq, nl),
                 "kernel": "k_mem_persist", "kernel_ms": round(kms, 3),
                 "bytes_per_read": round(bytes_per_launch / nq, 1),
-                "measured_random_record_ceiling": random_record_ceiling(achieved)}
+                "measured_random_record_ceiling": random_record_ceiling(achieved),
+                "cache": measured_cache("k_mem_persist", args, nq, nl)}
     loc_roof = None
     lkms = float(np.mean(lk))
     if loc_stats.get("occ_probes") and lkms > 0:
@@ -690,7 +709,8 @@ def main():
                     "frac": round(a / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_locate_search", args, nq, nl),
                     "kernel": "k_locate_search", "kernel_ms": round(lkms, 3),
                     "bytes_per_read": round(loc_stats["occ_probes"] * 64.0 / nl, 1),
-                    "measured_random_record_ceiling": random_record_ceiling(a)}
+                    "measured_random_record_ceiling": random_record_ceiling(a),
+                    "cache": measured_cache("k_locate_search", args, nq, nl)}
     cpu_line = None
     if "smem" in cpu:
         cpu_line = cpu["smem"]
