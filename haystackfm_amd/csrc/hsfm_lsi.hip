@@ -22,6 +22,7 @@
 // exact because err bounds every present key and an absent key is absent from
 // every window.
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <hipcub/hipcub.hpp>
@@ -340,6 +341,60 @@ __device__ __forceinline__ long long lsi_last_below(const unsigned long long *__
 // One key per lane (a quad of lanes per key, loading whole lines together, ran
 // 3.5x slower: fewer keys in flight); a final round of 33 keys (16-key fences
 // with rounds of at most nine loads, for eight waves per SIMD, ran 8% slower).
+__device__ __forceinline__ uint4 lsi_lookup_key(const LsiModel &md, const unsigned long long *__restrict__ keys,
+                                                const unsigned long long *__restrict__ fence0,
+                                                const unsigned long long *__restrict__ fence1,
+                                                const unsigned long long *__restrict__ fence2,
+                                                const uint4 *__restrict__ pay, const LsiLeaf &f,
+                                                unsigned long long key, uint32_t &probes) {
+    const long long pred = lsi_pred(f, key);
+    long long lo = pred - (long long)f.err, hi = pred + (long long)f.err; // inclusive window
+    lo = lo < 0 ? 0 : lo;
+    hi = hi >= (long long)md.nkeys ? (long long)md.nkeys - 1 : hi;
+    uint4 r = make_uint4(0, 0, 0, 0);
+    // lower bound of key in [lo, hi].  At each fence level (stride 2^sh
+    // keys) the last entry j in (lo >> sh, hi >> sh] below key, else
+    // lo >> sh, leaves the lower bound in [max(lo, j << sh), min(hi, (j + 1)
+    // << sh)] or just past hi.  A window of at most 2 max_err + 1 keys
+    // spans at most 17 entries of the top level (its stride is chosen so)
+    // or, without one, of the 512-key level; each level then leaves at
+    // most 17 entries of the next and the 32-key level at most 33 keys,
+    // read as 16-byte pairs in one round.
+    constexpr int kPairs = 17; // 33 keys from an even index: 17 pairs
+    if (lo <= hi) {
+#pragma unroll
+        for (int lv = 0; lv < kLsiFences; lv++) {
+            if (lv == 0 && !md.top) continue;
+            const int sh = lv == 0 ? (int)md.top : kLsiFenceShift[lv];
+            const unsigned long long *fl = lv == 0 ? fence0 : lv == 1 ? fence1 : fence2;
+            const long long j = lsi_last_below(fl, (lo >> sh) + 1, hi >> sh, lo >> sh, key, probes);
+            lo = lo > (j << sh) ? lo : (j << sh);
+            hi = hi < ((j + 1) << sh) ? hi : ((j + 1) << sh);
+        }
+    }
+    if (lo <= hi) {
+        const long long b = lo & ~1ll; // pairs from an even index (the key buffer is padded by one key)
+        probes += (uint32_t)((((hi - b) / 2 + 1) * 16 + 63) / 64); // the round's bytes in 64-byte units
+        const ulonglong2 *kp = reinterpret_cast<const ulonglong2 *>(keys + b);
+        uint32_t below = 0;
+        bool hit = false;
+#pragma unroll
+        for (int u = 0; u < kPairs; u++) {
+            const long long a0 = b + 2 * u;
+            ulonglong2 v = make_ulonglong2(~0ull, ~0ull);
+            if (a0 <= hi) v = kp[u];
+            const bool in0 = a0 >= lo && a0 <= hi, in1 = a0 + 1 >= lo && a0 + 1 <= hi;
+            below += (in0 && v.x < key ? 1u : 0u) + (in1 && v.y < key ? 1u : 0u);
+            hit |= (in0 && v.x == key) || (in1 && v.y == key);
+        }
+        if (hit) {
+            const uint4 v = pay[lo + below];
+            r = make_uint4(v.x, v.y, v.z, 1u);
+        }
+    }
+    return r;
+}
+
 __global__ __launch_bounds__(256) void k_lsi_lookup(LsiModel md, const unsigned long long *__restrict__ keys,
                                                     const unsigned long long *__restrict__ fence0,
                                                     const unsigned long long *__restrict__ fence1,
@@ -356,52 +411,39 @@ __global__ __launch_bounds__(256) void k_lsi_lookup(LsiModel md, const unsigned 
     if (t < nq && md.nkeys) {
         const unsigned long long key = q[t];
         const LsiLeaf f = lf[lsi_leaf(lsi_stage1(s_p, md.hidden, lsi_feature(key, md.k)), md.leaves)];
-        const long long pred = lsi_pred(f, key);
-        long long lo = pred - (long long)f.err, hi = pred + (long long)f.err; // inclusive window
-        lo = lo < 0 ? 0 : lo;
-        hi = hi >= (long long)md.nkeys ? (long long)md.nkeys - 1 : hi;
-        uint4 r = make_uint4(0, 0, 0, 0);
-        // lower bound of key in [lo, hi].  At each fence level (stride 2^sh
-        // keys) the last entry j in (lo >> sh, hi >> sh] below key, else
-        // lo >> sh, leaves the lower bound in [max(lo, j << sh), min(hi, (j + 1)
-        // << sh)] or just past hi.  A window of at most 2 max_err + 1 keys
-        // spans at most 17 entries of the top level (its stride is chosen so)
-        // or, without one, of the 512-key level; each level then leaves at
-        // most 17 entries of the next and the 32-key level at most 33 keys,
-        // read as 16-byte pairs in one round.
-        constexpr int kPairs = 17; // 33 keys from an even index: 17 pairs
-        if (lo <= hi) {
+        out[t] = lsi_lookup_key(md, keys, fence0, fence1, fence2, pay, f, key, probes);
+    }
 #pragma unroll
-            for (int lv = 0; lv < kLsiFences; lv++) {
-                if (lv == 0 && !md.top) continue;
-                const int sh = lv == 0 ? (int)md.top : kLsiFenceShift[lv];
-                const unsigned long long *fl = lv == 0 ? fence0 : lv == 1 ? fence1 : fence2;
-                const long long j = lsi_last_below(fl, (lo >> sh) + 1, hi >> sh, lo >> sh, key, probes);
-                lo = lo > (j << sh) ? lo : (j << sh);
-                hi = hi < ((j + 1) << sh) ? hi : ((j + 1) << sh);
-            }
+    for (int d = 32; d >= 1; d >>= 1) probes += __shfl_xor(probes, d, 64);
+    if ((threadIdx.x & 63) == 0 && steps) atomicAdd(steps, (unsigned long long)probes);
+}
+
+// The same lookup with the leaves in LDS (at most kLsiLdsLeaves): one
+// 1024-thread workgroup per CU copies them once and strides over the keys, so
+// the leaf costs an LDS read instead of a global round in each key's chain.
+constexpr uint32_t kLsiLdsLeaves = 4096;
+__global__ __launch_bounds__(1024) void k_lsi_lookup_lds(LsiModel md, const unsigned long long *__restrict__ keys,
+                                                         const unsigned long long *__restrict__ fence0,
+                                                         const unsigned long long *__restrict__ fence1,
+                                                         const unsigned long long *__restrict__ fence2,
+                                                         const uint4 *__restrict__ pay, const LsiLeaf *__restrict__ lf,
+                                                         const unsigned long long *__restrict__ q, uint64_t nq,
+                                                         uint4 *__restrict__ out, unsigned long long *__restrict__ steps) {
+    __shared__ float s_p[3 * kLsiMaxHidden + 1];
+    __shared__ LsiLeaf s_lf[kLsiLdsLeaves];
+    for (uint32_t i = threadIdx.x; i < 3 * md.hidden + 1; i += blockDim.x) s_p[i] = md.p[i];
+    for (uint32_t i = threadIdx.x; i < md.leaves; i += blockDim.x) s_lf[i] = lf[i];
+    __syncthreads();
+    uint32_t probes = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nq; t += stride) {
+        if (!md.nkeys) {
+            out[t] = make_uint4(0, 0, 0, 0);
+            continue;
         }
-        if (lo <= hi) {
-            const long long b = lo & ~1ll; // pairs from an even index (the key buffer is padded by one key)
-            probes += (uint32_t)((((hi - b) / 2 + 1) * 16 + 63) / 64); // the round's bytes in 64-byte units
-            const ulonglong2 *kp = reinterpret_cast<const ulonglong2 *>(keys + b);
-            uint32_t below = 0;
-            bool hit = false;
-#pragma unroll
-            for (int u = 0; u < kPairs; u++) {
-                const long long a0 = b + 2 * u;
-                ulonglong2 v = make_ulonglong2(~0ull, ~0ull);
-                if (a0 <= hi) v = kp[u];
-                const bool in0 = a0 >= lo && a0 <= hi, in1 = a0 + 1 >= lo && a0 + 1 <= hi;
-                below += (in0 && v.x < key ? 1u : 0u) + (in1 && v.y < key ? 1u : 0u);
-                hit |= (in0 && v.x == key) || (in1 && v.y == key);
-            }
-            if (hit) {
-                const uint4 v = pay[lo + below];
-                r = make_uint4(v.x, v.y, v.z, 1u);
-            }
-        }
-        out[t] = r;
+        const unsigned long long key = q[t];
+        const LsiLeaf f = s_lf[lsi_leaf(lsi_stage1(s_p, md.hidden, lsi_feature(key, md.k)), md.leaves)];
+        out[t] = lsi_lookup_key(md, keys, fence0, fence1, fence2, pay, f, key, probes);
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) probes += __shfl_xor(probes, d, 64);
@@ -707,12 +749,30 @@ extern "C" int hsfm_lsi_seed_device(hsfm_lsi *x, const uint64_t *d_keys, uint64_
     HSFM_HIP(hipMemsetAsync(idx->ws.counters.p, 0, 8, s));
     LsiModel md{x->params.as<float>(), x->cfg.hidden, x->cfg.leaves, x->cfg.k, x->info.num_keys, x->top};
     if ((rc = event_record(idx, 0, false))) return rc;
-    if (n)
+    // leaves in LDS when they fit and no top fence level is needed (measured:
+    // 100 Mbp, 4096 leaves, 0.66 ms against 0.88 ms; 1 Gbp, whose windows need
+    // the top level, 1.16 ms against 1.12 ms).  HSFM_LSI_LOOKUP=global / lds
+    // forces either kernel (lds only where the leaves fit).
+    const char *path = getenv("HSFM_LSI_LOOKUP");
+    const bool fits = x->cfg.leaves <= kLsiLdsLeaves;
+    const bool lds = fits && (path && strcmp(path, "lds") == 0 ? true
+                              : path && strcmp(path, "global") == 0 ? false
+                              : x->top == 0);
+    const unsigned long long *dk = reinterpret_cast<const unsigned long long *>(d_keys);
+    if (n && lds) {
+        const uint64_t need = (n + 1023) / 1024, cus = (uint64_t)idx->dev->num_cus;
+        hipLaunchKernelGGL(k_lsi_lookup_lds, dim3((uint32_t)(need < cus ? need : cus)), dim3(1024), 0, s, md,
+                           x->keys.as<unsigned long long>(), x->fence[0].as<unsigned long long>(),
+                           x->fence[1].as<unsigned long long>(), x->fence[2].as<unsigned long long>(),
+                           x->pay.as<uint4>(), x->leaves.as<LsiLeaf>(), dk, n, reinterpret_cast<uint4 *>(d_out),
+                           idx->ws.counters.as<unsigned long long>());
+    } else if (n) {
         hipLaunchKernelGGL(k_lsi_lookup, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, md,
                            x->keys.as<unsigned long long>(), x->fence[0].as<unsigned long long>(),
                            x->fence[1].as<unsigned long long>(), x->fence[2].as<unsigned long long>(),
-                           x->pay.as<uint4>(), x->leaves.as<LsiLeaf>(), reinterpret_cast<const unsigned long long *>(d_keys), n, reinterpret_cast<uint4 *>(d_out),
+                           x->pay.as<uint4>(), x->leaves.as<LsiLeaf>(), dk, n, reinterpret_cast<uint4 *>(d_out),
                            idx->ws.counters.as<unsigned long long>());
+    }
     HSFM_HIP(hipGetLastError());
     if ((rc = event_record(idx, 0, true))) return rc;
     if (st) {

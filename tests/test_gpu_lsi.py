@@ -78,6 +78,22 @@ def test_lsi_exact_with_wide_windows():
     assert got[:, 3].sum() > len(q) // 3
 
 
+@pytest.mark.parametrize("path", ["global", "lds"])
+def test_lsi_both_lookup_kernels(path, monkeypatch):
+    """The lookup with its leaves in global memory and in LDS: equal to plain
+    exact search, with and without the top fence level."""
+    monkeypatch.setenv("HSFM_LSI_LOOKUP", path)
+    text, o, g = _bidir(79, 150_000, repeat_frac=0.3, n_units=20, unit_min=300, unit_max=3000)
+    flat, _ = O.gen_reads(7, text, o.fwd.seq_boundaries, 1500, 150, sub=0.01, both_strands=True)
+    q = hf.kmer_keys(flat, 21)
+    q = q[q != np.uint64(0xFFFFFFFFFFFFFFFF)]
+    q = np.concatenate([q, _query_keys(np.random.default_rng(4), q, 21)])
+    plain = hf.kmer_search(g, q, 21)
+    for leaves, steps in ((4096, 300), (1, 0)):  # narrow windows; wide ones with the top level
+        lsi = hf.LearnedSeedIndex(g, k=21, leaves=leaves, hidden=8, train_steps=steps)
+        assert np.array_equal(lsi.seed(q), plain)
+
+
 def test_lsi_gradient_matches_oracle():
     rng = np.random.default_rng(11)
     H = 16
