@@ -233,11 +233,11 @@ class _Stats(C.Structure):
 
 class _LsiConfig(C.Structure):
     _fields_ = [("k", C.c_uint32), ("leaves", C.c_uint32), ("hidden", C.c_uint32), ("train_steps", C.c_uint32),
-                ("sample", C.c_uint32), ("lr", C.c_float)]
+                ("sample", C.c_uint32), ("lr", C.c_float), ("lookup", C.c_uint32)]
 
 
 class _LsiInfo(C.Structure):
-    _fields_ = [("num_keys", C.c_uint64), ("max_err", C.c_uint32), ("pad", C.c_uint32),
+    _fields_ = [("num_keys", C.c_uint64), ("max_err", C.c_uint32), ("fence_levels", C.c_uint32),
                 ("mean_window", C.c_double), ("mse_initial", C.c_double), ("mse_final", C.c_double),
                 ("enum_ms", C.c_double), ("train_ms", C.c_double), ("fit_ms", C.c_double)]
 
@@ -747,11 +747,13 @@ class LearnedSeedIndex:
     seed(keys) returns, per key, (fwd_lo, fwd_hi, rev_lo, found): the interval
     plain exact backward search gives for the k-mer (bidir.rs:112-130)."""
 
+    LOOKUP = {"auto": 0, "global": 1, "lds": 2}
+
     def __init__(self, index, k: int = 21, leaves: int = 4096, hidden: int = 16, train_steps: int = 400,
-                 sample: int = 1 << 16, lr: float = 0.01):
+                 sample: int = 1 << 16, lr: float = 0.01, lookup: str = "auto"):
         self.index = index
         self.k = k
-        cfg = _LsiConfig(k, leaves, hidden, train_steps, sample, lr)
+        cfg = _LsiConfig(k, leaves, hidden, train_steps, sample, lr, self.LOOKUP[lookup])
         h = C.c_void_p()
         _check(_lib().hsfm_lsi_build(index._handle(), C.byref(cfg), C.byref(h)), "lsi_build")
         self._h = h
@@ -765,7 +767,7 @@ class LearnedSeedIndex:
     def info(self) -> dict:
         st = _LsiInfo()
         _check(_lib().hsfm_lsi_get_info(self._h, C.byref(st)), "lsi_info")
-        return {k: getattr(st, k) for k, _ in _LsiInfo._fields_ if k != "pad"}
+        return {k: getattr(st, k) for k, _ in _LsiInfo._fields_}
 
     def seed(self, keys, stats: dict | None = None) -> np.ndarray:
         keys = np.ascontiguousarray(np.asarray(keys, np.uint64))

@@ -10,6 +10,8 @@
 
 #include "hsfm_internal.h"
 
+struct hsfm_index;
+
 struct hsfm_device {
     int ordinal = 0;
     int num_cus = 0;
@@ -61,6 +63,11 @@ struct Workspace {
     uint64_t *h_pinned = nullptr;             // small readbacks
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}; // main, resolve, fallback kernels
 };
+
+// Index whose batch callback runs on this thread: hsfm_find_mems_batched
+// holds the index lock across the callback, so entry points called from it on
+// the same index return HSFM_EINVAL (HSFM_LOCK_INDEX) instead of deadlocking.
+extern thread_local const hsfm_index *tl_callback_index;
 
 struct BatchBufs; // buffers of the batched MEM pipeline (hsfm_mem.hip)
 void free_batch_bufs(BatchBufs *b);
@@ -141,3 +148,10 @@ int launch_resolve(hsfm_index *idx, const DevDir &d, const uint32_t *d_rows, uin
                    const uint32_t *d_bounds, uint32_t nbounds, uint32_t *d_out_pairs,
                    unsigned long long *d_guard);
 } // namespace hsfm
+
+// Serialize the calls on one index (see tl_callback_index).
+#define HSFM_LOCK_INDEX(idx, name)                                                                    \
+    if (::hsfm::tl_callback_index == (idx))                                                           \
+        return ::hsfm::set_error(HSFM_EINVAL, "%s: called from a batch callback on the same index",   \
+                                 name);                                                               \
+    std::lock_guard<std::mutex> hsfm_index_lock_((idx)->mu)

@@ -412,6 +412,8 @@ extern "C" int hsfm_index_upload(hsfm_device *dev, const hsfm_index_view *fwd, c
 
 extern "C" int hsfm_index_free(hsfm_index *idx) {
     if (!idx) return HSFM_OK;
+    if (tl_callback_index == idx)
+        return set_error(HSFM_EINVAL, "hsfm_index_free: called from a batch callback on the same index");
     (void)hipSetDevice(idx->dev->ordinal);
     (void)hipStreamSynchronize(idx->dev->stream);
     for (hipStream_t bs : idx->bstream)

@@ -1399,7 +1399,7 @@ static int stage_queries(hsfm_index *idx, const uint8_t *qflat, const uint64_t *
 extern "C" int hsfm_locate_device(hsfm_index *idx, const uint8_t *d_qflat, const uint64_t *d_qoff, uint32_t nq,
                                   hsfm_locate_device_result *out) {
     if (!idx || !out || (nq && (!d_qflat || !d_qoff))) return set_error(HSFM_EINVAL, "hsfm_locate_device: null argument");
-    std::lock_guard<std::mutex> g(idx->mu);
+    HSFM_LOCK_INDEX(idx, "hsfm_locate_device");
     HSFM_HIP(hipSetDevice(idx->dev->ordinal));
     return locate_device_impl(idx, d_qflat, d_qoff, nq, out);
 }
@@ -1407,7 +1407,7 @@ extern "C" int hsfm_locate_device(hsfm_index *idx, const uint8_t *d_qflat, const
 extern "C" int hsfm_locate_batch(hsfm_index *idx, const uint8_t *qflat, const uint64_t *qoff, uint32_t nq,
                                  hsfm_locate_result **out) {
     if (!idx || !out || !qoff || (nq && !qflat)) return set_error(HSFM_EINVAL, "hsfm_locate_batch: null argument");
-    std::lock_guard<std::mutex> g(idx->mu);
+    HSFM_LOCK_INDEX(idx, "hsfm_locate_batch");
     HSFM_HIP(hipSetDevice(idx->dev->ordinal));
     int rc = stage_queries(idx, qflat, qoff, nq);
     if (rc) return rc;
@@ -1438,7 +1438,7 @@ extern "C" int hsfm_locate_positions_batch(hsfm_index *idx, const uint8_t *qflat
                                            hsfm_positions_result **out) {
     if (!idx || !out || !qoff || (nq && !qflat))
         return set_error(HSFM_EINVAL, "hsfm_locate_positions_batch: null argument");
-    std::lock_guard<std::mutex> g(idx->mu);
+    HSFM_LOCK_INDEX(idx, "hsfm_locate_positions_batch");
     HSFM_HIP(hipSetDevice(idx->dev->ordinal));
     int rc = stage_queries(idx, qflat, qoff, nq);
     if (rc) return rc;
@@ -1480,7 +1480,7 @@ extern "C" int hsfm_locate_positions_batch(hsfm_index *idx, const uint8_t *qflat
 extern "C" int hsfm_count_batch(hsfm_index *idx, const uint8_t *qflat, const uint64_t *qoff, uint32_t nq,
                                 uint32_t *counts) {
     if (!idx || !counts || !qoff || (nq && !qflat)) return set_error(HSFM_EINVAL, "hsfm_count_batch: null argument");
-    std::lock_guard<std::mutex> g(idx->mu);
+    HSFM_LOCK_INDEX(idx, "hsfm_count_batch");
     HSFM_HIP(hipSetDevice(idx->dev->ordinal));
     idx->stats = hsfm_stats{};
     int rc = stage_queries(idx, qflat, qoff, nq);

@@ -10,17 +10,23 @@
 //   1. enumerate every present k-mer from the index itself, level by level:
 //      the node of cS is one exact left extension of the node of S; children
 //      are placed digit-major, which keeps each level sorted by key
-//   2. stage 1 (FP32 MLP, x -> f(x) ~ slot / N) trained by Adam on a sample
-//      of (key, slot) pairs with the forward/backward kernels below
-//   3. stage 2: every key routed to leaf floor(f(x) * L); per leaf a
-//      least-squares line slot ~ s0 + a * (key - kmin) + b (float64 sums,
-//      FP32 parameters) and the maximum error of its keys
+//   2. stage 1 (FP32 MLP, x -> f(x) ~ slot / N, the key CDF) trained by
+//      Adam on a sample of (key, slot) pairs with the forward/backward kernels
+//      below
+//   3. stage 2: leaf = the key's leading log2(L) bits (L a power of two), so
+//      leaves are aligned with k-mer prefixes; per leaf a least-squares map
+//      slot ~ s0 + a * f(u) + b of the stage-1 output at u, the key's
+//      position inside its leaf in [0, 1) (float64 sums, FP32 parameters),
+//      and the maximum error of its keys.  The key distribution is close to
+//      a product of per-base frequencies, so the CDF inside a prefix has the
+//      shape of the global CDF: the trained f bends each leaf's map along it
+//      (an untrained f(x) = x leaves one straight line per leaf)
 // Lookup: route, predict, then a binary search of the sorted keys restricted
-// to [pred - err, pred + err] (through fences of every 512th and every 32nd
-// key, above them one of every 2^sh-th key when the widest window needs it,
-// then the last 33 keys, each level one round of independent loads);
-// exact because err bounds every present key and an absent key is absent from
-// every window.
+// to [pred - err, pred + err] through fence levels of every 2^5, 2^9, 2^13 ...
+// -th key (as many as the widest window needs; each level narrows the range
+// to 17 entries of the next) and a last round of at most 33 keys, each level
+// one round of independent loads; exact because err bounds every present key
+// and an absent key is absent from every window.
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -32,25 +38,40 @@
 namespace hsfm {
 
 constexpr uint32_t kLsiMaxHidden = 64;
-constexpr int kLsiFences = 3; // fence 0: every 2^top-th key (top = 0: none); 1, 2: every 512th, 32nd key
-constexpr int kLsiFenceShift[kLsiFences] = {0, 9, 5};
+// Fence level v holds every 2^(5 + 4v)-th key.  A range of at most 2^(9 + 4v)
+// + 1 keys spans at most 17 entries of level v + 1 ... of level v, and level
+// 0 leaves at most 33 keys.  Slots are u32, so a window of 2 max_err + 1 <=
+// 2^33 keys spans at most 17 entries of level 6 (stride 2^29): seven levels.
+constexpr int kLsiMaxFences = 7;
+__host__ __device__ constexpr int lsi_fence_shift(int v) { return 5 + 4 * v; }
+// Fence levels needed for a largest per-leaf error emax: the coarsest level v
+// must span the widest window in at most 17 entries, (2 emax) >> shift <= 16;
+// levels 0 and 1 are always built.
+inline int lsi_fence_levels(uint64_t emax) {
+    int nl = 2;
+    while (nl < kLsiMaxFences && ((2ull * emax) >> lsi_fence_shift(nl - 1)) > 16) nl++;
+    return nl;
+}
+struct LsiFences {
+    const unsigned long long *f[kLsiMaxFences];
+};
 
-struct LsiLeaf {             // 32 bytes
-    unsigned long long kmin; // smallest key routed here
-    uint32_t s0;             // smallest slot routed here
-    uint32_t err;            // max |pred - slot| over the leaf's keys
-    float a, b;              // slot - s0 ~ a * (key - kmin) + b
-    uint32_t n, pad;
+struct LsiLeaf {   // 16 bytes
+    uint32_t s0;   // first slot of the leaf
+    uint32_t err;  // max |pred - slot| over the leaf's keys
+    float a, b;    // slot - s0 ~ a * f(u) + b
 };
 
 struct LsiModel {
     const float *p; // stage 1: w1[H], b1[H], w2[H], b2
     uint32_t hidden, leaves, k;
     uint64_t nkeys;
-    uint32_t top; // shift of the top fence level, 0 if there is none
+    uint32_t nfence;  // fence levels in use (2..kLsiMaxFences)
+    uint32_t ubits;   // key bits below the leaf index (2k - log2 L, or 0)
+    uint32_t upshift; // log2 L - 2k when the leaves outnumber the keys' bits
 };
 
-// key / 4^k in [0, 1) in FP32 (the routing feature)
+// key / 4^k in [0, 1) in FP32 (the stage-1 training feature)
 __device__ __forceinline__ float lsi_feature(unsigned long long key, uint32_t k) {
     const int sh = (int)(2 * k) - 24;
     return sh > 0 ? (float)(uint32_t)(key >> sh) * 5.9604644775390625e-8f   // 2^-24
@@ -61,14 +82,19 @@ __device__ __forceinline__ float lsi_stage1(const float *p, uint32_t H, float x)
     for (uint32_t i = 0; i < H; i++) f = fmaf(p[2 * H + i], fmaxf(fmaf(p[i], x, p[H + i]), 0.f), f);
     return f;
 }
-__device__ __forceinline__ uint32_t lsi_leaf(float f, uint32_t L) {
-    const float t = f * (float)L;
-    return !(t > 0.f) ? 0u : (t >= (float)(L - 1) ? L - 1 : (uint32_t)t); // NaN -> leaf 0
+__device__ __forceinline__ uint32_t lsi_leaf(const LsiModel &md, unsigned long long key) {
+    return (uint32_t)((key >> md.ubits) << md.upshift);
 }
-// predicted slot of key in its leaf (identical in the error pass and lookups)
-__device__ __forceinline__ long long lsi_pred(const LsiLeaf &lf, unsigned long long key) {
-    const float dx = (float)(key - lf.kmin);
-    return (long long)lf.s0 + (long long)floorf(fmaf(lf.a, dx, lf.b));
+// position of key inside its leaf, in [0, 1) with 24 bits
+__device__ __forceinline__ float lsi_u(const LsiModel &md, unsigned long long key) {
+    if (md.ubits == 0) return 0.f;
+    const unsigned long long v = key & ((1ull << md.ubits) - 1ull);
+    return md.ubits > 24 ? (float)(uint32_t)(v >> (md.ubits - 24)) * 5.9604644775390625e-8f
+                         : (float)(uint32_t)v * ldexpf(1.f, -(int)md.ubits);
+}
+// predicted slot of key (identical in the error pass and lookups)
+__device__ __forceinline__ long long lsi_pred(const LsiLeaf &lf, float fu) {
+    return (long long)lf.s0 + (long long)floorf(fmaf(lf.a, fu, lf.b));
 }
 
 // ---- 1. enumeration ------------------------------------------------------
@@ -199,54 +225,49 @@ __global__ void k_lsi_sample(const unsigned long long *__restrict__ keys, uint64
 // ---- 3. stage 2 ----------------------------------------------------------
 constexpr uint32_t kLsiChunk = 64; // keys per thread in the leaf passes
 
-__global__ void k_lsi_leaf_init(LsiLeaf *lf, uint32_t L) {
+__global__ void k_lsi_leaf_init(LsiLeaf *lf, uint32_t *cnt, uint32_t L) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= L) return;
     LsiLeaf z;
-    z.kmin = ~0ull; z.s0 = 0xffffffffu; z.err = 0; z.a = 0.f; z.b = 0.f; z.n = 0; z.pad = 0;
+    z.s0 = 0xffffffffu; z.err = 0; z.a = 0.f; z.b = 0.f;
     lf[i] = z;
+    cnt[i] = 0;
 }
 
-// route keys; per leaf min key, min slot, count (chunk-aggregated atomics)
+// per leaf first slot and key count (chunk-aggregated atomics; keys sorted)
 __global__ __launch_bounds__(256) void k_lsi_route(LsiModel md, const unsigned long long *__restrict__ keys,
-                                                   uint32_t *__restrict__ leaf_of, LsiLeaf *__restrict__ lf) {
+                                                   LsiLeaf *__restrict__ lf, uint32_t *__restrict__ cnt) {
+    const uint64_t c0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kLsiChunk;
+    uint32_t cur = 0xffffffffu, c = 0, s0 = 0;
+    for (uint64_t s = c0; s < c0 + kLsiChunk && s < md.nkeys; s++) {
+        const uint32_t l = lsi_leaf(md, keys[s]);
+        if (l != cur) {
+            if (c) {
+                atomicMin(&lf[cur].s0, s0);
+                atomicAdd(&cnt[cur], c);
+            }
+            cur = l; c = 0; s0 = (uint32_t)s;
+        }
+        c++;
+    }
+    if (c) {
+        atomicMin(&lf[cur].s0, s0);
+        atomicAdd(&cnt[cur], c);
+    }
+}
+
+// least-squares sums per leaf of (f(u), slot - s0): sx, sy, sxx, sxy (float64)
+__global__ __launch_bounds__(256) void k_lsi_moments(LsiModel md, const unsigned long long *__restrict__ keys,
+                                                     const LsiLeaf *__restrict__ lf, double *__restrict__ mom) {
     __shared__ float s_p[3 * kLsiMaxHidden + 1];
     for (uint32_t i = threadIdx.x; i < 3 * md.hidden + 1; i += blockDim.x) s_p[i] = md.p[i];
     __syncthreads();
     const uint64_t c0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kLsiChunk;
-    uint32_t cur = 0xffffffffu, cnt = 0, s0 = 0;
-    unsigned long long kmin = 0;
-    for (uint64_t s = c0; s < c0 + kLsiChunk && s < md.nkeys; s++) {
-        const unsigned long long key = keys[s];
-        const uint32_t l = lsi_leaf(lsi_stage1(s_p, md.hidden, lsi_feature(key, md.k)), md.leaves);
-        leaf_of[s] = l;
-        if (l != cur) {
-            if (cnt) {
-                atomicMin(&lf[cur].kmin, kmin);
-                atomicMin(&lf[cur].s0, s0);
-                atomicAdd(&lf[cur].n, cnt);
-            }
-            cur = l; cnt = 0; kmin = key; s0 = (uint32_t)s;
-        }
-        cnt++;
-        kmin = key < kmin ? key : kmin;
-    }
-    if (cnt) {
-        atomicMin(&lf[cur].kmin, kmin);
-        atomicMin(&lf[cur].s0, s0);
-        atomicAdd(&lf[cur].n, cnt);
-    }
-}
-
-// least-squares sums per leaf: n-weighted dx, y, dx^2, dx*y (float64)
-__global__ __launch_bounds__(256) void k_lsi_moments(uint64_t nkeys, const unsigned long long *__restrict__ keys,
-                                                     const uint32_t *__restrict__ leaf_of,
-                                                     const LsiLeaf *__restrict__ lf, double *__restrict__ mom) {
-    const uint64_t c0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kLsiChunk;
     uint32_t cur = 0xffffffffu;
     double sx = 0, sy = 0, sxx = 0, sxy = 0;
-    for (uint64_t s = c0; s < c0 + kLsiChunk && s < nkeys; s++) {
-        const uint32_t l = leaf_of[s];
+    for (uint64_t s = c0; s < c0 + kLsiChunk && s < md.nkeys; s++) {
+        const unsigned long long key = keys[s];
+        const uint32_t l = lsi_leaf(md, key);
         if (l != cur) {
             if (cur != 0xffffffffu) {
                 atomicAdd(&mom[4 * cur + 0], sx); atomicAdd(&mom[4 * cur + 1], sy);
@@ -254,7 +275,7 @@ __global__ __launch_bounds__(256) void k_lsi_moments(uint64_t nkeys, const unsig
             }
             cur = l; sx = sy = sxx = sxy = 0;
         }
-        const double dx = (double)(keys[s] - lf[l].kmin), dy = (double)(s - lf[l].s0);
+        const double dx = (double)lsi_stage1(s_p, md.hidden, lsi_u(md, key)), dy = (double)(s - lf[l].s0);
         sx += dx; sy += dy; sxx += dx * dx; sxy += dx * dy;
     }
     if (cur != 0xffffffffu) {
@@ -263,17 +284,19 @@ __global__ __launch_bounds__(256) void k_lsi_moments(uint64_t nkeys, const unsig
     }
 }
 
-__global__ void k_lsi_solve(LsiLeaf *__restrict__ lf, const double *__restrict__ mom, uint32_t L) {
+__global__ void k_lsi_solve(LsiLeaf *__restrict__ lf, const uint32_t *__restrict__ cnt,
+                            const double *__restrict__ mom, uint32_t L) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= L) return;
     LsiLeaf f = lf[i];
-    if (f.n == 0) {
-        f.kmin = 0; f.s0 = 0; f.a = 0.f; f.b = 0.f;
+    const uint32_t c = cnt[i];
+    if (c == 0) {
+        f.s0 = 0; f.a = 0.f; f.b = 0.f;
     } else {
-        const double n = f.n, sx = mom[4 * i], sy = mom[4 * i + 1], sxx = mom[4 * i + 2], sxy = mom[4 * i + 3];
+        const double n = c, sx = mom[4 * i], sy = mom[4 * i + 1], sxx = mom[4 * i + 2], sxy = mom[4 * i + 3];
         const double den = n * sxx - sx * sx;
         double a = 0, b = sy / n;
-        if (f.n >= 2 && den > 0) {
+        if (c >= 2 && den > 0) {
             a = (n * sxy - sx * sy) / den;
             b = (sy - a * sx) / n;
         }
@@ -287,17 +310,21 @@ __global__ void k_lsi_solve(LsiLeaf *__restrict__ lf, const double *__restrict__
     lf[i] = f;
 }
 
-__global__ __launch_bounds__(256) void k_lsi_err(uint64_t nkeys, const unsigned long long *__restrict__ keys,
-                                                 const uint32_t *__restrict__ leaf_of, LsiLeaf *__restrict__ lf) {
+__global__ __launch_bounds__(256) void k_lsi_err(LsiModel md, const unsigned long long *__restrict__ keys,
+                                                 LsiLeaf *__restrict__ lf) {
+    __shared__ float s_p[3 * kLsiMaxHidden + 1];
+    for (uint32_t i = threadIdx.x; i < 3 * md.hidden + 1; i += blockDim.x) s_p[i] = md.p[i];
+    __syncthreads();
     const uint64_t c0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kLsiChunk;
     uint32_t cur = 0xffffffffu, e = 0;
-    for (uint64_t s = c0; s < c0 + kLsiChunk && s < nkeys; s++) {
-        const uint32_t l = leaf_of[s];
+    for (uint64_t s = c0; s < c0 + kLsiChunk && s < md.nkeys; s++) {
+        const unsigned long long key = keys[s];
+        const uint32_t l = lsi_leaf(md, key);
         if (l != cur) {
             if (cur != 0xffffffffu) atomicMax(&lf[cur].err, e);
             cur = l; e = 0;
         }
-        const long long d = lsi_pred(lf[l], keys[s]) - (long long)s;
+        const long long d = lsi_pred(lf[l], lsi_stage1(s_p, md.hidden, lsi_u(md, key))) - (long long)s;
         const uint32_t ad = (uint32_t)(d < 0 ? -d : d);
         e = ad > e ? ad : e;
     }
@@ -342,12 +369,10 @@ __device__ __forceinline__ long long lsi_last_below(const unsigned long long *__
 // 3.5x slower: fewer keys in flight); a final round of 33 keys (16-key fences
 // with rounds of at most nine loads, for eight waves per SIMD, ran 8% slower).
 __device__ __forceinline__ uint4 lsi_lookup_key(const LsiModel &md, const unsigned long long *__restrict__ keys,
-                                                const unsigned long long *__restrict__ fence0,
-                                                const unsigned long long *__restrict__ fence1,
-                                                const unsigned long long *__restrict__ fence2,
-                                                const uint4 *__restrict__ pay, const LsiLeaf &f,
-                                                unsigned long long key, uint32_t &probes) {
-    const long long pred = lsi_pred(f, key);
+                                                const LsiFences &fc, const uint4 *__restrict__ pay,
+                                                const LsiLeaf &f, float fu, unsigned long long key,
+                                                uint32_t &probes) {
+    const long long pred = lsi_pred(f, fu);
     long long lo = pred - (long long)f.err, hi = pred + (long long)f.err; // inclusive window
     lo = lo < 0 ? 0 : lo;
     hi = hi >= (long long)md.nkeys ? (long long)md.nkeys - 1 : hi;
@@ -355,19 +380,17 @@ __device__ __forceinline__ uint4 lsi_lookup_key(const LsiModel &md, const unsign
     // lower bound of key in [lo, hi].  At each fence level (stride 2^sh
     // keys) the last entry j in (lo >> sh, hi >> sh] below key, else
     // lo >> sh, leaves the lower bound in [max(lo, j << sh), min(hi, (j + 1)
-    // << sh)] or just past hi.  A window of at most 2 max_err + 1 keys
-    // spans at most 17 entries of the top level (its stride is chosen so)
-    // or, without one, of the 512-key level; each level then leaves at
-    // most 17 entries of the next and the 32-key level at most 33 keys,
+    // << sh)] or just past hi.  The window spans at most 17 entries of the
+    // coarsest level in use (the build picks the level count so); each level
+    // then leaves at most 17 entries of the next and level 0 at most 33 keys,
     // read as 16-byte pairs in one round.
     constexpr int kPairs = 17; // 33 keys from an even index: 17 pairs
     if (lo <= hi) {
 #pragma unroll
-        for (int lv = 0; lv < kLsiFences; lv++) {
-            if (lv == 0 && !md.top) continue;
-            const int sh = lv == 0 ? (int)md.top : kLsiFenceShift[lv];
-            const unsigned long long *fl = lv == 0 ? fence0 : lv == 1 ? fence1 : fence2;
-            const long long j = lsi_last_below(fl, (lo >> sh) + 1, hi >> sh, lo >> sh, key, probes);
+        for (int lv = kLsiMaxFences - 1; lv >= 0; lv--) {
+            if (lv >= (int)md.nfence) continue;
+            const int sh = lsi_fence_shift(lv);
+            const long long j = lsi_last_below(fc.f[lv], (lo >> sh) + 1, hi >> sh, lo >> sh, key, probes);
             lo = lo > (j << sh) ? lo : (j << sh);
             hi = hi < ((j + 1) << sh) ? hi : ((j + 1) << sh);
         }
@@ -396,10 +419,8 @@ __device__ __forceinline__ uint4 lsi_lookup_key(const LsiModel &md, const unsign
 }
 
 __global__ __launch_bounds__(256) void k_lsi_lookup(LsiModel md, const unsigned long long *__restrict__ keys,
-                                                    const unsigned long long *__restrict__ fence0,
-                                                    const unsigned long long *__restrict__ fence1,
-                                                    const unsigned long long *__restrict__ fence2,
-                                                    const uint4 *__restrict__ pay, const LsiLeaf *__restrict__ lf,
+                                                    LsiFences fc, const uint4 *__restrict__ pay,
+                                                    const LsiLeaf *__restrict__ lf,
                                                     const unsigned long long *__restrict__ q, uint64_t nq,
                                                     uint4 *__restrict__ out, unsigned long long *__restrict__ steps) {
     __shared__ float s_p[3 * kLsiMaxHidden + 1];
@@ -410,8 +431,8 @@ __global__ __launch_bounds__(256) void k_lsi_lookup(LsiModel md, const unsigned 
     if (t < nq && !md.nkeys) out[t] = make_uint4(0, 0, 0, 0);
     if (t < nq && md.nkeys) {
         const unsigned long long key = q[t];
-        const LsiLeaf f = lf[lsi_leaf(lsi_stage1(s_p, md.hidden, lsi_feature(key, md.k)), md.leaves)];
-        out[t] = lsi_lookup_key(md, keys, fence0, fence1, fence2, pay, f, key, probes);
+        const LsiLeaf f = lf[lsi_leaf(md, key)];
+        out[t] = lsi_lookup_key(md, keys, fc, pay, f, lsi_stage1(s_p, md.hidden, lsi_u(md, key)), key, probes);
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) probes += __shfl_xor(probes, d, 64);
@@ -423,10 +444,8 @@ __global__ __launch_bounds__(256) void k_lsi_lookup(LsiModel md, const unsigned 
 // the leaf costs an LDS read instead of a global round in each key's chain.
 constexpr uint32_t kLsiLdsLeaves = 4096;
 __global__ __launch_bounds__(1024) void k_lsi_lookup_lds(LsiModel md, const unsigned long long *__restrict__ keys,
-                                                         const unsigned long long *__restrict__ fence0,
-                                                         const unsigned long long *__restrict__ fence1,
-                                                         const unsigned long long *__restrict__ fence2,
-                                                         const uint4 *__restrict__ pay, const LsiLeaf *__restrict__ lf,
+                                                         LsiFences fc, const uint4 *__restrict__ pay,
+                                                         const LsiLeaf *__restrict__ lf,
                                                          const unsigned long long *__restrict__ q, uint64_t nq,
                                                          uint4 *__restrict__ out, unsigned long long *__restrict__ steps) {
     __shared__ float s_p[3 * kLsiMaxHidden + 1];
@@ -442,8 +461,8 @@ __global__ __launch_bounds__(1024) void k_lsi_lookup_lds(LsiModel md, const unsi
             continue;
         }
         const unsigned long long key = q[t];
-        const LsiLeaf f = s_lf[lsi_leaf(lsi_stage1(s_p, md.hidden, lsi_feature(key, md.k)), md.leaves)];
-        out[t] = lsi_lookup_key(md, keys, fence0, fence1, fence2, pay, f, key, probes);
+        const LsiLeaf f = s_lf[lsi_leaf(md, key)];
+        out[t] = lsi_lookup_key(md, keys, fc, pay, f, lsi_stage1(s_p, md.hidden, lsi_u(md, key)), key, probes);
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) probes += __shfl_xor(probes, d, 64);
@@ -490,8 +509,20 @@ struct hsfm_lsi {
     hsfm_lsi_config cfg{};
     hsfm_lsi_info info{};
     DevBuf keys, pay, leaves, params;
-    DevBuf fence[kLsiFences]; // keys[j << shift] for every j (lookup levels; see kLsiFenceShift)
-    uint32_t top = 0;         // shift of fence 0, 0 if it is not built
+    DevBuf fence[kLsiMaxFences]; // level v: keys[j << lsi_fence_shift(v)] for every j
+    uint32_t nfence = 0;         // levels built
+    LsiModel model() const {
+        uint32_t lbits = 0;
+        while ((1u << lbits) < cfg.leaves) lbits++;
+        const uint32_t kbits = 2 * cfg.k;
+        return LsiModel{params.as<float>(), cfg.hidden, cfg.leaves, cfg.k, info.num_keys, nfence,
+                        lbits <= kbits ? kbits - lbits : 0u, lbits > kbits ? lbits - kbits : 0u};
+    }
+    LsiFences fences() const {
+        LsiFences fc{};
+        for (int v = 0; v < kLsiMaxFences; v++) fc.f[v] = fence[v].as<unsigned long long>();
+        return fc;
+    }
 };
 
 namespace {
@@ -631,55 +662,53 @@ int lsi_train_and_fit(hsfm_lsi *x) {
     x->info.train_ms = elapsed_ms(e0, e1);
     xs.release(); ys.release(); gm.release(); lossb.release();
 
-    // stage 2
+    // stage 2: leaves are the keys' leading log2(L) bits
     HSFM_HIP(hipEventRecord(e0, s));
-    DevBuf leaf_of, mom;
-    if ((rc = x->leaves.ensure((size_t)L * sizeof(LsiLeaf))) || (rc = leaf_of.ensure(N * 4 + 4)) ||
+    DevBuf cnt, mom;
+    if ((rc = x->leaves.ensure((size_t)L * sizeof(LsiLeaf))) || (rc = cnt.ensure((size_t)L * 4)) ||
         (rc = mom.ensure((size_t)L * 32)))
         return rc;
     HSFM_HIP(hipMemsetAsync(mom.p, 0, (size_t)L * 32, s));
-    hipLaunchKernelGGL(k_lsi_leaf_init, dim3((L + 255) / 256), dim3(256), 0, s, x->leaves.as<LsiLeaf>(), L);
-    LsiModel md{x->params.as<float>(), H, L, x->cfg.k, N, 0};
+    hipLaunchKernelGGL(k_lsi_leaf_init, dim3((L + 255) / 256), dim3(256), 0, s, x->leaves.as<LsiLeaf>(),
+                       cnt.as<uint32_t>(), L);
+    const LsiModel md = x->model();
     const uint64_t chunks = (N + kLsiChunk - 1) / kLsiChunk;
     const dim3 cg((uint32_t)((chunks + 255) / 256));
     if (N) {
         hipLaunchKernelGGL(k_lsi_route, cg, dim3(256), 0, s, md, x->keys.as<unsigned long long>(),
-                           leaf_of.as<uint32_t>(), x->leaves.as<LsiLeaf>());
-        hipLaunchKernelGGL(k_lsi_moments, cg, dim3(256), 0, s, N, x->keys.as<unsigned long long>(),
-                           leaf_of.as<uint32_t>(), x->leaves.as<LsiLeaf>(), mom.as<double>());
+                           x->leaves.as<LsiLeaf>(), cnt.as<uint32_t>());
+        hipLaunchKernelGGL(k_lsi_moments, cg, dim3(256), 0, s, md, x->keys.as<unsigned long long>(),
+                           x->leaves.as<LsiLeaf>(), mom.as<double>());
     }
     hipLaunchKernelGGL(k_lsi_solve, dim3((L + 255) / 256), dim3(256), 0, s, x->leaves.as<LsiLeaf>(),
-                       mom.as<double>(), L);
+                       cnt.as<uint32_t>(), mom.as<double>(), L);
     if (N)
-        hipLaunchKernelGGL(k_lsi_err, cg, dim3(256), 0, s, N, x->keys.as<unsigned long long>(), leaf_of.as<uint32_t>(),
+        hipLaunchKernelGGL(k_lsi_err, cg, dim3(256), 0, s, md, x->keys.as<unsigned long long>(),
                            x->leaves.as<LsiLeaf>());
     HSFM_HIP(hipGetLastError());
     HSFM_HIP(hipEventRecord(e1, s));
     x->info.fit_ms = elapsed_ms(e0, e1);
     std::vector<LsiLeaf> h(L);
+    std::vector<uint32_t> hc(L);
     HSFM_HIP(hipMemcpyAsync(h.data(), x->leaves.p, (size_t)L * sizeof(LsiLeaf), hipMemcpyDeviceToHost, s));
+    HSFM_HIP(hipMemcpyAsync(hc.data(), cnt.p, (size_t)L * 4, hipMemcpyDeviceToHost, s));
     HSFM_HIP(hipStreamSynchronize(s));
     double wsum = 0;
     uint32_t emax = 0;
     for (uint32_t i = 0; i < L; i++) {
         emax = h[i].err > emax ? h[i].err : emax;
-        wsum += (double)h[i].n * (2.0 * h[i].err + 1.0);
+        wsum += (double)hc[i] * (2.0 * h[i].err + 1.0);
     }
     x->info.max_err = emax;
     x->info.mean_window = N ? wsum / (double)N : 0.0;
-    leaf_of.release(); mom.release();
-    // fences: the top level only when a window of 2 max_err + 1 keys can span
-    // more than 17 entries of the 512-key level, with the smallest stride at
-    // which it spans at most 17 (the lookup reads each level in one round of
-    // at most 17 entries; slots are u32, so a stride of 2^33 always suffices)
-    x->top = 0;
-    if ((2ull * emax) >> kLsiFenceShift[1] > 16) {
-        x->top = kLsiFenceShift[1] + 1;
-        while (x->top < 40 && (2ull * emax) >> x->top > 16) x->top++;
-    }
-    for (int lv = 0; lv < kLsiFences; lv++) { // padded past the last key (pair loads read one entry ahead)
-        const int sh = lv == 0 ? (int)x->top : kLsiFenceShift[lv];
-        if (lv == 0 && !x->top) continue;
+    cnt.release(); mom.release();
+    // fence levels: as many as the widest window (2 max_err + 1 keys) needs
+    // to span at most 17 entries of the coarsest; each padded past the last
+    // key (pair loads read one entry ahead)
+    x->nfence = (uint32_t)lsi_fence_levels(emax);
+    x->info.fence_levels = x->nfence;
+    for (int lv = 0; lv < (int)x->nfence; lv++) {
+        const int sh = lsi_fence_shift(lv);
         DevBuf &f = x->fence[lv];
         const uint64_t nf = (N >> sh) + 2;
         if ((rc = f.ensure(nf * 8))) return rc;
@@ -699,8 +728,12 @@ extern "C" int hsfm_lsi_build(hsfm_index *idx, const hsfm_lsi_config *cfg, hsfm_
     if (cfg->k == 0 || cfg->k > 31) return set_error(HSFM_EINVAL, "hsfm_lsi_build: k must be in 1..31");
     if (cfg->hidden == 0 || cfg->hidden > kLsiMaxHidden)
         return set_error(HSFM_EINVAL, "hsfm_lsi_build: hidden must be in 1..%u", kLsiMaxHidden);
-    if (cfg->leaves == 0 || cfg->leaves > (1u << 24)) return set_error(HSFM_EINVAL, "hsfm_lsi_build: bad leaf count");
-    std::lock_guard<std::mutex> lk(idx->mu);
+    if (cfg->leaves == 0 || cfg->leaves > (1u << 24) || (cfg->leaves & (cfg->leaves - 1)))
+        return set_error(HSFM_EINVAL, "hsfm_lsi_build: leaves must be a power of two in 1..2^24");
+    if (cfg->lookup > 2 || (cfg->lookup == 2 && cfg->leaves > kLsiLdsLeaves))
+        return set_error(HSFM_EINVAL, "hsfm_lsi_build: lookup must be 0, 1 or 2 (2 needs at most %u leaves)",
+                         kLsiLdsLeaves);
+    HSFM_LOCK_INDEX(idx, "hsfm_lsi_build");
     HSFM_HIP(hipSetDevice(idx->dev->ordinal));
     hsfm_lsi *x = new hsfm_lsi();
     x->idx = idx;
@@ -737,40 +770,31 @@ extern "C" int hsfm_lsi_free(hsfm_lsi *x) {
     return HSFM_OK;
 }
 
-extern "C" int hsfm_lsi_seed_device(hsfm_lsi *x, const uint64_t *d_keys, uint64_t n, uint32_t *d_out,
-                                    hsfm_seed_stats *st) {
-    if (!x || (n && (!d_keys || !d_out))) return set_error(HSFM_EINVAL, "hsfm_lsi_seed_device: null argument");
+namespace {
+// Lookup of n device keys; the caller holds idx->mu.
+int lsi_seed_impl(hsfm_lsi *x, const uint64_t *d_keys, uint64_t n, uint32_t *d_out, hsfm_seed_stats *st) {
     hsfm_index *idx = x->idx;
-    std::lock_guard<std::mutex> lk(idx->mu);
-    HSFM_HIP(hipSetDevice(idx->dev->ordinal));
     hipStream_t s = idx->dev->stream;
     int rc;
     if ((rc = idx->ws.counters.ensure(64))) return rc;
     HSFM_HIP(hipMemsetAsync(idx->ws.counters.p, 0, 8, s));
-    LsiModel md{x->params.as<float>(), x->cfg.hidden, x->cfg.leaves, x->cfg.k, x->info.num_keys, x->top};
+    const LsiModel md = x->model();
     if ((rc = event_record(idx, 0, false))) return rc;
-    // leaves in LDS when they fit and no top fence level is needed (measured:
+    // leaves in LDS when they fit and two fence levels suffice (measured:
     // 100 Mbp, 4096 leaves, 0.66 ms against 0.88 ms; 1 Gbp, whose windows need
-    // the top level, 1.16 ms against 1.12 ms).  HSFM_LSI_LOOKUP=global / lds
-    // forces either kernel (lds only where the leaves fit).
-    const char *path = getenv("HSFM_LSI_LOOKUP");
-    const bool fits = x->cfg.leaves <= kLsiLdsLeaves;
-    const bool lds = fits && (path && strcmp(path, "lds") == 0 ? true
-                              : path && strcmp(path, "global") == 0 ? false
-                              : x->top == 0);
+    // a third level, 1.16 ms against 1.12 ms)
+    const bool lds = x->cfg.lookup == 2 || (x->cfg.lookup == 0 && x->cfg.leaves <= kLsiLdsLeaves && x->nfence == 2);
     const unsigned long long *dk = reinterpret_cast<const unsigned long long *>(d_keys);
     if (n && lds) {
         const uint64_t need = (n + 1023) / 1024, cus = (uint64_t)idx->dev->num_cus;
         hipLaunchKernelGGL(k_lsi_lookup_lds, dim3((uint32_t)(need < cus ? need : cus)), dim3(1024), 0, s, md,
-                           x->keys.as<unsigned long long>(), x->fence[0].as<unsigned long long>(),
-                           x->fence[1].as<unsigned long long>(), x->fence[2].as<unsigned long long>(),
-                           x->pay.as<uint4>(), x->leaves.as<LsiLeaf>(), dk, n, reinterpret_cast<uint4 *>(d_out),
+                           x->keys.as<unsigned long long>(), x->fences(), x->pay.as<uint4>(),
+                           x->leaves.as<LsiLeaf>(), dk, n, reinterpret_cast<uint4 *>(d_out),
                            idx->ws.counters.as<unsigned long long>());
     } else if (n) {
         hipLaunchKernelGGL(k_lsi_lookup, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, md,
-                           x->keys.as<unsigned long long>(), x->fence[0].as<unsigned long long>(),
-                           x->fence[1].as<unsigned long long>(), x->fence[2].as<unsigned long long>(),
-                           x->pay.as<uint4>(), x->leaves.as<LsiLeaf>(), dk, n, reinterpret_cast<uint4 *>(d_out),
+                           x->keys.as<unsigned long long>(), x->fences(), x->pay.as<uint4>(),
+                           x->leaves.as<LsiLeaf>(), dk, n, reinterpret_cast<uint4 *>(d_out),
                            idx->ws.counters.as<unsigned long long>());
     }
     HSFM_HIP(hipGetLastError());
@@ -784,12 +808,9 @@ extern "C" int hsfm_lsi_seed_device(hsfm_lsi *x, const uint64_t *d_keys, uint64_
     return HSFM_OK;
 }
 
-extern "C" int hsfm_kmer_search_device(hsfm_index *idx, uint32_t k, const uint64_t *d_keys, uint64_t n,
-                                       uint32_t *d_out, hsfm_seed_stats *st) {
-    if (!idx || (n && (!d_keys || !d_out))) return set_error(HSFM_EINVAL, "hsfm_kmer_search_device: null argument");
-    if (k == 0 || k > 31) return set_error(HSFM_EINVAL, "hsfm_kmer_search_device: k must be in 1..31");
-    std::lock_guard<std::mutex> lk(idx->mu);
-    HSFM_HIP(hipSetDevice(idx->dev->ordinal));
+// Plain exact search of n device keys; the caller holds idx->mu.
+int kmer_search_impl(hsfm_index *idx, uint32_t k, const uint64_t *d_keys, uint64_t n, uint32_t *d_out,
+                     hsfm_seed_stats *st) {
     hipStream_t s = idx->dev->stream;
     int rc;
     if ((rc = event_record(idx, 0, false))) return rc;
@@ -804,6 +825,24 @@ extern "C" int hsfm_kmer_search_device(hsfm_index *idx, uint32_t k, const uint64
         st->probes = 0;
     }
     return HSFM_OK;
+}
+} // namespace
+
+extern "C" int hsfm_lsi_seed_device(hsfm_lsi *x, const uint64_t *d_keys, uint64_t n, uint32_t *d_out,
+                                    hsfm_seed_stats *st) {
+    if (!x || (n && (!d_keys || !d_out))) return set_error(HSFM_EINVAL, "hsfm_lsi_seed_device: null argument");
+    HSFM_LOCK_INDEX(x->idx, "hsfm_lsi_seed_device");
+    HSFM_HIP(hipSetDevice(x->idx->dev->ordinal));
+    return lsi_seed_impl(x, d_keys, n, d_out, st);
+}
+
+extern "C" int hsfm_kmer_search_device(hsfm_index *idx, uint32_t k, const uint64_t *d_keys, uint64_t n,
+                                       uint32_t *d_out, hsfm_seed_stats *st) {
+    if (!idx || (n && (!d_keys || !d_out))) return set_error(HSFM_EINVAL, "hsfm_kmer_search_device: null argument");
+    if (k == 0 || k > 31) return set_error(HSFM_EINVAL, "hsfm_kmer_search_device: k must be in 1..31");
+    HSFM_LOCK_INDEX(idx, "hsfm_kmer_search_device");
+    HSFM_HIP(hipSetDevice(idx->dev->ordinal));
+    return kmer_search_impl(idx, k, d_keys, n, d_out, st);
 }
 
 extern "C" int hsfm_lsi_grad(hsfm_device *dev, const float *params, uint32_t hidden, const float *x, const float *y,
@@ -836,7 +875,9 @@ extern "C" int hsfm_lsi_grad(hsfm_device *dev, const float *params, uint32_t hid
     return HSFM_OK;
 }
 
-// Host-array variants: stage keys and results through the index workspace.
+// Host-array variants: stage keys and results through the index workspace,
+// under one hold of the index lock (the staging buffers are shared with the
+// other entry points of the index).
 namespace {
 int stage_seed(hsfm_index *idx, const uint64_t *keys, uint64_t n, uint64_t **d_keys, uint32_t **d_out) {
     int rc;
@@ -858,27 +899,21 @@ extern "C" int hsfm_lsi_seed(hsfm_lsi *x, const uint64_t *keys, uint64_t n, uint
     uint64_t *dk;
     uint32_t *dout;
     int rc;
-    {
-        std::lock_guard<std::mutex> lk(x->idx->mu);
-        HSFM_HIP(hipSetDevice(x->idx->dev->ordinal));
-        if ((rc = stage_seed(x->idx, keys, n, &dk, &dout))) return rc;
-    }
-    if ((rc = hsfm_lsi_seed_device(x, dk, n, dout, st))) return rc;
-    std::lock_guard<std::mutex> lk(x->idx->mu);
+    HSFM_LOCK_INDEX(x->idx, "hsfm_lsi_seed");
+    HSFM_HIP(hipSetDevice(x->idx->dev->ordinal));
+    if ((rc = stage_seed(x->idx, keys, n, &dk, &dout)) || (rc = lsi_seed_impl(x, dk, n, dout, st))) return rc;
     return unstage_seed(x->idx, out, n);
 }
 
 extern "C" int hsfm_kmer_search(hsfm_index *idx, uint32_t k, const uint64_t *keys, uint64_t n, uint32_t *out) {
     if (!idx || (n && (!keys || !out))) return set_error(HSFM_EINVAL, "hsfm_kmer_search: null argument");
+    if (k == 0 || k > 31) return set_error(HSFM_EINVAL, "hsfm_kmer_search: k must be in 1..31");
     uint64_t *dk;
     uint32_t *dout;
     int rc;
-    {
-        std::lock_guard<std::mutex> lk(idx->mu);
-        HSFM_HIP(hipSetDevice(idx->dev->ordinal));
-        if ((rc = stage_seed(idx, keys, n, &dk, &dout))) return rc;
-    }
-    if ((rc = hsfm_kmer_search_device(idx, k, dk, n, dout, nullptr))) return rc;
-    std::lock_guard<std::mutex> lk(idx->mu);
+    HSFM_LOCK_INDEX(idx, "hsfm_kmer_search");
+    HSFM_HIP(hipSetDevice(idx->dev->ordinal));
+    if ((rc = stage_seed(idx, keys, n, &dk, &dout)) || (rc = kmer_search_impl(idx, k, dk, n, dout, nullptr)))
+        return rc;
     return unstage_seed(idx, out, n);
 }

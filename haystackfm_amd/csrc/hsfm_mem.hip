@@ -1961,7 +1961,7 @@ extern "C" int hsfm_find_mems_device(hsfm_index *idx, const uint8_t *d_qflat, co
         return set_error(HSFM_EINVAL, "hsfm_find_mems_device: null argument");
     if (!idx->has_rev) return set_error(HSFM_EINVAL, "MEM search needs an index uploaded with its reverse index");
     if (mode != HSFM_MODE_SMEM && mode != HSFM_MODE_MEM) return set_error(HSFM_EINVAL, "unknown mode %d", mode);
-    std::lock_guard<std::mutex> g(idx->mu);
+    HSFM_LOCK_INDEX(idx, "hsfm_find_mems_device");
     HSFM_HIP(hipSetDevice(idx->dev->ordinal));
     return mem_device_impl(idx, d_qflat, d_qoff, nq, min_len, d_ref_boundaries, n_refs, max_hits, out);
 }
@@ -1973,7 +1973,7 @@ extern "C" int hsfm_find_mems_batch(hsfm_index *idx, const uint8_t *qflat, const
         return set_error(HSFM_EINVAL, "hsfm_find_mems_batch: null argument");
     if (!idx->has_rev) return set_error(HSFM_EINVAL, "MEM search needs an index uploaded with its reverse index");
     if (mode != HSFM_MODE_SMEM && mode != HSFM_MODE_MEM) return set_error(HSFM_EINVAL, "unknown mode %d", mode);
-    std::lock_guard<std::mutex> g(idx->mu);
+    HSFM_LOCK_INDEX(idx, "hsfm_find_mems_batch");
     HSFM_HIP(hipSetDevice(idx->dev->ordinal));
     Workspace &ws = idx->ws;
     hipStream_t s = idx->dev->stream;
@@ -2284,6 +2284,8 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     if (cfg->batch_reads == 0) return set_error(HSFM_EINVAL, "batch_reads must be positive");
     const uint32_t F = 12;
     if (min_len <= F) return set_error(HSFM_EINVAL, "the batched pipeline needs min_len > %u", F);
+    if (tl_callback_index == idx)
+        return set_error(HSFM_EINVAL, "hsfm_find_mems_batched: called from a batch callback on the same index");
     std::unique_lock<std::mutex> lk(idx->mu);
     HSFM_HIP(hipSetDevice(idx->dev->ordinal));
     hipStream_t s = idx->dev->stream;
@@ -2517,7 +2519,13 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
         }
         stats.records += res.num_hits;
         stats.positions += res.num_positions;
-        if (fn) fn(user, pd.r0, pd.nb, &res);
+        if (fn) {
+            // the lock stays held: calls on this index from fn return HSFM_EINVAL
+            const hsfm_index *outer = tl_callback_index;
+            tl_callback_index = idx;
+            fn(user, pd.r0, pd.nb, &res);
+            tl_callback_index = outer;
+        }
         return HSFM_OK;
     };
     uint32_t kb = 0;

@@ -13,6 +13,8 @@ namespace hsfm {
 
 static thread_local std::string g_err;
 
+thread_local const hsfm_index *tl_callback_index = nullptr;
+
 int set_error(int code, const char *fmt, ...) {
     char buf[1024];
     va_list ap;

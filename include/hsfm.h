@@ -202,6 +202,9 @@ typedef struct {
     double tune_ms[6];  /* batch-0 time of main kernels 0..3 (fan-out 1), then
                            the best main kernel with fan-out 2 and 4           */
 } hsfm_batch_stats;
+/* The index stays locked while fn runs: an entry point called from fn on the
+ * same index (hsfm_index_free included) returns HSFM_EINVAL without doing
+ * anything.  Other indexes and the device helpers below may be used. */
 typedef void (*hsfm_batch_fn)(void *user, uint32_t first_read, uint32_t num_reads,
                               const hsfm_mem_device_result *result);
 int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_qflat, const uint64_t *d_qoff,
@@ -261,16 +264,21 @@ int hsfm_last_stats(hsfm_index *idx, hsfm_stats *out);
 typedef struct hsfm_lsi hsfm_lsi;
 typedef struct {
     uint32_t k;           /* k-mer length, 1..31                            */
-    uint32_t leaves;      /* stage-2 leaves (4096)                          */
+    uint32_t leaves;      /* stage-2 leaves, a power of two (4096): leaf =
+                             the key's leading log2(leaves) bits            */
     uint32_t hidden;      /* stage-1 MLP hidden units (<= 64)               */
     uint32_t train_steps; /* Adam steps of stage 1; 0 keeps f(x) = x        */
     uint32_t sample;      /* training sample size (keys, evenly strided)    */
     float lr;             /* Adam learning rate                             */
+    uint32_t lookup;      /* leaf placement of the lookup kernel: 0 = chosen
+                             by the build (LDS when the leaves fit and two
+                             fence levels suffice), 1 = global, 2 = LDS
+                             (needs leaves <= 4096)                          */
 } hsfm_lsi_config;
 typedef struct {
     uint64_t num_keys;    /* distinct present k-mers                        */
     uint32_t max_err;     /* largest per-leaf error bound (slots)           */
-    uint32_t pad;
+    uint32_t fence_levels; /* fence levels of the bounded search (2..7)     */
     double mean_window;   /* mean search window 2*err+1 over the keys       */
     double mse_initial, mse_final; /* stage-1 training loss before / after  */
     double enum_ms, train_ms, fit_ms;

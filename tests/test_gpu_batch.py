@@ -101,3 +101,39 @@ def test_batched_with_14mer_seeds(uneven, monkeypatch):
     got = g.find_mems_batched((flat, off), 19, hf.HSFM_MODE_SMEM, bounds, 1024, batch_reads=700, main_kernel=1,
                               resolve_fanout=1)
     _same(got[:3], ref)
+
+
+def test_callback_into_the_same_index_is_refused(uneven):
+    """The index stays locked while a batch callback runs: a call on the same
+    index from the callback returns an error instead of deadlocking, and the
+    pipeline's results are unaffected."""
+    o, g, (flat, off), bounds = uneven
+    sub = (flat[:int(off[200])], off[:201])
+    errors = []
+    L = hf._lib()
+    dev = g.fwd._dev[0]._h
+    ptrs = []
+    for arr in (sub[0], sub[1]):
+        p = hf.C.c_void_p()
+        hf._check(L.hsfm_device_alloc(dev, max(arr.nbytes, 8), hf.C.byref(p)), "alloc")
+        hf._check(L.hsfm_host_to_device(dev, p, arr.ctypes.data, arr.nbytes), "h2d")
+        ptrs.append(p)
+
+    def on_batch(first, n, res):
+        try:
+            g.find_mems_batch(sub, 19, hf.HSFM_MODE_SMEM, (), 16)
+        except hf.GpuError as e:
+            errors.append(str(e))
+
+    try:
+        st = g.find_mems_batched_device(ptrs[0].value, ptrs[1].value, 200, 19, hf.HSFM_MODE_SMEM, 0, 0, 16,
+                                        batch_reads=100, main_kernel=1, resolve_fanout=1, on_batch=on_batch)
+    finally:
+        for p in ptrs:
+            L.hsfm_device_free(dev, p)
+    assert st["batches"] == 2 and len(errors) == 2
+    assert all("batch callback on the same index" in e for e in errors)
+    # the index is usable again once the batched call has returned
+    ref = g.find_mems_batch(sub, 19, hf.HSFM_MODE_SMEM, (), 16)
+    got = g.find_mems_batched(sub, 19, hf.HSFM_MODE_SMEM, (), 16, batch_reads=100, main_kernel=1, resolve_fanout=1)
+    _same(got[:3], ref)

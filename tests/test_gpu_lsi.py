@@ -7,6 +7,7 @@ import pytest
 import haystackfm_amd as hf
 from oracle import lsi as L
 from oracle import oracle as O
+from tests.test_lsi_fences import levels_for
 
 pytestmark = pytest.mark.gpu
 
@@ -62,13 +63,14 @@ def test_lsi_exact_on_larger_index():
 
 def test_lsi_exact_with_wide_windows():
     """Untrained stage 1 and one leaf: error bounds of thousands of keys, so
-    the lookup adds its top fence level; still equal to plain exact search."""
+    the lookup adds a third fence level; still equal to plain exact search."""
     leaves = 1
     rng = np.random.default_rng(leaves)
     text, o, g = _bidir(78, 200_000, repeat_frac=0.3, n_units=20, unit_min=300, unit_max=3000)
     lsi = hf.LearnedSeedIndex(g, k=21, leaves=leaves, hidden=8, train_steps=0)
     info = lsi.info()
     assert (2 * info["max_err"]) >> 9 > 16  # wider than 17 entries of the 512-key fence
+    assert info["fence_levels"] == levels_for(info["max_err"]) >= 3
     flat, _ = O.gen_reads(6, text, o.fwd.seq_boundaries, 2000, 150, sub=0.01, both_strands=True)
     q = hf.kmer_keys(flat, 21)
     q = q[q != np.uint64(0xFFFFFFFFFFFFFFFF)]
@@ -79,19 +81,42 @@ def test_lsi_exact_with_wide_windows():
 
 
 @pytest.mark.parametrize("path", ["global", "lds"])
-def test_lsi_both_lookup_kernels(path, monkeypatch):
+def test_lsi_both_lookup_kernels(path):
     """The lookup with its leaves in global memory and in LDS: equal to plain
-    exact search, with and without the top fence level."""
-    monkeypatch.setenv("HSFM_LSI_LOOKUP", path)
+    exact search, with two fence levels and with more."""
     text, o, g = _bidir(79, 150_000, repeat_frac=0.3, n_units=20, unit_min=300, unit_max=3000)
     flat, _ = O.gen_reads(7, text, o.fwd.seq_boundaries, 1500, 150, sub=0.01, both_strands=True)
     q = hf.kmer_keys(flat, 21)
     q = q[q != np.uint64(0xFFFFFFFFFFFFFFFF)]
     q = np.concatenate([q, _query_keys(np.random.default_rng(4), q, 21)])
     plain = hf.kmer_search(g, q, 21)
-    for leaves, steps in ((4096, 300), (1, 0)):  # narrow windows; wide ones with the top level
-        lsi = hf.LearnedSeedIndex(g, k=21, leaves=leaves, hidden=8, train_steps=steps)
+    for leaves, steps in ((4096, 300), (1, 0)):  # narrow windows; wide ones with a third level
+        lsi = hf.LearnedSeedIndex(g, k=21, leaves=leaves, hidden=8, train_steps=steps, lookup=path)
         assert np.array_equal(lsi.seed(q), plain)
+
+
+@pytest.mark.parametrize("path", ["global", "lds"])
+def test_lsi_exact_beyond_three_fence_levels(path):
+    """A 3 Mbp text with GC 0.30, one leaf and the untrained stage 1: the
+    largest error bound is above 69,631 keys, where a fixed set of fence
+    levels (a top level plus the 512- and 32-key ones) lost present keys; the
+    build adds levels until the widest window fits, and every key of the
+    reads, every mutated key and random keys match plain exact search."""
+    rng = np.random.default_rng(12)
+    text, bounds = O.gen_reference(80, 3_000_000, 3, gc=0.3)
+    o = O.OracleBidir.build(text=text, bounds=bounds, sample_rate=32, threads=8)
+    g = hf.BidirFmIndex.from_bytes(o.to_bytes())
+    lsi = hf.LearnedSeedIndex(g, k=21, leaves=1, hidden=8, train_steps=0, lookup=path)
+    info = lsi.info()
+    assert info["max_err"] >= 69_632
+    assert info["fence_levels"] == levels_for(info["max_err"]) >= 4
+    flat, _ = O.gen_reads(9, text, bounds, 3000, 150, sub=0.01, both_strands=True)
+    q = hf.kmer_keys(flat, 21)
+    q = q[q != np.uint64(0xFFFFFFFFFFFFFFFF)]
+    q = np.concatenate([q, _query_keys(rng, q, 21), np.array([0, 4 ** 21 - 1, q.min(), q.max()], np.uint64)])
+    got = lsi.seed(q)
+    assert np.array_equal(got, hf.kmer_search(g, q, 21))
+    assert got[:, 3].sum() > len(q) // 3
 
 
 def test_lsi_gradient_matches_oracle():
@@ -110,6 +135,32 @@ def test_lsi_gradient_matches_oracle():
     assert abs(loss - L.mlp_loss(p32, x32, y32)) <= 1e-4 * max(1.0, abs(loss))
     scale = np.abs(ref_flat).max()
     assert np.abs(g - ref_flat).max() <= 2e-4 * scale
+
+
+def test_trained_stage1_beats_untrained():
+    """What training gains.  Leaves are k-mer prefixes; inside a leaf the
+    keys follow the shape of the global key CDF, which the trained stage 1
+    learns.  On a 2 Mbp text with 16 leaves (about 120k keys each) the
+    trained model's error windows are far narrower than the untrained f(x) =
+    x (one straight line per leaf), the bounded search needs no more fence
+    levels and loads fewer bytes per key, and both stay exact."""
+    text, o, g = _bidir(91, 2_000_000, repeat_frac=0.3, n_units=40, unit_min=300, unit_max=3000)
+    base = hf.LearnedSeedIndex(g, k=21, leaves=16, hidden=16, train_steps=0)
+    trained = hf.LearnedSeedIndex(g, k=21, leaves=16, hidden=16, train_steps=400)
+    bi, ti = base.info(), trained.info()
+    assert ti["num_keys"] == bi["num_keys"]
+    assert ti["mse_final"] < 0.5 * bi["mse_final"]
+    assert ti["mean_window"] < 0.6 * bi["mean_window"]
+    assert ti["max_err"] < bi["max_err"]
+    assert ti["fence_levels"] <= bi["fence_levels"]
+    flat, _ = O.gen_reads(8, text, o.fwd.seq_boundaries, 3000, 150, sub=0.01)
+    q = hf.kmer_keys(flat, 21)
+    q = q[q != np.uint64(0xFFFFFFFFFFFFFFFF)][:50000]
+    plain = hf.kmer_search(g, q, 21)
+    sb, st = {}, {}
+    assert np.array_equal(base.seed(q, sb), plain)
+    assert np.array_equal(trained.seed(q, st), plain)
+    assert st["probes"] < sb["probes"]
 
 
 def test_learned_index_gains_over_the_methods_without_it():

@@ -1,0 +1,152 @@
+"""CPU model of the learned seed index's bounded search (lsi_lookup_key and
+lsi_last_below in haystackfm_amd/csrc/hsfm_lsi.hip): fence levels of every
+2^(5+4v)-th key, each read as one round of at most 9 (fences) or 17 (keys)
+16-byte pairs.  The model reads exactly the entries the kernel reads, so a
+level count that leaves more than 17 entries for a round loses keys here; the
+check is bisection on the same sorted array.  The GPU tests assert that the
+library builds the level count this rule gives (hsfm_lsi_info.fence_levels)."""
+import bisect
+
+import numpy as np
+import pytest
+
+MAX_LEVELS = 7
+U64 = (1 << 64) - 1
+
+
+def shift(v):
+    return 5 + 4 * v
+
+
+def levels_for(emax):
+    nl = 2
+    while nl < MAX_LEVELS and ((2 * emax) >> shift(nl - 1)) > 16:
+        nl += 1
+    return nl
+
+
+class VirtualKeys:
+    """Strictly increasing keys k(i) = 3 i + ((i * 2654435761) >> 7) % 3 for
+    i < n, computed on demand (n up to 2^32 slots); entries past the end read
+    as the largest key value, as the padded device buffers do."""
+
+    def __init__(self, n):
+        self.n = n
+
+    def __getitem__(self, i):
+        if i >= self.n:
+            return U64
+        return 3 * i + ((i * 2654435761) >> 7) % 3
+
+    def fence(self, sh):
+        return lambda j: self[j << sh] if (j << sh) < self.n else U64
+
+
+def last_below(f, a, z, j, key):
+    if a <= z:
+        b = a & ~1
+        c = 0
+        for u in range(9):
+            e = b + 2 * u
+            x, y = (f(e), f(e + 1)) if e <= z else (U64, U64)
+            c += (a <= e <= z and x < key) + (a <= e + 1 <= z and y < key)
+        if c:
+            j = a + c - 1
+    return j
+
+
+def lookup(keys, nlev, pred, err, key):
+    """(found, lower-bound slot) as the kernel computes them."""
+    lo, hi = max(pred - err, 0), min(pred + err, keys.n - 1)
+    if lo <= hi:
+        for lv in range(nlev - 1, -1, -1):
+            sh = shift(lv)
+            j = last_below(keys.fence(sh), (lo >> sh) + 1, hi >> sh, lo >> sh, key)
+            lo, hi = max(lo, j << sh), min(hi, (j + 1) << sh)
+    if lo > hi:
+        return False, None
+    b = lo & ~1
+    below, hit = 0, False
+    for u in range(17):
+        a0 = b + 2 * u
+        x, y = (keys[a0], keys[a0 + 1]) if a0 <= hi else (U64, U64)
+        in0, in1 = lo <= a0 <= hi, lo <= a0 + 1 <= hi
+        below += (in0 and x < key) + (in1 and y < key)
+        hit |= (in0 and x == key) or (in1 and y == key)
+    return hit, lo + below
+
+
+def _cases(rng, n, emax, count):
+    for _ in range(count):
+        slot = int(rng.integers(0, n))
+        err = int(rng.integers(0, emax + 1))
+        pred = slot + int(rng.integers(-err, err + 1))  # the slot lies in the window
+        yield slot, pred, err
+
+
+@pytest.mark.parametrize("emax", [0, 7, 255, 4095, 69_632, 340_899, 1 << 20, 1 << 24, (1 << 31) - 1])
+def test_levels_rule_finds_every_present_key(emax):
+    rng = np.random.default_rng(emax)
+    keys = VirtualKeys((1 << 32) - 1)
+    nlev = levels_for(emax)
+    for slot, pred, err in _cases(rng, keys.n, emax, 300):
+        hit, at = lookup(keys, nlev, pred, err, keys[slot])
+        assert hit and at == slot, (emax, nlev, slot, pred, err)
+
+
+@pytest.mark.parametrize("emax", [4095, 69_632, 1 << 24])
+def test_absent_keys_are_absent(emax):
+    rng = np.random.default_rng(emax + 1)
+    keys = VirtualKeys(1 << 30)
+    nlev = levels_for(emax)
+    for slot, pred, err in _cases(rng, keys.n, emax, 200):
+        key = keys[slot] + 1
+        if key == keys[slot + 1]:
+            continue
+        hit, _ = lookup(keys, nlev, pred, err, key)
+        assert not hit
+
+
+def test_one_level_fewer_loses_keys():
+    """The rule is needed: with one level fewer than it gives, windows of the
+    largest error lose present keys (as a fixed two levels did beyond
+    max_err = 69,631, once the coarsest round could not cover its window)."""
+    rng = np.random.default_rng(5)
+    keys = VirtualKeys(1 << 32)
+    for emax in (200_000, 1 << 20, 1 << 26):
+        nlev = levels_for(emax)
+        assert nlev > 2
+        misses = 0
+        for slot, pred, err in _cases(rng, keys.n, emax, 200):
+            hit, at = lookup(keys, nlev - 1, pred, err, keys[slot])
+            misses += not (hit and at == slot)
+        assert misses > 0
+
+
+def test_small_arrays_match_bisection():
+    rng = np.random.default_rng(9)
+    for n in (1, 2, 3, 33, 34, 100, 5000):
+        arr = np.unique(rng.integers(0, 10 * n + 10, size=n * 2, dtype=np.uint64))[:n]
+        n = len(arr)
+        keys = VirtualKeys(n)
+        keys.__class__ = type("Arr", (VirtualKeys,), {"__getitem__": lambda s, i: int(arr[i]) if i < n else U64})
+        emax = n
+        nlev = levels_for(emax)
+        for q in range(0, int(arr[-1]) + 3):
+            pred = int(rng.integers(0, n))
+            hit, at = lookup(keys, nlev, pred, emax, q)
+            lb = bisect.bisect_left(arr.tolist(), q)
+            assert hit == (lb < n and int(arr[lb]) == q)
+            if hit:
+                assert at == lb
+
+
+def test_levels_rule_values():
+    assert levels_for(0) == 2
+    assert levels_for(4119) == 2          # 100 Mbp, 4096 leaves
+    assert levels_for(4351) == 2          # (2 emax) >> 9 <= 16
+    assert levels_for(4352) == 3
+    assert levels_for(69_631) == 3        # 1 Gbp, 4096 leaves: 3 levels
+    assert levels_for(69_632) == 4
+    assert levels_for(340_899) == 4
+    assert levels_for((1 << 32) - 1) == 7
