@@ -1,0 +1,295 @@
+"""The FmIndex / BidirFmIndex byte stream, assembled from the specification.
+
+Nothing here calls the oracle's builder or serializer to make the expected
+bytes.  Every field is derived in plain Python from the struct definitions and
+builders of the reference, and encoded with bincode 1's default rules:
+little-endian fixed-width integers, a u64 length before every Vec / slice /
+String, an Option as a u8 tag (0 None, 1 Some) before the value, an enum as
+its u32 variant index, usize as u64, fixed arrays and tuples and structs as
+their elements in order with no length.
+
+Field sources:
+  SerializableFmIndex     src/fm_index/serialize.rs:55-67 (field order)
+  CArray                  src/c_array.rs:8-29 (data[c] = #text chars < c)
+  OccTable                src/occ/mod.rs:44-78 (fields), 108-171 (record
+                          [sb u32 x L][delta u16 x L][plane or one-hot u64]),
+                          src/occ/cpu.rs:23-114 (lanes = symbols present in
+                          increasing code order, checkpoints every 512
+                          positions, deltas per 64-position block)
+  SampledSuffixArray      src/suffix_array/mod.rs:54-63, 71-118 (sampled iff
+                          SA[i] % rate == 0 or a sequence start; 14-byte
+                          records [word u64][sb u32][delta u16], sb every 8)
+  LookupTable             src/fm_index/lookup.rs:19-27, 37-91 (BFS over
+                          A,C,G,T; leftmost code is the least significant
+                          digit; last-level empty children keep their
+                          (lo, lo), pruned subtrees and absent symbols (0, 0))
+  build_cpu_inner         src/fm_index/mod.rs:117-195 (text = s1 $ s2 $ ...,
+                          seq_boundaries = cumulative ends, empty header ->
+                          "seq_i", suffix array = build_suffix_array_naive,
+                          suffix_array/cpu.rs:20-25; BWT[i] = T[SA[i]-1],
+                          bwt/cpu.rs:14-25)
+  BidirFmIndex            src/fm_index/bidir_index.rs:53-80, 183-210,
+                          515-531 ([u32 LE fwd_len][fwd][rev]; rev = the
+                          reversed text minus its last $, as one unnamed
+                          sequence, lookup depth 0)
+
+The oracle's to_bytes must equal these bytes, and the library's zero-copy
+view (hsfm_index_view_from_bytes) must read every field back.
+"""
+import bisect
+import ctypes as C
+import struct
+
+import numpy as np
+import pytest
+
+import haystackfm_amd as hf
+from oracle import oracle as O
+
+BASES = {"$": 0, "A": 1, "C": 2, "G": 3, "T": 4, "N": 5, "R": 6, "Y": 7, "S": 8, "W": 9, "K": 10, "M": 11,
+         "B": 12, "D": 13, "H": 14, "V": 15}
+
+
+def u8(v):
+    return struct.pack("<B", v)
+
+
+def u32(v):
+    return struct.pack("<I", v)
+
+
+def u64(v):
+    return struct.pack("<Q", v)
+
+
+def vec(items, enc):
+    return u64(len(items)) + b"".join(enc(x) for x in items)
+
+
+def string(s):
+    b = s.encode("utf-8")
+    return u64(len(b)) + b
+
+
+class SpecIndex:
+    """Every field of FmIndex, from the reference's definitions."""
+
+    def __init__(self, seqs, headers=None, rate=4, depth=0, encoding=0, tag=0, concat=True):
+        if concat:
+            text = []
+            bounds = []
+            for s in seqs:
+                text += s + [0]
+                bounds.append(len(text))
+        else:  # one pre-made text (the reverse index): its own single sequence
+            text = seqs[0] + [0]
+            bounds = [len(text)]
+        n = len(text)
+        self.text, self.bounds, self.n = text, bounds, n
+        self.headers = [(h if h else f"seq_{i}") for i, h in enumerate(headers or [""] * len(bounds))]
+        self.rate, self.depth, self.encoding, self.tag = rate, depth, encoding, tag
+        tb = bytes(text)
+        sa = sorted(range(n), key=lambda i: tb[i:])  # lexicographic slice order
+        self.sa = sa
+        self.sufs = [tb[s:] for s in sa]
+        bwt = [text[s - 1] if s else text[n - 1] for s in sa]
+        self.bwt = bwt
+        # C array
+        freq = [0] * 16
+        for ch in text:
+            freq[ch] += 1
+        self.c = [sum(freq[:c]) for c in range(16)]
+        # Occ
+        present = sorted({ch for ch in bwt if ch < 16})
+        self.s2l = [255] * 16
+        for lane, c in enumerate(present):
+            self.s2l[c] = lane
+        self.l2s = [0] * 16
+        for lane, c in enumerate(present):
+            self.l2s[lane] = c
+        L = len(present)
+        P = 0 if L <= 1 else (L - 1).bit_length()
+        W = P if encoding == 0 else L
+        nb = (n + 63) // 64
+        cum = [0] * L
+        sbc, recs = [], []
+        for b in range(nb):
+            if b % 8 == 0:
+                sbc += cum
+                sb_base = list(cum)
+            words = [0] * W
+            for j in range(64):
+                p = 64 * b + j
+                if p >= n:
+                    break
+                lane = self.s2l[bwt[p]]
+                if encoding == 0:
+                    for pl in range(P):
+                        if (lane >> pl) & 1:
+                            words[pl] |= 1 << j
+                else:
+                    words[lane] |= 1 << j
+            rec = b"".join(u32(sb_base[la]) for la in range(L))
+            rec += b"".join(struct.pack("<H", cum[la] - sb_base[la]) for la in range(L))
+            rec += b"".join(u64(w) for w in words)
+            recs.append(rec)
+            for j in range(64 * b, min(64 * b + 64, n)):
+                cum[self.s2l[bwt[j]]] += 1
+        self.L, self.P, self.stride = L, P, 4 * L + 2 * L + 8 * W
+        self.sbc, self.block_data = sbc, b"".join(recs)
+        # sampled suffix array
+        starts = {0} | set(bounds[:-1])
+        marks = [1 if (s % rate == 0 or s in starts) else 0 for s in sa]
+        self.sa_vals = [s for s, m in zip(sa, marks) if m]
+        wd, cumv = [], 0
+        for w in range(nb):
+            if w % 8 == 0:
+                base = cumv
+            word = sum(marks[i] << (i - 64 * w) for i in range(64 * w, min(64 * w + 64, n)))
+            wd.append(u64(word) + u32(base) + struct.pack("<H", cumv - base))
+            cumv += bin(word).count("1")
+        self.word_data = b"".join(wd)
+        # lookup table
+        self.lookup = None
+        if depth:
+            ivs = []
+            for idx in range(4 ** depth):
+                pat = [(idx // 4 ** i) % 4 + 1 for i in range(depth)]  # leftmost = least significant
+                ivs.append(self._lookup_entry(pat))
+            self.lookup = ivs
+
+    def _interval(self, pat):
+        """Rows of the suffixes that start with pat, or (lb, lb) at its
+        insertion point among the sorted suffixes."""
+        pb = bytes(pat)
+        return (bisect.bisect_left(self.sufs, pb), bisect.bisect_left(self.sufs, pb + b"\xff"))
+
+    def _lookup_entry(self, pat):
+        def absent(c):
+            return self.c[c] == (self.c[c + 1] if c < 15 else self.n)
+
+        for i in range(len(pat) - 1, 0, -1):  # the BFS consumes pat right to left
+            if absent(pat[i]):
+                return (0, 0)
+            lo, hi = self._interval(pat[i:])
+            if lo == hi:  # pruned before the last level
+                return (0, 0)
+        if absent(pat[0]):
+            return (0, 0)
+        return self._interval(pat)
+
+    def to_bytes(self):
+        out = b"".join(u32(v) for v in self.c)                                  # c_array: [u32; 16]
+        out += u8(self.L) + u8(self.P)                                          # occ.num_lanes, num_planes
+        out += bytes(self.s2l) + bytes(self.l2s)                                # [u8; 16] x 2
+        out += vec(self.sbc, u32)                                               # superblock_checkpoints
+        out += u64(len(self.block_data)) + self.block_data                      # block_data: Vec<u8>
+        out += u64(self.stride)                                                 # block_stride: usize
+        out += u32(self.encoding)                                               # encoding: enum
+        out += u32(self.n)                                                      # occ.text_len
+        out += u64(len(self.word_data)) + self.word_data                        # sa.word_data
+        out += vec(self.sa_vals, u32) + u32(self.rate) + u32(self.n)            # sa_vals, rate, text_len
+        out += u32(self.n) + u32(len(self.bounds))                              # text_len, num_sequences
+        out += vec(self.bounds, u32)                                            # seq_boundaries
+        out += vec(self.headers, string)                                        # seq_headers
+        if self.lookup is None:
+            out += u8(0)
+        else:
+            out += u8(1) + u32(self.depth) + vec([1, 2, 3, 4], u8)
+            out += vec(self.lookup, lambda iv: u32(iv[0]) + u32(iv[1]))
+        out += u8(self.tag)                                                     # alphabet_tag
+        return out
+
+
+def _codes(s):
+    return [BASES[ch] for ch in s]
+
+
+CASES = [
+    # (sequences, headers, rate, lookup depth, encoding, alphabet tag)
+    (["ACGTACGTTGCA"], None, 4, 0, 0, 0),
+    (["ACGTNACGTRYK", "TTGCA"], ["chr1", "chr2"], 4, 3, 0, 0),
+    (["GATTACA", "", "CCGGTTAA"], ["", "contig é", ""], 2, 2, 1, 0),
+    (["ACGT" * 40 + "N" * 7 + "TTAGGC" * 30], ["x"], 5, 2, 1, 1),
+    (["ACGTTGCA" * 90, "GGGCCCAAATTT" * 25, "ACGN" * 10], None, 32, 4, 0, 0),
+]
+
+
+def _oracle_index(seqs, headers, rate, depth, enc, tag):
+    text, bounds = O.concatenate(seqs)
+    return O.OracleIndex.build(text=text, bounds=bounds, headers=headers, sample_rate=rate, lookup_depth=depth,
+                               encoding=enc, tag=tag, naive_sa=True)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_oracle_to_bytes_matches_the_spec(case):
+    seqs, headers, rate, depth, enc, tag = CASES[case]
+    spec = SpecIndex([_codes(s) for s in seqs], headers, rate, depth, enc, tag).to_bytes()
+    got = _oracle_index(seqs, headers, rate, depth, enc, tag).to_bytes()
+    assert len(got) == len(spec)
+    assert got == spec
+
+
+@pytest.mark.parametrize("case", [1, 2, 4])
+def test_bidir_frame_matches_the_spec(case):
+    seqs, headers, rate, depth, enc, tag = CASES[case]
+    fwd = SpecIndex([_codes(s) for s in seqs], headers, rate, depth, enc, tag)
+    rev = SpecIndex([list(reversed(fwd.text[:-1]))], None, rate, 0, enc, tag, concat=False)
+    f, r = fwd.to_bytes(), rev.to_bytes()
+    spec = struct.pack("<I", len(f)) + f + r
+    text, bounds = O.concatenate(seqs)
+    ob = O.OracleBidir.build(text=text, bounds=bounds, headers=headers, sample_rate=rate, lookup_depth=depth,
+                             encoding=enc, tag=tag, naive_sa=True)
+    assert ob.to_bytes() == spec
+
+
+def _view(data: bytes):
+    buf = np.frombuffer(data, np.uint8)
+    v = hf._View()
+    rc = hf._lib().hsfm_index_view_from_bytes(buf.ctypes.data_as(C.POINTER(C.c_uint8)), len(data), C.byref(v))
+    return rc, v, buf
+
+
+def _arr(ptr, n, dt):
+    if n == 0:
+        return np.zeros(0, dt)
+    return np.frombuffer(C.string_at(ptr, n * np.dtype(dt).itemsize), dt)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_view_reads_every_spec_field(case):
+    seqs, headers, rate, depth, enc, tag = CASES[case]
+    sp = SpecIndex([_codes(s) for s in seqs], headers, rate, depth, enc, tag)
+    data = sp.to_bytes()
+    rc, v, _buf = _view(data)
+    assert rc == 0
+    assert list(v.c_array) == sp.c
+    assert (v.occ_num_lanes, v.occ_num_planes) == (sp.L, sp.P)
+    assert list(v.occ_symbol_to_lane) == sp.s2l and list(v.occ_lane_to_symbol) == sp.l2s
+    assert list(_arr(v.occ_superblock_checkpoints, v.occ_superblock_len, np.uint32)) == sp.sbc
+    assert C.string_at(v.occ_block_data, v.occ_block_data_len) == sp.block_data
+    assert (v.occ_block_stride, v.occ_encoding, v.occ_text_len) == (sp.stride, enc, sp.n)
+    assert C.string_at(v.sa_word_data, v.sa_word_data_len) == sp.word_data
+    assert list(_arr(v.sa_vals, v.sa_vals_len, np.uint32)) == sp.sa_vals
+    assert (v.sa_sample_rate, v.sa_text_len, v.text_len, v.num_sequences) == (rate, sp.n, sp.n, len(sp.bounds))
+    assert list(_arr(v.seq_boundaries, v.seq_boundaries_len, np.uint32)) == sp.bounds
+    assert v.has_lookup == (1 if depth else 0)
+    if depth:
+        assert v.lookup_depth == depth
+        assert C.string_at(v.lookup_core_symbols, v.lookup_core_len) == bytes([1, 2, 3, 4])
+        got = _arr(v.lookup_intervals, 2 * v.lookup_intervals_len, np.uint32).reshape(-1, 2)
+        assert [tuple(int(x) for x in r) for r in got] == sp.lookup
+    assert v.alphabet_tag == tag
+    assert hf.FmIndex.from_bytes(data).seq_headers == sp.headers
+
+
+def test_spec_pins_a_wrong_field_order():
+    """A stream with two fields swapped (sample_rate and sa.text_len) or a
+    missing Option tag is told apart from the spec stream."""
+    seqs, headers, rate, depth, enc, tag = CASES[1]
+    sp = SpecIndex([_codes(s) for s in seqs], headers, rate, depth, enc, tag)
+    good = sp.to_bytes()
+    sp.rate, sp.n = sp.n, sp.rate
+    assert sp.to_bytes() != good
+    got = _oracle_index(seqs, headers, rate, depth, enc, tag).to_bytes()
+    assert got == good
