@@ -297,6 +297,7 @@ def _lib():
                                             C.c_uint32, C.POINTER(_MemDev)]
         L.hsfm_free_result.argtypes = [vp]
         L.hsfm_last_stats.argtypes = [vp, C.POINTER(_Stats)]
+        L.hsfm_last_deferred.argtypes = [vp, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint64)]
         L.hsfm_lsi_build.argtypes = [vp, C.POINTER(_LsiConfig), C.POINTER(vp)]
         L.hsfm_lsi_get_info.argtypes = [vp, C.POINTER(_LsiInfo)]
         L.hsfm_lsi_seed_device.argtypes = [vp, vp, C.c_uint64, vp, C.POINTER(_SeedStats)]
@@ -585,6 +586,16 @@ class BidirFmIndex:
         st = _Stats()
         _check(_lib().hsfm_last_stats(self._handle(), C.byref(st)), "stats")
         return {k: (list(getattr(st, k)) if k == "diag" else getattr(st, k)) for k, _ in _Stats._fields_}
+
+    def last_deferred(self) -> np.ndarray:
+        """Reads the main kernel of the last one-shot MEM call deferred to the
+        workgroup passes (ascending)."""
+        n = C.c_uint64()
+        _check(_lib().hsfm_last_deferred(self._handle(), None, 0, C.byref(n)), "last_deferred")
+        out = np.zeros(max(n.value, 1), np.uint32)
+        _check(_lib().hsfm_last_deferred(self._handle(), out.ctypes.data_as(C.POINTER(C.c_uint32)), n.value,
+                                         C.byref(n)), "last_deferred")
+        return out[:n.value]
 
     def find_mems_batch(self, queries, min_len: int, mode: int, ref_boundaries=(), max_hits_per_mem: int = 1024):
         """Raw CSR result: (hit_offsets[nq+1], hits[nh, 8] u32 view, positions[np, 2])."""

@@ -60,6 +60,7 @@ struct Workspace {
     DevBuf wave_sets, mem_slots, mem_status;
     DevBuf items, item_cnt, item_first, read_off; // MEM work items (read segments)
     DevBuf qnib;                              // queries as base-set nibbles
+    DevBuf defer_ids;                         // work items the MEM main kernel deferred
     uint64_t *h_pinned = nullptr;             // small readbacks
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}; // main, resolve, fallback kernels
 };
@@ -85,6 +86,11 @@ struct hsfm_index {
     std::mutex mu;
     hsfm::Workspace ws;
     hsfm_stats stats{};
+    // work items the MEM main kernel deferred in the last call on the index
+    // (ws.defer_ids; item table in ws.items when reads were split), valid
+    // only when that call was a one-shot MEM call (hsfm_last_deferred)
+    uint64_t defer_n = 0;
+    bool defer_valid = false, defer_items = false;
     // bidirectional seed table of every ACGT F-mer (built on first MEM call)
     hsfm::DevBuf seed_off, seed_iv, seed_lsl;
     uint32_t seed_F = 0;

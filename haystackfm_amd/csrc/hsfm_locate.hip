@@ -1348,6 +1348,7 @@ static int locate_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_
     Workspace &ws = idx->ws;
     hipStream_t s = idx->dev->stream;
     idx->stats = hsfm_stats{};
+    idx->defer_valid = false;
     int rc = ensure_sa_full(idx);
     if (rc) return rc;
     if ((rc = locate_search_phase(idx, d_q, d_qoff, nq))) return rc;
@@ -1483,6 +1484,7 @@ extern "C" int hsfm_count_batch(hsfm_index *idx, const uint8_t *qflat, const uin
     HSFM_LOCK_INDEX(idx, "hsfm_count_batch");
     HSFM_HIP(hipSetDevice(idx->dev->ordinal));
     idx->stats = hsfm_stats{};
+    idx->defer_valid = false;
     int rc = stage_queries(idx, qflat, qoff, nq);
     if (rc) return rc;
     if ((rc = locate_search_phase(idx, idx->ws.qcopy.as<uint8_t>(), idx->ws.qoffcopy.as<uint64_t>(), nq))) return rc;

@@ -23,6 +23,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+
 #include <hipcub/hipcub.hpp>
 
 #include "hsfm_host.h"
@@ -1685,6 +1687,7 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     Workspace &ws = idx->ws;
     hipStream_t s = idx->dev->stream;
     idx->stats = hsfm_stats{};
+    idx->defer_valid = false;
     int rc;
     uint64_t h[16];
     if ((rc = ensure_sa_full(idx))) return rc;
@@ -1794,6 +1797,13 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         list = ws.ovf_list.as<uint32_t>();
         retry = ws.read_list.as<uint32_t>();
         idx->stats.fallback_reads = nlist;
+        // keep the deferred items for hsfm_last_deferred (the retry passes reuse ovf_list)
+        if ((rc = ws.defer_ids.ensure(nlist * 4 + 4))) return rc;
+        if (nlist)
+            HSFM_HIP(hipMemcpyAsync(ws.defer_ids.p, ws.ovf_list.p, nlist * 4, hipMemcpyDeviceToDevice, s));
+        idx->defer_n = nlist;
+        idx->defer_items = items != nullptr;
+        idx->defer_valid = true;
     }
 
     uint64_t rec_cap = nlist * 8 + 4096;
@@ -2560,6 +2570,31 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     release_graphs();
     if (st) *st = stats;
     return cleanup(rc);
+}
+
+extern "C" int hsfm_last_deferred(hsfm_index *idx, uint32_t *ids, uint64_t cap, uint64_t *count) {
+    if (!idx || !count || (cap && !ids)) return set_error(HSFM_EINVAL, "hsfm_last_deferred: null argument");
+    HSFM_LOCK_INDEX(idx, "hsfm_last_deferred");
+    if (!idx->defer_valid)
+        return set_error(HSFM_EINVAL, "hsfm_last_deferred: the last call on this index was not a one-shot MEM call");
+    HSFM_HIP(hipSetDevice(idx->dev->ordinal));
+    hipStream_t s = idx->dev->stream;
+    std::vector<uint32_t> w(idx->defer_n);
+    if (idx->defer_n)
+        HSFM_HIP(hipMemcpyAsync(w.data(), idx->ws.defer_ids.p, idx->defer_n * 4, hipMemcpyDeviceToHost, s));
+    HSFM_HIP(hipStreamSynchronize(s));
+    if (idx->defer_items) { // item -> its read
+        for (uint32_t &v : w) {
+            uint4 it;
+            HSFM_HIP(hipMemcpy(&it, idx->ws.items.as<uint4>() + v, sizeof(uint4), hipMemcpyDeviceToHost));
+            v = it.x;
+        }
+    }
+    std::sort(w.begin(), w.end());
+    w.erase(std::unique(w.begin(), w.end()), w.end());
+    *count = w.size();
+    for (uint64_t i = 0; i < w.size() && i < cap; i++) ids[i] = w[i];
+    return HSFM_OK;
 }
 
 extern "C" int hsfm_device_alloc(hsfm_device *dev, uint64_t bytes, void **out) {

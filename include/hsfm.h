@@ -254,6 +254,12 @@ typedef struct {
                                   resolve reads (0: LF walks to the samples)      */
 } hsfm_stats;
 int hsfm_last_stats(hsfm_index *idx, hsfm_stats *out);
+/* Reads whose work items the main kernel of the last one-shot MEM call on idx
+ * (hsfm_find_mems_batch / _device) deferred to the workgroup passes, ascending
+ * and distinct: min(cap, *count) ids are written.  No counterpart in the
+ * reference; for parity checks and profiling of the deferred path.
+ * HSFM_EINVAL if the last call on idx was not a one-shot MEM call. */
+int hsfm_last_deferred(hsfm_index *idx, uint32_t *ids, uint64_t cap, uint64_t *count);
 
 /* ---- learned seed index ---------------------------------------------------
  * New component (no reference entry point): replaces the first k exact

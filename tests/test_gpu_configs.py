@@ -1,0 +1,219 @@
+"""GPU parity at the stated sizes of the benchmark workloads.
+
+  workload 1  1 Mbp single contig (i.i.d. ACGT, 0.1% N), IupacDna, rate 32, no
+              lookup table; 10,000 x 32-mers at uniform offsets, 1% subs:
+              count and locate of every query against the oracle.
+  workload 2  100 Mbp in 8 contigs (GC 0.41, 20% copies of 200 repeat units of
+              300-6,000 bp at 5-15% divergence, 0.5% IUPAC codes), rate 32,
+              lookup depth 12; 1,000,000 x 150 bp reads from both strands, 1%
+              subs: locate of every read against the oracle.
+  workload 3  the same index; 1,000,000 x 150 bp reads (2% subs, 0.2% 1-3 bp
+              indels, both strands) + 100,000 random reads; SMEM min_len 18,
+              max_hits 1024: every record of the whole batch checked for what
+              a MEM is (compatible at every listed position, left- and
+              right-maximal, counts and truncation consistent), SMEM == MEM
+              mode, and the oracle on a sample holding every read the main
+              kernel defers (shown by the deferral count of the sample alone).
+  workload 4  the same index; 2,000,000 reads of lognormal length (median
+              150, sigma 1, clipped to 32-10,000), 1.5% subs, shuffled; SMEM
+              min_len 19: the graph-batched pipeline (64k-read batches,
+              autotuned) equals the one-shot call over the whole batch, and
+              the oracle on a sample.
+The learned-seed workload (1 Gbp) has its own test file.
+
+Index bytes come from the library's host builder, whose output equals the
+oracle's (tests/test_index_build.py) and whose format is pinned by the
+specification (tests/test_index_bytes_spec.py).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import haystackfm_amd as hf
+from haystackfm_amd import index_build as IB
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+COMPAT = np.zeros((256, 256), bool)  # COMPAT[query code, text code]
+for _q in range(16):
+    for _t in O.compatible(_q, 0):
+        COMPAT[_q, _t] = True
+
+
+def _log(*a):
+    print("[configs]", *a, flush=True)
+
+
+# ---------------------------------------------------------------- workload 1
+def test_workload1_count_and_locate_1Mbp():
+    text, bounds = IB.synth_reference(0x1111, 1_000_000, 1, gc=0.5, iupac_frac=0.001, iupac_codes=[5])
+    assert len(text) == 1_000_001
+    data = IB.build_index_bytes(text, bounds, sample_rate=32, lookup_depth=0, threads=THREADS)
+    qf, qo = IB.synth_reads(0x1112, text, bounds, 10_000, 32, sub=0.01)
+    assert len(qo) == 10_001 and np.all(np.diff(qo) == 32)
+    g = hf.FmIndex.from_bytes(data)
+    o = O.OracleIndex.from_bytes(data)
+    offs, hits = g.locate_batch((qf, qo))
+    eo, eh = o.locate_batch(qflat=qf, qoff=qo, threads=THREADS)
+    assert np.array_equal(offs, eo) and np.array_equal(hits, eh)
+    assert np.array_equal(g.count_gpu((qf, qo)), np.diff(eo).astype(np.uint32))
+    assert g.last_stats()["guard_hits"] == 0
+    assert int(eo[-1]) > 5_000  # most 32-mers without a substitution occur
+
+
+# ------------------------------------------------------ workloads 2, 3 and 4
+@pytest.fixture(scope="module")
+def ref100():
+    text, bounds = IB.synth_reference(0xC0FFEE, 100_000_000, 8, gc=0.41, repeat_frac=0.20, n_units=200,
+                                      unit_min=300, unit_max=6000, div_min=0.05, div_max=0.15, iupac_frac=0.005)
+    data = IB.build_bidir_bytes(text, bounds, sample_rate=32, lookup_depth=12, threads=THREADS)
+    _log(f"100 Mbp bidirectional index: {len(data) / 1e6:.0f} MB")
+    g = hf.BidirFmIndex.from_bytes(data)
+    ob = O.OracleBidir.from_bytes(data)
+    return text, bounds, g, ob
+
+
+def test_workload2_locate_every_read(ref100):
+    text, bounds, g, ob = ref100
+    lq, lo = IB.synth_reads(0x5EED2, text, bounds, 1_000_000, 150, sub=0.01, both_strands=True)
+    offs, hits = g.fwd.locate_batch((lq, lo))
+    st = g.fwd.last_stats()
+    _log(f"locate: {int(offs[-1])} hits, {st['fallback_reads']} reads past the register kernel")
+    assert st["fallback_reads"] > 10_000 and st["guard_hits"] == 0  # the later levels are exercised
+    eo, eh = ob.fwd.locate_batch(qflat=lq, qoff=lo, threads=THREADS)
+    assert np.array_equal(offs, eo)
+    assert np.array_equal(hits, eh)
+
+
+def _abs_positions(pos, bounds):
+    starts = np.concatenate([[0], np.asarray(bounds, np.int64)[:-1]])
+    return starts[pos[:, 0].astype(np.int64)] + pos[:, 1].astype(np.int64)
+
+
+def _check_mem_records(text, flat, off, ho, hits, pos, bounds, min_len, cap):
+    """What every record of a MemHit list must satisfy, over the whole batch."""
+    nq = len(off) - 1
+    assert ho[0] == 0 and np.all(np.diff(ho.astype(np.int64)) >= 0) and int(ho[-1]) == len(hits)
+    read = np.repeat(np.arange(nq), np.diff(ho.astype(np.int64)))
+    qs, qe = hits[:, 0].astype(np.int64), hits[:, 1].astype(np.int64)
+    cnt, trunc = hits[:, 2].astype(np.int64), hits[:, 3]
+    po = hits[:, 4].astype(np.int64) | (hits[:, 5].astype(np.int64) << 32)
+    plen = hits[:, 6].astype(np.int64)
+    rlen = (off[1:] - off[:-1]).astype(np.int64)[read]
+    assert np.all(qe - qs >= min_len) and np.all(qe <= rlen) and np.all(cnt >= 1)
+    assert np.array_equal(trunc.astype(bool), cnt > cap)
+    assert np.array_equal(plen, np.minimum(cnt, cap))
+    assert np.array_equal(po, np.concatenate([[0], np.cumsum(plen)[:-1]]))
+    # records of a read in (query_start, query_end) order
+    key = read * (1 << 40) + qs * (1 << 20) + qe
+    assert np.all(np.diff(key) >= 0)
+    # every listed position is a compatible occurrence of read[qs:qe]
+    prec = np.repeat(np.arange(len(hits)), plen)
+    p0 = _abs_positions(pos, bounds)
+    q0 = off[read[prec]].astype(np.int64) + qs[prec]
+    ln = (qe - qs)[prec]
+    assert np.all(p0 + ln <= len(text))
+    ok = np.ones(len(prec), bool)
+    for k in range(int(ln.max()) if len(ln) else 0):
+        act = ln > k
+        ok[act] &= COMPAT[flat[q0[act] + k], text[p0[act] + k]]
+    assert ok.all(), int((~ok).sum())
+    # maximality (smem.rs:250-295): read[qs-1:qe] and read[qs:qe+1] occur
+    # nowhere, so no listed occurrence extends left or right
+    left = (qs[prec] > 0) & (p0 > 0)
+    assert not np.any(COMPAT[flat[q0[left] - 1], text[p0[left] - 1]])
+    right = (qe[prec] < rlen[prec]) & (p0 + ln < len(text))
+    assert not np.any(COMPAT[flat[q0[right] + ln[right]], text[p0[right] + ln[right]]])
+    return read
+
+
+def _oracle_records(ob, flat, off, sample, bounds, min_len, cap, mode=0):
+    """The oracle's records of the sample reads as (read, qs, qe, size, truncated, positions...)."""
+    sub_f = np.concatenate([flat[off[r]:off[r + 1]] for r in sample]) if len(sample) else np.zeros(0, np.uint8)
+    sub_o = np.concatenate([[0], np.cumsum([int(off[r + 1] - off[r]) for r in sample])]).astype(np.uint64)
+    exp = ob.find_mems_batch(qflat=sub_f, qoff=sub_o, min_len=min_len, mode=mode, threads=THREADS)
+    recs, rows = [], []
+    for r, mems in zip(sample, exp):
+        for m in mems:
+            for lo, hi, _, _ in sorted(m.ivs.tolist()):
+                size = hi - lo
+                k = min(size, cap)
+                recs.append((int(r), m.query_start, m.query_end, size, int(size > cap), k))
+                rows.append(np.arange(lo, lo + k, dtype=np.uint32))
+    allrows = np.concatenate(rows) if rows else np.zeros(0, np.uint32)
+    p = ob.fwd.resolve_rows(allrows, threads=THREADS) if len(allrows) else np.zeros(0, np.uint32)
+    pairs = [O.ref_map(int(x), bounds) for x in p]
+    return recs, pairs
+
+
+def _device_records(read, hits, pos, sample):
+    want = set(int(r) for r in sample)
+    recs, pairs = [], []
+    po = hits[:, 4].astype(np.int64) | (hits[:, 5].astype(np.int64) << 32)
+    for i in np.nonzero(np.isin(read, list(want)))[0]:
+        h = hits[i]
+        recs.append((int(read[i]), int(h[0]), int(h[1]), int(h[2]), int(h[3]), int(h[6])))
+        pairs += [tuple(int(v) for v in x) for x in pos[po[i]:po[i] + int(h[6])]]
+    return recs, pairs
+
+
+def _ambiguity_dense(flat, off, at_least, limit):
+    """Reads with at least `at_least` ambiguity codes, the densest first."""
+    amb = ((flat >= 5) & (flat < 16)).astype(np.int64)
+    c = np.concatenate([[0], np.cumsum(amb)])
+    per = c[off[1:].astype(np.int64)] - c[off[:-1].astype(np.int64)]
+    idx = np.nonzero(per >= at_least)[0]
+    return idx[np.argsort(-per[idx], kind="stable")][:limit]
+
+
+def test_workload3_smem_whole_batch(ref100):
+    text, bounds, g, ob = ref100
+    sq, so = IB.synth_reads(0x5EED3, text, bounds, 1_000_000, 150, sub=0.02, indel=0.002, both_strands=True,
+                            n_random=100_000, gc=0.41, shuffle=True)
+    min_len, cap = 18, 1024
+    ho, hits, pos = g.find_mems_batch((sq, so), min_len, hf.HSFM_MODE_SMEM, bounds, cap)
+    st = g.last_stats()
+    deferred = g.last_deferred()
+    assert len(deferred) == st["fallback_reads"]
+    _log(f"SMEM: {len(hits)} records, {len(pos)} positions, {len(deferred)} reads deferred by the main kernel")
+    assert st["guard_hits"] == 0
+    read = _check_mem_records(text, sq, so, ho, hits, pos, bounds, min_len, cap)
+    # MEM mode gives the same records
+    mo = g.find_mems_batch((sq, so), min_len, hf.HSFM_MODE_MEM, bounds, cap)
+    assert np.array_equal(mo[0], ho) and np.array_equal(mo[1], hits) and np.array_equal(mo[2], pos)
+    # the oracle on a sample: every read the main kernel deferred, the most
+    # ambiguity-dense reads and reads spread over the batch
+    rng = np.random.default_rng(3)
+    dense = _ambiguity_dense(sq, so, 3, 10_000)
+    spread = rng.choice(len(so) - 1, 12_000, replace=False)
+    sample = np.unique(np.concatenate([deferred, dense, spread]))
+    exp_r, exp_p = _oracle_records(ob, sq, so, sample, bounds, min_len, cap)
+    got_r, got_p = _device_records(read, hits, pos, sample)
+    assert got_r == exp_r
+    assert got_p == exp_p
+
+
+def test_workload4_batched_equals_one_shot(ref100):
+    text, bounds, g, ob = ref100
+    uq, uo = IB.synth_reads(0x5EED4, text, bounds, 2_000_000, 150, sigma=1.0, len_min=32, len_max=10000,
+                            sub=0.015, both_strands=True, shuffle=True)
+    lens = np.diff(uo.astype(np.int64))
+    assert lens.min() >= 32 and lens.max() <= 10000 and 140 <= np.median(lens) <= 160
+    min_len, cap = 19, 1024
+    one = g.find_mems_batch((uq, uo), min_len, hf.HSFM_MODE_SMEM, bounds, cap)
+    bat = g.find_mems_batched((uq, uo), min_len, hf.HSFM_MODE_SMEM, bounds, cap, batch_reads=65536, autotune=True)
+    st = bat[3]
+    _log(f"batched: {st['batches']} batches, {st['fallback_batches']} re-run host-driven, {len(one[1])} records")
+    assert st["batches"] == (len(uo) - 1 + 65535) // 65536
+    assert np.array_equal(bat[0], one[0]) and np.array_equal(bat[1], one[1]) and np.array_equal(bat[2], one[2])
+    read = _check_mem_records(text, uq, uo, *one, bounds, min_len, cap)
+    rng = np.random.default_rng(4)
+    sample = np.sort(rng.choice(len(uo) - 1, 2500, replace=False))
+    sample = np.unique(np.concatenate([sample, np.argsort(lens)[-20:]]))  # the longest reads too
+    exp_r, exp_p = _oracle_records(ob, uq, uo, sample, bounds, min_len, cap)
+    got_r, got_p = _device_records(read, one[1], one[2], sample)
+    assert got_r == exp_r
+    assert got_p == exp_p

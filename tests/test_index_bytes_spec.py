@@ -74,7 +74,7 @@ def string(s):
 class SpecIndex:
     """Every field of FmIndex, from the reference's definitions."""
 
-    def __init__(self, seqs, headers=None, rate=4, depth=0, encoding=0, tag=0, concat=True):
+    def __init__(self, seqs, headers=None, rate=4, depth=0, encoding=0, tag=0, concat=True, drop_samples=()):
         if concat:
             text = []
             bounds = []
@@ -139,7 +139,7 @@ class SpecIndex:
         self.sbc, self.block_data = sbc, b"".join(recs)
         # sampled suffix array
         starts = {0} | set(bounds[:-1])
-        marks = [1 if (s % rate == 0 or s in starts) else 0 for s in sa]
+        marks = [1 if (s % rate == 0 or s in starts) and s not in drop_samples else 0 for s in sa]
         self.sa_vals = [s for s, m in zip(sa, marks) if m]
         wd, cumv = [], 0
         for w in range(nb):
