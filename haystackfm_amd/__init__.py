@@ -303,6 +303,7 @@ def _lib():
         L.hsfm_lsi_seed_device.argtypes = [vp, vp, C.c_uint64, vp, C.POINTER(_SeedStats)]
         L.hsfm_kmer_search_device.argtypes = [vp, C.c_uint32, vp, C.c_uint64, vp, C.POINTER(_SeedStats)]
         L.hsfm_lsi_seed.argtypes = [vp, vp, C.c_uint64, vp, C.POINTER(_SeedStats)]
+        L.hsfm_lsi_seed_device_noalloc.argtypes = [vp, vp, C.c_uint64, vp, C.POINTER(C.c_uint64)]
         L.hsfm_kmer_search.argtypes = [vp, C.c_uint32, vp, C.c_uint64, vp]
         L.hsfm_lsi_grad.argtypes = [vp, vp, C.c_uint32, vp, vp, C.c_uint32, vp, vp]
         L.hsfm_revcomp_device.argtypes = [vp, vp, vp, C.c_uint32, vp]
@@ -788,6 +789,14 @@ class LearnedSeedIndex:
         if stats is not None:
             stats.update(kernel_ms=st.kernel_ms, probes=st.probes)
         return out
+
+    def seed_device_noalloc(self, d_keys: int, n: int, d_out: int) -> int:
+        """seed_device through the allocation-free entry point; returns the
+        call's loads in 64-byte units."""
+        probes = C.c_uint64()
+        _check(_lib().hsfm_lsi_seed_device_noalloc(self._h, C.c_void_p(d_keys), n, C.c_void_p(d_out),
+                                                   C.byref(probes)), "lsi_seed_device_noalloc")
+        return probes.value
 
     def seed_device(self, d_keys: int, n: int, d_out: int) -> dict:
         st = _SeedStats()

@@ -443,6 +443,8 @@ __global__ __launch_bounds__(256) void k_lsi_lookup(LsiModel md, const unsigned 
 // 1024-thread workgroup per CU copies them once and strides over the keys, so
 // the leaf costs an LDS read instead of a global round in each key's chain.
 constexpr uint32_t kLsiLdsLeaves = 4096;
+// keys per call of hsfm_lsi_seed_device_noalloc: one thread each, 256 per block
+constexpr uint64_t kLsiNoAllocMaxKeys = (1ull << 32) - 1;
 __global__ __launch_bounds__(1024) void k_lsi_lookup_lds(LsiModel md, const unsigned long long *__restrict__ keys,
                                                          LsiFences fc, const uint4 *__restrict__ pay,
                                                          const LsiLeaf *__restrict__ lf,
@@ -827,6 +829,74 @@ int kmer_search_impl(hsfm_index *idx, uint32_t k, const uint64_t *d_keys, uint64
     return HSFM_OK;
 }
 } // namespace
+
+// ---- allocation-free lookup -----------------------------------------------
+// The scratch of hsfm_lsi_seed_device_noalloc: word 0 accumulates the call's
+// bounded-search loads.  Each GPU holds its own copy of the array until the
+// process ends; host code uses the address hipGetSymbolAddress gives on that
+// device, never the array's name.  Calls that overlap on one GPU share it.
+constexpr uint32_t kLsiScratchWords = 32; // 256 bytes
+static __device__ __align__(256) unsigned long long scratch[kLsiScratchWords];
+constexpr int kLsiScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned long long *g_scratch_ptr[kLsiScratchDevices];
+
+__global__ void k_lsi_zero(unsigned long long *p, uint32_t n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+namespace {
+// Device address of `scratch` on `ordinal` (that device current), looked up
+// once per device.
+int lsi_scratch(int ordinal, unsigned long long **out) {
+    if (ordinal < 0 || ordinal >= kLsiScratchDevices)
+        return set_error(HSFM_EINVAL, "allocation-free lookup: device ordinal %d out of range", ordinal);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[ordinal]) {
+        void *p = nullptr;
+        HSFM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[ordinal] = static_cast<unsigned long long *>(p);
+    }
+    *out = g_scratch_ptr[ordinal];
+    return HSFM_OK;
+}
+} // namespace
+
+extern "C" int hsfm_lsi_seed_device_noalloc(hsfm_lsi *x, const uint64_t *d_keys, uint64_t n, uint32_t *d_out,
+                                            uint64_t *probes) {
+    if (!x || (n && (!d_keys || !d_out)))
+        return set_error(HSFM_EINVAL, "hsfm_lsi_seed_device_noalloc: null argument");
+    if (n > kLsiNoAllocMaxKeys)
+        return set_error(HSFM_EINVAL, "hsfm_lsi_seed_device_noalloc: %llu keys, at most %llu per call",
+                         (unsigned long long)n, (unsigned long long)kLsiNoAllocMaxKeys);
+    hsfm_index *idx = x->idx;
+    HSFM_LOCK_INDEX(idx, "hsfm_lsi_seed_device_noalloc");
+    HSFM_HIP(hipSetDevice(idx->dev->ordinal));
+    unsigned long long *sc = nullptr;
+    int rc;
+    if ((rc = lsi_scratch(idx->dev->ordinal, &sc))) return rc;
+    hipStream_t s = idx->dev->stream;
+    hipLaunchKernelGGL(k_lsi_zero, dim3(1), dim3(64), 0, s, sc, kLsiScratchWords);
+    const LsiModel md = x->model();
+    const bool lds = x->cfg.lookup == 2 || (x->cfg.lookup == 0 && x->cfg.leaves <= kLsiLdsLeaves && x->nfence == 2);
+    const unsigned long long *dk = reinterpret_cast<const unsigned long long *>(d_keys);
+    if (n && lds) {
+        const uint64_t need = (n + 1023) / 1024, cus = (uint64_t)idx->dev->num_cus;
+        hipLaunchKernelGGL(k_lsi_lookup_lds, dim3((uint32_t)(need < cus ? need : cus)), dim3(1024), 0, s, md,
+                           x->keys.as<unsigned long long>(), x->fences(), x->pay.as<uint4>(),
+                           x->leaves.as<LsiLeaf>(), dk, n, reinterpret_cast<uint4 *>(d_out), sc);
+    } else if (n) {
+        hipLaunchKernelGGL(k_lsi_lookup, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, md,
+                           x->keys.as<unsigned long long>(), x->fences(), x->pay.as<uint4>(),
+                           x->leaves.as<LsiLeaf>(), dk, n, reinterpret_cast<uint4 *>(d_out), sc);
+    }
+    HSFM_HIP(hipGetLastError());
+    if (probes) {
+        HSFM_HIP(hipMemcpyAsync(probes, sc, 8, hipMemcpyDeviceToHost, s));
+        HSFM_HIP(hipStreamSynchronize(s));
+    }
+    return HSFM_OK;
+}
 
 extern "C" int hsfm_lsi_seed_device(hsfm_lsi *x, const uint64_t *d_keys, uint64_t n, uint32_t *d_out,
                                     hsfm_seed_stats *st) {

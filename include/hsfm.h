@@ -300,6 +300,15 @@ int hsfm_lsi_get_info(const hsfm_lsi *lsi, hsfm_lsi_info *out);
 /* Per key (device arrays): d_out[4i..4i+4) = {fwd_lo, fwd_hi, rev_lo, found}. */
 int hsfm_lsi_seed_device(hsfm_lsi *lsi, const uint64_t *d_keys, uint64_t n, uint32_t *d_out,
                          hsfm_seed_stats *stats);
+/* hsfm_lsi_seed_device without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops).  No
+ * counterpart in the reference.  Its only scratch, the load counter, lives in
+ * a 256-byte static device array each GPU holds until the process ends; calls
+ * that overlap on one GPU race on it.  n <= 2^32 - 1 keys per call; *probes
+ * (optional, host) receives the call's loads in 64-byte units, which
+ * synchronizes the index stream.  Same results as hsfm_lsi_seed_device. */
+int hsfm_lsi_seed_device_noalloc(hsfm_lsi *lsi, const uint64_t *d_keys, uint64_t n, uint32_t *d_out,
+                                 uint64_t *probes);
 /* The same answer by plain exact search (k left extensions per key). */
 int hsfm_kmer_search_device(hsfm_index *idx, uint32_t k, const uint64_t *d_keys, uint64_t n,
                             uint32_t *d_out, hsfm_seed_stats *stats);
