@@ -239,7 +239,8 @@ class _LsiConfig(C.Structure):
 class _LsiInfo(C.Structure):
     _fields_ = [("num_keys", C.c_uint64), ("max_err", C.c_uint32), ("fence_levels", C.c_uint32),
                 ("mean_window", C.c_double), ("mse_initial", C.c_double), ("mse_final", C.c_double),
-                ("enum_ms", C.c_double), ("train_ms", C.c_double), ("fit_ms", C.c_double)]
+                ("enum_ms", C.c_double), ("train_ms", C.c_double), ("fit_ms", C.c_double),
+                ("flagged", C.c_uint64), ("mem_seeding", C.c_uint32), ("pad2", C.c_uint32), ("flag_ms", C.c_double)]
 
 
 class _SeedStats(C.Structure):
@@ -261,6 +262,8 @@ _lib_obj = None
 _lib_lock = threading.Lock()
 
 HSFM_MODE_SMEM, HSFM_MODE_MEM = 0, 1
+# restart seeding of the one-shot MEM calls (hsfm_mem_seed_mode)
+HSFM_SEED_TABLES, HSFM_SEED_PLAIN, HSFM_SEED_LEARNED, HSFM_SEED_LEARNED_TABLES = 0, 1, 2, 3
 
 
 def lib_path() -> str:
@@ -298,6 +301,7 @@ def _lib():
         L.hsfm_free_result.argtypes = [vp]
         L.hsfm_last_stats.argtypes = [vp, C.POINTER(_Stats)]
         L.hsfm_last_deferred.argtypes = [vp, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint64)]
+        L.hsfm_mem_seed_mode.argtypes = [vp, C.c_int, vp]
         L.hsfm_lsi_build.argtypes = [vp, C.POINTER(_LsiConfig), C.POINTER(vp)]
         L.hsfm_lsi_get_info.argtypes = [vp, C.POINTER(_LsiInfo)]
         L.hsfm_lsi_seed_device.argtypes = [vp, vp, C.c_uint64, vp, C.POINTER(_SeedStats)]
@@ -588,6 +592,15 @@ class BidirFmIndex:
         _check(_lib().hsfm_last_stats(self._handle(), C.byref(st)), "stats")
         return {k: (list(getattr(st, k)) if k == "diag" else getattr(st, k)) for k, _ in _Stats._fields_}
 
+    def set_seed_mode(self, mode: int, lsi: "LearnedSeedIndex | None" = None):
+        """Restart seeding of the one-shot MEM calls: HSFM_SEED_TABLES (default),
+        HSFM_SEED_PLAIN, HSFM_SEED_LEARNED (lsi, plain fallback) or
+        HSFM_SEED_LEARNED_TABLES (lsi, table fallback).  Results do not depend
+        on it."""
+        _check(_lib().hsfm_mem_seed_mode(self._handle(), int(mode), lsi._h if lsi is not None else None),
+               "mem_seed_mode")
+        self._seed_lsi = lsi  # keep the learned index alive while it is attached
+
     def last_deferred(self) -> np.ndarray:
         """Reads the main kernel of the last one-shot MEM call deferred to the
         workgroup passes (ascending)."""
@@ -779,7 +792,7 @@ class LearnedSeedIndex:
     def info(self) -> dict:
         st = _LsiInfo()
         _check(_lib().hsfm_lsi_get_info(self._h, C.byref(st)), "lsi_info")
-        return {k: getattr(st, k) for k, _ in _LsiInfo._fields_}
+        return {k: getattr(st, k) for k, _ in _LsiInfo._fields_ if k != "pad2"}
 
     def seed(self, keys, stats: dict | None = None) -> np.ndarray:
         keys = np.ascontiguousarray(np.asarray(keys, np.uint64))

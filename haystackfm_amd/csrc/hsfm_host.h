@@ -71,6 +71,11 @@ struct Workspace {
 extern thread_local const hsfm_index *tl_callback_index;
 
 struct BatchBufs; // buffers of the batched MEM pipeline (hsfm_mem.hip)
+struct LsiView;   // learned-seed lookups inside the MEM kernel (hsfm_lsi_dev.h)
+// The lookup view of a learned seed index built on idx, for MEM restarts;
+// HSFM_EINVAL if it cannot seed them exactly.
+int lsi_mem_view(const hsfm_lsi *x, hsfm_index *idx, LsiView *out);
+uint32_t lsi_k(const hsfm_lsi *x);
 void free_batch_bufs(BatchBufs *b);
 
 } // namespace hsfm
@@ -91,6 +96,9 @@ struct hsfm_index {
     // only when that call was a one-shot MEM call (hsfm_last_deferred)
     uint64_t defer_n = 0;
     bool defer_valid = false, defer_items = false;
+    // restart seeding of the one-shot MEM calls (hsfm_mem_seed_mode)
+    int mem_seed_mode = HSFM_SEED_TABLES;
+    hsfm_lsi *mem_lsi = nullptr;
     // bidirectional seed table of every ACGT F-mer (built on first MEM call)
     hsfm::DevBuf seed_off, seed_iv, seed_lsl;
     uint32_t seed_F = 0;

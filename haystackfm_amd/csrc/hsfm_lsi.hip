@@ -34,68 +34,9 @@
 #include <hipcub/hipcub.hpp>
 
 #include "hsfm_host.h"
+#include "hsfm_lsi_dev.h"
 
 namespace hsfm {
-
-constexpr uint32_t kLsiMaxHidden = 64;
-// Fence level v holds every 2^(5 + 4v)-th key.  A range of at most 2^(9 + 4v)
-// + 1 keys spans at most 17 entries of level v + 1 ... of level v, and level
-// 0 leaves at most 33 keys.  Slots are u32, so a window of 2 max_err + 1 <=
-// 2^33 keys spans at most 17 entries of level 6 (stride 2^29): seven levels.
-constexpr int kLsiMaxFences = 7;
-__host__ __device__ constexpr int lsi_fence_shift(int v) { return 5 + 4 * v; }
-// Fence levels needed for a largest per-leaf error emax: the coarsest level v
-// must span the widest window in at most 17 entries, (2 emax) >> shift <= 16;
-// levels 0 and 1 are always built.
-inline int lsi_fence_levels(uint64_t emax) {
-    int nl = 2;
-    while (nl < kLsiMaxFences && ((2ull * emax) >> lsi_fence_shift(nl - 1)) > 16) nl++;
-    return nl;
-}
-struct LsiFences {
-    const unsigned long long *f[kLsiMaxFences];
-};
-
-struct LsiLeaf {   // 16 bytes
-    uint32_t s0;   // first slot of the leaf
-    uint32_t err;  // max |pred - slot| over the leaf's keys
-    float a, b;    // slot - s0 ~ a * f(u) + b
-};
-
-struct LsiModel {
-    const float *p; // stage 1: w1[H], b1[H], w2[H], b2
-    uint32_t hidden, leaves, k;
-    uint64_t nkeys;
-    uint32_t nfence;  // fence levels in use (2..kLsiMaxFences)
-    uint32_t ubits;   // key bits below the leaf index (2k - log2 L, or 0)
-    uint32_t upshift; // log2 L - 2k when the leaves outnumber the keys' bits
-};
-
-// key / 4^k in [0, 1) in FP32 (the stage-1 training feature)
-__device__ __forceinline__ float lsi_feature(unsigned long long key, uint32_t k) {
-    const int sh = (int)(2 * k) - 24;
-    return sh > 0 ? (float)(uint32_t)(key >> sh) * 5.9604644775390625e-8f   // 2^-24
-                  : (float)(uint32_t)key * ldexpf(1.f, -(int)(2 * k));
-}
-__device__ __forceinline__ float lsi_stage1(const float *p, uint32_t H, float x) {
-    float f = p[3 * H];
-    for (uint32_t i = 0; i < H; i++) f = fmaf(p[2 * H + i], fmaxf(fmaf(p[i], x, p[H + i]), 0.f), f);
-    return f;
-}
-__device__ __forceinline__ uint32_t lsi_leaf(const LsiModel &md, unsigned long long key) {
-    return (uint32_t)((key >> md.ubits) << md.upshift);
-}
-// position of key inside its leaf, in [0, 1) with 24 bits
-__device__ __forceinline__ float lsi_u(const LsiModel &md, unsigned long long key) {
-    if (md.ubits == 0) return 0.f;
-    const unsigned long long v = key & ((1ull << md.ubits) - 1ull);
-    return md.ubits > 24 ? (float)(uint32_t)(v >> (md.ubits - 24)) * 5.9604644775390625e-8f
-                         : (float)(uint32_t)v * ldexpf(1.f, -(int)md.ubits);
-}
-// predicted slot of key (identical in the error pass and lookups)
-__device__ __forceinline__ long long lsi_pred(const LsiLeaf &lf, float fu) {
-    return (long long)lf.s0 + (long long)floorf(fmaf(lf.a, fu, lf.b));
-}
 
 // ---- 1. enumeration ------------------------------------------------------
 // Exact lanes of A, C, G, T (lane_sym inverse); 0xff if the index lacks one.
@@ -339,31 +280,7 @@ __global__ void k_lsi_fence(const unsigned long long *__restrict__ keys, uint64_
     if (j < nf) fence[j] = (j << sh) < nkeys ? keys[j << sh] : ~0ull;
 }
 
-// Last e in [a, z] with f[e] < key, or j if there is none (f ascending), for
-// a range of at most 17 entries: one round of 16-byte pairs from an even index
-// (f holds one entry past z).  loads += 64-byte units.
-__device__ __forceinline__ long long lsi_last_below(const unsigned long long *__restrict__ f, long long a,
-                                                    long long z, long long j, unsigned long long key,
-                                                    uint32_t &loads) {
-    constexpr int kPairs = 9;
-    if (a <= z) {
-        const long long b = a & ~1ll;
-        const ulonglong2 *fp = reinterpret_cast<const ulonglong2 *>(f + b);
-        uint32_t c = 0;
-#pragma unroll
-        for (int u = 0; u < kPairs; u++) {
-            const long long e = b + 2 * u;
-            ulonglong2 v = make_ulonglong2(~0ull, ~0ull);
-            if (e <= z) v = fp[u];
-            c += (e >= a && e <= z && v.x < key ? 1u : 0u) + (e + 1 >= a && e + 1 <= z && v.y < key ? 1u : 0u);
-        }
-        loads += (uint32_t)((((z - b) / 2 + 1) * 16 + 63) / 64);
-        if (c) j = a + c - 1;
-    }
-    return j;
-}
-
-// out[4i..4i+4) = {fwd_lo, fwd_hi, rev_lo, found}; steps += the bounded
+// out[4i..4i+4) = {fwd_lo, fwd_hi, rev_lo, found}; probes += the bounded
 // search's loads in 64-byte units (the bytes of each round of paired loads).
 // One key per lane (a quad of lanes per key, loading whole lines together, ran
 // 3.5x slower: fewer keys in flight); a final round of 33 keys (16-key fences
@@ -372,50 +289,82 @@ __device__ __forceinline__ uint4 lsi_lookup_key(const LsiModel &md, const unsign
                                                 const LsiFences &fc, const uint4 *__restrict__ pay,
                                                 const LsiLeaf &f, float fu, unsigned long long key,
                                                 uint32_t &probes) {
-    const long long pred = lsi_pred(f, fu);
-    long long lo = pred - (long long)f.err, hi = pred + (long long)f.err; // inclusive window
-    lo = lo < 0 ? 0 : lo;
-    hi = hi >= (long long)md.nkeys ? (long long)md.nkeys - 1 : hi;
-    uint4 r = make_uint4(0, 0, 0, 0);
-    // lower bound of key in [lo, hi].  At each fence level (stride 2^sh
-    // keys) the last entry j in (lo >> sh, hi >> sh] below key, else
-    // lo >> sh, leaves the lower bound in [max(lo, j << sh), min(hi, (j + 1)
-    // << sh)] or just past hi.  The window spans at most 17 entries of the
-    // coarsest level in use (the build picks the level count so); each level
-    // then leaves at most 17 entries of the next and level 0 at most 33 keys,
-    // read as 16-byte pairs in one round.
-    constexpr int kPairs = 17; // 33 keys from an even index: 17 pairs
-    if (lo <= hi) {
-#pragma unroll
-        for (int lv = kLsiMaxFences - 1; lv >= 0; lv--) {
-            if (lv >= (int)md.nfence) continue;
-            const int sh = lsi_fence_shift(lv);
-            const long long j = lsi_last_below(fc.f[lv], (lo >> sh) + 1, hi >> sh, lo >> sh, key, probes);
-            lo = lo > (j << sh) ? lo : (j << sh);
-            hi = hi < ((j + 1) << sh) ? hi : ((j + 1) << sh);
+    const long long slot = lsi_find_slot(md, keys, fc, f, fu, key, probes);
+    if (slot < 0) return make_uint4(0, 0, 0, 0);
+    const uint4 v = pay[slot];
+    return make_uint4(v.x, v.y, v.z, 1u);
+}
+
+// Keys compatible with a text window that holds a symbol other than A, C, G,
+// T get kLsiFlagAmbiguous in their payload: the IUPAC set of such a k-mer is
+// more than its exact interval.  ls.qb[lane] holds the query bases (bit b:
+// base b+1) compatible with the lane's symbol; ls.exact[lane] is set for the
+// lanes of A, C, G and T.  One thread per text position p holding another
+// symbol takes the windows [s, s + k) with s in (p - k, p] whose first such
+// symbol is p, expands each (every combination of compatible bases) and
+// flags the expansions that are present keys.  ctr[0] counts flagged
+// expansions, ctr[1] windows with more than `cap` expansions (left
+// unflagged: the flags cannot be trusted then).
+constexpr uint32_t kLsiFlagMaxExpansions = 4096; // per window (six N)
+constexpr uint32_t kLsiMemMaxK = 25;               // k-mer windows the MEM kernel decodes
+struct LsiLaneSets {
+    uint8_t qb[16];
+    uint8_t exact[16];
+};
+__global__ __launch_bounds__(256) void k_lsi_flag(LsiModel md, const unsigned long long *__restrict__ keys,
+                                                  LsiFences fc, uint4 *__restrict__ pay,
+                                                  const LsiLeaf *__restrict__ lf, const uint8_t *__restrict__ text,
+                                                  uint32_t n, LsiLaneSets ls, uint32_t cap,
+                                                  unsigned long long *__restrict__ ctr) {
+    __shared__ float s_p[3 * kLsiMaxHidden + 1];
+    for (uint32_t i = threadIdx.x; i < 3 * md.hidden + 1; i += blockDim.x) s_p[i] = md.p[i];
+    __syncthreads();
+    const uint32_t K = md.k;
+    unsigned long long flagged = 0, over = 0;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t tp = text[p] & 15u;
+        if (ls.exact[tp] || ls.qb[tp] == 0) continue; // an exact base, or nothing matches here
+        const uint64_t s_lo = p + 1 >= K ? p + 1 - K : 0;
+        for (uint64_t s0 = s_lo; s0 <= p && s0 + K <= n; s0++) {
+            // the window's first non-exact symbol must be p; no position without
+            // compatible bases; expansion count
+            bool ok = true;
+            uint32_t cnt = 1;
+            for (uint32_t i = 0; i < K && ok; i++) {
+                const uint32_t t = text[s0 + i] & 15u;
+                if (s0 + i < p && !ls.exact[t]) ok = false;
+                else if (ls.qb[t] == 0) ok = false;
+                else if (cnt <= cap) cnt *= (uint32_t)__popc(ls.qb[t]);
+            }
+            if (!ok) continue;
+            if (cnt > cap) {
+                over++;
+                continue;
+            }
+            for (uint32_t e = 0; e < cnt; e++) {
+                unsigned long long key = 0;
+                uint32_t r = e;
+                for (uint32_t i = 0; i < K; i++) { // mixed-radix digit of each position
+                    const uint32_t m = ls.qb[text[s0 + i] & 15u];
+                    const uint32_t c = (uint32_t)__popc(m);
+                    uint32_t pick = r % c, mm = m;
+                    r /= c;
+                    for (uint32_t z = 0; z < pick; z++) mm &= mm - 1;
+                    key = (key << 2) | (unsigned long long)(__ffs(mm) - 1);
+                }
+                uint32_t pr = 0;
+                const long long slot = lsi_find_slot(md, keys, fc, lf[lsi_leaf(md, key)],
+                                                     lsi_stage1(s_p, md.hidden, lsi_u(md, key)), key, pr);
+                if (slot >= 0) {
+                    atomicOr(&pay[slot].w, kLsiFlagAmbiguous);
+                    flagged++;
+                }
+            }
         }
     }
-    if (lo <= hi) {
-        const long long b = lo & ~1ll; // pairs from an even index (the key buffer is padded by one key)
-        probes += (uint32_t)((((hi - b) / 2 + 1) * 16 + 63) / 64); // the round's bytes in 64-byte units
-        const ulonglong2 *kp = reinterpret_cast<const ulonglong2 *>(keys + b);
-        uint32_t below = 0;
-        bool hit = false;
-#pragma unroll
-        for (int u = 0; u < kPairs; u++) {
-            const long long a0 = b + 2 * u;
-            ulonglong2 v = make_ulonglong2(~0ull, ~0ull);
-            if (a0 <= hi) v = kp[u];
-            const bool in0 = a0 >= lo && a0 <= hi, in1 = a0 + 1 >= lo && a0 + 1 <= hi;
-            below += (in0 && v.x < key ? 1u : 0u) + (in1 && v.y < key ? 1u : 0u);
-            hit |= (in0 && v.x == key) || (in1 && v.y == key);
-        }
-        if (hit) {
-            const uint4 v = pay[lo + below];
-            r = make_uint4(v.x, v.y, v.z, 1u);
-        }
-    }
-    return r;
+    if (flagged) atomicAdd(&ctr[0], flagged);
+    if (over) atomicAdd(&ctr[1], over);
 }
 
 __global__ __launch_bounds__(256) void k_lsi_lookup(LsiModel md, const unsigned long long *__restrict__ keys,
@@ -723,7 +672,74 @@ int lsi_train_and_fit(hsfm_lsi *x) {
     return HSFM_OK;
 }
 
+// Flags of the keys whose IUPAC set is more than their exact interval
+// (k_lsi_flag); sets info.mem_seeding when the flags are complete.
+int lsi_flag(hsfm_lsi *x) {
+    hsfm_index *idx = x->idx;
+    hipStream_t s = idx->dev->stream;
+    const DevConst &hc = idx->h_const;
+    LsiLaneSets ls{};
+    bool any = false;
+    for (uint32_t l = 0; l < 16 && l < hc.num_lanes; l++) {
+        uint32_t m = 0;
+        for (uint32_t b = 0; b < 4; b++) m |= ((hc.compat_lanes[b + 1] >> l) & 1u) << b;
+        ls.qb[l] = (uint8_t)m;
+        ls.exact[l] = hc.lane_sym[l] >= 1 && hc.lane_sym[l] <= 4;
+        any |= !ls.exact[l] && m != 0;
+    }
+    x->info.flagged = 0;
+    x->info.mem_seeding = 0;
+    if (x->cfg.k > kLsiMemMaxK) return HSFM_OK; // the MEM kernel reads windows of at most kLsiMemMaxK codes
+    if (!any || x->info.num_keys == 0) { // no text window can hold another compatible symbol
+        x->info.mem_seeding = 1;
+        return HSFM_OK;
+    }
+    int rc;
+    if ((rc = ensure_sa_full(idx))) return rc;
+    if (!idx->fwd.view.text) return HSFM_OK; // no device text: flags unknown, no MEM seeding
+    DevBuf ctr;
+    if ((rc = ctr.ensure(16))) return rc;
+    HSFM_HIP(hipMemsetAsync(ctr.p, 0, 16, s));
+    hipEvent_t e0, e1;
+    HSFM_HIP(hipEventCreate(&e0));
+    HSFM_HIP(hipEventCreate(&e1));
+    HSFM_HIP(hipEventRecord(e0, s));
+    const uint32_t n = idx->fwd.view.n;
+    const uint32_t grid = (uint32_t)idx->dev->num_cus * 16;
+    hipLaunchKernelGGL(k_lsi_flag, dim3(grid), dim3(256), 0, s, x->model(), x->keys.as<unsigned long long>(),
+                       x->fences(), x->pay.as<uint4>(), x->leaves.as<LsiLeaf>(), idx->fwd.view.text, n, ls,
+                       kLsiFlagMaxExpansions, ctr.as<unsigned long long>());
+    HSFM_HIP(hipGetLastError());
+    HSFM_HIP(hipEventRecord(e1, s));
+    uint64_t h[2] = {0, 0};
+    HSFM_HIP(hipMemcpyAsync(h, ctr.p, 16, hipMemcpyDeviceToHost, s));
+    HSFM_HIP(hipStreamSynchronize(s));
+    x->info.flag_ms = elapsed_ms(e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    x->info.flagged = h[0];
+    x->info.mem_seeding = h[1] == 0 ? 1u : 0u;
+    return HSFM_OK;
+}
+
 } // namespace
+
+namespace hsfm {
+int lsi_mem_view(const hsfm_lsi *x, hsfm_index *idx, LsiView *out) {
+    if (!x) return set_error(HSFM_EINVAL, "learned-seed restarts need a learned seed index");
+    if (x->idx != idx) return set_error(HSFM_EINVAL, "the learned seed index was built on another index");
+    if (!x->info.mem_seeding)
+        return set_error(HSFM_EINVAL, "this learned seed index cannot seed MEM restarts exactly (k > %u, or the "
+                                      "text's ambiguity windows could not all be expanded)", kLsiMemMaxK);
+    out->md = x->model();
+    out->fc = x->fences();
+    out->keys = x->keys.as<unsigned long long>();
+    out->pay = x->pay.as<uint4>();
+    out->leaves = x->leaves.as<LsiLeaf>();
+    return HSFM_OK;
+}
+uint32_t lsi_k(const hsfm_lsi *x) { return x->cfg.k; }
+} // namespace hsfm
 
 extern "C" int hsfm_lsi_build(hsfm_index *idx, const hsfm_lsi_config *cfg, hsfm_lsi **out) {
     if (!idx || !cfg || !out) return set_error(HSFM_EINVAL, "hsfm_lsi_build: null argument");
@@ -750,6 +766,7 @@ extern "C" int hsfm_lsi_build(hsfm_index *idx, const hsfm_lsi_config *cfg, hsfm_
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (!rc) rc = lsi_train_and_fit(x);
+    if (!rc) rc = lsi_flag(x);
     if (rc) {
         x->keys.release(); x->pay.release(); x->leaves.release(); x->params.release(); for (DevBuf &f : x->fence) f.release();
         delete x;
@@ -767,6 +784,13 @@ extern "C" int hsfm_lsi_get_info(const hsfm_lsi *x, hsfm_lsi_info *out) {
 
 extern "C" int hsfm_lsi_free(hsfm_lsi *x) {
     if (!x) return HSFM_OK;
+    {
+        HSFM_LOCK_INDEX(x->idx, "hsfm_lsi_free");
+        if (x->idx->mem_lsi == x) { // back to table seeding
+            x->idx->mem_lsi = nullptr;
+            x->idx->mem_seed_mode = HSFM_SEED_TABLES;
+        }
+    }
     x->keys.release(); x->pay.release(); x->leaves.release(); x->params.release(); for (DevBuf &f : x->fence) f.release();
     delete x;
     return HSFM_OK;

@@ -28,6 +28,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "hsfm_host.h"
+#include "hsfm_lsi_dev.h"
 
 namespace hsfm {
 
@@ -708,6 +709,7 @@ __device__ __forceinline__ uint32_t mem_thread_read(const DevDir &fwd, const Dev
 enum : uint32_t {
     SM_FETCH = 0, // take the next read from the queue
     SM_CAND,      // next restart point j
+    SM_RESTART,   // seed the restart at j from the tables, or start it unseeded
     SM_SEEDX,     // next seed-table expansion of q[j-F..j)
     SM_ENDLEFT,   // left extension ended at k
     SM_EMIT,      // right extension ended at e: write the records
@@ -717,8 +719,36 @@ enum : uint32_t {
     SM_TEXT,      // load: 8 text and 8 query codes, compared directly
     SM_LEFT,      // probe: one interval of the set of q[k..j), by q[k-1]
     SM_RIGHT,     // probe: one interval of the set of q[k..e), by q[e]
+    SM_LSI_LEAF,  // load: the learned index's leaf of the k-mer q[j-K..j)
+    SM_LSI_FENCE, // loads: one fence level of its bounded search
+    SM_LSI_KEYS,  // loads: up to 18 keys of its last round (two passes at most)
+    SM_LSI_PAY,   // load: its payload {fwd_lo, fwd_hi, rev_lo, flags}
     SM_DONE
 };
+
+// The K (<= 25) codes of q[g..g+K) as a learned-index key (two bits per base,
+// first base most significant) if every one is a single base; false if any
+// is not (ambiguity code, sentinel, other byte).
+__device__ __forceinline__ bool lsi_window_key(const uint32_t *__restrict__ nib, uint64_t g, uint32_t K,
+                                               unsigned long long &key) {
+    const uint32_t *b = nib + (g >> 3);
+    const uint64_t lo = (uint64_t)b[0] | ((uint64_t)b[1] << 32), hi = (uint64_t)b[2] | ((uint64_t)b[3] << 32);
+    const uint32_t sh = 4 * (uint32_t)(g & 7);
+    uint64_t x0 = sh ? (lo >> sh) | (hi << (64 - sh)) : lo; // codes 0..15 of the window
+    uint64_t x1 = hi >> sh;                                 // codes 16..
+    bool ok = true;
+    unsigned long long k = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 25; i++) {
+        if (i < K) {
+            const uint32_t nbl = (uint32_t)((i < 16 ? x0 >> (4 * i) : x1 >> (4 * (i - 16))) & 15u);
+            ok &= nbl != 0 && (nbl & (nbl - 1)) == 0;
+            k = (k << 2) | (unsigned long long)((nbl >> 1) - (nbl >> 3)); // 1, 2, 4, 8 -> 0..3
+        }
+    }
+    key = k;
+    return ok;
+}
 
 // Base-set nibbles of the flat query array (eight codes per word; words past
 // the end hold zero nibbles).
@@ -780,7 +810,7 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
                                                     uint32_t min_len, uint4 *__restrict__ slots,
                                                     uint32_t *__restrict__ mem_cnt, uint8_t *__restrict__ status,
                                                     uint32_t *__restrict__ defer_list,
-                                                    MemCounters *__restrict__ ctr) {
+                                                    MemCounters *__restrict__ ctr, LsiView lv, uint32_t lsik) {
     static_assert((PR & (PR - 1)) == 0, "ring size must be a power of two");
     __shared__ uint32_t s_lo[PR][PB], s_hi[PR][PB], s_rl[PR][PB];
     constexpr uint32_t RM = PR - 1;
@@ -789,10 +819,13 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
     __shared__ uint32_t s_clane[16];
     // per wave: the unclaimed rest [s_qb, s_qb + s_ql) of its chunk of work indices
     __shared__ uint32_t s_qb[PB / 64], s_ql[PB / 64];
+    __shared__ float s_lp[3 * kLsiMaxHidden + 1]; // stage 1 of the learned index
     for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) {
         s_compat[i] = cst->compat_lanes[i];
         s_expand[i] = cst->seed_expand[i];
     }
+    if (lsik)
+        for (uint32_t i = threadIdx.x; i < 3 * lv.md.hidden + 1; i += blockDim.x) s_lp[i] = lv.md.p[i];
     if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
     if (threadIdx.x < PB / 64) {
         s_qb[threadIdx.x] = 0;
@@ -817,6 +850,13 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
     uint64_t slot_base = 0, ient = 0, iend = 0;
     const uint8_t *p = q;
     uint4 piv = make_uint4(0, 0, 0, 0);
+    // learned-seed lookup of q[j-K..j): key, inclusive search range (slots <
+    // 2^32), fence level or key-round part (2: read the next part), keys below
+    // the key so far.  They alias state that is dead from SM_CAND until the
+    // lookup ends (seed entries and windows, the right end), which keeps the
+    // kernel within its register budget.
+    uint64_t &lkey = ient;
+    uint32_t &llo = xi, &lhi = lslmax, &llv = sidx, &lbelow = e;
 
     for (;;) {
         // ---- control: move this lane to a memory state (or SM_DONE) ----
@@ -869,6 +909,18 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
                     st = SM_FETCH;
                     continue;
                 }
+                // learned seed of q[j-K..j) when all K codes are bases; tried
+                // only on windows past the code the last MEM stopped at (E),
+                // which hold no known mismatch (a window over it is mostly
+                // absent, and an absent key costs the whole lookup)
+                st = SM_RESTART;
+                unsigned long long key = 0;
+                if (lsik && j >= lsik && (E == 0 || j - lsik > E) &&
+                    lsi_window_key(qnib, (uint64_t)(p - q) + j - lsik, lsik, key)) {
+                    lkey = key;
+                    st = SM_LSI_LEAF;
+                }
+            } else if (st == SM_RESTART) {
                 // up to three ambiguity codes (64 expansions, one record load
                 // each); windows with more restart unseeded
                 bool usable = false;
@@ -971,9 +1023,48 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
         // The record load is issued first and consumed after the probes, so
         // its latency overlaps theirs instead of stalling the wave on its own.
         const bool srec = st == SM_SEEDR, stext = st == SM_TSA || st == SM_TEXT;
+        const bool lsis = st >= SM_LSI_LEAF && st <= SM_LSI_PAY;
         uint4 R = make_uint4(0u, 0u, 0u, 0u);
+        bool lhit = false;
         if (srec) {
             R = sv.rec[sidx];
+        } else if (st == SM_LSI_LEAF) {
+            R = *reinterpret_cast<const uint4 *>(lv.leaves + lsi_leaf(lv.md, lkey));
+            probes++;
+        } else if (st == SM_LSI_FENCE) {
+            const int sh = lsi_fence_shift((int)llv);
+            const unsigned long long *f = lv.fc.f[0];
+#pragma unroll
+            for (uint32_t v = 1; v < kLsiMaxFences; v++) f = llv == v ? lv.fc.f[v] : f;
+            const long long lo = llo, hi = lhi;
+            const long long jj = lsi_last_below(f, (lo >> sh) + 1, hi >> sh, lo >> sh, lkey, probes);
+            llo = (uint32_t)(lo > (jj << sh) ? lo : (jj << sh));
+            lhi = (uint32_t)(hi < ((jj + 1) << sh) ? hi : ((jj + 1) << sh));
+        } else if (st == SM_LSI_KEYS) {
+            // part llv of the last round over [llo, lhi] (at most 33 keys): the
+            // 18 keys from (llo & ~1) + 18 llv, read as nine 16-byte pairs
+            const long long b = (long long)(llo & ~1u) + 18 * (long long)llv, hi = lhi, lo = llo;
+            const ulonglong2 *kp = reinterpret_cast<const ulonglong2 *>(lv.keys + b);
+            uint32_t below = 0, inr = 0;
+#pragma unroll
+            for (int u = 0; u < 9; u++) {
+                const long long a0 = b + 2 * u;
+                ulonglong2 v = make_ulonglong2(~0ull, ~0ull);
+                if (a0 <= hi) v = kp[u];
+                const bool in0 = a0 >= lo && a0 <= hi, in1 = a0 + 1 >= lo && a0 + 1 <= hi;
+                below += (in0 && v.x < lkey ? 1u : 0u) + (in1 && v.y < lkey ? 1u : 0u);
+                inr += (in0 ? 1u : 0u) + (in1 ? 1u : 0u);
+                lhit |= (in0 && v.x == lkey) || (in1 && v.y == lkey);
+            }
+            const long long last = b + 17 < hi ? b + 17 : hi;
+            probes += (uint32_t)((((last - b) / 2 + 1) * 16 + 63) / 64);
+            lbelow += below;
+            // inr - below keys of this part are >= the key: the search ends here
+            // unless every key read was below it and the range goes on
+            if (!lhit && below == inr && hi > b + 17) llv = 2; // read the next part
+        } else if (st == SM_LSI_PAY) {
+            R = lv.pay[llo];
+            probes++;
         } else if (st == SM_TSA) {
             R.x = fwd.sa_full[piv.x];
         } else if (st == SM_TEXT) {
@@ -982,7 +1073,7 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
             R = make_uint4(tw.x, tw.y, qw.x, qw.y);
         }
         bool ovf = false;
-        if (!srec && !stext) {
+        if (!srec && !stext && !lsis) {
             const bool left = st != SM_RIGHT;
             const uint32_t size = piv.y - piv.x;
             const uint32_t xlo = left ? piv.x : piv.z;
@@ -1068,6 +1159,52 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
             }
             e += adv;
             if (!go || e >= m) st = SM_EMIT;
+        } else if (st == SM_LSI_LEAF) {
+            LsiLeaf lf;
+            lf.s0 = R.x;
+            lf.err = R.y;
+            lf.a = __uint_as_float(R.z);
+            lf.b = __uint_as_float(R.w);
+            const long long pred = lsi_pred(lf, lsi_stage1(s_lp, lv.md.hidden, lsi_u(lv.md, lkey)));
+            long long lo = pred - (long long)lf.err, hi = pred + (long long)lf.err;
+            lo = lo < 0 ? 0 : lo;
+            hi = hi >= (long long)lv.md.nkeys ? (long long)lv.md.nkeys - 1 : hi;
+            llo = (uint32_t)lo;
+            lhi = (uint32_t)(hi < 0 ? 0 : hi);
+            llv = lv.md.nfence - 1;
+            st = lo <= hi ? SM_LSI_FENCE : SM_RESTART;
+        } else if (st == SM_LSI_FENCE) {
+            if (llv == 0) {
+                lbelow = 0;
+                st = SM_LSI_KEYS;
+            } else {
+                llv--;
+            }
+        } else if (st == SM_LSI_KEYS) {
+            if (lhit) {
+                llo += lbelow; // the key's slot
+                st = SM_LSI_PAY;
+            } else if (llv == 2) {
+                llv = 1;
+            } else {
+                st = SM_RESTART; // absent: the restart falls back
+            }
+        } else if (st == SM_LSI_PAY) {
+            if (R.w & kLsiFlagAmbiguous) {
+                st = SM_RESTART; // also compatible with ambiguity-code windows
+            } else { // the set of q[j-K..j) is this one interval
+                s_lo[head][tid] = R.x;
+                s_hi[head][tid] = R.y;
+                s_rl[head][tid] = R.z;
+                ns = 1; nn = 0; t = 0;
+                k = j - lsik;
+                if (k == 0) {
+                    st = SM_ENDLEFT;
+                } else {
+                    piv = make_uint4(R.x, R.y, R.z, 0u);
+                    st = SM_LEFT;
+                }
+            }
         } else if (ovf) {
             status[w] = 1;
             mem_cnt[w] = 0;
@@ -1764,6 +1901,18 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         else
             svp.F = 0; // min_len 12: no seeding in the persistent kernel
         svw = svp;
+        // restart seeding chosen by hsfm_mem_seed_mode
+        LsiView lv{};
+        uint32_t lsik = 0;
+        const int smode = idx->mem_seed_mode;
+        if (smode == HSFM_SEED_LEARNED || smode == HSFM_SEED_LEARNED_TABLES) {
+            if ((rc = lsi_mem_view(idx->mem_lsi, idx, &lv))) return rc;
+            lsik = lsi_k(idx->mem_lsi);
+        }
+        if (smode == HSFM_SEED_PLAIN || smode == HSFM_SEED_LEARNED) {
+            svp.F = 0; // restarts not seeded by the tables
+            svw = svp;
+        }
         const bool old_thread = path && strcmp(path, "thread") == 0;
         uint64_t hd[16];
         if ((rc = event_record(idx, 0, false))) return rc;
@@ -1785,7 +1934,7 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
                                idx->d_const.as<DevConst>(), svp, d_q, ws.qnib.as<uint32_t>(), d_qoff,
                                (const uint32_t *)nullptr, (uint32_t)W,
                                items, min_len, ws.mem_slots.as<uint4>(), ws.mem_cnt.as<uint32_t>(),
-                               ws.mem_status.as<uint8_t>(), ws.ovf_list.as<uint32_t>(), ctr);
+                               ws.mem_status.as<uint8_t>(), ws.ovf_list.as<uint32_t>(), ctr, lv, lsik);
         }
         HSFM_HIP(hipGetLastError());
         if ((rc = event_record(idx, 0, true))) return rc;
@@ -2159,17 +2308,17 @@ static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, cons
         if (main_kernel == BM_PERSIST8)
             hipLaunchKernelGGL((k_mem_persist<8, 256>), dim3(persist_grid(idx, k_mem_persist<8, 256>, 256)), dim3(256),
                                0, s, idx->fwd.view, idx->rev.view, cst, svp, d_q, qnib, bq, (const uint32_t *)nullptr, W,
-                               items, min_len, slots, cnt, st, bb.l1.as<uint32_t>(), &c[0]);
+                               items, min_len, slots, cnt, st, bb.l1.as<uint32_t>(), &c[0], LsiView{}, 0u);
         else if (main_kernel == BM_PERSIST32)
             hipLaunchKernelGGL((k_mem_persist<32, 128>), dim3(persist_grid(idx, k_mem_persist<32, 128>, 128)),
                                dim3(128), 0, s, idx->fwd.view, idx->rev.view, cst, svp, d_q, qnib, bq,
                                (const uint32_t *)nullptr, W, items, min_len, slots, cnt, st, bb.l1.as<uint32_t>(),
-                               &c[0]);
+                               &c[0], LsiView{}, 0u);
         else
             hipLaunchKernelGGL((k_mem_persist<16, 256>), dim3(persist_grid(idx, k_mem_persist<16, 256>, 256)),
                                dim3(256), 0, s, idx->fwd.view, idx->rev.view, cst, svp, d_q, qnib, bq,
                                (const uint32_t *)nullptr, W, items, min_len, slots, cnt, st, bb.l1.as<uint32_t>(),
-                               &c[0]);
+                               &c[0], LsiView{}, 0u);
         HSFM_HIP(hipGetLastError());
         // deferred reads: kDeferTB threads each, sets in LDS
         hipLaunchKernelGGL((k_mem_wave<true, kDeferTB>),
@@ -2570,6 +2719,21 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     release_graphs();
     if (st) *st = stats;
     return cleanup(rc);
+}
+
+extern "C" int hsfm_mem_seed_mode(hsfm_index *idx, int mode, hsfm_lsi *lsi) {
+    if (!idx) return set_error(HSFM_EINVAL, "hsfm_mem_seed_mode: null index");
+    if (mode < HSFM_SEED_TABLES || mode > HSFM_SEED_LEARNED_TABLES)
+        return set_error(HSFM_EINVAL, "hsfm_mem_seed_mode: unknown mode %d", mode);
+    HSFM_LOCK_INDEX(idx, "hsfm_mem_seed_mode");
+    if (mode == HSFM_SEED_LEARNED || mode == HSFM_SEED_LEARNED_TABLES) {
+        LsiView lv;
+        int rc = lsi_mem_view(lsi, idx, &lv);
+        if (rc) return rc;
+    }
+    idx->mem_seed_mode = mode;
+    idx->mem_lsi = (mode == HSFM_SEED_LEARNED || mode == HSFM_SEED_LEARNED_TABLES) ? lsi : nullptr;
+    return HSFM_OK;
 }
 
 extern "C" int hsfm_last_deferred(hsfm_index *idx, uint32_t *ids, uint64_t cap, uint64_t *count) {

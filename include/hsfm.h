@@ -37,6 +37,7 @@ extern "C" {
 
 typedef struct hsfm_device hsfm_device; /* GpuContext (src/gpu/mod.rs:19-66)       */
 typedef struct hsfm_index hsfm_index;   /* device-resident FmIndex / BidirFmIndex */
+typedef struct hsfm_lsi hsfm_lsi;       /* learned seed index of an hsfm_index     */
 
 /*
  * Borrowed view of one FmIndex, field for field as serialized by
@@ -254,6 +255,22 @@ typedef struct {
                                   resolve reads (0: LF walks to the samples)      */
 } hsfm_stats;
 int hsfm_last_stats(hsfm_index *idx, hsfm_stats *out);
+/* Seeding of the restarts of the one-shot MEM calls on idx (hsfm_find_mems_batch
+ * / _device); every mode gives the same results.  HSFM_SEED_TABLES: packed
+ * 12-/14-mer tables built from idx (default).  HSFM_SEED_PLAIN: plain backward
+ * search from the full interval, the reference's find_smems restart
+ * (src/fm_index/smem.rs:154-222).  HSFM_SEED_LEARNED: the interval of
+ * q[j-k..j) from the learned seed index lsi (built on idx, k <= 25) where that
+ * k-mer is all bases and its IUPAC set is its exact interval, else plain
+ * search; HSFM_SEED_LEARNED_TABLES: the same with the tables as fallback.  The
+ * learned modes keep lsi attached until it is freed (which restores
+ * HSFM_SEED_TABLES).  The batched pipeline always seeds from the tables.
+ * No counterpart in the reference. */
+#define HSFM_SEED_TABLES 0
+#define HSFM_SEED_PLAIN 1
+#define HSFM_SEED_LEARNED 2
+#define HSFM_SEED_LEARNED_TABLES 3
+int hsfm_mem_seed_mode(hsfm_index *idx, int mode, hsfm_lsi *lsi);
 /* Reads whose work items the main kernel of the last one-shot MEM call on idx
  * (hsfm_find_mems_batch / _device) deferred to the workgroup passes, ascending
  * and distinct: min(cap, *count) ids are written.  No counterpart in the
@@ -267,7 +284,6 @@ int hsfm_last_deferred(hsfm_index *idx, uint32_t *ids, uint64_t cap, uint64_t *c
  * search, extend_left over the k-mer (bidir.rs:112-130), as LookupTable::build
  * computes it for depth-k prefixes (lookup.rs:37-91).  Keys pack the k-mer
  * two bits per base, first base most significant (A=0, C=1, G=2, T=3). */
-typedef struct hsfm_lsi hsfm_lsi;
 typedef struct {
     uint32_t k;           /* k-mer length, 1..31                            */
     uint32_t leaves;      /* stage-2 leaves, a power of two (4096): leaf =
@@ -288,6 +304,11 @@ typedef struct {
     double mean_window;   /* mean search window 2*err+1 over the keys       */
     double mse_initial, mse_final; /* stage-1 training loss before / after  */
     double enum_ms, train_ms, fit_ms;
+    uint64_t flagged;     /* k-mers also compatible with a text window holding
+                             an ambiguity code (their seeds fall back)       */
+    uint32_t mem_seeding; /* 1: usable for MEM restarts (hsfm_mem_seed_mode) */
+    uint32_t pad2;
+    double flag_ms;
 } hsfm_lsi_info;
 typedef struct {
     double kernel_ms;     /* HIP events around the kernel                  */
