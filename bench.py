@@ -32,7 +32,9 @@ import argparse
 import hashlib
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -42,6 +44,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SMEM+locate reads/s (150bp, 100Mbp ref) bit-exact; learned-seed vs plain speedup"
 HBM_PEAK_GBS = 8000.0
+BYTES_DEFINITION = ("64 B x the random accesses of the kernel's own algorithm, counted on the device: Occ record "
+                    "probes (one per interval step, two when the interval's ends lie in different records), seed "
+                    "records and entries, learned-index rounds (64-byte units), and the suffix-array and text "
+                    "loads that finish single-row sets; this is fewer than the reference algorithm's probes "
+                    "(reference_algorithm_probes_per_read), so the bytes per read fall as work is removed")
 
 
 def log(*a):
@@ -73,7 +80,9 @@ def parse():
     ap.add_argument("--seed-keys", type=int, default=4_000_000,
                     help="read 21-mers for the learned-seed vs plain-search leg (0 skips it)")
     ap.add_argument("--learned-1g", action="store_true",
-                    help="run the 1 Gbp learned-seed workload instead (host index build takes minutes)")
+                    help="run only the 1 Gbp learned-seed workload (host index build takes minutes)")
+    ap.add_argument("--no-1g", action="store_true",
+                    help="leave the 1 Gbp learned-seed workload out of the default run")
     ap.add_argument("--big-ref-bp", type=int, default=1_000_000_000,
                     help="reference length of the --learned-1g workload")
     ap.add_argument("--big-reads", type=int, default=4_000_000,
@@ -115,9 +124,25 @@ def workload(args):
     return d
 
 
+ORACLE_BUILD = "oracle/liboracle.so (gcc -O3 -march=x86-64-v2)"
+
+
+def oracle_module():
+    """The oracle, compiled with -march=native for this host on first use."""
+    global ORACLE_BUILD
+    from oracle import oracle as O
+    if O._lib is None:
+        try:
+            O.use_native_build(os.path.join(tempfile.gettempdir(), "hsfm_oracle_native"))
+            ORACLE_BUILD = "gcc -O3 -march=native, compiled on this host"
+        except (OSError, subprocess.CalledProcessError) as e:
+            log(f"native oracle build failed ({e}); using oracle/liboracle.so")
+    return O
+
+
 def cpu_baseline(idx_bytes, sq, so, lq, lo, bounds, args, gpu_smem_sample, gpu_loc_sample):
     """Oracle leg: CPU restatement timed on a bounded sample + parity diff."""
-    from oracle import oracle as O
+    O = oracle_module()
     ob = O.OracleBidir.from_bytes(idx_bytes)
     cores = args.threads
     res = {}
@@ -154,40 +179,50 @@ def cpu_baseline(idx_bytes, sq, so, lq, lo, bounds, args, gpu_smem_sample, gpu_l
     res["parity_sample"] = {"smem_reads": n, "locate_reads": nl}
     if args.no_cpu:
         return res
-    # timing: bounded samples sized to ~cpu-seconds of work in total
-    def timed(fn, nreads):
-        t0 = time.perf_counter()
-        fn(nreads)
-        return time.perf_counter() - t0
+    # timing, as the reference's bench_utils.rs:34-43: one warm-up, then the
+    # mean of three timed runs; the sample is sized so that the four runs of
+    # each leg and thread count take about cpu-seconds / 4 together
+    host = host_info(cores)
 
-    def smem_run(k):
+    def smem_run(k, th):
         # find_smems + locate of the first min(size, max_hits) rows of each interval,
         # the same work the GPU pipeline does (find -> resolve -> ref map)
         mems = ob.find_mems_batch(qflat=sq[:int(so[k])], qoff=so[:k + 1], min_len=args.min_len, mode=0,
-                                  threads=cores)
+                                  threads=th)
         rows = [np.arange(lo_, lo_ + min(hi_ - lo_, args.max_hits), dtype=np.uint32)
                 for ms in mems for m in ms for lo_, hi_, _, _ in m.ivs.tolist()]
         if rows:
-            pos = ob.fwd.resolve_rows(np.concatenate(rows), threads=cores)
+            pos = ob.fwd.resolve_rows(np.concatenate(rows), threads=th)
             np.searchsorted(bounds, pos, side="right")
 
-    def loc_run(k):
-        ob.fwd.locate_batch(qflat=lq[:int(lo[k])], qoff=lo[:k + 1], threads=cores)
+    def loc_run(k, th):
+        ob.fwd.locate_batch(qflat=lq[:int(lo[k])], qoff=lo[:k + 1], threads=th)
 
-    budget = args.cpu_seconds / 2
-    k = min(len(so) - 1, 256 * cores)
-    t = timed(smem_run, k)
-    k2 = int(min(len(so) - 1, max(k, k * budget / max(t, 1e-3))))
-    t2 = timed(smem_run, k2)
-    res["smem"] = {"value": k2 / t2, "unit": "reads/s", "cores": cores, "kind": "port",
-                   "sample": f"first {k2} reads of the SMEM batch: oracle find_smems + resolve of the reported rows "
-                             f"+ ref map, {cores} threads"}
-    k = min(len(lo) - 1, 1024 * cores)
-    t = timed(loc_run, k)
-    k2 = int(min(len(lo) - 1, max(k, k * budget / max(t, 1e-3))))
-    t2 = timed(loc_run, k2)
-    res["locate"] = {"value": k2 / t2, "unit": "reads/s", "cores": cores, "kind": "port",
-                     "sample": f"first {k2} reads of the locate batch, oracle FmIndex::locate, {cores} threads"}
+    def measure(fn, nmax, k0, th):
+        budget = args.cpu_seconds / 16.0  # per run: 4 runs x 2 thread counts x 2 legs
+        t0 = time.perf_counter()
+        fn(k0, th)  # calibration run, also the warm-up
+        t = time.perf_counter() - t0
+        k = int(min(nmax, max(k0, k0 * budget / max(t, 1e-3))))
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            fn(k, th)
+            ts.append(time.perf_counter() - t0)
+        return k, k / float(np.mean(ts))
+
+    for leg, fn, nmax, k0, what in (
+            ("smem", smem_run, len(so) - 1, 64, "reads of the SMEM batch: oracle find_smems + resolve of the "
+                                                "reported rows + ref map"),
+            ("locate", loc_run, len(lo) - 1, 1024, "reads of the locate batch: oracle FmIndex::locate")):
+        k1, v1 = measure(fn, nmax, k0, 1)
+        kn, vn = measure(fn, nmax, k0 * cores, cores)
+        res[leg] = {"value": vn, "unit": "reads/s", "cores": cores, "kind": "port",
+                    "sample": f"first {kn} {what}, {cores} threads; mean of 3 runs after 1 warm-up",
+                    "single_thread": {"value": v1, "unit": "reads/s", "cores": 1,
+                                      "sample": f"first {k1} {what}, 1 thread (the reference's query paths are "
+                                                "single-threaded); mean of 3 runs after 1 warm-up"},
+                    "host": host}
     # algorithmic probes of the reference algorithm on a small sample
     O.probes_enable(True)
     ns = min(300, len(so) - 1)
@@ -202,6 +237,22 @@ def cpu_baseline(idx_bytes, sq, so, lq, lo, bounds, args, gpu_smem_sample, gpu_l
     res["locate_search_probes_per_read"] = p["occ_probes"] / nls
     O.probes_enable(False)
     return res
+
+
+def host_info(threads):
+    """The GPU box's host as the CPU baseline saw it."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                model = ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return {"cpu_model": model, "host_logical_cpus": os.cpu_count(), "threads_used": threads,
+            "threads_note": "the all-core runs use the box's CPU share (16 threads for one GPU), not every "
+                            "logical CPU the host shows",
+            "oracle_build": ORACLE_BUILD}
 
 
 def random_record_ceiling(achieved_gbps):
@@ -329,8 +380,20 @@ def uneven_leg(bidir, wl, args):
         walls.append(time.perf_counter() - t0)
     w = float(np.mean(walls))
     log(f"uneven: {nq} reads, {w * 1e3:.1f} ms/pass, stats {stats}, tuning {tuned['tune_ms']}")
+    # the one-shot call on the same reads, for the cost of batching
+    one = lambda: bidir.find_mems_device(d_q.data_ptr(), d_o.data_ptr(), nq, 19, hf.HSFM_MODE_SMEM,
+                                         d_rb.data_ptr(), len(bounds), 1024)
+    one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one()
+    torch.cuda.synchronize()
+    w1 = (time.perf_counter() - t0) / args.steps
+    log(f"uneven one-shot: {w1 * 1e3:.1f} ms")
     return {"reads": nq, "mean_len": round(float(off[-1]) / nq, 1), "min_len": 19, "batch_reads": 65536,
             "value": round(nq / w, 1), "unit": "reads/s", "ms_per_pass": round(w * 1e3, 2),
+            "one_shot_ms": round(w1 * 1e3, 2), "batched_over_one_shot": round(w / w1, 3),
             "graph_ms": round(stats["total_ms"], 2), "batches": stats["batches"],
             "fallback_batches": stats["fallback_batches"], "records": stats["records"],
             "positions": stats["positions"],
@@ -341,9 +404,72 @@ def uneven_leg(bidir, wl, args):
                       "tuning_wall_s": round(tune_wall, 2)}}
 
 
-def learned_seed_leg(bidir, flat, off, args, leaves=None, keys=None):
+def _digest_mem_result(bidir, nq, r):
+    """sha1 of a device-resident MEM result (hit offsets, records, positions)."""
+    import ctypes as C
+    import haystackfm_amd as hf
+    L = hf._lib()
+    dev = bidir.fwd._dev[0]._h
+    nh, npos, d_ho, d_hits, d_pos = r
+    h = hashlib.sha1()
+    for ptr, nbytes in ((d_ho, 8 * (nq + 1)), (d_hits, 32 * nh), (d_pos, 8 * npos)):
+        buf = np.zeros(max(nbytes, 1), np.uint8)
+        if nbytes:
+            hf._check(L.hsfm_device_to_host(dev, buf.ctypes.data, C.c_void_p(ptr), nbytes), "d2h")
+        h.update(buf[:nbytes].tobytes())
+    return h.hexdigest()
+
+
+def smem_seed_modes_leg(bidir, d_sq, d_so, nq, d_rb, nb, min_len, lsi, args):
+    """The SMEM pipeline (find -> resolve -> ref map) over the same reads with
+    each restart seeding: plain backward search from the full interval, the
+    packed 12/14-mer tables, learned 21-mer seeds with plain search as the
+    fallback, learned seeds with the tables as the fallback.  Every mode's
+    result is digested and must be identical."""
+    import torch
+    import haystackfm_amd as hf
+    modes = [("plain", hf.HSFM_SEED_PLAIN, None), ("tables", hf.HSFM_SEED_TABLES, None),
+             ("learned", hf.HSFM_SEED_LEARNED, lsi), ("learned_tables", hf.HSFM_SEED_LEARNED_TABLES, lsi)]
+    out, digests = {}, set()
+    for name, mode, l in modes:
+        bidir.set_seed_mode(mode, l)
+
+        def step():
+            return bidir.find_mems_device(d_sq.data_ptr(), d_so.data_ptr(), nq, min_len, hf.HSFM_MODE_SMEM,
+                                          d_rb.data_ptr(), nb, args.max_hits)
+        for _ in range(max(1, args.warmup)):
+            r = step()
+        digests.add(_digest_mem_result(bidir, nq, r))
+        torch.cuda.synchronize()
+        kms, probes = [], 0
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+            st = bidir.last_stats()
+            kms.append(st["main_kernel_ms"])
+            probes = st["occ_probes"]
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        out[name] = {"ms_per_step": round(ms, 3), "reads_per_s": round(nq / (ms * 1e-3), 1),
+                     "main_kernel_ms": round(float(np.mean(kms)), 3), "loads_per_read": round(probes / nq, 2),
+                     "deferred_reads": int(st["fallback_reads"]), "capacity_retries": int(st["capacity_retries"]),
+                     "deferred_ms": round(st["fallback_kernel_ms"], 3)}
+        log(f"SMEM seeding {name}: {ms:.2f} ms/step, main kernel {np.mean(kms):.2f} ms, {probes / nq:.1f} loads/read, "
+            f"stats {st}")
+    bidir.set_seed_mode(hf.HSFM_SEED_TABLES)
+    same = len(digests) == 1
+    out["identical_results"] = same
+    if same:
+        out["learned_vs_plain"] = round(out["plain"]["ms_per_step"] / out["learned"]["ms_per_step"], 3)
+        out["learned_tables_vs_tables"] = round(out["tables"]["ms_per_step"] / out["learned_tables"]["ms_per_step"], 3)
+        out["learned_tables_vs_plain"] = round(out["plain"]["ms_per_step"] / out["learned_tables"]["ms_per_step"], 3)
+    return out
+
+
+def learned_seed_leg(bidir, flat, off, args, leaves=None, keys=None, keep=None):
     """Learned seed index vs plain exact search on the same read k-mers: both
-    on the device, outputs compared bit for bit, kernel time by HIP events."""
+    on the device, outputs compared bit for bit, kernel time by HIP events.
+    keep (a list) receives the learned index, for the SMEM seeding leg."""
     import torch
     import haystackfm_amd as hf
     k = 21
@@ -354,6 +480,8 @@ def learned_seed_leg(bidir, flat, off, args, leaves=None, keys=None):
     lsi = hf.LearnedSeedIndex(bidir, k=k, leaves=leaves, hidden=16, train_steps=400, sample=1 << 16)
     build_s = time.time() - t0
     info = lsi.info()
+    if keep is not None:
+        keep.append(lsi)
     dq = torch.from_numpy(keys.view(np.int64)).cuda()
     d_plain = torch.empty((len(keys), 4), dtype=torch.int32, device="cuda")
     d_lsi = torch.empty_like(d_plain)
@@ -380,7 +508,9 @@ def learned_seed_leg(bidir, flat, off, args, leaves=None, keys=None):
             "max_err": int(info["max_err"]), "mean_window": round(info["mean_window"], 2),
             "train_mse": [info["mse_initial"], info["mse_final"]],
             "build_s": round(build_s, 2), "enum_ms": round(info["enum_ms"], 1),
-            "train_ms": round(info["train_ms"], 1), "fit_ms": round(info["fit_ms"], 1)}
+            "train_ms": round(info["train_ms"], 1), "fit_ms": round(info["fit_ms"], 1),
+            "fence_levels": int(info["fence_levels"]), "flagged_keys": int(info["flagged"]),
+            "flag_ms": round(info["flag_ms"], 1), "mem_seeding": bool(info["mem_seeding"])}
 
 
 def workload_1g(args):
@@ -431,8 +561,13 @@ def exact_kmer_interval(O, ob, codes):
 
 
 def main_1g(args):
+    print(json.dumps(run_1g(args)), flush=True)
+
+
+def run_1g(args):
     """Learned seed index vs plain exact search on the 1 Gbp workload, plus the
-    SMEM pipeline (min_len 21) over its reads; one JSON line."""
+    SMEM pipeline (min_len 21) over its reads with each restart seeding; the
+    JSON object of its line."""
     import torch
     import haystackfm_amd as hf
     bounds, idx, sq, so = workload_1g(args)
@@ -443,7 +578,8 @@ def main_1g(args):
     k = 21
     keys = read_window_keys(sq, so, k, args.seed_keys)
     log(f"{len(keys)} read 21-mers")
-    lseed = learned_seed_leg(bidir, sq, so, args, keys=keys)
+    keep = []
+    lseed = learned_seed_leg(bidir, sq, so, args, keys=keys, keep=keep)
     lseed_alt = (learned_seed_leg(bidir, sq, so, args, leaves=args.lsi_leaves_alt, keys=keys)
                  if args.lsi_leaves_alt else None)
 
@@ -473,9 +609,10 @@ def main_1g(args):
     smem_s = (time.perf_counter() - t0) / args.steps
     smem_stats = bidir.last_stats()
     log(f"SMEM step {smem_s * 1e3:.1f} ms, {r[0]} MEM records, {r[1]} positions")
+    seeding = smem_seed_modes_leg(bidir, d_sq, d_so, nq, d_rb, len(bounds), min_len, keep[0], args)
 
     # parity samples: SMEM lists and 21-mer intervals against the oracle
-    from oracle import oracle as O
+    O = oracle_module()
     ob = O.OracleBidir.from_bytes(idx)
     del idx
     n = min(args.check_reads, nq)
@@ -564,11 +701,12 @@ def main_1g(args):
         "parity_sample": {"smem_reads": n, "kmers": int(len(ks))},
         "learned_seed": lseed,
         "learned_seed_more_leaves": lseed_alt,
+        "smem_seeding": seeding,
         "smem": {"value": round(nq / smem_s, 1) if ok else None, "unit": "reads/s", "ms_per_step": round(smem_s * 1e3, 3),
                  "main_kernel_ms": round(float(np.mean(kms)), 3),
                  "stats": {k_: (round(v, 3) if isinstance(v, float) else v) for k_, v in smem_stats.items()}},
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def random_access_calibration():
@@ -699,8 +837,11 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_mem_persist", args, nq, nl),
                 "kernel": "k_mem_persist", "kernel_ms": round(kms, 3),
                 "bytes_per_read": round(bytes_per_launch / nq, 1),
+                "bytes_definition": BYTES_DEFINITION,
                 "measured_random_record_ceiling": random_record_ceiling(achieved),
                 "cache": measured_cache("k_mem_persist", args, nq, nl)}
+        if roof["traffic"]:
+            roof["traffic_frac"] = round(roof["traffic"] / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     loc_roof = None
     lkms = float(np.mean(lk))
     if loc_stats.get("occ_probes") and lkms > 0:
@@ -709,8 +850,16 @@ def main():
                     "frac": round(a / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_locate_search", args, nq, nl),
                     "kernel": "k_locate_search", "kernel_ms": round(lkms, 3),
                     "bytes_per_read": round(loc_stats["occ_probes"] * 64.0 / nl, 1),
+                    "bytes_definition": BYTES_DEFINITION,
                     "measured_random_record_ceiling": random_record_ceiling(a),
                     "cache": measured_cache("k_locate_search", args, nq, nl)}
+        if loc_roof["traffic"]:
+            loc_roof["traffic_frac"] = round(loc_roof["traffic"] / (lkms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        if cpu.get("locate_search_probes_per_read"):
+            # the survey's definition: the reference algorithm's probes (counted by
+            # the oracle) x 64 B per read, over the whole locate step
+            b8 = cpu["locate_search_probes_per_read"] * 64.0 * nl
+            loc_roof["reference_algorithm_frac"] = round(b8 / loc_s / 1e9 / HBM_PEAK_GBS, 4)
     cpu_line = None
     if "smem" in cpu:
         cpu_line = cpu["smem"]
@@ -748,6 +897,13 @@ def main():
         "learned_seed_more_leaves": lseed_alt,
         "uneven_batched": uneven,
     }
+    if not args.no_1g:
+        del bidir
+        big = run_1g(args)
+        out["learned_seed_1g"] = {k: big.get(k) for k in ("config", "value", "ms_per_step", "roofline", "parity",
+                                                          "parity_sample", "cpu_baseline", "learned_seed",
+                                                          "smem_seeding", "smem")}
+        out["learned_seed_1g"]["value_metric"] = big.get("metric")
     print(json.dumps(out), flush=True)
 
 

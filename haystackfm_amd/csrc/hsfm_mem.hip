@@ -800,7 +800,7 @@ constexpr uint32_t kPersistChunk = 64; // work indices a wave claims per atomic
 // 256-thread blocks of the ring kernel fit three per CU by LDS; the second
 // launch bound keeps the registers within the same three waves per SIMD.
 
-template <int PR, int PB>
+template <int PR, int PB, bool LSI = false>
 __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fwd, DevDir rev, const DevConst *__restrict__ cst,
                                                     SeedView sv, const uint8_t *__restrict__ q,
                                                     const uint32_t *__restrict__ qnib,
@@ -824,7 +824,7 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
         s_compat[i] = cst->compat_lanes[i];
         s_expand[i] = cst->seed_expand[i];
     }
-    if (lsik)
+    if (LSI && lsik)
         for (uint32_t i = threadIdx.x; i < 3 * lv.md.hidden + 1; i += blockDim.x) s_lp[i] = lv.md.p[i];
     if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
     if (threadIdx.x < PB / 64) {
@@ -910,12 +910,13 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
                     continue;
                 }
                 // learned seed of q[j-K..j) when all K codes are bases; tried
-                // only on windows past the code the last MEM stopped at (E),
-                // which hold no known mismatch (a window over it is mostly
-                // absent, and an absent key costs the whole lookup)
+                // only where the window is likely to occur: on the first
+                // restart of a work item, and past the code the last MEM
+                // stopped at (E), so that it holds no known mismatch (an
+                // absent key costs the whole lookup)
                 st = SM_RESTART;
                 unsigned long long key = 0;
-                if (lsik && j >= lsik && (E == 0 || j - lsik > E) &&
+                if (LSI && lsik && j >= lsik && (E == 0 ? s0 == ia : j - lsik > E) &&
                     lsi_window_key(qnib, (uint64_t)(p - q) + j - lsik, lsik, key)) {
                     lkey = key;
                     st = SM_LSI_LEAF;
@@ -1023,15 +1024,15 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
         // The record load is issued first and consumed after the probes, so
         // its latency overlaps theirs instead of stalling the wave on its own.
         const bool srec = st == SM_SEEDR, stext = st == SM_TSA || st == SM_TEXT;
-        const bool lsis = st >= SM_LSI_LEAF && st <= SM_LSI_PAY;
+        const bool lsis = LSI && st >= SM_LSI_LEAF && st <= SM_LSI_PAY;
         uint4 R = make_uint4(0u, 0u, 0u, 0u);
         bool lhit = false;
         if (srec) {
             R = sv.rec[sidx];
-        } else if (st == SM_LSI_LEAF) {
+        } else if (LSI && st == SM_LSI_LEAF) {
             R = *reinterpret_cast<const uint4 *>(lv.leaves + lsi_leaf(lv.md, lkey));
             probes++;
-        } else if (st == SM_LSI_FENCE) {
+        } else if (LSI && st == SM_LSI_FENCE) {
             const int sh = lsi_fence_shift((int)llv);
             const unsigned long long *f = lv.fc.f[0];
 #pragma unroll
@@ -1040,7 +1041,7 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
             const long long jj = lsi_last_below(f, (lo >> sh) + 1, hi >> sh, lo >> sh, lkey, probes);
             llo = (uint32_t)(lo > (jj << sh) ? lo : (jj << sh));
             lhi = (uint32_t)(hi < ((jj + 1) << sh) ? hi : ((jj + 1) << sh));
-        } else if (st == SM_LSI_KEYS) {
+        } else if (LSI && st == SM_LSI_KEYS) {
             // part llv of the last round over [llo, lhi] (at most 33 keys): the
             // 18 keys from (llo & ~1) + 18 llv, read as nine 16-byte pairs
             const long long b = (long long)(llo & ~1u) + 18 * (long long)llv, hi = lhi, lo = llo;
@@ -1062,7 +1063,7 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
             // inr - below keys of this part are >= the key: the search ends here
             // unless every key read was below it and the range goes on
             if (!lhit && below == inr && hi > b + 17) llv = 2; // read the next part
-        } else if (st == SM_LSI_PAY) {
+        } else if (LSI && st == SM_LSI_PAY) {
             R = lv.pay[llo];
             probes++;
         } else if (st == SM_TSA) {
@@ -1159,7 +1160,7 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
             }
             e += adv;
             if (!go || e >= m) st = SM_EMIT;
-        } else if (st == SM_LSI_LEAF) {
+        } else if (LSI && st == SM_LSI_LEAF) {
             LsiLeaf lf;
             lf.s0 = R.x;
             lf.err = R.y;
@@ -1173,14 +1174,14 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
             lhi = (uint32_t)(hi < 0 ? 0 : hi);
             llv = lv.md.nfence - 1;
             st = lo <= hi ? SM_LSI_FENCE : SM_RESTART;
-        } else if (st == SM_LSI_FENCE) {
+        } else if (LSI && st == SM_LSI_FENCE) {
             if (llv == 0) {
                 lbelow = 0;
                 st = SM_LSI_KEYS;
             } else {
                 llv--;
             }
-        } else if (st == SM_LSI_KEYS) {
+        } else if (LSI && st == SM_LSI_KEYS) {
             if (lhit) {
                 llo += lbelow; // the key's slot
                 st = SM_LSI_PAY;
@@ -1189,7 +1190,7 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
             } else {
                 st = SM_RESTART; // absent: the restart falls back
             }
-        } else if (st == SM_LSI_PAY) {
+        } else if (LSI && st == SM_LSI_PAY) {
             if (R.w & kLsiFlagAmbiguous) {
                 st = SM_RESTART; // also compatible with ambiguity-code windows
             } else { // the set of q[j-K..j) is this one interval
@@ -1929,8 +1930,11 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
                                ws.mem_slots.as<uint4>(), ws.mem_cnt.as<uint32_t>(), ws.mem_status.as<uint8_t>(),
                                ws.ovf_list.as<uint32_t>(), ctr);
         } else {
-            const uint32_t g1 = persist_grid(idx, k_mem_persist<16, 256>, 256);
-            hipLaunchKernelGGL((k_mem_persist<16, 256>), dim3(g1), dim3(256), 0, s, idx->fwd.view, idx->rev.view,
+            // the learned-seed states are compiled into their own instance, so
+            // the other modes run the kernel without them
+            auto kern = lsik ? k_mem_persist<16, 256, true> : k_mem_persist<16, 256, false>;
+            const uint32_t g1 = persist_grid(idx, kern, 256);
+            hipLaunchKernelGGL(kern, dim3(g1), dim3(256), 0, s, idx->fwd.view, idx->rev.view,
                                idx->d_const.as<DevConst>(), svp, d_q, ws.qnib.as<uint32_t>(), d_qoff,
                                (const uint32_t *)nullptr, (uint32_t)W,
                                items, min_len, ws.mem_slots.as<uint4>(), ws.mem_cnt.as<uint32_t>(),
@@ -2036,8 +2040,10 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         // sets that outgrow the LDS are usually far larger: go straight to
         // 65536 intervals, then 4x per further pass
         cap = lds ? 65536 : cap * 4;
+        // grow the record buffer only when the retried reads may not fit it
+        // (a retry for set capacity alone keeps it)
         const uint64_t want = rec_used + nretry * 4096;
-        rec_cap = rec_cap * 2 > want ? rec_cap * 2 : want;
+        if (want > rec_cap) rec_cap = rec_cap * 2 > want ? rec_cap * 2 : want;
         // next pass runs only the retried reads
         uint32_t *next_list = retry;
         retry = (next_list == ws.ovf_list.as<uint32_t>()) ? ws.read_list.as<uint32_t>() : ws.ovf_list.as<uint32_t>();
@@ -2599,11 +2605,13 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     const uint32_t nbatches = nq ? (nq + B - 1) / B : 0;
     int nsets = 1;
     if (cfg->use_graph && nbatches > 1) {
-        const char *e = getenv("HSFM_BATCH_SETS");
-        const long v = e && *e ? strtol(e, nullptr, 10) : 3; // 3 measured best of 2-4
-        nsets = v < 1 ? 1 : v > KS ? KS : (int)v;
+        const int v = cfg->batch_sets > 0 ? cfg->batch_sets : 3; // 3 measured best of 1-4
+        nsets = v > KS ? KS : v;
         if ((uint32_t)nsets > nbatches) nsets = (int)nbatches;
     }
+    // (each batch keeps the whole persistent grid: a grid split between the
+    // batches in flight measured slower, 2M uneven reads: 54.5 ms with three
+    // thirds against 47 ms with three whole grids)
     auto release_graphs = [&]() {
         for (BatchGraph &g : gr) g.release();
     };

@@ -193,6 +193,8 @@ typedef struct {
     int resolve_fanout;    /* 0 = tuned/cached/default; else 1, 2 or 4 walks
                               per thread                                       */
     int use_graph;         /* capture each batch into one HIP graph            */
+    int batch_sets;        /* batches in flight, each with its own buffers,
+                              graph and stream (0 = 3, at most 4)            */
 } hsfm_batch_config;
 typedef struct {
     uint32_t batches, fallback_batches; /* batches, those re-run host-driven   */

@@ -39,15 +39,14 @@ def test_batched_matches_one_shot(uneven, main, fan, graph):
     assert st["records"] == len(ref[1]) and st["positions"] == len(ref[2])
 
 
-@pytest.mark.parametrize("sets", ["1", "2", "4"])
-def test_batches_in_flight(uneven, monkeypatch, sets):
-    """1, 2 or 4 batches in flight (their own buffers, graphs and streams):
-    the same results, handed over in batch order."""
+@pytest.mark.parametrize("sets", [1, 2, 3, 4])
+def test_batches_in_flight(uneven, sets):
+    """1 to 4 batches in flight (their own buffers, graphs and streams): the
+    same results, handed over in batch order."""
     o, g, (flat, off), bounds = uneven
     ref = g.find_mems_batch((flat, off), 19, hf.HSFM_MODE_SMEM, bounds, 1024)
-    monkeypatch.setenv("HSFM_BATCH_SETS", sets)
     got = g.find_mems_batched((flat, off), 19, hf.HSFM_MODE_SMEM, bounds, 1024, batch_reads=450,
-                              main_kernel=1, resolve_fanout=2)
+                              main_kernel=1, resolve_fanout=2, batch_sets=sets)
     _same(got[:3], ref)
     assert got[3]["batches"] == 7
 
