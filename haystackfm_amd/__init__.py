@@ -31,7 +31,7 @@ __all__ = [
     "FmIndex", "BidirFmIndex", "MemHit", "GpuContext", "FmIndexError", "GpuError",
     "DeserializeError", "Alphabet", "AlphabetFns", "IupacDna", "ExactDna", "DnaSequence",
     "locate_batch_gpu", "context_cache", "encode", "alphabet_fns_from_tag", "lib_path",
-    "LearnedSeedIndex", "kmer_keys", "kmer_search", "kmer_search_device", "lsi_gradient",
+    "LearnedSeedIndex", "kmer_keys", "kmer_search", "kmer_search_device", "lsi_gradient", "build_index_bytes_gpu",
     "reverse_complement", "reverse_complement_device",
 ]
 
@@ -302,6 +302,8 @@ def _lib():
         L.hsfm_last_stats.argtypes = [vp, C.POINTER(_Stats)]
         L.hsfm_last_deferred.argtypes = [vp, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint64)]
         L.hsfm_mem_seed_mode.argtypes = [vp, C.c_int, vp]
+        L.hsfm_build_index_device.argtypes = [vp, u8p, C.c_uint64, u32p, C.c_uint32, C.c_char_p, C.c_uint32,
+                                              C.c_uint32, C.c_uint32, C.c_uint8, C.c_int, C.POINTER(u8p), u64p]
         L.hsfm_lsi_build.argtypes = [vp, C.POINTER(_LsiConfig), C.POINTER(vp)]
         L.hsfm_lsi_get_info.argtypes = [vp, C.POINTER(_LsiInfo)]
         L.hsfm_lsi_seed_device.argtypes = [vp, vp, C.c_uint64, vp, C.POINTER(_SeedStats)]
@@ -832,6 +834,27 @@ def kmer_search_device(index, k: int, d_keys: int, n: int, d_out: int) -> dict:
     _check(_lib().hsfm_kmer_search_device(index._handle(), k, C.c_void_p(d_keys), n, C.c_void_p(d_out),
                                           C.byref(st)), "kmer_search_device")
     return {"kernel_ms": st.kernel_ms}
+
+
+def build_index_bytes_gpu(text, bounds, *, headers=None, sample_rate: int = 32, lookup_depth: int = 0,
+                          encoding: int = 0, tag: int = 0, bidir: bool = False,
+                          ctx: "GpuContext | None" = None) -> bytes:
+    """FmIndex::build_cpu + to_bytes (BidirFmIndex's with bidir=True) computed
+    on the device: the same byte stream (hsfm_build_index_device)."""
+    ctx = ctx or context_cache.get_or_init()
+    text = np.ascontiguousarray(text, np.uint8)
+    bounds = np.ascontiguousarray(bounds, np.uint32)
+    hb = b"".join(h.encode() + b"\0" for h in headers) if headers is not None else None
+    out = C.POINTER(C.c_uint8)()
+    n = C.c_uint64()
+    _check(_lib().hsfm_build_index_device(ctx._h, text.ctypes.data_as(C.POINTER(C.c_uint8)), len(text),
+                                          bounds.ctypes.data_as(C.POINTER(C.c_uint32)), len(bounds), hb,
+                                          sample_rate, lookup_depth, encoding, tag, 1 if bidir else 0,
+                                          C.byref(out), C.byref(n)), "build_index_device")
+    try:
+        return np.ctypeslib.as_array(out, shape=(n.value,)).tobytes() if n.value else b""
+    finally:
+        _lib().hsfm_free_result(C.cast(out, C.c_void_p))
 
 
 def reverse_complement(queries, ctx: "GpuContext | None" = None):

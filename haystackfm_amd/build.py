@@ -16,7 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhsfm.so")
 BUILD_LIB = os.path.join(HERE, "libhsfm_build.so")
 CALIB_LIB = os.path.join(HERE, "libhsfm_calib.so")  # measurement aid (random-load ceiling), not the query path
-SOURCES = ["hsfm_runtime.hip", "hsfm_index.hip", "hsfm_locate.hip", "hsfm_mem.hip", "hsfm_lsi.hip"]
+SOURCES = ["hsfm_runtime.hip", "hsfm_index.hip", "hsfm_locate.hip", "hsfm_mem.hip", "hsfm_lsi.hip",
+           "hsfm_construct.hip"]
 ARCH = os.environ.get("HSFM_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

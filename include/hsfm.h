@@ -90,6 +90,20 @@ int hsfm_index_upload(hsfm_device *dev, const hsfm_index_view *fwd, const hsfm_i
                       hsfm_index **out);
 int hsfm_index_free(hsfm_index *idx);
 
+/* FmIndex::build_cpu + FmIndex::to_bytes (src/fm_index/mod.rs:117-195,
+ * serialize.rs:13-26), or with bidir != 0 BidirFmIndex::build_cpu + to_bytes
+ * ([u32 LE fwd_len][fwd][rev], bidir_index.rs:53-80, 183-192), computed on the
+ * device: the same bytes, compacted Occ lanes included (the reference's GPU
+ * build leaves them out, src/occ/gpu.rs:94-105).  text: the concatenated codes
+ * with a 0 sentinel ending each sequence (concatenate_sequences,
+ * src/alphabet.rs:375-393), at most 2^31 - 2 codes; bounds[nseq]: exclusive
+ * ends; headers: '\0'-separated names or NULL ("seq_{i}" for empty ones);
+ * encoding 0 = Bitplane, 1 = OneHot; tag 0 = IupacDna, 1 = ExactDna.  *out is
+ * freed with hsfm_free_result. */
+int hsfm_build_index_device(hsfm_device *dev, const uint8_t *text, uint64_t n, const uint32_t *bounds,
+                            uint32_t nseq, const char *headers, uint32_t sample_rate, uint32_t lookup_depth,
+                            uint32_t encoding, uint8_t tag, int bidir, uint8_t **out, uint64_t *out_len);
+
 /* ---- batched locate ----------------------------------------------------- */
 typedef struct {
     uint32_t num_queries;
