@@ -111,8 +111,9 @@ def workload(args):
                                       iupac_frac=0.005)
     log(f"reference {len(text)} bp in {time.time() - t0:.1f}s")
     t0 = time.time()
-    idx = IB.build_bidir_bytes(text, bounds, sample_rate=32, lookup_depth=12, threads=args.threads)
-    log(f"bidirectional index {len(idx) / 1e6:.0f} MB in {time.time() - t0:.1f}s")
+    import haystackfm_amd as hf
+    idx = hf.build_index_bytes_gpu(text, bounds, sample_rate=32, lookup_depth=12, bidir=True)
+    log(f"bidirectional index {len(idx) / 1e6:.0f} MB built on the GPU in {time.time() - t0:.1f}s")
     sq, so = IB.synth_reads(0x5EED3, text, bounds, args.reads, 150, sub=0.02, indel=0.002, both_strands=True,
                             n_random=args.random_reads, gc=0.41, shuffle=True)
     lq, lo = IB.synth_reads(0x5EED2, text, bounds, args.locate_reads, 150, sub=0.01, both_strands=True)
@@ -533,8 +534,9 @@ def workload_1g(args):
                                       iupac_frac=0.002, iupac_codes=[5])
     log(f"reference {len(text)} bp in {time.time() - t0:.1f}s")
     t0 = time.time()
-    idx = IB.build_bidir_bytes(text, bounds, sample_rate=64, lookup_depth=0, threads=args.threads)
-    log(f"bidirectional index {len(idx) / 1e6:.0f} MB in {time.time() - t0:.1f}s")
+    import haystackfm_amd as hf
+    idx = hf.build_index_bytes_gpu(text, bounds, sample_rate=64, lookup_depth=0, bidir=True)
+    log(f"bidirectional index {len(idx) / 1e6:.0f} MB built on the GPU in {time.time() - t0:.1f}s")
     sq, so = IB.synth_reads(0x5EED5, text, bounds, args.big_reads, 150, sub=0.01, both_strands=True, shuffle=True)
     log(f"{args.big_reads} reads synthesised")
     del text

@@ -101,22 +101,32 @@ struct LaneMap {
 // (cnt[lane * nb + block]).  One wave per block.
 __global__ __launch_bounds__(256) void k_occ_counts(const uint8_t *__restrict__ bwt, uint32_t n, uint64_t nb,
                                                     LaneMap lm, uint32_t *__restrict__ cnt) {
-    const uint64_t b = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (b >= nb) return;
-    const uint64_t p = 64 * b + __lane_id();
-    const uint32_t lane = p < n ? lm.s2l[bwt[p]] : 0xffu;
-    for (uint32_t k = 0; k < lm.L; k++) {
-        const uint64_t m = __ballot(lane == k);
-        if (__lane_id() == 0) cnt[(uint64_t)k * nb + b] = (uint32_t)__popcll(m);
+    const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t b = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; b < nb; b += waves) {
+        const uint64_t p = 64 * b + __lane_id();
+        const uint32_t lane = p < n ? lm.s2l[bwt[p]] : 0xffu;
+        for (uint32_t k = 0; k < lm.L; k++) {
+            const uint64_t m = __ballot(lane == k);
+            if (__lane_id() == 0) cnt[(uint64_t)k * nb + b] = (uint32_t)__popcll(m);
+        }
     }
 }
+
+__device__ void occ_record(const uint8_t *__restrict__ bwt, uint32_t n, uint64_t nb, const LaneMap &lm,
+                           const uint32_t *__restrict__ cum, uint8_t *__restrict__ rec, uint64_t b);
 
 // Block records from the exclusive scans of the counts (cum[lane * nb + b]).
 __global__ __launch_bounds__(256) void k_occ_records(const uint8_t *__restrict__ bwt, uint32_t n, uint64_t nb,
                                                      LaneMap lm, const uint32_t *__restrict__ cum,
                                                      uint8_t *__restrict__ rec) {
-    const uint64_t b = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (b >= nb) return;
+    const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t b = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; b < nb; b += waves)
+        occ_record(bwt, n, nb, lm, cum, rec, b);
+}
+
+// The record of block b (all 64 lanes of the wave).
+__device__ void occ_record(const uint8_t *__restrict__ bwt, uint32_t n, uint64_t nb, const LaneMap &lm,
+                           const uint32_t *__restrict__ cum, uint8_t *__restrict__ rec, uint64_t b) {
     const uint64_t p = 64 * b + __lane_id();
     const uint32_t lane = p < n ? lm.s2l[bwt[p]] : 0xffu;
     uint64_t words[16];
@@ -166,10 +176,11 @@ __global__ __launch_bounds__(256) void k_occ_records(const uint8_t *__restrict__
 
 __global__ void k_occ_checkpoints(const uint32_t *__restrict__ cum, uint64_t nb, uint32_t L, uint64_t nsb,
                                   uint32_t *__restrict__ sb) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nsb * L) return;
-    const uint64_t s = i / L, k = i % L;
-    sb[i] = cum[k * nb + 8 * s];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nsb * L;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t s = i / L, k = i % L;
+        sb[i] = cum[k * nb + 8 * s];
+    }
 }
 
 __device__ __forceinline__ bool is_start(uint32_t v, const uint32_t *__restrict__ starts, uint32_t ns) {
@@ -186,28 +197,40 @@ __device__ __forceinline__ bool is_start(uint32_t v, const uint32_t *__restrict_
 __global__ __launch_bounds__(256) void k_sa_words(const uint32_t *__restrict__ sa, uint32_t n, uint64_t nw,
                                                   uint32_t rate, const uint32_t *__restrict__ starts, uint32_t ns,
                                                   unsigned long long *__restrict__ word, uint32_t *__restrict__ wcnt) {
-    const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (w >= nw) return;
-    const uint64_t i = 64 * w + __lane_id();
-    bool mk = false;
-    if (i < n) {
-        const uint32_t v = sa[i];
-        mk = v % rate == 0 || is_start(v, starts, ns);
-    }
-    const uint64_t m = __ballot(mk);
-    if (__lane_id() == 0) {
-        word[w] = m;
-        wcnt[w] = (uint32_t)__popcll(m);
+    const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nw; w += waves) {
+        const uint64_t i = 64 * w + __lane_id();
+        bool mk = false;
+        if (i < n) {
+            const uint32_t v = sa[i];
+            mk = v % rate == 0 || is_start(v, starts, ns);
+        }
+        const uint64_t m = __ballot(mk);
+        if (__lane_id() == 0) {
+            word[w] = m;
+            wcnt[w] = (uint32_t)__popcll(m);
+        }
     }
 }
+
+__device__ void sa_record(const uint32_t *__restrict__ sa, const unsigned long long *__restrict__ word,
+                          const uint32_t *__restrict__ wcum, uint8_t *__restrict__ rec, uint32_t *__restrict__ vals,
+                          uint64_t w);
 
 // 14-byte word records and the sampled values in row order.
 __global__ __launch_bounds__(256) void k_sa_records(const uint32_t *__restrict__ sa, uint32_t n, uint64_t nw,
                                                     const unsigned long long *__restrict__ word,
                                                     const uint32_t *__restrict__ wcum, uint8_t *__restrict__ rec,
                                                     uint32_t *__restrict__ vals) {
-    const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (w >= nw) return;
+    const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nw; w += waves)
+        sa_record(sa, word, wcum, rec, vals, w);
+}
+
+// The 14-byte record and sampled values of word w (all 64 lanes of the wave).
+__device__ void sa_record(const uint32_t *__restrict__ sa, const unsigned long long *__restrict__ word,
+                          const uint32_t *__restrict__ wcum, uint8_t *__restrict__ rec, uint32_t *__restrict__ vals,
+                          uint64_t w) {
     const uint32_t lid = __lane_id();
     const uint64_t m = word[w];
     const uint32_t base = wcum[w & ~7ull], cur = wcum[w];

@@ -285,165 +285,15 @@ __device__ __forceinline__ uint32_t locate_read(const DevDir &d, const DevConst 
 }
 
 // ---------------------------------------------------------------------------
-// Second register pass over the reads the first one gave up on: the same
-// backward search with room for C intervals per read.  Final sets of up to KR
-// intervals go to ivs as before; larger ones are published in row order to
-// fb_ivs (fb_meta = offset, count; niv = kNivOverflow).  Reads whose sets
-// outgrow C (or the published list) are queued for the workgroup kernel.
+// Second pass over the reads the register kernel gave up on, one group of G
+// lanes per read: the same backward search with room for G intervals, one per
+// lane, so a step costs one dependent round trip for the whole set.  Final
+// sets of up to KR intervals go to ivs; larger ones are published in row order
+// to fb_ivs (fb_meta = offset, count; niv = kNivOverflow); reads whose sets
+// outgrow G (or the published list) are queued for the workgroup kernel.
 // Reads the lookup table cannot seed start from the packed F-mer records of
-// the expansions of their last F codes (one ambiguity code) when those hold at
-// most C intervals together.
-// ---------------------------------------------------------------------------
-template <uint32_t C>
-__global__ __launch_bounds__(256) void k_locate_search_wide(DevDir d, const DevConst *__restrict__ cst,
-                                                            const uint8_t *__restrict__ q,
-                                                            const uint64_t *__restrict__ qoff,
-                                                            const uint32_t *__restrict__ list, uint32_t nlist,
-                                                            uint32_t *__restrict__ cnt, uint8_t *__restrict__ niv,
-                                                            uint2 *__restrict__ ivs, uint2 *__restrict__ fb_meta,
-                                                            uint2 *__restrict__ fb_ivs, uint64_t fb_cap,
-                                                            uint32_t *__restrict__ ovf_list,
-                                                            LocateCounters *__restrict__ ctr,
-                                                            const uint4 *__restrict__ seed_rec,
-                                                            const uint4 *__restrict__ seed_iv, uint32_t F) {
-    __shared__ uint16_t s_compat[256];
-    __shared__ int8_t s_digit[256];
-    __shared__ uint8_t s_expand[256];
-    __shared__ uint32_t s_clane[16];
-    load_const_lds(cst, s_compat, s_digit, s_clane);
-    for (uint32_t x = threadIdx.x; x < 256; x += blockDim.x) s_expand[x] = cst->seed_expand[x];
-    __syncthreads();
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nlist) return;
-    const uint32_t r = list[t];
-    const uint64_t b0 = qoff[r];
-    const uint32_t m = (uint32_t)(qoff[r + 1] - b0);
-    const uint8_t *p = q + b0;
-    uint32_t lo[C], hi[C], ns = 1;
-#pragma unroll
-    for (uint32_t j = 0; j < C; j++) { lo[j] = 0xffffffffu; hi[j] = 0xffffffffu; }
-    lo[0] = 0;
-    hi[0] = d.n;
-    int start = (int)m;
-    bool give_up = false;
-    const int sd = lookup_seed(d, s_digit, cst->lookup_depth, cst->lookup_radix, p, m, lo[0], hi[0]);
-    if (sd == 2) ns = 0;
-    else if (sd == 1) start = (int)(m - cst->lookup_depth);
-    else if (seed_rec && m >= F) {
-        SeedWin sw;
-        if (seed_window(p + m - F, F, s_expand, sw, 1)) {
-            uint32_t k2 = 0;
-            bool ok = true;
-            for (uint32_t c = 0; c < seed_ncombo(sw) && ok; c++) {
-                uint32_t ix = 0;
-                if (!seed_combo(sw, c, ix)) continue;
-                const uint4 R = seed_rec[ix];
-                const uint32_t l = R.w & ((1u << kSeedLslBits) - 1u), n2 = R.w >> kSeedLslBits;
-                if (l != F || n2 == 0) continue;
-                const uint64_t o = (uint64_t)R.x | ((uint64_t)R.y << 32);
-                for (uint32_t e = 0; e < n2 && ok; e++) {
-                    const uint2 iv = n2 == 1 ? make_uint2(R.x, R.y) : make_uint2(seed_iv[o + e].x, seed_iv[o + e].y);
-                    bool dup = false;
-#pragma unroll
-                    for (uint32_t u = 0; u < C; u++) dup |= u < k2 && lo[u] == iv.x;
-                    if (dup) continue;
-                    if (k2 >= C) { ok = false; break; }
-#pragma unroll
-                    for (uint32_t u = 0; u < C; u++)
-                        if (u == k2) { lo[u] = iv.x; hi[u] = iv.y; }
-                    k2++;
-                }
-            }
-            if (ok) {
-                ns = k2;
-                start = (int)(m - F);
-            } else {
-                give_up = true;
-            }
-        }
-    }
-    const bool textmode = d.isa != nullptr && d.text != nullptr && d.sa_full != nullptr;
-    uint32_t tprobes = 0;
-    if (!give_up && textmode && ns == 1 && hi[0] - lo[0] == 1 && start > 0) {
-        locate_finish_text(d, s_compat, p, (uint32_t)start, lo[0], hi[0], ns, tprobes);
-        start = 0;
-    }
-    for (int i = start - 1; i >= 0 && ns && !give_up; i--) {
-        const uint32_t cm = s_compat[p[i]];
-        uint32_t nlo[C], nhi[C], nn = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < C; j++) { nlo[j] = 0xffffffffu; nhi[j] = 0xffffffffu; }
-#pragma unroll
-        for (uint32_t j = 0; j < C; j++) {
-            if (j < ns) {
-                uint32_t cl[16], ch[16];
-                pair_counts(d, lo[j], hi[j], cl, ch);
-#pragma unroll
-                for (int k = 0; k < 16; k++) {
-                    if (((cm >> k) & 1u) && ch[k] > cl[k]) {
-                        const uint32_t a = s_clane[k] + cl[k], b = s_clane[k] + ch[k];
-                        if (nn >= C) give_up = true;
-#pragma unroll
-                        for (uint32_t u = 0; u < C; u++)
-                            if (u == nn) { nlo[u] = a; nhi[u] = b; }
-                        nn++;
-                    }
-                }
-            }
-        }
-        if (give_up) break;
-#pragma unroll
-        for (uint32_t j = 0; j < C; j++) { lo[j] = nlo[j]; hi[j] = nhi[j]; }
-        ns = nn;
-        if (textmode && ns == 1 && hi[0] - lo[0] == 1 && i > 0) {
-            locate_finish_text(d, s_compat, p, (uint32_t)i, lo[0], hi[0], ns, tprobes);
-            break;
-        }
-    }
-    if (!give_up) {
-        // row order (disjoint intervals; empty slots hold 0xffffffff)
-#pragma unroll
-        for (uint32_t a = 0; a < C; a++)
-#pragma unroll
-            for (uint32_t b = 0; b + 1 < C - a; b++) {
-                const bool sw = lo[b] > lo[b + 1];
-                const uint32_t tl = sw ? lo[b + 1] : lo[b], th = sw ? hi[b + 1] : hi[b];
-                const uint32_t ul = sw ? lo[b] : lo[b + 1], uh = sw ? hi[b] : hi[b + 1];
-                lo[b] = tl; hi[b] = th; lo[b + 1] = ul; hi[b + 1] = uh;
-            }
-        uint32_t total = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < C; j++)
-            if (j < ns) total += hi[j] - lo[j];
-        if (ns <= KR) {
-#pragma unroll
-            for (uint32_t j = 0; j < KR; j++)
-                if (j < ns) ivs[(size_t)r * KR + j] = make_uint2(lo[j], hi[j]);
-            niv[r] = (uint8_t)ns;
-            cnt[r] = total;
-            return;
-        }
-        const unsigned long long slot = atomicAdd(&ctr->fb_used, (unsigned long long)ns);
-        if (slot + ns <= fb_cap) {
-#pragma unroll
-            for (uint32_t j = 0; j < C; j++)
-                if (j < ns) fb_ivs[slot + j] = make_uint2(lo[j], hi[j]);
-            fb_meta[r] = make_uint2((uint32_t)slot, ns);
-            niv[r] = kNivOverflow;
-            cnt[r] = total;
-            return;
-        }
-    }
-    niv[r] = kNivOverflow;
-    cnt[r] = 0;
-    ovf_list[atomicAdd(&ctr->n_ovf, 1ull)] = r;
-}
-
-// ---------------------------------------------------------------------------
-// Second pass, one group of G lanes per read: the same search, seeding and
-// publication as k_locate_search_wide<G>, but each lane of the group owns one
-// interval of the set, so a step costs one dependent round trip instead of
-// one per interval.  A step's children are counted per lane, placed by a
+// the expansions of their last F codes (one ambiguity code) when those hold
+// at most G intervals together.  A step's children are counted per lane, placed by a
 // prefix sum over the group (sub-wave shuffles) and gathered through the
 // group's LDS list; seeding and the text finish run on lane 0.  Every
 // decision (set size, give-up, loop index) is uniform over the group.
@@ -479,7 +329,7 @@ __global__ __launch_bounds__(256) void k_locate_search_group(DevDir d, const Dev
     const uint8_t *p = q + b0;
     uint32_t *glo = s_lo[gi], *ghi = s_hi[gi];
 
-    // seeding on lane 0, into the group's list (k_locate_search_wide's rules)
+    // seeding on lane 0, into the group's list
     uint32_t ns = 0, give = 0;
     int start = (int)m;
     if (gl == 0) {
@@ -670,340 +520,6 @@ __global__ __launch_bounds__(256) void k_locate_search_group(DevDir d, const Dev
     }
 }
 
-// ---------------------------------------------------------------------------
-// Pipelined persistent backward search.  Each lane runs locate_read as a state
-// machine.  Every pass of the loop issues, for each lane, one round of
-// independent loads -- one probe (two Occ records, a superblock row and the
-// next query code) or one control load (the read's offsets, its last 16
-// codes, a lookup entry, a seed-table entry) -- and then waits once, so a
-// wave spends one memory round trip per pass whatever mix of states its lanes
-// are in.  Control loads are parked in a per-lane LDS stage and consumed at the
-// start of the next pass.  Lanes take reads from a device work queue one read
-// ahead (the refill atomic is consumed a pass later); list, when given, names
-// the reads to run.  Interval sets live in a ring of PR LDS slots per lane;
-// final sets of more than KR intervals are published in row order to fb_ivs;
-// sets that outgrow the ring send the read to k_locate_wave.
-//
-// Reads that the lookup table cannot seed (an ambiguity code among the last
-// lookup_depth codes, or no lookup table) start from the seed table: the
-// union of its interval lists over the expansions of q[m-F..m) is the set of
-// every compatible string, which backward search from the full interval
-// reaches after F steps (query.rs:219-339 semantics; only the work differs).
-// ---------------------------------------------------------------------------
-enum : uint32_t {
-    LP_FETCH = 0, LP_HDR, LP_WIN, LP_LOOK, LP_SEEDX, LP_SEEDE, LP_STEP, LP_DONE,
-    LP_START, LP_FINISH, LP_SEEDNEXT // transitions without loads
-};
-
-template <int PR, int PB>
-__global__ __launch_bounds__(PB) void k_locate_pipe(DevDir d, const DevConst *__restrict__ cst, SeedView sv,
-                                                    const uint8_t *__restrict__ q, const uint64_t *__restrict__ qoff,
-                                                    const uint32_t *__restrict__ list, uint32_t nq,
-                                                    uint32_t *__restrict__ cnt, uint8_t *__restrict__ niv,
-                                                    uint2 *__restrict__ ivs, uint2 *__restrict__ fb_meta,
-                                                    uint2 *__restrict__ fb_ivs, uint64_t fb_cap,
-                                                    uint32_t *__restrict__ ovf_list, LocateCounters *__restrict__ ctr) {
-    static_assert((PR & (PR - 1)) == 0, "ring size must be a power of two");
-    constexpr uint32_t RM = PR - 1;
-    __shared__ uint32_t s_lo[PR][PB], s_hi[PR][PB];
-    __shared__ uint32_t s_stage[5][PB]; // control loads of the last pass
-    __shared__ uint16_t s_compat[256];
-    __shared__ int8_t s_digit[256];
-    __shared__ uint8_t s_expand[256];
-    __shared__ uint32_t s_clane[16];
-    load_const_lds(cst, s_compat, s_digit, s_clane);
-    for (uint32_t x = threadIdx.x; x < 256; x += blockDim.x) s_expand[x] = cst->seed_expand[x];
-    __syncthreads();
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = __lane_id();
-    const uint32_t n = d.n;
-    const uint32_t depth = cst->lookup_depth, radix = cst->lookup_radix;
-    const uint32_t F = sv.F;
-
-    uint32_t st = LP_FETCH, r = 0, nr = 0, m = 0, i = 0, head = 0, ns = 0, nn = 0, t = 0, probes = 0;
-    uint32_t aidx = 0, xi = 0;
-    uint64_t ient = 0, iend = 0;
-    const uint8_t *p = q;
-    SeedWin sw{0, 0, 0x111u, 0};
-    uint2 piv = make_uint2(0, 0);
-    {
-        uint32_t b = 0;
-        if (lane == 0) b = (uint32_t)atomicAdd(&ctr->pwork, 64ull);
-        nr = __shfl(b, 0, 64) + lane;
-    }
-    uint32_t abase = 0, arank = 0, aleader = 0;
-    bool pending = false, want = false;
-
-    for (;;) {
-        // (A) next-read refill issued last pass
-        if (pending) {
-            const uint32_t bb = __shfl(abase, (int)aleader, 64);
-            if (want) nr = bb + arank;
-            pending = false;
-        }
-        want = false;
-        // (B) consume the staged control loads; run load-free transitions
-        for (;;) {
-            if (st == LP_HDR) {
-                const uint64_t b0 = (uint64_t)s_stage[0][tid] | ((uint64_t)s_stage[1][tid] << 32);
-                const uint64_t b1 = (uint64_t)s_stage[2][tid] | ((uint64_t)s_stage[3][tid] << 32);
-                m = (uint32_t)(b1 - b0);
-                p = q + b0;
-                if (m == 0) { // the empty pattern matches every row
-                    s_lo[head][tid] = 0;
-                    s_hi[head][tid] = n;
-                    ns = 1;
-                    i = 0;
-                    st = LP_START;
-                    continue;
-                }
-                st = LP_WIN;
-                break;
-            } else if (st == LP_WIN) {
-                // last 16 codes, four per word: code k is p[m - 16 + k]
-                uint32_t wb[16];
-#pragma unroll
-                for (int k = 0; k < 16; k++) wb[k] = (s_stage[k >> 2][tid] >> (8 * (k & 3))) & 0xffu;
-                bool looked = false;
-                if (depth && m >= depth && d.lookup) {
-                    // lookup.rs:101-111: leftmost code is the least significant digit
-                    uint64_t idx = 0;
-                    bool ok = true;
-                    if (depth <= 16) {
-#pragma unroll
-                        for (int k = 15; k >= 0; k--)
-                            if ((uint32_t)k + depth >= 16) {
-                                const int dg = s_digit[wb[k]];
-                                ok &= dg >= 0;
-                                idx = idx * radix + (uint64_t)(dg < 0 ? 0 : dg);
-                            }
-                    } else {
-                        for (int k = (int)depth - 1; k >= 0; k--) {
-                            const int dg = s_digit[p[m - depth + k]];
-                            ok &= dg >= 0;
-                            idx = idx * radix + (uint64_t)(dg < 0 ? 0 : dg);
-                        }
-                    }
-                    if (ok) {
-                        aidx = (uint32_t)(idx < d.lookup_len ? idx : d.lookup_len - 1);
-                        st = LP_LOOK;
-                        looked = true;
-                    }
-                }
-                if (looked) break;
-                if (F && m >= F && seed_window16(wb, F, s_expand, sw)) {
-                    xi = 0;
-                    nn = 0;
-                    ns = 0;
-                    st = LP_SEEDNEXT;
-                    continue;
-                }
-                s_lo[head][tid] = 0;
-                s_hi[head][tid] = n;
-                ns = 1;
-                i = m;
-                st = LP_START;
-                continue;
-            } else if (st == LP_LOOK) {
-                probes++;
-                const uint32_t a0 = s_stage[0][tid], a1 = s_stage[1][tid];
-                if (a0 >= a1) {
-                    ns = 0;
-                } else {
-                    s_lo[head][tid] = a0 < n ? a0 : n;
-                    s_hi[head][tid] = a1 < n ? a1 : n;
-                    ns = 1;
-                }
-                i = m - depth;
-                st = LP_START;
-                continue;
-            } else if (st == LP_SEEDX) {
-                probes++;
-                const uint32_t l = s_stage[4][tid];
-                ient = (uint64_t)s_stage[0][tid] | ((uint64_t)s_stage[1][tid] << 32);
-                iend = (uint64_t)s_stage[2][tid] | ((uint64_t)s_stage[3][tid] << 32);
-                if (l == F && ient < iend) {
-                    st = LP_SEEDE;
-                    break;
-                }
-                st = LP_SEEDNEXT;
-                continue;
-            } else if (st == LP_SEEDE) {
-                probes++;
-                const uint32_t lo = s_stage[0][tid], hi = s_stage[1][tid];
-                bool dup = false;
-                for (uint32_t u = 0; u < nn; u++) dup |= s_lo[(head + u) & RM][tid] == lo;
-                if (!dup) {
-                    if (nn == PR) { // the seed union outgrows the ring
-                        niv[r] = kNivOverflow;
-                        cnt[r] = 0;
-                        ovf_list[atomicAdd(&ctr->n_ovf, 1ull)] = r;
-                        st = LP_FETCH;
-                        break;
-                    }
-                    s_lo[(head + nn) & RM][tid] = lo;
-                    s_hi[(head + nn) & RM][tid] = hi;
-                    nn++;
-                }
-                if (++ient < iend) break; // next entry (LP_SEEDE)
-                st = LP_SEEDNEXT;
-                continue;
-            } else if (st == LP_SEEDNEXT) {
-                uint32_t idx = 0;
-                while (xi < seed_ncombo(sw) && !seed_combo(sw, xi, idx)) xi++;
-                if (xi < seed_ncombo(sw)) {
-                    xi++;
-                    aidx = idx;
-                    st = LP_SEEDX;
-                    break;
-                }
-                ns = nn;
-                nn = 0;
-                i = m - F;
-                st = LP_START;
-                continue;
-            } else if (st == LP_START) {
-                if (ns != 0 && i != 0) {
-                    t = 0;
-                    nn = 0;
-                    piv = make_uint2(s_lo[head][tid], s_hi[head][tid]);
-                    st = LP_STEP;
-                    break;
-                }
-                st = LP_FINISH;
-                continue;
-            } else if (st == LP_FINISH) {
-                // rows in ascending order: rank of each interval by its start
-                uint32_t total = 0;
-                for (uint32_t u = 0; u < ns; u++) total += s_hi[(head + u) & RM][tid] - s_lo[(head + u) & RM][tid];
-                uint2 *dst = ivs + (size_t)r * KR;
-                bool keep = true;
-                if (ns > KR) {
-                    const unsigned long long slot = atomicAdd(&ctr->fb_used, (unsigned long long)ns);
-                    if (slot + ns > fb_cap) {
-                        niv[r] = kNivOverflow;
-                        cnt[r] = 0;
-                        ovf_list[atomicAdd(&ctr->n_ovf, 1ull)] = r;
-                        keep = false;
-                    } else {
-                        dst = fb_ivs + slot;
-                        fb_meta[r] = make_uint2((uint32_t)slot, ns);
-                    }
-                }
-                if (keep) {
-                    for (uint32_t u = 0; u < ns; u++) {
-                        const uint32_t lo = s_lo[(head + u) & RM][tid];
-                        uint32_t rk = 0;
-                        for (uint32_t v = 0; v < ns; v++) rk += s_lo[(head + v) & RM][tid] < lo ? 1u : 0u;
-                        dst[rk] = make_uint2(lo, s_hi[(head + u) & RM][tid]);
-                    }
-                    niv[r] = ns > KR ? kNivOverflow : (uint8_t)ns;
-                    cnt[r] = total;
-                }
-                st = LP_FETCH;
-                continue;
-            } else if (st == LP_FETCH) {
-                if (nr >= nq) {
-                    st = LP_DONE;
-                } else {
-                    r = list ? list[nr] : nr;
-                    want = true;
-                    st = LP_HDR;
-                }
-                break;
-            } else {
-                break; // LP_STEP, LP_DONE
-            }
-        }
-        // (C) refill the prefetched read of lanes that just took theirs
-        const uint64_t act = __ballot(want);
-        if (act) {
-            aleader = (uint32_t)__ffsll((unsigned long long)act) - 1u;
-            arank = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
-            if (lane == aleader) abase = (uint32_t)atomicAdd(&ctr->pwork, (unsigned long long)__popcll(act));
-            pending = true;
-        }
-        if (__ballot(st != LP_DONE) == 0) break;
-        // (D) issue this pass's loads: control loads or one probe, all into w
-        PairRaw w;
-        uint32_t qc = 0;
-        if (st == LP_HDR) {
-            const uint64_t b0 = qoff[r], b1 = qoff[r + 1];
-            w.l0 = make_uint4((uint32_t)b0, (uint32_t)(b0 >> 32), (uint32_t)b1, (uint32_t)(b1 >> 32));
-        } else if (st == LP_WIN) {
-            w.l0.x = m >= 16 ? p[m - 16] : 0u; w.l0.y = m >= 15 ? p[m - 15] : 0u;
-            w.l0.z = m >= 14 ? p[m - 14] : 0u; w.l0.w = m >= 13 ? p[m - 13] : 0u;
-            w.l1.x = m >= 12 ? p[m - 12] : 0u; w.l1.y = m >= 11 ? p[m - 11] : 0u;
-            w.l1.z = m >= 10 ? p[m - 10] : 0u; w.l1.w = m >= 9 ? p[m - 9] : 0u;
-            w.l2.x = m >= 8 ? p[m - 8] : 0u; w.l2.y = m >= 7 ? p[m - 7] : 0u;
-            w.l2.z = m >= 6 ? p[m - 6] : 0u; w.l2.w = m >= 5 ? p[m - 5] : 0u;
-            w.l3.x = m >= 4 ? p[m - 4] : 0u; w.l3.y = m >= 3 ? p[m - 3] : 0u;
-            w.l3.z = m >= 2 ? p[m - 2] : 0u; w.l3.w = p[m - 1];
-        } else if (st == LP_LOOK) {
-            const uint2 iv = d.lookup[aidx];
-            w.l0.x = iv.x;
-            w.l0.y = iv.y;
-        } else if (st == LP_SEEDX) {
-            const uint64_t o0 = sv.off[aidx], o1 = sv.off[aidx + 1];
-            w.l0 = make_uint4((uint32_t)o0, (uint32_t)(o0 >> 32), (uint32_t)o1, (uint32_t)(o1 >> 32));
-            w.l1.x = sv.lsl[aidx];
-        } else if (st == LP_SEEDE) {
-            w.l0 = sv.iv[ient];
-        } else if (st == LP_STEP) {
-            pair_issue(d, piv.x, piv.y, w);
-            qc = p[i - 1];
-            probes += 1u + ((piv.x >> kRecShift) != (piv.y >> kRecShift) ? 1u : 0u);
-        }
-        // (E) one wait: evaluate the probe, park the control loads
-        if (st == LP_STEP) {
-            const uint32_t cm = s_compat[qc];
-            const uint32_t cbase = head + ns;
-            const uint32_t room = PR - (ns - t - 1);
-            bool ovf = false;
-            pair_visit(d, w, piv.x, piv.y, [&](int k, uint32_t a, uint32_t b) {
-                if (((cm >> k) & 1u) && b > a) {
-                    if (nn + 1 > room) {
-                        ovf = true;
-                    } else {
-                        const uint32_t x = (cbase + nn) & RM;
-                        s_lo[x][tid] = s_clane[k] + a;
-                        s_hi[x][tid] = s_clane[k] + b;
-                        nn++;
-                    }
-                }
-            });
-            if (ovf) {
-                niv[r] = kNivOverflow;
-                cnt[r] = 0;
-                ovf_list[atomicAdd(&ctr->n_ovf, 1ull)] = r;
-                st = LP_FETCH;
-            } else if (++t < ns) {
-                const uint32_t x = (head + t) & RM;
-                piv = make_uint2(s_lo[x][tid], s_hi[x][tid]);
-            } else {
-                head = (head + ns) & RM;
-                ns = nn;
-                i--;
-                st = LP_START;
-            }
-        } else if (st == LP_WIN) {
-            s_stage[0][tid] = w.l0.x | (w.l0.y << 8) | (w.l0.z << 16) | (w.l0.w << 24);
-            s_stage[1][tid] = w.l1.x | (w.l1.y << 8) | (w.l1.z << 16) | (w.l1.w << 24);
-            s_stage[2][tid] = w.l2.x | (w.l2.y << 8) | (w.l2.z << 16) | (w.l2.w << 24);
-            s_stage[3][tid] = w.l3.x | (w.l3.y << 8) | (w.l3.z << 16) | (w.l3.w << 24);
-        } else if (st != LP_DONE) {
-            s_stage[0][tid] = w.l0.x;
-            s_stage[1][tid] = w.l0.y;
-            s_stage[2][tid] = w.l0.z;
-            s_stage[3][tid] = w.l0.w;
-            s_stage[4][tid] = w.l1.x;
-        }
-    }
-#pragma unroll
-    for (int dd = 32; dd >= 1; dd >>= 1) probes += __shfl_xor(probes, dd, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&ctr->probes, (unsigned long long)probes);
-}
-
 // One wave per queued read.  Set buffers: per wave 2 * cap (lo,hi) pairs.
 __global__ __launch_bounds__(64) void k_locate_wave(DevDir d, const DevConst *__restrict__ cst,
                                                     const uint8_t *__restrict__ q,
@@ -1191,43 +707,30 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
     // final sets of more than KR intervals are published to fb_ivs
     if ((rc = ws.fb_ivs.ensure(((size_t)nq * 2 + 65536) * 8))) return rc;
     // 1) register kernel over every read (at most KR intervals per read)
-    // 2) the reads it gives up on: pipelined ring kernel, seed-table starts
-    // 3) what still outgrows the ring: one wave per read
-    const char *path = getenv("HSFM_LOCATE_PATH");
-    const bool pipe_all = path && strcmp(path, "pipe") == 0;
-    const bool pipe_second = path && strcmp(path, "pipe2") == 0; // pipelined kernel as the second level
-    const bool wide_second = path && strcmp(path, "wide") == 0;  // thread-per-read second level
-    SeedView sv{nullptr, nullptr, nullptr, 0};
-    if (nq) {
-        const uint32_t F = 12;
-        if (idx->fwd.view.n > (1u << 20) && (rc = build_seed_packed(idx, F))) return rc;
-        if (idx->seed_F)
-            sv = SeedView{idx->seed_off.as<uint64_t>(), idx->seed_iv.as<uint4>(), idx->seed_lsl.as<uint8_t>(),
-                          idx->seed_F};
-    }
-    const uint32_t pipe_grid = persist_grid(idx, k_locate_pipe<16, 256>, 256);
+    // 2) the reads it gives up on: 16 lanes per read, room for 16 intervals
+    // 3) what still outgrows those: one wave per read
+    // (a pipelined persistent ring kernel, for every read or as the second
+    // level, and a thread-per-read 16-interval second level were measured
+    // slower and are not kept)
+    if (nq && idx->fwd.view.n > (1u << 20) && (rc = build_seed_packed(idx, 12))) return rc;
     uint64_t h[8];
     uint64_t nlist = nq;
     const uint32_t *list = nullptr;
     if (nq) {
         if ((rc = event_record(idx, 0, false))) return rc;
-        if (!pipe_all)
-            hipLaunchKernelGGL(k_locate_search, dim3((nq + 255) / 256), dim3(256), 0, s, d,
-                               idx->d_const.as<DevConst>(), d_q, d_qoff, nq, ws.cnt.as<uint32_t>(),
-                               ws.niv.as<uint8_t>(), ws.ivs.as<uint2>(), ws.ovf_list.as<uint32_t>(), ctr,
-                               idx->seed_rec12.as<uint4>(), idx->seed_rec12.p ? 12u : 0u);
+        hipLaunchKernelGGL(k_locate_search, dim3((nq + 255) / 256), dim3(256), 0, s, d,
+                           idx->d_const.as<DevConst>(), d_q, d_qoff, nq, ws.cnt.as<uint32_t>(),
+                           ws.niv.as<uint8_t>(), ws.ivs.as<uint2>(), ws.ovf_list.as<uint32_t>(), ctr,
+                           idx->seed_rec12.as<uint4>(), idx->seed_rec12.p ? 12u : 0u);
         HSFM_HIP(hipGetLastError());
         if ((rc = event_record(idx, 0, true))) return rc;
         if ((rc = event_record(idx, 2, false))) return rc;
-        if (!pipe_all) {
-            if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
-            nlist = h[0];
-            list = ws.ovf_list.as<uint32_t>();
-            idx->stats.occ_probes = h[5]; // the main kernel's own probes
-        }
-        idx->stats.fallback_reads = pipe_all ? 0 : nlist;
-        if (nlist && !pipe_all && !pipe_second && !wide_second) {
-            // second pass: 16 lanes per read, room for 16 intervals
+        if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
+        nlist = h[0];
+        list = ws.ovf_list.as<uint32_t>();
+        idx->stats.occ_probes = h[5]; // the main kernel's own probes
+        idx->stats.fallback_reads = nlist;
+        if (nlist) {
             if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_ovf), 2, 0))) return rc;
             hipLaunchKernelGGL(k_locate_search_group<16>, dim3((uint32_t)((nlist + 15) / 16)), dim3(256), 0, s, d,
                                idx->d_const.as<DevConst>(), d_q, d_qoff, list, (uint32_t)nlist,
@@ -1236,26 +739,6 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
                                ws.read_list.as<uint32_t>(), ctr, idx->seed_rec12.as<uint4>(),
                                idx->seed_iv.as<uint4>(), idx->seed_rec12.p ? 12u : 0u);
             HSFM_HIP(hipGetLastError());
-        } else if (nlist && wide_second) {
-            // second register pass with room for 16 intervals per read
-            if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_ovf), 2, 0))) return rc;
-            hipLaunchKernelGGL(k_locate_search_wide<16>, dim3((uint32_t)((nlist + 255) / 256)), dim3(256), 0, s, d,
-                               idx->d_const.as<DevConst>(), d_q, d_qoff, list, (uint32_t)nlist,
-                               ws.cnt.as<uint32_t>(), ws.niv.as<uint8_t>(), ws.ivs.as<uint2>(),
-                               ws.fb_meta.as<uint2>(), ws.fb_ivs.as<uint2>(), (uint64_t)(ws.fb_ivs.cap / 8),
-                               ws.read_list.as<uint32_t>(), ctr, idx->seed_rec12.as<uint4>(),
-                               idx->seed_iv.as<uint4>(), idx->seed_rec12.p ? 12u : 0u);
-            HSFM_HIP(hipGetLastError());
-        } else if (nlist) {
-            if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_ovf), 2, 0))) return rc;
-            const uint32_t need = (uint32_t)((nlist + 255) / 256);
-            hipLaunchKernelGGL((k_locate_pipe<16, 256>), dim3(need < pipe_grid ? need : pipe_grid), dim3(256), 0, s,
-                               d, idx->d_const.as<DevConst>(), sv, d_q, d_qoff, list, (uint32_t)nlist,
-                               ws.cnt.as<uint32_t>(), ws.niv.as<uint8_t>(), ws.ivs.as<uint2>(),
-                               ws.fb_meta.as<uint2>(), ws.fb_ivs.as<uint2>(), (uint64_t)(ws.fb_ivs.cap / 8),
-                               ws.read_list.as<uint32_t>(), ctr);
-            HSFM_HIP(hipGetLastError());
-            if (pipe_all && (rc = event_record(idx, 0, true))) return rc;
         }
     }
     if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;

@@ -37,7 +37,7 @@ struct MemCounters {
     unsigned long long rec_used;
     unsigned long long n_retry;
     unsigned long long guard;
-    unsigned long long probes;   // records fetched by k_mem_thread
+    unsigned long long probes;   // records fetched by the main kernel
     unsigned long long diag[4];  // deferrals: seed union, extension, slots; plain restarts
     unsigned long long pwork;    // work index of k_mem_persist
     unsigned long long pad[6];
@@ -450,252 +450,8 @@ __global__ void k_seed_root(uint64_t *off, uint4 *iv, uint8_t *lsl, uint32_t n) 
 
 
 // ---------------------------------------------------------------------------
-// Thread-per-read MEM finder: the min_len-skip algorithm of k_mem_wave with the
-// interval set of each read in per-thread LDS slots (KT intervals).  Each
-// restart seeds q[j-F..j) from the seed table (ambiguity codes expand to the
-// union of their bases' entries, at most two per window).  Reads whose set or
-// record list outgrows the slots are deferred to k_mem_wave.
-// ---------------------------------------------------------------------------
-constexpr int KT = 8;
-constexpr int TPB = 256;
-
-template <bool LEFT>
-__device__ __forceinline__ uint32_t thread_extend(const DevDir &D, uint32_t L, const uint32_t *s_clane, uint32_t cm,
-                                                  uint32_t (*lo)[TPB], uint32_t (*hi)[TPB], uint32_t (*rl)[TPB],
-                                                  uint32_t (*nlo)[TPB], uint32_t (*nhi)[TPB], uint32_t (*nrl)[TPB],
-                                                  uint32_t ns, uint32_t tid, bool &overflow, uint32_t &probes) {
-    uint32_t nn = 0;
-    for (uint32_t j = 0; j < ns; j++) {
-        const uint4 iv = make_uint4(lo[j][tid], hi[j][tid], rl[j][tid], 0u);
-        uint32_t cl[16], ch[16];
-        interval_counts<LEFT>(D, L, iv, cl, ch);
-        const uint32_t xlo = LEFT ? iv.x : iv.z, xhi = LEFT ? iv.y : iv.z + (iv.y - iv.x);
-        probes += 1u + ((xlo >> kRecShift) != (xhi >> kRecShift) ? 1u : 0u);
-        uint32_t acc = 0;
-#pragma unroll
-        for (int k = 0; k < 16; k++) {
-            const uint32_t d = ch[k] - cl[k];
-            if (((cm >> k) & 1u) && ch[k] > cl[k]) {
-                if (nn < (uint32_t)KT) {
-                    const uint4 c = bidir_child<LEFT>(iv, s_clane[k], cl[k], ch[k], acc);
-                    nlo[nn][tid] = c.x;
-                    nhi[nn][tid] = c.y;
-                    nrl[nn][tid] = c.z;
-                } else {
-                    overflow = true;
-                }
-                nn++;
-            }
-            acc += d;
-        }
-    }
-    return nn;
-}
-
-__device__ __forceinline__ uint32_t mem_thread_read(const DevDir &fwd, const DevDir &rev, const DevConst *cst,
-                                                 const SeedView &sv, const uint8_t *q, const uint64_t *qoff, uint32_t r,
-                                                 uint32_t min_len, uint4 *slots, uint32_t *mem_cnt, uint8_t *status,
-                                                 uint32_t *defer_list, MemCounters *ctr,
-                                                 uint32_t (*s_lo)[KT][TPB], uint32_t (*s_hi)[KT][TPB],
-                                                 uint32_t (*s_rl)[KT][TPB], const uint16_t *s_compat,
-                                                 const uint8_t *s_expand, const uint32_t *s_clane, uint32_t tid);
-
-__global__ __launch_bounds__(TPB) void k_mem_thread(DevDir fwd, DevDir rev, const DevConst *__restrict__ cst,
-                                                    SeedView sv, const uint8_t *__restrict__ q,
-                                                    const uint64_t *__restrict__ qoff, uint32_t nq, uint32_t min_len,
-                                                    uint4 *__restrict__ slots, uint32_t *__restrict__ mem_cnt,
-                                                    uint8_t *__restrict__ status, uint32_t *__restrict__ defer_list,
-                                                    MemCounters *__restrict__ ctr) {
-    __shared__ uint32_t s_lo[2][KT][TPB], s_hi[2][KT][TPB], s_rl[2][KT][TPB];
-    __shared__ uint16_t s_compat[256];
-    __shared__ uint8_t s_expand[256];
-    __shared__ uint32_t s_clane[16];
-    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) {
-        s_compat[i] = cst->compat_lanes[i];
-        s_expand[i] = cst->seed_expand[i];
-    }
-    if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
-    __syncthreads();
-    const uint32_t tid = threadIdx.x;
-    const uint32_t r = blockIdx.x * blockDim.x + tid;
-    uint32_t probes = 0;
-    if (r < nq) probes = mem_thread_read(fwd, rev, cst, sv, q, qoff, r, min_len, slots, mem_cnt, status, defer_list, ctr,
-                                         s_lo, s_hi, s_rl, s_compat, s_expand, s_clane, tid);
-    // one atomic per wave for the probe total
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) probes += __shfl_xor(probes, d, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&ctr->probes, (unsigned long long)probes);
-}
-
-__device__ __forceinline__ uint32_t mem_thread_read(const DevDir &fwd, const DevDir &rev, const DevConst *cst,
-                                                 const SeedView &sv, const uint8_t *q, const uint64_t *qoff, uint32_t r,
-                                                 uint32_t min_len, uint4 *slots, uint32_t *mem_cnt, uint8_t *status,
-                                                 uint32_t *defer_list, MemCounters *ctr,
-                                                 uint32_t (*s_lo)[KT][TPB], uint32_t (*s_hi)[KT][TPB],
-                                                 uint32_t (*s_rl)[KT][TPB], const uint16_t *s_compat,
-                                                 const uint8_t *s_expand, const uint32_t *s_clane, uint32_t tid) {
-    const uint32_t L = cst->num_lanes;
-    const uint32_t n = fwd.n;
-    const uint64_t b0 = qoff[r], b1 = qoff[r + 1];
-    const uint32_t m = (uint32_t)(b1 - b0);
-    const uint8_t *p = q + b0;
-    const uint64_t slot_base = (b0 >> 3) + 8ull * r;
-    const uint32_t slot_cap = (uint32_t)((b1 >> 3) - (b0 >> 3)) + 8u;
-    const uint32_t F = sv.F;
-    uint32_t s0 = 0, E = 0, nrec = 0, probes = 0, why = 0, plain = 0;
-    int cur = 0;
-    bool defer = false;
-    while (min_len && !defer) {
-        const uint32_t j = (s0 + min_len > E + 1) ? s0 + min_len : E + 1;
-        if (j > m) break;
-        uint32_t ns = 0, k = j;
-        bool seeded = false;
-        if (F && j >= F) {
-            // digits of q[j-F..j); up to two ambiguity codes expand
-            uint32_t base_idx = 0, apos[2] = {0, 0}, amask[2] = {1, 1}, namb = 0;
-            bool usable = true;
-            for (uint32_t t = 0; t < F; t++) {
-                const uint32_t e = s_expand[p[j - F + t]];
-                if (e & 0x10u) base_idx |= (e & 3u) << (2 * t);
-                else if (e && namb < 2) { apos[namb] = t; amask[namb] = e; namb++; }
-                else usable = false;
-            }
-            if (usable) {
-                // Stream the table entries of every expansion straight into the
-                // first left extension (by q[j-F-1]) when min_len > F, so only the
-                // (much smaller) set of q[j-F-1..j) is held; with min_len == F the
-                // F-mer set itself is materialized.
-                const bool stream = min_len > F;
-                const uint32_t cm1 = stream ? s_compat[p[j - F - 1]] : 0u;
-                uint32_t lslmax = 0;
-                for (uint32_t x0 = 0; x0 < 4 && !defer; x0++) {
-                    if (!((amask[0] >> x0) & 1u)) continue;
-                    for (uint32_t x1 = 0; x1 < 4 && !defer; x1++) {
-                        if (!((amask[1] >> x1) & 1u)) continue;
-                        uint32_t idx = base_idx;
-                        if (namb > 0) idx |= x0 << (2 * apos[0]);
-                        if (namb > 1) idx |= x1 << (2 * apos[1]);
-                        const uint32_t l = sv.lsl[idx];
-                        probes++;
-                        lslmax = l > lslmax ? l : lslmax;
-                        if (l != F) continue;
-                        probes += 2; // offsets + intervals
-                        for (uint64_t i = sv.off[idx]; i < sv.off[idx + 1] && !defer; i++) {
-                            const uint4 iv = sv.iv[i];
-                            if (!stream) {
-                                bool dup = false;
-                                for (uint32_t u = 0; u < ns; u++) dup |= s_lo[cur][u][tid] == iv.x;
-                                if (dup) continue;
-                                if (ns == (uint32_t)KT) { defer = true; why = 0; break; }
-                                s_lo[cur][ns][tid] = iv.x;
-                                s_hi[cur][ns][tid] = iv.y;
-                                s_rl[cur][ns][tid] = iv.z;
-                                ns++;
-                                continue;
-                            }
-                            uint32_t cl[16], ch[16];
-                            interval_counts<true>(fwd, L, iv, cl, ch);
-                            probes += 1u + ((iv.x >> kRecShift) != (iv.y >> kRecShift) ? 1u : 0u);
-                            uint32_t acc = 0;
-#pragma unroll
-                            for (int kk = 0; kk < 16; kk++) {
-                                const uint32_t d = ch[kk] - cl[kk];
-                                if (((cm1 >> kk) & 1u) && ch[kk] > cl[kk]) {
-                                    const uint4 c = bidir_child<true>(iv, s_clane[kk], cl[kk], ch[kk], acc);
-                                    bool dup = false;
-                                    for (uint32_t u = 0; u < ns; u++) dup |= s_lo[cur][u][tid] == c.x;
-                                    if (!dup) {
-                                        if (ns == (uint32_t)KT) {
-                                            defer = true;
-                                            why = 0;
-                                        } else {
-                                            s_lo[cur][ns][tid] = c.x;
-                                            s_hi[cur][ns][tid] = c.y;
-                                            s_rl[cur][ns][tid] = c.z;
-                                            ns++;
-                                        }
-                                    }
-                                }
-                                acc += d;
-                            }
-                        }
-                    }
-                }
-                if (defer) break;
-                if (lslmax < F) { // q[j-lslmax-1..j) does not occur: no long MEM starts before it
-                    s0 = j - lslmax;
-                    continue;
-                }
-                if (stream && ns == 0) { // q[j-F..j) occurs, q[j-F-1..j) does not: L(j) = j-F
-                    s0 = j - F;
-                    continue;
-                }
-                k = stream ? j - F - 1 : j - F;
-                seeded = true;
-            }
-        }
-        if (!seeded) {
-            plain++;
-            s_lo[cur][0][tid] = 0;
-            s_hi[cur][0][tid] = n;
-            s_rl[cur][0][tid] = 0;
-            ns = 1;
-            k = j;
-        }
-        bool ovf = false;
-        while (k > 0) {
-            const uint32_t nn = thread_extend<true>(fwd, L, s_clane, s_compat[p[k - 1]], s_lo[cur], s_hi[cur], s_rl[cur],
-                                                    s_lo[cur ^ 1], s_hi[cur ^ 1], s_rl[cur ^ 1], ns, tid, ovf, probes);
-            if (ovf) break;
-            if (nn == 0) break;
-            cur ^= 1;
-            ns = nn;
-            k--;
-        }
-        if (ovf) { defer = true; why = 1; break; }
-        if (k + min_len > j) {
-            s0 = k;
-            continue;
-        }
-        uint32_t e = j;
-        while (e < m) {
-            const uint32_t nn = thread_extend<false>(rev, L, s_clane, s_compat[p[e]], s_lo[cur], s_hi[cur], s_rl[cur],
-                                                     s_lo[cur ^ 1], s_hi[cur ^ 1], s_rl[cur ^ 1], ns, tid, ovf, probes);
-            if (ovf) break;
-            if (nn == 0) break;
-            cur ^= 1;
-            ns = nn;
-            e++;
-        }
-        if (ovf) { defer = true; why = 1; break; }
-        if (nrec + ns > slot_cap) { defer = true; why = 2; break; }
-        for (uint32_t t = 0; t < ns; t++) {
-            const uint32_t lo = s_lo[cur][t][tid];
-            uint32_t rank = 0;
-            for (uint32_t u = 0; u < ns; u++) rank += s_lo[cur][u][tid] < lo ? 1u : 0u;
-            slots[slot_base + nrec + rank] = make_uint4(k, e, lo, s_hi[cur][t][tid]);
-        }
-        nrec += ns;
-        s0 = k + 1;
-        E = e;
-    }
-    if (defer) {
-        status[r] = 1;
-        mem_cnt[r] = 0;
-        const unsigned long long at = atomicAdd(&ctr->n_retry, 1ull);
-        defer_list[at] = r;
-        atomicAdd(&ctr->diag[why], 1ull);
-    } else {
-        status[r] = 0;
-        mem_cnt[r] = nrec;
-    }
-    if (plain) atomicAdd(&ctr->diag[3], (unsigned long long)plain);
-    return probes;
-}
-
-// ---------------------------------------------------------------------------
-// Persistent MEM finder: the min_len-skip algorithm of mem_thread_read run as a
-// per-lane state machine.  Restarts seed q[j-F..j) from the packed F-mer table
+// Persistent MEM finder: the min_len-skip algorithm of k_mem_wave run as a
+// per-lane state machine, one work item (read segment) per lane.  Restarts seed q[j-F..j) from the packed F-mer table
 // (F = 12 or 14): one record load gives lsl, the interval count and, for most
 // F-mers, the single interval.  Every pass of the main loop performs at most one
 // Occ probe per lane (one interval of the current set, or one seed-table entry
@@ -819,7 +575,7 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
     __shared__ uint32_t s_clane[16];
     // per wave: the unclaimed rest [s_qb, s_qb + s_ql) of its chunk of work indices
     __shared__ uint32_t s_qb[PB / 64], s_ql[PB / 64];
-    __shared__ float s_lp[3 * kLsiMaxHidden + 1]; // stage 1 of the learned index
+    __shared__ float s_lp[LSI ? 3 * kLsiMaxHidden + 1 : 1]; // stage 1 of the learned index
     for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) {
         s_compat[i] = cst->compat_lanes[i];
         s_expand[i] = cst->seed_expand[i];
@@ -901,27 +657,29 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
                 slot_cap = (uint32_t)(((b0 + ib) >> 3) - ((b0 + ia) >> 3)) + 8u;
                 s0 = ia; E = 0; nrec = 0;
                 st = SM_CAND;
-            } else if (st == SM_CAND) {
-                j = (s0 + min_len > E + 1) ? s0 + min_len : E + 1;
-                if (min_len == 0 || j > m || s0 >= ib) { // no MEM starts in [s0, ib)
-                    status[w] = 0;
-                    mem_cnt[w] = nrec;
-                    st = SM_FETCH;
-                    continue;
+            } else if (st == SM_CAND || st == SM_RESTART) {
+                // SM_RESTART: a learned-seed lookup that did not give the set
+                if (st == SM_CAND) {
+                    j = (s0 + min_len > E + 1) ? s0 + min_len : E + 1;
+                    if (min_len == 0 || j > m || s0 >= ib) { // no MEM starts in [s0, ib)
+                        status[w] = 0;
+                        mem_cnt[w] = nrec;
+                        st = SM_FETCH;
+                        continue;
+                    }
+                    // learned seed of q[j-K..j) when all K codes are bases;
+                    // tried only where the window is likely to occur: on the
+                    // first restart of a work item, and past the code the last
+                    // MEM stopped at (E), so that it holds no known mismatch (an
+                    // absent key costs the whole lookup)
+                    unsigned long long key = 0;
+                    if (LSI && lsik && j >= lsik && (E == 0 ? s0 == ia : j - lsik > E) &&
+                        lsi_window_key(qnib, (uint64_t)(p - q) + j - lsik, lsik, key)) {
+                        lkey = key;
+                        st = SM_LSI_LEAF;
+                        continue;
+                    }
                 }
-                // learned seed of q[j-K..j) when all K codes are bases; tried
-                // only where the window is likely to occur: on the first
-                // restart of a work item, and past the code the last MEM
-                // stopped at (E), so that it holds no known mismatch (an
-                // absent key costs the whole lookup)
-                st = SM_RESTART;
-                unsigned long long key = 0;
-                if (LSI && lsik && j >= lsik && (E == 0 ? s0 == ia : j - lsik > E) &&
-                    lsi_window_key(qnib, (uint64_t)(p - q) + j - lsik, lsik, key)) {
-                    lkey = key;
-                    st = SM_LSI_LEAF;
-                }
-            } else if (st == SM_RESTART) {
                 // up to three ambiguity codes (64 expansions, one record load
                 // each); windows with more restart unseeded
                 bool usable = false;
@@ -1242,336 +1000,6 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
     if ((threadIdx.x & 63) == 0) atomicAdd(&ctr->probes, (unsigned long long)probes);
 }
 
-// ---------------------------------------------------------------------------
-// Pipelined persistent MEM finder: the state machine of k_mem_persist with
-// every memory access split from its use.  In each pass of the loop a lane
-// issues one round of independent loads -- one probe (two Occ records, a
-// superblock row and the next query code), or one control load (the read's
-// offsets, the 16 codes ending at the restart point, a seed-table header or
-// entry) -- and the wave then waits once.  Control loads are parked in a
-// per-lane LDS stage and consumed at the start of the next pass, so a restart
-// never stalls the other lanes of the wave.  Reads come from a device work
-// queue one read ahead.  Sets live in a ring of PR LDS slots per lane; reads
-// that outgrow it (or their record slots) are deferred.
-// ---------------------------------------------------------------------------
-enum : uint32_t {
-    MP_FETCH = 0, MP_HDR, MP_WIN, MP_SEEDX, MP_SEEDE, MP_SPROBE, MP_LEFT, MP_RIGHT, MP_DONE, // loads
-    MP_CAND, MP_SEEDNEXT, MP_ENDLEFT, MP_EMIT,                                              // no loads
-    MP_SEEDI // next seed entry: issue its load (set after a probe, so not yet loaded)
-};
-
-template <int PR, int PB>
-__global__ __launch_bounds__(PB) void k_mem_pipe(DevDir fwd, DevDir rev, const DevConst *__restrict__ cst,
-                                                 SeedView sv, const uint8_t *__restrict__ q,
-                                                 const uint64_t *__restrict__ qoff, uint32_t nq, uint32_t min_len,
-                                                 uint4 *__restrict__ slots, uint32_t *__restrict__ mem_cnt,
-                                                 uint8_t *__restrict__ status, uint32_t *__restrict__ defer_list,
-                                                 MemCounters *__restrict__ ctr) {
-    static_assert((PR & (PR - 1)) == 0, "ring size must be a power of two");
-    constexpr uint32_t RM = PR - 1;
-    __shared__ uint32_t s_lo[PR][PB], s_hi[PR][PB], s_rl[PR][PB];
-    __shared__ uint32_t s_stage[4][PB];
-    __shared__ uint16_t s_compat[256];
-    __shared__ uint8_t s_expand[256];
-    __shared__ uint32_t s_clane[16];
-    for (uint32_t x = threadIdx.x; x < 256; x += blockDim.x) {
-        s_compat[x] = cst->compat_lanes[x];
-        s_expand[x] = cst->seed_expand[x];
-    }
-    if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
-    __syncthreads();
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = __lane_id();
-    const uint32_t n = fwd.n;
-    const uint32_t F = sv.F;
-    const bool seeding = F != 0 && min_len > F && F + 1 <= 16;
-
-    uint32_t st = MP_FETCH, r = 0, nr = 0, m = 0, s0 = 0, E = 0, j = 0, k = 0, e = 0;
-    uint32_t ns = 0, nn = 0, t = 0, head = 0, nrec = 0, slot_cap = 0, cm1 = 0;
-    uint32_t xi = 0, aidx = 0, lslmax = 0, probes = 0, restarts = 0;
-    uint64_t slot_base = 0, ient = 0, iend = 0;
-    const uint8_t *p = q;
-    SeedWin sw{0, 0, 0x111u, 0};
-    uint4 piv = make_uint4(0, 0, 0, 0);
-    {
-        uint32_t b = 0;
-        if (lane == 0) b = (uint32_t)atomicAdd(&ctr->pwork, 64ull);
-        nr = __shfl(b, 0, 64) + lane;
-    }
-    uint32_t abase = 0, arank = 0, aleader = 0;
-    bool pending = false, want = false;
-
-    for (;;) {
-        if (pending) {
-            const uint32_t bb = __shfl(abase, (int)aleader, 64);
-            if (want) nr = bb + arank;
-            pending = false;
-        }
-        want = false;
-        // ---- consume staged control loads, run load-free transitions ----
-        for (;;) {
-            if (st == MP_HDR) {
-                const uint64_t b0 = (uint64_t)s_stage[0][tid] | ((uint64_t)s_stage[1][tid] << 32);
-                const uint64_t b1 = (uint64_t)s_stage[2][tid] | ((uint64_t)s_stage[3][tid] << 32);
-                m = (uint32_t)(b1 - b0);
-                p = q + b0;
-                slot_base = (b0 >> 3) + 8ull * r;
-                slot_cap = (uint32_t)((b1 >> 3) - (b0 >> 3)) + 8u;
-                s0 = 0; E = 0; nrec = 0;
-                st = MP_CAND;
-                continue;
-            } else if (st == MP_CAND) {
-                j = (s0 + min_len > E + 1) ? s0 + min_len : E + 1;
-                if (min_len == 0 || j > m) {
-                    status[r] = 0;
-                    mem_cnt[r] = nrec;
-                    st = MP_FETCH;
-                    continue;
-                }
-                nn = 0;
-                t = 0;
-                if (seeding && j > F) {
-                    st = MP_WIN; // load q[j-16..j)
-                    break;
-                }
-                restarts++;
-                s_lo[head][tid] = 0;
-                s_hi[head][tid] = n;
-                s_rl[head][tid] = 0;
-                ns = 1; k = j;
-                piv = make_uint4(0, n, 0, 0);
-                st = MP_LEFT;
-                break;
-            } else if (st == MP_WIN) {
-                uint32_t wb[16];
-#pragma unroll
-                for (int x = 0; x < 16; x++) wb[x] = (s_stage[x >> 2][tid] >> (8 * (x & 3))) & 0xffu;
-                // at most two ambiguity codes: a third makes the expansion loop
-                // long; such windows restart plain and end on the wave kernel
-                if (seed_window16(wb, F, s_expand, sw) && sw.namb <= 2) {
-                    uint32_t c = 0;
-#pragma unroll
-                    for (int x = 0; x < 16; x++)
-                        if ((uint32_t)x + F == 15) c = wb[x]; // q[j-F-1]
-                    cm1 = s_compat[c];
-                    ns = 0; // seed entries come from the table, not the ring
-                    xi = 0;
-                    lslmax = 0;
-                    st = MP_SEEDNEXT;
-                    continue;
-                }
-                restarts++;
-                s_lo[head][tid] = 0;
-                s_hi[head][tid] = n;
-                s_rl[head][tid] = 0;
-                ns = 1; k = j;
-                piv = make_uint4(0, n, 0, 0);
-                st = MP_LEFT;
-                break;
-            } else if (st == MP_SEEDNEXT) {
-                uint32_t idx = 0;
-                while (xi < seed_ncombo(sw) && !seed_combo(sw, xi, idx)) xi++;
-                if (xi < seed_ncombo(sw)) {
-                    xi++;
-                    aidx = idx;
-                    st = MP_SEEDX;
-                    break;
-                }
-                if (lslmax < F) { // q[j-lslmax-1..j) does not occur: L(j) = j - lslmax
-                    s0 = j - lslmax;
-                    st = MP_CAND;
-                    continue;
-                }
-                if (nn == 0) { // q[j-F..j) occurs, q[j-F-1..j) does not: L(j) = j - F
-                    s0 = j - F;
-                    st = MP_CAND;
-                    continue;
-                }
-                ns = nn; nn = 0; t = 0;
-                k = j - F - 1;
-                if (k == 0) {
-                    st = MP_ENDLEFT;
-                    continue;
-                }
-                piv = make_uint4(s_lo[head][tid], s_hi[head][tid], s_rl[head][tid], 0u);
-                st = MP_LEFT;
-                break;
-            } else if (st == MP_SEEDX) {
-                probes += 2;
-                const uint32_t l = s_stage[3][tid];
-                lslmax = l > lslmax ? l : lslmax;
-                ient = (uint64_t)s_stage[0][tid] | ((uint64_t)s_stage[1][tid] << 32);
-                iend = ient + s_stage[2][tid];
-                if (l == F && ient < iend) {
-                    st = MP_SEEDE;
-                    break;
-                }
-                st = MP_SEEDNEXT;
-                continue;
-            } else if (st == MP_SEEDE) {
-                piv = make_uint4(s_stage[0][tid], s_stage[1][tid], s_stage[2][tid], 0u);
-                st = MP_SPROBE;
-                break;
-            } else if (st == MP_SEEDI) {
-                st = MP_SEEDE; // issue the entry load this pass
-                break;
-            } else if (st == MP_ENDLEFT) {
-                if (k + min_len > j) {
-                    s0 = k;
-                    st = MP_CAND;
-                    continue;
-                }
-                e = j; t = 0; nn = 0;
-                if (e < m) {
-                    piv = make_uint4(s_lo[head][tid], s_hi[head][tid], s_rl[head][tid], 0u);
-                    st = MP_RIGHT;
-                    break;
-                }
-                st = MP_EMIT;
-                continue;
-            } else if (st == MP_EMIT) {
-                if (nrec + ns > slot_cap) {
-                    status[r] = 1;
-                    mem_cnt[r] = 0;
-                    defer_list[atomicAdd(&ctr->n_retry, 1ull)] = r;
-                    atomicAdd(&ctr->diag[2], 1ull);
-                    st = MP_FETCH;
-                    continue;
-                }
-                for (uint32_t u = 0; u < ns; u++) {
-                    const uint32_t lo = s_lo[(head + u) & RM][tid];
-                    uint32_t rank = 0;
-                    for (uint32_t v = 0; v < ns; v++) rank += s_lo[(head + v) & RM][tid] < lo ? 1u : 0u;
-                    slots[slot_base + nrec + rank] = make_uint4(k, e, lo, s_hi[(head + u) & RM][tid]);
-                }
-                nrec += ns;
-                s0 = k + 1;
-                E = e;
-                st = MP_CAND;
-                continue;
-            } else if (st == MP_FETCH) {
-                if (nr >= nq) {
-                    st = MP_DONE;
-                } else {
-                    r = nr;
-                    want = true;
-                    st = MP_HDR;
-                }
-                break;
-            } else {
-                break; // probe states, MP_DONE
-            }
-        }
-        const uint64_t act = __ballot(want);
-        if (act) {
-            aleader = (uint32_t)__ffsll((unsigned long long)act) - 1u;
-            arank = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
-            if (lane == aleader) abase = (uint32_t)atomicAdd(&ctr->pwork, (unsigned long long)__popcll(act));
-            pending = true;
-        }
-        if (__ballot(st != MP_DONE) == 0) break;
-
-        // ---- issue this pass's loads ----
-        PairRaw w;
-        uint32_t qc = 0;
-        const bool probe = st == MP_SPROBE || st == MP_LEFT || st == MP_RIGHT;
-        const bool left = st != MP_RIGHT;
-        const DevDir &D = left ? fwd : rev;
-        const uint32_t xlo = left ? piv.x : piv.z;
-        const uint32_t xhi = left ? piv.y : piv.z + (piv.y - piv.x);
-        if (st == MP_HDR) {
-            const uint64_t b0 = qoff[r], b1 = qoff[r + 1];
-            w.l0 = make_uint4((uint32_t)b0, (uint32_t)(b0 >> 32), (uint32_t)b1, (uint32_t)(b1 >> 32));
-        } else if (st == MP_WIN) {
-            w.l0.x = j >= 16 ? p[j - 16] : 0u; w.l0.y = j >= 15 ? p[j - 15] : 0u;
-            w.l0.z = j >= 14 ? p[j - 14] : 0u; w.l0.w = j >= 13 ? p[j - 13] : 0u;
-            w.l1.x = j >= 12 ? p[j - 12] : 0u; w.l1.y = j >= 11 ? p[j - 11] : 0u;
-            w.l1.z = j >= 10 ? p[j - 10] : 0u; w.l1.w = j >= 9 ? p[j - 9] : 0u;
-            w.l2.x = j >= 8 ? p[j - 8] : 0u; w.l2.y = j >= 7 ? p[j - 7] : 0u;
-            w.l2.z = j >= 6 ? p[j - 6] : 0u; w.l2.w = j >= 5 ? p[j - 5] : 0u;
-            w.l3.x = j >= 4 ? p[j - 4] : 0u; w.l3.y = j >= 3 ? p[j - 3] : 0u;
-            w.l3.z = j >= 2 ? p[j - 2] : 0u; w.l3.w = p[j - 1];
-        } else if (st == MP_SEEDX) {
-            const uint64_t o0 = sv.off[aidx], o1 = sv.off[aidx + 1];
-            w.l0 = make_uint4((uint32_t)o0, (uint32_t)(o0 >> 32), (uint32_t)(o1 - o0), sv.lsl[aidx]);
-        } else if (st == MP_SEEDE) {
-            w.l0 = sv.iv[ient];
-        } else if (probe) {
-            pair_issue(D, xlo, xhi, w);
-            if (st != MP_SPROBE) qc = p[left ? k - 1 : e];
-            probes += 1u + ((xlo >> kRecShift) != (xhi >> kRecShift) ? 1u : 0u);
-        }
-
-        // ---- one wait: evaluate the probe, park the control loads ----
-        if (probe) {
-            const uint32_t cm = st == MP_SPROBE ? cm1 : s_compat[qc];
-            const uint32_t cbase = head + ns;
-            // slots not held by parents still to be processed (this one is in piv)
-            const uint32_t room = st == MP_SPROBE ? PR : PR - (ns - t - 1);
-            const bool dedup = st == MP_SPROBE;
-            bool ovf = false;
-            uint32_t acc = 0;
-            pair_visit_fused(D, w, xlo, xhi, [&](int kk, uint32_t a, uint32_t b) {
-                const uint32_t dd = b - a;
-                if (((cm >> kk) & 1u) && dd != 0u) {
-                    const uint32_t ca = s_clane[kk] + a;
-                    const uint4 c = left ? make_uint4(ca, ca + dd, piv.z + acc, 0u)
-                                         : make_uint4(piv.x + acc, piv.x + acc + dd, ca, 0u);
-                    bool dup = false;
-                    if (dedup)
-                        for (uint32_t u = 0; u < nn; u++) dup |= s_lo[(cbase + u) & RM][tid] == c.x;
-                    if (!dup) {
-                        if (nn + 1 > room) {
-                            ovf = true;
-                        } else {
-                            const uint32_t x = (cbase + nn) & RM;
-                            s_lo[x][tid] = c.x;
-                            s_hi[x][tid] = c.y;
-                            s_rl[x][tid] = c.z;
-                            nn++;
-                        }
-                    }
-                }
-                acc += dd;
-            });
-            if (ovf) {
-                status[r] = 1;
-                mem_cnt[r] = 0;
-                defer_list[atomicAdd(&ctr->n_retry, 1ull)] = r;
-                atomicAdd(&ctr->diag[st == MP_SPROBE ? 0 : 1], 1ull);
-                st = MP_FETCH;
-            } else if (st == MP_SPROBE) {
-                st = ++ient < iend ? MP_SEEDI : MP_SEEDNEXT;
-            } else if (++t < ns) {
-                const uint32_t x = (head + t) & RM;
-                piv = make_uint4(s_lo[x][tid], s_hi[x][tid], s_rl[x][tid], 0u);
-            } else if (nn == 0) {
-                st = st == MP_LEFT ? MP_ENDLEFT : MP_EMIT;
-            } else {
-                head = (head + ns) & RM; ns = nn; nn = 0; t = 0;
-                bool more;
-                if (st == MP_LEFT) { k--; more = k > 0; if (!more) st = MP_ENDLEFT; }
-                else { e++; more = e < m; if (!more) st = MP_EMIT; }
-                if (more) piv = make_uint4(s_lo[head][tid], s_hi[head][tid], s_rl[head][tid], 0u);
-            }
-        } else if (st == MP_WIN) {
-            s_stage[0][tid] = w.l0.x | (w.l0.y << 8) | (w.l0.z << 16) | (w.l0.w << 24);
-            s_stage[1][tid] = w.l1.x | (w.l1.y << 8) | (w.l1.z << 16) | (w.l1.w << 24);
-            s_stage[2][tid] = w.l2.x | (w.l2.y << 8) | (w.l2.z << 16) | (w.l2.w << 24);
-            s_stage[3][tid] = w.l3.x | (w.l3.y << 8) | (w.l3.z << 16) | (w.l3.w << 24);
-        } else if (st != MP_DONE) {
-            s_stage[0][tid] = w.l0.x;
-            s_stage[1][tid] = w.l0.y;
-            s_stage[2][tid] = w.l0.z;
-            s_stage[3][tid] = w.l0.w;
-        }
-    }
-    if (restarts) atomicAdd(&ctr->diag[3], (unsigned long long)restarts);
-#pragma unroll
-    for (int dd = 32; dd >= 1; dd >>= 1) probes += __shfl_xor(probes, dd, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&ctr->probes, (unsigned long long)probes);
-}
-
 // Copy the records of thread-path reads from their slots to the final order.
 __global__ __launch_bounds__(256) void k_mem_gather(uint32_t nw, const uint64_t *__restrict__ qoff,
                                                     const uint4 *__restrict__ items,
@@ -1830,18 +1258,20 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     uint64_t h[16];
     if ((rc = ensure_sa_full(idx))) return rc;
 
-    // path choice: the thread-per-read kernel seeds every restart from the
-    // 12-mer table, which needs min_len >= 12; shorter min_len (and
-    // HSFM_MEM_PATH=wave) run every read on the wave kernel.
+    // path choice: the persistent kernel (one lane per work item) for min_len
+    // >= 12, with its deferred items on the workgroup kernel; shorter min_len
+    // (and HSFM_MEM_PATH=wave, which tests hold equal) run every read on the
+    // workgroup kernel.  (A thread-per-read kernel with 8-interval sets and a
+    // fully pipelined variant of the persistent kernel measured slower and are
+    // not kept.)
     const uint32_t F = 12;
     const char *path = getenv("HSFM_MEM_PATH");
     const bool use_thread = nq > 0 && min_len >= F && !(path && strcmp(path, "wave") == 0);
-    const bool legacy = path && (strcmp(path, "thread") == 0 || strcmp(path, "pipe") == 0);
     const uint32_t FP = mem_seed_length(idx, min_len); // seed length of the persistent kernel
 
     // work items: reads longer than seg codes are split so that one long read
     // does not leave a single lane working after all others are done
-    const uint32_t seg = legacy ? 0u : mem_segment_length();
+    const uint32_t seg = mem_segment_length();
     uint64_t W = nq;
     const uint4 *items = nullptr;
     uint64_t total_len = 0; // flat length of the queries (read back with the item count)
@@ -1914,22 +1344,9 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
             svp.F = 0; // restarts not seeded by the tables
             svw = svp;
         }
-        const bool old_thread = path && strcmp(path, "thread") == 0;
         uint64_t hd[16];
         if ((rc = event_record(idx, 0, false))) return rc;
-        if (old_thread) {
-            hipLaunchKernelGGL(k_mem_thread, dim3((nq + TPB - 1) / TPB), dim3(TPB), 0, s, idx->fwd.view, idx->rev.view,
-                               idx->d_const.as<DevConst>(), sv, d_q, d_qoff, nq, min_len, ws.mem_slots.as<uint4>(),
-                               ws.mem_cnt.as<uint32_t>(), ws.mem_status.as<uint8_t>(), ws.ovf_list.as<uint32_t>(),
-                               ctr);
-        } else if (path && strcmp(path, "pipe") == 0) {
-            const uint32_t g1 = persist_grid(idx, k_mem_pipe<16, 256>, 256);
-            const uint32_t need = (nq + 255) / 256;
-            hipLaunchKernelGGL((k_mem_pipe<16, 256>), dim3(need < g1 ? need : g1), dim3(256), 0, s, idx->fwd.view,
-                               idx->rev.view, idx->d_const.as<DevConst>(), sv, d_q, d_qoff, nq, min_len,
-                               ws.mem_slots.as<uint4>(), ws.mem_cnt.as<uint32_t>(), ws.mem_status.as<uint8_t>(),
-                               ws.ovf_list.as<uint32_t>(), ctr);
-        } else {
+        {
             // the learned-seed states are compiled into their own instance, so
             // the other modes run the kernel without them
             auto kern = lsik ? k_mem_persist<16, 256, true> : k_mem_persist<16, 256, false>;

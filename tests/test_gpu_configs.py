@@ -21,9 +21,10 @@
               the oracle on a sample.
 The learned-seed workload (1 Gbp) has its own test file.
 
-Index bytes come from the library's host builder, whose output equals the
-oracle's (tests/test_index_build.py) and whose format is pinned by the
-specification (tests/test_index_bytes_spec.py).
+Index bytes come from the library's device builder, checked here against the
+host builder at 100 Mbp; both equal the oracle's (tests/test_index_build.py,
+tests/test_gpu_construct.py) and the format is pinned by the specification
+(tests/test_index_bytes_spec.py).
 """
 import os
 
@@ -69,7 +70,9 @@ def test_workload1_count_and_locate_1Mbp():
 def ref100():
     text, bounds = IB.synth_reference(0xC0FFEE, 100_000_000, 8, gc=0.41, repeat_frac=0.20, n_units=200,
                                       unit_min=300, unit_max=6000, div_min=0.05, div_max=0.15, iupac_frac=0.005)
-    data = IB.build_bidir_bytes(text, bounds, sample_rate=32, lookup_depth=12, threads=THREADS)
+    data = hf.build_index_bytes_gpu(text, bounds, sample_rate=32, lookup_depth=12, bidir=True)
+    # the device build at full size equals the host builder's bytes
+    assert data == IB.build_bidir_bytes(text, bounds, sample_rate=32, lookup_depth=12, threads=THREADS)
     _log(f"100 Mbp bidirectional index: {len(data) / 1e6:.0f} MB")
     g = hf.BidirFmIndex.from_bytes(data)
     ob = O.OracleBidir.from_bytes(data)

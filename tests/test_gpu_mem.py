@@ -159,8 +159,8 @@ def _mem_reads(rng, text, nreads, length, sub, amb):
 
 @pytest.mark.parametrize("seed", range(2))
 def test_thread_path_seed_table(seed):
-    """min_len >= 12 runs the thread-per-read kernel seeded from the 12-mer
-    table, including ambiguity codes in the seed windows."""
+    """min_len >= 12 runs the persistent kernel seeded from the packed 12-mer
+    records, including ambiguity codes in the seed windows."""
     rng = np.random.default_rng(700 + seed)
     text, bounds = O.gen_reference(40 + seed, 300_000, 4, gc=0.41, repeat_frac=0.2, n_units=30, unit_min=300,
                                    unit_max=2000, iupac_frac=0.005)
@@ -171,8 +171,10 @@ def test_thread_path_seed_table(seed):
     check(o, g, qs, 12 + 3 * seed)
 
 
-@pytest.mark.parametrize("other", ["wave", "thread", "pipe"])
+@pytest.mark.parametrize("other", ["wave"])
 def test_persistent_and_other_paths_agree(monkeypatch, other):
+    """The persistent kernel (with its deferred items on the workgroup
+    kernel) and the workgroup kernel for every read give identical records."""
     rng = np.random.default_rng(11)
     text, bounds = O.gen_reference(9, 200_000, 3, gc=0.41, repeat_frac=0.2, n_units=20, unit_min=300,
                                    unit_max=1500, iupac_frac=0.01)
