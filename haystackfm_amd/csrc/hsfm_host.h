@@ -102,9 +102,9 @@ struct hsfm_index {
     // bidirectional seed table of every ACGT F-mer (built on first MEM call)
     hsfm::DevBuf seed_off, seed_iv, seed_lsl;
     uint32_t seed_F = 0;
-    // packed one-load records of the 12-mer table and of the 14-mer table
-    // (with the 14-mers' own interval list)
-    hsfm::DevBuf seed_rec12, seed_rec14, seed_iv14;
+    // packed one-load records of the 12-, 14- and 15-mer tables (the 14- and
+    // 15-mers with their own interval lists)
+    hsfm::DevBuf seed_rec12, seed_rec14, seed_iv14, seed_rec15, seed_iv15;
     // batched MEM pipeline: configuration picked by the autotuner
     int tuned_main = -1, tuned_rpt = -1;
     uint32_t tuned_min_len = 0, tuned_batch = 0;

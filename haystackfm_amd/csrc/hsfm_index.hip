@@ -429,6 +429,8 @@ extern "C" int hsfm_index_free(hsfm_index *idx) {
     idx->seed_rec12.release();
     idx->seed_rec14.release();
     idx->seed_iv14.release();
+    idx->seed_rec15.release();
+    idx->seed_iv15.release();
     for (int k = 0; k < hsfm_index::kMaxBatchSets; k++) {
         free_batch_bufs(idx->batch[k]);
         if (idx->bstream[k]) (void)hipStreamDestroy(idx->bstream[k]);

@@ -1188,20 +1188,26 @@ static int seed_pack(hsfm_index *idx, uint32_t F, const DevBuf &off, const DevBu
     const uint32_t nchild = 1u << (2 * F);
     DevBuf flag;
     int rc;
-    if ((rc = rec.ensure((size_t)nchild * 16)) || (rc = flag.ensure(4))) return rc;
-    if ((rc = fill_u32(idx, flag.as<uint32_t>(), 1, 0))) return rc;
-    hipLaunchKernelGGL(k_seed_pack, dim3((nchild + 255) / 256), dim3(256), 0, s, nchild, off.as<uint64_t>(),
-                       iv.as<uint4>(), lsl.as<uint8_t>(), rec.as<uint4>(), flag.as<uint32_t>());
-    HSFM_HIP(hipGetLastError());
-    uint32_t bad = 0;
-    HSFM_HIP(hipMemcpyAsync(&bad, flag.p, 4, hipMemcpyDeviceToHost, s));
-    HSFM_HIP(hipStreamSynchronize(s));
-    flag.release();
-    if (bad) {
-        rec.release();
-        return set_error(HSFM_EINVAL, "a %u-mer has more compatible intervals than the seed record holds", F);
+    if ((rc = flag.ensure(4))) return rc;
+    if ((rc = rec.ensure((size_t)nchild * 16))) {
+        flag.release();
+        return rc;
     }
-    return HSFM_OK;
+    // on any failure the records are released, so that a later call builds them again
+    uint32_t bad = 0;
+    rc = fill_u32(idx, flag.as<uint32_t>(), 1, 0);
+    if (!rc) {
+        hipLaunchKernelGGL(k_seed_pack, dim3((nchild + 255) / 256), dim3(256), 0, s, nchild, off.as<uint64_t>(),
+                           iv.as<uint4>(), lsl.as<uint8_t>(), rec.as<uint4>(), flag.as<uint32_t>());
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(&bad, flag.p, 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = set_error(HSFM_EDEVICE, "seed records of %u-mers: %s", F, hipGetErrorString(e));
+    }
+    flag.release();
+    if (!rc && bad) rc = set_error(HSFM_EINVAL, "a %u-mer has more compatible intervals than the seed record holds", F);
+    if (rc) rec.release();
+    return rc;
 }
 
 int build_seed_packed(hsfm_index *idx, uint32_t F) {
@@ -1211,30 +1217,70 @@ int build_seed_packed(hsfm_index *idx, uint32_t F) {
         if (idx->seed_rec12.p) return HSFM_OK;
         return seed_pack(idx, 12, idx->seed_off, idx->seed_iv, idx->seed_lsl, idx->seed_rec12);
     }
-    if (F != 14) return set_error(HSFM_EINVAL, "seed length %u not supported", F);
-    if (idx->seed_rec14.p) return HSFM_OK;
-    // levels 13 and 14 from the 12-mer table (which stays as it is)
-    DevBuf cnt, off13, iv13, lsl13, off14, iv14, lsl14;
-    rc = seed_level(idx, 13, idx->seed_off, idx->seed_iv, idx->seed_lsl, cnt, off13, iv13, lsl13);
-    if (!rc) rc = seed_level(idx, 14, off13, iv13, lsl13, cnt, off14, iv14, lsl14);
-    cnt.release(); off13.release(); iv13.release(); lsl13.release();
-    if (!rc) rc = seed_pack(idx, 14, off14, iv14, lsl14, idx->seed_rec14);
-    off14.release(); lsl14.release();
+    if (F != 14 && F != 15) return set_error(HSFM_EINVAL, "seed length %u not supported", F);
+    DevBuf &rec = F == 14 ? idx->seed_rec14 : idx->seed_rec15;
+    DevBuf &ivs = F == 14 ? idx->seed_iv14 : idx->seed_iv15;
+    if (rec.p) return HSFM_OK;
+    // levels 13..F from the 12-mer table (which stays as it is)
+    DevBuf cnt, poff, piv, plsl, coff, civ, clsl;
+    for (uint32_t l = 13; l <= F && !rc; l++) {
+        rc = seed_level(idx, l, l == 13 ? idx->seed_off : poff, l == 13 ? idx->seed_iv : piv,
+                        l == 13 ? idx->seed_lsl : plsl, cnt, coff, civ, clsl);
+        if (l > 13) {
+            poff.release(); piv.release(); plsl.release();
+        }
+        poff = coff; piv = civ; plsl = clsl;
+        coff = DevBuf(); civ = DevBuf(); clsl = DevBuf();
+    }
+    cnt.release();
+    if (!rc) rc = seed_pack(idx, F, poff, piv, plsl, rec);
+    poff.release(); plsl.release();
     if (rc) {
-        iv14.release();
+        piv.release();
         return rc;
     }
-    idx->seed_iv14.release();
-    idx->seed_iv14 = iv14;
+    ivs.release();
+    ivs = piv;
     return HSFM_OK;
 }
 
+// Packed-record view of the FP-mer table (built by build_seed_packed); sv is
+// the 12-mer table's view.  FP = 0: no seeding.
+static SeedView packed_view(hsfm_index *idx, uint32_t FP, const SeedView &sv) {
+    if (FP == 15) return SeedView{nullptr, idx->seed_iv15.as<uint4>(), nullptr, 15u, idx->seed_rec15.as<uint4>()};
+    if (FP == 14) return SeedView{nullptr, idx->seed_iv14.as<uint4>(), nullptr, 14u, idx->seed_rec14.as<uint4>()};
+    SeedView v = sv;
+    if (FP == 12) v.rec = idx->seed_rec12.as<uint4>();
+    else v.F = 0;
+    return v;
+}
+
+// The packed table of FP-mers, or of 14-mers when the 15-mer table (17 GB)
+// does not fit in device memory: the seed length used (0: none).
+static int packed_seeds(hsfm_index *idx, uint32_t &FP) {
+    if (!FP) return HSFM_OK;
+    int rc = build_seed_packed(idx, FP);
+    if (rc == HSFM_ENOMEM && FP == 15) {
+        FP = 14;
+        rc = build_seed_packed(idx, FP);
+    }
+    return rc;
+}
+
 // 14-mer seeds pay off (fewer, smaller restart sets) once 12-mers repeat by
-// chance: texts of 2^22 positions and more.  HSFM_SEED_F=12 / 14 overrides.
+// chance: texts of 2^22 positions and more.  From 2^24 positions the 15-mer
+// table (2^30 records, 17 GB) also takes the first left extension of every
+// restart whose 14-mer occurs.  HSFM_SEED_F=12 / 14 / 15 overrides.  Seeds
+// are shorter than min_len.
 uint32_t mem_seed_length(hsfm_index *idx, uint32_t min_len) {
-    uint32_t F = idx->fwd.view.n >= (1u << 22) ? 14u : 12u;
+    const uint32_t n = idx->fwd.view.n;
+    uint32_t F = n >= (1u << 24) ? 15u : n >= (1u << 22) ? 14u : 12u;
     const char *e = getenv("HSFM_SEED_F");
-    if (e && *e) F = strtol(e, nullptr, 10) == 14 ? 14u : 12u;
+    if (e && *e) {
+        const long v = strtol(e, nullptr, 10);
+        F = v == 15 ? 15u : v == 14 ? 14u : 12u;
+    }
+    if (F == 15 && min_len <= 15) F = 14;
     if (F == 14 && min_len <= 14) F = 12;
     return min_len > 12 ? F : 0u;
 }
@@ -1267,7 +1313,7 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     const uint32_t F = 12;
     const char *path = getenv("HSFM_MEM_PATH");
     const bool use_thread = nq > 0 && min_len >= F && !(path && strcmp(path, "wave") == 0);
-    const uint32_t FP = mem_seed_length(idx, min_len); // seed length of the persistent kernel
+    uint32_t FP = mem_seed_length(idx, min_len); // seed length of the persistent kernel
 
     // work items: reads longer than seg codes are split so that one long read
     // does not leave a single lane working after all others are done
@@ -1323,14 +1369,8 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         if ((rc = pack_bases(idx, d_q, total_len, ws.qnib))) return rc;
         if ((rc = ws.mem_status.ensure(W + 16))) return rc;
         sv = SeedView{idx->seed_off.as<uint64_t>(), idx->seed_iv.as<uint4>(), idx->seed_lsl.as<uint8_t>(), F};
-        SeedView svp = sv;
-        if (FP && (rc = build_seed_packed(idx, FP))) return rc;
-        if (FP == 14)
-            svp = SeedView{nullptr, idx->seed_iv14.as<uint4>(), nullptr, 14u, idx->seed_rec14.as<uint4>()};
-        else if (FP == 12)
-            svp.rec = idx->seed_rec12.as<uint4>();
-        else
-            svp.F = 0; // min_len 12: no seeding in the persistent kernel
+        if ((rc = packed_seeds(idx, FP))) return rc;
+        SeedView svp = packed_view(idx, FP, sv); // FP = 0 (min_len 12): no seeding in the persistent kernel
         svw = svp;
         // restart seeding chosen by hsfm_mem_seed_mode
         LsiView lv{};
@@ -1876,10 +1916,9 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     if ((rc = ensure_sa_full(idx))) return rc;
     if ((rc = build_seed_table(idx, F))) return rc;
     SeedView sv{idx->seed_off.as<uint64_t>(), idx->seed_iv.as<uint4>(), idx->seed_lsl.as<uint8_t>(), F};
-    const uint32_t FP = mem_seed_length(idx, min_len);
-    if ((rc = build_seed_packed(idx, FP))) return rc;
-    const SeedView svp = FP == 14 ? SeedView{nullptr, idx->seed_iv14.as<uint4>(), nullptr, 14u, idx->seed_rec14.as<uint4>()}
-                                  : SeedView{sv.off, sv.iv, sv.lsl, F, idx->seed_rec12.as<uint4>()};
+    uint32_t FP = mem_seed_length(idx, min_len);
+    if ((rc = packed_seeds(idx, FP))) return rc;
+    const SeedView svp = packed_view(idx, FP, sv);
     // capacities from the host copy of the offsets
     std::vector<uint64_t> hoff((size_t)nq + 1);
     HSFM_HIP(hipMemcpyAsync(hoff.data(), d_qoff, ((size_t)nq + 1) * 8, hipMemcpyDeviceToHost, s));

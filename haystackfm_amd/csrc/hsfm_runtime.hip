@@ -37,6 +37,7 @@ int DevBuf::ensure(size_t bytes) {
     hipError_t e = hipMalloc(&p, want);
     if (e != hipSuccess) {
         p = nullptr;
+        (void)hipGetLastError(); // a later launch check must not report this failure again
         return set_error(HSFM_ENOMEM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
     }
     cap = want;

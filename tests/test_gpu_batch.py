@@ -90,13 +90,14 @@ def test_oversized_batch_runs_host_driven():
     assert got[3]["fallback_batches"] >= 1
 
 
-def test_batched_with_14mer_seeds(uneven, monkeypatch):
-    """The batched pipeline seeded from packed 14-mer records returns what the
-    one-shot entry point returns with 12-mer seeds."""
+@pytest.mark.parametrize("seed_f", ["14", "15"])
+def test_batched_with_longer_seeds(uneven, monkeypatch, seed_f):
+    """The batched pipeline seeded from packed 14- or 15-mer records returns
+    what the one-shot entry point returns with 12-mer seeds."""
     _, g, (flat, off), bounds = uneven
     monkeypatch.setenv("HSFM_SEED_F", "12")
     ref = g.find_mems_batch((flat, off), 19, hf.HSFM_MODE_SMEM, bounds, 1024)
-    monkeypatch.setenv("HSFM_SEED_F", "14")
+    monkeypatch.setenv("HSFM_SEED_F", seed_f)
     got = g.find_mems_batched((flat, off), 19, hf.HSFM_MODE_SMEM, bounds, 1024, batch_reads=700, main_kernel=1,
                               resolve_fanout=1)
     _same(got[:3], ref)

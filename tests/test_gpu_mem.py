@@ -189,10 +189,12 @@ def test_persistent_and_other_paths_agree(monkeypatch, other):
 
 
 @pytest.mark.parametrize("iupac", [0.005, 0.04])
-def test_packed_seed_tables_12_and_14(monkeypatch, iupac):
-    """Restarts seeded from the packed 12-mer and 14-mer records (one interval
-    inline, several through the interval list) give the oracle's MEMs; an
-    IUPAC-rich text makes many 14-mers own several compatible intervals."""
+def test_packed_seed_tables_12_14_15(monkeypatch, iupac):
+    """Restarts seeded from the packed 12-, 14- and 15-mer records (one
+    interval inline, several through the interval list) give the oracle's
+    MEMs; an IUPAC-rich text makes many 14- and 15-mers own several compatible
+    intervals.  min_len 15 runs the 15-mer setting on 14-mers (seeds are
+    shorter than min_len)."""
     rng = np.random.default_rng(57)
     text, bounds = O.gen_reference(58, 300_000, 4, gc=0.41, repeat_frac=0.2, n_units=30, unit_min=300,
                                    unit_max=2000, iupac_frac=iupac)
@@ -200,18 +202,18 @@ def test_packed_seed_tables_12_and_14(monkeypatch, iupac):
     g = hf.BidirFmIndex.from_bytes(o.to_bytes())
     qs = _mem_reads(rng, text, 300, 150, 0.02, 0.01)
     qs += [random_text(rng, 150, alphabet="ACGT") for _ in range(20)]
-    for F in ("12", "14"):
+    for F in ("12", "14", "15"):
         monkeypatch.setenv("HSFM_SEED_F", F)
         for min_len in (15, 19):
             check(o, g, qs, min_len, modes=(0,))
 
 
-@pytest.mark.parametrize("seed_f", ["12", "14"])
+@pytest.mark.parametrize("seed_f", ["12", "14", "15"])
 def test_ambiguity_clusters_in_seed_windows(monkeypatch, seed_f):
     """Reads with 4-7 ambiguity codes inside one seed window: the ring kernel
     hands them to the workgroup kernel, which expands up to five codes over its
-    threads (one packed 12- or 14-mer record each) and restarts unseeded beyond
-    that; all stay exact."""
+    threads (one packed 12-, 14- or 15-mer record each) and restarts unseeded
+    beyond that; all stay exact."""
     monkeypatch.setenv("HSFM_SEED_F", seed_f)
     rng = np.random.default_rng(77)
     text, bounds = O.gen_reference(78, 200_000, 3, gc=0.41, repeat_frac=0.2, n_units=20, unit_min=300,
