@@ -44,6 +44,7 @@ struct DirBuffers {
     DevBuf sa_full;           // every row's SA value (see ensure_sa_full)
     DevBuf text;              // the text as lanes, from sa_full
     DevBuf isa;               // row of every text position, from sa_full
+    DevBuf text4;             // text as base-set nibbles (DevDir::text4), from text
     bool sa_full_tried = false;
     DevDir view{};
 };

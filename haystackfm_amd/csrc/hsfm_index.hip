@@ -242,10 +242,12 @@ void free_dir(DirBuffers &d) {
     d.lookup.release();
     d.sa_full.release();
     d.text.release();
+    d.text4.release();
     d.isa.release();
     d.sa_full_tried = false;
     d.view.sa_full = nullptr;
     d.view.text = nullptr;
+    d.view.text4 = nullptr;
     d.view.isa = nullptr;
 }
 
@@ -380,6 +382,20 @@ extern "C" int hsfm_index_upload(hsfm_device *dev, const hsfm_index_view *fwd, c
                 if (bases && uni == mb) e = bases;
             }
             c.seed_expand[b] = e;
+        }
+        // text lanes as base sets, and whether nibble comparison equals the
+        // compat table for every query byte
+        for (uint32_t l = 0; l < L && l < 16; l++) {
+            uint8_t tn = 0;
+            for (int x = 0; x < 4; x++)
+                if ((c.compat_lanes[x + 1] >> l) & 1u) tn |= (uint8_t)(1u << x);
+            c.text_nib[l] = tn;
+        }
+        c.nib_text = 1;
+        for (int b = 0; b < 256 && c.nib_text; b++) {
+            const uint32_t qn = (b >= 1 && b <= 4) ? 1u << (b - 1) : (uint32_t)(c.seed_expand[b] & 15u);
+            for (uint32_t l = 0; l < L && l < 16; l++)
+                if ((((c.compat_lanes[b] >> l) & 1u) != 0) != ((qn & c.text_nib[l]) != 0)) c.nib_text = 0;
         }
         if (fwd->has_lookup) {
             c.lookup_depth = fwd->lookup_depth;

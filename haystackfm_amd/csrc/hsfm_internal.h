@@ -48,6 +48,13 @@ struct DevConst {
     // bits 0-1; else bits 0-3 = set of bases whose compat sets union to this
     // byte's compat set (IupacDna ambiguity codes); 0 = no expansion.
     uint8_t seed_expand[256];
+    // Base set of each text lane for matching (bit x: the lane is compatible
+    // with query base x); nib_text = 1 when, for every query byte b, lane l is
+    // compatible with b exactly when nibble(b) & text_nib[l] != 0, nibble(b)
+    // being b's base-set nibble of the query nibble stream (true for IupacDna
+    // and ExactDna), so texts and queries can be compared as nibbles.
+    uint8_t text_nib[16];
+    uint32_t nib_text;
 };
 
 // Device view of one index direction (passed by value to kernels).
@@ -65,6 +72,10 @@ struct DevDir {
     const uint8_t *text;
     // row of every text position (the inverse of sa_full), or null
     const uint32_t *isa;
+    // text_nib of every text position's lane, eight per word (position i in
+    // bits 4 (i & 7) of word i >> 3; zero past the end, padded by 16 bytes),
+    // or null when the alphabet does not allow nibble comparison
+    const uint32_t *text4;
     uint32_t num_samples;
     uint32_t sample_rate;
     uint32_t num_lanes;

@@ -598,6 +598,9 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
     // query code (no code is compatible with the sentinel, so it stops there
     // as the index extension does), and the row stays the same
     const bool textmode = fwd.text != nullptr && fwd.sa_full != nullptr;
+    // compare 16 codes per load pair as base-set nibbles (text4 against the
+    // query nibble stream) where the alphabet allows it, else 8 as bytes
+    const bool nibtext = fwd.text4 != nullptr;
 
     uint32_t st = SM_FETCH, w = 0, r = 0, m = 0, ia = 0, ib = 0, s0 = 0, E = 0, j = 0, k = 0, e = 0;
     uint32_t ns = 0, nn = 0, t = 0, head = 0, nrec = 0, slot_cap = 0;
@@ -827,9 +830,16 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
         } else if (st == SM_TSA) {
             R.x = fwd.sa_full[piv.x];
         } else if (st == SM_TEXT) {
-            const uint2 tw = *reinterpret_cast<const uint2 *>(fwd.text + ((sidx + e) & ~7u));
-            const uint2 qw = *reinterpret_cast<const uint2 *>(reinterpret_cast<uintptr_t>(p + e) & ~(uintptr_t)7);
-            R = make_uint4(tw.x, tw.y, qw.x, qw.y);
+            if (nibtext) { // the aligned 16-code groups holding text position sidx + e and query code e
+                const uint2 tw = *reinterpret_cast<const uint2 *>(fwd.text4 + (((sidx + e) >> 3) & ~1u));
+                const uint2 qw = *reinterpret_cast<const uint2 *>(qnib + ((((uint64_t)(p - q) + e) >> 3) & ~1ull));
+                R = make_uint4(tw.x, tw.y, qw.x, qw.y);
+            } else {
+                const uint2 tw = *reinterpret_cast<const uint2 *>(fwd.text + ((sidx + e) & ~7u));
+                const uint2 qw =
+                    *reinterpret_cast<const uint2 *>(reinterpret_cast<uintptr_t>(p + e) & ~(uintptr_t)7);
+                R = make_uint4(tw.x, tw.y, qw.x, qw.y);
+            }
         }
         bool ovf = false;
         if (!srec && !stext && !lsis) {
@@ -899,6 +909,23 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
             probes++;
             sidx = R.x - k; // text position of query code e is sidx + e
             st = SM_TEXT;
+        } else if (st == SM_TEXT && nibtext) {
+            probes++;
+            // codes e, e+1, ... match while text and query nibbles share a base
+            // (text nibbles are 0 at sentinels and past the end): the first
+            // nibble of (t & q) that is zero ends the match
+            const uint32_t to = (sidx + e) & 15u;
+            const uint32_t qo = (uint32_t)(((uint64_t)(p - q) + e) & 15u);
+            const uint64_t tw = ((uint64_t)R.x | ((uint64_t)R.y << 32)) >> (4 * to);
+            const uint64_t qw = ((uint64_t)R.z | ((uint64_t)R.w << 32)) >> (4 * qo);
+            const uint32_t lim = min(16u - max(to, qo), m - e); // codes both words hold (>= 1)
+            const uint64_t x = tw & qw;
+            const uint64_t ones = 0x1111111111111111ull;
+            uint64_t miss = ~(x | (x >> 1) | (x >> 2) | (x >> 3)) & ones;
+            if (lim < 16) miss |= 1ull << (4 * lim);
+            const uint32_t adv = miss ? (uint32_t)__ffsll((unsigned long long)miss) >> 2 : 16u; // (bit + 1) / 4
+            e += adv < lim ? adv : lim;
+            if (adv < lim || e >= m) st = SM_EMIT;
         } else if (st == SM_TEXT) {
             probes++;
             const uint32_t to = (sidx + e) & 7u;

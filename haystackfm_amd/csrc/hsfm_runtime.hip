@@ -373,6 +373,24 @@ __global__ __launch_bounds__(256) void k_text_from_sa(DevDir d, const DevConst *
     }
 }
 
+// Base-set nibbles of the text (DevDir::text4): word i from the lanes of
+// positions [8i, 8i + 8); positions past n give 0.
+__global__ __launch_bounds__(256) void k_text_nib(const uint8_t *__restrict__ text, uint32_t n,
+                                                  const DevConst *__restrict__ cst, uint32_t *__restrict__ text4,
+                                                  uint32_t nwords) {
+    __shared__ uint8_t s_nib[16];
+    if (threadIdx.x < 16) s_nib[threadIdx.x] = cst->text_nib[threadIdx.x];
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += gridDim.x * blockDim.x) {
+        uint32_t w = 0;
+        for (uint32_t u = 0; u < 8; u++) {
+            const uint64_t pos = 8ull * i + u;
+            if (pos < n) w |= (uint32_t)s_nib[text[pos] & 15u] << (4 * u);
+        }
+        text4[i] = w;
+    }
+}
+
 __global__ void k_count_value(const uint32_t *__restrict__ a, uint64_t n, uint32_t v,
                               unsigned long long *__restrict__ out) {
     uint32_t c = 0;
@@ -388,12 +406,14 @@ int ensure_sa_full(hsfm_index *idx) {
         db.view.sa_full = nullptr;
         db.view.text = nullptr;
         db.view.isa = nullptr;
+        db.view.text4 = nullptr;
         return HSFM_OK;
     }
     if (db.sa_full_tried) {
         db.view.sa_full = db.sa_full.as<uint32_t>();
         db.view.text = db.text.as<uint8_t>();
         db.view.isa = db.isa.as<uint32_t>();
+        db.view.text4 = db.text4.as<uint32_t>();
         return HSFM_OK;
     }
     db.sa_full_tried = true;
@@ -453,9 +473,18 @@ int ensure_sa_full(hsfm_index *idx) {
                            idx->d_const.as<DevConst>(), db.sa_full.as<uint32_t>(), db.text.as<uint8_t>(),
                            db.isa.as<uint32_t>());
         HSFM_HIP(hipGetLastError());
+        // and as base-set nibbles (16 codes per 8-byte load) where the alphabet allows
+        const uint32_t nwords = (uint32_t)(((uint64_t)n + 7) / 8 + 4);
+        if (hc.nib_text && !db.text4.ensure((size_t)nwords * 4)) {
+            const uint32_t blocks = (nwords + 255) / 256 < 65536 ? (nwords + 255) / 256 : 65536;
+            hipLaunchKernelGGL(k_text_nib, dim3(blocks), dim3(256), 0, s, db.text.as<uint8_t>(), n,
+                               idx->d_const.as<DevConst>(), db.text4.as<uint32_t>(), nwords);
+            HSFM_HIP(hipGetLastError());
+        }
         HSFM_HIP(hipStreamSynchronize(s));
         db.view.text = db.text.as<uint8_t>();
         db.view.isa = db.isa.as<uint32_t>();
+        db.view.text4 = db.text4.as<uint32_t>();
     } else {
         db.text.release();
     }
