@@ -96,7 +96,7 @@ def parse():
 
 def workload(args):
     """Reference, bidirectional index bytes and read sets (cached on disk)."""
-    from haystackfm_amd import index_build as IB
+    from workloads import index_build as IB
     key = hashlib.sha1(f"v2-{args.ref_bp}-{args.reads}-{args.random_reads}-{args.locate_reads}-{args.uneven_reads}"
                        .encode()).hexdigest()[:12]
     os.makedirs(args.cache, exist_ok=True)
@@ -519,7 +519,7 @@ def workload_1g(args):
     lengths, GC 0.41, 45% copies of 2,000 repeat families (300-6,000 bp) at
     5-25% divergence, 0.2% N; bidirectional index with SA sample rate 64 and no
     lookup table; reads of 150 bp from both strands with 1% substitutions."""
-    from haystackfm_amd import index_build as IB
+    from workloads import index_build as IB
     key = hashlib.sha1(f"1g-v1-{args.big_ref_bp}-{args.big_reads}".encode()).hexdigest()[:12]
     os.makedirs(args.cache, exist_ok=True)
     path = os.path.join(args.cache, f"wl_{key}.npz")

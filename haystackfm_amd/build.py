@@ -14,7 +14,6 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhsfm.so")
-BUILD_LIB = os.path.join(HERE, "libhsfm_build.so")
 CALIB_LIB = os.path.join(HERE, "libhsfm_calib.so")  # measurement aid (random-load ceiling), not the query path
 SOURCES = ["hsfm_runtime.hip", "hsfm_index.hip", "hsfm_locate.hip", "hsfm_mem.hip", "hsfm_lsi.hip",
            "hsfm_construct.hip"]
@@ -40,22 +39,7 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build_host_lib(verbose: bool = False) -> str:
-    """libhsfm_build.so: host-only index construction + workload synthesis."""
-    tmp = BUILD_LIB + ".tmp"
-    cmd = [os.environ.get("CXX", "g++"), "-O3", "-std=c++17", "-fPIC", "-shared", "-march=x86-64-v2",
-           "-Wall", "-o", tmp, os.path.join(CSRC, "hsfm_build.cpp"), "-lpthread"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
-    if r.returncode != 0:
-        raise RuntimeError("host library build failed:\n" + r.stdout.decode(errors="replace"))
-    os.replace(tmp, BUILD_LIB)
-    return BUILD_LIB
-
-
 def build(verbose: bool = False, jobs: int = 4) -> str:
-    build_host_lib(verbose)
     cc = hipcc()
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)

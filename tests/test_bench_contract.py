@@ -48,4 +48,4 @@ def test_profile_summaries_name_their_commands():
     for name in os.listdir(prof):
         if name.endswith(".md"):
             text = open(os.path.join(prof, name)).read()
-            assert "rocprofv3" in text or "bench.py" in text, name
+            assert "rocprofv3" in text or "bench.py" in text or "hipcc" in text, name

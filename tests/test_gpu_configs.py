@@ -32,7 +32,7 @@ import numpy as np
 import pytest
 
 import haystackfm_amd as hf
-from haystackfm_amd import index_build as IB
+from workloads import index_build as IB
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import haystackfm_amd as hf
-from haystackfm_amd import index_build as IB
+from workloads import index_build as IB
 from oracle import oracle as O
 from tests.helpers import random_text
 from tests.test_index_bytes_spec import CASES, SpecIndex, _codes

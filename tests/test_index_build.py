@@ -3,7 +3,7 @@ restatement of build_cpu + to_bytes (pins both constructions)."""
 import numpy as np
 import pytest
 
-from haystackfm_amd import index_build as IB
+from workloads import index_build as IB
 from oracle import oracle as O
 from tests.helpers import random_text
 

@@ -1,5 +1,6 @@
 // hsfm_build.cpp -- host-side FmIndex construction (the reference's build_cpu +
-// to_bytes) and seeded workload synthesis.  Not on the GPU query path.
+// to_bytes) and seeded workload synthesis for the benchmark and the tests.
+// Not part of libhsfm.so (the device builder hsfm_build_index_device is).
 //
 // Construction follows FmIndex::build_cpu_inner (src/fm_index/mod.rs:117-195):
 //   C array from the text (c_array.rs:14-31); suffix array in plain
@@ -22,7 +23,7 @@
 #include <thread>
 #include <vector>
 
-#include "../../include/hsfm_build.h"
+#include "hsfm_build.h"
 
 namespace {
 

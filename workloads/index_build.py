@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import subprocess
 
 import numpy as np
 
@@ -19,12 +20,25 @@ _lib = None
 IUPAC_AMBIG = [6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 5]  # R Y S W K M B D H V N
 
 
+def build_host_lib(verbose: bool = False) -> str:
+    """libhsfm_build.so from hsfm_build.cpp (host code only)."""
+    tmp = _PATH + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), "-O3", "-std=c++17", "-fPIC", "-shared", "-march=x86-64-v2",
+           "-Wall", "-o", tmp, os.path.join(_HERE, "hsfm_build.cpp"), "-lpthread"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("host library build failed:\n" + r.stdout.decode(errors="replace"))
+    os.replace(tmp, _PATH)
+    return _PATH
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_PATH):
-            from . import build as _b
-            _b.build_host_lib()
+            build_host_lib()
         L = C.CDLL(_PATH)
         u8p, u32p, u64p = C.POINTER(C.c_uint8), C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)
         for name in ("hsfm_build_index", "hsfm_build_bidir"):
