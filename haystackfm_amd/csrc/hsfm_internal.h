@@ -419,10 +419,16 @@ struct SeedView {
     uint32_t F;
     // packed records (build_seed_packed): {fwd_lo, fwd_hi, rev_lo, meta} for an
     // F-mer with one interval, {iv offset lo, hi, 0, meta} otherwise;
-    // meta = lsl | count << 5
+    // meta = lsl | extension mask << 5 | count << 9.  Bit b of the mask is
+    // set when the set of the F-mer has a left child under query base b
+    // (all four bits set when not known: several intervals)
     const uint4 *rec;
 };
 constexpr uint32_t kSeedLslBits = 5;
+constexpr uint32_t kSeedCntShift = 9;
+__host__ __device__ __forceinline__ uint32_t seed_rec_lsl(uint32_t meta) { return meta & 31u; }
+__host__ __device__ __forceinline__ uint32_t seed_rec_mask(uint32_t meta) { return (meta >> kSeedLslBits) & 15u; }
+__host__ __device__ __forceinline__ uint32_t seed_rec_cnt(uint32_t meta) { return meta >> kSeedCntShift; }
 
 // A seed window q[j-F..j) as table indices: ACGT codes give fixed digits, up
 // to kSeedAmb ambiguity codes expand to every base they contain (the union of

@@ -193,7 +193,7 @@ __device__ __forceinline__ uint32_t locate_read(const DevDir &d, const DevConst 
             for (uint32_t c = 0; c < 4; c++) {
                 if (!val[c]) continue;
                 probes++;
-                const uint32_t l = R[c].w & ((1u << kSeedLslBits) - 1u), n2 = R[c].w >> kSeedLslBits;
+                const uint32_t l = seed_rec_lsl(R[c].w), n2 = seed_rec_cnt(R[c].w);
                 if (l != F || n2 == 0) continue;
                 if (n2 > 1) ok = false;
                 bool dup = false;
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(256) void k_locate_search_group(DevDir d, const Dev
                     uint32_t ix = 0;
                     if (!seed_combo(sw, c, ix)) continue;
                     const uint4 R = seed_rec[ix];
-                    const uint32_t l = R.w & ((1u << kSeedLslBits) - 1u), n2 = R.w >> kSeedLslBits;
+                    const uint32_t l = seed_rec_lsl(R.w), n2 = seed_rec_cnt(R.w);
                     if (l != F || n2 == 0) continue;
                     const uint64_t o = (uint64_t)R.x | ((uint64_t)R.y << 32);
                     for (uint32_t e = 0; e < n2 && ok; e++) {

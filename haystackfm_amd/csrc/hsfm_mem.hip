@@ -203,10 +203,10 @@ __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const D
                         if (xi < nc && wide_combo(sw, xi, idx)) {
                             if (sv.rec) { // packed record: one load per expansion
                                 const uint4 R = sv.rec[idx];
-                                const uint32_t l = R.w & ((1u << kSeedLslBits) - 1u);
+                                const uint32_t l = seed_rec_lsl(R.w);
                                 atomicMax(&s_lsl, l);
                                 if (l == F) {
-                                    cnt = R.w >> kSeedLslBits;
+                                    cnt = seed_rec_cnt(R.w);
                                     o0 = (uint64_t)R.x | ((uint64_t)R.y << 32);
                                     one = make_uint4(R.x, R.y, R.z, 0u);
                                 }
@@ -889,9 +889,14 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
             // one packed record: lsl, interval count and the interval itself
             // when it is the only one
             probes++;
-            const uint32_t l = R.w & ((1u << kSeedLslBits) - 1u), cnt = R.w >> kSeedLslBits;
+            const uint32_t l = seed_rec_lsl(R.w), cnt = seed_rec_cnt(R.w);
             lslmax = l > lslmax ? l : lslmax;
-            if (l != F || cnt == 0) {
+            // the query code left of the window: when its bases have no child
+            // under the F-mer (extension mask), its first left extension is
+            // empty and the expansion adds nothing to the set
+            const uint32_t xe = j > F ? (uint32_t)s_expand[p[j - F - 1]] : 0u;
+            const uint32_t xb = (xe & 0x10u) ? 1u << (xe & 3u) : (xe & 15u);
+            if (l != F || cnt == 0 || (xb && (seed_rec_mask(R.w) & xb) == 0)) {
                 st = SM_SEEDX;
             } else if (cnt == 1) {
                 ient = 0;
@@ -1190,21 +1195,34 @@ int build_seed_table(hsfm_index *idx, uint32_t F) {
 
 // Packed record of every F-mer c from its level arrays (count = off[c+1] -
 // off[c]); a count beyond the meta field sets *bad.
-__global__ void k_seed_pack(uint32_t nchild, const uint64_t *__restrict__ off, const uint4 *__restrict__ iv,
-                            const uint8_t *__restrict__ lsl, uint4 *__restrict__ rec, uint32_t *__restrict__ bad) {
+__global__ __launch_bounds__(256) void k_seed_pack(DevDir fwd, const DevConst *__restrict__ cst, uint32_t nchild,
+                                                   const uint64_t *__restrict__ off, const uint4 *__restrict__ iv,
+                                                   const uint8_t *__restrict__ lsl, uint4 *__restrict__ rec,
+                                                   uint32_t *__restrict__ bad) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nchild) return;
     const uint64_t a = off[c], cnt = off[c + 1] - a;
-    if (cnt >= (1ull << (32 - kSeedLslBits))) {
+    if (cnt >= (1ull << (32 - kSeedCntShift))) {
         *bad = 1u;
         rec[c] = make_uint4(0, 0, 0, 0);
         return;
     }
-    const uint32_t meta = (uint32_t)lsl[c] | ((uint32_t)cnt << kSeedLslBits);
+    uint32_t meta = (uint32_t)lsl[c] | ((uint32_t)cnt << kSeedCntShift);
     if (cnt == 1) {
+        // which query bases extend the one interval left (the child rule of
+        // k_seed_level / the MEM kernels: a compatible lane with rows)
         const uint4 v = iv[a];
-        rec[c] = make_uint4(v.x, v.y, v.z, meta);
+        uint32_t cl[16], ch[16];
+        interval_counts<true>(fwd, cst->num_lanes, v, cl, ch);
+        uint32_t mask = 0;
+        for (uint32_t b = 0; b < 4; b++) {
+            const uint32_t cm = cst->compat_lanes[1 + b];
+            for (int k = 0; k < 16; k++)
+                if (((cm >> k) & 1u) && ch[k] > cl[k]) mask |= 1u << b;
+        }
+        rec[c] = make_uint4(v.x, v.y, v.z, meta | (mask << kSeedLslBits));
     } else {
+        meta |= 15u << kSeedLslBits;
         rec[c] = make_uint4((uint32_t)a, (uint32_t)(a >> 32), 0u, meta);
     }
 }
@@ -1224,8 +1242,9 @@ static int seed_pack(hsfm_index *idx, uint32_t F, const DevBuf &off, const DevBu
     uint32_t bad = 0;
     rc = fill_u32(idx, flag.as<uint32_t>(), 1, 0);
     if (!rc) {
-        hipLaunchKernelGGL(k_seed_pack, dim3((nchild + 255) / 256), dim3(256), 0, s, nchild, off.as<uint64_t>(),
-                           iv.as<uint4>(), lsl.as<uint8_t>(), rec.as<uint4>(), flag.as<uint32_t>());
+        hipLaunchKernelGGL(k_seed_pack, dim3((nchild + 255) / 256), dim3(256), 0, s, idx->fwd.view,
+                           idx->d_const.as<DevConst>(), nchild, off.as<uint64_t>(), iv.as<uint4>(),
+                           lsl.as<uint8_t>(), rec.as<uint4>(), flag.as<uint32_t>());
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpyAsync(&bad, flag.p, 4, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
