@@ -510,8 +510,9 @@ def learned_seed_leg(bidir, flat, off, args, leaves=None, keys=None, keep=None):
             "train_mse": [info["mse_initial"], info["mse_final"]],
             "build_s": round(build_s, 2), "enum_ms": round(info["enum_ms"], 1),
             "train_ms": round(info["train_ms"], 1), "fit_ms": round(info["fit_ms"], 1),
-            "fence_levels": int(info["fence_levels"]), "flagged_keys": int(info["flagged"]),
-            "flag_ms": round(info["flag_ms"], 1), "mem_seeding": bool(info["mem_seeding"])}
+            "fence_levels": int(info["fence_levels"]), "entries": int(info["entries"]),
+            "variant_entries": int(info["variant_entries"]), "short_entries": int(info["short_entries"]),
+            "extra_ms": round(info["extra_ms"], 1), "mem_seeding": bool(info["mem_seeding"])}
 
 
 def workload_1g(args):

@@ -237,10 +237,11 @@ class _LsiConfig(C.Structure):
 
 
 class _LsiInfo(C.Structure):
-    _fields_ = [("num_keys", C.c_uint64), ("max_err", C.c_uint32), ("fence_levels", C.c_uint32),
+    _fields_ = [("num_keys", C.c_uint64), ("entries", C.c_uint64), ("variant_entries", C.c_uint64),
+                ("short_entries", C.c_uint64), ("max_err", C.c_uint32), ("fence_levels", C.c_uint32),
                 ("mean_window", C.c_double), ("mse_initial", C.c_double), ("mse_final", C.c_double),
                 ("enum_ms", C.c_double), ("train_ms", C.c_double), ("fit_ms", C.c_double),
-                ("flagged", C.c_uint64), ("mem_seeding", C.c_uint32), ("pad2", C.c_uint32), ("flag_ms", C.c_double)]
+                ("extra_ms", C.c_double), ("mem_seeding", C.c_uint32), ("pad2", C.c_uint32)]
 
 
 class _SeedStats(C.Structure):

@@ -1,32 +1,46 @@
 // hsfm_lsi.hip -- learned seed index on gfx950.
 //
-// For an exact-ACGT k-mer W (k <= 31; 21 by default) the index answers the
-// bidirectional SA interval {fwd_lo, fwd_hi, rev_lo} that plain backward
-// search gives (extend_left over W from the full interval, bidir.rs:112-130,
-// the exact-code extension of LookupTable::build, lookup.rs:37-91), or "absent".
-//
-// Keys pack W two bits per base, first base most significant, so sorting the
-// keys sorts the k-mers in suffix-array order.  Build, all on the device:
-//   1. enumerate every present k-mer from the index itself, level by level:
-//      the node of cS is one exact left extension of the node of S; children
-//      are placed digit-major, which keeps each level sorted by key
-//   2. stage 1 (FP32 MLP, x -> f(x) ~ slot / N, the key CDF) trained by
-//      Adam on a sample of (key, slot) pairs with the forward/backward kernels
-//      below
-//   3. stage 2: leaf = the key's leading log2(L) bits (L a power of two), so
-//      leaves are aligned with k-mer prefixes; per leaf a least-squares map
-//      slot ~ s0 + a * f(u) + b of the stage-1 output at u, the key's
-//      position inside its leaf in [0, 1) (float64 sums, FP32 parameters),
-//      and the maximum error of its keys.  The key distribution is close to
-//      a product of per-base frequencies, so the CDF inside a prefix has the
-//      shape of the global CDF: the trained f bends each leaf's map along it
-//      (an untrained f(x) = x leaves one straight line per leaf)
-// Lookup: route, predict, then a binary search of the sorted keys restricted
-// to [pred - err, pred + err] through fence levels of every 2^5, 2^9, 2^13 ...
-// -th key (as many as the widest window needs; each level narrows the range
-// to 17 entries of the next) and a last round of at most 33 keys, each level
-// one round of independent loads; exact because err bounds every present key
-// and an absent key is absent from every window.
+// For a window W of k codes (k <= 31; 21 by default) the index answers what
+// the MEM restarts and exact k-mer seeding need without k backward-search
+// steps:
+//   * exact lookup: for an exact-ACGT k-mer, the bidirectional SA interval
+//     {fwd_lo, fwd_hi, rev_lo} of plain backward search (extend_left over W
+//     from the full interval, bidir.rs:112-130, the exact-code extension of
+//     LookupTable::build, lookup.rs:37-91), or "absent";
+//   * restart lookup (k <= 25): for an all-base window q[j-k..j), every
+//     interval of its IUPAC-compatible set (the set find_smems extends,
+//     smem.rs:154-222), or, when the set is empty, the length of the longest
+//     suffix of W with a compatible occurrence (so L(j) = j - that length).
+// Entries (hsfm_lsi_dev.h) are sorted keys that hold the codes last base
+// first, so k-mers sharing a suffix are neighbours:
+//   1. exact entries: every present exact-ACGT k-mer, enumerated from the
+//      index itself level by level (the node of cS is one exact left extension
+//      of the node of S; children placed parent by parent, which keeps each
+//      level sorted);
+//   2. ambiguity entries: every distinct text window of k codes that holds an
+//      ambiguity code, expanded into the base strings compatible with it, each
+//      with the window's interval;
+//   3. short entries: windows of r < k codes after a contig start or a
+//      symbol no base matches, with the neighbour bounds of their r;
+//   4. stage 1, an FP32 MLP x -> f(x) ~ slot / N (the key CDF) of H clipped
+//      ReLU ramps, trained by Adam on a sample of (key, slot) pairs with the
+//      forward/backward kernels below and kept non-decreasing (w1, w2
+//      projected to >= 0; a sum of non-negative ramps is any monotone
+//      piecewise-linear curve);
+//   5. stage 2: leaf = the key's top log2(L) bits (the last bases of the
+//      k-mer); per leaf a least-squares map slot ~ s0 + a * f(u) + b (a >= 0)
+//      of the stage-1 output at u, the key's position inside its leaf, and the
+//      maximum error of its entries.  The key distribution is close to a
+//      product of per-base frequencies, so the CDF inside a leaf has the shape
+//      of the global CDF: the trained f bends each leaf's map along it (an
+//      untrained f(x) = x leaves one straight line per leaf).
+// Lookup: route, predict, then the lower bound of the key inside the error
+// window through fence levels of every 2^5, 2^9, 2^13 ... -th key (as many as
+// the widest window needs; each level narrows the range to 17 entries of the
+// next) and a last round of at most 35 keys, each level one round of
+// independent loads.  The prediction is non-decreasing inside a leaf, so the
+// window holds the lower bound of any key, present or not, and its two
+// neighbours give the longest-suffix answer.
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -38,7 +52,7 @@
 
 namespace hsfm {
 
-// ---- 1. enumeration ------------------------------------------------------
+// ---- 1. exact entries ------------------------------------------------------
 // Exact lanes of A, C, G, T (lane_sym inverse); 0xff if the index lacks one.
 __device__ __forceinline__ void base_lanes(const DevConst *cst, uint32_t *s_bl) {
     if (threadIdx.x < 4) {
@@ -50,15 +64,14 @@ __device__ __forceinline__ void base_lanes(const DevConst *cst, uint32_t *s_bl) 
 }
 
 // Children of every parent node (key of S, {fwd_lo, fwd_hi, rev_lo}): cS for
-// c in ACGT.  COUNT: flags[d * P + p] = 1 if child d exists.  Else: write the
-// child at off[d][p] + base[d] (digit-major: sorted by key).
+// c in ACGT, c in the key's digit below S's.  COUNT: nch[p] = number of
+// children.  WRITE: the children of p at off[p], in c order (sorted).
 template <bool WRITE>
 __global__ __launch_bounds__(256) void k_lsi_expand(DevDir fwd, const DevConst *__restrict__ cst,
                                                     const unsigned long long *__restrict__ pkey,
                                                     const uint4 *__restrict__ piv, uint32_t P, uint32_t level,
-                                                    uint8_t *__restrict__ flags, const uint32_t *__restrict__ off,
-                                                    uint4 base, unsigned long long *__restrict__ ckey,
-                                                    uint4 *__restrict__ civ) {
+                                                    uint8_t *__restrict__ nch, const uint32_t *__restrict__ off,
+                                                    unsigned long long *__restrict__ ckey, uint4 *__restrict__ civ) {
     __shared__ uint32_t s_bl[4], s_clane[16];
     base_lanes(cst, s_bl);
     if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
@@ -72,35 +85,248 @@ __global__ __launch_bounds__(256) void k_lsi_expand(DevDir fwd, const DevConst *
     below[0] = 0;
 #pragma unroll
     for (int j = 0; j < 16; j++) below[j + 1] = below[j] + (ch[j] - cl[j]);
-    const unsigned long long pk = pkey[p];
+    const unsigned long long pk = WRITE ? pkey[p] : 0ull;
+    uint32_t at = WRITE ? off[p] : 0u, cnt = 0;
 #pragma unroll
     for (uint32_t d = 0; d < 4; d++) {
         const uint32_t l = s_bl[d];
-        uint32_t a = 0, b = 0, acc = 0;
+        uint32_t a = 0, b = 0, acc = 0, cs = 0;
 #pragma unroll
         for (int j = 0; j < 16; j++)
-            if ((uint32_t)j == l) { a = cl[j]; b = ch[j]; acc = below[j]; }
-        const bool ok = l < 16 && b > a;
-        if (!WRITE) {
-            flags[(size_t)d * P + p] = ok ? 1 : 0;
-        } else if (ok) {
-            uint32_t cs = 0;
-#pragma unroll
-            for (int j = 0; j < 16; j++)
-                if ((uint32_t)j == l) cs = s_clane[j];
-            const uint32_t bd = d == 0 ? base.x : d == 1 ? base.y : d == 2 ? base.z : base.w;
-            const size_t at = (size_t)bd + off[(size_t)d * P + p];
-            ckey[at] = ((unsigned long long)d << (2 * level)) | pk;
-            civ[at] = make_uint4(cs + a, cs + b, iv.z + acc, 0u);
+            if ((uint32_t)j == l) { a = cl[j]; b = ch[j]; acc = below[j]; cs = s_clane[j]; }
+        if (l < 16 && b > a) {
+            cnt++;
+            if (WRITE) {
+                ckey[at] = pk | ((unsigned long long)d << (62 - 2 * level));
+                civ[at] = make_uint4(cs + a, cs + b, iv.z + acc, kLsiPayExact);
+                at++;
+            }
         }
     }
+    if (!WRITE) nch[p] = (uint8_t)cnt;
 }
 
 struct U8ToU32 {
     __device__ __forceinline__ uint32_t operator()(uint8_t v) const { return v; }
 };
 
-// ---- 2. stage-1 training -----------------------------------------------
+// ---- 2./3. ambiguity and short entries -------------------------------------
+// ls.qb[lane]: the query bases (bit b: base b+1) compatible with the lane's
+// symbol (0: no base, the lane cuts windows); ls.exact[lane]: the lane of A,
+// C, G or T.
+struct LsiLaneSets {
+    uint8_t qb[16];
+    uint8_t exact[16];
+};
+constexpr uint32_t kLsiMaxExpansions = 4096; // per window (six N)
+
+// Windows of K codes that hold an ambiguity code and no cutting symbol, each
+// taken once: by the thread of its first ambiguity position p.  PASS 0 counts
+// them (ctr[0]) and those with more than cap expansions (ctr[1]).  PASS 1
+// computes each one's interval (K exact-lane extend_left steps of its own
+// codes) and keeps it when its start is the interval's first row (one window
+// per distinct string): rec[] = {start, fwd_lo, fwd_hi, rev_lo}, rexp[] =
+// its expansion count, at ctr[2]++ (capacity rcap).
+template <int PASS>
+__global__ __launch_bounds__(256) void k_lsi_windows(DevDir fwd, const DevConst *__restrict__ cst, uint32_t K,
+                                                     LsiLaneSets ls, uint32_t cap, unsigned long long *__restrict__ ctr,
+                                                     uint4 *__restrict__ rec, uint32_t *__restrict__ rexp,
+                                                     uint64_t rcap) {
+    __shared__ uint32_t s_clane[16];
+    if (threadIdx.x < 16) s_clane[threadIdx.x] = cst->c_lane[threadIdx.x];
+    __syncthreads();
+    const uint8_t *__restrict__ text = fwd.text;
+    const uint32_t n = fwd.n;
+    unsigned long long nwin = 0, over = 0;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t tp = text[p] & 15u;
+        if (ls.exact[tp] || ls.qb[tp] == 0) continue;
+        for (uint64_t s = p;; s--) {
+            if (s < p && !ls.exact[text[s] & 15u]) break; // p is not the first ambiguity code
+            if (s + K <= n) {
+                bool ok = true;
+                uint32_t ne = 1;
+                for (uint32_t i = 0; i < K && ok; i++) {
+                    const uint32_t m = ls.qb[text[s + i] & 15u];
+                    if (m == 0) ok = false;
+                    else if (ne <= cap) ne *= (uint32_t)__popc(m);
+                }
+                if (ok) {
+                    if (PASS == 0) {
+                        nwin++;
+                        if (ne > cap) over++;
+                    } else {
+                        uint32_t lo = 0, hi = n, rl = 0;
+                        for (int i = (int)K - 1; i >= 0; i--) {
+                            const uint32_t L = text[s + (uint32_t)i] & 15u;
+                            uint32_t cl[16], ch[16];
+                            pair_counts(fwd, lo, hi, cl, ch);
+                            uint32_t a = 0, b = 0, acc = 0, run = 0, cs = 0;
+#pragma unroll
+                            for (int j = 0; j < 16; j++) {
+                                if ((uint32_t)j == L) { a = cl[j]; b = ch[j]; acc = run; cs = s_clane[j]; }
+                                run += ch[j] - cl[j];
+                            }
+                            lo = cs + a;
+                            hi = cs + b;
+                            rl += acc;
+                        }
+                        if (fwd.isa[s] == lo) {
+                            const unsigned long long at = atomicAdd(&ctr[2], 1ull);
+                            if (at < rcap) {
+                                rec[at] = make_uint4((uint32_t)s, lo, hi, rl);
+                                rexp[at] = ne;
+                            }
+                        }
+                    }
+                }
+            }
+            if (s == 0 || s + K == p + 1) break;
+        }
+    }
+    if (PASS == 0) {
+        if (nwin) atomicAdd(&ctr[0], nwin);
+        if (over) atomicAdd(&ctr[1], over);
+    }
+}
+
+// The base strings compatible with text[s, s + r) as keys: expansion e picks,
+// at each position, the (e's mixed-radix digit)-th compatible base.
+__device__ __forceinline__ unsigned long long lsi_expansion_key(const uint8_t *__restrict__ text, uint64_t s,
+                                                                uint32_t r, const LsiLaneSets &ls, uint32_t e) {
+    unsigned long long key = 0;
+    for (uint32_t i = 0; i < r; i++) {
+        const uint32_t m = ls.qb[text[s + i] & 15u];
+        const uint32_t c = (uint32_t)__popc(m);
+        const uint32_t pick = e % c;
+        e /= c;
+        uint32_t mm = m;
+        for (uint32_t z = 0; z < pick; z++) mm &= mm - 1;
+        key |= (unsigned long long)(__ffs(mm) - 1) << (64 - 2 * r + 2 * i);
+    }
+    return key;
+}
+
+// Ambiguity entries of the kept windows: window i writes its rexp[i]
+// expansions from off[i].
+__global__ __launch_bounds__(256) void k_lsi_window_entries(const uint8_t *__restrict__ text, uint32_t K,
+                                                            LsiLaneSets ls, const uint4 *__restrict__ rec,
+                                                            const uint32_t *__restrict__ rexp,
+                                                            const uint64_t *__restrict__ off, uint64_t nrec,
+                                                            unsigned long long *__restrict__ keys,
+                                                            uint4 *__restrict__ pay) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrec;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint4 r = rec[i];
+        const uint64_t o = off[i];
+        for (uint32_t e = 0; e < rexp[i]; e++) {
+            keys[o + e] = lsi_expansion_key(text, r.x, K, ls, e) | kLsiVariant;
+            pay[o + e] = make_uint4(r.y, r.z, r.w, 0u);
+        }
+    }
+}
+
+// Short entries: for every contig start s (position 0, or one past a symbol
+// no base matches) the windows [s, s + r), r < K, that hold no cutting
+// symbol, expanded.  PASS 0 counts the entries (ctr[0]) and windows with more
+// than cap expansions (ctr[1]); PASS 1 writes them from out[ctr[2]++].
+template <int PASS>
+__global__ __launch_bounds__(256) void k_lsi_short(const uint8_t *__restrict__ text, uint32_t n, uint32_t K,
+                                                   LsiLaneSets ls, uint32_t cap, unsigned long long *__restrict__ ctr,
+                                                   unsigned long long *__restrict__ keys, uint4 *__restrict__ pay,
+                                                   uint64_t kcap) {
+    unsigned long long cnt = 0, over = 0;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (uint64_t)gridDim.x * blockDim.x) {
+        if (s > 0 && ls.qb[text[s - 1] & 15u] != 0) continue; // not a contig start
+        uint32_t ne = 1;
+        for (uint32_t r = 1; r < K && s + r <= n; r++) {
+            const uint32_t m = ls.qb[text[s + r - 1] & 15u];
+            if (m == 0) break;
+            if (ne <= cap) ne *= (uint32_t)__popc(m);
+            if (ne > cap) {
+                over++;
+                break;
+            }
+            if (PASS == 0) {
+                cnt += ne;
+            } else {
+                const unsigned long long at = atomicAdd(&ctr[2], (unsigned long long)ne);
+                const unsigned long long meta =
+                    kLsiShort | ((unsigned long long)r << kLsiGShift) | ((unsigned long long)r << kLsiHShift);
+                for (uint32_t e = 0; e < ne && at + e < kcap; e++) {
+                    keys[at + e] = lsi_expansion_key(text, s, r, ls, e) | meta;
+                    pay[at + e] = make_uint4(0u, 0u, 0u, 0u);
+                }
+            }
+        }
+    }
+    if (PASS == 0) {
+        if (cnt) atomicAdd(&ctr[0], cnt);
+        if (over) atomicAdd(&ctr[1], over);
+    }
+}
+
+__global__ void k_lsi_iota(uint32_t *v, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        v[i] = (uint32_t)i;
+}
+__global__ void k_lsi_gather(const uint4 *__restrict__ in, const uint32_t *__restrict__ perm, uint64_t n,
+                             uint4 *__restrict__ out) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = in[perm[i]];
+}
+__global__ void k_lsi_pad(unsigned long long *keys, uint64_t n) {
+    if (threadIdx.x < 2) keys[n + threadIdx.x] = ~0ull;
+}
+
+// Full entries: the first of every run of equal codes records the run's
+// length in its payload.
+__global__ __launch_bounds__(256) void k_lsi_runs(const unsigned long long *__restrict__ keys, uint64_t n,
+                                                  unsigned long long smask, uint4 *__restrict__ pay) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long key = keys[i];
+        if (key & kLsiShort) continue;
+        const unsigned long long w = key & smask;
+        if (i > 0 && (keys[i - 1] & smask) == w) continue;
+        uint64_t j = i + 1;
+        while (j < n && (keys[j] & smask) == w && !(keys[j] & kLsiShort)) j++;
+        pay[i].w |= (uint32_t)(j - i) << 8;
+    }
+}
+
+// Neighbour bounds of short entries.  With entries x_0 < x_1 < ... of reach
+// r_m (k for full entries), G_i = max over m <= i of min(r_m, lcs(x_m, x_i))
+// satisfies G_i = max(r_i, min(G_{i-1}, lcs(x_{i-1}, x_i))) (the common
+// prefix of sorted strings is the minimum along the run between them), and
+// H likewise from the right; a full entry's own term makes its bound k.  One
+// thread per run of consecutive short entries, G pass then H pass.
+template <bool G>
+__global__ __launch_bounds__(256) void k_lsi_bounds(unsigned long long *__restrict__ keys, uint64_t n, LsiModel md) {
+    const int sh = G ? kLsiGShift : kLsiHShift;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (!(keys[i] & kLsiShort)) continue;
+        const uint64_t nb = G ? i - 1 : i + 1; // neighbour towards the run's outside
+        const bool edge = G ? i == 0 : i + 1 == n;
+        if (!edge && (keys[nb] & kLsiShort)) continue; // not the first entry of its run (in this direction)
+        uint32_t bound = edge ? 0u : md.k;
+        unsigned long long prev = edge ? 0ull : keys[nb];
+        for (uint64_t j = i;; j = G ? j + 1 : j - 1) {
+            const unsigned long long x = keys[j];
+            if (!(x & kLsiShort)) break;
+            const uint32_t r = (uint32_t)(x >> sh) & 31u;
+            const uint32_t l = lsi_lcs(md, prev, x);
+            uint32_t b = bound < l ? bound : l;
+            if (j == i && edge) b = 0;
+            b = b > r ? b : r;
+            keys[j] = (x & ~(31ull << sh)) | ((unsigned long long)b << sh);
+            bound = b;
+            prev = x;
+            if (G ? j + 1 == n : j == 0) break;
+        }
+    }
+}
+
+// ---- 4. stage-1 training -----------------------------------------------
 // Gradient of mean((f - y)^2) over n samples, accumulated into g[3H+1]
 // (zeroed by the caller); loss sum into *loss.
 __global__ __launch_bounds__(256) void k_lsi_grad(const float *__restrict__ p, uint32_t H,
@@ -117,8 +343,7 @@ __global__ __launch_bounds__(256) void k_lsi_grad(const float *__restrict__ p, u
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = t < n;
     const float xi = act ? x[t] : 0.f, yi = act ? y[t] : 0.f;
-    float f = s_p[3 * H];
-    for (uint32_t i = 0; i < H; i++) f = fmaf(s_p[2 * H + i], fmaxf(fmaf(s_p[i], xi, s_p[H + i]), 0.f), f);
+    const float f = lsi_stage1(s_p, H, xi);
     const float e = act ? f - yi : 0.f;
     const float df = 2.f * e / (float)n;
     // per-parameter contributions, reduced over the wave then over the block
@@ -129,8 +354,8 @@ __global__ __launch_bounds__(256) void k_lsi_grad(const float *__restrict__ p, u
     };
     for (uint32_t i = 0; i < H; i++) {
         const float pre = fmaf(s_p[i], xi, s_p[H + i]);
-        const float h = fmaxf(pre, 0.f);
-        const float dh = pre > 0.f ? df * s_p[2 * H + i] : 0.f;
+        const float h = fminf(fmaxf(pre, 0.f), 1.f);
+        const float dh = pre > 0.f && pre < 1.f ? df * s_p[2 * H + i] : 0.f;
         reduce_add(dh * xi, i);          // w1
         reduce_add(dh, H + i);           // b1
         reduce_add(df * h, 2 * H + i);   // w2
@@ -142,59 +367,58 @@ __global__ __launch_bounds__(256) void k_lsi_grad(const float *__restrict__ p, u
     if (threadIdx.x == 0) atomicAdd(loss, s_g[np]);
 }
 
+// One Adam step, then w1 and w2 projected onto >= 0 (f stays non-decreasing).
 __global__ void k_lsi_adam(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
-                           const float *__restrict__ g, uint32_t np, float lr, float b1t, float b2t) {
-    const uint32_t i = threadIdx.x;
+                           const float *__restrict__ g, uint32_t H, float lr, float b1t, float b2t) {
+    const uint32_t i = threadIdx.x, np = 3 * H + 1;
     if (i >= np) return;
     const float gi = g[i];
     const float mi = 0.9f * m[i] + 0.1f * gi, vi = 0.999f * v[i] + 0.001f * gi * gi;
     m[i] = mi;
     v[i] = vi;
-    p[i] -= lr * (mi / (1.f - b1t)) / (sqrtf(vi / (1.f - b2t)) + 1e-8f);
+    float pi = p[i] - lr * (mi / (1.f - b1t)) / (sqrtf(vi / (1.f - b2t)) + 1e-8f);
+    if (i < H || (i >= 2 * H && i < 3 * H)) pi = fmaxf(pi, 0.f);
+    p[i] = pi;
 }
 
 // training sample: every stride-th slot
 __global__ void k_lsi_sample(const unsigned long long *__restrict__ keys, uint64_t nkeys, uint64_t stride,
-                             uint32_t ns, uint32_t k, float *__restrict__ x, float *__restrict__ y) {
+                             uint32_t ns, LsiModel md, float *__restrict__ x, float *__restrict__ y) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ns) return;
     const uint64_t s = (uint64_t)i * stride;
-    x[i] = lsi_feature(keys[s], k);
+    x[i] = lsi_feature(md, keys[s]);
     y[i] = (float)((double)s / (double)nkeys);
 }
 
-// ---- 3. stage 2 ----------------------------------------------------------
+// ---- 5. stage 2 ----------------------------------------------------------
 constexpr uint32_t kLsiChunk = 64; // keys per thread in the leaf passes
 
-__global__ void k_lsi_leaf_init(LsiLeaf *lf, uint32_t *cnt, uint32_t L) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= L) return;
-    LsiLeaf z;
-    z.s0 = 0xffffffffu; z.err = 0; z.a = 0.f; z.b = 0.f;
-    lf[i] = z;
-    cnt[i] = 0;
-}
-
-// per leaf first slot and key count (chunk-aggregated atomics; keys sorted)
+// entries per leaf (chunk-aggregated atomics; keys sorted)
 __global__ __launch_bounds__(256) void k_lsi_route(LsiModel md, const unsigned long long *__restrict__ keys,
-                                                   LsiLeaf *__restrict__ lf, uint32_t *__restrict__ cnt) {
+                                                   uint32_t *__restrict__ cnt) {
     const uint64_t c0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kLsiChunk;
-    uint32_t cur = 0xffffffffu, c = 0, s0 = 0;
+    uint32_t cur = 0xffffffffu, c = 0;
     for (uint64_t s = c0; s < c0 + kLsiChunk && s < md.nkeys; s++) {
         const uint32_t l = lsi_leaf(md, keys[s]);
         if (l != cur) {
-            if (c) {
-                atomicMin(&lf[cur].s0, s0);
-                atomicAdd(&cnt[cur], c);
-            }
-            cur = l; c = 0; s0 = (uint32_t)s;
+            if (c) atomicAdd(&cnt[cur], c);
+            cur = l;
+            c = 0;
         }
         c++;
     }
-    if (c) {
-        atomicMin(&lf[cur].s0, s0);
-        atomicAdd(&cnt[cur], c);
-    }
+    if (c) atomicAdd(&cnt[cur], c);
+}
+
+// s0 of every leaf from the scanned counts (an empty leaf gets the next
+// leaf's first slot)
+__global__ void k_lsi_leaf_init(LsiLeaf *lf, const uint32_t *first, uint32_t L) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L) return;
+    LsiLeaf z;
+    z.s0 = first[i]; z.err = 0; z.a = 0.f; z.b = 0.f;
+    lf[i] = z;
 }
 
 // least-squares sums per leaf of (f(u), slot - s0): sx, sy, sxx, sxy (float64)
@@ -225,6 +449,8 @@ __global__ __launch_bounds__(256) void k_lsi_moments(LsiModel md, const unsigned
     }
 }
 
+// per leaf: the least-squares line with a >= 0 (a negative slope becomes the
+// mean), so the prediction stays non-decreasing in the key
 __global__ void k_lsi_solve(LsiLeaf *__restrict__ lf, const uint32_t *__restrict__ cnt,
                             const double *__restrict__ mom, uint32_t L) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -232,7 +458,7 @@ __global__ void k_lsi_solve(LsiLeaf *__restrict__ lf, const uint32_t *__restrict
     LsiLeaf f = lf[i];
     const uint32_t c = cnt[i];
     if (c == 0) {
-        f.s0 = 0; f.a = 0.f; f.b = 0.f;
+        f.a = 0.f; f.b = 0.f;
     } else {
         const double n = c, sx = mom[4 * i], sy = mom[4 * i + 1], sxx = mom[4 * i + 2], sxy = mom[4 * i + 3];
         const double den = n * sxx - sx * sx;
@@ -241,7 +467,7 @@ __global__ void k_lsi_solve(LsiLeaf *__restrict__ lf, const uint32_t *__restrict
             a = (n * sxy - sx * sy) / den;
             b = (sy - a * sx) / n;
         }
-        if (!isfinite(a) || !isfinite(b)) {
+        if (!isfinite(a) || !isfinite(b) || a < 0) {
             a = 0;
             b = sy / n;
         }
@@ -280,91 +506,22 @@ __global__ void k_lsi_fence(const unsigned long long *__restrict__ keys, uint64_
     if (j < nf) fence[j] = (j << sh) < nkeys ? keys[j << sh] : ~0ull;
 }
 
-// out[4i..4i+4) = {fwd_lo, fwd_hi, rev_lo, found}; probes += the bounded
-// search's loads in 64-byte units (the bytes of each round of paired loads).
-// One key per lane (a quad of lanes per key, loading whole lines together, ran
-// 3.5x slower: fewer keys in flight); a final round of 33 keys (16-key fences
-// with rounds of at most nine loads, for eight waves per SIMD, ran 8% slower).
+// Exact lookup of one k-mer (packed first base most significant):
+// {fwd_lo, fwd_hi, rev_lo, 1} of its exact entry, or zeros.  probes += the
+// bounded search's loads in 64-byte units (the bytes of each round of paired
+// loads).  One key per lane (a quad of lanes per key, loading whole lines
+// together, ran 3.5x slower: fewer keys in flight).
 __device__ __forceinline__ uint4 lsi_lookup_key(const LsiModel &md, const unsigned long long *__restrict__ keys,
                                                 const LsiFences &fc, const uint4 *__restrict__ pay,
-                                                const LsiLeaf &f, float fu, unsigned long long key,
+                                                const LsiLeaf &f, unsigned long long w, const float *s_p,
                                                 uint32_t &probes) {
-    const long long slot = lsi_find_slot(md, keys, fc, f, fu, key, probes);
-    if (slot < 0) return make_uint4(0, 0, 0, 0);
-    const uint4 v = pay[slot];
-    return make_uint4(v.x, v.y, v.z, 1u);
-}
-
-// Keys compatible with a text window that holds a symbol other than A, C, G,
-// T get kLsiFlagAmbiguous in their payload: the IUPAC set of such a k-mer is
-// more than its exact interval.  ls.qb[lane] holds the query bases (bit b:
-// base b+1) compatible with the lane's symbol; ls.exact[lane] is set for the
-// lanes of A, C, G and T.  One thread per text position p holding another
-// symbol takes the windows [s, s + k) with s in (p - k, p] whose first such
-// symbol is p, expands each (every combination of compatible bases) and
-// flags the expansions that are present keys.  ctr[0] counts flagged
-// expansions, ctr[1] windows with more than `cap` expansions (left
-// unflagged: the flags cannot be trusted then).
-constexpr uint32_t kLsiFlagMaxExpansions = 4096; // per window (six N)
-constexpr uint32_t kLsiMemMaxK = 25;               // k-mer windows the MEM kernel decodes
-struct LsiLaneSets {
-    uint8_t qb[16];
-    uint8_t exact[16];
-};
-__global__ __launch_bounds__(256) void k_lsi_flag(LsiModel md, const unsigned long long *__restrict__ keys,
-                                                  LsiFences fc, uint4 *__restrict__ pay,
-                                                  const LsiLeaf *__restrict__ lf, const uint8_t *__restrict__ text,
-                                                  uint32_t n, LsiLaneSets ls, uint32_t cap,
-                                                  unsigned long long *__restrict__ ctr) {
-    __shared__ float s_p[3 * kLsiMaxHidden + 1];
-    for (uint32_t i = threadIdx.x; i < 3 * md.hidden + 1; i += blockDim.x) s_p[i] = md.p[i];
-    __syncthreads();
-    const uint32_t K = md.k;
-    unsigned long long flagged = 0, over = 0;
-    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-         p += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t tp = text[p] & 15u;
-        if (ls.exact[tp] || ls.qb[tp] == 0) continue; // an exact base, or nothing matches here
-        const uint64_t s_lo = p + 1 >= K ? p + 1 - K : 0;
-        for (uint64_t s0 = s_lo; s0 <= p && s0 + K <= n; s0++) {
-            // the window's first non-exact symbol must be p; no position without
-            // compatible bases; expansion count
-            bool ok = true;
-            uint32_t cnt = 1;
-            for (uint32_t i = 0; i < K && ok; i++) {
-                const uint32_t t = text[s0 + i] & 15u;
-                if (s0 + i < p && !ls.exact[t]) ok = false;
-                else if (ls.qb[t] == 0) ok = false;
-                else if (cnt <= cap) cnt *= (uint32_t)__popc(ls.qb[t]);
-            }
-            if (!ok) continue;
-            if (cnt > cap) {
-                over++;
-                continue;
-            }
-            for (uint32_t e = 0; e < cnt; e++) {
-                unsigned long long key = 0;
-                uint32_t r = e;
-                for (uint32_t i = 0; i < K; i++) { // mixed-radix digit of each position
-                    const uint32_t m = ls.qb[text[s0 + i] & 15u];
-                    const uint32_t c = (uint32_t)__popc(m);
-                    uint32_t pick = r % c, mm = m;
-                    r /= c;
-                    for (uint32_t z = 0; z < pick; z++) mm &= mm - 1;
-                    key = (key << 2) | (unsigned long long)(__ffs(mm) - 1);
-                }
-                uint32_t pr = 0;
-                const long long slot = lsi_find_slot(md, keys, fc, lf[lsi_leaf(md, key)],
-                                                     lsi_stage1(s_p, md.hidden, lsi_u(md, key)), key, pr);
-                if (slot >= 0) {
-                    atomicOr(&pay[slot].w, kLsiFlagAmbiguous);
-                    flagged++;
-                }
-            }
-        }
-    }
-    if (flagged) atomicAdd(&ctr[0], flagged);
-    if (over) atomicAdd(&ctr[1], over);
+    if (!md.nkeys) return make_uint4(0, 0, 0, 0);
+    unsigned long long pv, sv;
+    bool hs;
+    const long long lb = lsi_search(md, keys, fc, f, lsi_stage1(s_p, md.hidden, lsi_u(md, w)), w, probes, pv, sv, hs);
+    if (!lsi_present(md, w, sv, hs)) return make_uint4(0, 0, 0, 0);
+    const uint4 v = pay[lb];
+    return (v.w & kLsiPayExact) ? make_uint4(v.x, v.y, v.z, 1u) : make_uint4(0, 0, 0, 0);
 }
 
 __global__ __launch_bounds__(256) void k_lsi_lookup(LsiModel md, const unsigned long long *__restrict__ keys,
@@ -377,11 +534,9 @@ __global__ __launch_bounds__(256) void k_lsi_lookup(LsiModel md, const unsigned 
     __syncthreads();
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t probes = 0;
-    if (t < nq && !md.nkeys) out[t] = make_uint4(0, 0, 0, 0);
-    if (t < nq && md.nkeys) {
-        const unsigned long long key = q[t];
-        const LsiLeaf f = lf[lsi_leaf(md, key)];
-        out[t] = lsi_lookup_key(md, keys, fc, pay, f, lsi_stage1(s_p, md.hidden, lsi_u(md, key)), key, probes);
+    if (t < nq) {
+        const unsigned long long w = lsi_key_of_fwd(q[t]) & md.smask;
+        out[t] = lsi_lookup_key(md, keys, fc, pay, lf[lsi_leaf(md, w)], w, s_p, probes);
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) probes += __shfl_xor(probes, d, 64);
@@ -407,13 +562,8 @@ __global__ __launch_bounds__(1024) void k_lsi_lookup_lds(LsiModel md, const unsi
     uint32_t probes = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nq; t += stride) {
-        if (!md.nkeys) {
-            out[t] = make_uint4(0, 0, 0, 0);
-            continue;
-        }
-        const unsigned long long key = q[t];
-        const LsiLeaf f = s_lf[lsi_leaf(md, key)];
-        out[t] = lsi_lookup_key(md, keys, fc, pay, f, lsi_stage1(s_p, md.hidden, lsi_u(md, key)), key, probes);
+        const unsigned long long w = lsi_key_of_fwd(q[t]) & md.smask;
+        out[t] = lsi_lookup_key(md, keys, fc, pay, s_lf[lsi_leaf(md, w)], w, s_p, probes);
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) probes += __shfl_xor(probes, d, 64);
@@ -465,14 +615,17 @@ struct hsfm_lsi {
     LsiModel model() const {
         uint32_t lbits = 0;
         while ((1u << lbits) < cfg.leaves) lbits++;
-        const uint32_t kbits = 2 * cfg.k;
-        return LsiModel{params.as<float>(), cfg.hidden, cfg.leaves, cfg.k, info.num_keys, nfence,
-                        lbits <= kbits ? kbits - lbits : 0u, lbits > kbits ? lbits - kbits : 0u};
+        return LsiModel{params.as<float>(), cfg.hidden, cfg.leaves, cfg.k, info.entries, nfence, lbits,
+                        ~0ull << (64 - 2 * cfg.k)};
     }
     LsiFences fences() const {
         LsiFences fc{};
         for (int v = 0; v < kLsiMaxFences; v++) fc.f[v] = fence[v].as<unsigned long long>();
         return fc;
+    }
+    void release() {
+        keys.release(); pay.release(); leaves.release(); params.release();
+        for (DevBuf &f : fence) f.release();
     }
 };
 
@@ -490,18 +643,43 @@ int lsi_scan_u8(hsfm_index *idx, const uint8_t *in, uint32_t *out, uint32_t n) {
     return HSFM_OK;
 }
 
+int lsi_scan_u32(hsfm_index *idx, const uint32_t *in, uint32_t *out, uint32_t n) {
+    hipStream_t s = idx->dev->stream;
+    if (n == 0) return HSFM_OK;
+    size_t tmp = 0;
+    HSFM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, s));
+    int rc = idx->ws.scan_tmp.ensure(tmp);
+    if (rc) return rc;
+    HSFM_HIP(hipcub::DeviceScan::ExclusiveSum(idx->ws.scan_tmp.p, tmp, in, out, (int)n, s));
+    return HSFM_OK;
+}
+
+// Releases local device buffers on every return (DevBuf frees only on release()).
+template <int N> struct Releaser {
+    DevBuf *b[N];
+    ~Releaser() {
+        for (DevBuf *x : b) x->release();
+    }
+};
+
 double elapsed_ms(hipEvent_t a, hipEvent_t b) {
     float ms = 0.f;
     if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
     return ms;
 }
 
-// Enumerate every present exact-ACGT k-mer: sorted keys + {fwd_lo, fwd_hi, rev_lo}.
-int lsi_enumerate(hsfm_lsi *x) {
+uint32_t grid_of(uint64_t n, uint32_t cap = 65536) {
+    const uint64_t b = (n + 255) / 256;
+    return (uint32_t)(b < 1 ? 1 : b > cap ? cap : b);
+}
+
+// Every present exact-ACGT k-mer: sorted keys + {fwd_lo, fwd_hi, rev_lo, exact}.
+int lsi_enumerate(hsfm_lsi *x, DevBuf &out_keys, DevBuf &out_pay, uint64_t &out_n) {
     hsfm_index *idx = x->idx;
     hipStream_t s = idx->dev->stream;
     int rc;
-    DevBuf pk, pi;
+    DevBuf pk, pi, nch, off, ck, ci;
+    Releaser<6> rel{{&pk, &pi, &nch, &off, &ck, &ci}};
     if ((rc = pk.ensure(8)) || (rc = pi.ensure(16))) return rc;
     const unsigned long long k0 = 0;
     const uint4 root = make_uint4(0, idx->fwd.view.n, 0, 0);
@@ -509,48 +687,201 @@ int lsi_enumerate(hsfm_lsi *x) {
     HSFM_HIP(hipMemcpyAsync(pi.p, &root, 16, hipMemcpyHostToDevice, s));
     uint64_t P = 1;
     for (uint32_t level = 0; level < x->cfg.k; level++) {
-        if (P > 0x7fffffffull) {
-            pk.release(); pi.release();
+        if (P > 0x7fffffffull)
             return set_error(HSFM_EINVAL, "learned seed index: %llu nodes at one level", (unsigned long long)P);
-        }
-        DevBuf flags, off, ck, ci;
-        if ((rc = flags.ensure(4 * P)) || (rc = off.ensure(4 * P * 4 + 16))) return rc;
+        if ((rc = nch.ensure(P)) || (rc = off.ensure(4 * P + 16))) return rc;
         const dim3 grid((uint32_t)((P + 255) / 256));
         hipLaunchKernelGGL(k_lsi_expand<false>, grid, dim3(256), 0, s, idx->fwd.view, idx->d_const.as<DevConst>(),
-                           pk.as<unsigned long long>(), pi.as<uint4>(), (uint32_t)P, level, flags.as<uint8_t>(),
-                           nullptr, make_uint4(0, 0, 0, 0), nullptr, nullptr);
+                           pk.as<unsigned long long>(), pi.as<uint4>(), (uint32_t)P, level, nch.as<uint8_t>(),
+                           nullptr, nullptr, nullptr);
         HSFM_HIP(hipGetLastError());
-        uint32_t tot[4];
-        uint4 base = make_uint4(0, 0, 0, 0);
-        for (int d = 0; d < 4; d++)
-            if ((rc = lsi_scan_u8(idx, flags.as<uint8_t>() + (size_t)d * P, off.as<uint32_t>() + (size_t)d * P,
-                                  (uint32_t)P)))
-                return rc;
-        // totals: last offset + last flag of each digit
-        for (int d = 0; d < 4; d++) {
-            uint32_t lo = 0;
-            uint8_t lf = 0;
-            HSFM_HIP(hipMemcpyAsync(&lo, off.as<uint32_t>() + (size_t)d * P + P - 1, 4, hipMemcpyDeviceToHost, s));
-            HSFM_HIP(hipMemcpyAsync(&lf, flags.as<uint8_t>() + (size_t)d * P + P - 1, 1, hipMemcpyDeviceToHost, s));
-            HSFM_HIP(hipStreamSynchronize(s));
-            tot[d] = lo + lf;
-        }
-        base.x = 0; base.y = tot[0]; base.z = tot[0] + tot[1]; base.w = tot[0] + tot[1] + tot[2];
-        const uint64_t C = (uint64_t)base.w + tot[3];
-        if ((rc = ck.ensure(C * 8 + 8)) || (rc = ci.ensure(C * 16 + 16))) return rc;
+        if ((rc = lsi_scan_u8(idx, nch.as<uint8_t>(), off.as<uint32_t>(), (uint32_t)P))) return rc;
+        uint32_t lo = 0;
+        uint8_t lf = 0;
+        HSFM_HIP(hipMemcpyAsync(&lo, off.as<uint32_t>() + P - 1, 4, hipMemcpyDeviceToHost, s));
+        HSFM_HIP(hipMemcpyAsync(&lf, nch.as<uint8_t>() + P - 1, 1, hipMemcpyDeviceToHost, s));
+        HSFM_HIP(hipStreamSynchronize(s));
+        const uint64_t C = (uint64_t)lo + lf;
+        if ((rc = ck.ensure(C * 8 + 16)) || (rc = ci.ensure(C * 16 + 16))) return rc;
         hipLaunchKernelGGL(k_lsi_expand<true>, grid, dim3(256), 0, s, idx->fwd.view, idx->d_const.as<DevConst>(),
                            pk.as<unsigned long long>(), pi.as<uint4>(), (uint32_t)P, level, nullptr,
-                           off.as<uint32_t>(), base, ck.as<unsigned long long>(), ci.as<uint4>());
+                           off.as<uint32_t>(), ck.as<unsigned long long>(), ci.as<uint4>());
         HSFM_HIP(hipGetLastError());
         HSFM_HIP(hipStreamSynchronize(s));
-        flags.release(); off.release(); pk.release(); pi.release();
+        nch.release(); off.release(); pk.release(); pi.release();
         pk = ck; pi = ci;
+        ck = DevBuf{}; ci = DevBuf{};
         P = C;
         if (P == 0) break;
     }
-    x->keys = pk;
-    x->pay = pi;
-    x->info.num_keys = P;
+    out_keys = pk;
+    out_pay = pi;
+    pk = DevBuf{};
+    pi = DevBuf{};
+    out_n = P;
+    return HSFM_OK;
+}
+
+// Lane sets of the index's alphabet for the restart entries; false if its
+// exact lanes are not compatible with exactly their own base (then the
+// exact entries are not the sets of all-base windows).
+bool lsi_lane_sets(const DevConst &hc, LsiLaneSets &ls) {
+    memset(&ls, 0, sizeof(ls));
+    bool ok = true;
+    for (uint32_t l = 0; l < 16 && l < hc.num_lanes; l++) {
+        uint32_t m = 0;
+        for (uint32_t b = 0; b < 4; b++) m |= ((hc.compat_lanes[b + 1] >> l) & 1u) << b;
+        ls.qb[l] = (uint8_t)m;
+        ls.exact[l] = hc.lane_sym[l] >= 1 && hc.lane_sym[l] <= 4;
+        if (ls.exact[l] && m != (1u << (hc.lane_sym[l] - 1))) ok = false;
+    }
+    return ok;
+}
+
+// Entries of the restart lookups beyond the exact k-mers, merged with them in
+// sorted order; sets info.mem_seeding.  Leaves the exact entries alone (and
+// MEM seeding off) when the index cannot have them.
+int lsi_restart_entries(hsfm_lsi *x, DevBuf &ek, DevBuf &ep, uint64_t E) {
+    hsfm_index *idx = x->idx;
+    hipStream_t s = idx->dev->stream;
+    const uint32_t K = x->cfg.k;
+    LsiLaneSets ls;
+    x->keys = ek;
+    x->pay = ep;
+    ek = DevBuf{};
+    ep = DevBuf{};
+    x->info.entries = E;
+    x->info.mem_seeding = 0;
+    if (K > kLsiMemMaxK || !lsi_lane_sets(idx->h_const, ls)) return HSFM_OK;
+    int rc;
+    if ((rc = ensure_sa_full(idx))) return rc;
+    const DevDir &fwd = idx->fwd.view;
+    if (!fwd.text || !fwd.isa) return HSFM_OK; // no device text: exact lookups only
+    const uint32_t n = fwd.n;
+    const uint32_t grid = (uint32_t)idx->dev->num_cus * 16;
+    DevBuf ctr, rec, rexp, roff, k0, k1, v0, v1, p0, p1, tb;
+    Releaser<11> rel{{&ctr, &rec, &rexp, &roff, &k0, &k1, &v0, &v1, &p0, &p1, &tb}};
+    if ((rc = ctr.ensure(64))) return rc;
+    HSFM_HIP(hipMemsetAsync(ctr.p, 0, 64, s));
+    unsigned long long *c = ctr.as<unsigned long long>();
+    hipLaunchKernelGGL(k_lsi_windows<0>, dim3(grid), dim3(256), 0, s, fwd, idx->d_const.as<DevConst>(), K, ls,
+                       kLsiMaxExpansions, c, nullptr, nullptr, 0ull);
+    hipLaunchKernelGGL(k_lsi_short<0>, dim3(grid), dim3(256), 0, s, fwd.text, n, K, ls, kLsiMaxExpansions, c + 3,
+                       nullptr, nullptr, 0ull);
+    HSFM_HIP(hipGetLastError());
+    uint64_t h[8];
+    HSFM_HIP(hipMemcpyAsync(h, c, 64, hipMemcpyDeviceToHost, s));
+    HSFM_HIP(hipStreamSynchronize(s));
+    if (h[1] || h[4]) return HSFM_OK; // a window with too many expansions: the sets cannot all be stored
+    const uint64_t nwin = h[0], S = h[3];
+    // kept windows and their expansions
+    if ((rc = rec.ensure(nwin * 16 + 16)) || (rc = rexp.ensure(nwin * 4 + 16)) || (rc = roff.ensure(nwin * 8 + 16)))
+        return rc;
+    HSFM_HIP(hipMemsetAsync(ctr.p, 0, 64, s));
+    if (nwin)
+        hipLaunchKernelGGL(k_lsi_windows<1>, dim3(grid), dim3(256), 0, s, fwd, idx->d_const.as<DevConst>(), K, ls,
+                           kLsiMaxExpansions, c, rec.as<uint4>(), rexp.as<uint32_t>(), (uint64_t)nwin);
+    HSFM_HIP(hipGetLastError());
+    uint64_t nrec = 0;
+    HSFM_HIP(hipMemcpyAsync(&nrec, c + 2, 8, hipMemcpyDeviceToHost, s));
+    HSFM_HIP(hipStreamSynchronize(s));
+    if (nrec > nwin) return set_error(HSFM_EDEVICE, "learned seed index: window count changed between passes");
+    if (nrec > 0xffffffffull) return HSFM_OK;
+    if (nrec && (rc = exclusive_scan_u32_u64(idx, rexp.as<uint32_t>(), roff.as<uint64_t>(), (uint32_t)nrec)))
+        return rc;
+    uint64_t V = 0;
+    if (nrec) HSFM_HIP(hipMemcpyAsync(&V, roff.as<uint64_t>() + nrec, 8, hipMemcpyDeviceToHost, s));
+    HSFM_HIP(hipStreamSynchronize(s));
+    const uint64_t T = E + V + S;
+    if (T > 0xfffffffeull || T > 0x7fffffffull) return HSFM_OK; // slots are 32-bit; sort sizes are int
+    x->info.variant_entries = V;
+    x->info.short_entries = S;
+    if (V + S == 0) {
+        x->info.mem_seeding = 1;
+        return HSFM_OK;
+    }
+    // all entries: exact ones, then the new ones, sorted together
+    if ((rc = k0.ensure(T * 8 + 16)) || (rc = k1.ensure(T * 8 + 16)) || (rc = v0.ensure(T * 4 + 16)) ||
+        (rc = v1.ensure(T * 4 + 16)) || (rc = p0.ensure(T * 16 + 16)))
+        return rc;
+    if (E) {
+        HSFM_HIP(hipMemcpyAsync(k0.p, x->keys.p, E * 8, hipMemcpyDeviceToDevice, s));
+        HSFM_HIP(hipMemcpyAsync(p0.p, x->pay.p, E * 16, hipMemcpyDeviceToDevice, s));
+    }
+    HSFM_HIP(hipStreamSynchronize(s));
+    x->keys.release();
+    x->pay.release();
+    if (nrec)
+        hipLaunchKernelGGL(k_lsi_window_entries, dim3(grid_of(nrec)), dim3(256), 0, s, fwd.text, K, ls,
+                           rec.as<uint4>(), rexp.as<uint32_t>(), roff.as<uint64_t>(), nrec,
+                           k0.as<unsigned long long>() + E, p0.as<uint4>() + E);
+    HSFM_HIP(hipMemsetAsync(ctr.p, 0, 64, s));
+    if (S)
+        hipLaunchKernelGGL(k_lsi_short<1>, dim3(grid), dim3(256), 0, s, fwd.text, n, K, ls, kLsiMaxExpansions, c,
+                           k0.as<unsigned long long>() + E + V, p0.as<uint4>() + E + V, (uint64_t)S);
+    hipLaunchKernelGGL(k_lsi_iota, dim3(grid_of(T)), dim3(256), 0, s, v0.as<uint32_t>(), T);
+    HSFM_HIP(hipGetLastError());
+    rec.release(); rexp.release(); roff.release();
+    hipcub::DoubleBuffer<unsigned long long> kb(k0.as<unsigned long long>(), k1.as<unsigned long long>());
+    hipcub::DoubleBuffer<uint32_t> vb(v0.as<uint32_t>(), v1.as<uint32_t>());
+    size_t tmp = 0;
+    HSFM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kb, vb, (int)T, 0, 64, s));
+    if ((rc = tb.ensure(tmp))) return rc;
+    HSFM_HIP(hipcub::DeviceRadixSort::SortPairs(tb.p, tmp, kb, vb, (int)T, 0, 64, s));
+    HSFM_HIP(hipStreamSynchronize(s));
+    tb.release();
+    DevBuf &ks = kb.Current() == k0.as<unsigned long long>() ? k0 : k1;
+    DevBuf &kother = kb.Current() == k0.as<unsigned long long>() ? k1 : k0;
+    DevBuf &vs = vb.Current() == v0.as<uint32_t>() ? v0 : v1;
+    DevBuf &vother = vb.Current() == v0.as<uint32_t>() ? v1 : v0;
+    kother.release();
+    vother.release();
+    if ((rc = p1.ensure(T * 16 + 16))) return rc;
+    hipLaunchKernelGGL(k_lsi_gather, dim3(grid_of(T)), dim3(256), 0, s, p0.as<uint4>(), vs.as<uint32_t>(), T,
+                       p1.as<uint4>());
+    HSFM_HIP(hipGetLastError());
+    HSFM_HIP(hipStreamSynchronize(s));
+    p0.release();
+    vs.release();
+    x->keys = ks;
+    ks = DevBuf{};
+    x->pay = p1;
+    p1 = DevBuf{};
+    x->info.entries = T;
+    x->info.mem_seeding = 1;
+    return HSFM_OK;
+}
+
+// Run lengths of full entries, neighbour bounds of short entries; pads the
+// key buffer.
+int lsi_finish_entries(hsfm_lsi *x) {
+    hsfm_index *idx = x->idx;
+    hipStream_t s = idx->dev->stream;
+    const uint64_t N = x->info.entries;
+    int rc;
+    if (x->keys.cap < N * 8 + 16) { // room for the two pad keys
+        DevBuf nk;
+        if ((rc = nk.ensure(N * 8 + 16))) return rc;
+        if (N) HSFM_HIP(hipMemcpyAsync(nk.p, x->keys.p, N * 8, hipMemcpyDeviceToDevice, s));
+        HSFM_HIP(hipStreamSynchronize(s));
+        x->keys.release();
+        x->keys = nk;
+    }
+    if (!x->pay.p && (rc = x->pay.ensure(16))) return rc;
+    hipLaunchKernelGGL(k_lsi_pad, dim3(1), dim3(64), 0, s, x->keys.as<unsigned long long>(), N);
+    const LsiModel md = x->model();
+    if (N) {
+        hipLaunchKernelGGL(k_lsi_runs, dim3(grid_of(N)), dim3(256), 0, s, x->keys.as<unsigned long long>(), N,
+                           md.smask, x->pay.as<uint4>());
+        if (x->info.short_entries) {
+            hipLaunchKernelGGL(k_lsi_bounds<true>, dim3(grid_of(N)), dim3(256), 0, s,
+                               x->keys.as<unsigned long long>(), N, md);
+            hipLaunchKernelGGL(k_lsi_bounds<false>, dim3(grid_of(N)), dim3(256), 0, s,
+                               x->keys.as<unsigned long long>(), N, md);
+        }
+    }
+    HSFM_HIP(hipGetLastError());
+    HSFM_HIP(hipStreamSynchronize(s));
     return HSFM_OK;
 }
 
@@ -558,19 +889,21 @@ int lsi_train_and_fit(hsfm_lsi *x) {
     hsfm_index *idx = x->idx;
     hipStream_t s = idx->dev->stream;
     const uint32_t H = x->cfg.hidden, L = x->cfg.leaves, np = 3 * H + 1;
-    const uint64_t N = x->info.num_keys;
+    const uint64_t N = x->info.entries;
     int rc;
-    // stage-1 init: hidden unit i = relu(x - i/H); output = unit 0 (f(x) = x)
+    // stage-1 init: ramp i rises over [i/H, (i+1)/H], each weighted 1/H, so
+    // f(x) = x; training moves the slope of every segment (w2 >= 0)
     float hp[3 * kLsiMaxHidden + 1];
     for (uint32_t i = 0; i < H; i++) {
-        hp[i] = 1.f;
-        hp[H + i] = -(float)i / (float)H;
-        hp[2 * H + i] = i == 0 ? 1.f : 0.f;
+        hp[i] = (float)H;
+        hp[H + i] = -(float)i;
+        hp[2 * H + i] = 1.f / (float)H;
     }
     hp[3 * H] = 0.f;
     if ((rc = x->params.ensure(np * 4))) return rc;
     HSFM_HIP(hipMemcpyAsync(x->params.p, hp, np * 4, hipMemcpyHostToDevice, s));
-    DevBuf xs, ys, gm, lossb;
+    DevBuf xs, ys, gm, lossb, cnt, first, mom;
+    Releaser<7> rel{{&xs, &ys, &gm, &lossb, &cnt, &first, &mom}};
     const uint32_t ns = (uint32_t)(N < x->cfg.sample ? N : x->cfg.sample);
     if ((rc = xs.ensure((size_t)ns * 4 + 4)) || (rc = ys.ensure((size_t)ns * 4 + 4)) ||
         (rc = gm.ensure(np * 4 * 3)) || (rc = lossb.ensure(8)))
@@ -581,10 +914,11 @@ int lsi_train_and_fit(hsfm_lsi *x) {
     HSFM_HIP(hipEventCreate(&e0));
     HSFM_HIP(hipEventCreate(&e1));
     HSFM_HIP(hipEventRecord(e0, s));
+    const LsiModel md0 = x->model();
     if (ns) {
         const uint64_t stride = N / ns;
         hipLaunchKernelGGL(k_lsi_sample, dim3((ns + 255) / 256), dim3(256), 0, s, x->keys.as<unsigned long long>(), N,
-                           stride ? stride : 1, ns, x->cfg.k, xs.as<float>(), ys.as<float>());
+                           stride ? stride : 1, ns, md0, xs.as<float>(), ys.as<float>());
         HSFM_HIP(hipGetLastError());
         float b1t = 1.f, b2t = 1.f;
         for (uint32_t step = 0; step <= x->cfg.train_steps; step++) {
@@ -603,7 +937,7 @@ int lsi_train_and_fit(hsfm_lsi *x) {
             if (step == x->cfg.train_steps) break;
             b1t *= 0.9f;
             b2t *= 0.999f;
-            hipLaunchKernelGGL(k_lsi_adam, dim3(1), dim3(256), 0, s, x->params.as<float>(), m, v, g, np, x->cfg.lr,
+            hipLaunchKernelGGL(k_lsi_adam, dim3(1), dim3(256), 0, s, x->params.as<float>(), m, v, g, H, x->cfg.lr,
                                b1t, b2t);
             HSFM_HIP(hipGetLastError());
         }
@@ -613,24 +947,24 @@ int lsi_train_and_fit(hsfm_lsi *x) {
     x->info.train_ms = elapsed_ms(e0, e1);
     xs.release(); ys.release(); gm.release(); lossb.release();
 
-    // stage 2: leaves are the keys' leading log2(L) bits
+    // stage 2: leaves are the keys' top log2(L) bits
     HSFM_HIP(hipEventRecord(e0, s));
-    DevBuf cnt, mom;
-    if ((rc = x->leaves.ensure((size_t)L * sizeof(LsiLeaf))) || (rc = cnt.ensure((size_t)L * 4)) ||
-        (rc = mom.ensure((size_t)L * 32)))
+    if ((rc = x->leaves.ensure((size_t)L * sizeof(LsiLeaf))) || (rc = cnt.ensure((size_t)L * 4 + 4)) ||
+        (rc = first.ensure((size_t)L * 4 + 4)) || (rc = mom.ensure((size_t)L * 32)))
         return rc;
     HSFM_HIP(hipMemsetAsync(mom.p, 0, (size_t)L * 32, s));
-    hipLaunchKernelGGL(k_lsi_leaf_init, dim3((L + 255) / 256), dim3(256), 0, s, x->leaves.as<LsiLeaf>(),
-                       cnt.as<uint32_t>(), L);
+    HSFM_HIP(hipMemsetAsync(cnt.p, 0, (size_t)L * 4, s));
     const LsiModel md = x->model();
     const uint64_t chunks = (N + kLsiChunk - 1) / kLsiChunk;
     const dim3 cg((uint32_t)((chunks + 255) / 256));
-    if (N) {
-        hipLaunchKernelGGL(k_lsi_route, cg, dim3(256), 0, s, md, x->keys.as<unsigned long long>(),
-                           x->leaves.as<LsiLeaf>(), cnt.as<uint32_t>());
+    if (N)
+        hipLaunchKernelGGL(k_lsi_route, cg, dim3(256), 0, s, md, x->keys.as<unsigned long long>(), cnt.as<uint32_t>());
+    if ((rc = lsi_scan_u32(idx, cnt.as<uint32_t>(), first.as<uint32_t>(), L))) return rc;
+    hipLaunchKernelGGL(k_lsi_leaf_init, dim3((L + 255) / 256), dim3(256), 0, s, x->leaves.as<LsiLeaf>(),
+                       first.as<uint32_t>(), L);
+    if (N)
         hipLaunchKernelGGL(k_lsi_moments, cg, dim3(256), 0, s, md, x->keys.as<unsigned long long>(),
                            x->leaves.as<LsiLeaf>(), mom.as<double>());
-    }
     hipLaunchKernelGGL(k_lsi_solve, dim3((L + 255) / 256), dim3(256), 0, s, x->leaves.as<LsiLeaf>(),
                        cnt.as<uint32_t>(), mom.as<double>(), L);
     if (N)
@@ -648,12 +982,12 @@ int lsi_train_and_fit(hsfm_lsi *x) {
     uint32_t emax = 0;
     for (uint32_t i = 0; i < L; i++) {
         emax = h[i].err > emax ? h[i].err : emax;
-        wsum += (double)hc[i] * (2.0 * h[i].err + 1.0);
+        wsum += (double)hc[i] * (2.0 * h[i].err + 2.0);
     }
     x->info.max_err = emax;
     x->info.mean_window = N ? wsum / (double)N : 0.0;
-    cnt.release(); mom.release();
-    // fence levels: as many as the widest window (2 max_err + 1 keys) needs
+    cnt.release(); first.release(); mom.release();
+    // fence levels: as many as the widest window (2 max_err + 2 keys) needs
     // to span at most 17 entries of the coarsest; each padded past the last
     // key (pair loads read one entry ahead)
     x->nfence = (uint32_t)lsi_fence_levels(emax);
@@ -672,56 +1006,6 @@ int lsi_train_and_fit(hsfm_lsi *x) {
     return HSFM_OK;
 }
 
-// Flags of the keys whose IUPAC set is more than their exact interval
-// (k_lsi_flag); sets info.mem_seeding when the flags are complete.
-int lsi_flag(hsfm_lsi *x) {
-    hsfm_index *idx = x->idx;
-    hipStream_t s = idx->dev->stream;
-    const DevConst &hc = idx->h_const;
-    LsiLaneSets ls{};
-    bool any = false;
-    for (uint32_t l = 0; l < 16 && l < hc.num_lanes; l++) {
-        uint32_t m = 0;
-        for (uint32_t b = 0; b < 4; b++) m |= ((hc.compat_lanes[b + 1] >> l) & 1u) << b;
-        ls.qb[l] = (uint8_t)m;
-        ls.exact[l] = hc.lane_sym[l] >= 1 && hc.lane_sym[l] <= 4;
-        any |= !ls.exact[l] && m != 0;
-    }
-    x->info.flagged = 0;
-    x->info.mem_seeding = 0;
-    if (x->cfg.k > kLsiMemMaxK) return HSFM_OK; // the MEM kernel reads windows of at most kLsiMemMaxK codes
-    if (!any || x->info.num_keys == 0) { // no text window can hold another compatible symbol
-        x->info.mem_seeding = 1;
-        return HSFM_OK;
-    }
-    int rc;
-    if ((rc = ensure_sa_full(idx))) return rc;
-    if (!idx->fwd.view.text) return HSFM_OK; // no device text: flags unknown, no MEM seeding
-    DevBuf ctr;
-    if ((rc = ctr.ensure(16))) return rc;
-    HSFM_HIP(hipMemsetAsync(ctr.p, 0, 16, s));
-    hipEvent_t e0, e1;
-    HSFM_HIP(hipEventCreate(&e0));
-    HSFM_HIP(hipEventCreate(&e1));
-    HSFM_HIP(hipEventRecord(e0, s));
-    const uint32_t n = idx->fwd.view.n;
-    const uint32_t grid = (uint32_t)idx->dev->num_cus * 16;
-    hipLaunchKernelGGL(k_lsi_flag, dim3(grid), dim3(256), 0, s, x->model(), x->keys.as<unsigned long long>(),
-                       x->fences(), x->pay.as<uint4>(), x->leaves.as<LsiLeaf>(), idx->fwd.view.text, n, ls,
-                       kLsiFlagMaxExpansions, ctr.as<unsigned long long>());
-    HSFM_HIP(hipGetLastError());
-    HSFM_HIP(hipEventRecord(e1, s));
-    uint64_t h[2] = {0, 0};
-    HSFM_HIP(hipMemcpyAsync(h, ctr.p, 16, hipMemcpyDeviceToHost, s));
-    HSFM_HIP(hipStreamSynchronize(s));
-    x->info.flag_ms = elapsed_ms(e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    x->info.flagged = h[0];
-    x->info.mem_seeding = h[1] == 0 ? 1u : 0u;
-    return HSFM_OK;
-}
-
 } // namespace
 
 namespace hsfm {
@@ -729,8 +1013,9 @@ int lsi_mem_view(const hsfm_lsi *x, hsfm_index *idx, LsiView *out) {
     if (!x) return set_error(HSFM_EINVAL, "learned-seed restarts need a learned seed index");
     if (x->idx != idx) return set_error(HSFM_EINVAL, "the learned seed index was built on another index");
     if (!x->info.mem_seeding)
-        return set_error(HSFM_EINVAL, "this learned seed index cannot seed MEM restarts exactly (k > %u, or the "
-                                      "text's ambiguity windows could not all be expanded)", kLsiMemMaxK);
+        return set_error(HSFM_EINVAL, "this learned seed index cannot seed MEM restarts exactly (k > %u, an "
+                                      "alphabet whose bases match other lanes, no device text, or text windows "
+                                      "with more than %u base strings)", kLsiMemMaxK, kLsiMaxExpansions);
     out->md = x->model();
     out->fc = x->fences();
     out->keys = x->keys.as<unsigned long long>();
@@ -756,19 +1041,31 @@ extern "C" int hsfm_lsi_build(hsfm_index *idx, const hsfm_lsi_config *cfg, hsfm_
     hsfm_lsi *x = new hsfm_lsi();
     x->idx = idx;
     x->cfg = *cfg;
-    hipEvent_t e0, e1;
+    hipStream_t s = idx->dev->stream;
+    hipEvent_t e0, e1, e2;
     HSFM_HIP(hipEventCreate(&e0));
     HSFM_HIP(hipEventCreate(&e1));
-    HSFM_HIP(hipEventRecord(e0, idx->dev->stream));
-    int rc = lsi_enumerate(x);
-    HSFM_HIP(hipEventRecord(e1, idx->dev->stream));
+    HSFM_HIP(hipEventCreate(&e2));
+    HSFM_HIP(hipEventRecord(e0, s));
+    DevBuf ek, ep;
+    uint64_t E = 0;
+    int rc = lsi_enumerate(x, ek, ep, E);
+    HSFM_HIP(hipEventRecord(e1, s));
+    x->info.num_keys = E;
     x->info.enum_ms = elapsed_ms(e0, e1);
+    if (!rc && E > 0xfffffffeull) rc = set_error(HSFM_EINVAL, "learned seed index: more than 2^32 - 2 k-mers");
+    if (!rc) rc = lsi_restart_entries(x, ek, ep, E);
+    ek.release();
+    ep.release();
+    if (!rc) rc = lsi_finish_entries(x);
+    HSFM_HIP(hipEventRecord(e2, s));
+    x->info.extra_ms = elapsed_ms(e1, e2);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
     if (!rc) rc = lsi_train_and_fit(x);
-    if (!rc) rc = lsi_flag(x);
     if (rc) {
-        x->keys.release(); x->pay.release(); x->leaves.release(); x->params.release(); for (DevBuf &f : x->fence) f.release();
+        x->release();
         delete x;
         return rc;
     }
@@ -791,7 +1088,7 @@ extern "C" int hsfm_lsi_free(hsfm_lsi *x) {
             x->idx->mem_seed_mode = HSFM_SEED_TABLES;
         }
     }
-    x->keys.release(); x->pay.release(); x->leaves.release(); x->params.release(); for (DevBuf &f : x->fence) f.release();
+    x->release();
     delete x;
     return HSFM_OK;
 }

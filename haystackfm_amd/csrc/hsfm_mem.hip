@@ -469,6 +469,7 @@ enum : uint32_t {
     SM_SEEDX,     // next seed-table expansion of q[j-F..j)
     SM_ENDLEFT,   // left extension ended at k
     SM_EMIT,      // right extension ended at e: write the records
+    SM_LSI_NEXT,  // a learned lookup's base string is done: the next one, or the answer
     SM_SEEDE,     // probe: one seed entry, extended left by q[j-F-1]
     SM_SEEDR,     // load: the packed seed record of one expansion
     SM_TSA,       // load: text position of a single-row set (right extension)
@@ -478,15 +479,17 @@ enum : uint32_t {
     SM_LSI_LEAF,  // load: the learned index's leaf of the k-mer q[j-K..j)
     SM_LSI_FENCE, // loads: one fence level of its bounded search
     SM_LSI_KEYS,  // loads: up to 18 keys of its last round (two passes at most)
-    SM_LSI_PAY,   // load: its payload {fwd_lo, fwd_hi, rev_lo, flags}
+    SM_LSI_PAY,   // load: the payload of one entry of the run with the k-mer's codes
     SM_DONE
 };
 
-// The K (<= 25) codes of q[g..g+K) as a learned-index key (two bits per base,
-// first base most significant) if every one is a single base; false if any
-// is not (ambiguity code, sentinel, other byte).
-__device__ __forceinline__ bool lsi_window_key(const uint32_t *__restrict__ nib, uint64_t g, uint32_t K,
-                                               unsigned long long &key) {
+// The K (<= 25) codes of q[g..g+K) as a learned-index key template (two bits
+// per base from the top of the word, last code first; 0 at ambiguity codes)
+// and the ambiguity codes, at most three: positions in bits 5a.., base sets
+// in bits 15 + 4a.., their number in bits 27..  False if a code has no base
+// or there are more ambiguity codes.
+__device__ __forceinline__ bool lsi_window_amb(const uint32_t *__restrict__ nib, uint64_t g, uint32_t K,
+                                               unsigned long long &tmpl, uint32_t &amb) {
     const uint32_t *b = nib + (g >> 3);
     const uint64_t lo = (uint64_t)b[0] | ((uint64_t)b[1] << 32), hi = (uint64_t)b[2] | ((uint64_t)b[3] << 32);
     const uint32_t sh = 4 * (uint32_t)(g & 7);
@@ -494,16 +497,43 @@ __device__ __forceinline__ bool lsi_window_key(const uint32_t *__restrict__ nib,
     uint64_t x1 = hi >> sh;                                 // codes 16..
     bool ok = true;
     unsigned long long k = 0;
+    uint32_t a = 0, na = 0;
 #pragma unroll
     for (uint32_t i = 0; i < 25; i++) {
         if (i < K) {
             const uint32_t nbl = (uint32_t)((i < 16 ? x0 >> (4 * i) : x1 >> (4 * (i - 16))) & 15u);
-            ok &= nbl != 0 && (nbl & (nbl - 1)) == 0;
-            k = (k << 2) | (unsigned long long)((nbl >> 1) - (nbl >> 3)); // 1, 2, 4, 8 -> 0..3
+            if ((nbl & (nbl - 1)) == 0) {
+                ok &= nbl != 0;
+                k |= (unsigned long long)((nbl >> 1) - (nbl >> 3)) << (64 - 2 * K + 2 * i); // 1, 2, 4, 8 -> 0..3
+            } else {
+                ok &= na < 3;
+                if (na < 3) a |= (i << (5 * na)) | (nbl << (15 + 4 * na));
+                na++;
+            }
         }
     }
-    key = k;
+    tmpl = k;
+    amb = a | ((na < 3 ? na : 3u) << 27);
     return ok;
+}
+// base strings of a window: the product of its ambiguity codes' base counts
+__device__ __forceinline__ uint32_t lsi_num_expansions(uint32_t amb) {
+    uint32_t c = 1;
+    for (uint32_t a = 0; a < (amb >> 27); a++) c *= (uint32_t)__popc((amb >> (15 + 4 * a)) & 15u);
+    return c;
+}
+// key of expansion x: the x-th combination of the ambiguity codes' bases
+__device__ __forceinline__ unsigned long long lsi_expansion(unsigned long long tmpl, uint32_t amb, uint32_t K,
+                                                            uint32_t x) {
+    for (uint32_t a = 0; a < (amb >> 27); a++) {
+        uint32_t m = (amb >> (15 + 4 * a)) & 15u;
+        const uint32_t c = (uint32_t)__popc(m), pick = x % c;
+        x /= c;
+        for (uint32_t z = 0; z < pick; z++) m &= m - 1;
+        const uint32_t pos = (amb >> (5 * a)) & 31u;
+        tmpl |= (unsigned long long)(__ffs(m) - 1) << (64 - 2 * K + 2 * pos);
+    }
+    return tmpl;
 }
 
 // Base-set nibbles of the flat query array (eight codes per word; words past
@@ -609,13 +639,19 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
     uint64_t slot_base = 0, ient = 0, iend = 0;
     const uint8_t *p = q;
     uint4 piv = make_uint4(0, 0, 0, 0);
-    // learned-seed lookup of q[j-K..j): key, inclusive search range (slots <
-    // 2^32), fence level or key-round part (2: read the next part), keys below
-    // the key so far.  They alias state that is dead from SM_CAND until the
-    // lookup ends (seed entries and windows, the right end), which keeps the
-    // kernel within its register budget.
-    uint64_t &lkey = ient;
+    // learned-seed lookup of q[j-K..j): key of the current base string, search
+    // range [llo, lhi) (slots < 2^32; the lower bound lies in [llo, lhi]),
+    // fence level, key-round part or run entry, keys below the key so far and
+    // the last of them, the first key not below it (~0: none yet); the
+    // window's key template, ambiguity codes and longest occurring suffix so
+    // far live in piv, the base string's number in t, the set built so far in
+    // the ring (ns intervals).  They alias state that is dead from SM_CAND
+    // until the lookup ends (seed entries and windows, the right end, the
+    // interval being extended), which keeps the kernel within its register
+    // budget.
+    uint64_t &lkey = ient, &lpv = iend;
     uint32_t &llo = xi, &lhi = lslmax, &llv = sidx, &lbelow = e;
+    unsigned long long lsv = ~0ull;
 
     for (;;) {
         // ---- control: move this lane to a memory state (or SM_DONE) ----
@@ -670,15 +706,18 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
                         st = SM_FETCH;
                         continue;
                     }
-                    // learned seed of q[j-K..j) when all K codes are bases;
-                    // tried only where the window is likely to occur: on the
-                    // first restart of a work item, and past the code the last
-                    // MEM stopped at (E), so that it holds no known mismatch (an
-                    // absent key costs the whole lookup)
-                    unsigned long long key = 0;
-                    if (LSI && lsik && j >= lsik && (E == 0 ? s0 == ia : j - lsik > E) &&
-                        lsi_window_key(qnib, (uint64_t)(p - q) + j - lsik, lsik, key)) {
-                        lkey = key;
+                    // learned seed of q[j-K..j) (at most three ambiguity codes,
+                    // one lookup per base string): the union of the base
+                    // strings' sets, or, when every one is empty, the longest
+                    // occurring suffix of any of them, which gives L(j)
+                    unsigned long long tmpl = 0;
+                    uint32_t amb = 0;
+                    if (LSI && lsik && j >= lsik &&
+                        lsi_window_amb(qnib, (uint64_t)(p - q) + j - lsik, lsik, tmpl, amb)) {
+                        piv = make_uint4((uint32_t)tmpl, (uint32_t)(tmpl >> 32), amb, 0u);
+                        t = 0;
+                        ns = 0;
+                        lkey = lsi_expansion(tmpl, amb, lsik, 0);
                         st = SM_LSI_LEAF;
                         continue;
                     }
@@ -745,6 +784,34 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
                 } else {
                     st = SM_EMIT;
                 }
+            } else if (LSI && st == SM_LSI_NEXT) {
+                // this base string is done: the next one, or the window's answer
+                const unsigned long long tmpl = (unsigned long long)piv.x | ((unsigned long long)piv.y << 32);
+                if (++t < lsi_num_expansions(piv.z)) {
+                    lkey = lsi_expansion(tmpl, piv.z, lsik, t);
+                    st = SM_LSI_LEAF;
+                } else if (ns > 0) { // the set of q[j-K..j)
+                    nn = 0;
+                    t = 0;
+                    k = j - lsik;
+                    if (k == 0) {
+                        st = SM_ENDLEFT;
+                    } else {
+                        piv = make_uint4(s_lo[head][tid], s_hi[head][tid], s_rl[head][tid], 0u);
+                        st = SM_LEFT;
+                    }
+                } else {
+                    // no compatible occurrence: L(j) = j - (longest occurring
+                    // suffix); below min_len no MEM starts before it, else the
+                    // restart searches for its set
+                    const uint32_t lam = piv.w;
+                    if (lam < min_len) {
+                        s0 = j - lam;
+                        st = SM_CAND;
+                    } else {
+                        st = SM_RESTART;
+                    }
+                }
             } else { // SM_EMIT
                 if (k >= ib) { // starts in a later segment of the read
                     status[w] = 0;
@@ -794,38 +861,28 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
             R = *reinterpret_cast<const uint4 *>(lv.leaves + lsi_leaf(lv.md, lkey));
             probes++;
         } else if (LSI && st == SM_LSI_FENCE) {
-            const int sh = lsi_fence_shift((int)llv);
             const unsigned long long *f = lv.fc.f[0];
 #pragma unroll
             for (uint32_t v = 1; v < kLsiMaxFences; v++) f = llv == v ? lv.fc.f[v] : f;
-            const long long lo = llo, hi = lhi;
-            const long long jj = lsi_last_below(f, (lo >> sh) + 1, hi >> sh, lo >> sh, lkey, probes);
-            llo = (uint32_t)(lo > (jj << sh) ? lo : (jj << sh));
-            lhi = (uint32_t)(hi < ((jj + 1) << sh) ? hi : ((jj + 1) << sh));
+            long long lo = llo, hi = (long long)lhi - 1;
+            lsi_fence_step(f, lsi_fence_shift((int)llv), lo, hi, lkey, probes);
+            llo = (uint32_t)lo;
+            lhi = (uint32_t)(hi + 1);
         } else if (LSI && st == SM_LSI_KEYS) {
-            // part llv of the last round over [llo, lhi] (at most 33 keys): the
-            // 18 keys from (llo & ~1) + 18 llv, read as nine 16-byte pairs
-            const long long b = (long long)(llo & ~1u) + 18 * (long long)llv, hi = lhi, lo = llo;
-            const ulonglong2 *kp = reinterpret_cast<const ulonglong2 *>(lv.keys + b);
-            uint32_t below = 0, inr = 0;
-#pragma unroll
-            for (int u = 0; u < 9; u++) {
-                const long long a0 = b + 2 * u;
-                ulonglong2 v = make_ulonglong2(~0ull, ~0ull);
-                if (a0 <= hi) v = kp[u];
-                const bool in0 = a0 >= lo && a0 <= hi, in1 = a0 + 1 >= lo && a0 + 1 <= hi;
-                below += (in0 && v.x < lkey ? 1u : 0u) + (in1 && v.y < lkey ? 1u : 0u);
-                inr += (in0 ? 1u : 0u) + (in1 ? 1u : 0u);
-                lhit |= (in0 && v.x == lkey) || (in1 && v.y == lkey);
-            }
-            const long long last = b + 17 < hi ? b + 17 : hi;
-            probes += (uint32_t)((((last - b) / 2 + 1) * 16 + 63) / 64);
-            lbelow += below;
-            // inr - below keys of this part are >= the key: the search ends here
-            // unless every key read was below it and the range goes on
-            if (!lhit && below == inr && hi > b + 17) llv = 2; // read the next part
+            // part llv of the last round: slots [llo - 1, lhi] (at most 35
+            // keys), 18 from the even slot below the first per part
+            const long long first = llo > 0 ? (long long)llo - 1 : 0;
+            const long long last = (long long)lhi < (long long)lv.md.nkeys - 1 ? (long long)lhi
+                                                                              : (long long)lv.md.nkeys - 1;
+            bool hs = lsv != ~0ull;
+            unsigned long long pv = lpv, sv = lsv;
+            lsi_keys_part(lv.keys, first, last, (first & ~1ll) + 18 * (long long)llv, lkey, lbelow, pv, sv, hs,
+                          probes);
+            lpv = pv;
+            lsv = sv;
+            lhit = !hs && llv == 0 && (first & ~1ll) + 18 <= last; // read the second part
         } else if (LSI && st == SM_LSI_PAY) {
-            R = lv.pay[llo];
+            R = lv.pay[llo + llv];
             probes++;
         } else if (st == SM_TSA) {
             R.x = fwd.sa_full[piv.x];
@@ -956,45 +1013,64 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
             lf.err = R.y;
             lf.a = __uint_as_float(R.z);
             lf.b = __uint_as_float(R.w);
-            const long long pred = lsi_pred(lf, lsi_stage1(s_lp, lv.md.hidden, lsi_u(lv.md, lkey)));
-            long long lo = pred - (long long)lf.err, hi = pred + (long long)lf.err;
-            lo = lo < 0 ? 0 : lo;
-            hi = hi >= (long long)lv.md.nkeys ? (long long)lv.md.nkeys - 1 : hi;
+            long long lo, hi;
+            lsi_range(lv.md, lf, lsi_stage1(s_lp, lv.md.hidden, lsi_u(lv.md, lkey)), lo, hi);
             llo = (uint32_t)lo;
-            lhi = (uint32_t)(hi < 0 ? 0 : hi);
-            llv = lv.md.nfence - 1;
-            st = lo <= hi ? SM_LSI_FENCE : SM_RESTART;
-        } else if (LSI && st == SM_LSI_FENCE) {
-            if (llv == 0) {
-                lbelow = 0;
-                st = SM_LSI_KEYS;
+            lhi = (uint32_t)(hi + 1);
+            lbelow = 0;
+            lsv = ~0ull;
+            if (lo <= hi) {
+                llv = lv.md.nfence - 1;
+                st = SM_LSI_FENCE;
             } else {
-                llv--;
+                llv = 0;
+                st = SM_LSI_KEYS;
             }
+        } else if (LSI && st == SM_LSI_FENCE) {
+            if (llv == 0) st = SM_LSI_KEYS;
+            else llv--;
         } else if (LSI && st == SM_LSI_KEYS) {
             if (lhit) {
-                llo += lbelow; // the key's slot
-                st = SM_LSI_PAY;
-            } else if (llv == 2) {
                 llv = 1;
             } else {
-                st = SM_RESTART; // absent: the restart falls back
+                const uint32_t lb = (llo > 0 ? llo - 1 : 0u) + lbelow; // the first slot whose key is not below
+                bool next = true;
+                if (lsi_present(lv.md, lkey, lsv, lsv != ~0ull)) {
+                    llo = lb;
+                    llv = 0;
+                    st = SM_LSI_PAY;
+                    next = false;
+                } else {
+                    // no compatible occurrence of this base string: its longest
+                    // occurring suffix
+                    const uint32_t lam = lsi_suffix_len(lv.md, lkey, lpv, lbelow > 0, lsv, lsv != ~0ull);
+                    piv.w = lam > piv.w ? lam : piv.w;
+                }
+                if (next) st = SM_LSI_NEXT;
             }
         } else if (LSI && st == SM_LSI_PAY) {
-            if (R.w & kLsiFlagAmbiguous) {
-                st = SM_RESTART; // also compatible with ambiguity-code windows
-            } else { // the set of q[j-K..j) is this one interval
-                s_lo[head][tid] = R.x;
-                s_hi[head][tid] = R.y;
-                s_rl[head][tid] = R.z;
-                ns = 1; nn = 0; t = 0;
-                k = j - lsik;
-                if (k == 0) {
-                    st = SM_ENDLEFT;
-                } else {
-                    piv = make_uint4(R.x, R.y, R.z, 0u);
-                    st = SM_LEFT;
+            // one interval of the set: the run of full entries with the base
+            // string's codes (its length on the first entry); base strings of
+            // one window can share text windows, whose intervals are kept once
+            if (llv == 0) lhi = R.w >> 8;
+            bool dup = false;
+            if (piv.z >> 27)
+                for (uint32_t u = 0; u < ns; u++) dup |= s_lo[(head + u) & RM][tid] == R.x;
+            if (!dup && ns >= PR) {
+                status[w] = 1;
+                mem_cnt[w] = 0;
+                defer_list[atomicAdd(&ctr->n_retry, 1ull)] = w;
+                atomicAdd(&ctr->diag[1], 1ull);
+                st = SM_FETCH;
+            } else {
+                if (!dup) {
+                    const uint32_t wi = (head + ns) & RM;
+                    s_lo[wi][tid] = R.x;
+                    s_hi[wi][tid] = R.y;
+                    s_rl[wi][tid] = R.z;
+                    ns++;
                 }
+                if (++llv >= lhi) st = SM_LSI_NEXT;
             }
         } else if (ovf) {
             status[w] = 1;

@@ -314,17 +314,20 @@ typedef struct {
                              (needs leaves <= 4096)                          */
 } hsfm_lsi_config;
 typedef struct {
-    uint64_t num_keys;    /* distinct present k-mers                        */
+    uint64_t num_keys;    /* distinct present exact-ACGT k-mers             */
+    uint64_t entries;     /* sorted entries: the k-mers, plus (k <= 25) the
+                             base strings of text windows holding ambiguity
+                             codes and the windows cut short by contig starts */
+    uint64_t variant_entries; /* entries of ambiguity-code windows           */
+    uint64_t short_entries;   /* entries of windows shorter than k           */
     uint32_t max_err;     /* largest per-leaf error bound (slots)           */
     uint32_t fence_levels; /* fence levels of the bounded search (2..7)     */
-    double mean_window;   /* mean search window 2*err+1 over the keys       */
+    double mean_window;   /* mean search window 2*err+2 over the entries    */
     double mse_initial, mse_final; /* stage-1 training loss before / after  */
     double enum_ms, train_ms, fit_ms;
-    uint64_t flagged;     /* k-mers also compatible with a text window holding
-                             an ambiguity code (their seeds fall back)       */
+    double extra_ms;      /* ambiguity and short entries, sort, run bounds  */
     uint32_t mem_seeding; /* 1: usable for MEM restarts (hsfm_mem_seed_mode) */
     uint32_t pad2;
-    double flag_ms;
 } hsfm_lsi_info;
 typedef struct {
     double kernel_ms;     /* HIP events around the kernel                  */

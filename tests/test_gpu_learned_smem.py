@@ -1,9 +1,11 @@
 """Learned seeds inside the SMEM/MEM restarts (hsfm_mem_seed_mode): every
 seeding mode -- the packed tables, plain backward search from the full
 interval, learned 21-mer seeds with either fallback -- gives the oracle's
-MemHit lists, on IUPAC texts (where k-mers compatible with an ambiguity-code
-window are flagged and fall back) and ExactDna, with min_len below, at and
-above k."""
+MemHit lists, on IUPAC texts (where a window's set also holds the intervals of
+ambiguity-code windows compatible with it, stored as entries of their own) and
+ExactDna (where ambiguity codes cut windows, leaving short entries), with
+min_len below, at and above k.  At min_len = k a restart of an all-base read
+never falls back to plain search: the lookup gives the set or L(j)."""
 import numpy as np
 import pytest
 
@@ -31,7 +33,9 @@ def test_every_seed_mode_matches_the_oracle(tag):
     info = lsi.info()
     assert info["mem_seeding"] == 1
     if tag == 0:
-        assert info["flagged"] > 0  # N and other codes in the text make some 21-mers ambiguous
+        assert info["variant_entries"] > 0  # ambiguity codes in the text expand into base strings
+    else:
+        assert info["short_entries"] > 0  # under ExactDna they cut windows short
     flat, off = O.gen_reads(71, text, bounds, 1500, 150, sub=0.01, indel=0.001, both_strands=True)
     reads = (flat, off)
     ref = {}
@@ -51,9 +55,9 @@ def test_every_seed_mode_matches_the_oracle(tag):
     g.set_seed_mode(hf.HSFM_SEED_TABLES)
 
 
-def test_learned_seeding_is_refused_when_flags_cannot_be_complete():
-    """A run of 40 N: its windows have 4^21 expansions, more than the flag
-    pass enumerates, so the index cannot seed MEM restarts exactly and the
+def test_learned_seeding_is_refused_when_sets_cannot_be_complete():
+    """A run of 40 N: its windows have 4^21 expansions, more than the build
+    stores per window, so the index cannot seed MEM restarts exactly and the
     learned modes are refused (the tables and plain search still work)."""
     rng = np.random.default_rng(3)
     seq = random_text(rng, 30_000)
@@ -82,3 +86,37 @@ def test_freeing_the_learned_index_restores_table_seeding():
     b = g.find_mems_batch((flat, off), 21, hf.HSFM_MODE_MEM, bounds, 16)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("tag", [0, 1])
+def test_learned_restarts_never_fall_back_at_min_len_k(tag):
+    """Reads of bases only, min_len = k: every restart window is all bases and
+    j >= k, so the learned lookup answers every restart (its set, or L(j) from
+    the longest occurring suffix) and none runs the unseeded search; the MEM
+    lists equal plain search and the oracle.  Twelve contigs, ambiguity codes
+    in the text (expanded under IupacDna, cutting windows under ExactDna)."""
+    text, bounds = O.gen_reference(90 + tag, 200_000, 12, gc=0.41, repeat_frac=0.4, n_units=25, unit_min=200,
+                                   unit_max=2000, iupac_frac=0.01)
+    o = O.OracleBidir.build(text=text, bounds=bounds, sample_rate=16, tag=tag, threads=8)
+    g = hf.BidirFmIndex.from_bytes(o.to_bytes())
+    lsi = hf.LearnedSeedIndex(g, k=21, leaves=256, hidden=16, train_steps=200)
+    info = lsi.info()
+    assert info["mem_seeding"] == 1 and info["entries"] > info["num_keys"]
+    flat, off = O.gen_reads(91, text, bounds, 2000, 150, sub=0.02, both_strands=True, n_random=200)
+    rng = np.random.default_rng(tag)
+    amb = (flat < 1) | (flat > 4)  # reads copy the text's ambiguity codes: make them bases
+    flat[amb] = rng.integers(1, 5, int(amb.sum())).astype(np.uint8)
+    reads = (flat, off)
+    g.set_seed_mode(hf.HSFM_SEED_PLAIN)
+    plain = g.find_mems_batch(reads, 21, hf.HSFM_MODE_SMEM, bounds, 64)
+    g.set_seed_mode(hf.HSFM_SEED_LEARNED, lsi)
+    got = g.find_mems_batch(reads, 21, hf.HSFM_MODE_SMEM, bounds, 64)
+    st = g.last_stats()
+    g.set_seed_mode(hf.HSFM_SEED_TABLES)
+    for a, b in zip(got, plain):
+        assert np.array_equal(a, b)
+    assert st["diag"][3] == 0  # unseeded restarts
+    sub = [flat[off[i]:off[i + 1]] for i in range(0, 2200, 11)]
+    g.set_seed_mode(hf.HSFM_SEED_LEARNED, lsi)
+    check_mems(o, g, sub, 21, cap=64)
+    g.set_seed_mode(hf.HSFM_SEED_TABLES)
