@@ -421,16 +421,19 @@ def _digest_mem_result(bidir, nq, r):
     return h.hexdigest()
 
 
-def smem_seed_modes_leg(bidir, d_sq, d_so, nq, d_rb, nb, min_len, lsi, args):
+def smem_seed_modes_leg(bidir, d_sq, d_so, nq, d_rb, nb, min_len, lsi, args, lsi_alt=None):
     """The SMEM pipeline (find -> resolve -> ref map) over the same reads with
     each restart seeding: plain backward search from the full interval, the
-    packed 12/14-mer tables, learned 21-mer seeds with plain search as the
-    fallback, learned seeds with the tables as the fallback.  Every mode's
-    result is digested and must be identical."""
+    packed 12/14/15-mer tables, learned 21-mer seeds with plain search as the
+    fallback, learned seeds with the tables as the fallback, and (lsi_alt)
+    learned seeds from an index with more leaves.  Every mode's result is
+    digested and must be identical."""
     import torch
     import haystackfm_amd as hf
     modes = [("plain", hf.HSFM_SEED_PLAIN, None), ("tables", hf.HSFM_SEED_TABLES, None),
              ("learned", hf.HSFM_SEED_LEARNED, lsi), ("learned_tables", hf.HSFM_SEED_LEARNED_TABLES, lsi)]
+    if lsi_alt is not None:
+        modes.append(("learned_more_leaves", hf.HSFM_SEED_LEARNED, lsi_alt))
     out, digests = {}, set()
     for name, mode, l in modes:
         bidir.set_seed_mode(mode, l)
@@ -464,6 +467,10 @@ def smem_seed_modes_leg(bidir, d_sq, d_so, nq, d_rb, nb, min_len, lsi, args):
         out["learned_vs_plain"] = round(out["plain"]["ms_per_step"] / out["learned"]["ms_per_step"], 3)
         out["learned_tables_vs_tables"] = round(out["tables"]["ms_per_step"] / out["learned_tables"]["ms_per_step"], 3)
         out["learned_tables_vs_plain"] = round(out["plain"]["ms_per_step"] / out["learned_tables"]["ms_per_step"], 3)
+        out["learned_vs_tables"] = round(out["tables"]["ms_per_step"] / out["learned"]["ms_per_step"], 3)
+        if "learned_more_leaves" in out:
+            out["learned_more_leaves_vs_plain"] = round(out["plain"]["ms_per_step"] /
+                                                        out["learned_more_leaves"]["ms_per_step"], 3)
     return out
 
 
@@ -583,7 +590,8 @@ def run_1g(args):
     log(f"{len(keys)} read 21-mers")
     keep = []
     lseed = learned_seed_leg(bidir, sq, so, args, keys=keys, keep=keep)
-    lseed_alt = (learned_seed_leg(bidir, sq, so, args, leaves=args.lsi_leaves_alt, keys=keys)
+    keep_alt = []
+    lseed_alt = (learned_seed_leg(bidir, sq, so, args, leaves=args.lsi_leaves_alt, keys=keys, keep=keep_alt)
                  if args.lsi_leaves_alt else None)
 
     # SMEM pipeline over the reads (min_len 21)
@@ -612,7 +620,8 @@ def run_1g(args):
     smem_s = (time.perf_counter() - t0) / args.steps
     smem_stats = bidir.last_stats()
     log(f"SMEM step {smem_s * 1e3:.1f} ms, {r[0]} MEM records, {r[1]} positions")
-    seeding = smem_seed_modes_leg(bidir, d_sq, d_so, nq, d_rb, len(bounds), min_len, keep[0], args)
+    seeding = smem_seed_modes_leg(bidir, d_sq, d_so, nq, d_rb, len(bounds), min_len, keep[0], args,
+                                  keep_alt[0] if keep_alt else None)
 
     # parity samples: SMEM lists and 21-mer intervals against the oracle
     O = oracle_module()
@@ -678,19 +687,34 @@ def run_1g(args):
     sp = lseed["speedup"]
     probes = lseed["learned_probes_per_key"] * lseed["keys"]
     lms = lseed["learned_kernel_ms"]
-    roof = None
+    lroof = None
     if lms > 0:
         a = probes * 64.0 / (lms * 1e-3) / 1e9
+        lroof = {"bound": "hbm", "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(a / HBM_PEAK_GBS, 4), "traffic": measured_traffic_1g("k_lsi_lookup", args, keys),
+                 "kernel": "k_lsi_lookup", "kernel_ms": lms,
+                 "bytes_per_key": round(lseed["learned_probes_per_key"] * 64.0, 1)}
+    # the learned-seed SMEM step's main kernel: 64 B x its device-counted
+    # accesses (learned-index rounds in 64-byte units, interval probes, text
+    # and suffix-array loads) over its HIP-event time
+    lm = seeding["learned"]
+    roof = None
+    if lm["main_kernel_ms"] > 0:
+        a = lm["loads_per_read"] * nq * 64.0 / (lm["main_kernel_ms"] * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(a / HBM_PEAK_GBS, 4), "traffic": measured_traffic_1g("k_lsi_lookup", args, keys),
-                "kernel": "k_lsi_lookup", "kernel_ms": lms,
-                "bytes_per_key": round(lseed["learned_probes_per_key"] * 64.0, 1)}
+                "frac": round(a / HBM_PEAK_GBS, 4), "traffic": measured_traffic_1g("k_mem_persist_lsi", args, keys),
+                "kernel": "k_mem_persist<16,256,true>", "kernel_ms": lm["main_kernel_ms"],
+                "bytes_per_read": round(lm["loads_per_read"] * 64.0, 1),
+                "bytes_definition": "64 B x the kernel's device-counted random accesses (see the 100 Mbp line)"}
+    same = bool(seeding.get("identical_results"))
     out = {
-        "metric": "learned-seed vs plain exact search speedup (read 21-mers, 1 Gbp ref) bit-exact",
-        "value": sp if ok else None,
+        "metric": "learned-seed vs plain backward-search SMEM speedup (1 Gbp ref, 150 bp reads, min_len 21) bit-exact",
+        "value": seeding.get("learned_vs_plain") if ok and same else None,
         "unit": "x",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": lms,
+        "ms_per_step": seeding["learned"]["ms_per_step"],
+        "lookup": {"metric": "learned lookup vs plain 21-step exact search speedup (read 21-mers) bit-exact",
+                   "value": sp if ok else None, "unit": "x", "kernel_ms": lms, "roofline": lroof},
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+u64", "data": "synthetic",
         "config": {"workload": f"learned_seed: {args.big_ref_bp / 1e9:g} Gbp 24-contig reference (45% repeats from "
@@ -895,7 +919,8 @@ def main():
                    "reference_algorithm_probes_per_read": cpu.get("locate_search_probes_per_read")},
         "smem_stats": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in smem_stats.items()},
         "reference_algorithm_probes_per_read": cpu.get("smem_probes_per_read"),
-        "learned_seed_speedup": lseed["speedup"] if lseed else None,
+        "learned_seed_speedup": None,  # the 1 Gbp learned-seed SMEM speedup, below
+        "learned_seed_lookup_speedup": lseed["speedup"] if lseed else None,
         "learned_seed": lseed,
         "learned_seed_more_leaves": lseed_alt,
         "uneven_batched": uneven,
@@ -903,10 +928,11 @@ def main():
     if not args.no_1g:
         del bidir
         big = run_1g(args)
-        out["learned_seed_1g"] = {k: big.get(k) for k in ("config", "value", "ms_per_step", "roofline", "parity",
-                                                          "parity_sample", "cpu_baseline", "learned_seed",
+        out["learned_seed_1g"] = {k: big.get(k) for k in ("config", "value", "ms_per_step", "lookup", "roofline",
+                                                          "parity", "parity_sample", "cpu_baseline", "learned_seed",
                                                           "smem_seeding", "smem")}
         out["learned_seed_1g"]["value_metric"] = big.get("metric")
+        out["learned_seed_speedup"] = big.get("value")
     print(json.dumps(out), flush=True)
 
 

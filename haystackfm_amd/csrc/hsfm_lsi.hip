@@ -381,14 +381,19 @@ __global__ void k_lsi_adam(float *__restrict__ p, float *__restrict__ m, float *
     p[i] = pi;
 }
 
-// training sample: every stride-th slot
-__global__ void k_lsi_sample(const unsigned long long *__restrict__ keys, uint64_t nkeys, uint64_t stride,
-                             uint32_t ns, LsiModel md, float *__restrict__ x, float *__restrict__ y) {
+// training sample, every stride-th slot: the key's position inside its leaf
+// (u) and its rank there as a fraction of the leaf's entries -- the in-leaf
+// CDF that every leaf's map then scales
+__global__ void k_lsi_sample(const unsigned long long *__restrict__ keys, uint64_t stride, uint32_t ns, LsiModel md,
+                             const LsiLeaf *__restrict__ lf, const uint32_t *__restrict__ cnt, float *__restrict__ x,
+                             float *__restrict__ y) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ns) return;
     const uint64_t s = (uint64_t)i * stride;
-    x[i] = lsi_feature(md, keys[s]);
-    y[i] = (float)((double)s / (double)nkeys);
+    const unsigned long long key = keys[s];
+    const uint32_t l = lsi_leaf(md, key);
+    x[i] = lsi_u(md, key);
+    y[i] = (float)((double)(s - lf[l].s0) / (double)cnt[l]);
 }
 
 // ---- 5. stage 2 ----------------------------------------------------------
@@ -904,6 +909,21 @@ int lsi_train_and_fit(hsfm_lsi *x) {
     HSFM_HIP(hipMemcpyAsync(x->params.p, hp, np * 4, hipMemcpyHostToDevice, s));
     DevBuf xs, ys, gm, lossb, cnt, first, mom;
     Releaser<7> rel{{&xs, &ys, &gm, &lossb, &cnt, &first, &mom}};
+    // leaves: entries per leaf and each leaf's first slot (the model has no
+    // part in routing, so they come first and give the training targets)
+    if ((rc = x->leaves.ensure((size_t)L * sizeof(LsiLeaf))) || (rc = cnt.ensure((size_t)L * 4 + 4)) ||
+        (rc = first.ensure((size_t)L * 4 + 4)) || (rc = mom.ensure((size_t)L * 32)))
+        return rc;
+    HSFM_HIP(hipMemsetAsync(cnt.p, 0, (size_t)L * 4, s));
+    const LsiModel md0 = x->model();
+    const uint64_t chunks = (N + kLsiChunk - 1) / kLsiChunk;
+    const dim3 cg((uint32_t)((chunks + 255) / 256));
+    if (N)
+        hipLaunchKernelGGL(k_lsi_route, cg, dim3(256), 0, s, md0, x->keys.as<unsigned long long>(), cnt.as<uint32_t>());
+    if ((rc = lsi_scan_u32(idx, cnt.as<uint32_t>(), first.as<uint32_t>(), L))) return rc;
+    hipLaunchKernelGGL(k_lsi_leaf_init, dim3((L + 255) / 256), dim3(256), 0, s, x->leaves.as<LsiLeaf>(),
+                       first.as<uint32_t>(), L);
+    HSFM_HIP(hipGetLastError());
     const uint32_t ns = (uint32_t)(N < x->cfg.sample ? N : x->cfg.sample);
     if ((rc = xs.ensure((size_t)ns * 4 + 4)) || (rc = ys.ensure((size_t)ns * 4 + 4)) ||
         (rc = gm.ensure(np * 4 * 3)) || (rc = lossb.ensure(8)))
@@ -914,11 +934,11 @@ int lsi_train_and_fit(hsfm_lsi *x) {
     HSFM_HIP(hipEventCreate(&e0));
     HSFM_HIP(hipEventCreate(&e1));
     HSFM_HIP(hipEventRecord(e0, s));
-    const LsiModel md0 = x->model();
     if (ns) {
         const uint64_t stride = N / ns;
-        hipLaunchKernelGGL(k_lsi_sample, dim3((ns + 255) / 256), dim3(256), 0, s, x->keys.as<unsigned long long>(), N,
-                           stride ? stride : 1, ns, md0, xs.as<float>(), ys.as<float>());
+        hipLaunchKernelGGL(k_lsi_sample, dim3((ns + 255) / 256), dim3(256), 0, s, x->keys.as<unsigned long long>(),
+                           stride ? stride : 1, ns, md0, x->leaves.as<LsiLeaf>(), cnt.as<uint32_t>(), xs.as<float>(),
+                           ys.as<float>());
         HSFM_HIP(hipGetLastError());
         float b1t = 1.f, b2t = 1.f;
         for (uint32_t step = 0; step <= x->cfg.train_steps; step++) {
@@ -947,21 +967,10 @@ int lsi_train_and_fit(hsfm_lsi *x) {
     x->info.train_ms = elapsed_ms(e0, e1);
     xs.release(); ys.release(); gm.release(); lossb.release();
 
-    // stage 2: leaves are the keys' top log2(L) bits
+    // stage 2: per leaf the least-squares map of the trained f, its error
     HSFM_HIP(hipEventRecord(e0, s));
-    if ((rc = x->leaves.ensure((size_t)L * sizeof(LsiLeaf))) || (rc = cnt.ensure((size_t)L * 4 + 4)) ||
-        (rc = first.ensure((size_t)L * 4 + 4)) || (rc = mom.ensure((size_t)L * 32)))
-        return rc;
     HSFM_HIP(hipMemsetAsync(mom.p, 0, (size_t)L * 32, s));
-    HSFM_HIP(hipMemsetAsync(cnt.p, 0, (size_t)L * 4, s));
     const LsiModel md = x->model();
-    const uint64_t chunks = (N + kLsiChunk - 1) / kLsiChunk;
-    const dim3 cg((uint32_t)((chunks + 255) / 256));
-    if (N)
-        hipLaunchKernelGGL(k_lsi_route, cg, dim3(256), 0, s, md, x->keys.as<unsigned long long>(), cnt.as<uint32_t>());
-    if ((rc = lsi_scan_u32(idx, cnt.as<uint32_t>(), first.as<uint32_t>(), L))) return rc;
-    hipLaunchKernelGGL(k_lsi_leaf_init, dim3((L + 255) / 256), dim3(256), 0, s, x->leaves.as<LsiLeaf>(),
-                       first.as<uint32_t>(), L);
     if (N)
         hipLaunchKernelGGL(k_lsi_moments, cg, dim3(256), 0, s, md, x->keys.as<unsigned long long>(),
                            x->leaves.as<LsiLeaf>(), mom.as<double>());
@@ -1103,10 +1112,10 @@ int lsi_seed_impl(hsfm_lsi *x, const uint64_t *d_keys, uint64_t n, uint32_t *d_o
     HSFM_HIP(hipMemsetAsync(idx->ws.counters.p, 0, 8, s));
     const LsiModel md = x->model();
     if ((rc = event_record(idx, 0, false))) return rc;
-    // leaves in LDS when they fit and two fence levels suffice (measured:
+    // leaves in LDS when they fit and at most two fence levels are needed (measured:
     // 100 Mbp, 4096 leaves, 0.66 ms against 0.88 ms; 1 Gbp, whose windows need
     // a third level, 1.16 ms against 1.12 ms)
-    const bool lds = x->cfg.lookup == 2 || (x->cfg.lookup == 0 && x->cfg.leaves <= kLsiLdsLeaves && x->nfence == 2);
+    const bool lds = x->cfg.lookup == 2 || (x->cfg.lookup == 0 && x->cfg.leaves <= kLsiLdsLeaves && x->nfence <= 2);
     const unsigned long long *dk = reinterpret_cast<const unsigned long long *>(d_keys);
     if (n && lds) {
         const uint64_t need = (n + 1023) / 1024, cus = (uint64_t)idx->dev->num_cus;
@@ -1199,7 +1208,7 @@ extern "C" int hsfm_lsi_seed_device_noalloc(hsfm_lsi *x, const uint64_t *d_keys,
     hipStream_t s = idx->dev->stream;
     hipLaunchKernelGGL(k_lsi_zero, dim3(1), dim3(64), 0, s, sc, kLsiScratchWords);
     const LsiModel md = x->model();
-    const bool lds = x->cfg.lookup == 2 || (x->cfg.lookup == 0 && x->cfg.leaves <= kLsiLdsLeaves && x->nfence == 2);
+    const bool lds = x->cfg.lookup == 2 || (x->cfg.lookup == 0 && x->cfg.leaves <= kLsiLdsLeaves && x->nfence <= 2);
     const unsigned long long *dk = reinterpret_cast<const unsigned long long *>(d_keys);
     if (n && lds) {
         const uint64_t need = (n + 1023) / 1024, cus = (uint64_t)idx->dev->num_cus;

@@ -30,11 +30,13 @@ constexpr uint32_t kLsiMaxHidden = 64;
 // 2^33 keys spans at most 17 entries of level 6 (stride 2^29): seven levels.
 constexpr int kLsiMaxFences = 7;
 __host__ __device__ constexpr int lsi_fence_shift(int v) { return 5 + 4 * v; }
-// Fence levels needed for a largest per-leaf error emax: the coarsest level v
-// must span the widest window (2 emax + 2 keys) in at most 17 entries,
-// (2 emax + 1) >> shift <= 16; levels 0 and 1 are always built.
+// Fence levels needed for a largest per-leaf error emax: none when the widest
+// range [lo, hi] (hi - lo <= 2 emax + 1) already fits the last round (hi - lo
+// <= 32), else as many as make the coarsest level v span it in at most 17
+// entries, (2 emax + 1) >> shift(v) <= 16.
 inline int lsi_fence_levels(uint64_t emax) {
-    int nl = 2;
+    if (2ull * emax + 1ull <= 32ull) return 0;
+    int nl = 1;
     while (nl < kLsiMaxFences && ((2ull * emax + 1ull) >> lsi_fence_shift(nl - 1)) > 16) nl++;
     return nl;
 }
@@ -62,7 +64,7 @@ struct LsiModel {
                     // of H clipped ramps
     uint32_t hidden, leaves, k;
     uint64_t nkeys;           // entries
-    uint32_t nfence;          // fence levels in use (2..kLsiMaxFences)
+    uint32_t nfence;          // fence levels in use (0..kLsiMaxFences)
     uint32_t lbits;           // log2 leaves: leaf = the key's top lbits bits
     unsigned long long smask; // the code bits of a key (top 2k bits)
 };
@@ -74,10 +76,6 @@ __host__ __device__ __forceinline__ unsigned long long lsi_key_of_fwd(unsigned l
     return ((y >> 1) & 0x5555555555555555ull) | ((y & 0x5555555555555555ull) << 1);
 }
 
-// top 24 code bits / 2^24 in [0, 1) (the stage-1 training feature)
-__device__ __forceinline__ float lsi_feature(const LsiModel &md, unsigned long long key) {
-    return (float)(uint32_t)((key & md.smask) >> 40) * 5.9604644775390625e-8f;
-}
 // f(x) = b2 + sum_i w2_i clamp(w1_i x + b1_i, 0, 1): with w1, w2 >= 0 every
 // step (fma, clamp, fma) is non-decreasing in x, rounding included
 __device__ __forceinline__ float lsi_stage1(const float *p, uint32_t H, float x) {

@@ -1019,7 +1019,7 @@ __global__ __launch_bounds__(PB, PB >= 256 ? 3 : 1) void k_mem_persist(DevDir fw
             lhi = (uint32_t)(hi + 1);
             lbelow = 0;
             lsv = ~0ull;
-            if (lo <= hi) {
+            if (lo <= hi && lv.md.nfence) {
                 llv = lv.md.nfence - 1;
                 st = SM_LSI_FENCE;
             } else {

@@ -273,12 +273,15 @@ typedef struct {
 int hsfm_last_stats(hsfm_index *idx, hsfm_stats *out);
 /* Seeding of the restarts of the one-shot MEM calls on idx (hsfm_find_mems_batch
  * / _device); every mode gives the same results.  HSFM_SEED_TABLES: packed
- * 12-/14-mer tables built from idx (default).  HSFM_SEED_PLAIN: plain backward
- * search from the full interval, the reference's find_smems restart
- * (src/fm_index/smem.rs:154-222).  HSFM_SEED_LEARNED: the interval of
- * q[j-k..j) from the learned seed index lsi (built on idx, k <= 25) where that
- * k-mer is all bases and its IUPAC set is its exact interval, else plain
- * search; HSFM_SEED_LEARNED_TABLES: the same with the tables as fallback.  The
+ * 12-/14-/15-mer tables built from idx (default).  HSFM_SEED_PLAIN: plain
+ * backward search from the full interval, the reference's find_smems restart
+ * (src/fm_index/smem.rs:154-222).  HSFM_SEED_LEARNED: the learned seed index
+ * lsi (built on idx, k <= 25) answers the restart at j when q[j-k..j) holds at
+ * most three ambiguity codes: the set of q[j-k..j) (every interval of its
+ * IUPAC-compatible text windows), or, when that set is empty, the longest
+ * occurring suffix of q[j-k..j), which is L(j); plain search otherwise (j < k,
+ * more ambiguity codes, or an empty set with L(j) <= j - min_len);
+ * HSFM_SEED_LEARNED_TABLES: the same with the tables as fallback.  The
  * learned modes keep lsi attached until it is freed (which restores
  * HSFM_SEED_TABLES).  The batched pipeline always seeds from the tables.
  * No counterpart in the reference. */
@@ -298,8 +301,13 @@ int hsfm_last_deferred(hsfm_index *idx, uint32_t *ids, uint64_t cap, uint64_t *c
  * New component (no reference entry point): replaces the first k exact
  * backward-search steps of an exact-ACGT k-mer.  Its answer is that of plain
  * search, extend_left over the k-mer (bidir.rs:112-130), as LookupTable::build
- * computes it for depth-k prefixes (lookup.rs:37-91).  Keys pack the k-mer
- * two bits per base, first base most significant (A=0, C=1, G=2, T=3). */
+ * computes it for depth-k prefixes (lookup.rs:37-91).  Keys of these calls
+ * pack the k-mer two bits per base, first base most significant (A=0, C=1,
+ * G=2, T=3).  For k <= 25 the same index also answers MEM restarts
+ * (hsfm_mem_seed_mode): it holds, besides the k-mers, the base strings of every
+ * text window with ambiguity codes and the windows cut short by contig starts,
+ * sorted by their reversed codes, so a lookup gives a window's whole IUPAC set
+ * or its longest occurring suffix. */
 typedef struct {
     uint32_t k;           /* k-mer length, 1..31                            */
     uint32_t leaves;      /* stage-2 leaves, a power of two (4096): leaf =
@@ -309,8 +317,8 @@ typedef struct {
     uint32_t sample;      /* training sample size (keys, evenly strided)    */
     float lr;             /* Adam learning rate                             */
     uint32_t lookup;      /* leaf placement of the lookup kernel: 0 = chosen
-                             by the build (LDS when the leaves fit and two
-                             fence levels suffice), 1 = global, 2 = LDS
+                             by the build (LDS when the leaves fit and at
+                             most two fence levels), 1 = global, 2 = LDS
                              (needs leaves <= 4096)                          */
 } hsfm_lsi_config;
 typedef struct {
@@ -321,7 +329,7 @@ typedef struct {
     uint64_t variant_entries; /* entries of ambiguity-code windows           */
     uint64_t short_entries;   /* entries of windows shorter than k           */
     uint32_t max_err;     /* largest per-leaf error bound (slots)           */
-    uint32_t fence_levels; /* fence levels of the bounded search (2..7)     */
+    uint32_t fence_levels; /* fence levels of the bounded search (0..7)     */
     double mean_window;   /* mean search window 2*err+2 over the entries    */
     double mse_initial, mse_final; /* stage-1 training loss before / after  */
     double enum_ms, train_ms, fit_ms;
@@ -334,7 +342,8 @@ typedef struct {
     uint64_t probes;      /* learned: 64 B units the bounded search loaded  *
                            * (fence and key rounds); plain search: 0        */
 } hsfm_seed_stats;
-/* Build from idx itself: every present k-mer, its interval, both stages. */
+/* Build from idx itself: every present k-mer, its interval, the restart
+ * entries (k <= 25, when the device text is available), both stages. */
 int hsfm_lsi_build(hsfm_index *idx, const hsfm_lsi_config *cfg, hsfm_lsi **out);
 int hsfm_lsi_get_info(const hsfm_lsi *lsi, hsfm_lsi_info *out);
 /* Per key (device arrays): d_out[4i..4i+4) = {fwd_lo, fwd_hi, rev_lo, found}. */

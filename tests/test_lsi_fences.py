@@ -23,7 +23,9 @@ def shift(v):
 
 
 def levels_for(emax):
-    nl = 2
+    if 2 * emax + 1 <= 32:
+        return 0
+    nl = 1
     while nl < MAX_LEVELS and ((2 * emax + 1) >> shift(nl - 1)) > 16:
         nl += 1
     return nl
@@ -143,11 +145,14 @@ def test_small_arrays_match_bisection():
 
 
 def test_levels_rule_values():
-    assert levels_for(0) == 2
-    assert levels_for(4119) == 2          # 100 Mbp, 4096 leaves
-    assert levels_for(4351) == 2          # (2 emax) >> 9 <= 16
+    assert levels_for(0) == 0
+    assert levels_for(15) == 0            # the last round alone covers [lo, hi]
+    assert levels_for(16) == 1
+    assert levels_for(271) == 1           # (2 emax + 1) >> 5 <= 16
+    assert levels_for(272) == 2
+    assert levels_for(4351) == 2          # (2 emax + 1) >> 9 <= 16
     assert levels_for(4352) == 3
-    assert levels_for(69_631) == 3        # 1 Gbp, 4096 leaves: 3 levels
+    assert levels_for(69_631) == 3
     assert levels_for(69_632) == 4
     assert levels_for(340_899) == 4
     assert levels_for((1 << 32) - 1) == 7
