@@ -76,6 +76,10 @@ struct DevDir {
     // bits 4 (i & 7) of word i >> 3; zero past the end, padded by 16 bytes),
     // or null when the alphabet does not allow nibble comparison
     const uint32_t *text4;
+    // base-set nibble of query bytes 0..15 (nibble b = byte b's set) when no
+    // other byte has one, for comparing query bytes with text4 in registers;
+    // 0 otherwise
+    unsigned long long qnib16;
     uint32_t num_samples;
     uint32_t sample_rate;
     uint32_t num_lanes;

@@ -31,13 +31,13 @@ constexpr uint32_t KR = kLocateRegCap;
 constexpr uint8_t kNivOverflow = 0xff;
 
 struct LocateCounters {            // device counters (u64 each)
-    unsigned long long n_ovf;      // reads queued for the wave path
+    unsigned long long n_ovf;      // reads the register kernel gives to the group pass
     unsigned long long work;       // wave path work index
     unsigned long long fb_used;    // fallback interval slots reserved
     unsigned long long n_retry;    // reads that need a larger wave capacity
     unsigned long long guard;      // walk guard hits
     unsigned long long probes;     // records fetched by the main search kernel
-    unsigned long long pwork;      // work index of k_locate_persist
+    unsigned long long n_wave;     // reads the group pass gives to the wave pass
     unsigned long long pad[1];
 };
 
@@ -108,38 +108,81 @@ __global__ __launch_bounds__(256) void k_locate_search(DevDir d, const DevConst 
 // compatible with the text), so the read matches iff the text left of the
 // occurrence is compatible with q[0..i), and then its one row is the row of
 // text position sa_full[lo] - i.  Compares 8 codes per load pair.
-__device__ __forceinline__ void locate_finish_text(const DevDir &d, const uint16_t *s_compat, const uint8_t *p,
-                                                   uint32_t i, uint32_t &lo, uint32_t &hi, uint32_t &ns,
-                                                   uint32_t &probes) {
+__device__ __forceinline__ void locate_finish_text(const DevDir &d, const uint16_t *s_compat,
+                                                   const uint8_t *s_expand, const uint8_t *p, uint32_t i,
+                                                   uint32_t &lo, uint32_t &hi, uint32_t &ns, uint32_t &probes) {
     uint32_t tp = d.sa_full[lo]; // text position of q[i]
     probes++;
     uint32_t tq = i;
     bool ok = true;
-    while (tq > 0 && ok) {
-        if (tp == 0) { // nothing left of the text start
-            ok = false;
-            break;
-        }
-        const uint32_t tl = tp - 1;
-        const uintptr_t qa = reinterpret_cast<uintptr_t>(p + tq - 1);
-        const uint2 tw = *reinterpret_cast<const uint2 *>(d.text + (tl & ~7u));
-        const uint2 qw = *reinterpret_cast<const uint2 *>(qa & ~(uintptr_t)7);
-        probes++;
-        const uint32_t to = tl & 7u, qo = (uint32_t)(qa & 7);
-        const uint32_t lim = min(min(to + 1, qo + 1), tq);
-        const uint64_t t64 = (uint64_t)tw.x | ((uint64_t)tw.y << 32), q64 = (uint64_t)qw.x | ((uint64_t)qw.y << 32);
-        uint32_t adv = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < 8; k++) {
-            if (k < lim && ok) {
-                const uint32_t lane = (uint32_t)(t64 >> (8 * (to - k))) & 0xffu;
-                const uint32_t code = (uint32_t)(q64 >> (8 * (qo - k))) & 0xffu;
-                if ((s_compat[code] >> lane) & 1u) adv++;
-                else ok = false;
+    if (d.text4 && d.qnib16) {
+        // up to 16 codes per load pair, leftwards: the aligned 16-code group of
+        // text nibbles holding text position tp - 1 and the aligned 16 query
+        // bytes holding q[tq - 1], compared as base sets in registers (a code
+        // matches while its set and the text lane's share a base; the
+        // sentinel's set is empty, so the comparison stops there as the index
+        // extension does)
+        const unsigned long long tab = d.qnib16;
+        while (tq > 0 && ok) {
+            if (tp == 0) { // nothing left of the text start
+                ok = false;
+                break;
             }
+            const uint32_t tl = tp - 1;
+            const uintptr_t qa = reinterpret_cast<uintptr_t>(p + tq - 1);
+            const uint2 tw = *reinterpret_cast<const uint2 *>(d.text4 + ((tl >> 3) & ~1u));
+            const uint4 qw = *reinterpret_cast<const uint4 *>(qa & ~(uintptr_t)15);
+            probes++;
+            const uint32_t to = tl & 15u, qo = (uint32_t)(qa & 15);
+            const uint32_t lim = min(min(to + 1, qo + 1), tq);
+            // query bytes -> nibbles (byte j of the group -> nibble j); bytes
+            // of 16 and more have no base
+            const uint32_t qb[4] = {qw.x, qw.y, qw.z, qw.w};
+            unsigned long long qn = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < 16; j++) {
+                const uint32_t c = (qb[j >> 2] >> (8 * (j & 3))) & 0xffu;
+                qn |= (c < 16 ? (tab >> (4 * c)) & 15ull : 0ull) << (4 * j);
+            }
+            // position tl at the top nibble of t, q[tq - 1] at the top of q:
+            // nibble 15 - k holds the k-th code leftwards of both
+            const unsigned long long t = ((unsigned long long)tw.x | ((unsigned long long)tw.y << 32)) << (4 * (15 - to));
+            const unsigned long long x = t & (qn << (4 * (15 - qo)));
+            const unsigned long long z = ~(x | (x >> 1) | (x >> 2) | (x >> 3)) & 0x1111111111111111ull;
+            // leading codes that match: up to the highest zero nibble
+            const uint32_t run = z ? (uint32_t)__clzll((long long)z) >> 2 : 16u;
+            const uint32_t adv = run < lim ? run : lim;
+            if (run < lim) ok = false;
+            tq -= adv;
+            tp -= adv;
         }
-        tq -= adv;
-        tp -= adv;
+    } else {
+        while (tq > 0 && ok) {
+            if (tp == 0) { // nothing left of the text start
+                ok = false;
+                break;
+            }
+            const uint32_t tl = tp - 1;
+            const uintptr_t qa = reinterpret_cast<uintptr_t>(p + tq - 1);
+            const uint2 tw = *reinterpret_cast<const uint2 *>(d.text + (tl & ~7u));
+            const uint2 qw = *reinterpret_cast<const uint2 *>(qa & ~(uintptr_t)7);
+            probes++;
+            const uint32_t to = tl & 7u, qo = (uint32_t)(qa & 7);
+            const uint32_t lim = min(min(to + 1, qo + 1), tq);
+            const uint64_t t64 = (uint64_t)tw.x | ((uint64_t)tw.y << 32), q64 = (uint64_t)qw.x | ((uint64_t)qw.y << 32);
+            uint32_t adv = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < 8; k++) {
+                if (k < lim && ok) {
+                    const uint32_t lane = (uint32_t)(t64 >> (8 * (to - k))) & 0xffu;
+                    const uint32_t code = (uint32_t)(q64 >> (8 * (qo - k))) & 0xffu;
+                    if ((s_compat[code] >> lane) & 1u) adv++;
+                    else ok = false;
+                }
+            }
+            tq -= adv;
+            tp -= adv;
+        }
     }
     if (ok) {
         lo = d.isa[tp];
@@ -219,7 +262,7 @@ __device__ __forceinline__ uint32_t locate_read(const DevDir &d, const DevConst 
     }
     const bool textmode = d.isa != nullptr && d.text != nullptr && d.sa_full != nullptr;
     if (textmode && ns == 1 && hi[0] - lo[0] == 1 && start > 0) {
-        locate_finish_text(d, s_compat, p, (uint32_t)start, lo[0], hi[0], ns, probes);
+        locate_finish_text(d, s_compat, s_expand, p, (uint32_t)start, lo[0], hi[0], ns, probes);
         start = 0;
     }
     for (int i = start - 1; i >= 0 && ns; i--) {
@@ -258,7 +301,7 @@ __device__ __forceinline__ uint32_t locate_read(const DevDir &d, const DevConst 
         for (uint32_t j = 0; j < KR; j++) { lo[j] = nlo[j]; hi[j] = nhi[j]; }
         ns = nn;
         if (textmode && ns == 1 && hi[0] - lo[0] == 1 && i > 0) {
-            locate_finish_text(d, s_compat, p, (uint32_t)i, lo[0], hi[0], ns, probes);
+            locate_finish_text(d, s_compat, s_expand, p, (uint32_t)i, lo[0], hi[0], ns, probes);
             break;
         }
     }
@@ -298,36 +341,21 @@ __device__ __forceinline__ uint32_t locate_read(const DevDir &d, const DevConst 
 // group's LDS list; seeding and the text finish run on lane 0.  Every
 // decision (set size, give-up, loop index) is uniform over the group.
 // ---------------------------------------------------------------------------
+// One read of the group pass (G lanes of one wave).
 template <uint32_t G>
-__global__ __launch_bounds__(256) void k_locate_search_group(DevDir d, const DevConst *__restrict__ cst,
-                                                             const uint8_t *__restrict__ q,
-                                                             const uint64_t *__restrict__ qoff,
-                                                             const uint32_t *__restrict__ list, uint32_t nlist,
-                                                             uint32_t *__restrict__ cnt, uint8_t *__restrict__ niv,
-                                                             uint2 *__restrict__ ivs, uint2 *__restrict__ fb_meta,
-                                                             uint2 *__restrict__ fb_ivs, uint64_t fb_cap,
-                                                             uint32_t *__restrict__ ovf_list,
-                                                             LocateCounters *__restrict__ ctr,
-                                                             const uint4 *__restrict__ seed_rec,
-                                                             const uint4 *__restrict__ seed_iv, uint32_t F) {
-    static_assert(G >= 16 && G <= 64 && (G & (G - 1)) == 0, "group size: a power of two in 16..64");
-    constexpr uint32_t NG = 256 / G;
-    __shared__ uint16_t s_compat[256];
-    __shared__ int8_t s_digit[256];
-    __shared__ uint8_t s_expand[256];
-    __shared__ uint32_t s_clane[16];
-    __shared__ uint32_t s_lo[NG][G], s_hi[NG][G];
-    load_const_lds(cst, s_compat, s_digit, s_clane);
-    for (uint32_t x = threadIdx.x; x < 256; x += blockDim.x) s_expand[x] = cst->seed_expand[x];
-    __syncthreads();
-    const uint32_t gl = threadIdx.x % G, gi = threadIdx.x / G;
-    const uint32_t t = blockIdx.x * NG + gi;
-    if (t >= nlist) return; // the whole group
-    const uint32_t r = list[t];
+__device__ __forceinline__ void locate_group_read(const DevDir &d, const DevConst *__restrict__ cst,
+                                                  const uint8_t *__restrict__ q, const uint64_t *__restrict__ qoff,
+                                                  uint32_t r, uint32_t *__restrict__ cnt, uint8_t *__restrict__ niv,
+                                                  uint2 *__restrict__ ivs, uint2 *__restrict__ fb_meta,
+                                                  uint2 *__restrict__ fb_ivs, uint64_t fb_cap,
+                                                  uint32_t *__restrict__ ovf_list, LocateCounters *__restrict__ ctr,
+                                                  const uint4 *__restrict__ seed_rec, const uint4 *__restrict__ seed_iv,
+                                                  uint32_t F, const uint16_t *s_compat, const int8_t *s_digit,
+                                                  const uint8_t *s_expand, const uint32_t *s_clane, uint32_t *glo,
+                                                  uint32_t *ghi, uint32_t gl) {
     const uint64_t b0 = qoff[r];
     const uint32_t m = (uint32_t)(qoff[r + 1] - b0);
     const uint8_t *p = q + b0;
-    uint32_t *glo = s_lo[gi], *ghi = s_hi[gi];
 
     // seeding on lane 0, into the group's list
     uint32_t ns = 0, give = 0;
@@ -417,7 +445,7 @@ __global__ __launch_bounds__(256) void k_locate_search_group(DevDir d, const Dev
         uint32_t rlo = gl < nrows ? glo[gl] : 0u, rhi = rlo + 1u, rns = gl < nrows ? 1u : 0u;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        if (rns) locate_finish_text(d, s_compat, p, at, rlo, rhi, rns, tprobes);
+        if (rns) locate_finish_text(d, s_compat, s_expand, p, at, rlo, rhi, rns, tprobes);
         uint32_t pos = rns;
 #pragma unroll
         for (uint32_t dd = 1; dd < G; dd <<= 1) {
@@ -516,8 +544,39 @@ __global__ __launch_bounds__(256) void k_locate_search_group(DevDir d, const Dev
     if (gl == 0) {
         niv[r] = kNivOverflow;
         cnt[r] = 0;
-        ovf_list[atomicAdd(&ctr->n_ovf, 1ull)] = r;
+        ovf_list[atomicAdd(&ctr->n_wave, 1ull)] = r;
     }
+}
+
+template <uint32_t G>
+__global__ __launch_bounds__(256) void k_locate_search_group(DevDir d, const DevConst *__restrict__ cst,
+                                                             const uint8_t *__restrict__ q,
+                                                             const uint64_t *__restrict__ qoff,
+                                                             const uint32_t *__restrict__ list, uint32_t nlist,
+                                                             uint32_t *__restrict__ cnt, uint8_t *__restrict__ niv,
+                                                             uint2 *__restrict__ ivs, uint2 *__restrict__ fb_meta,
+                                                             uint2 *__restrict__ fb_ivs, uint64_t fb_cap,
+                                                             uint32_t *__restrict__ ovf_list,
+                                                             LocateCounters *__restrict__ ctr,
+                                                             const uint4 *__restrict__ seed_rec,
+                                                             const uint4 *__restrict__ seed_iv, uint32_t F) {
+    static_assert(G >= 16 && G <= 64 && (G & (G - 1)) == 0, "group size: a power of two in 16..64");
+    constexpr uint32_t NG = 256 / G;
+    __shared__ uint16_t s_compat[256];
+    __shared__ int8_t s_digit[256];
+    __shared__ uint8_t s_expand[256];
+    __shared__ uint32_t s_clane[16];
+    __shared__ uint32_t s_lo[NG][G], s_hi[NG][G];
+    load_const_lds(cst, s_compat, s_digit, s_clane);
+    for (uint32_t x = threadIdx.x; x < 256; x += blockDim.x) s_expand[x] = cst->seed_expand[x];
+    __syncthreads();
+    const uint32_t gl = threadIdx.x % G, gi = threadIdx.x / G;
+    // a persistent grid: the register kernel's count of reads comes from the
+    // device, so no host round trip sits between the two passes
+    const uint32_t nl = (uint32_t)ctr->n_ovf;
+    for (uint32_t t = blockIdx.x * NG + gi; t < nl; t += gridDim.x * NG)
+        locate_group_read<G>(d, cst, q, qoff, list[t], cnt, niv, ivs, fb_meta, fb_ivs, fb_cap, ovf_list, ctr,
+                             seed_rec, seed_iv, F, s_compat, s_digit, s_expand, s_clane, s_lo[gi], s_hi[gi], gl);
 }
 
 // One wave per queued read.  Set buffers: per wave 2 * cap (lo,hi) pairs.
@@ -529,16 +588,22 @@ __global__ __launch_bounds__(64) void k_locate_wave(DevDir d, const DevConst *__
                                                     uint32_t *__restrict__ cnt, uint2 *__restrict__ fb_meta,
                                                     uint2 *__restrict__ fb_ivs, uint64_t fb_cap,
                                                     uint32_t *__restrict__ retry_list,
-                                                    LocateCounters *__restrict__ ctr) {
+                                                    LocateCounters *__restrict__ ctr,
+                                                    const unsigned long long *__restrict__ nlist_dev,
+                                                    const uint4 *__restrict__ seed_rec,
+                                                    const uint4 *__restrict__ seed_iv, uint32_t F) {
     __shared__ uint16_t s_compat[256];
     __shared__ int8_t s_digit[256];
+    __shared__ uint8_t s_expand[256];
     __shared__ uint32_t s_clane[16];
     load_const_lds(cst, s_compat, s_digit, s_clane);
+    for (uint32_t x = threadIdx.x; x < 256; x += blockDim.x) s_expand[x] = cst->seed_expand[x];
     __syncthreads();
     const uint32_t lane = threadIdx.x;
     const bool textmode = d.isa != nullptr && d.text != nullptr && d.sa_full != nullptr && cap >= 64;
     uint2 *bufA = sets + (size_t)blockIdx.x * 2 * cap;
     uint2 *bufB = bufA + cap;
+    if (nlist_dev) nlist = (uint32_t)*nlist_dev; // the group pass's count, without a host round trip
     for (;;) {
         uint32_t wi = 0;
         if (lane == 0) wi = (uint32_t)atomicAdd(&ctr->work, 1ull);
@@ -554,7 +619,68 @@ __global__ __launch_bounds__(64) void k_locate_wave(DevDir d, const DevConst *__
         const int sd = lookup_seed(d, s_digit, cst->lookup_depth, cst->lookup_radix, p, m, lo0, hi0);
         if (sd == 2) ns = 0;
         else if (sd == 1) start = (int)(m - cst->lookup_depth);
-        if (lane == 0) A[0] = make_uint2(lo0, hi0);
+        bool listed = false;
+        if (sd == 0 && seed_rec && m >= F) {
+            // the lookup cannot seed (ambiguity codes among the last codes): the
+            // set of q[m-F..m) is the union of the F-mer table entries of its
+            // expansions (up to five ambiguity codes, spread over the lanes),
+            // an interval shared by several expansions kept once
+            WideWin sw;
+            if (wide_window(p + m - F, F, s_expand, sw)) {
+                const uint32_t nc = 1u << (2 * sw.namb);
+                uint32_t total = 0;
+                bool over = false;
+                for (uint32_t x0 = 0; x0 < nc && !over; x0 += 64) {
+                    const uint32_t xi = x0 + lane;
+                    uint32_t ix = 0, c = 0;
+                    uint64_t o0 = 0;
+                    uint2 one = make_uint2(0u, 0u);
+                    if (xi < nc && wide_combo(sw, xi, ix)) {
+                        const uint4 R = seed_rec[ix];
+                        if (seed_rec_lsl(R.w) == F) {
+                            c = seed_rec_cnt(R.w);
+                            o0 = (uint64_t)R.x | ((uint64_t)R.y << 32);
+                            one = make_uint2(R.x, R.y);
+                        }
+                    }
+                    const uint32_t incl = wave_incl_scan(c), tot = __shfl(incl, 63, 64);
+                    if (total + tot > cap) {
+                        over = true;
+                    } else {
+                        const uint32_t at = total + incl - c;
+                        if (c == 1) A[at] = one;
+                        else
+                            for (uint32_t t = 0; t < c; t++) A[at + t] = make_uint2(seed_iv[o0 + t].x, seed_iv[o0 + t].y);
+                        total += tot;
+                    }
+                }
+                if (!over) {
+                    __syncthreads();
+                    uint32_t kept = total;
+                    if (sw.namb) {
+                        kept = 0;
+                        for (uint32_t base = 0; base < total; base += 64) {
+                            const uint32_t t = base + lane;
+                            bool keep = t < total;
+                            uint2 iv = make_uint2(0u, 0u);
+                            if (keep) {
+                                iv = A[t];
+                                for (uint32_t u = 0; u < t && keep; u++) keep = A[u].x != iv.x;
+                            }
+                            const uint32_t incl = wave_incl_scan(keep ? 1u : 0u);
+                            if (keep) B[kept + incl - 1] = iv;
+                            kept += __shfl(incl, 63, 64);
+                        }
+                        __syncthreads();
+                        uint2 *tmp = A; A = B; B = tmp;
+                    }
+                    ns = kept;
+                    start = (int)(m - F);
+                    listed = true;
+                }
+            }
+        }
+        if (!listed && lane == 0) A[0] = make_uint2(lo0, hi0);
         __syncthreads();
         bool too_big = false;
         // At most 64 rows with q[at..m) matched: every row is one text
@@ -571,7 +697,7 @@ __global__ __launch_bounds__(64) void k_locate_wave(DevDir d, const DevConst *__
             for (uint32_t u = 0; u < sz; u++) B[off + u] = make_uint2(iv.x + u, 0u);
             __syncthreads();
             uint32_t rlo = lane < rows ? B[lane].x : 0u, rhi = rlo + 1u, rns = lane < rows ? 1u : 0u, pr = 0;
-            if (rns) locate_finish_text(d, s_compat, p, at, rlo, rhi, rns, pr);
+            if (rns) locate_finish_text(d, s_compat, s_expand, p, at, rlo, rhi, rns, pr);
             const uint32_t pos = wave_incl_scan(rns);
             ns = __shfl(pos, 63, 64);
             if (rns) A[pos - 1] = make_uint2(rlo, rlo + 1u);
@@ -714,8 +840,13 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
     // slower and are not kept)
     if (nq && idx->fwd.view.n > (1u << 20) && (rc = build_seed_packed(idx, 12))) return rc;
     uint64_t h[8];
-    uint64_t nlist = nq;
-    const uint32_t *list = nullptr;
+    // register kernel -> group pass -> wave pass without host round trips
+    // between them (each takes its read count from the device counters), then
+    // one read-back of the counters
+    const uint32_t wgrid = (uint32_t)idx->dev->num_cus * 8;
+    uint32_t cap = 4096;
+    if ((rc = ws.wave_sets.ensure((size_t)wgrid * 2 * cap * 8))) return rc;
+    uint64_t fb_cap = ws.fb_ivs.cap / 8;
     if (nq) {
         if ((rc = event_record(idx, 0, false))) return rc;
         hipLaunchKernelGGL(k_locate_search, dim3((nq + 255) / 256), dim3(256), 0, s, d,
@@ -725,83 +856,83 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
         HSFM_HIP(hipGetLastError());
         if ((rc = event_record(idx, 0, true))) return rc;
         if ((rc = event_record(idx, 2, false))) return rc;
-        if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
-        nlist = h[0];
-        list = ws.ovf_list.as<uint32_t>();
-        idx->stats.occ_probes = h[5]; // the main kernel's own probes
-        idx->stats.fallback_reads = nlist;
-        if (nlist) {
-            if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_ovf), 2, 0))) return rc;
-            hipLaunchKernelGGL(k_locate_search_group<16>, dim3((uint32_t)((nlist + 15) / 16)), dim3(256), 0, s, d,
-                               idx->d_const.as<DevConst>(), d_q, d_qoff, list, (uint32_t)nlist,
-                               ws.cnt.as<uint32_t>(), ws.niv.as<uint8_t>(), ws.ivs.as<uint2>(),
-                               ws.fb_meta.as<uint2>(), ws.fb_ivs.as<uint2>(), (uint64_t)(ws.fb_ivs.cap / 8),
-                               ws.read_list.as<uint32_t>(), ctr, idx->seed_rec12.as<uint4>(),
-                               idx->seed_iv.as<uint4>(), idx->seed_rec12.p ? 12u : 0u);
-            HSFM_HIP(hipGetLastError());
-        }
+        hipLaunchKernelGGL(k_locate_search_group<16>, dim3(persist_grid(idx, k_locate_search_group<16>, 256)),
+                           dim3(256), 0, s, d, idx->d_const.as<DevConst>(), d_q, d_qoff,
+                           ws.ovf_list.as<uint32_t>(), nq, ws.cnt.as<uint32_t>(), ws.niv.as<uint8_t>(),
+                           ws.ivs.as<uint2>(), ws.fb_meta.as<uint2>(), ws.fb_ivs.as<uint2>(), fb_cap,
+                           ws.read_list.as<uint32_t>(), ctr, idx->seed_rec12.as<uint4>(), idx->seed_iv.as<uint4>(),
+                           idx->seed_rec12.p ? 12u : 0u);
+        HSFM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_locate_wave, dim3(wgrid), dim3(64), 0, s, d, idx->d_const.as<DevConst>(), d_q, d_qoff,
+                           ws.read_list.as<uint32_t>(), 0u, cap, ws.wave_sets.as<uint2>(), ws.cnt.as<uint32_t>(),
+                           ws.fb_meta.as<uint2>(), ws.fb_ivs.as<uint2>(), fb_cap, ws.ovf_list.as<uint32_t>(), ctr,
+                           &ctr->n_wave, idx->seed_rec12.as<uint4>(), idx->seed_iv.as<uint4>(),
+                           idx->seed_rec12.p ? 12u : 0u);
+        HSFM_HIP(hipGetLastError());
     }
     if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
-    nlist = nq ? h[0] : 0; // reads for the wave kernel, listed in read_list
-    list = ws.read_list.as<uint32_t>();
+    idx->stats.occ_probes = h[5]; // the main kernel's own probes
+    idx->stats.fallback_reads = h[0];
+    const uint64_t n_wave = h[6];
+    idx->stats.diag[0] = n_wave; // reads the workgroup-per-read pass takes
+    uint64_t nlist = h[3], used = h[2];
     if (nlist) {
-        if ((rc = ws.status.ensure(nlist * 4))) return rc;
-        HSFM_HIP(hipMemcpyAsync(ws.status.p, ws.read_list.p, nlist * 4, hipMemcpyDeviceToDevice, s));
-    }
-    const uint64_t n_wave = nlist;
-    uint32_t cap = 4096;
-    uint64_t fb_cap = h[2] + nlist * 256 + 4096;
-    uint32_t *retry = ws.ovf_list.as<uint32_t>();
-    while (nlist) {
-        // per-wave set space is 2 * cap intervals; keep the total under 4 GiB
-        uint64_t waves = nlist < (uint64_t)idx->dev->num_cus * 8 ? nlist : (uint64_t)idx->dev->num_cus * 8;
-        const uint64_t wave_limit = (4ull << 30) / (2ull * cap * 8);
-        if (waves > wave_limit) waves = wave_limit;
-        if (waves == 0) waves = 1;
-        if ((rc = ws.wave_sets.ensure((size_t)waves * 2 * cap * 8))) return rc;
-        if (ws.fb_ivs.cap < fb_cap * 8) {
-            // keep the intervals already published by earlier passes
-            DevBuf grown;
-            if ((rc = grown.ensure(fb_cap * 8))) return rc;
-            if (ws.fb_ivs.p) HSFM_HIP(hipMemcpyAsync(grown.p, ws.fb_ivs.p, ws.fb_ivs.cap, hipMemcpyDeviceToDevice, s));
-            HSFM_HIP(hipStreamSynchronize(s));
-            ws.fb_ivs.release();
-            ws.fb_ivs = grown;
+        // reads that outgrew their wave's set space or found the output list
+        // full (rare): keep the wave pass's read list for the row-order sort,
+        // grow both and re-run only those reads, until none is left
+        if ((rc = ws.status.ensure(n_wave * 4 + 4))) return rc;
+        HSFM_HIP(hipMemcpyAsync(ws.status.p, ws.read_list.p, n_wave * 4, hipMemcpyDeviceToDevice, s));
+        const uint32_t *list = ws.ovf_list.as<uint32_t>();
+        uint32_t *retry = ws.read_list.as<uint32_t>();
+        while (nlist) {
+            idx->stats.capacity_retries++;
+            if (cap >= (1u << 28))
+                return set_error(HSFM_ENOMEM, "interval set of a single read exceeds 2^28 intervals");
+            cap *= 4;
+            const uint64_t want = used + nlist * 1024;
+            const uint64_t grown_cap = fb_cap * 2 > want ? fb_cap * 2 : want;
+            // per-wave set space is 2 * cap intervals; keep the total under 4 GiB
+            uint64_t waves = nlist < (uint64_t)wgrid ? nlist : (uint64_t)wgrid;
+            const uint64_t wave_limit = (4ull << 30) / (2ull * cap * 8);
+            if (waves > wave_limit) waves = wave_limit;
+            if (waves == 0) waves = 1;
+            if ((rc = ws.wave_sets.ensure((size_t)waves * 2 * cap * 8))) return rc;
+            if (ws.fb_ivs.cap < grown_cap * 8) {
+                // keep the intervals already published by earlier passes
+                DevBuf grown;
+                if ((rc = grown.ensure(grown_cap * 8))) return rc;
+                if (ws.fb_ivs.p) HSFM_HIP(hipMemcpyAsync(grown.p, ws.fb_ivs.p, ws.fb_ivs.cap, hipMemcpyDeviceToDevice, s));
+                HSFM_HIP(hipStreamSynchronize(s));
+                ws.fb_ivs.release();
+                ws.fb_ivs = grown;
+            }
+            fb_cap = ws.fb_ivs.cap / 8;
+            if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->work), 2, 0))) return rc;
+            if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_retry), 2, 0))) return rc;
+            hipLaunchKernelGGL(k_locate_wave, dim3((uint32_t)waves), dim3(64), 0, s, d, idx->d_const.as<DevConst>(),
+                               d_q, d_qoff, list, (uint32_t)nlist, cap, ws.wave_sets.as<uint2>(),
+                               ws.cnt.as<uint32_t>(), ws.fb_meta.as<uint2>(), ws.fb_ivs.as<uint2>(), fb_cap, retry,
+                               ctr, nullptr, idx->seed_rec12.as<uint4>(), idx->seed_iv.as<uint4>(),
+                               idx->seed_rec12.p ? 12u : 0u);
+            HSFM_HIP(hipGetLastError());
+            if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
+            nlist = h[3];
+            used = h[2];
+            const uint32_t *next_list = retry;
+            retry = const_cast<uint32_t *>(list);
+            list = next_list;
         }
-        fb_cap = ws.fb_ivs.cap / 8;
-        if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->work), 2, 0))) return rc;
-        if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_retry), 2, 0))) return rc;
-        hipLaunchKernelGGL(k_locate_wave, dim3((uint32_t)waves), dim3(64), 0, s, d, idx->d_const.as<DevConst>(), d_q,
-                           d_qoff, list, (uint32_t)nlist, cap, ws.wave_sets.as<uint2>(), ws.cnt.as<uint32_t>(),
-                           ws.fb_meta.as<uint2>(), ws.fb_ivs.as<uint2>(), fb_cap, retry, ctr);
-        HSFM_HIP(hipGetLastError());
-        if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
-        const uint64_t nretry = h[3], used = h[2];
-        if (nretry == 0) break;
-        idx->stats.capacity_retries++;
-        // A retried read either outgrew its wave's set space or found the
-        // output list full: grow both and re-run only those reads.
-        if (cap >= (1u << 28)) return set_error(HSFM_ENOMEM, "interval set of a single read exceeds 2^28 intervals");
-        cap *= 4;
-        const uint64_t want = used + nretry * 1024;
-        fb_cap = fb_cap * 2 > want ? fb_cap * 2 : want;
-        const uint32_t *next_list = retry;
-        retry = const_cast<uint32_t *>(list);
-        list = next_list;
-        nlist = nretry;
     }
     if (n_wave) {
         // put every wave-path read's intervals in row order (FmIndex::locate order)
         const uint32_t nfb = (uint32_t)n_wave;
-        uint64_t hh[8];
-        if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), hh, 8))) return rc;
-        const uint64_t used = hh[2];
+        const uint32_t *wave_reads = idx->stats.capacity_retries ? ws.status.as<uint32_t>() : ws.read_list.as<uint32_t>();
         if ((rc = ws.fb_sets.ensure(used * 16 + (uint64_t)nfb * 8 + 64))) return rc;
         unsigned long long *k_in = ws.fb_sets.as<unsigned long long>();
         unsigned long long *k_out = k_in + used;
         uint32_t *seg_b = reinterpret_cast<uint32_t *>(k_out + used);
         uint32_t *seg_e = seg_b + nfb;
-        hipLaunchKernelGGL(k_fb_segments, dim3((nfb + 255) / 256), dim3(256), 0, s, ws.status.as<uint32_t>(), nfb,
+        hipLaunchKernelGGL(k_fb_segments, dim3((nfb + 255) / 256), dim3(256), 0, s, wave_reads, nfb,
                            ws.fb_meta.as<uint2>(), seg_b, seg_e);
         HSFM_HIP(hipGetLastError());
         const uint32_t kb = (uint32_t)((used + 255) / 256 < 65536 ? (used + 255) / 256 : 65536);

@@ -399,6 +399,19 @@ __global__ void k_count_value(const uint32_t *__restrict__ a, uint64_t n, uint32
     if (c) atomicAdd(out, (unsigned long long)c);
 }
 
+// DevDir::qnib16 from the alphabet's query-byte base sets (with text4 only)
+static void set_qnib16(const DevConst &hc, DirBuffers &db) {
+    db.view.qnib16 = 0;
+    if (!db.view.text4) return;
+    unsigned long long t16 = 0;
+    for (int b = 0; b < 256; b++) {
+        const uint32_t qn = (b >= 1 && b <= 4) ? 1u << (b - 1) : (uint32_t)(hc.seed_expand[b] & 15u);
+        if (b < 16) t16 |= (unsigned long long)qn << (4 * b);
+        else if (qn) return;
+    }
+    db.view.qnib16 = t16;
+}
+
 int ensure_sa_full(hsfm_index *idx) {
     DirBuffers &db = idx->fwd;
     const char *env = getenv("HSFM_SA_FULL"); // "0": keep the LF walks (for comparisons)
@@ -407,6 +420,7 @@ int ensure_sa_full(hsfm_index *idx) {
         db.view.text = nullptr;
         db.view.isa = nullptr;
         db.view.text4 = nullptr;
+        db.view.qnib16 = 0;
         return HSFM_OK;
     }
     if (db.sa_full_tried) {
@@ -414,6 +428,7 @@ int ensure_sa_full(hsfm_index *idx) {
         db.view.text = db.text.as<uint8_t>();
         db.view.isa = db.isa.as<uint32_t>();
         db.view.text4 = db.text4.as<uint32_t>();
+        set_qnib16(idx->h_const, db);
         return HSFM_OK;
     }
     db.sa_full_tried = true;
@@ -485,6 +500,7 @@ int ensure_sa_full(hsfm_index *idx) {
         db.view.text = db.text.as<uint8_t>();
         db.view.isa = db.isa.as<uint32_t>();
         db.view.text4 = db.text4.as<uint32_t>();
+        set_qnib16(hc, db);
     } else {
         db.text.release();
     }

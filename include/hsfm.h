@@ -264,7 +264,8 @@ typedef struct {
     double resolve_kernel_ms;  /* LF-walk resolve kernel (HIP events)             */
     uint64_t occ_probes;       /* 64 B records the main kernel's algorithm fetched */
     uint64_t diag[4];          /* MEM thread path: deferrals by seed union, by
-                                  extension, by record slots; plain restarts      */
+                                  extension, by record slots; plain restarts.
+                                  locate: [0] = reads of the workgroup pass       */
     double fallback_kernel_ms; /* kernels that re-ran the reads the main kernel
                                   deferred (larger sets)                          */
     uint64_t sa_full_rows;     /* rows of the device-built full suffix array that
