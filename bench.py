@@ -356,9 +356,10 @@ def read_window_keys(flat, off, k, limit):
 
 
 def uneven_leg(bidir, wl, args):
-    """Uneven-length reads through the batched pipeline: batches of 64k reads,
-    one HIP graph per batch, configuration picked by the autotuner on the
-    first call; timed calls reuse it."""
+    """Uneven-length reads through the batched pipeline: batches of 64k reads
+    handed to the callback in order, one HIP graph per window of batches
+    (one persistent kernel over the window's work items), configuration
+    picked by the autotuner on the first call; timed calls reuse it."""
     import torch
     import haystackfm_amd as hf
     flat, off = wl["uq"], wl["uo"]
@@ -396,6 +397,7 @@ def uneven_leg(bidir, wl, args):
             "value": round(nq / w, 1), "unit": "reads/s", "ms_per_pass": round(w * 1e3, 2),
             "one_shot_ms": round(w1 * 1e3, 2), "batched_over_one_shot": round(w / w1, 3),
             "graph_ms": round(stats["total_ms"], 2), "batches": stats["batches"],
+            "window_batches": stats["window_batches"], "windows": stats["windows"],
             "fallback_batches": stats["fallback_batches"], "records": stats["records"],
             "positions": stats["positions"],
             "tuned": {"main_kernel": ["thread/read ring 8", "thread/read ring 16", "thread/read ring 32",

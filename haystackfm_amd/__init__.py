@@ -250,13 +250,15 @@ class _SeedStats(C.Structure):
 
 class _BatchConfig(C.Structure):
     _fields_ = [("batch_reads", C.c_uint32), ("autotune", C.c_int), ("main_kernel", C.c_int),
-                ("resolve_fanout", C.c_int), ("use_graph", C.c_int), ("batch_sets", C.c_int)]
+                ("resolve_fanout", C.c_int), ("use_graph", C.c_int), ("batch_sets", C.c_int),
+                ("window_batches", C.c_uint32)]
 
 
 class _BatchStats(C.Structure):
     _fields_ = [("batches", C.c_uint32), ("fallback_batches", C.c_uint32), ("main_kernel", C.c_int),
                 ("resolve_fanout", C.c_int), ("records", C.c_uint64), ("positions", C.c_uint64),
-                ("total_ms", C.c_double), ("tune_total_ms", C.c_double), ("tune_ms", C.c_double * 6)]
+                ("total_ms", C.c_double), ("tune_total_ms", C.c_double), ("tune_ms", C.c_double * 6),
+                ("window_batches", C.c_uint32), ("windows", C.c_uint32)]
 
 
 _lib_obj = None
@@ -640,7 +642,8 @@ class BidirFmIndex:
 
     def find_mems_batched(self, queries, min_len: int, mode: int, ref_boundaries=(), max_hits_per_mem: int = 1024,
                           batch_reads: int = 65536, autotune: bool = False, main_kernel: int = -1,
-                          resolve_fanout: int = 0, use_graph: bool = True, batch_sets: int = 0):
+                          resolve_fanout: int = 0, use_graph: bool = True, batch_sets: int = 0,
+                          window_batches: int = 0):
         """The batched pipeline (one HIP graph per batch); same raw CSR as
         find_mems_batch, plus the pipeline statistics."""
         flat, off = _to_csr(queries)
@@ -677,7 +680,8 @@ class BidirFmIndex:
         cb = _BATCH_FN(on_batch)
         try:
             dq, do, dr = dalloc(flat), dalloc(off), dalloc(rb)
-            cfg = _BatchConfig(batch_reads, int(autotune), main_kernel, resolve_fanout, int(use_graph), batch_sets)
+            cfg = _BatchConfig(batch_reads, int(autotune), main_kernel, resolve_fanout, int(use_graph), batch_sets,
+                               window_batches)
             st = _BatchStats()
             _check(L.hsfm_find_mems_batched(h, dq, do, nq, int(min_len), mode, dr if len(rb) else None, len(rb),
                                             int(max_hits_per_mem), C.byref(cfg), cb, None, C.byref(st)),
@@ -707,10 +711,11 @@ class BidirFmIndex:
                                  d_ref_boundaries: int, n_refs: int, max_hits_per_mem: int = 1024,
                                  batch_reads: int = 65536, autotune: bool = False, main_kernel: int = -1,
                                  resolve_fanout: int = 0, use_graph: bool = True, on_batch=None,
-                                 batch_sets: int = 0) -> dict:
+                                 batch_sets: int = 0, window_batches: int = 0) -> dict:
         """Batched pipeline on device-resident reads; on_batch(first, n, result)
         receives each batch's device result (None: results are dropped)."""
-        cfg = _BatchConfig(batch_reads, int(autotune), main_kernel, resolve_fanout, int(use_graph), batch_sets)
+        cfg = _BatchConfig(batch_reads, int(autotune), main_kernel, resolve_fanout, int(use_graph), batch_sets,
+                           window_batches)
         st = _BatchStats()
         cb = _BATCH_FN(lambda _u, first, n, r: on_batch(first, n, r.contents)) if on_batch else _BATCH_FN()
         _check(_lib().hsfm_find_mems_batched(self._handle(), C.c_void_p(d_qflat), C.c_void_p(d_qoff), nq,

@@ -71,6 +71,11 @@ struct Workspace {
 // the same index return HSFM_EINVAL (HSFM_LOCK_INDEX) instead of deadlocking.
 extern thread_local const hsfm_index *tl_callback_index;
 
+// Pinned host words of an index (Workspace::h_pinned): small read-backs at
+// [0, 64), the batched pipeline's per-set read-back slots at 64 (k + 1) and
+// its per-set batch arguments at kPinnedArgs + 8 k (k < kMaxBatchSets).
+constexpr size_t kPinnedBytes = 16384;
+constexpr size_t kPinnedArgs = 1024;
 struct BatchBufs; // buffers of the batched MEM pipeline (hsfm_mem.hip)
 struct LsiView;   // learned-seed lookups inside the MEM kernel (hsfm_lsi_dev.h)
 // The lookup view of a learned seed index built on idx, for MEM restarts;
@@ -112,9 +117,9 @@ struct hsfm_index {
     // its buffers, kept across calls (grown when a call needs more): set 0 on
     // the device stream, further sets with streams of their own, so that
     // consecutive batches overlap
-    static constexpr int kMaxBatchSets = 4;
-    hsfm::BatchBufs *batch[kMaxBatchSets] = {nullptr, nullptr, nullptr, nullptr};
-    hipStream_t bstream[kMaxBatchSets] = {nullptr, nullptr, nullptr, nullptr};
+    static constexpr int kMaxBatchSets = 8;
+    hsfm::BatchBufs *batch[kMaxBatchSets] = {};
+    hipStream_t bstream[kMaxBatchSets] = {};
 };
 
 namespace hsfm {

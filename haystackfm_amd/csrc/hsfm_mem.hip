@@ -1453,7 +1453,7 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         if ((rc = exclusive_scan_u32_u64(idx, ws.item_cnt.as<uint32_t>(), ws.item_first.as<uint64_t>(), nq)))
             return rc;
         {   // item count and flat length in one round trip
-            if (!ws.h_pinned) HSFM_HIP(hipHostMalloc((void **)&ws.h_pinned, 4096, hipHostMallocDefault));
+            if (!ws.h_pinned) HSFM_HIP(hipHostMalloc((void **)&ws.h_pinned, kPinnedBytes, hipHostMallocDefault));
             HSFM_HIP(hipMemcpyAsync(ws.h_pinned, ws.item_first.as<uint64_t>() + nq, 8, hipMemcpyDeviceToHost, s));
             HSFM_HIP(hipMemcpyAsync(ws.h_pinned + 1, d_qoff + nq, 8, hipMemcpyDeviceToHost, s));
             HSFM_HIP(hipStreamSynchronize(s));
@@ -1602,7 +1602,7 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         // back with the pass counters in one round trip (redone after retries)
         if ((rc = exclusive_scan_u32_u64(idx, ws.mem_cnt.as<uint32_t>(), ws.mem_off.as<uint64_t>(), (uint32_t)W)))
             return rc;
-        if (!ws.h_pinned) HSFM_HIP(hipHostMalloc((void **)&ws.h_pinned, 4096, hipHostMallocDefault));
+        if (!ws.h_pinned) HSFM_HIP(hipHostMalloc((void **)&ws.h_pinned, kPinnedBytes, hipHostMallocDefault));
         HSFM_HIP(hipMemcpyAsync(ws.h_pinned, ctr, 64, hipMemcpyDeviceToHost, s));
         HSFM_HIP(hipMemcpyAsync(ws.h_pinned + 8, ws.mem_off.as<uint64_t>() + W, 8, hipMemcpyDeviceToHost, s));
         HSFM_HIP(hipStreamSynchronize(s));
@@ -1798,21 +1798,48 @@ __global__ void k_fill_u64(uint64_t *d, uint64_t n, uint64_t v) {
         d[i] = v;
 }
 
+struct BatchGraph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    void release() {
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+        x = nullptr;
+        g = nullptr;
+    }
+};
+
 struct BatchBufs {
     uint32_t B = 0, W = 0, seg = 0; // reads, work items (capacity), codes per item
+    uint32_t Bc = 0, Wb = 1;        // reads per callback batch, batches per window (B = Bc Wb)
     uint64_t slot_cap = 0, rec_cap = 0, hit_cap = 0, pos_cap = 0;
     DevBuf bq_off, slots, mem_cnt, status, pass_of, l1, l2, l3, ctr, recs, wsets;
     DevBuf mem_off, sorted, npos, pos_off, hits, rows, positions, misc, tmp;
     DevBuf items, item_cnt, item_first, read_off;
     DevBuf qnib; // all queries as base-set nibbles (read by every batch's graph)
     DevBuf bargs; // {first read, reads} of the batch, copied in before each launch
+    DevBuf sub_off, slices; // per-batch read offsets and {first hit}, {first position} of each batch of a window
+    DevBuf bcount;          // set-up: work items per callback batch and read offsets at batch starts
+    // the set's instantiated graph, kept across calls while everything it
+    // captured (pointers, sizes, configuration) is unchanged
+    BatchGraph graph;
+    std::vector<uint64_t> graph_key;
     const uint64_t *qoff_all = nullptr; // the call's read offsets (nq + 1)
     size_t tmp_bytes = 0;
     void release() {
         DevBuf *all[] = {&bq_off, &slots, &mem_cnt, &status, &pass_of, &l1, &l2, &l3, &ctr, &recs, &wsets,
                          &mem_off, &sorted, &npos, &pos_off, &hits, &rows, &positions, &misc, &tmp,
-                         &items, &item_cnt, &item_first, &read_off, &qnib, &bargs};
+                         &items, &item_cnt, &item_first, &read_off, &qnib, &bargs, &sub_off, &slices, &bcount};
+        graph.release();
+        graph_key.clear();
         for (DevBuf *b : all) b->release();
+    }
+    // every buffer address of the set (a graph captures them)
+    void addresses(std::vector<uint64_t> &k) const {
+        const DevBuf *all[] = {&bq_off, &slots, &mem_cnt, &status, &pass_of, &l1, &l2, &l3, &ctr, &recs, &wsets,
+                               &mem_off, &sorted, &npos, &pos_off, &hits, &rows, &positions, &misc, &tmp,
+                               &items, &item_cnt, &item_first, &read_off, &qnib, &bargs, &sub_off, &slices};
+        for (const DevBuf *b : all) k.push_back(reinterpret_cast<uint64_t>(b->p));
     }
 };
 void free_batch_bufs(BatchBufs *b) {
@@ -1843,6 +1870,86 @@ static int batch_scan(hsfm_index *idx, BatchBufs &bb, const uint32_t *in, uint64
     size_t tmp = bb.tmp_bytes;
     HSFM_HIP(hipcub::DeviceScan::InclusiveSum(bb.tmp.p, tmp, it, out + 1, (int)n, s));
     return HSFM_OK;
+}
+
+// Per callback batch of Bc reads: items[b] = its work items (reads of more
+// than seg codes split), bound[b] = the offset of its first read, bound[ncb]
+// = the total length; items zeroed by the caller.  Each thread sums a chunk
+// of consecutive reads (at most two batches when Bc >= the chunk).
+__global__ __launch_bounds__(256) void k_batch_counts(const uint64_t *__restrict__ qoff, uint32_t nq, uint32_t Bc,
+                                                      uint32_t seg, unsigned long long *__restrict__ items,
+                                                      uint64_t *__restrict__ bound, uint32_t ncb) {
+    constexpr uint64_t kChunk = 256;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * kChunk;
+    for (uint64_t c = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kChunk; c < nq; c += stride) {
+        uint32_t cur = 0xffffffffu;
+        unsigned long long acc = 0;
+        for (uint64_t r = c; r < c + kChunk && r < nq; r++) {
+            const uint32_t b = (uint32_t)(r / Bc);
+            if (b != cur) {
+                if (acc) atomicAdd(&items[cur], acc);
+                cur = b;
+                acc = 0;
+            }
+            const uint64_t m = qoff[r + 1] - qoff[r];
+            acc += m <= seg ? 1ull : (m + seg - 1) / seg;
+            if (r % Bc == 0) bound[b] = qoff[r];
+        }
+        if (acc) atomicAdd(&items[cur], acc);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) bound[ncb] = qoff[nq];
+}
+
+// A window of Wb batches (Bc reads each; the window's read count nb in
+// bargs[1]): out[j] = batch j's first hit, out[Wb + 1 + j] = its first
+// position (j = 0..Wb; batches past nb start at the window's end).
+constexpr uint32_t kMaxWindowBatches = 16;
+__global__ void k_batch_slices(const uint64_t *__restrict__ read_off, const uint64_t *__restrict__ pos_off,
+                               uint64_t hit_cap, const uint64_t *__restrict__ bargs, uint32_t Bc, uint32_t Wb,
+                               uint64_t *__restrict__ out) {
+    const uint32_t j = threadIdx.x;
+    if (j > Wb) return;
+    const uint64_t nb = bargs[1];
+    uint64_t r = (uint64_t)j * Bc;
+    r = r < nb ? r : nb;
+    uint64_t hs = read_off[r];
+    hs = hs < hit_cap ? hs : hit_cap;
+    out[j] = hs;
+    out[Wb + 1 + j] = pos_off[hs];
+}
+// Each batch's result relative to itself: read offsets from its first hit
+// (Bc + 1 per batch in sub_off) and position offsets of its hits from its
+// first position.
+__global__ __launch_bounds__(256) void k_batch_rebase(const uint64_t *__restrict__ read_off,
+                                                      const uint64_t *__restrict__ bargs, uint32_t Bc, uint32_t Wb,
+                                                      const uint64_t *__restrict__ slices,
+                                                      uint64_t *__restrict__ sub_off, hsfm_mem_hit *__restrict__ hits,
+                                                      const uint64_t *__restrict__ nhits_p, uint64_t hit_cap) {
+    __shared__ uint64_t s_hs[kMaxWindowBatches + 1], s_ps[kMaxWindowBatches + 1];
+    if (threadIdx.x <= Wb) {
+        s_hs[threadIdx.x] = slices[threadIdx.x];
+        s_ps[threadIdx.x] = slices[Wb + 1 + threadIdx.x];
+    }
+    __syncthreads();
+    const uint64_t nb = bargs[1];
+    uint64_t nh = *nhits_p;
+    nh = nh < hit_cap ? nh : hit_cap;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r <= nb; r += stride) {
+        if (r < nb) { // read r: entry r - j Bc of batch j
+            const uint64_t j = r / Bc;
+            sub_off[j * (Bc + 1) + (r - j * Bc)] = read_off[r] - s_hs[j];
+        }
+        if (r > 0 && (r % Bc == 0 || r == nb)) { // the closing entry of the batch that ends at r
+            const uint64_t j = (r - 1) / Bc;
+            sub_off[j * (Bc + 1) + (r - j * Bc)] = read_off[r] - s_hs[j];
+        }
+    }
+    for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += stride) {
+        uint32_t j = 0; // the last batch starting at or before hit h
+        for (uint32_t t = 1; t < Wb; t++) j = s_hs[t] <= h ? t : j;
+        hits[h].pos_offset -= s_ps[j];
+    }
 }
 
 // The launches of one batch (captured into a graph, or run eagerly).
@@ -1948,19 +2055,17 @@ static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, cons
                                      reinterpret_cast<const uint64_t *>(&misc->npos), d_refb, n_refs,
                                      bb.positions.as<uint32_t>(), &misc->guard, rpt, idx->dev->num_cus * 8)))
             return rc;
+    if (bb.Wb > 1) { // the window's batches, each relative to itself
+        hipLaunchKernelGGL(k_batch_slices, dim3(1), dim3(64), 0, s, bb.read_off.as<uint64_t>(),
+                           bb.pos_off.as<uint64_t>(), bb.hit_cap, bb.bargs.as<uint64_t>(), bb.Bc, bb.Wb,
+                           bb.slices.as<uint64_t>());
+        hipLaunchKernelGGL(k_batch_rebase, dim3(1024), dim3(256), 0, s, bb.read_off.as<uint64_t>(),
+                           bb.bargs.as<uint64_t>(), bb.Bc, bb.Wb, bb.slices.as<uint64_t>(), bb.sub_off.as<uint64_t>(),
+                           bb.hits.as<hsfm_mem_hit>(), mem_off + W, bb.hit_cap);
+        HSFM_HIP(hipGetLastError());
+    }
     return HSFM_OK;
 }
-
-struct BatchGraph {
-    hipGraph_t g = nullptr;
-    hipGraphExec_t x = nullptr;
-    void release() {
-        if (x) (void)hipGraphExecDestroy(x);
-        if (g) (void)hipGraphDestroy(g);
-        x = nullptr;
-        g = nullptr;
-    }
-};
 
 static int batch_capture(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, const SeedView &svp,
                          const uint8_t *d_q, const uint32_t *qnib, uint32_t min_len,
@@ -1981,6 +2086,38 @@ static int batch_capture(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, con
     return HSFM_OK;
 }
 
+// The set's graph for this call: the cached one when nothing it captured has
+// changed, else a new capture (replacing it).
+static int batch_graph(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, const SeedView &svp,
+                       const uint8_t *d_q, const uint32_t *qnib, uint32_t min_len, const uint32_t *d_refb,
+                       uint32_t n_refs, uint32_t max_hits, int main_kernel, int rpt, hipGraphExec_t &out) {
+    std::vector<uint64_t> key = {
+        reinterpret_cast<uint64_t>(d_q), reinterpret_cast<uint64_t>(qnib), reinterpret_cast<uint64_t>(d_refb),
+        reinterpret_cast<uint64_t>(bb.qoff_all), n_refs, min_len, max_hits, (uint64_t)main_kernel, (uint64_t)rpt,
+        bb.B, bb.W, bb.Bc, bb.Wb, bb.seg, bb.slot_cap, bb.rec_cap, bb.hit_cap, bb.pos_cap, bb.tmp_bytes,
+        reinterpret_cast<uint64_t>(sv.off), reinterpret_cast<uint64_t>(sv.iv), reinterpret_cast<uint64_t>(sv.lsl), sv.F,
+        reinterpret_cast<uint64_t>(svp.rec), reinterpret_cast<uint64_t>(svp.off), reinterpret_cast<uint64_t>(svp.iv),
+        svp.F, reinterpret_cast<uint64_t>(idx->d_const.p)};
+    for (const DevDir *d : {&idx->fwd.view, &idx->rev.view})
+        key.insert(key.end(), {reinterpret_cast<uint64_t>(d->occ), reinterpret_cast<uint64_t>(d->sb),
+                               reinterpret_cast<uint64_t>(d->samp), reinterpret_cast<uint64_t>(d->sa_vals),
+                               reinterpret_cast<uint64_t>(d->lookup), d->n, reinterpret_cast<uint64_t>(d->sa_full),
+                               reinterpret_cast<uint64_t>(d->text), reinterpret_cast<uint64_t>(d->isa),
+                               reinterpret_cast<uint64_t>(d->text4), d->qnib16, d->num_samples, d->sample_rate,
+                               d->num_lanes, d->lookup_len});
+    bb.addresses(key);
+    if (!bb.graph.x || key != bb.graph_key) {
+        bb.graph.release();
+        bb.graph_key.clear();
+        int rc = batch_capture(idx, bb, sv, svp, d_q, qnib, min_len, d_refb, n_refs, max_hits, main_kernel, rpt,
+                               bb.graph);
+        if (rc) return rc;
+        bb.graph_key = key;
+    }
+    out = bb.graph.x;
+    return HSFM_OK;
+}
+
 // Device buffers of one batch set for the capacities in bb (kept and grown).
 static int batch_alloc(BatchBufs &bb) {
     const size_t Wn = bb.W;
@@ -1998,7 +2135,8 @@ static int batch_alloc(BatchBufs &bb) {
         (rc = bb.npos.ensure(bb.hit_cap * 4 + 4)) || (rc = bb.pos_off.ensure((bb.hit_cap + 1) * 8)) ||
         (rc = bb.hits.ensure(bb.hit_cap * sizeof(hsfm_mem_hit))) || (rc = bb.rows.ensure(bb.pos_cap * 4)) ||
         (rc = bb.positions.ensure(bb.pos_cap * 8)) || (rc = bb.misc.ensure(sizeof(BatchMisc))) ||
-        (rc = bb.tmp.ensure(bb.tmp_bytes + 256)))
+        (rc = bb.tmp.ensure(bb.tmp_bytes + 256)) || (rc = bb.sub_off.ensure(((size_t)B + bb.Wb + 1) * 8)) ||
+        (rc = bb.slices.ensure((2ull * bb.Wb + 2) * 8 + 16)))
         return rc;
     return HSFM_OK;
 }
@@ -2007,8 +2145,8 @@ static int batch_alloc(BatchBufs &bb) {
 // offsets and work items itself).  The pinned slot is rewritten only after
 // the previous batch's wait, so the copy has run by then.
 static int batch_set_args(hsfm_index *idx, BatchBufs &bb, uint32_t r0, uint32_t nb, hipStream_t s, int k) {
-    if (!idx->ws.h_pinned) HSFM_HIP(hipHostMalloc((void **)&idx->ws.h_pinned, 4096, hipHostMallocDefault));
-    uint64_t *a = idx->ws.h_pinned + 384 + 8 * k; // this set's slot (read-back slots: 64 per set below 320)
+    if (!idx->ws.h_pinned) HSFM_HIP(hipHostMalloc((void **)&idx->ws.h_pinned, kPinnedBytes, hipHostMallocDefault));
+    uint64_t *a = idx->ws.h_pinned + kPinnedArgs + 8 * k; // this set's batch arguments
     a[0] = r0;
     a[1] = nb;
     HSFM_HIP(hipMemcpyAsync(bb.bargs.p, a, 16, hipMemcpyHostToDevice, s));
@@ -2041,40 +2179,72 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     uint32_t FP = mem_seed_length(idx, min_len);
     if ((rc = packed_seeds(idx, FP))) return rc;
     const SeedView svp = packed_view(idx, FP, sv);
-    // capacities from the host copy of the offsets
-    std::vector<uint64_t> hoff((size_t)nq + 1);
-    HSFM_HIP(hipMemcpyAsync(hoff.data(), d_qoff, ((size_t)nq + 1) * 8, hipMemcpyDeviceToHost, s));
-    HSFM_HIP(hipStreamSynchronize(s));
-    const uint32_t B = cfg->batch_reads < nq ? cfg->batch_reads : (nq ? nq : 1);
+    // capacities from per-batch counts computed on the device (work items of
+    // every callback batch, read offsets at batch starts)
+    const uint32_t Bc = cfg->batch_reads < nq ? cfg->batch_reads : (nq ? nq : 1);
+    const uint32_t ncb = nq ? (nq + Bc - 1) / Bc : 0;
+    const uint32_t seg0 = mem_segment_length() ? mem_segment_length() : 0xffffffffu;
     if (!idx->batch[0]) idx->batch[0] = new BatchBufs();
     BatchBufs &bb = *idx->batch[0]; // kept across calls: ensure() below only grows it
+    std::vector<uint64_t> bitems(ncb), bound((size_t)ncb + 1, 0);
+    if (ncb) {
+        if ((rc = bb.bcount.ensure(((size_t)2 * ncb + 1) * 8))) return rc;
+        unsigned long long *d_items = bb.bcount.as<unsigned long long>();
+        uint64_t *d_bound = bb.bcount.as<uint64_t>() + ncb;
+        HSFM_HIP(hipMemsetAsync(d_items, 0, (size_t)ncb * 8, s));
+        const uint64_t chunks = ((uint64_t)nq + 255) / 256;
+        const uint32_t grid = (uint32_t)((chunks + 255) / 256 < 4096 ? (chunks + 255) / 256 : 4096);
+        hipLaunchKernelGGL(k_batch_counts, dim3(grid ? grid : 1), dim3(256), 0, s, d_qoff, nq, Bc, seg0, d_items,
+                           d_bound, ncb);
+        HSFM_HIP(hipGetLastError());
+        HSFM_HIP(hipMemcpyAsync(bitems.data(), d_items, (size_t)ncb * 8, hipMemcpyDeviceToHost, s));
+        HSFM_HIP(hipMemcpyAsync(bound.data(), d_bound, ((size_t)ncb + 1) * 8, hipMemcpyDeviceToHost, s));
+        HSFM_HIP(hipStreamSynchronize(s));
+    }
+    // callback batches of Bc reads; graphs over windows of Wb batches, so that
+    // the persistent kernel has about eight work items per lane (a 64k-read
+    // batch alone leaves most lanes idle while its last items finish; 2M
+    // uneven reads: 38 ms with one batch per graph, 18 ms with 11, 17 ms with
+    // 16, against 10 ms for one call over all reads)
+    uint32_t Wb = cfg->window_batches;
+    if (!Wb) {
+        uint64_t items = 0;
+        for (uint32_t b = 0; b < ncb; b++) items += bitems[b];
+        const double per_read = nq ? (double)items / nq : 1.0;
+        const double lanes = (double)persist_grid(idx, k_mem_persist<16, 256>, 256) * 256.0;
+        Wb = (uint32_t)ceil(8.0 * lanes / (per_read * (double)Bc));
+    }
+    Wb = Wb < 1 ? 1 : (Wb > kMaxWindowBatches ? kMaxWindowBatches : Wb);
+    if (Wb > ncb) Wb = ncb ? ncb : 1;
+    if ((uint64_t)Bc * Wb > 0x7fffffffull) Wb = (uint32_t)(0x7fffffffull / Bc) ? (uint32_t)(0x7fffffffull / Bc) : 1;
+    const uint32_t B = (uint64_t)Bc * Wb < nq ? Bc * Wb : (nq ? nq : 1);
+    stats.window_batches = Wb;
     bb.B = B;
+    bb.Bc = Bc;
+    bb.Wb = Wb;
     bb.qoff_all = d_qoff;
-    if ((rc = pack_bases(idx, d_q, hoff[nq], bb.qnib))) {
+    if ((rc = pack_bases(idx, d_q, bound[ncb], bb.qnib))) {
         bb.release();
         return rc;
     }
-    bb.seg = mem_segment_length();
-    if (!bb.seg) bb.seg = 0xffffffffu; // one item per read
-    // item capacity and slot ranges over all batches (slot index of item w:
+    bb.seg = seg0;
+    // item capacity and slot ranges over all windows (slot index of item w:
     // ((qoff + start) >> 3) + 8 w)
     uint64_t max_w = 0, max_slots = 0, max_batch_slots = 0;
-    for (uint32_t r0 = 0; r0 < nq; r0 += B) {
-        const uint32_t r1 = r0 + B < nq ? r0 + B : nq;
+    for (uint32_t b0 = 0; b0 < ncb; b0 += Wb) {
+        const uint32_t b1 = b0 + Wb < ncb ? b0 + Wb : ncb;
+        const uint32_t r0 = b0 * Bc, r1 = b1 * Bc < nq ? b1 * Bc : nq;
         uint64_t w = B - (r1 - r0); // padding reads: one item each
-        for (uint32_t r = r0; r < r1; r++) {
-            const uint64_t m = hoff[r + 1] - hoff[r];
-            w += m <= bb.seg ? 1 : (m + bb.seg - 1) / bb.seg;
-        }
+        for (uint32_t b = b0; b < b1; b++) w += bitems[b];
         max_w = w > max_w ? w : max_w;
     }
     if (max_w > 0x7fffffffull) return set_error(HSFM_EINVAL, "too many read segments in a batch");
-    bb.W = (uint32_t)max_w;
-    for (uint32_t r0 = 0; r0 < nq; r0 += B) {
-        const uint32_t r1 = r0 + B < nq ? r0 + B : nq;
-        const uint64_t sl = (hoff[r1] >> 3) + 8ull * bb.W + 16;
+    bb.W = (uint32_t)(max_w ? max_w : 1);
+    for (uint32_t b0 = 0; b0 < ncb; b0 += Wb) {
+        const uint32_t b1 = b0 + Wb < ncb ? b0 + Wb : ncb;
+        const uint64_t sl = (bound[b1] >> 3) + 8ull * bb.W + 16;
         max_slots = sl > max_slots ? sl : max_slots;
-        const uint64_t v = (hoff[r1] >> 3) - (hoff[r0] >> 3) + 9ull * bb.W + 8;
+        const uint64_t v = (bound[b1] >> 3) - (bound[b0] >> 3) + 9ull * bb.W + 8;
         max_batch_slots = v > max_batch_slots ? v : max_batch_slots;
     }
     bb.slot_cap = max_slots;
@@ -2140,8 +2310,13 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
                 double ms;
                 if ((rc = time_cfg(mk, 1, ms))) return cleanup(rc);
                 stats.tune_ms[mk] = ms;
-                if (ms < best) { best = ms; bm = mk; }
             }
+            // the 16-interval ring unless another main kernel is clearly faster
+            // (the ring of 8 defers more items, whose passes then contend with
+            // the next window's main kernel)
+            best = stats.tune_ms[BM_PERSIST16];
+            for (int mk = 0; mk < BM_COUNT; mk++)
+                if (stats.tune_ms[mk] < 0.95 * best) { best = stats.tune_ms[mk]; bm = mk; }
             int br = 1;
             const int rps[2] = {2, 4};
             for (int i = 0; i < 2; i++) {
@@ -2177,9 +2352,11 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     // hands batch b to the callback.  Batches complete and reach the callback
     // in order.
     constexpr int KS = hsfm_index::kMaxBatchSets;
-    BatchGraph gr[KS];
-    BatchBufs *set[KS] = {&bb, nullptr, nullptr, nullptr};
-    hipStream_t sst[KS] = {s, nullptr, nullptr, nullptr};
+    hipGraphExec_t gx[KS] = {};
+    BatchBufs *set[KS] = {};
+    hipStream_t sst[KS] = {};
+    set[0] = &bb;
+    sst[0] = s;
     const uint32_t nbatches = nq ? (nq + B - 1) / B : 0;
     int nsets = 1;
     if (cfg->use_graph && nbatches > 1) {
@@ -2190,34 +2367,24 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     // (each batch keeps the whole persistent grid: a grid split between the
     // batches in flight measured slower, 2M uneven reads: 54.5 ms with three
     // thirds against 47 ms with three whole grids)
-    auto release_graphs = [&]() {
-        for (BatchGraph &g : gr) g.release();
-    };
-    if (cfg->use_graph && (rc = batch_capture(idx, bb, sv, svp, d_q, bb.qnib.as<uint32_t>(), min_len, d_refb, n_refs,
-                                              max_hits, main_kernel, rpt, gr[0])))
+    if (cfg->use_graph && (rc = batch_graph(idx, bb, sv, svp, d_q, bb.qnib.as<uint32_t>(), min_len, d_refb, n_refs,
+                                            max_hits, main_kernel, rpt, gx[0])))
         return cleanup(rc);
     for (int k = 1; k < nsets; k++) {
         if (!idx->batch[k]) idx->batch[k] = new BatchBufs();
         BatchBufs &bk = *idx->batch[k];
-        bk.B = bb.B; bk.W = bb.W; bk.seg = bb.seg;
+        bk.B = bb.B; bk.W = bb.W; bk.seg = bb.seg; bk.Bc = bb.Bc; bk.Wb = bb.Wb;
         bk.slot_cap = bb.slot_cap; bk.rec_cap = bb.rec_cap; bk.hit_cap = bb.hit_cap; bk.pos_cap = bb.pos_cap;
         bk.tmp_bytes = bb.tmp_bytes;
         bk.qoff_all = d_qoff;
-        if ((rc = batch_alloc(bk))) {
-            release_graphs();
-            return cleanup(rc);
-        }
-        if (!idx->bstream[k] && hipStreamCreateWithFlags(&idx->bstream[k], hipStreamNonBlocking) != hipSuccess) {
-            release_graphs();
+        if ((rc = batch_alloc(bk))) return cleanup(rc);
+        if (!idx->bstream[k] && hipStreamCreateWithFlags(&idx->bstream[k], hipStreamNonBlocking) != hipSuccess)
             return cleanup(set_error(HSFM_EDEVICE, "hipStreamCreate failed"));
-        }
         set[k] = &bk;
         sst[k] = idx->bstream[k];
-        if ((rc = batch_capture(idx, bk, sv, svp, d_q, bb.qnib.as<uint32_t>(), min_len, d_refb, n_refs, max_hits,
-                                main_kernel, rpt, gr[k]))) {
-            release_graphs();
+        if ((rc = batch_graph(idx, bk, sv, svp, d_q, bb.qnib.as<uint32_t>(), min_len, d_refb, n_refs, max_hits,
+                              main_kernel, rpt, gx[k])))
             return cleanup(rc);
-        }
     }
     HSFM_HIP(hipEventRecord(e0, s));
     for (int k = 1; k < nsets; k++) HSFM_HIP(hipStreamWaitEvent(sst[k], e0, 0)); // after the set-up on s
@@ -2233,43 +2400,58 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
         const hipStream_t st = sst[pd.k];
         MemCounters *c = b.ctr.as<MemCounters>();
         const uint64_t *m = b.misc.as<uint64_t>();
-        uint64_t h[8];
+        uint64_t h[8 + 2 * (kMaxWindowBatches + 1)];
         uint64_t *hp = idx->ws.h_pinned + 64 * (pd.k + 1); // this set's read-back slots
+        const size_t nsl = b.Wb > 1 ? 2 * ((size_t)b.Wb + 1) : 0;
         if (hipMemcpyAsync(hp, &c[2].n_retry, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipMemcpyAsync(hp + 1, m, 24, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipMemcpyAsync(hp + 4, b.mem_off.as<uint64_t>() + b.W, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            (nsl && hipMemcpyAsync(hp + 8, b.slices.p, nsl * 8, hipMemcpyDeviceToHost, st) != hipSuccess) ||
             hipStreamSynchronize(st) != hipSuccess)
             return set_error(HSFM_EDEVICE, "batch result read-back failed");
-        memcpy(h, hp, 5 * 8);
+        memcpy(h, hp, (8 + nsl) * 8);
         const uint64_t late = h[0], pos_ovf = h[1], npos = h[2], guard = h[3], nhits = h[4];
         if (guard)
             return set_error(HSFM_EINDEX, "%llu LF walks exceeded the sample rate: index is inconsistent",
                              (unsigned long long)guard);
-        stats.batches++;
-        hsfm_mem_device_result res{};
-        if (late || pos_ovf || nhits > b.hit_cap) {
-            // outgrew the in-graph capacities: this batch through the host-driven path
-            stats.fallback_batches++;
-            // (with the index lock still held: its result lives in the index's
-            // workspace until the callback below has read it)
-            const int r2 = mem_device_impl(idx, d_q, d_qoff + pd.r0, pd.nb, min_len, d_refb, n_refs, max_hits, &res);
-            if (r2) return r2;
-        } else {
-            res.num_queries = pd.nb;
-            res.num_hits = nhits;
-            res.num_positions = n_refs ? npos : 0;
-            res.d_hit_offsets = b.read_off.as<uint64_t>();
-            res.d_hits = b.hits.as<hsfm_mem_hit>();
-            res.d_positions = b.positions.as<uint32_t>();
-        }
-        stats.records += res.num_hits;
-        stats.positions += res.num_positions;
-        if (fn) {
-            // the lock stays held: calls on this index from fn return HSFM_EINVAL
-            const hsfm_index *outer = tl_callback_index;
-            tl_callback_index = idx;
-            fn(user, pd.r0, pd.nb, &res);
-            tl_callback_index = outer;
+        stats.windows++;
+        const bool redo = late || pos_ovf || nhits > b.hit_cap;
+        for (uint32_t j = 0; j * b.Bc < pd.nb; j++) {
+            const uint32_t r0 = pd.r0 + j * b.Bc, nb = pd.nb - j * b.Bc < b.Bc ? pd.nb - j * b.Bc : b.Bc;
+            stats.batches++;
+            hsfm_mem_device_result res{};
+            if (redo) {
+                // outgrew the in-graph capacities: this batch through the host-driven path
+                stats.fallback_batches++;
+                // (with the index lock still held: its result lives in the index's
+                // workspace until the callback below has read it)
+                const int r2 = mem_device_impl(idx, d_q, d_qoff + r0, nb, min_len, d_refb, n_refs, max_hits, &res);
+                if (r2) return r2;
+            } else if (b.Wb > 1) {
+                const uint64_t *hs = h + 8, *ps = h + 8 + b.Wb + 1;
+                res.num_queries = nb;
+                res.num_hits = hs[j + 1] - hs[j];
+                res.num_positions = n_refs ? ps[j + 1] - ps[j] : 0;
+                res.d_hit_offsets = b.sub_off.as<uint64_t>() + (size_t)j * (b.Bc + 1);
+                res.d_hits = b.hits.as<hsfm_mem_hit>() + hs[j];
+                res.d_positions = b.positions.as<uint32_t>() + 2 * ps[j];
+            } else {
+                res.num_queries = nb;
+                res.num_hits = nhits;
+                res.num_positions = n_refs ? npos : 0;
+                res.d_hit_offsets = b.read_off.as<uint64_t>();
+                res.d_hits = b.hits.as<hsfm_mem_hit>();
+                res.d_positions = b.positions.as<uint32_t>();
+            }
+            stats.records += res.num_hits;
+            stats.positions += res.num_positions;
+            if (fn) {
+                // the lock stays held: calls on this index from fn return HSFM_EINVAL
+                const hsfm_index *outer = tl_callback_index;
+                tl_callback_index = idx;
+                fn(user, r0, nb, &res);
+                tl_callback_index = outer;
+            }
         }
         return HSFM_OK;
     };
@@ -2285,7 +2467,7 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
         }
         if ((rc = batch_set_args(idx, *set[k], r0, nb, sst[k], k))) break;
         if (cfg->use_graph) {
-            if (hipGraphLaunch(gr[k].x, sst[k]) != hipSuccess) {
+            if (hipGraphLaunch(gx[k], sst[k]) != hipSuccess) {
                 rc = set_error(HSFM_EDEVICE, "hipGraphLaunch failed");
                 break;
             }
@@ -2302,7 +2484,6 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     float t = 0.f;
     HSFM_HIP(hipEventElapsedTime(&t, e0, e1));
     stats.total_ms = t;
-    release_graphs();
     if (st) *st = stats;
     return cleanup(rc);
 }

@@ -102,7 +102,7 @@ int exclusive_scan_u32_u64(hsfm_index *idx, const uint32_t *d_in, uint64_t *d_ou
 
 int readback_u64(hsfm_index *idx, const uint64_t *d_src, uint64_t *h_dst, int count) {
     hipStream_t s = idx->dev->stream;
-    if (!idx->ws.h_pinned) HSFM_HIP(hipHostMalloc((void **)&idx->ws.h_pinned, 4096, hipHostMallocDefault));
+    if (!idx->ws.h_pinned) HSFM_HIP(hipHostMalloc((void **)&idx->ws.h_pinned, kPinnedBytes, hipHostMallocDefault));
     HSFM_HIP(hipMemcpyAsync(idx->ws.h_pinned, d_src, (size_t)count * 8, hipMemcpyDeviceToHost, s));
     HSFM_HIP(hipStreamSynchronize(s));
     memcpy(h_dst, idx->ws.h_pinned, (size_t)count * 8);

@@ -192,11 +192,14 @@ int hsfm_find_mems_device(hsfm_index *idx, const uint8_t *d_qflat, const uint64_
                           hsfm_mem_device_result *out);
 
 /* ---- batched MEM pipeline (uneven reads) ----------------------------------
- * The same results as hsfm_find_mems_device, produced batch by batch:
- * reads [r0, r0 + batch_reads) run through one captured HIP graph (find,
- * second-level passes, positions, resolve, reference map) and fn receives the
- * batch's device-resident result (hit offsets relative to the batch, valid
- * only during the call).  Reference analogue: the query chunking of
+ * The same results as hsfm_find_mems_device, produced batch by batch: fn
+ * receives the device-resident result of reads [r0, r0 + batch_reads), in
+ * order (hit offsets and position offsets relative to the batch, valid only
+ * during the call).  A window of window_batches consecutive batches runs
+ * through one captured HIP graph (find, second-level passes, positions,
+ * resolve, reference map): its persistent kernel takes the work items of
+ * every batch of the window from one device queue, so that a batch's last
+ * items do not leave the GPU idle.  Reference analogue: the query chunking of
  * resolve_mem_hits_gpu (bidir_index.rs:418-439) under output_budget_u32
  * (gpu/mod.rs:68-82).  Needs min_len > 12 (seed-table restarts). */
 typedef struct {
@@ -207,8 +210,11 @@ typedef struct {
     int resolve_fanout;    /* 0 = tuned/cached/default; else 1, 2 or 4 walks
                               per thread                                       */
     int use_graph;         /* capture each batch into one HIP graph            */
-    int batch_sets;        /* batches in flight, each with its own buffers,
-                              graph and stream (0 = 3, at most 4)            */
+    int batch_sets;        /* windows in flight, each with its own buffers,
+                              graph and stream (0 = 3, at most 8)            */
+    uint32_t window_batches; /* batches per graph (0: enough work items for
+                              about four per lane of the persistent grid,
+                              at most 16)                                     */
 } hsfm_batch_config;
 typedef struct {
     uint32_t batches, fallback_batches; /* batches, those re-run host-driven   */
@@ -216,8 +222,9 @@ typedef struct {
     uint64_t records, positions;
     double total_ms;                    /* HIP events around the batch loop    */
     double tune_total_ms;
-    double tune_ms[6];  /* batch-0 time of main kernels 0..3 (fan-out 1), then
+    double tune_ms[6];  /* window-0 time of main kernels 0..3 (fan-out 1), then
                            the best main kernel with fan-out 2 and 4           */
+    uint32_t window_batches, windows; /* batches per graph, graphs launched   */
 } hsfm_batch_stats;
 /* The index stays locked while fn runs: an entry point called from fn on the
  * same index (hsfm_index_free included) returns HSFM_EINVAL without doing

@@ -1,5 +1,7 @@
 """GPU checks of the batched (graph-captured) MEM pipeline for uneven reads:
-every configuration returns exactly what the one-shot entry point returns."""
+every configuration -- main kernel, resolve fan-out, graphs or eager
+launches, windows of one or more batches per graph, windows in flight --
+returns exactly what the one-shot entry point returns, batch by batch."""
 import numpy as np
 import pytest
 
@@ -27,28 +29,31 @@ def _same(a, b):
     assert np.array_equal(a[2], b[2])
 
 
+@pytest.mark.parametrize("window", [1, 2, 0])
 @pytest.mark.parametrize("main,fan,graph", [(1, 1, True), (0, 2, True), (2, 4, True), (3, 1, True), (1, 2, False)])
-def test_batched_matches_one_shot(uneven, main, fan, graph):
+def test_batched_matches_one_shot(uneven, main, fan, graph, window):
     o, g, (flat, off), bounds = uneven
     ref = g.find_mems_batch((flat, off), 19, hf.HSFM_MODE_SMEM, bounds, 1024)
     got = g.find_mems_batched((flat, off), 19, hf.HSFM_MODE_SMEM, bounds, 1024, batch_reads=700,
-                              main_kernel=main, resolve_fanout=fan, use_graph=graph)
+                              main_kernel=main, resolve_fanout=fan, use_graph=graph, window_batches=window)
     _same(got[:3], ref)
     st = got[3]
     assert st["batches"] == 5 and st["main_kernel"] == main and st["resolve_fanout"] == fan
     assert st["records"] == len(ref[1]) and st["positions"] == len(ref[2])
+    # windows of 1, 2 and (default) all 5 batches: 5, 3 and 1 graphs
+    assert st["window_batches"] == (window or 5) and st["windows"] == {1: 5, 2: 3, 0: 1}[window]
 
 
-@pytest.mark.parametrize("sets", [1, 2, 3, 4])
+@pytest.mark.parametrize("sets", [1, 2, 3, 4, 8])
 def test_batches_in_flight(uneven, sets):
-    """1 to 4 batches in flight (their own buffers, graphs and streams): the
-    same results, handed over in batch order."""
+    """1 to 8 windows of two batches in flight (their own buffers, graphs and
+    streams): the same results, handed over in batch order."""
     o, g, (flat, off), bounds = uneven
     ref = g.find_mems_batch((flat, off), 19, hf.HSFM_MODE_SMEM, bounds, 1024)
     got = g.find_mems_batched((flat, off), 19, hf.HSFM_MODE_SMEM, bounds, 1024, batch_reads=450,
-                              main_kernel=1, resolve_fanout=2, batch_sets=sets)
+                              main_kernel=1, resolve_fanout=2, batch_sets=sets, window_batches=2)
     _same(got[:3], ref)
-    assert got[3]["batches"] == 7
+    assert got[3]["batches"] == 7 and got[3]["windows"] == 4
 
 
 def test_batched_matches_oracle_sample(uneven):
@@ -85,9 +90,10 @@ def test_oversized_batch_runs_host_driven():
     g = hf.BidirFmIndex.from_bytes(o.to_bytes())
     qs = [random_text(rng, 60, iupac_frac=0.01) for _ in range(20)] + [np.full(40, 5, np.uint8)]
     ref = g.find_mems_batch(qs, 13, hf.HSFM_MODE_SMEM, [], 16)
-    got = g.find_mems_batched(qs, 13, hf.HSFM_MODE_SMEM, [], 16, batch_reads=8)
-    _same(got[:3], ref)
-    assert got[3]["fallback_batches"] >= 1
+    for window in (1, 0):
+        got = g.find_mems_batched(qs, 13, hf.HSFM_MODE_SMEM, [], 16, batch_reads=8, window_batches=window)
+        _same(got[:3], ref)
+        assert got[3]["fallback_batches"] >= 1
 
 
 @pytest.mark.parametrize("seed_f", ["14", "15"])
