@@ -199,12 +199,14 @@ int hsfm_find_mems_device(hsfm_index *idx, const uint8_t *d_qflat, const uint64_
  * through one captured HIP graph (find, second-level passes, positions,
  * resolve, reference map): its persistent kernel takes the work items of
  * every batch of the window from one device queue, so that a batch's last
- * items do not leave the GPU idle.  Reference analogue: the query chunking of
+ * items do not leave the GPU idle.  Each buffer set keeps its instantiated
+ * graph and reuses it on later calls while the captured buffers, index views
+ * and configuration are unchanged.  Reference analogue: the query chunking of
  * resolve_mem_hits_gpu (bidir_index.rs:418-439) under output_budget_u32
  * (gpu/mod.rs:68-82).  Needs min_len > 12 (seed-table restarts). */
 typedef struct {
     uint32_t batch_reads;  /* reads per batch (65536)                          */
-    int autotune;          /* time the candidate configurations on batch 0     */
+    int autotune;          /* time the candidate configurations on window 0    */
     int main_kernel;       /* -1 = tuned/cached/default; 0..2 thread per read
                               with 8/16/32-interval LDS sets, 3 wave per read  */
     int resolve_fanout;    /* 0 = tuned/cached/default; else 1, 2 or 4 walks
@@ -213,7 +215,7 @@ typedef struct {
     int batch_sets;        /* windows in flight, each with its own buffers,
                               graph and stream (0 = 3, at most 8)            */
     uint32_t window_batches; /* batches per graph (0: enough work items for
-                              about four per lane of the persistent grid,
+                              about eight per lane of the persistent grid,
                               at most 16)                                     */
 } hsfm_batch_config;
 typedef struct {
