@@ -257,20 +257,30 @@ def host_info(threads):
 
 
 def random_record_ceiling(achieved_gbps):
-    """The measured rate of random dependent 64-byte record loads on this GPU
-    (committed calibration run) and the fraction of it `achieved_gbps` reaches."""
+    """The measured rate of random dependent loads in the SMEM kernel's access
+    mix on this GPU (committed calibration run: 64-byte records within 1 GiB,
+    every sixth load 16 bytes anywhere in a 16 GiB table) and the fraction of it
+    `achieved_gbps` reaches; the 4 GiB record rate beside it."""
     path = os.path.join(ROOT, "profiles", "random_access_ceiling.json")
     try:
         with open(path) as f:
             rows = json.load(f)["rows"]
     except (OSError, ValueError, KeyError):
         return None
+    mix = big = None
     for r in rows:
-        if r.get("mode") == "64B record" and r.get("window_bytes") == 1 << 32 and r.get("lanes") == 196608:
-            g = float(r["GBps_at_64B"])
-            return {"GBps": g, "frac": round(achieved_gbps / g, 4), "what": "random dependent 64-byte record loads, "
-                    "4 GiB table (profiles/random_access_ceiling.md)"}
-    return None
+        if r.get("lanes") != 196608:
+            continue
+        if r.get("mode") == "mixed" and r.get("table_bytes") == 1 << 34 and r.get("window_bytes") == 1 << 30:
+            mix = float(r["GBps_at_64B"])
+        if r.get("mode") == "64B record" and r.get("table_bytes") == 1 << 32 and r.get("window_bytes") == 1 << 32:
+            big = float(r["GBps_at_64B"])
+    if mix is None:
+        return None
+    return {"GBps": mix, "frac": round(achieved_gbps / mix, 4),
+            "what": "random dependent loads in the SMEM kernel's mix: 64-byte records within 1 GiB, every sixth load "
+                    "16 bytes anywhere in 16 GiB (profiles/random_access_ceiling.md)",
+            "records_4GiB_GBps": big}
 
 
 def measured_traffic(kernel, args, nq, nl):
@@ -741,8 +751,9 @@ def run_1g(args):
 def random_access_calibration():
     """Records per second served to chains of random dependent loads (the
     access pattern of a backward-search step) by libhsfm_calib.so: whole 64-byte
-    records, 16 bytes of a record, and lo/hi record pairs, over a 4 GiB table
-    (and a 256 MiB window of it), at the persistent kernels' lane count and 4x it."""
+    records, 16 bytes of a record, lo/hi record pairs and the SMEM kernel's mix,
+    over tables of 256 MiB to 32 GiB, at the persistent kernels' lane count (and
+    4x it for 4 GiB)."""
     import ctypes as C
     lib = C.CDLL(os.path.join(ROOT, "haystackfm_amd", "libhsfm_calib.so"))
     fn = lib.hsfm_calib_random_records
@@ -750,15 +761,19 @@ def random_access_calibration():
     fn.argtypes = [C.c_int, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_int,
                    C.POINTER(C.c_double), C.POINTER(C.c_double)]
     res = []
-    for window, lanes, mode in [(1 << 32, 196608, 0), (1 << 32, 196608, 1), (1 << 32, 196608, 2),
-                                (1 << 32, 196608, 3), (1 << 32, 786432, 0), (1 << 32, 786432, 2),
-                                (1 << 28, 196608, 0)]:
+    G = 1 << 30
+    for table, window, lanes, mode in [(4 * G, 4 * G, 196608, 0), (4 * G, 4 * G, 196608, 1), (4 * G, 4 * G, 196608, 2),
+                                       (4 * G, 4 * G, 196608, 3), (4 * G, 4 * G, 786432, 0), (4 * G, 4 * G, 786432, 2),
+                                       (G // 4, G // 4, 196608, 0), (G, G, 196608, 0), (2 * G, 2 * G, 196608, 0),
+                                       (16 * G, 16 * G, 196608, 0), (G, G, 196608, 1), (16 * G, 16 * G, 196608, 1),
+                                       (16 * G, G, 196608, 4), (G, G, 196608, 4)]:
         ms, recs = C.c_double(), C.c_double()
-        rc = fn(0, 1 << 32, window, lanes, 200, mode, 3, C.byref(ms), C.byref(recs))
+        rc = fn(0, table, window, lanes, 200, mode, 3, C.byref(ms), C.byref(recs))
         if rc != 0:
             raise RuntimeError(f"hsfm_calib_random_records failed ({rc})")
-        row = {"mode": ["64B record", "16B of a record", "lo/hi record pair", "64B record, quad-cooperative"][mode],
-               "window_bytes": window,
+        row = {"mode": ["64B record", "16B of a record", "lo/hi record pair", "64B record, quad-cooperative",
+                        "mixed"][mode],
+               "table_bytes": table, "window_bytes": window,
                "lanes": lanes, "ms": round(ms.value, 3), "G_records_per_s": round(recs.value / ms.value / 1e6, 2),
                "GBps_at_64B": round(recs.value * 64 / ms.value / 1e6, 1)}
         log(row)

@@ -16,10 +16,12 @@ namespace {
 // mode 0: a whole 64-byte record (four 16-byte loads); mode 1: 16 bytes of a
 // record; mode 2: two records a few positions apart (the lo/hi ends of an
 // interval), both whole; mode 3: a whole record, loaded by the four lanes of a
-// quad together and exchanged by shuffles.
+// quad together and exchanged by shuffles; mode 4: every sixth load is 16
+// bytes anywhere in the whole table (`mask_all`), the others whole records in
+// the window (the mix of a seed-table lookup and the probes that follow it).
 template <int MODE>
-__global__ __launch_bounds__(256) void k_calib_chase(const uint4 *__restrict__ tab, uint64_t mask, uint32_t iters,
-                                                     uint32_t *__restrict__ sink) {
+__global__ __launch_bounds__(256) void k_calib_chase(const uint4 *__restrict__ tab, uint64_t mask, uint64_t mask_all,
+                                                     uint32_t iters, uint32_t *__restrict__ sink) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t x = (t + 1) * 0x9E3779B97F4A7C15ull;
     uint32_t acc = 0;
@@ -58,6 +60,9 @@ __global__ __launch_bounds__(256) void k_calib_chase(const uint4 *__restrict__ t
         } else if (MODE == 1) {
             const uint4 v = p[0];
             a = v.x ^ v.w;
+        } else if (MODE == 4 && i % 6 == 5) {
+            const uint4 v = tab[(((x >> 20) ^ acc) & mask_all) * 4];
+            a = v.x ^ v.w;
         } else {
 #pragma unroll
             for (int w = 0; w < 4; w++) {
@@ -93,7 +98,7 @@ extern "C" int hsfm_calib_random_records(int ordinal, uint64_t table_bytes, uint
                                          uint32_t iters, int mode, int reps, double *ms_out, double *records_out) {
     if (table_bytes < 65536 || (table_bytes & (table_bytes - 1)) || window_bytes < 64 ||
         (window_bytes & (window_bytes - 1)) || window_bytes > table_bytes || lanes == 0 || lanes % 256 ||
-        mode < 0 || mode > 3 || reps < 1 || !ms_out || !records_out)
+        mode < 0 || mode > 4 || reps < 1 || !ms_out || !records_out)
         return (int)hipErrorInvalidValue;
     hipError_t e = hipSetDevice(ordinal);
     if (e != hipSuccess) return (int)e;
@@ -101,7 +106,7 @@ extern "C" int hsfm_calib_random_records(int ordinal, uint64_t table_bytes, uint
     uint32_t *sink = nullptr;
     hipEvent_t a = nullptr, b = nullptr;
     float best = 1e30f;
-    const uint64_t mask = window_bytes / 64 - 1;
+    const uint64_t mask = window_bytes / 64 - 1, mask_all = table_bytes / 64 - 1;
     const dim3 grid(lanes / 256), block(256);
     if ((e = hipMalloc((void **)&tab, table_bytes)) != hipSuccess) return (int)e;
     if ((e = hipMalloc((void **)&sink, 64)) != hipSuccess) goto done;
@@ -109,10 +114,11 @@ extern "C" int hsfm_calib_random_records(int ordinal, uint64_t table_bytes, uint
     hipLaunchKernelGGL(k_calib_fill, dim3(4096), dim3(256), 0, 0, tab, table_bytes / 16);
     for (int rep = 0; rep <= reps && e == hipSuccess; rep++) { // launch 0 warms up
         e = hipEventRecord(a, 0);
-        if (mode == 0) hipLaunchKernelGGL(k_calib_chase<0>, grid, block, 0, 0, tab, mask, iters, sink);
-        else if (mode == 1) hipLaunchKernelGGL(k_calib_chase<1>, grid, block, 0, 0, tab, mask, iters, sink);
-        else if (mode == 2) hipLaunchKernelGGL(k_calib_chase<2>, grid, block, 0, 0, tab, mask, iters, sink);
-        else hipLaunchKernelGGL(k_calib_chase<3>, grid, block, 0, 0, tab, mask, iters, sink);
+        if (mode == 0) hipLaunchKernelGGL(k_calib_chase<0>, grid, block, 0, 0, tab, mask, mask_all, iters, sink);
+        else if (mode == 1) hipLaunchKernelGGL(k_calib_chase<1>, grid, block, 0, 0, tab, mask, mask_all, iters, sink);
+        else if (mode == 2) hipLaunchKernelGGL(k_calib_chase<2>, grid, block, 0, 0, tab, mask, mask_all, iters, sink);
+        else if (mode == 3) hipLaunchKernelGGL(k_calib_chase<3>, grid, block, 0, 0, tab, mask, mask_all, iters, sink);
+        else hipLaunchKernelGGL(k_calib_chase<4>, grid, block, 0, 0, tab, mask, mask_all, iters, sink);
         if (e == hipSuccess) e = hipGetLastError();
         if (e == hipSuccess) e = hipEventRecord(b, 0);
         if (e == hipSuccess) e = hipEventSynchronize(b);
