@@ -1473,14 +1473,17 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     if ((rc = ws.mem_flags.ensure(W * 4 + 4))) return rc;
     if ((rc = ws.ovf_list.ensure(W * 4 + 4))) return rc;
     if ((rc = ws.read_list.ensure(W * 4 + 4))) return rc;
-    if ((rc = ws.counters.ensure(sizeof(MemCounters)))) return rc;
+    // [0]: the persistent kernel (and the resolve guard), [1]: the workgroup passes
+    if ((rc = ws.counters.ensure(2 * sizeof(MemCounters)))) return rc;
     if ((rc = ws.mem_off.ensure((W + 1) * 8))) return rc;
     MemCounters *ctr = ws.counters.as<MemCounters>();
-    if ((rc = fill_u32(idx, ws.counters.as<uint32_t>(), sizeof(MemCounters) / 4, 0))) return rc;
+    if ((rc = fill_u32(idx, ws.counters.as<uint32_t>(), 2 * sizeof(MemCounters) / 4, 0))) return rc;
+    MemCounters *wc = ctr + 1;
     if ((rc = fill_u32(idx, ws.mem_cnt.as<uint32_t>(), W, 0))) return rc;
     if ((rc = fill_u32(idx, ws.mem_flags.as<uint32_t>(), W, 0xffffffffu))) return rc;
     const uint32_t *list = nullptr; // identity list
     uint64_t nlist = W;
+    const unsigned long long *first_count = nullptr; // device count of the first workgroup pass's list
     SeedView sv{nullptr, nullptr, nullptr, 0};
     SeedView svw = sv; // seeds of the workgroup passes
     uint32_t *retry = ws.ovf_list.as<uint32_t>();
@@ -1506,7 +1509,6 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
             svp.F = 0; // restarts not seeded by the tables
             svw = svp;
         }
-        uint64_t hd[16];
         if ((rc = event_record(idx, 0, false))) return rc;
         {
             // the learned-seed states are compiled into their own instance, so
@@ -1522,23 +1524,16 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         HSFM_HIP(hipGetLastError());
         if ((rc = event_record(idx, 0, true))) return rc;
         if ((rc = event_record(idx, 2, false))) return rc;
-        if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), hd, 16))) return rc;
-        nlist = hd[2];
-        idx->stats.occ_probes = hd[4];
-        for (int i = 0; i < 4; i++) idx->stats.diag[i] = hd[5 + i];
+        // the deferred items go to the workgroup pass with no host round trip:
+        // it reads their number from the persistent kernel's counter, and the
+        // counters of both come back together after it
+        first_count = &ctr->n_retry;
+        nlist = W; // a bound; the count is on the device
         list = ws.ovf_list.as<uint32_t>();
         retry = ws.read_list.as<uint32_t>();
-        idx->stats.fallback_reads = nlist;
-        // keep the deferred items for hsfm_last_deferred (the retry passes reuse ovf_list)
-        if ((rc = ws.defer_ids.ensure(nlist * 4 + 4))) return rc;
-        if (nlist)
-            HSFM_HIP(hipMemcpyAsync(ws.defer_ids.p, ws.ovf_list.p, nlist * 4, hipMemcpyDeviceToDevice, s));
-        idx->defer_n = nlist;
-        idx->defer_items = items != nullptr;
-        idx->defer_valid = true;
     }
 
-    uint64_t rec_cap = nlist * 8 + 4096;
+    uint64_t rec_cap = first_count ? 65536 : nlist * 8 + 4096;
     if (ws.mem_recs.cap / sizeof(MemRec) > rec_cap) rec_cap = ws.mem_recs.cap / sizeof(MemRec);
     // First pass: sets in LDS (kWaveLdsCap intervals); reads that outgrow them
     // re-run with global-memory sets of 4x the capacity per pass.
@@ -1574,27 +1569,28 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
             ws.mem_recs = grown;
         }
         rec_cap = ws.mem_recs.cap / sizeof(MemRec);
-        if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->work), 2, 0))) return rc;
-        if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->n_retry), 2, 0))) return rc;
+        if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&wc->work), 2, 0))) return rc;
+        if ((rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&wc->n_retry), 2, 0))) return rc;
+        const unsigned long long *dcount = pass == 0 ? first_count : nullptr;
         const bool timed = !use_thread && pass == 0;
         if (timed && (rc = event_record(idx, 0, false))) return rc;
         if (lds && use_thread)
             hipLaunchKernelGGL((k_mem_wave<true, kDeferTB>), dim3((uint32_t)waves), dim3(kDeferTB), 0, s,
                                idx->fwd.view, idx->rev.view, idx->d_const.as<DevConst>(), svw, d_q, d_qoff, list,
                                (uint32_t)nlist, items, min_len, cap, (uint4 *)nullptr, ws.mem_recs.as<MemRec>(),
-                               rec_cap, ws.mem_cnt.as<uint32_t>(), ws.mem_flags.as<uint32_t>(), pass, retry, ctr,
-                               nullptr);
+                               rec_cap, ws.mem_cnt.as<uint32_t>(), ws.mem_flags.as<uint32_t>(), pass, retry, wc,
+                               dcount);
         else if (lds)
             hipLaunchKernelGGL((k_mem_wave<true, 64>), dim3((uint32_t)waves), dim3(64), 0, s, idx->fwd.view,
                                idx->rev.view,
                                idx->d_const.as<DevConst>(), svw, d_q, d_qoff, list, (uint32_t)nlist, items, min_len, cap,
                                (uint4 *)nullptr, ws.mem_recs.as<MemRec>(), rec_cap, ws.mem_cnt.as<uint32_t>(),
-                               ws.mem_flags.as<uint32_t>(), pass, retry, ctr, nullptr);
+                               ws.mem_flags.as<uint32_t>(), pass, retry, wc, dcount);
         else
             hipLaunchKernelGGL((k_mem_wave<false, 1024>), dim3((uint32_t)waves), dim3(1024), 0, s, idx->fwd.view,
                                idx->rev.view, idx->d_const.as<DevConst>(), svw, d_q, d_qoff, list, (uint32_t)nlist,
                                items, min_len, cap, ws.wave_sets.as<uint4>(), ws.mem_recs.as<MemRec>(), rec_cap,
-                               ws.mem_cnt.as<uint32_t>(), ws.mem_flags.as<uint32_t>(), pass, retry, ctr, nullptr);
+                               ws.mem_cnt.as<uint32_t>(), ws.mem_flags.as<uint32_t>(), pass, retry, wc, dcount);
         HSFM_HIP(hipGetLastError());
         if (timed && (rc = event_record(idx, 0, true))) return rc;
         if (use_thread && (rc = event_record(idx, 2, true))) return rc;
@@ -1603,12 +1599,27 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         if ((rc = exclusive_scan_u32_u64(idx, ws.mem_cnt.as<uint32_t>(), ws.mem_off.as<uint64_t>(), (uint32_t)W)))
             return rc;
         if (!ws.h_pinned) HSFM_HIP(hipHostMalloc((void **)&ws.h_pinned, kPinnedBytes, hipHostMallocDefault));
-        HSFM_HIP(hipMemcpyAsync(ws.h_pinned, ctr, 64, hipMemcpyDeviceToHost, s));
+        HSFM_HIP(hipMemcpyAsync(ws.h_pinned, wc, 64, hipMemcpyDeviceToHost, s));
         HSFM_HIP(hipMemcpyAsync(ws.h_pinned + 8, ws.mem_off.as<uint64_t>() + W, 8, hipMemcpyDeviceToHost, s));
+        if (dcount) HSFM_HIP(hipMemcpyAsync(ws.h_pinned + 16, ctr, 128, hipMemcpyDeviceToHost, s));
         HSFM_HIP(hipStreamSynchronize(s));
         memcpy(h, ws.h_pinned, 64);
         const uint64_t nretry = h[2];
         rec_used = h[1];
+        if (dcount) { // the persistent kernel's counters: deferred items, accesses, deferral causes
+            const uint64_t *hd = ws.h_pinned + 16;
+            const uint64_t ndef = hd[2];
+            idx->stats.occ_probes = hd[4];
+            for (int i = 0; i < 4; i++) idx->stats.diag[i] = hd[5 + i];
+            idx->stats.fallback_reads = ndef;
+            // keep the deferred items for hsfm_last_deferred (a retry pass rewrites ovf_list)
+            if ((rc = ws.defer_ids.ensure(ndef * 4 + 4))) return rc;
+            if (ndef)
+                HSFM_HIP(hipMemcpyAsync(ws.defer_ids.p, ws.ovf_list.p, ndef * 4, hipMemcpyDeviceToDevice, s));
+            idx->defer_n = ndef;
+            idx->defer_items = items != nullptr;
+            idx->defer_valid = true;
+        }
         if (nretry == 0) {
             nhits = ws.h_pinned[8];
             scanned = true;
