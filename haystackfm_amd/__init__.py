@@ -258,7 +258,7 @@ class _BatchStats(C.Structure):
     _fields_ = [("batches", C.c_uint32), ("fallback_batches", C.c_uint32), ("main_kernel", C.c_int),
                 ("resolve_fanout", C.c_int), ("records", C.c_uint64), ("positions", C.c_uint64),
                 ("total_ms", C.c_double), ("tune_total_ms", C.c_double), ("tune_ms", C.c_double * 6),
-                ("window_batches", C.c_uint32), ("windows", C.c_uint32)]
+                ("window_batches", C.c_uint32), ("windows", C.c_uint32), ("graphs_captured", C.c_uint32)]
 
 
 _lib_obj = None

@@ -2101,7 +2101,8 @@ static int batch_capture(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, con
 // changed, else a new capture (replacing it).
 static int batch_graph(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, const SeedView &svp,
                        const uint8_t *d_q, const uint32_t *qnib, uint32_t min_len, const uint32_t *d_refb,
-                       uint32_t n_refs, uint32_t max_hits, int main_kernel, int rpt, hipGraphExec_t &out) {
+                       uint32_t n_refs, uint32_t max_hits, int main_kernel, int rpt, hipGraphExec_t &out,
+                       uint32_t &captured) {
     std::vector<uint64_t> key = {
         reinterpret_cast<uint64_t>(d_q), reinterpret_cast<uint64_t>(qnib), reinterpret_cast<uint64_t>(d_refb),
         reinterpret_cast<uint64_t>(bb.qoff_all), n_refs, min_len, max_hits, (uint64_t)main_kernel, (uint64_t)rpt,
@@ -2124,6 +2125,7 @@ static int batch_graph(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, const
                                bb.graph);
         if (rc) return rc;
         bb.graph_key = key;
+        captured++;
     }
     out = bb.graph.x;
     return HSFM_OK;
@@ -2379,7 +2381,7 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
     // batches in flight measured slower, 2M uneven reads: 54.5 ms with three
     // thirds against 47 ms with three whole grids)
     if (cfg->use_graph && (rc = batch_graph(idx, bb, sv, svp, d_q, bb.qnib.as<uint32_t>(), min_len, d_refb, n_refs,
-                                            max_hits, main_kernel, rpt, gx[0])))
+                                            max_hits, main_kernel, rpt, gx[0], stats.graphs_captured)))
         return cleanup(rc);
     for (int k = 1; k < nsets; k++) {
         if (!idx->batch[k]) idx->batch[k] = new BatchBufs();
@@ -2394,7 +2396,7 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
         set[k] = &bk;
         sst[k] = idx->bstream[k];
         if ((rc = batch_graph(idx, bk, sv, svp, d_q, bb.qnib.as<uint32_t>(), min_len, d_refb, n_refs, max_hits,
-                              main_kernel, rpt, gx[k])))
+                              main_kernel, rpt, gx[k], stats.graphs_captured)))
             return cleanup(rc);
     }
     HSFM_HIP(hipEventRecord(e0, s));

@@ -227,6 +227,8 @@ typedef struct {
     double tune_ms[6];  /* window-0 time of main kernels 0..3 (fan-out 1), then
                            the best main kernel with fan-out 2 and 4           */
     uint32_t window_batches, windows; /* batches per graph, graphs launched   */
+    uint32_t graphs_captured;         /* graphs captured by this call (0: every
+                                         set reused its graph from an earlier call) */
 } hsfm_batch_stats;
 /* The index stays locked while fn runs: an entry point called from fn on the
  * same index (hsfm_index_free included) returns HSFM_EINVAL without doing

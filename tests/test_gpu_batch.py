@@ -143,3 +143,80 @@ def test_callback_into_the_same_index_is_refused(uneven):
     ref = g.find_mems_batch(sub, 19, hf.HSFM_MODE_SMEM, (), 16)
     got = g.find_mems_batched(sub, 19, hf.HSFM_MODE_SMEM, (), 16, batch_reads=100, main_kernel=1, resolve_fanout=1)
     _same(got[:3], ref)
+
+
+def _device_batched(g, dq, do, dr, nq, nrefs, **kw):
+    """find_mems_batched_device on device buffers; the batches' results copied
+    to the host and joined as find_mems_batch returns them."""
+    L = hf._lib()
+    dev = g.fwd._dev[0]._h
+    parts = []
+
+    def on_batch(first, n, r):
+        nh, npos = int(r.num_hits), int(r.num_positions)
+        ho = np.zeros(n + 1, np.uint64)
+        hits = np.zeros((nh, 8), np.uint32)
+        pos = np.zeros((npos, 2), np.uint32)
+        for arr, src in ((ho, r.d_hit_offsets), (hits, r.d_hits), (pos, r.d_positions)):
+            if arr.nbytes:
+                hf._check(L.hsfm_device_to_host(dev, arr.ctypes.data, hf.C.c_void_p(src), arr.nbytes), "d2h")
+        parts.append((int(first), ho, hits, pos))
+
+    st = g.find_mems_batched_device(dq, do, nq, 19, hf.HSFM_MODE_SMEM, dr, nrefs, 1024, on_batch=on_batch, **kw)
+    parts.sort(key=lambda t: t[0])
+    ho_all, hits_all, pos_all, nh, npos = [np.zeros(1, np.uint64)], [], [], 0, 0
+    for _, ho, hits, pos in parts:
+        ho_all.append(ho[1:] + np.uint64(nh))
+        if len(hits):
+            po = (hits[:, 4].astype(np.uint64) | (hits[:, 5].astype(np.uint64) << np.uint64(32))) + np.uint64(npos)
+            hits = hits.copy()
+            hits[:, 4] = (po & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+            hits[:, 5] = (po >> np.uint64(32)).astype(np.uint32)
+        hits_all.append(hits)
+        pos_all.append(pos)
+        nh += len(hits)
+        npos += len(pos)
+    return (np.concatenate(ho_all), np.concatenate(hits_all), np.concatenate(pos_all)), st
+
+
+def test_graphs_reused_across_calls(uneven):
+    """A second call with the same buffers and configuration launches the
+    graphs the first one captured; after the read buffer is overwritten with
+    other reads of the same lengths, the reused graphs return the new reads'
+    results; another batch size captures again."""
+    _, g, (flat, off), bounds = uneven
+    rng = np.random.default_rng(31)
+    other = flat.copy()
+    flip = rng.random(len(other)) < 0.05
+    other[flip] = rng.integers(1, 5, int(flip.sum())).astype(np.uint8) # A, C, G, T codes
+    rb = np.asarray(bounds, np.uint32)
+    nq = len(off) - 1
+    L = hf._lib()
+    dev = g.fwd._dev[0]._h
+    ptrs = []
+    for arr in (flat, off.astype(np.uint64), rb):
+        p = hf.C.c_void_p()
+        hf._check(L.hsfm_device_alloc(dev, max(arr.nbytes, 8), hf.C.byref(p)), "alloc")
+        hf._check(L.hsfm_host_to_device(dev, p, arr.ctypes.data, arr.nbytes), "h2d")
+        ptrs.append(p)
+    dq, do, dr = (p.value for p in ptrs)
+    kw = dict(batch_reads=500, main_kernel=1, resolve_fanout=2, batch_sets=2, window_batches=2)
+    try:
+        first, st1 = _device_batched(g, dq, do, dr, nq, len(rb), **kw)
+        _same(first, g.find_mems_batch((flat, off), 19, hf.HSFM_MODE_SMEM, bounds, 1024))
+        assert st1["windows"] == 3 and st1["graphs_captured"] >= 1
+        again, st2 = _device_batched(g, dq, do, dr, nq, len(rb), **kw)
+        _same(again, first)
+        assert st2["graphs_captured"] == 0 and st2["windows"] == 3
+        hf._check(L.hsfm_host_to_device(dev, ptrs[0], other.ctypes.data, other.nbytes), "h2d")
+        new, st3 = _device_batched(g, dq, do, dr, nq, len(rb), **kw)
+        assert st3["graphs_captured"] == 0
+        _same(new, g.find_mems_batch((other, off), 19, hf.HSFM_MODE_SMEM, bounds, 1024))
+        kw["batch_reads"] = 700
+        hf._check(L.hsfm_host_to_device(dev, ptrs[0], flat.ctypes.data, flat.nbytes), "h2d")
+        redo, st4 = _device_batched(g, dq, do, dr, nq, len(rb), **kw)
+        assert st4["graphs_captured"] >= 1
+        _same(redo, first)
+    finally:
+        for p in ptrs:
+            L.hsfm_device_free(dev, p)
