@@ -19,7 +19,7 @@
               min_len 19: the graph-batched pipeline (64k-read batches,
               autotuned) equals the one-shot call over the whole batch, and
               the oracle on a sample.
-The learned-seed workload (1 Gbp) has its own test file.
+The learned-seed workload (1 Gbp, 4,000,000 reads) is tests/test_gpu_config5.py.
 
 Index bytes come from the library's device builder, checked here against the
 host builder at 100 Mbp; both equal the oracle's (tests/test_index_build.py,
