@@ -503,6 +503,25 @@ __global__ __launch_bounds__(256) void k_lsi_err(LsiModel md, const unsigned lon
     if (cur != 0xffffffffu) atomicMax(&lf[cur].err, e);
 }
 
+// A key above every entry of its leaf (or any key of an empty leaf) has its
+// lower bound one past the leaf, at the next leaf's s0, which [pred - err,
+// pred + err + 1] holds only if err also covers how far the leaf predicts
+// past that slot.  The prediction is non-decreasing inside a leaf, so the
+// leaf's largest key (its leaf bits, every code bit below them set) bounds
+// it: err >= pred(top) - end.  One thread per leaf, after k_lsi_err.
+__global__ __launch_bounds__(256) void k_lsi_err_top(LsiModel md, LsiLeaf *__restrict__ lf) {
+    __shared__ float s_p[3 * kLsiMaxHidden + 1];
+    for (uint32_t i = threadIdx.x; i < 3 * md.hidden + 1; i += blockDim.x) s_p[i] = md.p[i];
+    __syncthreads();
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= md.leaves) return;
+    const long long end = l + 1 < md.leaves ? (long long)lf[l + 1].s0 : (long long)md.nkeys;
+    const unsigned long long top = (md.lbits ? (unsigned long long)l << (64 - md.lbits) : 0ull) |
+                                   (md.smask & (~0ull >> md.lbits));
+    const long long d = lsi_pred(lf[l], lsi_stage1(s_p, md.hidden, lsi_u(md, top))) - end;
+    if (d > 0 && (unsigned long long)d > lf[l].err) lf[l].err = (uint32_t)d;
+}
+
 // ---- lookup ----------------------------------------------------------------
 // fence[j] = keys[j << sh]; entries past the last key hold the largest key value
 __global__ void k_lsi_fence(const unsigned long long *__restrict__ keys, uint64_t nkeys, int sh,
@@ -979,6 +998,7 @@ int lsi_train_and_fit(hsfm_lsi *x) {
     if (N)
         hipLaunchKernelGGL(k_lsi_err, cg, dim3(256), 0, s, md, x->keys.as<unsigned long long>(),
                            x->leaves.as<LsiLeaf>());
+    hipLaunchKernelGGL(k_lsi_err_top, dim3((L + 255) / 256), dim3(256), 0, s, md, x->leaves.as<LsiLeaf>());
     HSFM_HIP(hipGetLastError());
     HSFM_HIP(hipEventRecord(e1, s));
     x->info.fit_ms = elapsed_ms(e0, e1);

@@ -1533,7 +1533,9 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         retry = ws.read_list.as<uint32_t>();
     }
 
-    uint64_t rec_cap = first_count ? 65536 : nlist * 8 + 4096;
+    // records of the workgroup passes: eight per listed item (the count of a
+    // device-counted list is not known here, so its bound, every item)
+    uint64_t rec_cap = nlist * 8 + 4096;
     if (ws.mem_recs.cap / sizeof(MemRec) > rec_cap) rec_cap = ws.mem_recs.cap / sizeof(MemRec);
     // First pass: sets in LDS (kWaveLdsCap intervals); reads that outgrow them
     // re-run with global-memory sets of 4x the capacity per pass.

@@ -120,3 +120,30 @@ def test_learned_restarts_never_fall_back_at_min_len_k(tag):
     g.set_seed_mode(hf.HSFM_SEED_LEARNED, lsi)
     check_mems(o, g, sub, 21, cap=64)
     g.set_seed_mode(hf.HSFM_SEED_TABLES)
+
+
+@pytest.mark.parametrize("leaves", [1 << 12, 1 << 16, 1 << 18])
+def test_learned_restarts_exact_with_many_leaves(leaves):
+    """Many small leaves (down to a key or two per leaf): a window above every
+    key of its leaf has its lower bound one past the leaf, which the search
+    range must still hold however far the leaf's model predicts past its end;
+    the MEM lists equal plain search and the oracle."""
+    text, bounds = O.gen_reference(95, 200_000, 6, gc=0.41, repeat_frac=0.4, n_units=25, unit_min=200,
+                                   unit_max=2000, iupac_frac=0.005)
+    o = O.OracleBidir.build(text=text, bounds=bounds, sample_rate=16, threads=8)
+    g = hf.BidirFmIndex.from_bytes(o.to_bytes())
+    lsi = hf.LearnedSeedIndex(g, k=21, leaves=leaves, hidden=16, train_steps=200)
+    assert lsi.info()["mem_seeding"] == 1
+    flat, off = O.gen_reads(96, text, bounds, 2000, 150, sub=0.02, both_strands=True, n_random=300)
+    reads = (flat, off)
+    g.set_seed_mode(hf.HSFM_SEED_PLAIN)
+    plain = g.find_mems_batch(reads, 21, hf.HSFM_MODE_SMEM, bounds, 64)
+    try:
+        g.set_seed_mode(hf.HSFM_SEED_LEARNED, lsi)
+        got = g.find_mems_batch(reads, 21, hf.HSFM_MODE_SMEM, bounds, 64)
+        for a, b in zip(got, plain):
+            assert np.array_equal(a, b)
+        sub = [flat[off[i]:off[i + 1]] for i in range(0, 2300, 23)]
+        check_mems(o, g, sub, 21, cap=64)
+    finally:
+        g.set_seed_mode(hf.HSFM_SEED_TABLES)

@@ -156,3 +156,42 @@ def test_levels_rule_values():
     assert levels_for(69_632) == 4
     assert levels_for(340_899) == 4
     assert levels_for((1 << 32) - 1) == 7
+
+
+def leaf_range(pred, err, s0, n):
+    """lsi_range: [lo, hi] with the lower bound in [lo, hi + 1]."""
+    lo = min(max(pred - err, s0), n)
+    hi = max(min(pred + err + 1, n - 1), lo - 1)
+    return lo, hi
+
+
+def test_leaf_range_holds_every_lower_bound():
+    """Brute force over small leaves: for every key of a leaf's key space (not
+    only its entries), the lower bound lies in [lo, hi + 1] when the leaf's
+    error covers its entries and its prediction at the leaf's largest key past
+    the leaf's end (k_lsi_err after k_lsi_err_top); with the entries' error
+    alone, keys above the last entry (and the keys of empty leaves) are missed
+    whenever the model predicts past the end."""
+    rng = np.random.default_rng(5)
+    space = 64  # the keys of one leaf: 0 .. 63, u = key / 64
+    misses_entries_only = 0
+    for _ in range(400):
+        s0 = int(rng.integers(0, 50))
+        ent = np.sort(rng.choice(space, size=int(rng.integers(0, 8)), replace=False))
+        after = int(rng.integers(0, 50))
+        n = s0 + len(ent) + after
+        a, b = float(rng.uniform(0, 40)), float(rng.uniform(-10, 10))
+
+        def pred(w):  # non-decreasing in the key, as the stage-1 ramps and a >= 0 make it
+            return s0 + int(np.floor(a * (w / space) + b))
+
+        err_entries = max((abs(pred(int(x)) - (s0 + i)) for i, x in enumerate(ent)), default=0)
+        end = s0 + len(ent)
+        err = max(err_entries, pred(space - 1) - end)
+        for w in range(space):
+            lb = s0 + int(np.searchsorted(ent, w))
+            lo, hi = leaf_range(pred(w), err, s0, n)
+            assert lo <= lb <= hi + 1, (s0, ent.tolist(), a, b, w)
+            lo, hi = leaf_range(pred(w), err_entries, s0, n)
+            misses_entries_only += not (lo <= lb <= hi + 1)
+    assert misses_entries_only > 0
