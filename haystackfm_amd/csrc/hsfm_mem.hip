@@ -1158,53 +1158,50 @@ __global__ __launch_bounds__(256) void k_mem_scatter(const MemRec *__restrict__ 
                                                      const uint32_t *__restrict__ pass_of,
                                                      const uint64_t *__restrict__ mem_off, uint4 *__restrict__ out,
                                                      const unsigned long long *__restrict__ d_nrec) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (d_nrec) nrec = *d_nrec < nrec ? *d_nrec : nrec;
-    if (t >= nrec) return;
-    const MemRec rc = recs[t];
-    if (rc.pass == 0xffffffffu || rc.read >= nq || pass_of[rc.read] != rc.pass) return;
-    out[mem_off[rc.read] + rc.local] = make_uint4(rc.qs, rc.qe, rc.lo, rc.hi);
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nrec; t += (uint64_t)gridDim.x * blockDim.x) {
+        const MemRec rc = recs[t];
+        if (rc.pass == 0xffffffffu || rc.read >= nq || pass_of[rc.read] != rc.pass) continue;
+        out[mem_off[rc.read] + rc.local] = make_uint4(rc.qs, rc.qe, rc.lo, rc.hi);
+    }
 }
 
 // Per record: positions to resolve = min(size, max_hits) (bidir_index.rs:315-337).
 __global__ __launch_bounds__(256) void k_mem_npos(const uint4 *__restrict__ recs, uint64_t nrec, uint32_t max_hits,
                                                   uint32_t resolve, uint32_t *__restrict__ npos,
                                                   const uint64_t *__restrict__ d_nrec) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nrec) return;
-    if (d_nrec && t >= *d_nrec) { // capacity-sized launch: no record here
-        npos[t] = 0;
-        return;
+    if (d_nrec) nrec = *d_nrec < nrec ? *d_nrec : nrec; // capacity-sized buffers: the count is on the device
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nrec; t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint4 r = recs[t];
+        const uint32_t size = r.w - r.z;
+        npos[t] = resolve ? (size < max_hits ? size : max_hits) : 0u;
     }
-    const uint4 r = recs[t];
-    const uint32_t size = r.w - r.z;
-    npos[t] = resolve ? (size < max_hits ? size : max_hits) : 0u;
 }
 
 __global__ __launch_bounds__(256) void k_mem_hits(const uint4 *__restrict__ recs, uint64_t nrec, uint32_t max_hits,
                                                   const uint64_t *__restrict__ pos_off, hsfm_mem_hit *__restrict__ hits,
                                                   uint32_t *__restrict__ rows, const uint64_t *__restrict__ d_nrec,
                                                   uint64_t pos_cap, unsigned long long *__restrict__ ovf) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (d_nrec) nrec = *d_nrec < nrec ? *d_nrec : nrec;
-    if (t >= nrec) return;
-    if (pos_off[t + 1] > pos_cap) { // positions would overrun a capacity-sized buffer
-        atomicAdd(ovf, 1ull);
-        return;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nrec; t += (uint64_t)gridDim.x * blockDim.x) {
+        if (pos_off[t + 1] > pos_cap) { // positions would overrun a capacity-sized buffer
+            atomicAdd(ovf, 1ull);
+            continue;
+        }
+        const uint4 r = recs[t];
+        const uint32_t size = r.w - r.z;
+        const uint64_t po = pos_off[t], pl = pos_off[t + 1] - po;
+        hsfm_mem_hit h;
+        h.query_start = r.x;
+        h.query_end = r.y;
+        h.match_count = size;
+        h.truncated = size > max_hits ? 1u : 0u;
+        h.pos_offset = po;
+        h.pos_len = (uint32_t)pl;
+        h._reserved = 0;
+        hits[t] = h;
+        for (uint32_t i = 0; i < (uint32_t)pl; i++) rows[po + i] = r.z + i;
     }
-    const uint4 r = recs[t];
-    const uint32_t size = r.w - r.z;
-    const uint64_t po = pos_off[t], pl = pos_off[t + 1] - po;
-    hsfm_mem_hit h;
-    h.query_start = r.x;
-    h.query_end = r.y;
-    h.match_count = size;
-    h.truncated = size > max_hits ? 1u : 0u;
-    h.pos_offset = po;
-    h.pos_len = (uint32_t)pl;
-    h._reserved = 0;
-    hits[t] = h;
-    for (uint32_t i = 0; i < (uint32_t)pl; i++) rows[po + i] = r.z + i;
 }
 
 } // namespace hsfm
@@ -1833,6 +1830,8 @@ struct BatchBufs {
     DevBuf bargs; // {first read, reads} of the batch, copied in before each launch
     DevBuf sub_off, slices; // per-batch read offsets and {first hit}, {first position} of each batch of a window
     DevBuf bcount;          // set-up: work items per callback batch and read offsets at batch starts
+    DevBuf cscan;           // block sums of the device-count scan
+    const void *recs_dead = nullptr; // recs as last filled with dead records throughout
     // the set's instantiated graph, kept across calls while everything it
     // captured (pointers, sizes, configuration) is unchanged
     BatchGraph graph;
@@ -1842,8 +1841,10 @@ struct BatchBufs {
     void release() {
         DevBuf *all[] = {&bq_off, &slots, &mem_cnt, &status, &pass_of, &l1, &l2, &l3, &ctr, &recs, &wsets,
                          &mem_off, &sorted, &npos, &pos_off, &hits, &rows, &positions, &misc, &tmp,
-                         &items, &item_cnt, &item_first, &read_off, &qnib, &bargs, &sub_off, &slices, &bcount};
+                         &items, &item_cnt, &item_first, &read_off, &qnib, &bargs, &sub_off, &slices, &bcount,
+                         &cscan};
         graph.release();
+        recs_dead = nullptr;
         graph_key.clear();
         for (DevBuf *b : all) b->release();
     }
@@ -1851,7 +1852,8 @@ struct BatchBufs {
     void addresses(std::vector<uint64_t> &k) const {
         const DevBuf *all[] = {&bq_off, &slots, &mem_cnt, &status, &pass_of, &l1, &l2, &l3, &ctr, &recs, &wsets,
                                &mem_off, &sorted, &npos, &pos_off, &hits, &rows, &positions, &misc, &tmp,
-                               &items, &item_cnt, &item_first, &read_off, &qnib, &bargs, &sub_off, &slices};
+                               &items, &item_cnt, &item_first, &read_off, &qnib, &bargs, &sub_off, &slices,
+                               &cscan};
         for (const DevBuf *b : all) k.push_back(reinterpret_cast<uint64_t>(b->p));
     }
 };
@@ -1874,6 +1876,86 @@ struct ToU64 {
     __device__ __forceinline__ uint64_t operator()(uint32_t v) const { return v; }
 };
 __global__ void k_zero_u64(uint64_t *p) { *p = 0; }
+
+// Exclusive prefix sums of in[0, n) into out[0, n] (out[n] = the total) for
+// n = min(*d_n, cap) read on the device: a graph cannot size a library scan by
+// a device count, and a scan over the whole capacity reads and writes the
+// capacity.  kCScanBlocks blocks each take a chunk of [0, n): their sums, one
+// block's scan of the sums, then each chunk in tiles of 256 from its base.
+constexpr uint32_t kCScanBlocks = 512;
+__global__ __launch_bounds__(256) void k_cscan_sums(const uint32_t *__restrict__ in, uint64_t cap,
+                                                    const uint64_t *__restrict__ d_n, uint64_t *__restrict__ part) {
+    __shared__ uint64_t s_w[4];
+    const uint64_t n = *d_n < cap ? *d_n : cap;
+    const uint64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    uint64_t sum = 0;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) sum += in[i];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+__global__ __launch_bounds__(kCScanBlocks) void k_cscan_part(uint64_t *__restrict__ part) {
+    __shared__ uint64_t s_w[kCScanBlocks / 64];
+    const uint64_t v = part[threadIdx.x];
+    uint64_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t x = __shfl_up(incl, d, 64);
+        if ((threadIdx.x & 63) >= (uint32_t)d) incl += x;
+    }
+    if ((threadIdx.x & 63) == 63) s_w[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    uint64_t before = 0, all = 0;
+    for (uint32_t u = 0; u < kCScanBlocks / 64; u++) {
+        before += u < (threadIdx.x >> 6) ? s_w[u] : 0ull;
+        all += s_w[u];
+    }
+    __syncthreads();
+    part[threadIdx.x] = before + incl - v;
+    if (threadIdx.x == 0) part[kCScanBlocks] = all;
+}
+__global__ __launch_bounds__(256) void k_cscan_apply(const uint32_t *__restrict__ in, uint64_t cap,
+                                                     const uint64_t *__restrict__ d_n,
+                                                     const uint64_t *__restrict__ part, uint64_t *__restrict__ out) {
+    __shared__ uint32_t s_part[256 / 64 + 1];
+    const uint64_t n = *d_n < cap ? *d_n : cap;
+    const uint64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    uint64_t base = part[blockIdx.x];
+    for (uint64_t t0 = lo; t0 < hi; t0 += 256) {
+        const uint64_t i = t0 + threadIdx.x;
+        const uint32_t v = i < hi ? in[i] : 0u;
+        uint32_t tile = 0;
+        const uint32_t incl = block_incl_scan<256>(v, s_part, tile);
+        if (i < hi) out[i] = base + incl - v;
+        base += tile;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = part[gridDim.x];
+}
+// The records of the previous window of this buffer set back to dead (its
+// counters still hold how many it reserved; the rest of the buffer has been
+// dead since the set's buffers were allocated).
+__global__ void k_recs_reset(uint64_t *r0, uint64_t *r1, uint64_t half, const unsigned long long *used0,
+                             const unsigned long long *used1) {
+    const uint64_t n0 = *used0 < half ? *used0 : half, n1 = *used1 < half ? *used1 : half;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 4 * n0; i += stride) r0[i] = ~0ull;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 4 * n1; i += stride) r1[i] = ~0ull;
+}
+
+static int count_scan(hsfm_index *idx, BatchBufs &bb, const uint32_t *in, uint64_t *out, uint64_t cap,
+                      const uint64_t *d_n) {
+    hipStream_t s = idx->dev->stream;
+    uint64_t *part = bb.cscan.as<uint64_t>();
+    hipLaunchKernelGGL(k_cscan_sums, dim3(kCScanBlocks), dim3(256), 0, s, in, cap, d_n, part);
+    hipLaunchKernelGGL(k_cscan_part, dim3(1), dim3(kCScanBlocks), 0, s, part);
+    hipLaunchKernelGGL(k_cscan_apply, dim3(kCScanBlocks), dim3(256), 0, s, in, cap, d_n, part, out);
+    HSFM_HIP(hipGetLastError());
+    return HSFM_OK;
+}
 
 static int batch_scan(hsfm_index *idx, BatchBufs &bb, const uint32_t *in, uint64_t *out, uint64_t n) {
     hipStream_t s = idx->dev->stream;
@@ -1916,7 +1998,7 @@ __global__ __launch_bounds__(256) void k_batch_counts(const uint64_t *__restrict
 // A window of Wb batches (Bc reads each; the window's read count nb in
 // bargs[1]): out[j] = batch j's first hit, out[Wb + 1 + j] = its first
 // position (j = 0..Wb; batches past nb start at the window's end).
-constexpr uint32_t kMaxWindowBatches = 16;
+constexpr uint32_t kMaxWindowBatches = 64;
 __global__ void k_batch_slices(const uint64_t *__restrict__ read_off, const uint64_t *__restrict__ pos_off,
                                uint64_t hit_cap, const uint64_t *__restrict__ bargs, uint32_t Bc, uint32_t Wb,
                                uint64_t *__restrict__ out) {
@@ -1988,15 +2070,17 @@ static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, cons
     hipLaunchKernelGGL(k_pad_items, dim3((bb.W + 255) / 256), dim3(256), 0, s, bb.items.as<uint4>(),
                        bb.item_first.as<uint64_t>() + B, bb.W);
     HSFM_HIP(hipGetLastError());
+    const uint64_t half = bb.rec_cap / 2;
+    MemRec *r0 = bb.recs.as<MemRec>(), *r1 = r0 + half;
+    // the previous window's records back to dead, before its counters are cleared
+    hipLaunchKernelGGL(k_recs_reset, dim3(idx->dev->num_cus * 4), dim3(256), 0, s, reinterpret_cast<uint64_t *>(r0),
+                       reinterpret_cast<uint64_t *>(r1), half, &c[1].rec_used, &c[2].rec_used);
     if ((rc = fill_u32(idx, bb.ctr.as<uint32_t>(), 3 * sizeof(MemCounters) / 4, 0)) ||
         (rc = fill_u32(idx, bb.misc.as<uint32_t>(), sizeof(BatchMisc) / 4, 0)) ||
         (rc = fill_u32(idx, bb.mem_cnt.as<uint32_t>(), W, 0)) ||
         (rc = fill_u32(idx, bb.pass_of.as<uint32_t>(), W, 0xffffffffu)) ||
         (rc = fill_u32(idx, bb.status.as<uint32_t>(), (W + 3) / 4, main_kernel == BM_WAVE ? 0x01010101u : 0u)))
         return rc;
-    hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, s, bb.recs.as<uint64_t>(), bb.rec_cap * 4, ~0ull);
-    const uint64_t half = bb.rec_cap / 2;
-    MemRec *r0 = bb.recs.as<MemRec>(), *r1 = r0 + half;
     const uint32_t lds_waves = persist_grid(idx, k_mem_wave<true, 64>, 64);
     const unsigned long long *second_count = &c[0].n_retry;
     if (main_kernel == BM_WAVE) {
@@ -2047,16 +2131,17 @@ static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, cons
                            bb.status.as<uint8_t>(), bb.mem_cnt.as<uint32_t>(), mem_off, bb.slots.as<uint4>(), sorted);
     hipLaunchKernelGGL(k_read_off, dim3((B + 256) / 256), dim3(256), 0, s, B, bb.item_first.as<uint64_t>(), mem_off,
                        bb.read_off.as<uint64_t>());
-    const dim3 hg((uint32_t)((half + 255) / 256));
+    const dim3 hg(idx->dev->num_cus * 8); // grid-stride over the device counts
     hipLaunchKernelGGL(k_mem_scatter, hg, dim3(256), 0, s, r0, half, W, bb.pass_of.as<uint32_t>(), mem_off, sorted,
                        &c[1].rec_used);
     hipLaunchKernelGGL(k_mem_scatter, hg, dim3(256), 0, s, r1, half, W, bb.pass_of.as<uint32_t>(), mem_off, sorted,
                        &c[2].rec_used);
     HSFM_HIP(hipGetLastError());
-    const dim3 cg((uint32_t)((bb.hit_cap + 255) / 256));
+    const dim3 cg(idx->dev->num_cus * 8);
     hipLaunchKernelGGL(k_mem_npos, cg, dim3(256), 0, s, sorted, bb.hit_cap, max_hits, n_refs ? 1u : 0u,
                        bb.npos.as<uint32_t>(), mem_off + W);
-    if ((rc = batch_scan(idx, bb, bb.npos.as<uint32_t>(), bb.pos_off.as<uint64_t>(), bb.hit_cap))) return rc;
+    if ((rc = count_scan(idx, bb, bb.npos.as<uint32_t>(), bb.pos_off.as<uint64_t>(), bb.hit_cap, mem_off + W)))
+        return rc;
     BatchMisc *misc = bb.misc.as<BatchMisc>();
     hipLaunchKernelGGL(k_mem_hits, cg, dim3(256), 0, s, sorted, bb.hit_cap, max_hits, bb.pos_off.as<uint64_t>(),
                        bb.hits.as<hsfm_mem_hit>(), bb.rows.as<uint32_t>(), mem_off + W, bb.pos_cap, &misc->pos_ovf);
@@ -2151,8 +2236,20 @@ static int batch_alloc(BatchBufs &bb) {
         (rc = bb.hits.ensure(bb.hit_cap * sizeof(hsfm_mem_hit))) || (rc = bb.rows.ensure(bb.pos_cap * 4)) ||
         (rc = bb.positions.ensure(bb.pos_cap * 8)) || (rc = bb.misc.ensure(sizeof(BatchMisc))) ||
         (rc = bb.tmp.ensure(bb.tmp_bytes + 256)) || (rc = bb.sub_off.ensure(((size_t)B + bb.Wb + 1) * 8)) ||
-        (rc = bb.slices.ensure((2ull * bb.Wb + 2) * 8 + 16)))
+        (rc = bb.slices.ensure((2ull * bb.Wb + 2) * 8 + 16)) || (rc = bb.cscan.ensure((kCScanBlocks + 1) * 8)))
         return rc;
+    return HSFM_OK;
+}
+// A record buffer new to the set starts dead throughout (later windows reset
+// only what the previous one reserved) with zero counters.
+static int batch_recs_dead(hsfm_index *idx, BatchBufs &bb) {
+    if (bb.recs_dead == bb.recs.p) return HSFM_OK;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, idx->dev->stream, bb.recs.as<uint64_t>(), bb.rec_cap * 4,
+                       ~0ull);
+    HSFM_HIP(hipGetLastError());
+    int rc = fill_u32(idx, bb.ctr.as<uint32_t>(), 3 * sizeof(MemCounters) / 4, 0);
+    if (rc) return rc;
+    bb.recs_dead = bb.recs.p;
     return HSFM_OK;
 }
 
@@ -2227,7 +2324,9 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
         for (uint32_t b = 0; b < ncb; b++) items += bitems[b];
         const double per_read = nq ? (double)items / nq : 1.0;
         const double lanes = (double)persist_grid(idx, k_mem_persist<16, 256>, 256) * 256.0;
-        Wb = (uint32_t)ceil(8.0 * lanes / (per_read * (double)Bc));
+        // about sixteen work items per lane: the persistent kernel's end (its
+        // last items) is then a small part of the window
+        Wb = (uint32_t)ceil(16.0 * lanes / (per_read * (double)Bc));
     }
     Wb = Wb < 1 ? 1 : (Wb > kMaxWindowBatches ? kMaxWindowBatches : Wb);
     if (Wb > ncb) Wb = ncb ? ncb : 1;
@@ -2270,10 +2369,9 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
         size_t t1 = 0, t2 = 0;
         hipcub::TransformInputIterator<uint64_t, ToU64, const uint32_t *> it(nullptr, ToU64());
         HSFM_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, t1, it, (uint64_t *)nullptr, (int)bb.W, s));
-        HSFM_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, t2, it, (uint64_t *)nullptr, (int)bb.hit_cap, s));
         bb.tmp_bytes = t1 > t2 ? t1 : t2;
     }
-    if ((rc = batch_alloc(bb))) {
+    if ((rc = batch_alloc(bb)) || (rc = batch_recs_dead(idx, bb))) {
         bb.release();
         return rc;
     }
@@ -2392,7 +2490,7 @@ extern "C" int hsfm_find_mems_batched(hsfm_index *idx, const uint8_t *d_q, const
         bk.slot_cap = bb.slot_cap; bk.rec_cap = bb.rec_cap; bk.hit_cap = bb.hit_cap; bk.pos_cap = bb.pos_cap;
         bk.tmp_bytes = bb.tmp_bytes;
         bk.qoff_all = d_qoff;
-        if ((rc = batch_alloc(bk))) return cleanup(rc);
+        if ((rc = batch_alloc(bk)) || (rc = batch_recs_dead(idx, bk))) return cleanup(rc);
         if (!idx->bstream[k] && hipStreamCreateWithFlags(&idx->bstream[k], hipStreamNonBlocking) != hipSuccess)
             return cleanup(set_error(HSFM_EDEVICE, "hipStreamCreate failed"));
         set[k] = &bk;

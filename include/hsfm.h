@@ -215,8 +215,8 @@ typedef struct {
     int batch_sets;        /* windows in flight, each with its own buffers,
                               graph and stream (0 = 3, at most 8)            */
     uint32_t window_batches; /* batches per graph (0: enough work items for
-                              about eight per lane of the persistent grid,
-                              at most 16)                                     */
+                              about sixteen per lane of the persistent grid;
+                              at most 64)                                     */
 } hsfm_batch_config;
 typedef struct {
     uint32_t batches, fallback_batches; /* batches, those re-run host-driven   */
