@@ -816,7 +816,11 @@ __global__ __launch_bounds__(256) void k_expand(uint32_t nq, const uint8_t *__re
 }
 
 // Runs steps 1-2 and leaves per-read counts in ws.cnt (plus intervals).
-static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64_t *d_qoff, uint32_t nq) {
+// total (optional): the scan of the hit counts into ws.hit_off runs behind
+// the passes, so its total comes back with their counters in one read-back
+// (redone after capacity retries, which change counts).
+static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64_t *d_qoff, uint32_t nq,
+                               uint64_t *total = nullptr) {
     Workspace &ws = idx->ws;
     hipStream_t s = idx->dev->stream;
     const DevDir &d = idx->fwd.view;
@@ -870,7 +874,18 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
                            idx->seed_rec12.p ? 12u : 0u);
         HSFM_HIP(hipGetLastError());
     }
-    if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
+    if (total) {
+        if ((rc = ws.hit_off.ensure(((size_t)nq + 1) * 8))) return rc;
+        if ((rc = exclusive_scan_u32_u64(idx, ws.cnt.as<uint32_t>(), ws.hit_off.as<uint64_t>(), nq))) return rc;
+        if (!ws.h_pinned) HSFM_HIP(hipHostMalloc((void **)&ws.h_pinned, kPinnedBytes, hipHostMallocDefault));
+        HSFM_HIP(hipMemcpyAsync(ws.h_pinned, ctr, 64, hipMemcpyDeviceToHost, s));
+        HSFM_HIP(hipMemcpyAsync(ws.h_pinned + 8, ws.hit_off.as<uint64_t>() + nq, 8, hipMemcpyDeviceToHost, s));
+        HSFM_HIP(hipStreamSynchronize(s));
+        memcpy(h, ws.h_pinned, 64);
+        *total = ws.h_pinned[8];
+    } else if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) {
+        return rc;
+    }
     idx->stats.occ_probes = h[5]; // the main kernel's own probes
     idx->stats.fallback_reads = h[0];
     const uint64_t n_wave = h[6];
@@ -922,6 +937,10 @@ static int locate_search_phase(hsfm_index *idx, const uint8_t *d_q, const uint64
             retry = const_cast<uint32_t *>(list);
             list = next_list;
         }
+        if (total) { // retried reads changed their counts
+            if ((rc = exclusive_scan_u32_u64(idx, ws.cnt.as<uint32_t>(), ws.hit_off.as<uint64_t>(), nq))) return rc;
+            if ((rc = readback_u64(idx, ws.hit_off.as<uint64_t>() + nq, total, 1))) return rc;
+        }
     }
     if (n_wave) {
         // put every wave-path read's intervals in row order (FmIndex::locate order)
@@ -965,11 +984,8 @@ static int locate_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_
     idx->defer_valid = false;
     int rc = ensure_sa_full(idx);
     if (rc) return rc;
-    if ((rc = locate_search_phase(idx, d_q, d_qoff, nq))) return rc;
-    if ((rc = ws.hit_off.ensure(((size_t)nq + 1) * 8))) return rc;
-    if ((rc = exclusive_scan_u32_u64(idx, ws.cnt.as<uint32_t>(), ws.hit_off.as<uint64_t>(), nq))) return rc;
     uint64_t total = 0;
-    if ((rc = readback_u64(idx, ws.hit_off.as<uint64_t>() + nq, &total, 1))) return rc;
+    if ((rc = locate_search_phase(idx, d_q, d_qoff, nq, &total))) return rc;
     if ((rc = ws.rows.ensure(total * 4 + 4))) return rc;
     if ((rc = ws.hits.ensure(total * 8 + 8))) return rc;
     LocateCounters *ctr = ws.counters.as<LocateCounters>();
@@ -982,10 +998,14 @@ static int locate_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_
                                  idx->nseq, ws.hits.as<uint32_t>(), &ctr->guard)))
             return rc;
     }
-    uint64_t h[8];
-    if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
+    // resolve from the full suffix array makes no LF walk: no guard to read back
+    uint64_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (total && !idx->fwd.view.sa_full) {
+        if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
+    } else {
+        HSFM_HIP(hipStreamSynchronize(s)); // results complete on return
+    }
     idx->stats.guard_hits = h[4];
-    if (!idx->stats.occ_probes) idx->stats.occ_probes = h[5];
     idx->stats.main_kernel_ms = event_ms(idx, 0);
     idx->stats.resolve_kernel_ms = total ? event_ms(idx, 1) : 0.0;
     idx->stats.fallback_kernel_ms = idx->stats.fallback_reads ? event_ms(idx, 2) : 0.0;
