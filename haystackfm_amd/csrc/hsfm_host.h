@@ -62,6 +62,7 @@ struct Workspace {
     DevBuf items, item_cnt, item_first, read_off; // MEM work items (read segments)
     DevBuf qnib;                              // queries as base-set nibbles
     DevBuf defer_ids;                         // work items the MEM main kernel deferred
+    DevBuf part_cnt;                          // record counts of the parts of split deferred items
     uint64_t *h_pinned = nullptr;             // small readbacks
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}; // main, resolve, fallback kernels
 };

@@ -134,6 +134,11 @@ constexpr uint32_t kWaveLdsCap = 1024;
 // wider than its ring, so a step extends up to 256 intervals at once instead
 // of 64 per dependent round.
 constexpr int kDeferTB = 256;
+// Parts per deferred item in the first workgroup pass (at most; see k_mem_wave)
+// and the pass_of mark of an item one of whose parts failed.
+constexpr uint32_t kDeferParts = 64;
+constexpr uint32_t kPartFailed = 0xfffffffeu;
+constexpr uint32_t kPartCntMax = 65536; // part counts a batch set holds (workgroups of the deferred pass)
 
 template <bool LDS, int TB>
 __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const DevConst *__restrict__ cst,
@@ -145,7 +150,8 @@ __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const D
                                                  uint64_t rec_cap, uint32_t *__restrict__ mem_cnt,
                                                  uint32_t *__restrict__ pass_of, uint32_t pass,
                                                  uint32_t *__restrict__ retry_list, MemCounters *__restrict__ ctr,
-                                                 const unsigned long long *__restrict__ d_nlist) {
+                                                 const unsigned long long *__restrict__ d_nlist,
+                                                 uint32_t *__restrict__ part_cnt, uint32_t parts_max) {
     __shared__ uint16_t s_compat[256];
     __shared__ uint8_t s_expand[256];
     __shared__ uint32_t s_clane[16];
@@ -164,20 +170,37 @@ __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const D
     const uint32_t F = sv.F;
     if (LDS) cap = kWaveLdsCap;
     if (d_nlist) nlist = *d_nlist < nlist ? (uint32_t)*d_nlist : nlist; // count left by the previous kernel
+    // A short list (the few items the persistent kernel defers) would leave
+    // most workgroups idle while each item runs one long dependent chain: its
+    // MEM starts are split into P parts (a power of two, at most parts_max,
+    // with every part on a workgroup of this grid), each swept like an item of
+    // its own.  Part counts go to part_cnt[work index]; the scatter places a
+    // part's records behind those of the item's earlier parts.
+    uint32_t P = 1;
+    if (part_cnt)
+        while (P * 2 <= parts_max && (uint64_t)nlist * P * 2 <= gridDim.x) P *= 2;
+    const uint32_t nwork = nlist * P; // P > 1 only when nlist * P <= gridDim.x
     uint4 *bufA = LDS ? s_sets : sets + (size_t)blockIdx.x * 2 * cap;
     uint4 *bufB = bufA + cap;
     for (;;) {
         uint32_t wi = 0;
         if (lane == 0) wi = (uint32_t)atomicAdd(&ctr->work, 1ull);
         wi = block_bcast<TB>(wi, &s_b32);
-        if (wi >= nlist) break;
-        const uint32_t w = list ? list[wi] : wi;
+        if (wi >= nwork) break;
+        const uint32_t li = wi / P, part = wi % P;
+        const uint32_t w = list ? list[li] : li;
         const uint4 it = items ? items[w] : make_uint4(w, 0u, 0xffffffffu, 0u);
         const uint32_t r = it.x;
         const uint64_t b0 = qoff[r];
         const uint32_t m = (uint32_t)(qoff[r + 1] - b0);
         const uint8_t *p = q + b0;
-        const uint32_t ia = it.y, ib = it.z < m ? it.z : m; // starts owned by this item
+        uint32_t ia = it.y, ib = it.z < m ? it.z : m; // starts owned by this item
+        if (P > 1) { // starts owned by this part
+            const uint64_t span = ib > ia ? ib - ia : 0u;
+            const uint32_t a = ia + (uint32_t)(span * part / P);
+            ib = ia + (uint32_t)(span * (part + 1) / P);
+            ia = a;
+        }
         uint32_t s0 = ia, E = 0, nrec = 0;
         bool too_big = false;
         while (min_len && !too_big) {
@@ -372,7 +395,7 @@ __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const D
                 rc.lo = iv.x;
                 rc.hi = iv.y;
                 rc.pass = pass;
-                rc.pad = 0;
+                rc.pad = P > 1 ? (part << 24) | (wi + 1u) : 0u; // wi + 1 <= gridDim.x < 2^24
                 recs[slot + rank] = rc;
             }
             nrec += ns;
@@ -381,7 +404,22 @@ __global__ __launch_bounds__(TB) void k_mem_wave(DevDir fwd, DevDir rev, const D
             __syncthreads();
         }
         if (lane == 0) {
-            if (too_big) {
+            if (P > 1) {
+                // the item succeeds in this pass when every part does: a failed
+                // part marks it (the first one lists it for the retry pass,
+                // which runs the whole item), a successful part adds its count
+                // and sets the pass unless a failure came first
+                part_cnt[wi] = too_big ? 0u : nrec;
+                if (too_big) {
+                    if (atomicExch(&pass_of[w], kPartFailed) != kPartFailed) {
+                        const unsigned long long at = atomicAdd(&ctr->n_retry, 1ull);
+                        retry_list[at] = w;
+                    }
+                } else {
+                    atomicAdd(&mem_cnt[w], nrec);
+                    atomicCAS(&pass_of[w], 0xffffffffu, pass);
+                }
+            } else if (too_big) {
                 const unsigned long long at = atomicAdd(&ctr->n_retry, 1ull);
                 retry_list[at] = w;
             } else {
@@ -1154,15 +1192,23 @@ __global__ void k_read_off(uint32_t nq, const uint64_t *__restrict__ first, cons
 // Place the records of successful attempts at their final position.
 // A reservation that did not fit the record buffer leaves dead slots (pass
 // ~0, or never written): skip them before using their read id.
+// Records of an item split into parts (pad = part << 24 | work index + 1) go
+// behind the records of the item's earlier parts (part_cnt, by work index).
 __global__ __launch_bounds__(256) void k_mem_scatter(const MemRec *__restrict__ recs, uint64_t nrec, uint32_t nq,
                                                      const uint32_t *__restrict__ pass_of,
                                                      const uint64_t *__restrict__ mem_off, uint4 *__restrict__ out,
-                                                     const unsigned long long *__restrict__ d_nrec) {
+                                                     const unsigned long long *__restrict__ d_nrec,
+                                                     const uint32_t *__restrict__ part_cnt) {
     if (d_nrec) nrec = *d_nrec < nrec ? *d_nrec : nrec;
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nrec; t += (uint64_t)gridDim.x * blockDim.x) {
         const MemRec rc = recs[t];
         if (rc.pass == 0xffffffffu || rc.read >= nq || pass_of[rc.read] != rc.pass) continue;
-        out[mem_off[rc.read] + rc.local] = make_uint4(rc.qs, rc.qe, rc.lo, rc.hi);
+        uint64_t base = 0;
+        if (rc.pad && part_cnt) {
+            const uint32_t wi = (rc.pad & 0xffffffu) - 1u, part = rc.pad >> 24;
+            for (uint32_t i = 1; i <= part && i <= wi; i++) base += part_cnt[wi - i];
+        }
+        out[mem_off[rc.read] + base + rc.local] = make_uint4(rc.qs, rc.qe, rc.lo, rc.hi);
     }
 }
 
@@ -1545,6 +1591,9 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     // read on the workgroup path (no persistent kernel): one wave each
     const uint32_t lds_waves = use_thread ? persist_grid(idx, k_mem_wave<true, kDeferTB>, kDeferTB)
                                           : persist_grid(idx, k_mem_wave<true, 64>, 64);
+    // part counts of the deferred items split by the first workgroup pass
+    // (one per workgroup of its grid)
+    if ((rc = ws.part_cnt.ensure((size_t)lds_waves * 4 + 4))) return rc;
     while (nlist) {
         const bool lds = cap == kWaveLdsCap;
         uint64_t waves = lds ? lds_waves : (uint64_t)idx->dev->num_cus * 2;
@@ -1578,18 +1627,19 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
                                idx->fwd.view, idx->rev.view, idx->d_const.as<DevConst>(), svw, d_q, d_qoff, list,
                                (uint32_t)nlist, items, min_len, cap, (uint4 *)nullptr, ws.mem_recs.as<MemRec>(),
                                rec_cap, ws.mem_cnt.as<uint32_t>(), ws.mem_flags.as<uint32_t>(), pass, retry, wc,
-                               dcount);
+                               dcount, dcount ? ws.part_cnt.as<uint32_t>() : nullptr, kDeferParts);
         else if (lds)
             hipLaunchKernelGGL((k_mem_wave<true, 64>), dim3((uint32_t)waves), dim3(64), 0, s, idx->fwd.view,
                                idx->rev.view,
                                idx->d_const.as<DevConst>(), svw, d_q, d_qoff, list, (uint32_t)nlist, items, min_len, cap,
                                (uint4 *)nullptr, ws.mem_recs.as<MemRec>(), rec_cap, ws.mem_cnt.as<uint32_t>(),
-                               ws.mem_flags.as<uint32_t>(), pass, retry, wc, dcount);
+                               ws.mem_flags.as<uint32_t>(), pass, retry, wc, dcount, (uint32_t *)nullptr, 1u);
         else
             hipLaunchKernelGGL((k_mem_wave<false, 1024>), dim3((uint32_t)waves), dim3(1024), 0, s, idx->fwd.view,
                                idx->rev.view, idx->d_const.as<DevConst>(), svw, d_q, d_qoff, list, (uint32_t)nlist,
                                items, min_len, cap, ws.wave_sets.as<uint4>(), ws.mem_recs.as<MemRec>(), rec_cap,
-                               ws.mem_cnt.as<uint32_t>(), ws.mem_flags.as<uint32_t>(), pass, retry, wc, dcount);
+                               ws.mem_cnt.as<uint32_t>(), ws.mem_flags.as<uint32_t>(), pass, retry, wc, dcount,
+                               (uint32_t *)nullptr, 1u);
         HSFM_HIP(hipGetLastError());
         if (timed && (rc = event_record(idx, 0, true))) return rc;
         if (use_thread && (rc = event_record(idx, 2, true))) return rc;
@@ -1658,7 +1708,8 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     if (rec_used) {
         hipLaunchKernelGGL(k_mem_scatter, dim3((uint32_t)((rec_used + 255) / 256)), dim3(256), 0, s,
                            ws.mem_recs.as<MemRec>(), rec_used, (uint32_t)W, ws.mem_flags.as<uint32_t>(),
-                           ws.mem_off.as<uint64_t>(), ws.mem_sorted.as<uint4>(), nullptr);
+                           ws.mem_off.as<uint64_t>(), ws.mem_sorted.as<uint4>(), nullptr,
+                           ws.part_cnt.as<uint32_t>());
         HSFM_HIP(hipGetLastError());
     }
     if ((rc = ws.cnt.ensure(nhits * 4 + 4))) return rc;
@@ -1831,6 +1882,7 @@ struct BatchBufs {
     DevBuf sub_off, slices; // per-batch read offsets and {first hit}, {first position} of each batch of a window
     DevBuf bcount;          // set-up: work items per callback batch and read offsets at batch starts
     DevBuf cscan;           // block sums of the device-count scan
+    DevBuf part_cnt;        // record counts of the parts of split deferred items (kPartCntMax)
     const void *recs_dead = nullptr; // recs as last filled with dead records throughout
     // the set's instantiated graph, kept across calls while everything it
     // captured (pointers, sizes, configuration) is unchanged
@@ -1842,7 +1894,7 @@ struct BatchBufs {
         DevBuf *all[] = {&bq_off, &slots, &mem_cnt, &status, &pass_of, &l1, &l2, &l3, &ctr, &recs, &wsets,
                          &mem_off, &sorted, &npos, &pos_off, &hits, &rows, &positions, &misc, &tmp,
                          &items, &item_cnt, &item_first, &read_off, &qnib, &bargs, &sub_off, &slices, &bcount,
-                         &cscan};
+                         &cscan, &part_cnt};
         graph.release();
         recs_dead = nullptr;
         graph_key.clear();
@@ -1853,7 +1905,7 @@ struct BatchBufs {
         const DevBuf *all[] = {&bq_off, &slots, &mem_cnt, &status, &pass_of, &l1, &l2, &l3, &ctr, &recs, &wsets,
                                &mem_off, &sorted, &npos, &pos_off, &hits, &rows, &positions, &misc, &tmp,
                                &items, &item_cnt, &item_first, &read_off, &qnib, &bargs, &sub_off, &slices,
-                               &cscan};
+                               &cscan, &part_cnt};
         for (const DevBuf *b : all) k.push_back(reinterpret_cast<uint64_t>(b->p));
     }
 };
@@ -2089,7 +2141,7 @@ static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, cons
                            svp, d_q, bq, (const uint32_t *)nullptr, W, items, min_len, kWaveLdsCap, (uint4 *)nullptr, r0,
                            half,
                            bb.mem_cnt.as<uint32_t>(), bb.pass_of.as<uint32_t>(), 0u, bb.l2.as<uint32_t>(), &c[1],
-                           (const unsigned long long *)nullptr);
+                           (const unsigned long long *)nullptr, (uint32_t *)nullptr, 1u);
     } else {
         uint4 *slots = bb.slots.as<uint4>();
         uint32_t *cnt = bb.mem_cnt.as<uint32_t>();
@@ -2109,19 +2161,21 @@ static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, cons
                                (const uint32_t *)nullptr, W, items, min_len, slots, cnt, st, bb.l1.as<uint32_t>(),
                                &c[0], LsiView{}, 0u);
         HSFM_HIP(hipGetLastError());
-        // deferred reads: kDeferTB threads each, sets in LDS
-        hipLaunchKernelGGL((k_mem_wave<true, kDeferTB>),
-                           dim3(persist_grid(idx, k_mem_wave<true, kDeferTB>, kDeferTB)), dim3(kDeferTB), 0, s,
+        // deferred reads: kDeferTB threads each, sets in LDS, split into parts
+        // when they are few
+        const uint32_t dg = persist_grid(idx, k_mem_wave<true, kDeferTB>, kDeferTB);
+        hipLaunchKernelGGL((k_mem_wave<true, kDeferTB>), dim3(dg), dim3(kDeferTB), 0, s,
                            idx->fwd.view, idx->rev.view, cst, svp, d_q, bq, bb.l1.as<uint32_t>(), W, items, min_len,
                            kWaveLdsCap, (uint4 *)nullptr, r0, half, cnt, bb.pass_of.as<uint32_t>(), 0u,
-                           bb.l2.as<uint32_t>(), &c[1], second_count);
+                           bb.l2.as<uint32_t>(), &c[1], second_count,
+                           dg <= kPartCntMax ? bb.part_cnt.as<uint32_t>() : nullptr, kDeferParts);
     }
     HSFM_HIP(hipGetLastError());
     // sets beyond the LDS: 1024 threads per read, global sets
     hipLaunchKernelGGL((k_mem_wave<false, 1024>), dim3(kBatchGlobalWaves), dim3(1024), 0, s, idx->fwd.view,
                        idx->rev.view, cst, svp, d_q, bq, bb.l2.as<uint32_t>(), W, items, min_len, kBatchGlobalCap,
                        bb.wsets.as<uint4>(), r1, half, bb.mem_cnt.as<uint32_t>(), bb.pass_of.as<uint32_t>(), 1u,
-                       bb.l3.as<uint32_t>(), &c[2], &c[1].n_retry);
+                       bb.l3.as<uint32_t>(), &c[2], &c[1].n_retry, (uint32_t *)nullptr, 1u);
     HSFM_HIP(hipGetLastError());
     uint64_t *mem_off = bb.mem_off.as<uint64_t>();
     if ((rc = batch_scan(idx, bb, bb.mem_cnt.as<uint32_t>(), mem_off, W))) return rc;
@@ -2133,9 +2187,9 @@ static int batch_record(hsfm_index *idx, BatchBufs &bb, const SeedView &sv, cons
                        bb.read_off.as<uint64_t>());
     const dim3 hg(idx->dev->num_cus * 8); // grid-stride over the device counts
     hipLaunchKernelGGL(k_mem_scatter, hg, dim3(256), 0, s, r0, half, W, bb.pass_of.as<uint32_t>(), mem_off, sorted,
-                       &c[1].rec_used);
+                       &c[1].rec_used, (const uint32_t *)bb.part_cnt.as<uint32_t>());
     hipLaunchKernelGGL(k_mem_scatter, hg, dim3(256), 0, s, r1, half, W, bb.pass_of.as<uint32_t>(), mem_off, sorted,
-                       &c[2].rec_used);
+                       &c[2].rec_used, (const uint32_t *)nullptr);
     HSFM_HIP(hipGetLastError());
     const dim3 cg(idx->dev->num_cus * 8);
     hipLaunchKernelGGL(k_mem_npos, cg, dim3(256), 0, s, sorted, bb.hit_cap, max_hits, n_refs ? 1u : 0u,
@@ -2236,7 +2290,8 @@ static int batch_alloc(BatchBufs &bb) {
         (rc = bb.hits.ensure(bb.hit_cap * sizeof(hsfm_mem_hit))) || (rc = bb.rows.ensure(bb.pos_cap * 4)) ||
         (rc = bb.positions.ensure(bb.pos_cap * 8)) || (rc = bb.misc.ensure(sizeof(BatchMisc))) ||
         (rc = bb.tmp.ensure(bb.tmp_bytes + 256)) || (rc = bb.sub_off.ensure(((size_t)B + bb.Wb + 1) * 8)) ||
-        (rc = bb.slices.ensure((2ull * bb.Wb + 2) * 8 + 16)) || (rc = bb.cscan.ensure((kCScanBlocks + 1) * 8)))
+        (rc = bb.slices.ensure((2ull * bb.Wb + 2) * 8 + 16)) || (rc = bb.cscan.ensure((kCScanBlocks + 1) * 8)) ||
+        (rc = bb.part_cnt.ensure((size_t)kPartCntMax * 4)))
         return rc;
     return HSFM_OK;
 }

@@ -255,6 +255,32 @@ def test_persistent_kernel_deferral_levels():
     assert g.last_stats()["fallback_reads"] > 0
 
 
+def test_deferred_reads_split_into_parts():
+    """Reads the persistent kernel defers (IUPAC-rich text, some reads with a
+    run of N) are fewer than the workgroups of the deferred pass, so each is
+    split into parts by MEM start; parts whose sets outgrow the LDS send the
+    whole read to the global-set pass while the other parts' records are
+    dropped.  Both modes stay equal to the oracle."""
+    rng = np.random.default_rng(77)
+    rich = random_text(rng, 20000, iupac_frac=0.3)
+    plain = random_text(rng, 30000)
+    o, g = _pair([O.decode(rich), O.decode(plain)], sample_rate=8)
+    qs = []
+    for i in range(40):
+        src = rich if i < 30 else plain
+        st = int(rng.integers(0, len(src) - 150))
+        q = src[st:st + 150].copy()
+        flip = rng.random(150) < 0.01
+        q[flip] = random_text(rng, int(flip.sum()), alphabet="ACGT")
+        if i % 4 == 0:
+            at = int(rng.integers(30, 110))
+            q[at:at + 8] = 5  # N
+        qs.append(q)
+    check(o, g, qs, 13)
+    st = g.last_stats()
+    assert st["fallback_reads"] > 0 and st["capacity_retries"] > 0
+
+
 def test_wave_sets_beyond_lds_capacity():
     """All-N queries over an IUPAC-rich text: the interval sets exceed the
     1024-interval LDS sets and re-run with global-memory sets."""
