@@ -299,6 +299,29 @@ def measured_traffic(kernel, args, nq, nl):
     return k["bytes_per_launch"] if k else None
 
 
+def fetch_per_access(kernel, args, nq, nl, accesses):
+    """The kernel's committed FETCH_SIZE per device-counted access against what
+    the same counter reads for one lone random 64-byte record (the random-load
+    calibration kernel): how many lone-record fetches each access costs."""
+    path = os.path.join(ROOT, "profiles", "traffic_smem_locate_100Mbp.json")
+    try:
+        with open(path) as f:
+            doc = json.load(f)
+    except (OSError, ValueError):
+        return None
+    w = doc.get("workload", {})
+    k = doc.get("kernels", {}).get(kernel)
+    cal = doc.get("random_record_calibration")
+    if (not k or not cal or not accesses or w.get("reference_bp") != args.ref_bp or w.get("smem_reads") != nq
+            or w.get("locate_reads") != nl):
+        return None
+    per = k["fetch_kib"] * 1024.0 / accesses
+    return {"fetch_counter_bytes_per_access": round(per, 1),
+            "lone_random_record_counter_bytes": cal["fetch_bytes_per_record"],
+            "records_fetched_per_access": round(per / cal["fetch_bytes_per_record"], 2),
+            "source": "profiles/smem_locate_100Mbp_traffic.md"}
+
+
 def measured_cache(kernel, args, nq, nl):
     """L2 hit rate and L1-to-L2 read requests per probe of `kernel` from the
     committed rocprofv3 TCC_HIT / TCC_MISS / TCP_TCC_READ_REQ pass, when it was
@@ -883,7 +906,8 @@ def main():
                 "bytes_per_read": round(bytes_per_launch / nq, 1),
                 "bytes_definition": BYTES_DEFINITION,
                 "measured_random_record_ceiling": random_record_ceiling(achieved),
-                "cache": measured_cache("k_mem_persist", args, nq, nl)}
+                "cache": measured_cache("k_mem_persist", args, nq, nl),
+                "fetch_vs_random_record": fetch_per_access("k_mem_persist", args, nq, nl, probes)}
         if roof["traffic"]:
             roof["traffic_frac"] = round(roof["traffic"] / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     loc_roof = None
@@ -896,7 +920,9 @@ def main():
                     "bytes_per_read": round(loc_stats["occ_probes"] * 64.0 / nl, 1),
                     "bytes_definition": BYTES_DEFINITION,
                     "measured_random_record_ceiling": random_record_ceiling(a),
-                    "cache": measured_cache("k_locate_search", args, nq, nl)}
+                    "cache": measured_cache("k_locate_search", args, nq, nl),
+                    "fetch_vs_random_record": fetch_per_access("k_locate_search", args, nq, nl,
+                                                               loc_stats["occ_probes"])}
         if loc_roof["traffic"]:
             loc_roof["traffic_frac"] = round(loc_roof["traffic"] / (lkms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
         if cpu.get("locate_search_probes_per_read"):
