@@ -60,6 +60,7 @@ struct Workspace {
     DevBuf mem_recs, mem_cnt, mem_off, mem_sorted, mem_hits, pos_off, positions, mem_flags, read_list;
     DevBuf wave_sets, mem_slots, mem_status;
     DevBuf items, item_cnt, item_first, read_off; // MEM work items (read segments)
+    DevBuf item_tot;                          // {work items, flat query length} of a MEM call
     DevBuf qnib;                              // queries as base-set nibbles
     DevBuf defer_ids;                         // work items the MEM main kernel deferred
     DevBuf part_cnt;                          // record counts of the parts of split deferred items

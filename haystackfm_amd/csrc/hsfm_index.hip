@@ -257,7 +257,7 @@ void free_workspace(Workspace &w) {
                      &w.scan_tmp, &w.mem_recs, &w.mem_cnt, &w.mem_off, &w.mem_sorted, &w.mem_hits,
                      &w.pos_off, &w.positions, &w.mem_flags, &w.read_list, &w.wave_sets,
                      &w.mem_slots, &w.mem_status, &w.items, &w.item_cnt, &w.item_first, &w.read_off,
-                     &w.qnib, &w.defer_ids, &w.part_cnt};
+                     &w.qnib, &w.defer_ids, &w.part_cnt, &w.item_tot};
     for (DevBuf *b : all) b->release();
     if (w.h_pinned) (void)hipHostFree(w.h_pinned);
     w.h_pinned = nullptr;

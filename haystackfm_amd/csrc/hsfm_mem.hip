@@ -40,7 +40,8 @@ struct MemCounters {
     unsigned long long probes;   // records fetched by the main kernel
     unsigned long long diag[4];  // deferrals: seed union, extension, slots; plain restarts
     unsigned long long pwork;    // work index of k_mem_persist
-    unsigned long long pad[6];
+    unsigned long long pos_ovf;  // records whose rows did not fit the position buffers
+    unsigned long long pad[5];
 };
 
 struct MemRec { // 32 bytes
@@ -1169,6 +1170,27 @@ __global__ void k_items_count(uint32_t nq, const uint64_t *__restrict__ qoff, ui
     const uint64_t m = qoff[r + 1] - qoff[r];
     cnt[r] = m <= seg ? 1u : (uint32_t)((m + seg - 1) / seg);
 }
+// Work items of all reads together into total[0] (zeroed by the caller) and
+// the flat length of the queries into total[1], for one read-back.
+__global__ __launch_bounds__(256) void k_items_total(uint32_t nq, const uint64_t *__restrict__ qoff, uint32_t seg,
+                                                     unsigned long long *__restrict__ total) {
+    unsigned long long c = 0;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nq; r += gridDim.x * blockDim.x) {
+        const uint64_t m = qoff[r + 1] - qoff[r];
+        c += m <= seg ? 1u : (m + seg - 1) / seg;
+    }
+    // one atomic per block (atomics on one address serialize in L2)
+    __shared__ unsigned long long s_w[4];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long b = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        if (b) atomicAdd(total, b);
+        if (blockIdx.x == 0) total[1] = qoff[nq];
+    }
+}
 __global__ void k_items_fill(uint32_t nq, const uint64_t *__restrict__ qoff, uint32_t seg,
                              const uint64_t *__restrict__ first, uint4 *__restrict__ items) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1488,23 +1510,25 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     uint64_t total_len = 0; // flat length of the queries (read back with the item count)
     if (nq && !(seg && nq) && (rc = readback_u64(idx, d_qoff + nq, &total_len, 1))) return rc;
     if (seg && nq) {
-        if ((rc = ws.item_cnt.ensure((size_t)nq * 4 + 4)) || (rc = ws.item_first.ensure(((size_t)nq + 1) * 8)))
-            return rc;
-        hipLaunchKernelGGL(k_items_count, dim3((nq + 255) / 256), dim3(256), 0, s, nq, d_qoff, seg,
-                           ws.item_cnt.as<uint32_t>());
+        // item count and flat length in one round trip; the per-read counts,
+        // their scan and the item table only when some read is split
+        if ((rc = ws.item_tot.ensure(16)) || (rc = fill_u32(idx, ws.item_tot.as<uint32_t>(), 4, 0))) return rc;
+        const uint32_t tg = std::min<uint32_t>((nq + 255) / 256, idx->dev->num_cus);
+        hipLaunchKernelGGL(k_items_total, dim3(tg), dim3(256), 0, s, nq, d_qoff, seg,
+                           ws.item_tot.as<unsigned long long>());
         HSFM_HIP(hipGetLastError());
-        if ((rc = exclusive_scan_u32_u64(idx, ws.item_cnt.as<uint32_t>(), ws.item_first.as<uint64_t>(), nq)))
-            return rc;
-        {   // item count and flat length in one round trip
-            if (!ws.h_pinned) HSFM_HIP(hipHostMalloc((void **)&ws.h_pinned, kPinnedBytes, hipHostMallocDefault));
-            HSFM_HIP(hipMemcpyAsync(ws.h_pinned, ws.item_first.as<uint64_t>() + nq, 8, hipMemcpyDeviceToHost, s));
-            HSFM_HIP(hipMemcpyAsync(ws.h_pinned + 1, d_qoff + nq, 8, hipMemcpyDeviceToHost, s));
-            HSFM_HIP(hipStreamSynchronize(s));
-            W = ws.h_pinned[0];
-            total_len = ws.h_pinned[1];
-        }
+        if ((rc = readback_u64(idx, ws.item_tot.as<uint64_t>(), h, 2))) return rc;
+        W = h[0];
+        total_len = h[1];
         if (W > 0xffffffffull - 16) return set_error(HSFM_EINVAL, "too many read segments (%llu)", (unsigned long long)W);
         if (W > nq) {
+            if ((rc = ws.item_cnt.ensure((size_t)nq * 4 + 4)) || (rc = ws.item_first.ensure(((size_t)nq + 1) * 8)))
+                return rc;
+            hipLaunchKernelGGL(k_items_count, dim3((nq + 255) / 256), dim3(256), 0, s, nq, d_qoff, seg,
+                               ws.item_cnt.as<uint32_t>());
+            HSFM_HIP(hipGetLastError());
+            if ((rc = exclusive_scan_u32_u64(idx, ws.item_cnt.as<uint32_t>(), ws.item_first.as<uint64_t>(), nq)))
+                return rc;
             if ((rc = ws.items.ensure(W * 16))) return rc;
             hipLaunchKernelGGL(k_items_fill, dim3((nq + 255) / 256), dim3(256), 0, s, nq, d_qoff, seg,
                                ws.item_first.as<uint64_t>(), ws.items.as<uint4>());
@@ -1522,7 +1546,9 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     MemCounters *ctr = ws.counters.as<MemCounters>();
     if ((rc = fill_u32(idx, ws.counters.as<uint32_t>(), 2 * sizeof(MemCounters) / 4, 0))) return rc;
     MemCounters *wc = ctr + 1;
-    if ((rc = fill_u32(idx, ws.mem_cnt.as<uint32_t>(), W, 0))) return rc;
+    // the persistent kernel writes every item's count (zero when it defers
+    // the item), so the counts start at zero only for the workgroup path
+    if (!use_thread && (rc = fill_u32(idx, ws.mem_cnt.as<uint32_t>(), W, 0))) return rc;
     if ((rc = fill_u32(idx, ws.mem_flags.as<uint32_t>(), W, 0xffffffffu))) return rc;
     const uint32_t *list = nullptr; // identity list
     uint64_t nlist = W;
@@ -1714,6 +1740,7 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
     }
     if ((rc = ws.cnt.ensure(nhits * 4 + 4))) return rc;
     if ((rc = ws.pos_off.ensure((nhits + 1) * 8))) return rc;
+    if ((rc = ws.mem_hits.ensure(nhits * sizeof(hsfm_mem_hit) + 64))) return rc;
     if (nhits) {
         hipLaunchKernelGGL(k_mem_npos, dim3((uint32_t)((nhits + 255) / 256)), dim3(256), 0, s,
                            ws.mem_sorted.as<uint4>(), nhits, max_hits, n_refs ? 1u : 0u, ws.cnt.as<uint32_t>(),
@@ -1721,24 +1748,49 @@ static int mem_device_impl(hsfm_index *idx, const uint8_t *d_q, const uint64_t *
         HSFM_HIP(hipGetLastError());
     }
     if ((rc = exclusive_scan_u32_u64(idx, ws.cnt.as<uint32_t>(), ws.pos_off.as<uint64_t>(), (uint32_t)nhits))) return rc;
+    // The position count stays on the device: the row and position buffers
+    // keep the capacity of earlier calls (at least two rows per record), the
+    // hits kernel counts the records whose rows would not fit, resolve runs
+    // over the device count, and one read-back after it gives the count, the
+    // overflow and the walk guard.  An overflow grows the buffers to the count
+    // and runs the two kernels again.
     uint64_t npos = 0;
-    if ((rc = readback_u64(idx, ws.pos_off.as<uint64_t>() + nhits, &npos, 1))) return rc;
-    if ((rc = ws.mem_hits.ensure(nhits * sizeof(hsfm_mem_hit) + 64))) return rc;
-    if ((rc = ws.rows.ensure(npos * 4 + 4))) return rc;
-    if ((rc = ws.positions.ensure(npos * 8 + 8))) return rc;
-    if (nhits) {
-        hipLaunchKernelGGL(k_mem_hits, dim3((uint32_t)((nhits + 255) / 256)), dim3(256), 0, s,
-                           ws.mem_sorted.as<uint4>(), nhits, max_hits, ws.pos_off.as<uint64_t>(),
-                           ws.mem_hits.as<hsfm_mem_hit>(), ws.rows.as<uint32_t>(), nullptr, ~0ull, nullptr);
-        HSFM_HIP(hipGetLastError());
+    uint64_t pos_cap = std::min<uint64_t>(ws.rows.cap / 4, ws.positions.cap / 8);
+    if (pos_cap < 2 * nhits + 64) {
+        pos_cap = 2 * nhits + 64;
+        if ((rc = ws.rows.ensure(pos_cap * 4)) || (rc = ws.positions.ensure(pos_cap * 8))) return rc;
     }
-    if (npos) {
-        if ((rc = launch_resolve(idx, idx->fwd.view, ws.rows.as<uint32_t>(), npos, d_refb, n_refs,
-                                 ws.positions.as<uint32_t>(), &ctr->guard)))
+    for (;;) {
+        if (nhits) {
+            hipLaunchKernelGGL(k_mem_hits, dim3((uint32_t)((nhits + 255) / 256)), dim3(256), 0, s,
+                               ws.mem_sorted.as<uint4>(), nhits, max_hits, ws.pos_off.as<uint64_t>(),
+                               ws.mem_hits.as<hsfm_mem_hit>(), ws.rows.as<uint32_t>(), nullptr, pos_cap,
+                               &ctr->pos_ovf);
+            HSFM_HIP(hipGetLastError());
+        }
+        idx->stats.sa_full_rows = idx->fwd.view.sa_full ? idx->fwd.view.n : 0;
+        if (nhits && n_refs) {
+            if ((rc = event_record(idx, 1, false))) return rc;
+            if ((rc = launch_resolve_fan(idx, idx->fwd.view, ws.rows.as<uint32_t>(), pos_cap,
+                                         ws.pos_off.as<uint64_t>() + nhits, d_refb, n_refs,
+                                         ws.positions.as<uint32_t>(), &ctr->guard, 4, idx->dev->num_cus * 8)))
+                return rc;
+            if ((rc = event_record(idx, 1, true))) return rc;
+        }
+        if (!ws.h_pinned) HSFM_HIP(hipHostMalloc((void **)&ws.h_pinned, kPinnedBytes, hipHostMallocDefault));
+        HSFM_HIP(hipMemcpyAsync(ws.h_pinned, ws.pos_off.as<uint64_t>() + nhits, 8, hipMemcpyDeviceToHost, s));
+        HSFM_HIP(hipMemcpyAsync(ws.h_pinned + 1, &ctr->guard, 8, hipMemcpyDeviceToHost, s));
+        HSFM_HIP(hipMemcpyAsync(ws.h_pinned + 2, &ctr->pos_ovf, 8, hipMemcpyDeviceToHost, s));
+        HSFM_HIP(hipStreamSynchronize(s));
+        npos = ws.h_pinned[0];
+        const uint64_t guard = ws.h_pinned[1], ovf = ws.h_pinned[2];
+        idx->stats.guard_hits = guard;
+        if (guard) return set_error(HSFM_EINDEX, "%llu LF walks exceeded the sample rate: index is inconsistent", (unsigned long long)guard);
+        if (ovf == 0) break;
+        pos_cap = npos;
+        if ((rc = ws.rows.ensure(pos_cap * 4 + 4)) || (rc = ws.positions.ensure(pos_cap * 8 + 8)) ||
+            (rc = fill_u32(idx, reinterpret_cast<uint32_t *>(&ctr->pos_ovf), 2, 0)))
             return rc;
-        if ((rc = readback_u64(idx, reinterpret_cast<const uint64_t *>(ctr), h, 8))) return rc;
-        idx->stats.guard_hits = h[3];
-        if (h[3]) return set_error(HSFM_EINDEX, "%llu LF walks exceeded the sample rate: index is inconsistent", (unsigned long long)h[3]);
     }
     idx->stats.main_kernel_ms = event_ms(idx, 0);
     idx->stats.resolve_kernel_ms = npos ? event_ms(idx, 1) : 0.0;
