@@ -586,6 +586,7 @@ __global__ __launch_bounds__(256) void k_pack_bases(const uint8_t *__restrict__ 
     // two words (16 codes) per thread; whole 16-byte groups with one load when
     // the array is 16-byte aligned
     const bool vec = (reinterpret_cast<uintptr_t>(q) & 15) == 0;
+#pragma unroll 4
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; 2 * t < nwords;
          t += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t g0 = t * 16;
@@ -604,8 +605,8 @@ __global__ __launch_bounds__(256) void k_pack_bases(const uint8_t *__restrict__ 
             if (u < 8) w0 |= nb << (4 * u);
             else w1 |= nb << (4 * (u - 8));
         }
-        nib[2 * t] = w0;
-        if (2 * t + 1 < nwords) nib[2 * t + 1] = w1;
+        if (2 * t + 1 < nwords) *reinterpret_cast<uint2 *>(nib + 2 * t) = make_uint2(w0, w1);
+        else nib[2 * t] = w0;
     }
 }
 
@@ -613,8 +614,9 @@ int pack_bases(hsfm_index *idx, const uint8_t *d_q, uint64_t len, DevBuf &out) {
     const uint64_t nwords = (len + 7) / 8 + 4; // +4: the window reads three words from any start
     int rc = out.ensure(nwords * 4);
     if (rc) return rc;
+    // one resident round of blocks, each thread striding over several groups
     uint64_t blocks = (nwords / 2 + 256) / 256;
-    if (blocks > 65536) blocks = 65536;
+    if (blocks > (uint64_t)idx->dev->num_cus * 8) blocks = (uint64_t)idx->dev->num_cus * 8;
     hipLaunchKernelGGL(k_pack_bases, dim3((uint32_t)blocks), dim3(256), 0, idx->dev->stream, d_q, len,
                        idx->d_const.as<DevConst>(), out.as<uint32_t>(), nwords);
     HSFM_HIP(hipGetLastError());
