@@ -281,6 +281,26 @@ def test_deferred_reads_split_into_parts():
     assert st["fallback_reads"] > 0 and st["capacity_retries"] > 0
 
 
+def test_position_buffers_grow_between_calls():
+    """A call with few positions leaves small row and position buffers; the
+    next call's MEMs hit 60 copies of a unit, so its positions outgrow them
+    (more than two per record): the call grows them and redoes the hits and
+    resolve.  Both calls equal the oracle, and a third call of the first
+    reads still does after the growth."""
+    rng = np.random.default_rng(5)
+    unit = random_text(rng, 300)
+    seqs = [O.decode(np.concatenate([unit] * 60)), O.decode(random_text(rng, 40000))]
+    o, g = _pair(seqs, sample_rate=8)
+    text, _ = O.concatenate(seqs)
+    start = len(unit) * 60 + 1
+    few = [text[start + 100 * i:start + 100 * i + 100].copy() for i in range(5)]
+    many = [unit[i:i + 120].copy() for i in range(0, 180, 3)]
+    check(o, g, few, 20, modes=(0,))
+    check(o, g, many, 20, modes=(0,))
+    assert sum(len(h.positions) for hits in g.find_smems_gpu(many, 20, o.fwd.seq_boundaries) for h in hits) > 2000
+    check(o, g, few, 20, modes=(0,))
+
+
 def test_wave_sets_beyond_lds_capacity():
     """All-N queries over an IUPAC-rich text: the interval sets exceed the
     1024-interval LDS sets and re-run with global-memory sets."""
